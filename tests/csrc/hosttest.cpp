@@ -1,0 +1,139 @@
+// Host build of the product's per-pair geometry (zedm-x_amd/csrc/zdx_geom.h)
+// for CPU-side tests: the same source the GPU kernels run, compiled with g++
+// -ffp-contract=off, compared against the oracle in tests/test_host_geom.py.
+#define ZDX_HD
+#include "../../zedm-x_amd/csrc/zdx_geom.h"
+
+#include <vector>
+
+using namespace zdx;
+
+extern "C" {
+
+// loop start order with the libstdc++-equivalent introsort
+void ht_start_order(const int* deg, int n, int* order) {
+    std::vector<int> d(deg, deg + n), o(n);
+    for (int i = 0; i < n; ++i) o[i] = i;
+    SArr<int> oa{o.data(), 1};
+    StartOrder so;
+    so.deg = SArr<int>{d.data(), 1};
+    ref_sort(oa, n, so);
+    for (int i = 0; i < n; ++i) order[i] = o[i];
+}
+
+// generic: sort int keys with a plain < comparator, returning the permutation
+// of (key, original index) records compared by key only
+struct KI { int k; int i; };
+struct KIArr {
+    using value_type = KI;
+    KI* p;
+    KI get(int i) const { return p[i]; }
+    void set(int i, KI v) { p[i] = v; }
+};
+struct KILess { bool operator()(const KI& a, const KI& b) const { return a.k < b.k; } };
+void ht_sort_keys(const int* keys, int n, int* perm) {
+    std::vector<KI> v(n);
+    for (int i = 0; i < n; ++i) v[i] = KI{keys[i], i};
+    ref_sort(KIArr{v.data()}, n, KILess{});
+    for (int i = 0; i < n; ++i) perm[i] = v[i].i;
+}
+
+int ht_tri_tri(const double* a, const double* b, double* out) {
+    V3 A[3], B[3];
+    for (int k = 0; k < 3; ++k) {
+        A[k] = v3(a[3 * k], a[3 * k + 1], a[3 * k + 2]);
+        B[k] = v3(b[3 * k], b[3 * k + 1], b[3 * k + 2]);
+    }
+    V3 nA, nB;
+    double dA, dB;
+    if (!plane_of(A[0], A[1], A[2], nA, dA)) return 0;
+    if (!plane_of(B[0], B[1], B[2], nB, dB)) return 0;
+    V3 p0, p1;
+    if (!tri_tri(A, nA, dA, B, nB, dB, p0, p1)) return 0;
+    out[0] = p0.x; out[1] = p0.y; out[2] = p0.z; out[3] = p1.x; out[4] = p1.y; out[5] = p1.z;
+    return 1;
+}
+
+struct RecSink {
+    double* rec;
+    int cap;
+    double* vtx;
+    int vcap;
+    int nv;
+    int operator()(int li, int comp, int raw, const V3& Sn, const V3& Gn, const V3& nA, const V3& xc, double area,
+                   int plen, SArr<int> path, SArr<double> Vs) {
+        if (li >= cap) return TOPO_LOOP_OVERFLOW;
+        double* r = rec + 28 * li;
+        r[0] = Sn.x; r[1] = Sn.y; r[2] = Sn.z; r[3] = Gn.x; r[4] = Gn.y; r[5] = Gn.z;
+        r[6] = nA.x; r[7] = nA.y; r[8] = nA.z; r[9] = xc.x; r[10] = xc.y; r[11] = xc.z;
+        r[12] = area; r[23] = comp; r[24] = plen; r[25] = raw;
+        if (nv + plen > vcap) return TOPO_VTX_OVERFLOW;
+        for (int m = 0; m < plen; ++m) {
+            int v = path[m];
+            vtx[3 * (nv + m)] = Vs[3 * v]; vtx[3 * (nv + m) + 1] = Vs[3 * v + 1]; vtx[3 * (nv + m) + 2] = Vs[3 * v + 2];
+        }
+        nv += plen;
+        return 0;
+    }
+};
+
+// per-pair pipeline on host: world transform, brute-force narrow phase with
+// the product's tri_tri, then topology(); counts = {tri_checks, segs, comps,
+// loops_raw, loops, loop_verts, status}
+int ht_pair(const double* trisA, int TA, const double* trisB, int TB, const double* pA, const double* qA,
+            const double* pB, const double* qB, double mean_edge_a, double bbox_diag_a, int split,
+            double* seg_out, int seg_cap, double* rec, int rec_cap, double* vtx, int vcap, long long* counts) {
+    Qt qa; qa.w = qA[0]; qa.x = qA[1]; qa.y = qA[2]; qa.z = qA[3];
+    Qt qb; qb.w = qB[0]; qb.x = qB[1]; qb.y = qB[2]; qb.z = qB[3];
+    V3 pa = v3(pA[0], pA[1], pA[2]), pb = v3(pB[0], pB[1], pB[2]);
+    std::vector<V3> wa(3 * TA), wb(3 * TB);
+    for (int i = 0; i < 3 * TA; ++i) wa[i] = add(qrotate(qa, v3(trisA[3 * i], trisA[3 * i + 1], trisA[3 * i + 2])), pa);
+    for (int i = 0; i < 3 * TB; ++i) wb[i] = add(qrotate(qb, v3(trisB[3 * i], trisB[3 * i + 1], trisB[3 * i + 2])), pb);
+    std::vector<V3> nA(TA), nB(TB);
+    std::vector<double> dA(TA), dB(TB);
+    long long nondeg = 0;
+    for (int i = 0; i < TA; ++i) nondeg += plane_of(wa[3 * i], wa[3 * i + 1], wa[3 * i + 2], nA[i], dA[i]) ? 1 : 0;
+    for (int i = 0; i < TB; ++i) plane_of(wb[3 * i], wb[3 * i + 1], wb[3 * i + 2], nB[i], dB[i]);
+    std::vector<double> seg;
+    for (int i = 0; i < TA; ++i) {
+        if (norm2(nA[i]) < 1e-30) continue;
+        const V3* A = &wa[3 * i];
+        double a0x = min3(A[0].x, A[1].x, A[2].x), a0y = min3(A[0].y, A[1].y, A[2].y), a0z = min3(A[0].z, A[1].z, A[2].z);
+        double a1x = max3(A[0].x, A[1].x, A[2].x), a1y = max3(A[0].y, A[1].y, A[2].y), a1z = max3(A[0].z, A[1].z, A[2].z);
+        for (int j = 0; j < TB; ++j) {
+            const V3* B = &wb[3 * j];
+            double b0x = min3(B[0].x, B[1].x, B[2].x), b0y = min3(B[0].y, B[1].y, B[2].y), b0z = min3(B[0].z, B[1].z, B[2].z);
+            double b1x = max3(B[0].x, B[1].x, B[2].x), b1y = max3(B[0].y, B[1].y, B[2].y), b1z = max3(B[0].z, B[1].z, B[2].z);
+            if (a1x < b0x || a0x > b1x || a1y < b0y || a0y > b1y || a1z < b0z || a0z > b1z) continue;
+            if (norm2(nB[j]) < 1e-30) continue;
+            V3 p0, p1;
+            if (tri_tri(A, nA[i], dA[i], B, nB[j], dB[j], p0, p1)) {
+                seg.push_back(p0.x); seg.push_back(p0.y); seg.push_back(p0.z);
+                seg.push_back(p1.x); seg.push_back(p1.y); seg.push_back(p1.z);
+            }
+        }
+    }
+    int S = (int)(seg.size() / 6);
+    for (int k = 0; k < 6 * S && k < 6 * seg_cap; ++k) seg_out[k] = seg[k];
+    double tol = mean_edge_a > 0.0 ? (mean_edge_a * 0.1) : (bbox_diag_a * 1e-2);
+    if (tol <= 0.0) tol = 1e-6;
+    double area_eps = mean_edge_a > 0.0 ? (mean_edge_a * mean_edge_a * 1e-4) : 1e-12;
+    TopoCaps caps;
+    caps.S = S > 0 ? S : 1;
+    caps.E2 = 4 * caps.S;
+    std::vector<int> Iv(caps.n_ints());
+    std::vector<double> Dv(caps.n_dbls());
+    RecSink sink{rec, rec_cap, vtx, vcap, 0};
+    TopoOut t = topology(seg.data(), S, tol, area_eps, split != 0, sub(pb, pa), SArr<int>{Iv.data(), 1},
+                         SArr<double>{Dv.data(), 1}, caps, sink);
+    counts[0] = nondeg * TB;
+    counts[1] = S;
+    counts[2] = S > 0 ? t.comps : 0;
+    counts[3] = t.loops_raw;
+    counts[4] = t.loops;
+    counts[5] = sink.nv;
+    counts[6] = t.status;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,135 @@
+"""The product's per-pair geometry (zedm-x_amd/csrc/zdx_geom.h — the code the
+GPU kernels run) compiled for the host and compared bit-for-bit with the
+oracle: tri-tri, the libstdc++-equivalent introsort, and the whole per-pair
+topology pipeline."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import golden_inputs as GI
+import oracle as O
+from zdx import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "tests", "csrc", "hosttest.cpp")
+HDRS = [os.path.join(ROOT, "zedm-x_amd", "csrc", f) for f in ("zdx_geom.h", "zdx_sort.h")]
+
+
+@pytest.fixture(scope="module")
+def H(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("hosttest") / "libhosttest.so")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-o", out, SRC],
+                   check=True)
+    L = C.CDLL(out)
+    ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    L.ht_start_order.argtypes = [ip, C.c_int, ip]
+    L.ht_sort_keys.argtypes = [ip, C.c_int, ip]
+    L.ht_tri_tri.argtypes = [dp, dp, dp]
+    L.ht_pair.argtypes = [dp, C.c_int, dp, C.c_int, dp, dp, dp, dp, C.c_double, C.c_double, C.c_int,
+                          dp, C.c_int, dp, C.c_int, dp, C.c_int, np.ctypeslib.ndpointer(dtype=np.int64)]
+    return L
+
+
+def _degree_patterns(rng):
+    for n in list(range(0, 40)) + [63, 64, 65, 100, 257, 1000, 4096]:
+        yield rng.integers(0, 4, n)                         # few distinct degrees
+        yield rng.integers(0, 9, n)
+        yield np.sort(rng.integers(0, 9, n))
+        yield np.sort(rng.integers(0, 9, n))[::-1]
+        yield np.full(n, 2)
+        yield np.where(rng.random(n) < 0.9, 2, 3)
+        yield 1000 - np.arange(n) % 97                     # many distinct keys, periodic
+
+
+def test_introsort_matches_std_sort(H):
+    """Loop start order (src/main.cpp:977-982) must be std::sort's permutation,
+    including its instability for more than 16 vertices."""
+    rng = np.random.default_rng(5)
+    L = O.lib()
+    n_unstable = 0
+    for deg in _degree_patterns(rng):
+        deg = np.ascontiguousarray(deg, dtype=np.int32)
+        n = len(deg)
+        a = np.zeros(n, dtype=np.int32)
+        b = np.zeros(n, dtype=np.int32)
+        L.orc_loop_start_order(deg, n, a)
+        H.ht_start_order(deg, n, b)
+        assert np.array_equal(a, b), (n, deg[:20])
+        stable = np.array(sorted(range(n), key=lambda i: (0 if deg[i] == 2 else 1, -deg[i])), dtype=np.int32)
+        n_unstable += int(not np.array_equal(a, stable))
+    assert n_unstable > 5     # the cases do exercise the unstable permutation
+
+
+def test_tri_tri_host_matches_oracle(H, golden_geometry):
+    g = golden_geometry["tri_tri"]
+    A, B = GI.tri_pairs(g["seed"], g["n"])
+    L = O.lib()
+    for i in range(len(A)):
+        a = np.ascontiguousarray(A[i].reshape(-1))
+        b = np.ascontiguousarray(B[i].reshape(-1))
+        o1, o2 = np.zeros(6), np.zeros(6)
+        f1 = L.orc_tri_tri(a, b, o1)
+        f2 = H.ht_tri_tri(a, b, o2)
+        assert f1 == f2, i
+        assert np.array_equal(o1, o2), i
+
+
+def host_pair(H, ta, tb, pose, me, bd, split):
+    seg = np.zeros((20000, 6))
+    rec = np.zeros((512, 28))
+    vtx = np.zeros((20000, 3))
+    cnt = np.zeros(7, dtype=np.int64)
+    f = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+    H.ht_pair(f(ta).reshape(-1), len(ta), f(tb).reshape(-1), len(tb), f(pose[0]), f(pose[1]), f(pose[2]),
+              f(pose[3]), me, bd, int(split), seg.reshape(-1), 20000, rec.reshape(-1), 512, vtx.reshape(-1),
+              20000, cnt)
+    return cnt, seg[:cnt[1]], rec[:cnt[4]], vtx[:cnt[5]]
+
+
+def _check_pair(H, ta, tb, pose, pa, split, name):
+    r = O.pair_contact(ta, tb, *pose, pa[5], pa[6], split=split)
+    cnt, seg, rec, vtx = host_pair(H, ta, tb, pose, pa[5], pa[6], split)
+    assert cnt[6] == 0, name
+    assert cnt[0] == r["tri_checks"], name
+    assert np.array_equal(seg, r["segments"]), name
+    assert cnt[2] == r["comps"] and cnt[3] == r["loops_raw"], name
+    assert np.array_equal(rec[:, :13], r["loops"][:, :13]), name
+    assert np.array_equal(rec[:, 23:26], r["loops"][:, 23:26]), name
+    assert np.array_equal(vtx, r["loop_vtx"]), name
+    return len(seg), len(rec)
+
+
+def test_pair_pipeline_host_matches_oracle(H, ref_inputs):
+    tot_loops = 0
+    for name, mA, mB, pose, split in GI.pair_cases(ref_inputs):
+        ta, pa, _ = O.build_mesh(mA)
+        tb, _, _ = O.build_mesh(mB)
+        _, nl = _check_pair(H, ta, tb, pose, pa, split, name)
+        tot_loops += nl
+    assert tot_loops >= 15
+
+
+def test_pair_pipeline_host_random(H):
+    """Seeded random poses over concave stars and hulls (many multi-loop pairs)."""
+    m = GI.synth_meshes()
+    built = {k: O.build_mesh(v) for k, v in m.items()}
+    rng = synth.Rng(77)
+    names = list(m)
+    n_multi = 0
+    for it in range(60):
+        a = names[int(rng.uniform(1)[0] * len(names))]
+        b = names[int(rng.uniform(1)[0] * len(names))]
+        if it % 2 == 0:
+            a, b = "star2a", "star2b"      # concave pairs: several loops per pair
+        ta, pa, _ = built[a]
+        tb, pb, _ = built[b]
+        dep = 0.05 + 0.35 * float(rng.uniform(1)[0])
+        pose = GI._pose(rng, pa[4], pb[4], dep, True)
+        split = bool(it % 5)
+        ns, nl = _check_pair(H, ta, tb, pose, pa, split, f"{a}-{b}-{it}")
+        n_multi += nl > 1
+    assert n_multi >= 2
