@@ -1,0 +1,177 @@
+/*
+ * zdx.h — C ABI of libzdx.so, the MI355X (gfx950) implementation of the
+ * ZDEM-X Route-B polyhedral contact step.
+ *
+ * The reference (zhl-handsome/ZEDM-X) has no library or FFI boundary: its
+ * interfaces are the `zdem_cpu --config` program (src/main.cpp:1691-1708 with
+ * the config format of src/main.cpp:1531-1689) and the per-pair Python tool
+ * polyline.py (polyline.py:707-834).  Each entry point below names the
+ * reference code it replaces.  Conventions:
+ *   - plain C types only; every function returns a zdx_status;
+ *   - the caller owns every host buffer; a context owns device memory and one
+ *     HIP stream; calls on one context must not overlap (distinct contexts may
+ *     be used from different threads); calls return when results are on the host;
+ *   - quaternions are (w, x, y, z); vectors are 3 doubles; all values FP64;
+ *   - a ZDX_CAPACITY return fills the required sizes so the caller can retry.
+ */
+#ifndef ZDX_H
+#define ZDX_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t zdx_status;
+enum {
+    ZDX_OK = 0,
+    ZDX_BAD_ARG = 1,
+    ZDX_IO = 2,
+    ZDX_CAPACITY = 3,
+    ZDX_HIP_ERROR = 4,
+    ZDX_INTERNAL = 5
+};
+
+typedef struct zdx_ctx zdx_ctx;
+typedef struct zdx_sim zdx_sim;
+
+/* ---------------------------------------------------------------- context */
+/* Opens device `device` (HIP ordinal) and creates the context's stream. */
+zdx_status zdx_create(zdx_ctx** out, int32_t device);
+void zdx_destroy(zdx_ctx* ctx);
+/* Message of the last failing call on this context (static storage per ctx). */
+const char* zdx_last_error(const zdx_ctx* ctx);
+
+/* ---------------------------------------------------------------- meshes */
+/* load_stl, src/main.cpp:308-371 (binary when 84 + 50 n == file size, else
+ * ASCII).  Writes up to cap triangles (9 doubles each) and the count to *T. */
+zdx_status zdx_load_stl(const char* path, double* tris, int32_t cap, int32_t* T);
+
+/* build_mesh, src/main.cpp:445-497: registers a triangle soup (T x 9 doubles,
+ * already scaled) with the context, centring it at its centroid when center
+ * != 0, and uploads it.  props (may be NULL) receives 16 doubles:
+ * volume, centroid(3), radius, mean_edge, bbox_diag, inertia_unit (9, row-major). */
+zdx_status zdx_mesh_register(zdx_ctx* ctx, const double* tris, int32_t T, int32_t center, int32_t* mesh_id,
+                             double* props);
+
+/* ------------------------------------------------------ per-pair contact */
+/* One contact evaluation of two registered meshes at given poses: the inner
+ * body of the reference step, src/main.cpp:2061-2219, with the force of
+ * polyline.py:655-701 (Fn = k/2 |Sn|). */
+typedef struct {
+    int32_t mesh_a, mesh_b;
+    double pos_a[3], quat_a[4];
+    double pos_b[3], quat_b[4];
+} zdx_pair_in;
+
+typedef struct {
+    int32_t split_contacts; /* 1: split segments into components (main.cpp:2154-2162) */
+    int32_t mode;           /* ZDX_MODE_PLAIN or ZDX_MODE_BATCHED */
+    double k;               /* stiffness of Fn = 0.5 k |Sn| */
+} zdx_contact_opts;
+
+enum { ZDX_MODE_PLAIN = 0, ZDX_MODE_BATCHED = 1 };
+
+/* Record of one accepted contact loop (28 doubles):
+ *   [0:3] Sn  [3:6] Gn  [6:9] n (unit, oriented A->B)  [9:12] xc0  [12] area=|Sn|
+ *   [13] Fn  [14:17] F (force on B; A receives -F)  [17:20] torque on A
+ *   [20:23] torque on B  [23] component index  [24] vertex count
+ *   [25] raw loop index within the pair  [26:28] reserved (0) */
+#define ZDX_LOOP_REC 28
+
+typedef struct {
+    /* per pair (n_pairs entries each; caller-owned, may be NULL except loop_offset) */
+    int64_t* tri_checks;   /* main.cpp:2129 counter */
+    int32_t* n_segments;
+    int32_t* n_comps;
+    int32_t* n_loops_raw;
+    int32_t* loop_offset;  /* n_pairs + 1: accepted loops of pair p are [loop_offset[p], loop_offset[p+1]) */
+    int32_t* seg_offset;   /* n_pairs + 1, only when segments != NULL */
+    /* per accepted loop */
+    double* loop_rec;      /* loop_cap x ZDX_LOOP_REC */
+    int64_t loop_cap;
+    int32_t* vtx_offset;   /* loop_cap + 1, only when loop_vtx != NULL */
+    double* loop_vtx;      /* vtx_cap x 3: loop vertices in walk order */
+    int64_t vtx_cap;
+    double* segments;      /* seg_cap x 6 (p0, p1) in (ta, tb) order */
+    int64_t seg_cap;
+    /* out: totals (also filled on ZDX_CAPACITY) */
+    int64_t n_loops_total, n_vtx_total, n_seg_total;
+} zdx_pair_out;
+
+zdx_status zdx_contact_pairs(zdx_ctx* ctx, const zdx_pair_in* pairs, int32_t n_pairs, const zdx_contact_opts* opts,
+                             zdx_pair_out* out);
+
+/* ------------------------------------------------------------- simulation */
+/* The zdem_cpu time step (src/main.cpp:2034-2468): broad phase, narrow phase,
+ * topology, particle-particle and particle-wall forces with tangential
+ * history, force/torque accumulation and the rigid-body integrator. */
+
+/* Loads a zdem_cpu config file (same format; unknown keys ignored) and builds
+ * the particles and walls as src/main.cpp:1887-2022 does.  Relative STL paths
+ * are resolved against base_dir when it is not NULL. */
+zdx_status zdx_sim_load_config(zdx_ctx* ctx, const char* path, const char* base_dir, zdx_sim** out);
+
+/* Programmatic set-up with registered meshes (equivalent to a config whose
+ * meshes were loaded once per template and scale). */
+typedef struct {
+    int32_t n_particles;
+    const int32_t* mesh_id;     /* per particle (registered with zdx_mesh_register) */
+    const double* init;         /* per particle 13 doubles: pos(3) quat(4) vel(3) omega(3) */
+    const double* material;     /* per particle 6 doubles: density young poisson mu restitution (unused) */
+    int32_t n_walls;
+    const int32_t* wall_mesh_id;
+    const double* wall_pose;    /* per wall 7 doubles: pos(3) quat(4) */
+    const double* wall_material;/* per wall 2 doubles: mu restitution */
+    double gravity[3];
+    double dt;
+    int32_t split_contacts;
+} zdx_sim_desc;
+zdx_status zdx_sim_create(zdx_ctx* ctx, const zdx_sim_desc* desc, zdx_sim** out);
+void zdx_sim_destroy(zdx_sim* sim);
+int32_t zdx_sim_num_particles(const zdx_sim* sim);
+
+/* Step counters (int64 each), as printed on the reference's step line
+ * (main.cpp:2489-2499) plus broad-phase and wall totals. */
+enum {
+    ZDX_CTR_CONTACTS = 0,
+    ZDX_CTR_TRI_CHECKS = 1,
+    ZDX_CTR_SEGMENTS = 2,
+    ZDX_CTR_COMPS = 3,
+    ZDX_CTR_LOOPS = 4,
+    ZDX_CTR_PAIRS = 5,        /* broad-phase candidate pairs (contact pairs evaluated) */
+    ZDX_CTR_WALL_CONTACTS = 6,
+    ZDX_CTR_TRI_TESTS = 7,    /* triangle AABB tests actually executed after culling */
+    ZDX_CTR_TRI_HITS = 8,     /* triangle pairs that reached the exact intersection test */
+    ZDX_N_COUNTERS = 9
+};
+/* Runs n_steps steps in mode ZDX_MODE_PLAIN (one kernel per stage,
+ * host-sized launches) or ZDX_MODE_BATCHED (persistent work-queue kernels,
+ * whole step captured in a hipGraph).  counters (may be NULL) receives the
+ * counters of the last step. */
+zdx_status zdx_sim_step(zdx_sim* sim, int32_t n_steps, int32_t mode, int64_t* counters);
+
+/* Per particle 16 doubles: pos(3) quat(4) vel(3) omega(3) L(3). */
+zdx_status zdx_sim_get_state(zdx_sim* sim, double* state);
+zdx_status zdx_sim_set_state(zdx_sim* sim, const double* state);
+/* Force, torque and contact count of each particle from the last step
+ * (main.cpp:2027-2029); any pointer may be NULL. */
+zdx_status zdx_sim_get_forces(zdx_sim* sim, double* F, double* T, int32_t* contact_counts);
+/* Exact binary checkpoint of state and tangential histories (the
+ * reference's only restart file, write_particle_state_txt main.cpp:1842-1870,
+ * keeps 9 decimals and no history). */
+zdx_status zdx_sim_checkpoint(zdx_sim* sim, const char* path);
+zdx_status zdx_sim_restore(zdx_sim* sim, const char* path);
+/* Per-stage device times of the last step in ms (n entries):
+ * transform, broad phase, narrow phase, topology+forces, walls, fold+integrate. */
+zdx_status zdx_sim_stage_times(zdx_sim* sim, double* ms, int32_t n);
+/* Narrow-phase launch shape; 0 keeps the current value.  The autotuner
+ * (zdx_sim_autotune) sets them from timed trial steps. */
+zdx_status zdx_sim_set_tiles(zdx_sim* sim, int32_t np_block, int32_t np_groups_per_cu);
+zdx_status zdx_sim_autotune(zdx_sim* sim, int32_t* np_block, int32_t* np_groups_per_cu);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ZDX_H */

@@ -1,0 +1,1300 @@
+// zdx_api.cpp — the C ABI of libzdx.so (include/zdx.h): contexts, mesh
+// registry, the batched per-pair contact call and the simulation step in its
+// plain (one launch per stage, host-sized) and batched (persistent queue,
+// hipGraph-captured step) forms.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/zdx.h"
+#include "zdx_geom.h"
+#include "zdx_internal.h"
+#include "zdx_kernels.h"
+
+using namespace zdx;
+
+namespace {
+
+struct HipError {
+    hipError_t e;
+    const char* what;
+};
+#define HIPC(x)                                                \
+    do {                                                       \
+        hipError_t e_ = (x);                                   \
+        if (e_ != hipSuccess) throw HipError{e_, #x};          \
+    } while (0)
+
+struct ApiError {
+    zdx_status st;
+    std::string msg;
+};
+
+// device buffer that grows on demand (contents are not preserved on growth)
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    T* ensure(size_t m) {
+        if (m == 0) m = 1;
+        if (m <= n) return p;
+        release();
+        HIPC(hipMalloc((void**)&p, m * sizeof(T)));
+        n = m;
+        return p;
+    }
+    void upload(const T* h, size_t m, hipStream_t s) {
+        ensure(m);
+        if (m) HIPC(hipMemcpyAsync(p, h, m * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+struct MeshRec {
+    HostMesh h;
+    double tol, area_eps;
+};
+
+}  // namespace
+
+struct zdx_ctx {
+    int device = 0;
+    int n_cu = 256;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<MeshRec> meshes;
+    bool mesh_dirty = true;
+    uint64_t mesh_version = 0;
+    int32_t max_tris = 1;
+    // device mesh store
+    DBuf<double> m_verts;
+    DBuf<int32_t> m_tri;
+    DBuf<int32_t> m_voff, m_nv, m_toff, m_nt;
+    DBuf<double> m_tol, m_aeps;
+    // per-pair API buffers
+    struct Api {
+        DBuf<int32_t> mesh;
+        DBuf<double> state;
+        DBuf<int64_t> wv_off, wt_off;
+        DBuf<double> wv, aabb;
+        DBuf<uint8_t> degen;
+        DBuf<int32_t> nondeg;
+        DBuf<int32_t> pi, pj, order, npairs, status;
+        DBuf<int64_t> seg_off, tri_checks, tri_tests;
+        DBuf<int32_t> nseg, tri_hits;
+        DBuf<double> arena;
+        DBuf<unsigned long long> arena_top;
+        DBuf<int32_t> list_i;
+        DBuf<float> list_f;
+        DBuf<double> stage;
+        DBuf<unsigned int> queue;
+        DBuf<int32_t> bcnt, bcur;
+        DBuf<int32_t> ncontrib, comps, loops_raw, tstatus;
+        DBuf<int32_t> scr_i;
+        DBuf<double> scr_d;
+        DBuf<double> rec, vtx;
+        int32_t stage_cap = 1024;
+        int64_t arena_cap = 1 << 20;
+        int32_t lmax = 32;
+    } api;
+
+    DMeshes dmeshes() {
+        DMeshes M;
+        M.verts = m_verts.p; M.tri_idx = m_tri.p; M.vert_off = m_voff.p; M.nverts = m_nv.p;
+        M.tri_off = m_toff.p; M.ntris = m_nt.p; M.tol = m_tol.p; M.area_eps = m_aeps.p;
+        return M;
+    }
+    void sync_meshes() {
+        if (!mesh_dirty) return;
+        std::vector<double> v;
+        std::vector<int32_t> t, voff, nv, toff, nt;
+        std::vector<double> tol, aeps;
+        for (auto& m : meshes) {
+            voff.push_back((int32_t)(v.size() / 3));
+            nv.push_back((int32_t)(m.h.verts.size() / 3));
+            toff.push_back((int32_t)(t.size() / 3));
+            nt.push_back((int32_t)(m.h.tri_idx.size() / 3));
+            v.insert(v.end(), m.h.verts.begin(), m.h.verts.end());
+            t.insert(t.end(), m.h.tri_idx.begin(), m.h.tri_idx.end());
+            tol.push_back(m.tol);
+            aeps.push_back(m.area_eps);
+        }
+        m_verts.upload(v.data(), v.size(), stream);
+        m_tri.upload(t.data(), t.size(), stream);
+        m_voff.upload(voff.data(), voff.size(), stream);
+        m_nv.upload(nv.data(), nv.size(), stream);
+        m_toff.upload(toff.data(), toff.size(), stream);
+        m_nt.upload(nt.data(), nt.size(), stream);
+        m_tol.upload(tol.data(), tol.size(), stream);
+        m_aeps.upload(aeps.data(), aeps.size(), stream);
+        HIPC(hipStreamSynchronize(stream));
+        mesh_dirty = false;
+        ++mesh_version;
+    }
+    int add_mesh(HostMesh&& h) {
+        MeshRec r;
+        // tol and area_eps of mesh A (main.cpp:2118-2121, 2205)
+        double tol = h.mean_edge > 0.0 ? (h.mean_edge * 0.1) : (h.bbox_diag * 1e-2);
+        if (tol <= 0.0) tol = 1e-6;
+        r.tol = tol;
+        r.area_eps = h.mean_edge > 0.0 ? (h.mean_edge * h.mean_edge * 1e-4) : 1e-12;
+        max_tris = std::max<int32_t>(max_tris, (int32_t)(h.tri_idx.size() / 3));
+        r.h = std::move(h);
+        meshes.push_back(std::move(r));
+        mesh_dirty = true;
+        return (int)meshes.size() - 1;
+    }
+};
+
+namespace {
+
+template <class F>
+zdx_status guard(zdx_ctx* ctx, F&& f) {
+    try {
+        f();
+        return ZDX_OK;
+    } catch (const HipError& e) {
+        if (ctx) ctx->err = std::string("HIP error ") + hipGetErrorString(e.e) + " in " + e.what;
+        return ZDX_HIP_ERROR;
+    } catch (const ApiError& e) {
+        if (ctx) ctx->err = e.msg;
+        return e.st;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->err = e.what();
+        return ZDX_INTERNAL;
+    }
+}
+
+int32_t persistent_groups(const zdx_ctx* c, int per_cu) { return std::max(1, c->n_cu * per_cu); }
+
+}  // namespace
+
+// ======================================================================= sim
+struct zdx_sim {
+    zdx_ctx* ctx = nullptr;
+    int32_t N = 0, W = 0;
+    SimConfig cfg;
+    SimParams sp{};
+    std::vector<ParticleConst> pc;
+    std::vector<int32_t> mesh_of;
+    int64_t n_steps_done = 0;
+    int parity = 0;
+    // per-particle constants
+    DBuf<int32_t> d_mesh;
+    DBuf<double> d_mass, d_inv_mass, d_Ibinv, d_eqr, d_young, d_poisson, d_mu, d_rest, d_radius;
+    DBuf<int64_t> d_wv_off, d_wt_off;
+    DBuf<double> d_wv, d_aabb;
+    DBuf<uint8_t> d_degen;
+    DBuf<int32_t> d_nondeg;
+    DBuf<double> d_state[2];
+    DBuf<double> d_F, d_T;
+    DBuf<int32_t> d_counts;
+    // broad phase
+    DBuf<DGrid> d_grid;
+    int32_t max_cells = 1 << 24;
+    DBuf<int32_t> d_cell_of, d_cell_count, d_cell_start, d_cell_cursor, d_sorted, d_cnt_up, d_cnt_all;
+    DBuf<int32_t> d_pair_off, d_adj_off, d_adj_j, d_adj_pair, d_n_adj, d_scan_tmp;
+    int32_t pair_cap = 0, adj_cap = 0;
+    // pair sets (double-buffered: this step / previous step for the histories)
+    struct PairSet {
+        DBuf<int32_t> pi, pj, n;
+        DBuf<int64_t> key;
+        DBuf<double> ds;
+        DBuf<uint8_t> active;
+        DBuf<double> wds;
+        DBuf<uint8_t> wactive;
+    } ps[2];
+    DBuf<int32_t> d_order, d_bcnt, d_bcur;
+    DBuf<int64_t> d_seg_off, d_tri_checks, d_tri_tests;
+    DBuf<int32_t> d_nseg, d_tri_hits;
+    DBuf<double> d_arena;
+    DBuf<unsigned long long> d_arena_top;
+    int64_t arena_cap = 0;
+    DBuf<int32_t> d_list_i;
+    DBuf<float> d_list_f;
+    DBuf<double> d_stage;
+    int32_t stage_cap = 512;
+    int32_t tmax = 1;            // largest triangle count of the meshes in use
+    uint64_t mesh_version = 0;   // context mesh upload the graphs were captured with
+    DBuf<unsigned int> d_queue;
+    // topology
+    DBuf<double> d_contrib;
+    DBuf<int32_t> d_ncontrib, d_comps, d_loops_raw, d_tstatus;
+    DBuf<int32_t> d_scr_i;
+    DBuf<double> d_scr_d;
+    int32_t lmax = 8;
+    int64_t topo_threads = 8192;
+    int32_t seg_max = 512;
+    // walls
+    DBuf<double> d_wtris, d_wnorm, d_wpos, d_wrad, d_wmu;
+    DBuf<int32_t> d_wtoff, d_wnt;
+    int32_t max_wall_tris = 1;
+    DBuf<double> d_wcontrib;
+    DBuf<int32_t> d_wnc;
+    DBuf<double> d_wscr;
+    int64_t wall_threads = 8192;
+    int32_t wall_pts_cap = 64;
+    // status / counters
+    DBuf<int32_t> d_status;
+    DBuf<unsigned long long> d_ctr;
+    int64_t last_ctr[ZDX_N_COUNTERS] = {0};
+    double stage_ms[6] = {0, 0, 0, 0, 0, 0};
+    NarrowCfg ncfg{256, 4, 0, 0};
+    int32_t np_groups_per_cu = 8;
+    // graphs (one per parity)
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    bool graphs_valid = false;
+
+    ~zdx_sim() { drop_graphs(); }
+    void drop_graphs() {
+        for (int k = 0; k < 2; ++k) {
+            if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
+            if (graph[k]) (void)hipGraphDestroy(graph[k]);
+            gexec[k] = nullptr;
+            graph[k] = nullptr;
+        }
+        graphs_valid = false;
+    }
+
+    hipStream_t st() const { return ctx->stream; }
+
+    DBodies bodies(int par) {
+        DBodies B;
+        B.n = N; B.mesh = d_mesh.p; B.state = d_state[par].p; B.wv_off = d_wv_off.p; B.wt_off = d_wt_off.p;
+        B.wv = d_wv.p; B.degen = d_degen.p; B.nondeg = d_nondeg.p; B.aabb = d_aabb.p;
+        return B;
+    }
+    DMat mat() {
+        DMat m;
+        m.mass = d_mass.p; m.inv_mass = d_inv_mass.p; m.Ibinv = d_Ibinv.p; m.eqr = d_eqr.p; m.young = d_young.p;
+        m.poisson = d_poisson.p; m.mu = d_mu.p; m.rest = d_rest.p; m.radius = d_radius.p;
+        return m;
+    }
+    BroadPhaseBufs bpbufs(int par) {
+        BroadPhaseBufs b;
+        b.n = N; b.state = d_state[par].p; b.radius = d_radius.p; b.grid = d_grid.p; b.max_cells = max_cells;
+        b.cell_of = d_cell_of.p; b.cell_count = d_cell_count.p; b.cell_start = d_cell_start.p;
+        b.cell_cursor = d_cell_cursor.p; b.sorted = d_sorted.p; b.cnt_up = d_cnt_up.p; b.cnt_all = d_cnt_all.p;
+        b.pair_off = d_pair_off.p; b.adj_off = d_adj_off.p; b.pi = ps[par].pi.p; b.pj = ps[par].pj.p;
+        b.pkey = ps[par].key.p; b.pair_cap = pair_cap; b.adj_j = d_adj_j.p; b.adj_pair = d_adj_pair.p;
+        b.adj_cap = adj_cap; b.n_pairs = ps[par].n.p; b.n_adj = d_n_adj.p; b.scan_tmp = d_scan_tmp.p;
+        b.status = d_status.p;
+        return b;
+    }
+    DPairs dpairs(int par, bool lpt) {
+        DPairs P;
+        P.cap = pair_cap; P.n = ps[par].n.p; P.pi = ps[par].pi.p; P.pj = ps[par].pj.p;
+        P.order = lpt ? d_order.p : nullptr;
+        P.seg_off = d_seg_off.p; P.nseg = d_nseg.p; P.tri_checks = d_tri_checks.p; P.tri_tests = d_tri_tests.p;
+        P.tri_hits = d_tri_hits.p; P.arena = d_arena.p; P.arena_cap = arena_cap; P.arena_top = d_arena_top.p;
+        P.status = d_status.p;
+        return P;
+    }
+    HistBufs hist(int par) {
+        HistBufs H;
+        const int pv = 1 - par;
+        H.prev_key = ps[pv].key.p; H.prev_ds = ps[pv].ds.p; H.prev_active = ps[pv].active.p; H.prev_n = ps[pv].n.p;
+        H.cur_ds = ps[par].ds.p; H.cur_active = ps[par].active.p;
+        H.wprev_ds = ps[pv].wds.p; H.wprev_active = ps[pv].wactive.p;
+        H.wcur_ds = ps[par].wds.p; H.wcur_active = ps[par].wactive.p;
+        return H;
+    }
+    TopoBufs topobufs() {
+        TopoBufs T{};
+        T.lmax = lmax; T.contrib = d_contrib.p; T.ncontrib = d_ncontrib.p; T.comps = d_comps.p;
+        T.loops_raw = d_loops_raw.p; T.tstatus = d_tstatus.p; T.scr_i = d_scr_i.p; T.scr_d = d_scr_d.p;
+        T.nthreads = topo_threads; T.seg_max = seg_max;
+        return T;
+    }
+    WallBufs wallbufs() {
+        WallBufs Wb;
+        Wb.n_walls = W; Wb.tris = d_wtris.p; Wb.normals = d_wnorm.p; Wb.tri_off = d_wtoff.p; Wb.ntri = d_wnt.p;
+        Wb.pos = d_wpos.p; Wb.radius = d_wrad.p; Wb.mu = d_wmu.p; Wb.max_wall_tris = max_wall_tris;
+        return Wb;
+    }
+
+    void alloc_pairs(int32_t cap) {
+        drop_graphs();
+        pair_cap = cap;
+        adj_cap = 2 * cap;
+        for (int k = 0; k < 2; ++k) {
+            // preserve the previous step's history when growing
+            std::vector<int64_t> key;
+            std::vector<double> ds;
+            std::vector<uint8_t> act;
+            int32_t n = 0;
+            if (ps[k].n.p) {
+                HIPC(hipMemcpy(&n, ps[k].n.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+                if (n > 0) {
+                    key.resize(n); ds.resize(3 * (size_t)n); act.resize(n);
+                    HIPC(hipMemcpy(key.data(), ps[k].key.p, n * sizeof(int64_t), hipMemcpyDeviceToHost));
+                    HIPC(hipMemcpy(ds.data(), ps[k].ds.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+                    HIPC(hipMemcpy(act.data(), ps[k].active.p, n * sizeof(uint8_t), hipMemcpyDeviceToHost));
+                }
+            }
+            ps[k].pi.ensure(cap); ps[k].pj.ensure(cap); ps[k].n.ensure(1);
+            if (ps[k].key.n < (size_t)cap) {
+                ps[k].key.release(); ps[k].ds.release(); ps[k].active.release();
+            }
+            ps[k].key.ensure(cap); ps[k].ds.ensure(3 * (size_t)cap); ps[k].active.ensure(cap);
+            if (n > 0) {
+                HIPC(hipMemcpy(ps[k].key.p, key.data(), n * sizeof(int64_t), hipMemcpyHostToDevice));
+                HIPC(hipMemcpy(ps[k].ds.p, ds.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
+                HIPC(hipMemcpy(ps[k].active.p, act.data(), n * sizeof(uint8_t), hipMemcpyHostToDevice));
+            }
+            HIPC(hipMemcpy(ps[k].n.p, &n, sizeof(int32_t), hipMemcpyHostToDevice));
+        }
+        d_adj_j.ensure(adj_cap); d_adj_pair.ensure(adj_cap);
+        d_order.ensure(cap);
+        d_seg_off.ensure(cap); d_tri_checks.ensure(cap); d_tri_tests.ensure(cap);
+        d_nseg.ensure(cap); d_tri_hits.ensure(cap);
+        d_ncontrib.ensure(cap); d_comps.ensure(cap); d_loops_raw.ensure(cap); d_tstatus.ensure(cap);
+        d_contrib.ensure((size_t)cap * lmax * 9);
+    }
+    void alloc_lmax(int32_t l) {
+        drop_graphs();
+        lmax = l;
+        d_contrib.release();
+        d_contrib.ensure((size_t)pair_cap * lmax * 9);
+    }
+    void alloc_arena(int64_t cap) {
+        drop_graphs();
+        arena_cap = cap;
+        d_arena.release();
+        d_arena.ensure((size_t)cap * 6);
+    }
+    void alloc_stage(int32_t cap) {
+        drop_graphs();
+        stage_cap = cap;
+        seg_max = std::max(seg_max, cap);
+        size_t groups = (size_t)persistent_groups(ctx, 16);
+        d_stage.release();
+        d_stage.ensure(groups * 6 * (size_t)stage_cap);
+        alloc_topo_scratch();
+    }
+    void alloc_topo_scratch() {
+        drop_graphs();
+        TopoCaps c; c.S = seg_max; c.E2 = 4 * seg_max;
+        d_scr_i.release(); d_scr_d.release();
+        d_scr_i.ensure((size_t)c.n_ints() * topo_threads);
+        d_scr_d.ensure((size_t)c.n_dbls() * topo_threads);
+    }
+};
+
+namespace {
+
+void sim_setup_device(zdx_sim* s) {
+    zdx_ctx* c = s->ctx;
+    c->sync_meshes();
+    const int N = s->N;
+    std::vector<double> mass(N), im(N), Ibinv(9 * (size_t)N), eqr(N), young(N), pois(N), mu(N), rest(N), rad(N);
+    std::vector<int64_t> wvo(N + 1), wto(N + 1);
+    int64_t av = 0, at = 0;
+    for (int i = 0; i < N; ++i) {
+        const ParticleConst& p = s->pc[i];
+        mass[i] = p.mass; im[i] = p.inv_mass; eqr[i] = p.eqr; young[i] = p.young; pois[i] = p.poisson;
+        mu[i] = p.mu; rest[i] = p.rest; rad[i] = p.radius;
+        for (int k = 0; k < 9; ++k) Ibinv[9 * (size_t)i + k] = p.Ibinv[k];
+        wvo[i] = av; wto[i] = at;
+        av += (int64_t)(c->meshes[p.mesh].h.verts.size() / 3);
+        at += (int64_t)(c->meshes[p.mesh].h.tri_idx.size() / 3);
+    }
+    wvo[N] = av; wto[N] = at;
+    hipStream_t st = c->stream;
+    s->d_mesh.upload(s->mesh_of.data(), N, st);
+    s->d_mass.upload(mass.data(), N, st); s->d_inv_mass.upload(im.data(), N, st);
+    s->d_Ibinv.upload(Ibinv.data(), Ibinv.size(), st); s->d_eqr.upload(eqr.data(), N, st);
+    s->d_young.upload(young.data(), N, st); s->d_poisson.upload(pois.data(), N, st);
+    s->d_mu.upload(mu.data(), N, st); s->d_rest.upload(rest.data(), N, st); s->d_radius.upload(rad.data(), N, st);
+    s->d_wv_off.upload(wvo.data(), N + 1, st); s->d_wt_off.upload(wto.data(), N + 1, st);
+    s->d_wv.ensure(3 * (size_t)av); s->d_aabb.ensure(6 * (size_t)N); s->d_degen.ensure((size_t)at);
+    s->d_nondeg.ensure(N);
+    s->d_state[0].ensure(16 * (size_t)N); s->d_state[1].ensure(16 * (size_t)N);
+    s->d_F.ensure(3 * (size_t)N); s->d_T.ensure(3 * (size_t)N); s->d_counts.ensure(N);
+    // broad phase
+    s->d_grid.ensure(1);
+    int64_t cells = std::min<int64_t>((int64_t)1 << 24, std::max<int64_t>(64, 8 * (int64_t)N));
+    s->max_cells = (int32_t)cells;
+    s->d_cell_of.ensure(N); s->d_cell_count.ensure(cells); s->d_cell_start.ensure(cells + 1);
+    s->d_cell_cursor.ensure(cells); s->d_sorted.ensure(N); s->d_cnt_up.ensure(N); s->d_cnt_all.ensure(N);
+    s->d_pair_off.ensure(N + 1); s->d_adj_off.ensure(N + 1); s->d_n_adj.ensure(1);
+    s->d_scan_tmp.ensure((size_t)(std::max<int64_t>(cells, N) / 1024 + 4));
+    s->d_status.ensure(1); s->d_ctr.ensure(16); s->d_queue.ensure(1);
+    s->d_bcnt.ensure(64); s->d_bcur.ensure(64);
+    s->d_arena_top.ensure(1);
+    s->alloc_pairs(std::max(1024, 8 * N));
+    s->alloc_arena(std::max<int64_t>(1 << 16, 64 * (int64_t)N));
+    size_t groups = (size_t)persistent_groups(c, 16);
+    s->tmax = c->max_tris;
+    s->d_list_i.ensure(groups * 2 * (size_t)s->tmax);
+    s->d_list_f.ensure(groups * 12 * (size_t)s->tmax);
+    s->topo_threads = std::min<int64_t>(8192, std::max<int64_t>(256, ((int64_t)N + 255) / 256 * 256));
+    s->alloc_stage(s->stage_cap);
+    // particle/wall histories start empty
+    for (int k = 0; k < 2; ++k) {
+        int32_t z = 0;
+        HIPC(hipMemcpyAsync(s->ps[k].n.p, &z, sizeof(int32_t), hipMemcpyHostToDevice, st));
+    }
+    HIPC(hipStreamSynchronize(st));
+}
+
+struct WallHost {
+    std::vector<double> tris, normals;  // world
+    double pos[3];
+    double radius, mu, rest;
+};
+
+void sim_setup_walls(zdx_sim* s, const std::vector<WallHost>& walls) {
+    hipStream_t st = s->st();
+    s->W = (int32_t)walls.size();
+    std::vector<double> wt, wn, wp, wr, wmu;
+    std::vector<int32_t> wto, wnt;
+    int32_t maxwt = 1;
+    for (const auto& w : walls) {
+        wto.push_back((int32_t)(wt.size() / 9));
+        wnt.push_back((int32_t)(w.tris.size() / 9));
+        maxwt = std::max<int32_t>(maxwt, (int32_t)(w.tris.size() / 9));
+        wt.insert(wt.end(), w.tris.begin(), w.tris.end());
+        wn.insert(wn.end(), w.normals.begin(), w.normals.end());
+        wp.insert(wp.end(), w.pos, w.pos + 3);
+        wr.push_back(w.radius);
+        wmu.push_back(w.mu);
+    }
+    s->max_wall_tris = maxwt;
+    const size_t npw = (size_t)s->N * std::max(1, s->W);
+    s->d_wtris.upload(wt.data(), wt.size(), st); s->d_wnorm.upload(wn.data(), wn.size(), st);
+    s->d_wpos.upload(wp.data(), wp.size(), st); s->d_wrad.upload(wr.data(), wr.size(), st);
+    s->d_wmu.upload(wmu.data(), wmu.size(), st); s->d_wtoff.upload(wto.data(), wto.size(), st);
+    s->d_wnt.upload(wnt.data(), wnt.size(), st);
+    s->d_wcontrib.ensure(npw * (size_t)maxwt * 6);
+    s->d_wnc.ensure(npw);
+    launch_fill_i32(s->d_wnc.p, 0, (int64_t)npw, st);
+    for (int k = 0; k < 2; ++k) {
+        s->ps[k].wds.ensure(3 * npw);
+        s->ps[k].wactive.ensure(npw);
+        HIPC(hipMemsetAsync(s->ps[k].wactive.p, 0, npw, st));
+    }
+    s->wall_pts_cap = std::max<int32_t>(64, s->ctx->max_tris);
+    s->wall_threads = std::min<int64_t>(8192, std::max<int64_t>(256, ((int64_t)npw + 255) / 256 * 256));
+    s->d_wscr.ensure((size_t)s->wall_threads * (size_t)(3 * s->wall_pts_cap + 4 * (s->wall_pts_cap + 8)));
+    HIPC(hipStreamSynchronize(st));
+}
+
+WallHost make_wall(const HostMesh& m, const double* pos, const double* quat, double mu, double rest) {
+    WallHost w;
+    Qt q; q.w = quat[0]; q.x = quat[1]; q.y = quat[2]; q.z = quat[3];
+    V3 p = v3(pos[0], pos[1], pos[2]);
+    const size_t T = m.tris.size() / 9;
+    for (size_t t = 0; t < T; ++t) {
+        V3 c[3];
+        for (int k = 0; k < 3; ++k) {
+            const double* v = &m.tris[9 * t + 3 * k];
+            c[k] = add(qrotate(q, v3(v[0], v[1], v[2])), p);  // transform_tris main.cpp:499-510
+            w.tris.push_back(c[k].x); w.tris.push_back(c[k].y); w.tris.push_back(c[k].z);
+        }
+        V3 n = normalize(cross(sub(c[1], c[0]), sub(c[2], c[0])));  // tri_normal main.cpp:512-516
+        w.normals.push_back(n.x); w.normals.push_back(n.y); w.normals.push_back(n.z);
+    }
+    std::memcpy(w.pos, pos, 3 * sizeof(double));
+    w.radius = m.radius;
+    w.mu = mu;
+    w.rest = rest;
+    return w;
+}
+
+// Launch sequence of one step; `host_sized` inserts the plain mode's syncs and
+// retries, otherwise everything is device-sized (graph-capturable).
+void enqueue_step(zdx_sim* s, int par, bool batched) {
+    hipStream_t st = s->st();
+    const int nxt = 1 - par;
+    DBodies B = s->bodies(par);
+    DMeshes M = s->ctx->dmeshes();
+    DMat mat = s->mat();
+    launch_transform(B, M, st);
+    launch_broadphase(s->bpbufs(par), st);
+    DPairs P = s->dpairs(par, batched);
+    if (batched) launch_lpt_order(P, M, s->d_mesh.p, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st);
+    launch_fill_u64(s->d_arena_top.p, 0ull, 1, st);
+    launch_fill_i32((int32_t*)s->d_queue.p, 0, 1, st);
+    NarrowCfg nc = s->ncfg;
+    nc.use_queue = batched ? 1 : 0;
+    nc.groups = persistent_groups(s->ctx, s->np_groups_per_cu);
+    launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->d_stage.p, s->stage_cap, s->tmax,
+                  s->d_queue.p, st);
+    TopoBufs T = s->topobufs();
+    HistBufs H = s->hist(par);
+    launch_topo_sim(P, M, B, mat, H, T, s->sp, st);
+    if (s->W > 0)
+        launch_walls(B, M, mat, s->wallbufs(), H, s->d_wcontrib.p, s->d_wnc.p, s->d_wscr.p, s->wall_threads,
+                     s->wall_pts_cap, s->sp, s->d_status.p, st);
+    launch_fold_integrate(s->N, s->d_state[par].p, s->d_state[nxt].p, mat, s->d_adj_off.p, s->d_adj_j.p,
+                          s->d_adj_pair.p, s->adj_cap, T, s->W, s->d_wcontrib.p, s->d_wnc.p, s->max_wall_tris, s->d_F.p, s->d_T.p,
+                          s->d_counts.p, s->sp, st);
+    launch_counters(P, T, s->N * s->W, s->d_wnc.p, s->d_ctr.p, st);
+}
+
+int32_t read_status(zdx_sim* s) {
+    int32_t v = 0;
+    HIPC(hipMemcpyAsync(&v, s->d_status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s->st()));
+    HIPC(hipStreamSynchronize(s->st()));
+    return v;
+}
+
+// Grow whatever a failed step reported; returns false when nothing can grow.
+bool grow_for(zdx_sim* s, int32_t status, int par) {
+    bool grown = false;
+    if (status & ST_PAIRS) {
+        int32_t np = 0;
+        HIPC(hipMemcpy(&np, s->d_pair_off.p + s->N, sizeof(int32_t), hipMemcpyDeviceToHost));
+        int32_t na = 0;
+        HIPC(hipMemcpy(&na, s->d_adj_off.p + s->N, sizeof(int32_t), hipMemcpyDeviceToHost));
+        int32_t need = std::max(np, (na + 1) / 2);
+        s->alloc_pairs(std::max<int32_t>(2 * s->pair_cap, need + need / 4 + 16));
+        grown = true;
+    }
+    if (status & ST_ARENA) { s->alloc_arena(s->arena_cap * 2); grown = true; }
+    if (status & ST_STAGE) {
+        if (s->stage_cap >= (1 << 16)) return false;
+        s->alloc_stage(s->stage_cap * 4);
+        grown = true;
+    }
+    if (status & ST_TOPO) {
+        // find what overflowed: segments beyond the scratch, or loops beyond lmax
+        int32_t n = 0;
+        HIPC(hipMemcpy(&n, s->ps[par].n.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+        std::vector<int32_t> ts(n), ns(n), nl(n);
+        if (n) {
+            HIPC(hipMemcpy(ts.data(), s->d_tstatus.p, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(ns.data(), s->d_nseg.p, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+        }
+        int32_t smax = 0;
+        bool loops = false;
+        for (int p = 0; p < n; ++p) {
+            if (ts[p] & TOPO_LOOP_OVERFLOW) loops = true;
+            if (ts[p] & TOPO_E2_OVERFLOW) smax = std::max(smax, ns[p]);
+        }
+        if (loops) { s->alloc_lmax(s->lmax * 2); grown = true; }
+        if (smax > 0) {
+            if (smax > s->seg_max) s->seg_max = smax;
+            else s->seg_max = s->seg_max * 2;
+            s->alloc_topo_scratch();
+            grown = true;
+        }
+    }
+    if (status & ST_WALL) {
+        s->wall_pts_cap *= 2;
+        s->d_wscr.release();
+        s->d_wscr.ensure((size_t)s->wall_threads * (size_t)(3 * s->wall_pts_cap + 4 * (s->wall_pts_cap + 8)));
+        s->drop_graphs();
+        grown = true;
+    }
+    if (status & ST_NEIGH) return false;
+    if (status & ST_GRID) return false;
+    return grown;
+}
+
+void read_counters(zdx_sim* s) {
+    unsigned long long c[16];
+    HIPC(hipMemcpyAsync(c, s->d_ctr.p, sizeof(c), hipMemcpyDeviceToHost, s->st()));
+    HIPC(hipStreamSynchronize(s->st()));
+    for (int k = 0; k < ZDX_N_COUNTERS; ++k) s->last_ctr[k] = (int64_t)c[k];
+}
+
+void step_once(zdx_sim* s, bool batched) {
+    hipStream_t st = s->st();
+    s->ctx->sync_meshes();
+    if (s->mesh_version != s->ctx->mesh_version) {
+        s->drop_graphs();
+        s->mesh_version = s->ctx->mesh_version;
+    }
+    for (int attempt = 0; attempt < 12; ++attempt) {
+        const int par = s->parity;
+        launch_fill_i32(s->d_status.p, 0, 1, st);
+        if (batched) {
+            if (!s->graphs_valid) {
+                for (int k = 0; k < 2; ++k) {
+                    HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                    enqueue_step(s, k, true);
+                    HIPC(hipStreamEndCapture(st, &s->graph[k]));
+                    HIPC(hipGraphInstantiate(&s->gexec[k], s->graph[k], nullptr, nullptr, 0));
+                }
+                s->graphs_valid = true;
+            }
+            HIPC(hipGraphLaunch(s->gexec[par], st));
+        } else {
+            enqueue_step(s, par, false);
+        }
+        int32_t status = read_status(s);
+        if (status == 0) {
+            read_counters(s);
+            s->parity = 1 - par;
+            s->n_steps_done += 1;
+            return;
+        }
+        // the step wrote only the other state buffer and this step's pair set:
+        // grow and run it again from the same state
+        if (!grow_for(s, status, par))
+            throw ApiError{ZDX_CAPACITY, "simulation step capacity exceeded (status " + std::to_string(status) + ")"};
+    }
+    throw ApiError{ZDX_CAPACITY, "simulation step did not fit after repeated growth"};
+}
+
+// plain-mode stage timing with events (one extra synchronising step)
+void time_stages(zdx_sim* s) {
+    hipStream_t st = s->st();
+    hipEvent_t ev[7];
+    for (auto& e : ev) HIPC(hipEventCreate(&e));
+    const int par = s->parity;
+    DBodies B = s->bodies(par);
+    DMeshes M = s->ctx->dmeshes();
+    DMat mat = s->mat();
+    launch_fill_i32(s->d_status.p, 0, 1, st);
+    HIPC(hipEventRecord(ev[0], st));
+    launch_transform(B, M, st);
+    HIPC(hipEventRecord(ev[1], st));
+    launch_broadphase(s->bpbufs(par), st);
+    HIPC(hipEventRecord(ev[2], st));
+    DPairs P = s->dpairs(par, false);
+    launch_fill_u64(s->d_arena_top.p, 0ull, 1, st);
+    NarrowCfg nc = s->ncfg;
+    nc.use_queue = 0;
+    nc.groups = persistent_groups(s->ctx, s->np_groups_per_cu);
+    launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->d_stage.p, s->stage_cap, s->tmax,
+                  s->d_queue.p, st);
+    HIPC(hipEventRecord(ev[3], st));
+    TopoBufs T = s->topobufs();
+    HistBufs H = s->hist(par);
+    launch_topo_sim(P, M, B, mat, H, T, s->sp, st);
+    HIPC(hipEventRecord(ev[4], st));
+    if (s->W > 0)
+        launch_walls(B, M, mat, s->wallbufs(), H, s->d_wcontrib.p, s->d_wnc.p, s->d_wscr.p, s->wall_threads,
+                     s->wall_pts_cap, s->sp, s->d_status.p, st);
+    HIPC(hipEventRecord(ev[5], st));
+    // fold into a scratch state so the timed pass does not advance the run
+    launch_fold_integrate(s->N, s->d_state[par].p, s->d_state[1 - par].p, mat, s->d_adj_off.p, s->d_adj_j.p,
+                          s->d_adj_pair.p, s->adj_cap, T, s->W, s->d_wcontrib.p, s->d_wnc.p, s->max_wall_tris, s->d_F.p,
+                          s->d_T.p, s->d_counts.p, s->sp, st);
+    HIPC(hipEventRecord(ev[6], st));
+    HIPC(hipEventSynchronize(ev[6]));
+    for (int k = 0; k < 6; ++k) {
+        float ms = 0;
+        HIPC(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+        s->stage_ms[k] = ms;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+}
+
+}  // namespace
+
+// ======================================================================= C ABI
+extern "C" {
+
+zdx_status zdx_create(zdx_ctx** out, int32_t device) {
+    if (!out) return ZDX_BAD_ARG;
+    *out = nullptr;
+    auto* c = new zdx_ctx();
+    zdx_status st = guard(c, [&] {
+        int n = 0;
+        HIPC(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) throw ApiError{ZDX_BAD_ARG, "no such HIP device"};
+        HIPC(hipSetDevice(device));
+        c->device = device;
+        hipDeviceProp_t prop;
+        HIPC(hipGetDeviceProperties(&prop, device));
+        c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    });
+    if (st != ZDX_OK) {
+        delete c;
+        return st;
+    }
+    *out = c;
+    return ZDX_OK;
+}
+
+void zdx_destroy(zdx_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* zdx_last_error(const zdx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+zdx_status zdx_load_stl(const char* path, double* tris, int32_t cap, int32_t* T) {
+    if (!path || !T) return ZDX_BAD_ARG;
+    std::vector<double> t;
+    if (!load_stl(path, t)) return ZDX_IO;
+    *T = (int32_t)(t.size() / 9);
+    if (tris) {
+        if (*T > cap) return ZDX_CAPACITY;
+        std::memcpy(tris, t.data(), t.size() * sizeof(double));
+    }
+    return ZDX_OK;
+}
+
+zdx_status zdx_mesh_register(zdx_ctx* c, const double* tris, int32_t T, int32_t center, int32_t* mesh_id,
+                             double* props) {
+    if (!c || !tris || T <= 0 || !mesh_id) return ZDX_BAD_ARG;
+    return guard(c, [&] {
+        std::vector<double> t(tris, tris + 9 * (size_t)T);
+        HostMesh h = build_mesh(t, center != 0);
+        if (props) {
+            props[0] = h.volume;
+            std::memcpy(props + 1, h.center, 3 * sizeof(double));
+            props[4] = h.radius; props[5] = h.mean_edge; props[6] = h.bbox_diag;
+            std::memcpy(props + 7, h.inertia, 9 * sizeof(double));
+        }
+        *mesh_id = c->add_mesh(std::move(h));
+    });
+}
+
+zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, const zdx_contact_opts* opts,
+                             zdx_pair_out* out) {
+    if (!c || (!pairs && n > 0) || n < 0 || !opts || !out || !out->loop_offset) return ZDX_BAD_ARG;
+    return guard(c, [&] {
+        HIPC(hipSetDevice(c->device));
+        for (int p = 0; p < n; ++p)
+            if (pairs[p].mesh_a < 0 || pairs[p].mesh_a >= (int)c->meshes.size() || pairs[p].mesh_b < 0 ||
+                pairs[p].mesh_b >= (int)c->meshes.size())
+                throw ApiError{ZDX_BAD_ARG, "pair references an unregistered mesh"};
+        c->sync_meshes();
+        auto& A = c->api;
+        hipStream_t st = c->stream;
+        const int nb = 2 * n;
+        std::vector<int32_t> mesh(nb);
+        std::vector<double> state(16 * (size_t)nb, 0.0);
+        std::vector<int64_t> wvo(nb + 1), wto(nb + 1);
+        int64_t av = 0, at = 0;
+        for (int p = 0; p < n; ++p) {
+            const zdx_pair_in& q = pairs[p];
+            for (int side = 0; side < 2; ++side) {
+                int b = 2 * p + side;
+                mesh[b] = side ? q.mesh_b : q.mesh_a;
+                const double* pos = side ? q.pos_b : q.pos_a;
+                const double* qu = side ? q.quat_b : q.quat_a;
+                double* s = &state[16 * (size_t)b];
+                s[0] = pos[0]; s[1] = pos[1]; s[2] = pos[2];
+                s[3] = qu[0]; s[4] = qu[1]; s[5] = qu[2]; s[6] = qu[3];
+                wvo[b] = av; wto[b] = at;
+                av += (int64_t)(c->meshes[mesh[b]].h.verts.size() / 3);
+                at += (int64_t)(c->meshes[mesh[b]].h.tri_idx.size() / 3);
+            }
+        }
+        wvo[nb] = av; wto[nb] = at;
+        if (n == 0) {
+            out->loop_offset[0] = 0;
+            if (out->seg_offset) out->seg_offset[0] = 0;
+            out->n_loops_total = out->n_vtx_total = out->n_seg_total = 0;
+            return;
+        }
+        A.mesh.upload(mesh.data(), nb, st);
+        A.state.upload(state.data(), state.size(), st);
+        A.wv_off.upload(wvo.data(), nb + 1, st);
+        A.wt_off.upload(wto.data(), nb + 1, st);
+        A.wv.ensure(3 * (size_t)av); A.aabb.ensure(6 * (size_t)nb); A.degen.ensure((size_t)at); A.nondeg.ensure(nb);
+        DBodies B;
+        B.n = nb; B.mesh = A.mesh.p; B.state = A.state.p; B.wv_off = A.wv_off.p; B.wt_off = A.wt_off.p;
+        B.wv = A.wv.p; B.degen = A.degen.p; B.nondeg = A.nondeg.p; B.aabb = A.aabb.p;
+        DMeshes M = c->dmeshes();
+        launch_transform(B, M, st);
+        A.pi.ensure(n); A.pj.ensure(n); A.npairs.ensure(1); A.status.ensure(1); A.order.ensure(n);
+        launch_iota_pairs(A.pi.p, A.pj.p, n, st);
+        HIPC(hipMemcpyAsync(A.npairs.p, &n, sizeof(int32_t), hipMemcpyHostToDevice, st));
+        A.seg_off.ensure(n); A.tri_checks.ensure(n); A.tri_tests.ensure(n); A.nseg.ensure(n); A.tri_hits.ensure(n);
+        A.arena_top.ensure(1); A.queue.ensure(1); A.bcnt.ensure(64); A.bcur.ensure(64);
+        A.ncontrib.ensure(n); A.comps.ensure(n); A.loops_raw.ensure(n); A.tstatus.ensure(n);
+        const bool batched = opts->mode == ZDX_MODE_BATCHED;
+        const int groups = std::min<int>(persistent_groups(c, 8), std::max(1, n));
+        A.list_i.ensure((size_t)groups * 2 * c->max_tris);
+        A.list_f.ensure((size_t)groups * 12 * c->max_tris);
+        const bool want_vtx = out->loop_vtx != nullptr;
+        for (int attempt = 0; attempt < 10; ++attempt) {
+            A.arena.ensure((size_t)A.arena_cap * 6);
+            A.stage.ensure((size_t)groups * 6 * A.stage_cap);
+            DPairs P;
+            P.cap = n; P.n = A.npairs.p; P.pi = A.pi.p; P.pj = A.pj.p; P.order = batched ? A.order.p : nullptr;
+            P.seg_off = A.seg_off.p; P.nseg = A.nseg.p; P.tri_checks = A.tri_checks.p; P.tri_tests = A.tri_tests.p;
+            P.tri_hits = A.tri_hits.p; P.arena = A.arena.p; P.arena_cap = A.arena_cap; P.arena_top = A.arena_top.p;
+            P.status = A.status.p;
+            launch_fill_i32(A.status.p, 0, 1, st);
+            launch_fill_u64(A.arena_top.p, 0ull, 1, st);
+            launch_fill_i32((int32_t*)A.queue.p, 0, 1, st);
+            if (batched) launch_lpt_order(P, M, A.mesh.p, A.order.p, A.bcnt.p, A.bcur.p, n, st);
+            NarrowCfg nc{256, 4, groups, batched ? 1 : 0};
+            launch_narrow(B, M, P, nc, A.list_i.p, A.list_f.p, A.stage.p, A.stage_cap, c->max_tris, A.queue.p, st);
+            int32_t status = 0;
+            HIPC(hipMemcpyAsync(&status, A.status.p, 4, hipMemcpyDeviceToHost, st));
+            HIPC(hipStreamSynchronize(st));
+            if (status & ST_STAGE) { A.stage_cap *= 4; continue; }
+            if (status & ST_ARENA) { A.arena_cap *= 2; continue; }
+            // topology with the minimal-energy force
+            TopoBufs T{};
+            T.lmax = A.lmax; T.ncontrib = A.ncontrib.p; T.comps = A.comps.p; T.loops_raw = A.loops_raw.p;
+            T.tstatus = A.tstatus.p; T.nthreads = std::min<int64_t>(8192, ((int64_t)n + 63) / 64 * 64);
+            T.seg_max = A.stage_cap;
+            T.vmax = 4 * A.stage_cap + 16;
+            T.k_stiff = opts->k;
+            TopoCaps caps; caps.S = T.seg_max; caps.E2 = 4 * T.seg_max;
+            A.scr_i.ensure((size_t)caps.n_ints() * T.nthreads);
+            A.scr_d.ensure((size_t)caps.n_dbls() * T.nthreads);
+            A.rec.ensure((size_t)n * T.lmax * 28);
+            if (want_vtx) A.vtx.ensure((size_t)n * T.vmax * 3);
+            T.scr_i = A.scr_i.p; T.scr_d = A.scr_d.p; T.rec = A.rec.p; T.vtx = want_vtx ? A.vtx.p : nullptr;
+            launch_topo_api(P, M, B, T, opts->split_contacts, st);
+            HIPC(hipMemcpyAsync(&status, A.status.p, 4, hipMemcpyDeviceToHost, st));
+            HIPC(hipStreamSynchronize(st));
+            if (status & ST_TOPO) {
+                std::vector<int32_t> ts(n);
+                HIPC(hipMemcpy(ts.data(), A.tstatus.p, n * 4, hipMemcpyDeviceToHost));
+                bool lo = false, vo = false;
+                for (int p = 0; p < n; ++p) { lo |= (ts[p] & TOPO_LOOP_OVERFLOW) != 0; vo |= (ts[p] & (TOPO_VTX_OVERFLOW | TOPO_E2_OVERFLOW)) != 0; }
+                if (lo) A.lmax *= 2;
+                if (vo) A.stage_cap *= 2;
+                continue;
+            }
+            // gather to the host and pack
+            std::vector<int32_t> nl(n), ns(n), nc2(n), nr(n);
+            std::vector<int64_t> so(n), tc(n);
+            HIPC(hipMemcpy(nl.data(), A.ncontrib.p, n * 4, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(ns.data(), A.nseg.p, n * 4, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(nc2.data(), A.comps.p, n * 4, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(nr.data(), A.loops_raw.p, n * 4, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(so.data(), A.seg_off.p, n * 8, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(tc.data(), A.tri_checks.p, n * 8, hipMemcpyDeviceToHost));
+            std::vector<double> rec((size_t)n * T.lmax * 28);
+            HIPC(hipMemcpy(rec.data(), A.rec.p, rec.size() * 8, hipMemcpyDeviceToHost));
+            int64_t nloops = 0, nseg = 0, nvtx = 0;
+            for (int p = 0; p < n; ++p) {
+                nloops += nl[p];
+                nseg += ns[p];
+                for (int l = 0; l < nl[p]; ++l) nvtx += (int64_t)rec[((size_t)p * T.lmax + l) * 28 + 24];
+            }
+            out->n_loops_total = nloops;
+            out->n_seg_total = nseg;
+            out->n_vtx_total = nvtx;
+            if (nloops > out->loop_cap || (out->segments && nseg > out->seg_cap) ||
+                (out->loop_vtx && nvtx > out->vtx_cap))
+                throw ApiError{ZDX_CAPACITY, "zdx_pair_out capacity too small (see n_*_total)"};
+            std::vector<double> vtx;
+            if (want_vtx) {
+                vtx.resize((size_t)n * T.vmax * 3);
+                HIPC(hipMemcpy(vtx.data(), A.vtx.p, vtx.size() * 8, hipMemcpyDeviceToHost));
+            }
+            std::vector<double> arena;
+            if (out->segments && nseg > 0) {
+                unsigned long long top = 0;
+                HIPC(hipMemcpy(&top, A.arena_top.p, 8, hipMemcpyDeviceToHost));
+                arena.resize((size_t)top * 6);
+                HIPC(hipMemcpy(arena.data(), A.arena.p, arena.size() * 8, hipMemcpyDeviceToHost));
+            }
+            int64_t lo = 0, vo = 0, sg = 0;
+            for (int p = 0; p < n; ++p) {
+                out->loop_offset[p] = (int32_t)lo;
+                if (out->tri_checks) out->tri_checks[p] = tc[p];
+                if (out->n_segments) out->n_segments[p] = ns[p];
+                if (out->n_comps) out->n_comps[p] = nc2[p];
+                if (out->n_loops_raw) out->n_loops_raw[p] = nr[p];
+                int64_t pv = 0;
+                for (int l = 0; l < nl[p]; ++l) {
+                    const double* r = &rec[((size_t)p * T.lmax + l) * 28];
+                    if (out->loop_rec) std::memcpy(out->loop_rec + 28 * (lo + l), r, 28 * sizeof(double));
+                    int nv = (int)r[24];
+                    if (want_vtx) {
+                        out->vtx_offset[lo + l] = (int32_t)vo;
+                        std::memcpy(out->loop_vtx + 3 * vo, &vtx[((size_t)p * T.vmax + pv) * 3], 3 * nv * sizeof(double));
+                    }
+                    vo += nv;
+                    pv += nv;
+                }
+                lo += nl[p];
+                if (out->segments) {
+                    out->seg_offset[p] = (int32_t)sg;
+                    if (ns[p] > 0) std::memcpy(out->segments + 6 * sg, &arena[6 * (size_t)so[p]], 6 * ns[p] * sizeof(double));
+                    sg += ns[p];
+                }
+            }
+            out->loop_offset[n] = (int32_t)lo;
+            if (want_vtx) out->vtx_offset[lo] = (int32_t)vo;
+            if (out->segments) out->seg_offset[n] = (int32_t)sg;
+            return;
+        }
+        throw ApiError{ZDX_CAPACITY, "per-pair buffers did not fit after repeated growth"};
+    });
+}
+
+zdx_status zdx_sim_create(zdx_ctx* c, const zdx_sim_desc* d, zdx_sim** out) {
+    if (!c || !d || !out || d->n_particles <= 0 || !d->mesh_id || !d->init || !d->material) return ZDX_BAD_ARG;
+    *out = nullptr;
+    auto* s = new zdx_sim();
+    zdx_status st = guard(c, [&] {
+        HIPC(hipSetDevice(c->device));
+        s->ctx = c;
+        s->N = d->n_particles;
+        for (int i = 0; i < s->N; ++i)
+            if (d->mesh_id[i] < 0 || d->mesh_id[i] >= (int)c->meshes.size())
+                throw ApiError{ZDX_BAD_ARG, "particle references an unregistered mesh"};
+        s->sp.g[0] = d->gravity[0]; s->sp.g[1] = d->gravity[1]; s->sp.g[2] = d->gravity[2];
+        s->sp.dt = d->dt;
+        s->sp.split = d->split_contacts;
+        s->pc.resize(s->N);
+        s->mesh_of.resize(s->N);
+        std::vector<double> state(16 * (size_t)s->N);
+        for (int i = 0; i < s->N; ++i) {
+            const double* m = d->material + 6 * (size_t)i;
+            s->pc[i] = particle_const(c->meshes[d->mesh_id[i]].h, d->mesh_id[i], m[0], m[1], m[2], m[3], m[4]);
+            s->mesh_of[i] = d->mesh_id[i];
+            const double* in = d->init + 13 * (size_t)i;
+            double* x = &state[16 * (size_t)i];
+            std::memcpy(x, in, 13 * sizeof(double));
+            initial_L(s->pc[i], in + 3, in + 10, x + 13);
+        }
+        sim_setup_device(s);
+        std::vector<WallHost> walls;
+        for (int w = 0; w < d->n_walls; ++w) {
+            int mid = d->wall_mesh_id[w];
+            if (mid < 0 || mid >= (int)c->meshes.size()) throw ApiError{ZDX_BAD_ARG, "wall references an unregistered mesh"};
+            walls.push_back(make_wall(c->meshes[mid].h, d->wall_pose + 7 * (size_t)w, d->wall_pose + 7 * (size_t)w + 3,
+                                      d->wall_material[2 * w], d->wall_material[2 * w + 1]));
+        }
+        sim_setup_walls(s, walls);
+        HIPC(hipMemcpy(s->d_state[0].p, state.data(), state.size() * 8, hipMemcpyHostToDevice));
+    });
+    if (st != ZDX_OK) {
+        delete s;
+        return st;
+    }
+    *out = s;
+    return ZDX_OK;
+}
+
+zdx_status zdx_sim_load_config(zdx_ctx* c, const char* path, const char* base_dir, zdx_sim** out) {
+    if (!c || !path || !out) return ZDX_BAD_ARG;
+    *out = nullptr;
+    SimConfig cfg;
+    if (!parse_config(path, cfg)) {
+        c->err = std::string("failed to parse config ") + path;
+        return ZDX_IO;
+    }
+    auto resolve = [&](const std::string& p) {
+        if (!base_dir || p.empty() || p[0] == '/') return p;
+        std::string b(base_dir);
+        if (!b.empty() && b.back() != '/') b.push_back('/');
+        return b + p;
+    };
+    std::vector<int32_t> mesh_ids;
+    std::vector<double> init, mat;
+    std::vector<int32_t> wmesh;
+    std::vector<double> wpose, wmat;
+    zdx_status st = guard(c, [&] {
+        // mesh cache keyed by path|to_string(scale), main.cpp:1887-1914
+        std::map<std::string, std::vector<double>> tcache;
+        std::map<std::string, int> mcache;
+        auto load_for = [&](const std::string& p, double scale) -> int {
+            std::string key = p + "|" + std::to_string(scale);
+            auto it = mcache.find(key);
+            if (it != mcache.end()) return it->second;
+            auto jt = tcache.find(p);
+            if (jt == tcache.end()) {
+                std::vector<double> t;
+                if (!load_stl(resolve(p), t)) throw ApiError{ZDX_IO, "failed to load STL " + p};
+                jt = tcache.emplace(p, std::move(t)).first;
+            }
+            std::vector<double> t = jt->second;
+            for (auto& v : t) v *= scale;
+            int id = c->add_mesh(build_mesh(t, cfg.center));
+            mcache[key] = id;
+            return id;
+        };
+        std::vector<ParticleInit> inits = cfg.parts;
+        if (inits.empty()) {
+            inits.resize(cfg.n);
+            for (int i = 0; i < cfg.n; ++i) {
+                inits[i].stl = cfg.stl;
+                inits[i].pos[0] = cfg.spacing * i;
+                const double* v = (i == 0) ? cfg.v0 : (i == 1 ? cfg.v1 : nullptr);
+                for (int k = 0; k < 3; ++k) inits[i].vel[k] = v ? v[k] : 0.0;
+                inits[i].scale = 1.0; inits[i].density = 1.0; inits[i].young = 1e7; inits[i].poisson = 0.25;
+                inits[i].mu = 0.5; inits[i].rest = 0.5;
+            }
+        }
+        for (const auto& in : inits) {
+            std::string sp = in.stl.empty() ? cfg.stl : in.stl;
+            if (sp.empty()) throw ApiError{ZDX_BAD_ARG, "particle without stl"};
+            double sc = in.scale > 0.0 ? in.scale : 1.0;
+            int id = load_for(sp, sc);
+            if (c->meshes[id].h.verts.empty()) throw ApiError{ZDX_BAD_ARG, "mesh has no vertices"};
+            mesh_ids.push_back(id);
+            init.insert(init.end(), in.pos, in.pos + 3);
+            init.insert(init.end(), in.quat, in.quat + 4);
+            init.insert(init.end(), in.vel, in.vel + 3);
+            init.insert(init.end(), in.omega, in.omega + 3);
+            double m[6] = {in.density, in.young, in.poisson, in.mu, in.rest, 0.0};
+            mat.insert(mat.end(), m, m + 6);
+        }
+        // walls: main.cpp:1990-2022 (a wall whose STL fails is skipped)
+        for (const auto& wi : cfg.walls) {
+            if (wi.stl.empty()) continue;
+            std::vector<double> t;
+            if (!load_stl(resolve(wi.stl), t)) continue;
+            double sc = wi.scale > 0.0 ? wi.scale : 1.0;
+            for (auto& v : t) v = v * sc;
+            int id = c->add_mesh(build_mesh(t, cfg.center));
+            wmesh.push_back(id);
+            wpose.insert(wpose.end(), wi.pos, wi.pos + 3);
+            wpose.insert(wpose.end(), wi.quat, wi.quat + 4);
+            wmat.push_back(wi.mu);
+            wmat.push_back(wi.rest);
+        }
+    });
+    if (st != ZDX_OK) return st;
+    zdx_sim_desc d{};
+    d.n_particles = (int32_t)mesh_ids.size();
+    d.mesh_id = mesh_ids.data();
+    d.init = init.data();
+    d.material = mat.data();
+    d.n_walls = (int32_t)wmesh.size();
+    d.wall_mesh_id = wmesh.data();
+    d.wall_pose = wpose.data();
+    d.wall_material = wmat.data();
+    std::memcpy(d.gravity, cfg.gravity, sizeof(d.gravity));
+    d.dt = cfg.dt;
+    d.split_contacts = cfg.split ? 1 : 0;
+    st = zdx_sim_create(c, &d, out);
+    if (st == ZDX_OK) (*out)->cfg = cfg;
+    return st;
+}
+
+void zdx_sim_destroy(zdx_sim* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    delete s;
+}
+
+int32_t zdx_sim_num_particles(const zdx_sim* s) { return s ? s->N : 0; }
+
+zdx_status zdx_sim_step(zdx_sim* s, int32_t n_steps, int32_t mode, int64_t* counters) {
+    if (!s || n_steps < 0 || (mode != ZDX_MODE_PLAIN && mode != ZDX_MODE_BATCHED)) return ZDX_BAD_ARG;
+    return guard(s->ctx, [&] {
+        HIPC(hipSetDevice(s->ctx->device));
+        for (int k = 0; k < n_steps; ++k) step_once(s, mode == ZDX_MODE_BATCHED);
+        if (counters) std::memcpy(counters, s->last_ctr, sizeof(s->last_ctr));
+    });
+}
+
+zdx_status zdx_sim_get_state(zdx_sim* s, double* state) {
+    if (!s || !state) return ZDX_BAD_ARG;
+    return guard(s->ctx, [&] {
+        HIPC(hipStreamSynchronize(s->st()));
+        HIPC(hipMemcpy(state, s->d_state[s->parity].p, 16 * (size_t)s->N * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+zdx_status zdx_sim_set_state(zdx_sim* s, const double* state) {
+    if (!s || !state) return ZDX_BAD_ARG;
+    return guard(s->ctx, [&] {
+        HIPC(hipStreamSynchronize(s->st()));
+        HIPC(hipMemcpy(s->d_state[s->parity].p, state, 16 * (size_t)s->N * 8, hipMemcpyHostToDevice));
+    });
+}
+
+zdx_status zdx_sim_get_forces(zdx_sim* s, double* F, double* T, int32_t* cnt) {
+    if (!s) return ZDX_BAD_ARG;
+    return guard(s->ctx, [&] {
+        HIPC(hipStreamSynchronize(s->st()));
+        if (F) HIPC(hipMemcpy(F, s->d_F.p, 3 * (size_t)s->N * 8, hipMemcpyDeviceToHost));
+        if (T) HIPC(hipMemcpy(T, s->d_T.p, 3 * (size_t)s->N * 8, hipMemcpyDeviceToHost));
+        if (cnt) HIPC(hipMemcpy(cnt, s->d_counts.p, (size_t)s->N * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+// Checkpoint file: magic, N, W, parity-independent payload: state (16 N),
+// pair history (n, keys, ds, active) of the last step, wall history (3 N W ds,
+// N W active).
+zdx_status zdx_sim_checkpoint(zdx_sim* s, const char* path) {
+    if (!s || !path) return ZDX_BAD_ARG;
+    return guard(s->ctx, [&] {
+        HIPC(hipStreamSynchronize(s->st()));
+        const int prev = 1 - s->parity;  // set written by the last completed step
+        std::vector<double> state(16 * (size_t)s->N);
+        HIPC(hipMemcpy(state.data(), s->d_state[s->parity].p, state.size() * 8, hipMemcpyDeviceToHost));
+        int32_t n = 0;
+        HIPC(hipMemcpy(&n, s->ps[prev].n.p, 4, hipMemcpyDeviceToHost));
+        std::vector<int64_t> key(n);
+        std::vector<double> ds(3 * (size_t)n);
+        std::vector<uint8_t> act(n);
+        if (n) {
+            HIPC(hipMemcpy(key.data(), s->ps[prev].key.p, n * 8, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(ds.data(), s->ps[prev].ds.p, ds.size() * 8, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(act.data(), s->ps[prev].active.p, n, hipMemcpyDeviceToHost));
+        }
+        const size_t npw = (size_t)s->N * s->W;
+        std::vector<double> wds(3 * npw);
+        std::vector<uint8_t> wact(npw);
+        if (npw) {
+            HIPC(hipMemcpy(wds.data(), s->ps[prev].wds.p, wds.size() * 8, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(wact.data(), s->ps[prev].wactive.p, npw, hipMemcpyDeviceToHost));
+        }
+        FILE* f = std::fopen(path, "wb");
+        if (!f) throw ApiError{ZDX_IO, std::string("cannot write ") + path};
+        const char magic[8] = {'Z', 'D', 'X', 'C', 'K', 'P', '1', 0};
+        int64_t hdr[4] = {s->N, s->W, n, s->n_steps_done};
+        bool ok = std::fwrite(magic, 1, 8, f) == 8 && std::fwrite(hdr, 8, 4, f) == 4 &&
+                  std::fwrite(state.data(), 8, state.size(), f) == state.size() &&
+                  std::fwrite(key.data(), 8, key.size(), f) == key.size() &&
+                  std::fwrite(ds.data(), 8, ds.size(), f) == ds.size() &&
+                  std::fwrite(act.data(), 1, act.size(), f) == act.size() &&
+                  std::fwrite(wds.data(), 8, wds.size(), f) == wds.size() &&
+                  std::fwrite(wact.data(), 1, wact.size(), f) == wact.size();
+        std::fclose(f);
+        if (!ok) throw ApiError{ZDX_IO, "checkpoint write failed"};
+    });
+}
+
+zdx_status zdx_sim_restore(zdx_sim* s, const char* path) {
+    if (!s || !path) return ZDX_BAD_ARG;
+    return guard(s->ctx, [&] {
+        HIPC(hipStreamSynchronize(s->st()));
+        FILE* f = std::fopen(path, "rb");
+        if (!f) throw ApiError{ZDX_IO, std::string("cannot read ") + path};
+        char magic[8];
+        int64_t hdr[4];
+        bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "ZDXCKP1", 7) == 0 &&
+                  std::fread(hdr, 8, 4, f) == 4 && hdr[0] == s->N && hdr[1] == s->W;
+        if (!ok) { std::fclose(f); throw ApiError{ZDX_IO, "checkpoint does not match this simulation"}; }
+        const int32_t n = (int32_t)hdr[2];
+        std::vector<double> state(16 * (size_t)s->N);
+        std::vector<int64_t> key(n);
+        std::vector<double> ds(3 * (size_t)n);
+        std::vector<uint8_t> act(n);
+        const size_t npw = (size_t)s->N * s->W;
+        std::vector<double> wds(3 * npw);
+        std::vector<uint8_t> wact(npw);
+        ok = std::fread(state.data(), 8, state.size(), f) == state.size() &&
+             std::fread(key.data(), 8, key.size(), f) == key.size() &&
+             std::fread(ds.data(), 8, ds.size(), f) == ds.size() &&
+             std::fread(act.data(), 1, act.size(), f) == act.size() &&
+             std::fread(wds.data(), 8, wds.size(), f) == wds.size() &&
+             std::fread(wact.data(), 1, wact.size(), f) == wact.size();
+        std::fclose(f);
+        if (!ok) throw ApiError{ZDX_IO, "checkpoint truncated"};
+        if (n > s->pair_cap) s->alloc_pairs(n + n / 4 + 16);
+        const int prev = 1 - s->parity;
+        HIPC(hipMemcpy(s->d_state[s->parity].p, state.data(), state.size() * 8, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(s->ps[prev].n.p, &n, 4, hipMemcpyHostToDevice));
+        if (n) {
+            HIPC(hipMemcpy(s->ps[prev].key.p, key.data(), n * 8, hipMemcpyHostToDevice));
+            HIPC(hipMemcpy(s->ps[prev].ds.p, ds.data(), ds.size() * 8, hipMemcpyHostToDevice));
+            HIPC(hipMemcpy(s->ps[prev].active.p, act.data(), n, hipMemcpyHostToDevice));
+        }
+        if (npw) {
+            HIPC(hipMemcpy(s->ps[prev].wds.p, wds.data(), wds.size() * 8, hipMemcpyHostToDevice));
+            HIPC(hipMemcpy(s->ps[prev].wactive.p, wact.data(), npw, hipMemcpyHostToDevice));
+        }
+        s->n_steps_done = hdr[3];
+    });
+}
+
+zdx_status zdx_sim_stage_times(zdx_sim* s, double* ms, int32_t n) {
+    if (!s || !ms) return ZDX_BAD_ARG;
+    return guard(s->ctx, [&] {
+        time_stages(s);
+        for (int k = 0; k < n && k < 6; ++k) ms[k] = s->stage_ms[k];
+    });
+}
+
+zdx_status zdx_sim_set_tiles(zdx_sim* s, int32_t block, int32_t groups_per_cu) {
+    if (!s) return ZDX_BAD_ARG;
+    if (block > 0) {
+        s->ncfg.block = block / 100 > 0 ? block / 100 : block;
+        s->ncfg.items = block % 100 > 0 ? block % 100 : 4;
+        if (block < 100) { s->ncfg.block = block; s->ncfg.items = 4; }
+    }
+    if (groups_per_cu > 0) {
+        if (groups_per_cu > 16) return ZDX_BAD_ARG;
+        s->np_groups_per_cu = groups_per_cu;
+    }
+    s->drop_graphs();
+    return ZDX_OK;
+}
+
+// Autotuner: times the narrow phase alone on the current step's pair set for
+// each (threads per workgroup, tests per thread, workgroups per CU) and keeps
+// the fastest.  Every configuration produces the identical segment lists.
+zdx_status zdx_sim_autotune(zdx_sim* s, int32_t* out_block, int32_t* out_groups) {
+    if (!s) return ZDX_BAD_ARG;
+    return guard(s->ctx, [&] {
+        hipStream_t st = s->st();
+        const int par = s->parity;
+        DBodies B = s->bodies(par);
+        DMeshes M = s->ctx->dmeshes();
+        launch_fill_i32(s->d_status.p, 0, 1, st);
+        launch_transform(B, M, st);
+        launch_broadphase(s->bpbufs(par), st);
+        HIPC(hipStreamSynchronize(st));
+        if (read_status(s) & ST_PAIRS) {
+            grow_for(s, ST_PAIRS, par);
+            launch_fill_i32(s->d_status.p, 0, 1, st);
+            launch_broadphase(s->bpbufs(par), st);
+        }
+        DPairs P = s->dpairs(par, true);
+        launch_lpt_order(P, M, s->d_mesh.p, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st);
+        const int cand[][2] = {{128, 4}, {128, 8}, {256, 1}, {256, 2}, {256, 4}, {256, 8}, {512, 2}, {512, 4}};
+        const int gpc[] = {4, 8, 16};
+        hipEvent_t e0, e1;
+        HIPC(hipEventCreate(&e0));
+        HIPC(hipEventCreate(&e1));
+        float best = 1e30f;
+        NarrowCfg bestc = s->ncfg;
+        int bestg = s->np_groups_per_cu;
+        for (auto& cb : cand) {
+            for (int g : gpc) {
+                NarrowCfg nc{cb[0], cb[1], persistent_groups(s->ctx, g), 1};
+                float tot = 0;
+                for (int rep = 0; rep < 2; ++rep) {
+                    launch_fill_u64(s->d_arena_top.p, 0ull, 1, st);
+                    launch_fill_i32((int32_t*)s->d_queue.p, 0, 1, st);
+                    HIPC(hipEventRecord(e0, st));
+                    launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->d_stage.p, s->stage_cap,
+                                  s->tmax, s->d_queue.p, st);
+                    HIPC(hipEventRecord(e1, st));
+                    HIPC(hipEventSynchronize(e1));
+                    float ms = 0;
+                    HIPC(hipEventElapsedTime(&ms, e0, e1));
+                    if (rep > 0) tot += ms;
+                }
+                if (tot < best) { best = tot; bestc = nc; bestg = g; }
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        s->ncfg.block = bestc.block;
+        s->ncfg.items = bestc.items;
+        s->np_groups_per_cu = bestg;
+        s->drop_graphs();
+        if (out_block) *out_block = bestc.block * 100 + bestc.items;
+        if (out_groups) *out_groups = bestg;
+    });
+}
+
+}  // extern "C"

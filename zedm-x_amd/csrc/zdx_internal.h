@@ -1,0 +1,131 @@
+// zdx_internal.h — data layouts shared by the host code and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace zdx {
+
+// ---------------------------------------------------------------- host mesh
+// Result of build_mesh (src/main.cpp:445-497) plus the indexed form the GPU
+// uses: corners of the (centred) triangles that are bitwise equal share one
+// vertex slot, so each particle's vertices are rotated once per step and the
+// triangle corners read back the identical doubles.
+struct HostMesh {
+    std::vector<double> tris;      // T x 9, centred when requested
+    std::vector<double> verts;     // unique corners (bitwise), 3 doubles each
+    std::vector<int32_t> tri_idx;  // T x 3 indices into verts
+    double volume = 0, radius = 0, mean_edge = 0, bbox_diag = 0;
+    double center[3] = {0, 0, 0};
+    double inertia[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int n_unique_ref = 0;          // the reference's unique_vertices count (tolerance merge)
+};
+
+bool load_stl(const std::string& path, std::vector<double>& tris);
+HostMesh build_mesh(const std::vector<double>& tris, bool center);
+
+// config (src/main.cpp:1512-1689)
+struct ParticleInit {
+    std::string stl;
+    double pos[3] = {0, 0, 0}, vel[3] = {0, 0, 0}, omega[3] = {0, 0, 0};
+    double quat[4] = {1, 0, 0, 0};
+    double scale = 0.0, density = 1.0, young = 1e7, poisson = 0.25, mu = 0.5, rest = 0.5;
+};
+struct WallInit {
+    std::string stl;
+    double pos[3] = {0, 0, 0};
+    double quat[4] = {1, 0, 0, 0};
+    double scale = 1.0, mu = 0.5, rest = 0.5;
+};
+struct SimConfig {
+    int n = 2, steps = 1;
+    double dt = 1e-4, spacing = 2.5;
+    double gravity[3] = {0.0, -9.81, 0.0};
+    bool split = true, center = true, contact_debug = false;
+    std::string stl, vtk_prefix = "particles", output_dir = "output";
+    int output_interval = 1;
+    double v0[3] = {0, 0, 0}, v1[3] = {0, 0, 0};
+    std::vector<ParticleInit> parts;
+    std::vector<WallInit> walls;
+};
+bool parse_config(const std::string& path, SimConfig& cfg);
+
+// per-particle constants derived at set-up (src/main.cpp:1955-1985)
+struct ParticleConst {
+    double mass, inv_mass, Ibody[9], Ibinv[9], radius, eqr, young, poisson, mu, rest;
+    int32_t mesh;
+};
+ParticleConst particle_const(const HostMesh& m, int mesh_id, double density, double young, double poisson,
+                             double mu, double rest);
+// L = I_world * omega (src/main.cpp:1985)
+void initial_L(const ParticleConst& c, const double* quat, const double* omega, double* L);
+
+// ---------------------------------------------------------------- device views
+// Registered meshes, concatenated.
+struct DMeshes {
+    const double* verts;     // body-frame vertices
+    const int32_t* tri_idx;  // local vertex indices, 3 per triangle
+    const int32_t* vert_off; // per mesh
+    const int32_t* nverts;
+    const int32_t* tri_off;
+    const int32_t* ntris;
+    const double* tol;       // per mesh: mean_edge*0.1 rule (main.cpp:2118-2121)
+    const double* area_eps;  // per mesh (main.cpp:2205)
+};
+
+// Bodies whose world geometry is evaluated each step.
+struct DBodies {
+    int32_t n;
+    const int32_t* mesh;     // per body
+    const double* state;     // 16 doubles per body (pos quat vel omega L)
+    const int64_t* wv_off;   // per body: first world vertex
+    const int64_t* wt_off;   // per body: first triangle flag
+    double* wv;              // world vertices (3 doubles each)
+    uint8_t* degen;          // per triangle: 1 if plane_from_tri fails on the world triangle
+    int32_t* nondeg;         // per body: non-degenerate triangle count
+    double* aabb;            // per body: min(3) max(3) over world vertices
+};
+
+struct DMat {                // per particle material / inertia (sim only)
+    const double* mass;
+    const double* inv_mass;
+    const double* Ibinv;     // 9 per particle
+    const double* eqr;
+    const double* young;
+    const double* poisson;
+    const double* mu;
+    const double* rest;
+    const double* radius;
+};
+
+// Narrow-phase output arena and per-pair results.
+struct DPairs {
+    int32_t cap;             // capacity of the pair arrays
+    const int32_t* n;        // device count of pairs
+    const int32_t* pi;       // body A (lower index in the sim)
+    const int32_t* pj;       // body B
+    const int32_t* order;    // processing order (LPT in batched mode)
+    int64_t* seg_off;        // per pair: first segment in the arena
+    int32_t* nseg;           // per pair
+    int64_t* tri_checks;     // per pair (reference counter)
+    int64_t* tri_tests;      // per pair (executed AABB tests)
+    int32_t* tri_hits;       // per pair
+    double* arena;           // segments, 6 doubles each
+    int64_t arena_cap;       // in segments
+    unsigned long long* arena_top;
+    int32_t* status;         // global error bits
+};
+
+enum : int32_t {
+    ST_ARENA = 1,        // segment arena full
+    ST_STAGE = 2,        // a pair produced more segments than the staging capacity
+    ST_TOPO = 4,         // topology scratch or loop capacity exceeded
+    ST_PAIRS = 8,        // broad-phase pair capacity exceeded
+    ST_NEIGH = 16,       // more neighbours than the per-particle candidate capacity
+    ST_WALL = 32,        // wall section capacity exceeded
+    ST_GRID = 64         // broad-phase grid could not be built (non-finite positions)
+};
+
+}  // namespace zdx

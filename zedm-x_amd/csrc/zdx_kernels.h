@@ -1,0 +1,127 @@
+// zdx_kernels.h — host-side launchers of the HIP kernels (zdx_kernels.hip).
+#pragma once
+#include "zdx_internal.h"
+
+namespace zdx {
+
+// Broad-phase grid: origin, cell size and dimensions chosen on the device.
+struct DGrid {
+    double ox, oy, oz, h;
+    int32_t nx, ny, nz;
+    int32_t ncells;
+};
+
+struct BroadPhaseBufs {
+    int32_t n;                 // particles
+    const double* state;       // 16 per particle
+    const double* radius;
+    DGrid* grid;               // device
+    int32_t max_cells;
+    int32_t* cell_of;          // per particle
+    int32_t* cell_count;       // max_cells
+    int32_t* cell_start;       // max_cells + 1
+    int32_t* cell_cursor;      // max_cells
+    int32_t* sorted;           // per particle
+    int32_t* cnt_up;           // per particle
+    int32_t* cnt_all;          // per particle
+    int32_t* pair_off;         // n + 1
+    int32_t* adj_off;          // n + 1
+    int32_t* pi;               // pair capacity
+    int32_t* pj;
+    int64_t* pkey;
+    int32_t pair_cap;
+    int32_t* adj_j;            // adjacency capacity
+    int32_t* adj_pair;
+    int32_t adj_cap;
+    int32_t* n_pairs;          // device scalar
+    int32_t* n_adj;            // device scalar
+    int32_t* scan_tmp;         // scan block sums (>= max_cells/1024 + 2)
+    int32_t* status;
+};
+
+struct WallBufs {
+    int32_t n_walls;
+    const double* tris;        // world wall triangles, 9 doubles each
+    const double* normals;     // per wall triangle
+    const int32_t* tri_off;    // per wall
+    const int32_t* ntri;       // per wall
+    const double* pos;         // per wall
+    const double* radius;      // per wall (mesh radius)
+    const double* mu;          // per wall
+    int32_t max_wall_tris;     // contributions per (particle, wall)
+};
+
+// Per-step history (tangential springs): the previous step's sorted pair
+// keys with their final ds and active flags, and this step's output.
+struct HistBufs {
+    const int64_t* prev_key;
+    const double* prev_ds;
+    const uint8_t* prev_active;
+    const int32_t* prev_n;
+    double* cur_ds;            // per pair of this step
+    uint8_t* cur_active;
+    // walls, dense [particle * n_walls + w]
+    const double* wprev_ds;
+    const uint8_t* wprev_active;
+    double* wcur_ds;
+    uint8_t* wcur_active;
+};
+
+struct TopoBufs {
+    int32_t lmax;              // contact records per pair
+    double* contrib;           // pair * lmax * 9  (f, rA, rB)
+    int32_t* ncontrib;         // per pair: accepted loops
+    int32_t* comps;            // per pair
+    int32_t* loops_raw;        // per pair
+    int32_t* tstatus;          // per pair
+    int32_t* scr_i;
+    double* scr_d;
+    int64_t nthreads;          // persistent threads (scratch stride)
+    int32_t seg_max;           // scratch sized for this many segments
+    // API mode outputs
+    double* rec;               // pair * lmax * 28
+    double* vtx;               // pair * vmax * 3
+    int32_t vmax;
+    double k_stiff;
+};
+
+struct SimParams {
+    double g[3];
+    double dt;
+    int32_t split;
+};
+
+struct NarrowCfg {
+    int32_t block;             // 128, 256 or 512 threads
+    int32_t items;             // AABB tests per thread per round
+    int32_t groups;            // persistent workgroups
+    int32_t use_queue;         // 1: atomic work queue (batched)
+};
+
+void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s);
+void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s);      // all stages, device-sized
+void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* list_i,
+                   float* list_f, double* stage, int32_t stage_cap, int32_t tmax, unsigned int* queue,
+                   hipStream_t s);
+void launch_lpt_order(const DPairs& P, const DMeshes& M, const int32_t* body_mesh, int32_t* order, int32_t* bucket_cnt,
+                      int32_t* bucket_cur, int32_t pair_cap, hipStream_t s);
+void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
+                     const TopoBufs& T, const SimParams& sp, hipStream_t s);
+void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int32_t split,
+                     hipStream_t s);
+void launch_walls(const DBodies& B, const DMeshes& M, const DMat& mat, const WallBufs& W, const HistBufs& H,
+                  double* wcontrib, int32_t* wncontrib, double* scr, int64_t nthreads, int32_t pts_cap,
+                  const SimParams& sp, int32_t* status, hipStream_t s);
+void launch_fold_integrate(int32_t n, const double* state_in, double* state_out, const DMat& mat,
+                           const int32_t* adj_off, const int32_t* adj_j, const int32_t* adj_pair,
+                           int32_t adj_cap, const TopoBufs& T, int32_t n_walls, const double* wcontrib, const int32_t* wncontrib,
+                           int32_t max_wall_tris, double* F, double* Tq, int32_t* counts, const SimParams& sp,
+                           hipStream_t s);
+// counters[ZDX_N_COUNTERS] (int64) reduced from per-pair / per-wall arrays
+void launch_counters(const DPairs& P, const TopoBufs& T, int32_t n_pw, const int32_t* wncontrib,
+                     unsigned long long* counters, hipStream_t s);
+void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
+void launch_fill_u64(unsigned long long* p, unsigned long long v, int64_t n, hipStream_t s);
+void launch_iota_pairs(int32_t* pi, int32_t* pj, int32_t n, hipStream_t s);
+
+}  // namespace zdx

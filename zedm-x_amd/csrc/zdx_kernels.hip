@@ -1,0 +1,1037 @@
+// zdx_kernels.hip — gfx950 kernels of the Route-B contact step.
+//
+// FP64 throughout; built with -ffp-contract=off so that every expression
+// rounds as the reference's CPU build does (see zdx_geom.h).
+//
+//   k_transform       one workgroup per body: world vertices (quat_rotate of
+//                     each unique body vertex), body AABB, degenerate-triangle
+//                     flags and counts
+//   broad phase       device-sized uniform grid (cell >= 2 r_max), counting
+//                     sort of particles by cell, per-particle neighbour
+//                     enumeration, scans, ordered pair list (i < j) and the
+//                     per-particle adjacency used by the ordered force fold
+//   k_narrow          persistent workgroups, one pair at a time: cull each
+//                     mesh's triangles against the other mesh's AABB, then a
+//                     flattened, order-preserving sweep of the culled product
+//                     with float (outward-rounded) AABB prefilter, exact FP64
+//                     AABB re-test and tri-tri on the hits; ballot/prefix-sum
+//                     compaction keeps the reference's (ta, tb) segment order
+//   k_topo_*          one thread per pair over strided scratch: components,
+//                     snap, T-split, loops, Sn/Gn, force (zdx_geom.h)
+//   k_walls           one thread per (particle, wall)
+//   k_fold_integrate  one thread per particle: ordered, atomic-free fold of
+//                     the contact records in the reference's summation order,
+//                     then the rigid-body integrator
+#include <hip/hip_runtime.h>
+
+#include "zdx_geom.h"
+#include "zdx_kernels.h"
+
+namespace zdx {
+
+namespace {
+
+constexpr int WAVE = 64;
+
+__device__ inline int lane_id() { return __lane_id(); }
+
+// Exclusive rank of `pred` among the workgroup's threads (thread order) and
+// the total count.  Uses wave ballots and one LDS array of per-wave counts.
+template <int BS>
+__device__ inline int block_rank(bool pred, int* s_wave, int& total) {
+    constexpr int NW = BS / WAVE;
+    const int lane = lane_id();
+    const int wid = threadIdx.x / WAVE;
+    unsigned long long m = __ballot(pred);
+    unsigned long long below = (lane == 0) ? 0ull : (m & ((~0ull) >> (64 - lane)));
+    int r = __popcll(below);
+    if (lane == 0) s_wave[wid] = __popcll(m);
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        int c = s_wave[w];
+        off += (w < wid) ? c : 0;
+        tot += c;
+    }
+    __syncthreads();
+    total = tot;
+    return off + r;
+}
+
+// Exclusive prefix of per-thread counts (thread order) and the total.
+template <int BS>
+__device__ inline int block_excl(int v, int* s_wave, int& total) {
+    constexpr int NW = BS / WAVE;
+    const int lane = lane_id();
+    const int wid = threadIdx.x / WAVE;
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        int y = __shfl_up(x, d, WAVE);
+        if (lane >= d) x += y;
+    }
+    if (lane == WAVE - 1) s_wave[wid] = x;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        int c = s_wave[w];
+        off += (w < wid) ? c : 0;
+        tot += c;
+    }
+    __syncthreads();
+    total = tot;
+    return off + x - v;
+}
+
+__device__ inline V3 ld3(const double* p) { return v3(p[0], p[1], p[2]); }
+
+// ------------------------------------------------------------ transform
+__global__ void __launch_bounds__(256) k_transform(DBodies B, DMeshes M) {
+    __shared__ double s_red[6][256 / WAVE];
+    __shared__ int s_cnt;
+    const int b = blockIdx.x;
+    if (b >= B.n) return;
+    const int m = B.mesh[b];
+    const double* st = B.state + 16 * (int64_t)b;
+    const V3 pos = v3(st[0], st[1], st[2]);
+    Qt q; q.w = st[3]; q.x = st[4]; q.y = st[5]; q.z = st[6];
+    const int nv = M.nverts[m];
+    const double* bv = M.verts + 3 * (int64_t)M.vert_off[m];
+    double* wv = B.wv + 3 * B.wv_off[b];
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int v = threadIdx.x; v < nv; v += blockDim.x) {
+        V3 w = add(qrotate(q, ld3(bv + 3 * v)), pos);
+        wv[3 * v] = w.x; wv[3 * v + 1] = w.y; wv[3 * v + 2] = w.z;
+        mn[0] = fmin(mn[0], w.x); mn[1] = fmin(mn[1], w.y); mn[2] = fmin(mn[2], w.z);
+        mx[0] = fmax(mx[0], w.x); mx[1] = fmax(mx[1], w.y); mx[2] = fmax(mx[2], w.z);
+    }
+    for (int k = 0; k < 3; ++k) {
+        for (int d = 32; d >= 1; d >>= 1) {
+            mn[k] = fmin(mn[k], __shfl_xor(mn[k], d, WAVE));
+            mx[k] = fmax(mx[k], __shfl_xor(mx[k], d, WAVE));
+        }
+    }
+    const int wid = threadIdx.x / WAVE;
+    if (lane_id() == 0) {
+        for (int k = 0; k < 3; ++k) { s_red[k][wid] = mn[k]; s_red[3 + k][wid] = mx[k]; }
+    }
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double* box = B.aabb + 6 * (int64_t)b;
+        for (int k = 0; k < 3; ++k) {
+            double a = s_red[k][0], c = s_red[3 + k][0];
+            for (int w = 1; w < (int)(blockDim.x / WAVE); ++w) { a = fmin(a, s_red[k][w]); c = fmax(c, s_red[3 + k][w]); }
+            box[k] = a; box[3 + k] = c;
+        }
+    }
+    // degenerate flags on the world triangles (plane_from_tri, main.cpp:518-527)
+    const int T = M.ntris[m];
+    const int32_t* ti = M.tri_idx + 3 * (int64_t)M.tri_off[m];
+    uint8_t* dg = B.degen + B.wt_off[b];
+    int good = 0;
+    for (int t = threadIdx.x; t < T; t += blockDim.x) {
+        V3 a = ld3(wv + 3 * ti[3 * t]), c1 = ld3(wv + 3 * ti[3 * t + 1]), c2 = ld3(wv + 3 * ti[3 * t + 2]);
+        V3 n; double d;
+        bool ok = plane_of(a, c1, c2, n, d);
+        dg[t] = ok ? 0 : 1;
+        good += ok ? 1 : 0;
+    }
+    for (int d = 32; d >= 1; d >>= 1) good += __shfl_xor(good, d, WAVE);
+    if (lane_id() == 0) atomicAdd(&s_cnt, good);
+    __syncthreads();
+    if (threadIdx.x == 0) B.nondeg[b] = s_cnt;
+}
+
+// ------------------------------------------------------------ scan
+// exclusive scan of int32 counts (length *n_ptr or n) into out[0..n], out[n] = total
+__global__ void __launch_bounds__(256) k_scan_reduce(const int32_t* in, const int32_t* n_ptr, int32_t n_fixed,
+                                                    int32_t* bsum) {
+    const int n = n_ptr ? *n_ptr : n_fixed;
+    const int64_t base = (int64_t)blockIdx.x * 1024;
+    int s = 0;
+    for (int k = 0; k < 4; ++k) {
+        int64_t i = base + threadIdx.x * 4 + k;
+        if (i < n) s += in[i];
+    }
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, WAVE);
+    __shared__ int sw[4];
+    if (lane_id() == 0) sw[threadIdx.x / WAVE] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) bsum[blockIdx.x] = sw[0] + sw[1] + sw[2] + sw[3];
+}
+__global__ void __launch_bounds__(1024) k_scan_blocks(int32_t* bsum, int32_t nb_fixed, const int32_t* n_ptr) {
+    // single workgroup: exclusive scan of the block sums in place; bsum[nb] = total
+    __shared__ int sw[16];
+    const int n = n_ptr ? *n_ptr : 0;
+    const int nb = n_ptr ? (n + 1023) / 1024 : nb_fixed;
+    int carry = 0;
+    for (int base = 0; base < nb; base += 1024) {
+        int i = base + threadIdx.x;
+        int v = (i < nb) ? bsum[i] : 0;
+        int x = v;
+        for (int d = 1; d < WAVE; d <<= 1) {
+            int y = __shfl_up(x, d, WAVE);
+            if (lane_id() >= d) x += y;
+        }
+        if (lane_id() == WAVE - 1) sw[threadIdx.x / WAVE] = x;
+        __syncthreads();
+        int off = 0, tot = 0;
+        for (int w = 0; w < 16; ++w) { off += (w < (int)threadIdx.x / WAVE) ? sw[w] : 0; tot += sw[w]; }
+        if (i < nb) bsum[i] = carry + off + x - v;
+        __syncthreads();
+        carry += tot;
+    }
+    if (threadIdx.x == 0) bsum[nb] = carry;
+}
+__global__ void __launch_bounds__(256) k_scan_apply(const int32_t* in, const int32_t* n_ptr, int32_t n_fixed,
+                                                   const int32_t* bsum, int32_t* out, int32_t* total_out) {
+    __shared__ int sw[4];
+    const int n = n_ptr ? *n_ptr : n_fixed;
+    const int64_t base = (int64_t)blockIdx.x * 1024;
+    if (base > n) return;
+    int v[4], s = 0;
+    for (int k = 0; k < 4; ++k) {
+        int64_t i = base + threadIdx.x * 4 + k;
+        v[k] = (i < n) ? in[i] : 0;
+        s += v[k];
+    }
+    int total;
+    int off = block_excl<256>(s, sw, total);
+    int run = bsum[blockIdx.x] + off;
+    for (int k = 0; k < 4; ++k) {
+        int64_t i = base + threadIdx.x * 4 + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+    const int nb = (n + 1023) / 1024;
+    if (blockIdx.x == (unsigned)(nb > 0 ? nb - 1 : 0) && threadIdx.x == 0) {
+        out[n] = bsum[nb];
+        if (total_out) *total_out = bsum[nb];
+    }
+    (void)total;
+}
+
+void scan_i32(const int32_t* in, const int32_t* n_ptr, int32_t n_max, int32_t* tmp, int32_t* out, int32_t* total,
+              hipStream_t s) {
+    int nb = (n_max + 1023) / 1024;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, in, n_ptr, n_max, tmp);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, tmp, nb, n_ptr);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, in, n_ptr, n_max, tmp, out, total);
+}
+
+// ------------------------------------------------------------ broad phase
+constexpr int NEIGH_CAP = 128;
+
+__global__ void __launch_bounds__(1024) k_bp_bounds(const double* state, const double* radius, int n, DGrid* g,
+                                                    int max_cells, int32_t* status) {
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY}, rm = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double* p = state + 16 * (int64_t)i;
+        for (int k = 0; k < 3; ++k) { mn[k] = fmin(mn[k], p[k]); mx[k] = fmax(mx[k], p[k]); }
+        rm = fmax(rm, radius[i]);
+    }
+    __shared__ double s[7][16];
+    for (int d = 32; d >= 1; d >>= 1) {
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = fmin(mn[k], __shfl_xor(mn[k], d, WAVE));
+            mx[k] = fmax(mx[k], __shfl_xor(mx[k], d, WAVE));
+        }
+        rm = fmax(rm, __shfl_xor(rm, d, WAVE));
+    }
+    if (lane_id() == 0) {
+        int w = threadIdx.x / WAVE;
+        for (int k = 0; k < 3; ++k) { s[k][w] = mn[k]; s[3 + k][w] = mx[k]; }
+        s[6][w] = rm;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    int nw = blockDim.x / WAVE;
+    for (int w = 1; w < nw; ++w) {
+        for (int k = 0; k < 3; ++k) { s[k][0] = fmin(s[k][0], s[k][w]); s[3 + k][0] = fmax(s[3 + k][0], s[3 + k][w]); }
+        s[6][0] = fmax(s[6][0], s[6][w]);
+    }
+    DGrid G;
+    G.nx = G.ny = G.nz = 1;
+    G.ox = s[0][0]; G.oy = s[1][0]; G.oz = s[2][0];
+    double ex = s[3][0] - s[0][0], ey = s[4][0] - s[1][0], ez = s[5][0] - s[2][0];
+    bool finite = isfinite(ex) && isfinite(ey) && isfinite(ez) && isfinite(s[6][0]);
+    // cell edge >= 2 r_max (plus margin for the rounding of (p - o) / h), so
+    // every pair with |dp| <= r_i + r_j lies in adjacent cells
+    double h = 2.0 * s[6][0] * (1.0 + 1e-6) + 1e-300;
+    if (!finite) { h = 1.0; ex = ey = ez = 0.0; atomicOr(status, (int)ST_GRID); }
+    for (int it = 0; it < 64; ++it) {
+        double fx = floor(ex / h) + 1.0, fy = floor(ey / h) + 1.0, fz = floor(ez / h) + 1.0;
+        if (fx * fy * fz <= (double)max_cells) {
+            G.nx = (int)fx; G.ny = (int)fy; G.nz = (int)fz;
+            break;
+        }
+        h *= 1.25;
+    }
+    G.h = h;
+    G.ncells = G.nx * G.ny * G.nz;
+    *g = G;
+}
+
+__device__ inline void cell_coords(const DGrid& G, const double* p, int& cx, int& cy, int& cz) {
+    cx = (int)floor((p[0] - G.ox) / G.h);
+    cy = (int)floor((p[1] - G.oy) / G.h);
+    cz = (int)floor((p[2] - G.oz) / G.h);
+    cx = cx < 0 ? 0 : (cx >= G.nx ? G.nx - 1 : cx);
+    cy = cy < 0 ? 0 : (cy >= G.ny ? G.ny - 1 : cy);
+    cz = cz < 0 ? 0 : (cz >= G.nz ? G.nz - 1 : cz);
+}
+
+__global__ void k_fill_i32(int32_t* p, int32_t v, const int32_t* n_ptr, int64_t n) {
+    int64_t m = n_ptr ? (int64_t)*n_ptr : n;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void __launch_bounds__(256) k_bp_cells(BroadPhaseBufs bp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= bp.n) return;
+    const DGrid G = *bp.grid;
+    int cx, cy, cz;
+    cell_coords(G, bp.state + 16 * (int64_t)i, cx, cy, cz);
+    int c = (cz * G.ny + cy) * G.nx + cx;
+    bp.cell_of[i] = c;
+    atomicAdd(&bp.cell_count[c], 1);
+}
+
+__global__ void __launch_bounds__(256) k_bp_scatter(BroadPhaseBufs bp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= bp.n) return;
+    int c = bp.cell_of[i];
+    int slot = bp.cell_start[c] + atomicAdd(&bp.cell_cursor[c], 1);
+    bp.sorted[slot] = i;
+}
+
+// sphere test of the reference (main.cpp:2054-2059) for the pair (min, max)
+__device__ inline bool sphere_pass(const BroadPhaseBufs& bp, int i, int j) {
+    int a = i < j ? i : j, b = i < j ? j : i;
+    const double* pa = bp.state + 16 * (int64_t)a;
+    const double* pb = bp.state + 16 * (int64_t)b;
+    V3 dp = sub(ld3(pb), ld3(pa));
+    double d2 = dot(dp, dp);
+    double rs = bp.radius[a] + bp.radius[b];
+    return !(d2 > rs * rs);
+}
+
+template <class F>
+__device__ inline void for_neighbours(const BroadPhaseBufs& bp, const DGrid& G, int i, F&& f) {
+    int cx, cy, cz;
+    cell_coords(G, bp.state + 16 * (int64_t)i, cx, cy, cz);
+    for (int dz = -1; dz <= 1; ++dz) {
+        int z = cz + dz;
+        if (z < 0 || z >= G.nz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int y = cy + dy;
+            if (y < 0 || y >= G.ny) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                int x = cx + dx;
+                if (x < 0 || x >= G.nx) continue;
+                int c = (z * G.ny + y) * G.nx + x;
+                int s0 = bp.cell_start[c], s1 = bp.cell_start[c + 1];
+                for (int k = s0; k < s1; ++k) {
+                    int j = bp.sorted[k];
+                    if (j != i && sphere_pass(bp, i, j)) f(j);
+                }
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_bp_count(BroadPhaseBufs bp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= bp.n) return;
+    const DGrid G = *bp.grid;
+    int up = 0, all = 0;
+    for_neighbours(bp, G, i, [&](int j) { all++; if (j > i) up++; });
+    if (all > NEIGH_CAP) { atomicOr(bp.status, (int)ST_NEIGH); all = NEIGH_CAP; up = up > NEIGH_CAP ? NEIGH_CAP : up; }
+    bp.cnt_up[i] = up;
+    bp.cnt_all[i] = all;
+}
+
+__global__ void __launch_bounds__(128) k_bp_fill(BroadPhaseBufs bp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= bp.n) return;
+    const DGrid G = *bp.grid;
+    int js[NEIGH_CAP];
+    int n = 0;
+    for_neighbours(bp, G, i, [&](int j) { if (n < NEIGH_CAP) js[n++] = j; });
+    for (int a = 1; a < n; ++a) {
+        int v = js[a], b = a - 1;
+        while (b >= 0 && js[b] > v) { js[b + 1] = js[b]; --b; }
+        js[b + 1] = v;
+    }
+    const int ao = bp.adj_off[i];
+    const int po = bp.pair_off[i];
+    if (bp.adj_off[bp.n] > bp.adj_cap || bp.pair_off[bp.n] > bp.pair_cap) {
+        atomicOr(bp.status, (int)ST_PAIRS);
+        return;
+    }
+    int u = 0;
+    for (int k = 0; k < n; ++k) {
+        int j = js[k];
+        bp.adj_j[ao + k] = j;
+        if (j > i) {
+            int p = po + u;
+            bp.pi[p] = i;
+            bp.pj[p] = j;
+            bp.pkey[p] = ((int64_t)i << 32) | (int64_t)(uint32_t)j;
+            bp.adj_pair[ao + k] = p;
+            ++u;
+        } else {
+            bp.adj_pair[ao + k] = -1;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_bp_adjfix(BroadPhaseBufs bp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= bp.n) return;
+    if (bp.adj_off[bp.n] > bp.adj_cap || bp.pair_off[bp.n] > bp.pair_cap) return;
+    const int a0 = bp.adj_off[i], a1 = bp.adj_off[i + 1];
+    for (int k = a0; k < a1; ++k) {
+        int j = bp.adj_j[k];
+        if (j > i) continue;
+        int lo = bp.pair_off[j], hi = bp.pair_off[j + 1] - 1, found = -1;
+        while (lo <= hi) {
+            int mid = (lo + hi) >> 1;
+            int v = bp.pj[mid];
+            if (v == i) { found = mid; break; }
+            if (v < i) lo = mid + 1; else hi = mid - 1;
+        }
+        bp.adj_pair[k] = found;
+    }
+}
+
+__global__ void k_bp_totals(BroadPhaseBufs bp) {
+    int np = bp.pair_off[bp.n], na = bp.adj_off[bp.n];
+    // on overflow the pair and adjacency lists were not written: later stages
+    // see no pairs (the fold checks adj_off[n] against the capacity) and the
+    // host grows the buffers and runs the step again
+    bool over = np > bp.pair_cap || na > bp.adj_cap || (*bp.status & (int)ST_NEIGH);
+    if (over) atomicOr(bp.status, (int)ST_PAIRS);
+    *bp.n_pairs = over ? 0 : np;
+    *bp.n_adj = over ? 0 : na;
+}
+
+__global__ void k_grid_cells(const DGrid* g, int32_t* ncells) { *ncells = g->ncells; }
+
+// ------------------------------------------------------------ narrow phase
+// outward-rounded float AABB of a double box
+__device__ inline void box_f(const double* lo, const double* hi, float* f) {
+    f[0] = __double2float_rd(lo[0]); f[1] = __double2float_rd(lo[1]); f[2] = __double2float_rd(lo[2]);
+    f[3] = __double2float_ru(hi[0]); f[4] = __double2float_ru(hi[1]); f[5] = __double2float_ru(hi[2]);
+}
+__device__ inline bool ovl_f(const float* a, const float* b) {
+    return !(a[3] < b[0] || a[0] > b[3] || a[4] < b[1] || a[1] > b[4] || a[5] < b[2] || a[2] > b[5]);
+}
+__device__ inline void tri_box(const V3* t, double* lo, double* hi) {
+    lo[0] = min3(t[0].x, t[1].x, t[2].x); lo[1] = min3(t[0].y, t[1].y, t[2].y); lo[2] = min3(t[0].z, t[1].z, t[2].z);
+    hi[0] = max3(t[0].x, t[1].x, t[2].x); hi[1] = max3(t[0].y, t[1].y, t[2].y); hi[2] = max3(t[0].z, t[1].z, t[2].z);
+}
+__device__ inline void load_tri(const double* wv, const int32_t* ti, int t, V3* out) {
+    out[0] = ld3(wv + 3 * ti[3 * t]);
+    out[1] = ld3(wv + 3 * ti[3 * t + 1]);
+    out[2] = ld3(wv + 3 * ti[3 * t + 2]);
+}
+
+template <int BS, int K>
+__global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, int32_t* list_i, float* list_f,
+                                               double* stage, int32_t stage_cap, int32_t tmax, int32_t use_queue,
+                                               unsigned int* queue) {
+    __shared__ int s_wave[BS / WAVE];
+    __shared__ int s_item;
+    __shared__ int s_ha[BS * K];
+    __shared__ int s_hb[BS * K];
+    __shared__ unsigned long long s_base;
+    const int tid = threadIdx.x;
+    int32_t* LA = list_i + (int64_t)blockIdx.x * 2 * tmax;
+    int32_t* LB = LA + tmax;
+    float* FA = list_f + (int64_t)blockIdx.x * 12 * tmax;
+    float* FB = FA + 6 * tmax;
+    double* ST = stage + (int64_t)blockIdx.x * 6 * stage_cap;
+    const int npairs = *P.n;
+    int item = blockIdx.x;
+    while (true) {
+        if (use_queue) {
+            if (tid == 0) s_item = (int)atomicAdd(queue, 1u);
+            __syncthreads();
+            item = s_item;
+            __syncthreads();
+        }
+        if (item >= npairs) break;
+        const int p = P.order ? P.order[item] : item;
+        const int i = P.pi[p], j = P.pj[p];
+        const int mA = B.mesh[i], mB = B.mesh[j];
+        const int TA = M.ntris[mA], TB = M.ntris[mB];
+        const int32_t* tiA = M.tri_idx + 3 * (int64_t)M.tri_off[mA];
+        const int32_t* tiB = M.tri_idx + 3 * (int64_t)M.tri_off[mB];
+        const double* wvA = B.wv + 3 * B.wv_off[i];
+        const double* wvB = B.wv + 3 * B.wv_off[j];
+        const uint8_t* dgA = B.degen + B.wt_off[i];
+        const uint8_t* dgB = B.degen + B.wt_off[j];
+        float boxA[6], boxB[6];
+        box_f(B.aabb + 6 * (int64_t)i, B.aabb + 6 * (int64_t)i + 3, boxA);
+        box_f(B.aabb + 6 * (int64_t)j, B.aabb + 6 * (int64_t)j + 3, boxB);
+        // ---- cull each side against the other body's AABB (conservative)
+        int nA = 0, nB = 0;
+        for (int base = 0; base < TA; base += BS) {
+            int t = base + tid;
+            bool keep = false;
+            float f[6];
+            if (t < TA && !dgA[t]) {
+                V3 tr[3]; double lo[3], hi[3];
+                load_tri(wvA, tiA, t, tr);
+                tri_box(tr, lo, hi);
+                box_f(lo, hi, f);
+                keep = ovl_f(f, boxB);
+            }
+            int tot;
+            int r = block_rank<BS>(keep, s_wave, tot);
+            if (keep) {
+                LA[nA + r] = t;
+                for (int k = 0; k < 6; ++k) FA[6 * (nA + r) + k] = f[k];
+            }
+            nA += tot;
+        }
+        for (int base = 0; base < TB; base += BS) {
+            int t = base + tid;
+            bool keep = false;
+            float f[6];
+            if (t < TB && !dgB[t]) {
+                V3 tr[3]; double lo[3], hi[3];
+                load_tri(wvB, tiB, t, tr);
+                tri_box(tr, lo, hi);
+                box_f(lo, hi, f);
+                keep = ovl_f(f, boxA);
+            }
+            int tot;
+            int r = block_rank<BS>(keep, s_wave, tot);
+            if (keep) {
+                LB[nB + r] = t;
+                for (int k = 0; k < 6; ++k) FB[6 * (nB + r) + k] = f[k];
+            }
+            nB += tot;
+        }
+        __syncthreads();
+        // ---- ordered sweep of the culled product
+        const int64_t total = (int64_t)nA * nB;
+        int nseg = 0, hits = 0;
+        for (int64_t base = 0; base < total; base += (int64_t)BS * K) {
+            int a[K], b[K];
+            int myh = 0;
+            int64_t g0 = base + (int64_t)tid * K;
+            int ca = 0, cb = 0;
+            if (g0 < total) { ca = (int)(g0 / nB); cb = (int)(g0 - (int64_t)ca * nB); }
+            float fa[6];
+            for (int k = 0; k < 6; ++k) fa[k] = FA[6 * ca + k];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                int64_t g = g0 + k;
+                if (g < total) {
+                    if (ovl_f(fa, FB + 6 * cb)) { a[myh] = ca; b[myh] = cb; ++myh; }
+                    if (++cb == nB) {
+                        cb = 0; ++ca;
+                        if (ca < nA) for (int q = 0; q < 6; ++q) fa[q] = FA[6 * ca + q];
+                    }
+                }
+            }
+            int nh;
+            int off = block_excl<BS>(myh, s_wave, nh);
+            for (int k = 0; k < myh; ++k) { s_ha[off + k] = a[k]; s_hb[off + k] = b[k]; }
+            __syncthreads();
+            hits += nh;
+            for (int hb = 0; hb < nh; hb += BS) {
+                const int h = hb + tid;
+                bool sg = false;
+                V3 p0, p1;
+                if (h < nh) {
+                    const int ta = LA[s_ha[h]], tb = LB[s_hb[h]];
+                    V3 A[3], Bt[3];
+                    load_tri(wvA, tiA, ta, A);
+                    load_tri(wvB, tiB, tb, Bt);
+                    double alo[3], ahi[3], blo[3], bhi[3];
+                    tri_box(A, alo, ahi);
+                    tri_box(Bt, blo, bhi);
+                    // exact reference reject (main.cpp:2130-2132)
+                    bool rej = ahi[0] < blo[0] || alo[0] > bhi[0] || ahi[1] < blo[1] || alo[1] > bhi[1] ||
+                               ahi[2] < blo[2] || alo[2] > bhi[2];
+                    if (!rej) {
+                        V3 nA3, nB3; double dA, dB;
+                        plane_of(A[0], A[1], A[2], nA3, dA);
+                        plane_of(Bt[0], Bt[1], Bt[2], nB3, dB);
+                        sg = tri_tri(A, nA3, dA, Bt, nB3, dB, p0, p1);
+                    }
+                }
+                int ns;
+                int r = block_rank<BS>(sg, s_wave, ns);
+                if (sg && nseg + r < stage_cap) {
+                    double* o = ST + 6 * (int64_t)(nseg + r);
+                    o[0] = p0.x; o[1] = p0.y; o[2] = p0.z; o[3] = p1.x; o[4] = p1.y; o[5] = p1.z;
+                }
+                nseg += ns;
+            }
+            __syncthreads();
+        }
+        // ---- publish: allocate arena space, copy the ordered segments
+        int nstore = nseg;
+        if (nseg > stage_cap) {
+            if (tid == 0) atomicOr(P.status, (int)ST_STAGE);
+            nstore = 0;
+        }
+        if (tid == 0) {
+            unsigned long long bse = nstore > 0 ? atomicAdd(P.arena_top, (unsigned long long)nstore) : 0ull;
+            if (bse + (unsigned long long)nstore > (unsigned long long)P.arena_cap) {
+                atomicOr(P.status, (int)ST_ARENA);
+                bse = ~0ull;
+            }
+            s_base = bse;
+        }
+        __syncthreads();
+        const unsigned long long bse = s_base;
+        if (bse != ~0ull) {
+            double* dst = P.arena + 6 * (int64_t)bse;
+            for (int k = tid; k < 6 * nstore; k += BS) dst[k] = ST[k];
+        }
+        if (tid == 0) {
+            P.seg_off[p] = bse != ~0ull ? (int64_t)bse : 0;
+            P.nseg[p] = bse != ~0ull ? nstore : 0;
+            P.tri_checks[p] = (int64_t)B.nondeg[i] * TB;
+            P.tri_tests[p] = total;
+            P.tri_hits[p] = hits;
+        }
+        __syncthreads();
+        if (!use_queue) item += gridDim.x;
+    }
+}
+
+// LPT order: pairs by descending cost bucket floor(log2(TA * TB)), so the
+// persistent queue hands out the expensive pairs first
+__global__ void k_lpt_count(DPairs P, DMeshes M, const int32_t* body_mesh, int32_t* bucket_cnt) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= *P.n) return;
+    int64_t c = (int64_t)M.ntris[body_mesh[P.pi[p]]] * M.ntris[body_mesh[P.pj[p]]];
+    int b = 63 - __clzll((unsigned long long)(c | 1));
+    atomicAdd(&bucket_cnt[63 - b], 1);
+}
+__global__ void k_lpt_scan(int32_t* bucket_cnt, int32_t* bucket_cur) {
+    if (threadIdx.x != 0) return;
+    int acc = 0;
+    for (int b = 0; b < 64; ++b) { int c = bucket_cnt[b]; bucket_cur[b] = acc; acc += c; }
+}
+__global__ void k_lpt_scatter(DPairs P, DMeshes M, const int32_t* body_mesh, int32_t* bucket_cur, int32_t* order) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= *P.n) return;
+    int64_t c = (int64_t)M.ntris[body_mesh[P.pi[p]]] * M.ntris[body_mesh[P.pj[p]]];
+    int b = 63 - __clzll((unsigned long long)(c | 1));
+    order[atomicAdd(&bucket_cur[63 - b], 1)] = p;
+}
+
+// ------------------------------------------------------------ topology
+struct SimSink {
+    double* contrib;
+    int lmax;
+    const Body* a;
+    const Body* b;
+    const Mat* ma;
+    const Mat* mb;
+    double dt;
+    V3* ds;
+    __device__ int operator()(int li, int, int, const V3&, const V3&, const V3& nA, const V3& xc, double area, int,
+                              SArr<int>, SArr<double>) {
+        Contact c = pp_force(nA, area, xc, *a, *b, *ma, *mb, dt, *ds);
+        if (li >= lmax) return TOPO_LOOP_OVERFLOW;
+        double* o = contrib + 9 * (int64_t)li;
+        o[0] = c.f.x; o[1] = c.f.y; o[2] = c.f.z;
+        o[3] = c.r1.x; o[4] = c.r1.y; o[5] = c.r1.z;
+        o[6] = c.r2.x; o[7] = c.r2.y; o[8] = c.r2.z;
+        return 0;
+    }
+};
+
+__device__ inline Mat load_mat(const DMat& m, int i) {
+    Mat r;
+    r.eqr = m.eqr[i]; r.young = m.young[i]; r.poisson = m.poisson[i]; r.mu = m.mu[i]; r.rest = m.rest[i];
+    r.inv_mass = m.inv_mass[i]; r.mass = m.mass[i];
+    return r;
+}
+
+__global__ void __launch_bounds__(256) k_topo_sim(DPairs P, DMeshes M, DBodies B, DMat mat, HistBufs H, TopoBufs T,
+                                                  SimParams sp) {
+    const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (tid >= T.nthreads) return;
+    const int n = *P.n;
+    TopoCaps caps; caps.S = T.seg_max; caps.E2 = 4 * T.seg_max;
+    SArr<int> I; I.p = T.scr_i + tid; I.s = T.nthreads;
+    SArr<double> D; D.p = T.scr_d + tid; D.s = T.nthreads;
+    const int nprev = *H.prev_n;
+    for (int p = (int)tid; p < n; p += (int)T.nthreads) {
+        const int i = P.pi[p], j = P.pj[p];
+        const int S = P.nseg[p];
+        const int mA = B.mesh[i];
+        Body a = load_body(B.state + 16 * (int64_t)i);
+        Body b = load_body(B.state + 16 * (int64_t)j);
+        Mat ma = load_mat(mat, i), mb = load_mat(mat, j);
+        // previous tangential spring of this pair (the reference's map entry,
+        // erased when the pair was not in contact on the previous step)
+        const int64_t key = ((int64_t)i << 32) | (int64_t)(uint32_t)j;
+        V3 ds = v3(0.0, 0.0, 0.0);
+        {
+            int lo = 0, hi = nprev - 1;
+            while (lo <= hi) {
+                int mid = (lo + hi) >> 1;
+                int64_t k = H.prev_key[mid];
+                if (k == key) {
+                    if (H.prev_active[mid]) ds = ld3(H.prev_ds + 3 * (int64_t)mid);
+                    break;
+                }
+                if (k < key) lo = mid + 1; else hi = mid - 1;
+            }
+        }
+        SimSink sink{T.contrib + 9 * (int64_t)p * T.lmax, T.lmax, &a, &b, &ma, &mb, sp.dt, &ds};
+        TopoOut to;
+        to.comps = 0; to.loops_raw = 0; to.loops = 0; to.status = 0;
+        if (S > T.seg_max) to.status = TOPO_E2_OVERFLOW;
+        else if (S > 0)
+            to = topology(P.arena + 6 * P.seg_off[p], S, M.tol[mA], M.area_eps[mA], sp.split != 0, sub(b.pos, a.pos),
+                          I, D, caps, sink);
+        T.ncontrib[p] = to.loops < T.lmax ? to.loops : T.lmax;
+        T.comps[p] = to.comps;
+        T.loops_raw[p] = to.loops_raw;
+        T.tstatus[p] = to.status;
+        H.cur_ds[3 * (int64_t)p] = ds.x; H.cur_ds[3 * (int64_t)p + 1] = ds.y; H.cur_ds[3 * (int64_t)p + 2] = ds.z;
+        H.cur_active[p] = to.loops > 0 ? 1 : 0;
+        if (to.status) atomicOr(P.status, (int)ST_TOPO);
+    }
+}
+
+struct ApiSink {
+    double* rec;
+    double* vtx;
+    int lmax, vmax, nv;
+    V3 posA, posB;
+    double k;
+    __device__ int operator()(int li, int comp, int raw, const V3& Sn, const V3& Gn, const V3& nA, const V3& xc,
+                              double area, int plen, SArr<int> path, SArr<double> Vs) {
+        if (li >= lmax) return TOPO_LOOP_OVERFLOW;
+        double* r = rec + ZDX_LOOP_REC_D * (int64_t)li;
+        r[0] = Sn.x; r[1] = Sn.y; r[2] = Sn.z; r[3] = Gn.x; r[4] = Gn.y; r[5] = Gn.z;
+        r[6] = nA.x; r[7] = nA.y; r[8] = nA.z; r[9] = xc.x; r[10] = xc.y; r[11] = xc.z;
+        r[12] = area;
+        // minimal-energy force, polyline.py:676-679: Fn = 0.5 k |Sn|, along n
+        double Fn = 0.5 * k * area;
+        V3 f = mul(nA, Fn);
+        V3 rA = sub(xc, posA), rB = sub(xc, posB);
+        V3 tA = cross(rA, mul(f, -1.0)), tB = cross(rB, f);
+        r[13] = Fn; r[14] = f.x; r[15] = f.y; r[16] = f.z;
+        r[17] = tA.x; r[18] = tA.y; r[19] = tA.z; r[20] = tB.x; r[21] = tB.y; r[22] = tB.z;
+        r[23] = comp; r[24] = plen; r[25] = raw; r[26] = 0.0; r[27] = 0.0;
+        if (vtx) {
+            if (nv + plen > vmax) return TOPO_VTX_OVERFLOW;
+            for (int m = 0; m < plen; ++m) {
+                int v = path[m];
+                double* o = vtx + 3 * (int64_t)(nv + m);
+                o[0] = Vs[3 * v]; o[1] = Vs[3 * v + 1]; o[2] = Vs[3 * v + 2];
+            }
+            nv += plen;
+        }
+        return 0;
+    }
+    static constexpr int ZDX_LOOP_REC_D = 28;
+};
+
+__global__ void __launch_bounds__(256) k_topo_api(DPairs P, DMeshes M, DBodies B, TopoBufs T, int32_t split) {
+    const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (tid >= T.nthreads) return;
+    const int n = *P.n;
+    TopoCaps caps; caps.S = T.seg_max; caps.E2 = 4 * T.seg_max;
+    SArr<int> I; I.p = T.scr_i + tid; I.s = T.nthreads;
+    SArr<double> D; D.p = T.scr_d + tid; D.s = T.nthreads;
+    for (int p = (int)tid; p < n; p += (int)T.nthreads) {
+        const int i = P.pi[p], j = P.pj[p];
+        const int S = P.nseg[p];
+        const int mA = B.mesh[i];
+        V3 pa = ld3(B.state + 16 * (int64_t)i), pb = ld3(B.state + 16 * (int64_t)j);
+        ApiSink sink{T.rec + 28 * (int64_t)p * T.lmax, T.vtx ? T.vtx + 3 * (int64_t)p * T.vmax : nullptr,
+                     T.lmax, T.vmax, 0, pa, pb, T.k_stiff};
+        TopoOut to;
+        to.comps = 0; to.loops_raw = 0; to.loops = 0; to.status = 0;
+        if (S > T.seg_max) to.status = TOPO_E2_OVERFLOW;
+        else if (S > 0)
+            to = topology(P.arena + 6 * P.seg_off[p], S, M.tol[mA], M.area_eps[mA], split != 0, sub(pb, pa), I, D,
+                          caps, sink);
+        T.ncontrib[p] = to.loops;
+        T.comps[p] = to.comps;
+        T.loops_raw[p] = to.loops_raw;
+        T.tstatus[p] = to.status;
+        if (to.status) atomicOr(P.status, (int)ST_TOPO);
+    }
+}
+
+// ------------------------------------------------------------ walls
+__global__ void __launch_bounds__(256) k_walls(DBodies B, DMeshes M, DMat mat, WallBufs W, HistBufs H, double* wcontrib,
+                                               int32_t* wnc, double* scr, int64_t nthreads, int32_t pts_cap,
+                                               SimParams sp, int32_t* status) {
+    const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (tid >= nthreads) return;
+    const int64_t total = (int64_t)B.n * W.n_walls;
+    SArr<double> PTS; PTS.p = scr + tid; PTS.s = nthreads;
+    SArr<double> PA = PTS.at(3 * (int64_t)pts_cap);
+    SArr<double> PB = PA.at(2 * (int64_t)(pts_cap + 8));
+    for (int64_t idx = tid; idx < total; idx += nthreads) {
+        const int i = (int)(idx / W.n_walls), w = (int)(idx % W.n_walls);
+        Body p = load_body(B.state + 16 * (int64_t)i);
+        const double r = mat.radius[i];
+        // sphere test main.cpp:2318-2323
+        V3 dp = sub(p.pos, ld3(W.pos + 3 * w));
+        double d2 = dot(dp, dp);
+        double rs = r + W.radius[w];
+        int nc = 0;
+        V3 ds = H.wprev_active[idx] ? ld3(H.wprev_ds + 3 * idx) : v3(0.0, 0.0, 0.0);
+        if (!(d2 > rs * rs)) {
+            const int m = B.mesh[i];
+            const double tol = M.tol[m];
+            const double* wv = B.wv + 3 * B.wv_off[i];
+            const int32_t* ti = M.tri_idx + 3 * (int64_t)M.tri_off[m];
+            const int T = M.ntris[m];
+            Mat mt = load_mat(mat, i);
+            V3 pmn = sub(p.pos, v3(r, r, r)), pmx = add(p.pos, v3(r, r, r));
+            const int t0 = W.tri_off[w], nt = W.ntri[w];
+            for (int t = 0; t < nt; ++t) {
+                const double* wt9 = W.tris + 9 * (int64_t)(t0 + t);
+                V3 wt[3] = {ld3(wt9), ld3(wt9 + 3), ld3(wt9 + 6)};
+                double lo[3], hi[3];
+                tri_box(wt, lo, hi);
+                if (pmx.x < lo[0] || pmn.x > hi[0] || pmx.y < lo[1] || pmn.y > hi[1] || pmx.z < lo[2] || pmn.z > hi[2])
+                    continue;
+                double area;
+                V3 xc, cn;
+                int st = 0;
+                if (!wall_contact(wv, ti, T, wt, ld3(W.normals + 3 * (int64_t)(t0 + t)), tol, PTS, PA, PB, pts_cap,
+                                  area, xc, cn, st)) {
+                    if (st) atomicOr(status, (int)ST_WALL);
+                    continue;
+                }
+                Contact c = pw_force(cn, area, xc, p, mt, W.mu[w], sp.dt, ds);
+                if (nc < W.max_wall_tris) {
+                    double* o = wcontrib + 6 * (idx * W.max_wall_tris + nc);
+                    o[0] = c.f.x; o[1] = c.f.y; o[2] = c.f.z; o[3] = c.r1.x; o[4] = c.r1.y; o[5] = c.r1.z;
+                }
+                ++nc;
+            }
+        }
+        wnc[idx] = nc;
+        H.wcur_ds[3 * idx] = ds.x; H.wcur_ds[3 * idx + 1] = ds.y; H.wcur_ds[3 * idx + 2] = ds.z;
+        H.wcur_active[idx] = nc > 0 ? 1 : 0;
+    }
+}
+
+// ------------------------------------------------------------ fold + integrate
+__global__ void __launch_bounds__(256) k_fold_integrate(int32_t n, const double* sin, double* sout, DMat mat,
+                                                        const int32_t* adj_off, const int32_t* adj_j,
+                                                        const int32_t* adj_pair, int32_t adj_cap, TopoBufs T,
+                                                        int32_t n_walls,
+                                                        const double* wcontrib, const int32_t* wnc, int32_t mwt,
+                                                        double* Fo, double* To, int32_t* counts, SimParams sp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // forces[i] and torques[i] start at zero and receive, in the reference's
+    // order (main.cpp:2049-2050 pair loop, then walls 2310-2447): pairs with
+    // partners in ascending index, loops in order, then walls and their tris
+    V3 F = v3(0.0, 0.0, 0.0), Tq = v3(0.0, 0.0, 0.0);
+    int cnt = 0;
+    if (adj_off && adj_off[n] <= adj_cap) {
+        const int a0 = adj_off[i], a1 = adj_off[i + 1];
+        for (int k = a0; k < a1; ++k) {
+            const int j = adj_j[k];
+            const int p = adj_pair[k];
+            if (p < 0) continue;
+            const int nl = T.ncontrib[p];
+            const double* c = T.contrib + 9 * (int64_t)p * T.lmax;
+            for (int l = 0; l < nl; ++l, c += 9) {
+                V3 f = ld3(c);
+                if (i < j) {
+                    F = sub(F, f);
+                    Tq = add(Tq, cross(ld3(c + 3), mul(f, -1.0)));
+                } else {
+                    F = add(F, f);
+                    Tq = add(Tq, cross(ld3(c + 6), f));
+                }
+                ++cnt;
+            }
+        }
+    }
+    for (int w = 0; w < n_walls; ++w) {
+        const int64_t idx = (int64_t)i * n_walls + w;
+        const int nc = wnc[idx] < mwt ? wnc[idx] : mwt;
+        for (int k = 0; k < nc; ++k) {
+            const double* c = wcontrib + 6 * (idx * mwt + k);
+            V3 f = ld3(c);
+            F = add(F, f);
+            Tq = add(Tq, cross(ld3(c + 3), f));
+            ++cnt;
+        }
+    }
+    Fo[3 * (int64_t)i] = F.x; Fo[3 * (int64_t)i + 1] = F.y; Fo[3 * (int64_t)i + 2] = F.z;
+    To[3 * (int64_t)i] = Tq.x; To[3 * (int64_t)i + 1] = Tq.y; To[3 * (int64_t)i + 2] = Tq.z;
+    counts[i] = cnt;
+    Body b = load_body(sin + 16 * (int64_t)i);
+    M3 Ib;
+    for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) Ib.m[r][c] = mat.Ibinv[9 * (int64_t)i + 3 * r + c];
+    integrate(b, Ib, mat.inv_mass[i], F, Tq, v3(sp.g[0], sp.g[1], sp.g[2]), sp.dt);
+    store_body(sout + 16 * (int64_t)i, b);
+}
+
+// ------------------------------------------------------------ counters
+__global__ void __launch_bounds__(256) k_counters(DPairs P, TopoBufs T, int32_t n_pw, const int32_t* wnc,
+                                                  unsigned long long* ctr) {
+    const int n = *P.n;
+    unsigned long long c_loops = 0, c_checks = 0, c_seg = 0, c_comps = 0, c_raw = 0, c_tests = 0, c_hits = 0,
+                       c_wall = 0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        c_loops += T.ncontrib[p];
+        c_checks += P.tri_checks[p];
+        c_seg += P.nseg[p];
+        c_comps += T.comps[p];
+        c_raw += T.loops_raw[p];
+        c_tests += P.tri_tests[p];
+        c_hits += P.tri_hits[p];
+    }
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n_pw; k += gridDim.x * blockDim.x) c_wall += wnc[k];
+    unsigned long long v[8] = {c_loops, c_checks, c_seg, c_comps, c_raw, c_tests, c_hits, c_wall};
+    for (int k = 0; k < 8; ++k)
+        for (int d = 32; d >= 1; d >>= 1) v[k] += __shfl_xor(v[k], d, WAVE);
+    if (lane_id() == 0) {
+        atomicAdd(&ctr[0], v[0] + v[7]);   // contacts = accepted loops + wall contacts
+        atomicAdd(&ctr[1], v[1]);
+        atomicAdd(&ctr[2], v[2]);
+        atomicAdd(&ctr[3], v[3]);
+        atomicAdd(&ctr[4], v[4]);
+        atomicAdd(&ctr[6], v[7]);
+        atomicAdd(&ctr[7], v[5]);
+        atomicAdd(&ctr[8], v[6]);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctr[5], (unsigned long long)n);
+}
+
+__global__ void k_iota_pairs(int32_t* pi, int32_t* pj, int32_t n) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) { pi[p] = 2 * p; pj[p] = 2 * p + 1; }
+}
+
+inline int nblk(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers
+void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s) {
+    if (B.n <= 0) return;
+    hipLaunchKernelGGL(k_transform, dim3(B.n), dim3(256), 0, s, B, M);
+}
+
+void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s) {
+    const int n = bp.n;
+    hipLaunchKernelGGL(k_bp_bounds, dim3(1), dim3(1024), 0, s, bp.state, bp.radius, n, bp.grid, bp.max_cells,
+                       bp.status);
+    // cell counters for the grid actually in use
+    int32_t* ncells = bp.n_adj;  // reuse as a scalar before it is written below
+    hipLaunchKernelGGL(k_grid_cells, dim3(1), dim3(1), 0, s, bp.grid, ncells);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, s, bp.cell_count, 0, ncells, (int64_t)bp.max_cells);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, s, bp.cell_cursor, 0, ncells, (int64_t)bp.max_cells);
+    hipLaunchKernelGGL(k_bp_cells, dim3(nblk(n, 256)), dim3(256), 0, s, bp);
+    scan_i32(bp.cell_count, ncells, bp.max_cells, bp.scan_tmp, bp.cell_start, nullptr, s);
+    hipLaunchKernelGGL(k_bp_scatter, dim3(nblk(n, 256)), dim3(256), 0, s, bp);
+    hipLaunchKernelGGL(k_bp_count, dim3(nblk(n, 256)), dim3(256), 0, s, bp);
+    scan_i32(bp.cnt_up, nullptr, n, bp.scan_tmp, bp.pair_off, nullptr, s);
+    scan_i32(bp.cnt_all, nullptr, n, bp.scan_tmp, bp.adj_off, nullptr, s);
+    hipLaunchKernelGGL(k_bp_fill, dim3(nblk(n, 128)), dim3(128), 0, s, bp);
+    hipLaunchKernelGGL(k_bp_adjfix, dim3(nblk(n, 256)), dim3(256), 0, s, bp);
+    hipLaunchKernelGGL(k_bp_totals, dim3(1), dim3(1), 0, s, bp);
+}
+
+template <int BS, int K>
+static void narrow_t(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
+                     double* st, int32_t sc, int32_t tmax, unsigned int* q, hipStream_t s) {
+    hipLaunchKernelGGL((k_narrow<BS, K>), dim3(cfg.groups), dim3(BS), 0, s, B, M, P, li, lf, st, sc, tmax,
+                       cfg.use_queue, q);
+}
+
+void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
+                   double* st, int32_t sc, int32_t tmax, unsigned int* q, hipStream_t s) {
+    const int key = cfg.block * 100 + cfg.items;
+    switch (key) {
+        case 12804: narrow_t<128, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s); break;
+        case 12808: narrow_t<128, 8>(B, M, P, cfg, li, lf, st, sc, tmax, q, s); break;
+        case 25601: narrow_t<256, 1>(B, M, P, cfg, li, lf, st, sc, tmax, q, s); break;
+        case 25602: narrow_t<256, 2>(B, M, P, cfg, li, lf, st, sc, tmax, q, s); break;
+        case 25604: narrow_t<256, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s); break;
+        case 25608: narrow_t<256, 8>(B, M, P, cfg, li, lf, st, sc, tmax, q, s); break;
+        case 51202: narrow_t<512, 2>(B, M, P, cfg, li, lf, st, sc, tmax, q, s); break;
+        case 51204: narrow_t<512, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s); break;
+        default: narrow_t<256, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s); break;
+    }
+}
+
+void launch_lpt_order(const DPairs& P, const DMeshes& M, const int32_t* body_mesh, int32_t* order, int32_t* bcnt,
+                      int32_t* bcur, int32_t pair_cap, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, bcnt, 0, nullptr, (int64_t)64);
+    hipLaunchKernelGGL(k_lpt_count, dim3(nblk(pair_cap, 256)), dim3(256), 0, s, P, M, body_mesh, bcnt);
+    hipLaunchKernelGGL(k_lpt_scan, dim3(1), dim3(64), 0, s, bcnt, bcur);
+    hipLaunchKernelGGL(k_lpt_scatter, dim3(nblk(pair_cap, 256)), dim3(256), 0, s, P, M, body_mesh, bcur, order);
+}
+
+void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
+                     const TopoBufs& T, const SimParams& sp, hipStream_t s) {
+    hipLaunchKernelGGL(k_topo_sim, dim3(nblk(T.nthreads, 256)), dim3(256), 0, s, P, M, B, mat, H, T, sp);
+}
+void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int32_t split,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_topo_api, dim3(nblk(T.nthreads, 256)), dim3(256), 0, s, P, M, B, T, split);
+}
+void launch_walls(const DBodies& B, const DMeshes& M, const DMat& mat, const WallBufs& W, const HistBufs& H,
+                  double* wcontrib, int32_t* wnc, double* scr, int64_t nthreads, int32_t pts_cap, const SimParams& sp,
+                  int32_t* status, hipStream_t s) {
+    if (W.n_walls <= 0) return;
+    hipLaunchKernelGGL(k_walls, dim3(nblk(nthreads, 256)), dim3(256), 0, s, B, M, mat, W, H, wcontrib, wnc, scr,
+                       nthreads, pts_cap, sp, status);
+}
+void launch_fold_integrate(int32_t n, const double* sin, double* sout, const DMat& mat, const int32_t* adj_off,
+                           const int32_t* adj_j, const int32_t* adj_pair, int32_t adj_cap, const TopoBufs& T,
+                           int32_t n_walls,
+                           const double* wcontrib, const int32_t* wnc, int32_t mwt, double* F, double* Tq,
+                           int32_t* counts, const SimParams& sp, hipStream_t s) {
+    hipLaunchKernelGGL(k_fold_integrate, dim3(nblk(n, 256)), dim3(256), 0, s, n, sin, sout, mat, adj_off, adj_j,
+                       adj_pair, adj_cap, T, n_walls, wcontrib, wnc, mwt, F, Tq, counts, sp);
+}
+void launch_counters(const DPairs& P, const TopoBufs& T, int32_t n_pw, const int32_t* wnc, unsigned long long* ctr,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctr, 0ull, (int64_t)16);
+    hipLaunchKernelGGL(k_counters, dim3(256), dim3(256), 0, s, P, T, n_pw, wnc, ctr);
+}
+void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    int g = nblk(n, 256);
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_fill_i32, dim3(g), dim3(256), 0, s, p, v, nullptr, n);
+}
+void launch_fill_u64(unsigned long long* p, unsigned long long v, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    int g = nblk(n, 256);
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_fill_u64, dim3(g), dim3(256), 0, s, p, v, n);
+}
+void launch_iota_pairs(int32_t* pi, int32_t* pj, int32_t n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_iota_pairs, dim3(nblk(n, 256)), dim3(256), 0, s, pi, pj, n);
+}
+
+}  // namespace zdx
