@@ -1,0 +1,273 @@
+#!/usr/bin/env python3
+"""Throughput of the Route-B contact step on one MI355X.
+
+Workload: the granular bed of 100k polyhedra (8 templates: convex hulls of
+32-512 sphere points and radial-noise stars of 320-1280 triangles, radii
+log-uniform 0.5-2 mm, density 2500, young 1e7) in a 0.2 m box bounded by a
+12-triangle box wall, gravity -9.81, dt 1e-5 -- see zdx.workloads.granular_bed.
+
+A step is one full time step (transform, cell-binned broad phase, narrow phase,
+topology, forces with tangential history, walls, ordered force fold,
+integrator).  W warmup steps run first (untimed: the agitated bed starts
+colliding), then K timed steps in batched mode (persistent work queue,
+hipGraph-captured step, autotuned narrow-phase tiles), then the same K steps
+again in plain mode (one kernel per stage, host-sized launches) from a
+checkpoint of the post-warmup state; the two trajectories must be bitwise
+equal.  value = contact pairs/s of the batched step, where a contact pair is a
+broad-phase candidate pair evaluated by the narrow phase.
+
+Prints one JSON line.  --gpus N with N > 1 exits: this build measures one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "zedm-x_amd"))
+
+# MI355X published peaks used for the roofline
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+# algorithmic work of one narrow-phase launch (per step), from the counters:
+# FP64 ops: 24 per triangle scanned by the cull (3 corner loads, 12 min/max,
+#   6 compares, 6 outward roundings), 6 per executed AABB test, 200 per hit
+#   (exact AABB re-test 30, two planes 70, tri-tri 100), 100 per segment;
+# HBM bytes: 36 per triangle scanned (12 B of corner indices, 24 B of world
+#   vertex data: each vertex is shared by ~2 triangles) + 48 per segment written.
+OPS_PER_SCANNED, OPS_PER_TEST, OPS_PER_HIT, OPS_PER_SEG = 24, 6, 200, 100
+BYTES_PER_SCANNED, BYTES_PER_SEG = 36, 48
+
+
+def narrow_model(ctr: dict):
+    ops = (OPS_PER_SCANNED * ctr["tri_scanned"] + OPS_PER_TEST * ctr["tri_tests"] + OPS_PER_HIT * ctr["tri_hits"] +
+           OPS_PER_SEG * ctr["segments"])
+    byts = BYTES_PER_SCANNED * ctr["tri_scanned"] + BYTES_PER_SEG * ctr["segments"]
+    return ops, byts
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(wl, state, sample_target, timeout_s):
+    """Times the reference program (zdem_cpu, compiled from the reference's own
+    source into oracle/_ref/) for one step on a sub-box of the post-warmup bed,
+    single-threaded.  Returns None when that binary is not present."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from zdx import workloads as W
+
+    if not os.path.exists(O.REF_BIN):
+        return None
+    pos = state[:, :3]
+    c = np.median(pos, axis=0)
+    d = np.max(np.abs(pos - c), axis=1)
+    idx = np.argsort(d, kind="stable")[:sample_target]
+    idx = np.sort(idx)
+    tmp = tempfile.mkdtemp(prefix="zdx_cpu_")
+    try:
+        wl_sample = wl
+        cfg = W.write_reference_config(wl_sample, idx, tmp, steps=1, output_interval=1, state=state,
+                                       prefix="sample")
+        # broad-phase candidate pairs of the sample (the same sphere test)
+        rad = np.array([wl.scale[i] * _tpl_radius(wl, wl.tpl[i]) for i in idx])
+        P = pos[idx]
+        n_pairs = 0
+        from scipy.spatial import cKDTree
+
+        tree = cKDTree(P)
+        for a, b in tree.query_pairs(2.0 * float(rad.max()) + 1e-9):
+            dp = P[b] - P[a]
+            if not (dp @ dp > (rad[a] + rad[b]) ** 2):
+                n_pairs += 1
+        env = dict(os.environ)
+        env["OMP_NUM_THREADS"] = "1"
+        cmd = [O.REF_BIN, "--config", cfg]
+        if shutil.which("taskset"):
+            cmd = ["taskset", "-c", "0"] + cmd
+        t0 = time.time()
+        out = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s, env=env, cwd=tmp)
+        wall = time.time() - t0
+        m = re.search(r"step=0 contacts=(\d+) step_ms=(\d+) tri_ms=(\d+) rebuild_ms=(\d+) output_ms=(\d+) "
+                      r"tri_checks=(\d+)", out.stdout)
+        if not m:
+            return dict(error="reference run produced no step line", stderr=out.stderr[-400:])
+        contact_ms = int(m.group(3)) + int(m.group(4))
+        return dict(value=n_pairs / max(contact_ms, 1) * 1000.0, unit="contact pairs/s", cores=1, kind="reference",
+                    sample=f"{len(idx)} particles around the bed centre after warmup, {n_pairs} candidate pairs, "
+                           f"1 step of zdem_cpu (reference src/main.cpp, g++ -O2), narrow phase + topology + "
+                           f"forces {contact_ms} ms (tri_ms + rebuild_ms), tri_checks {m.group(6)}, "
+                           f"wall {wall:.1f} s incl. VTK output",
+                    cpu=cpu_model(), ms_per_pair=contact_ms / max(n_pairs, 1))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+_TPL_RAD = {}
+
+
+def _tpl_radius(wl, t):
+    from zdx import workloads as W
+
+    if t not in _TPL_RAD:
+        _TPL_RAD[t] = W._centred_radius(wl.templates[t])
+    return _TPL_RAD[t]
+
+
+def run_steps(sim, k, mode):
+    import zdx
+
+    tot = {c: 0 for c in zdx.COUNTERS}
+    t0 = time.perf_counter()
+    for _ in range(k):
+        c = sim.step(1, mode)
+        for key in tot:
+            tot[key] += c[key]
+    return time.perf_counter() - t0, tot
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--particles", type=int, default=100_000)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=4000)
+    ap.add_argument("--cpu-timeout", type=float, default=120.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU: the step runs on a single MI355X and no "
+                                   "collective or multi-GPU path exists; run with --gpus 1"}))
+        sys.exit(2)
+
+    import zdx
+    from zdx import workloads as W
+
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
+    t0 = time.time()
+    wl = W.granular_bed(n=args.particles)
+    ctx = zdx.Context(0)
+    sim, n_meshes = W.build_sim(ctx, wl)
+    log(f"setup {time.time() - t0:.1f}s: {wl.n} particles, {n_meshes} meshes")
+
+    # warmup (untimed): half, autotune the narrow-phase tiles, the rest
+    half = args.warmup // 2
+    sim.step(half, zdx.MODE_BATCHED)
+    tiles = None
+    if not args.no_autotune:
+        tiles = sim.autotune()
+        log(f"autotune: block*100+items={tiles[0]} groups/CU={tiles[1]}")
+    sim.step(args.warmup - half, zdx.MODE_BATCHED)
+    log(f"warmup done {time.time() - t0:.1f}s")
+    tmpdir = tempfile.mkdtemp(prefix="zdx_bench_")
+    ck = os.path.join(tmpdir, "warm.ckpt")
+    sim.checkpoint(ck)
+    state_warm = sim.state()
+
+    # timed: batched
+    sim.stage_times()  # reset the per-stage accumulators
+    tb, cb = run_steps(sim, args.steps, zdx.MODE_BATCHED)
+    st_b = sim.stage_times()
+    state_b = sim.state()
+    log(f"batched {args.steps} steps {tb:.3f}s pairs/step {cb['pairs'] / args.steps:.0f} stages {st_b}")
+
+    plain = None
+    if not args.no_plain:
+        sim.restore(ck)
+        sim.stage_times()
+        tp, cp = run_steps(sim, args.steps, zdx.MODE_PLAIN)
+        st_p = sim.stage_times()
+        state_p = sim.state()
+        plain = dict(seconds=tp, counters=cp, stages=st_p, bitwise_equal=bool(np.array_equal(state_p, state_b)))
+        log(f"plain {tp:.3f}s equal={plain['bitwise_equal']}")
+
+    # roofline of the dominant kernel of the batched step, from the live events
+    stage_ms = {k: st_b[k] for k in zdx.Sim.STAGES}
+    dom = max(stage_ms, key=stage_ms.get)
+    per_step = {k: v / args.steps for k, v in st_b["counters"].items()}
+    ops, byts = narrow_model(per_step)
+    t_np = stage_ms["narrow"] * 1e-3
+    ach_tflops = ops / t_np / 1e12 if t_np > 0 else 0.0
+    ach_gbs = byts / t_np / 1e9 if t_np > 0 else 0.0
+    f_flops, f_hbm = ach_tflops / FP64_PEAK_TFLOPS, ach_gbs / HBM_PEAK_GBS
+    if f_hbm >= f_flops:
+        roof = dict(bound="hbm", achieved=ach_gbs, peak=HBM_PEAK_GBS, unit="GB/s", frac=f_hbm, traffic=None)
+    else:
+        roof = dict(bound="mfma", achieved=ach_tflops, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=f_flops,
+                    traffic=None)
+    roof.update(kernel="k_narrow (narrow phase)", compute="FP64 VALU (no matrix-core work in this path)",
+                avg_launch_ms=stage_ms["narrow"], fp64_ops_per_launch=ops, bytes_per_launch=byts,
+                frac_fp64=f_flops, frac_hbm=f_hbm, dominant_stage=dom, stage_ms=stage_ms)
+    traffic_file = os.path.join(ROOT, "profiles", "narrow_traffic.json")
+    if os.path.exists(traffic_file):
+        try:
+            roof["traffic"] = json.load(open(traffic_file)).get("bytes_per_launch")
+        except (OSError, ValueError):
+            pass
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(wl, state_warm, args.cpu_sample, args.cpu_timeout)
+    if cpu is None:
+        cpu = dict(value=None, unit="contact pairs/s", cores=1, kind="reference", sample="not run")
+
+    pairs_b = cb["pairs"]
+    res = {
+        "metric": "contact pairs/sec + steps/sec @100k polyhedra, batched vs plain GPU",
+        "value": pairs_b / tb,
+        "unit": "contact pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": tb / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "granular bed, 100k polyhedra (8 templates, 60-1280 triangles), 0.2 m box, "
+                               "12-triangle box wall, gravity -9.81, dt 1e-5, agitated non-overlapping seeding",
+                   "particles": wl.n, "meshes": n_meshes, "mode": "batched (persistent queue + hipGraph)",
+                   "tiles": tiles},
+        "steps_per_s": args.steps / tb,
+        "contact_pairs_per_step": pairs_b / args.steps,
+        "active_contacts_per_step": cb["contacts"] / args.steps,
+        "counters_per_step": per_step,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if plain:
+        res["plain"] = {"contact_pairs_per_s": plain["counters"]["pairs"] / plain["seconds"],
+                        "steps_per_s": args.steps / plain["seconds"],
+                        "ms_per_step": plain["seconds"] / args.steps * 1e3,
+                        "stage_ms": {k: plain["stages"][k] for k in zdx.Sim.STAGES},
+                        "bitwise_equal_to_batched": plain["bitwise_equal"]}
+        res["batched_over_plain"] = (pairs_b / tb) / (plain["counters"]["pairs"] / plain["seconds"])
+    shutil.rmtree(tmpdir, ignore_errors=True)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

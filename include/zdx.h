@@ -144,7 +144,8 @@ enum {
     ZDX_CTR_WALL_CONTACTS = 6,
     ZDX_CTR_TRI_TESTS = 7,    /* triangle AABB tests actually executed after culling */
     ZDX_CTR_TRI_HITS = 8,     /* triangle pairs that reached the exact intersection test */
-    ZDX_N_COUNTERS = 9
+    ZDX_CTR_TRI_SCANNED = 9,  /* triangles of both meshes scanned by the per-pair cull (sum of TA + TB) */
+    ZDX_N_COUNTERS = 10
 };
 /* Runs n_steps steps in mode ZDX_MODE_PLAIN (one kernel per stage,
  * host-sized launches) or ZDX_MODE_BATCHED (persistent work-queue kernels,
@@ -163,9 +164,13 @@ zdx_status zdx_sim_get_forces(zdx_sim* sim, double* F, double* T, int32_t* conta
  * keeps 9 decimals and no history). */
 zdx_status zdx_sim_checkpoint(zdx_sim* sim, const char* path);
 zdx_status zdx_sim_restore(zdx_sim* sim, const char* path);
-/* Per-stage device times of the last step in ms (n entries):
- * transform, broad phase, narrow phase, topology+forces, walls, fold+integrate. */
-zdx_status zdx_sim_stage_times(zdx_sim* sim, double* ms, int32_t n);
+/* Stage timing and counters accumulated over the steps since the previous
+ * call (HIP events recorded on the context stream around each stage of every
+ * step, inside the captured graph in batched mode), then reset.  out[0..5]:
+ * mean ms per step of transform, broad phase, narrow phase, topology+forces,
+ * walls, fold+integrate+counters; out[6]: steps; out[7..7+ZDX_N_COUNTERS):
+ * counter sums.  Writes min(n, 7 + ZDX_N_COUNTERS) values. */
+zdx_status zdx_sim_stage_times(zdx_sim* sim, double* out, int32_t n);
 /* Narrow-phase launch shape; 0 keeps the current value.  The autotuner
  * (zdx_sim_autotune) sets them from timed trial steps. */
 zdx_status zdx_sim_set_tiles(zdx_sim* sim, int32_t np_block, int32_t np_groups_per_cu);

@@ -98,7 +98,7 @@ struct zdx_ctx {
         DBuf<int32_t> nondeg;
         DBuf<int32_t> pi, pj, order, npairs, status;
         DBuf<int64_t> seg_off, tri_checks, tri_tests;
-        DBuf<int32_t> nseg, tri_hits;
+        DBuf<int32_t> nseg, tri_hits, tri_scanned;
         DBuf<double> arena;
         DBuf<unsigned long long> arena_top;
         DBuf<int32_t> list_i;
@@ -223,7 +223,7 @@ struct zdx_sim {
     } ps[2];
     DBuf<int32_t> d_order, d_bcnt, d_bcur;
     DBuf<int64_t> d_seg_off, d_tri_checks, d_tri_tests;
-    DBuf<int32_t> d_nseg, d_tri_hits;
+    DBuf<int32_t> d_nseg, d_tri_hits, d_tri_scanned;
     DBuf<double> d_arena;
     DBuf<unsigned long long> d_arena_top;
     int64_t arena_cap = 0;
@@ -258,12 +258,21 @@ struct zdx_sim {
     double stage_ms[6] = {0, 0, 0, 0, 0, 0};
     NarrowCfg ncfg{256, 4, 0, 0};
     int32_t np_groups_per_cu = 8;
+    // per-stage events recorded inside every step (captured into the graphs)
+    hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    double acc_ms[6] = {0, 0, 0, 0, 0, 0};
+    double acc_ctr[ZDX_N_COUNTERS] = {0};
+    int64_t acc_steps = 0;
     // graphs (one per parity)
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
     bool graphs_valid = false;
 
-    ~zdx_sim() { drop_graphs(); }
+    ~zdx_sim() {
+        drop_graphs();
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
     void drop_graphs() {
         for (int k = 0; k < 2; ++k) {
             if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
@@ -304,7 +313,7 @@ struct zdx_sim {
         P.cap = pair_cap; P.n = ps[par].n.p; P.pi = ps[par].pi.p; P.pj = ps[par].pj.p;
         P.order = lpt ? d_order.p : nullptr;
         P.seg_off = d_seg_off.p; P.nseg = d_nseg.p; P.tri_checks = d_tri_checks.p; P.tri_tests = d_tri_tests.p;
-        P.tri_hits = d_tri_hits.p; P.arena = d_arena.p; P.arena_cap = arena_cap; P.arena_top = d_arena_top.p;
+        P.tri_hits = d_tri_hits.p; P.tri_scanned = d_tri_scanned.p; P.arena = d_arena.p; P.arena_cap = arena_cap; P.arena_top = d_arena_top.p;
         P.status = d_status.p;
         return P;
     }
@@ -365,7 +374,7 @@ struct zdx_sim {
         d_adj_j.ensure(adj_cap); d_adj_pair.ensure(adj_cap);
         d_order.ensure(cap);
         d_seg_off.ensure(cap); d_tri_checks.ensure(cap); d_tri_tests.ensure(cap);
-        d_nseg.ensure(cap); d_tri_hits.ensure(cap);
+        d_nseg.ensure(cap); d_tri_hits.ensure(cap); d_tri_scanned.ensure(cap);
         d_ncontrib.ensure(cap); d_comps.ensure(cap); d_loops_raw.ensure(cap); d_tstatus.ensure(cap);
         d_contrib.ensure((size_t)cap * lmax * 9);
     }
@@ -440,6 +449,8 @@ void sim_setup_device(zdx_sim* s) {
     s->d_status.ensure(1); s->d_ctr.ensure(16); s->d_queue.ensure(1);
     s->d_bcnt.ensure(64); s->d_bcur.ensure(64);
     s->d_arena_top.ensure(1);
+    for (auto& e : s->ev)
+        if (!e) HIPC(hipEventCreate(&e));
     s->alloc_pairs(std::max(1024, 8 * N));
     s->alloc_arena(std::max<int64_t>(1 << 16, 64 * (int64_t)N));
     size_t groups = (size_t)persistent_groups(c, 16);
@@ -528,27 +539,34 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     DBodies B = s->bodies(par);
     DMeshes M = s->ctx->dmeshes();
     DMat mat = s->mat();
+    HIPC(hipEventRecord(s->ev[0], st));
     launch_transform(B, M, st);
+    HIPC(hipEventRecord(s->ev[1], st));
     launch_broadphase(s->bpbufs(par), st);
     DPairs P = s->dpairs(par, batched);
     if (batched) launch_lpt_order(P, M, s->d_mesh.p, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st);
     launch_fill_u64(s->d_arena_top.p, 0ull, 1, st);
     launch_fill_i32((int32_t*)s->d_queue.p, 0, 1, st);
+    HIPC(hipEventRecord(s->ev[2], st));
     NarrowCfg nc = s->ncfg;
     nc.use_queue = batched ? 1 : 0;
     nc.groups = persistent_groups(s->ctx, s->np_groups_per_cu);
     launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->d_stage.p, s->stage_cap, s->tmax,
                   s->d_queue.p, st);
+    HIPC(hipEventRecord(s->ev[3], st));
     TopoBufs T = s->topobufs();
     HistBufs H = s->hist(par);
     launch_topo_sim(P, M, B, mat, H, T, s->sp, st);
+    HIPC(hipEventRecord(s->ev[4], st));
     if (s->W > 0)
         launch_walls(B, M, mat, s->wallbufs(), H, s->d_wcontrib.p, s->d_wnc.p, s->d_wscr.p, s->wall_threads,
                      s->wall_pts_cap, s->sp, s->d_status.p, st);
+    HIPC(hipEventRecord(s->ev[5], st));
     launch_fold_integrate(s->N, s->d_state[par].p, s->d_state[nxt].p, mat, s->d_adj_off.p, s->d_adj_j.p,
                           s->d_adj_pair.p, s->adj_cap, T, s->W, s->d_wcontrib.p, s->d_wnc.p, s->max_wall_tris, s->d_F.p, s->d_T.p,
                           s->d_counts.p, s->sp, st);
     launch_counters(P, T, s->N * s->W, s->d_wnc.p, s->d_ctr.p, st);
+    HIPC(hipEventRecord(s->ev[6], st));
 }
 
 int32_t read_status(zdx_sim* s) {
@@ -645,6 +663,13 @@ void step_once(zdx_sim* s, bool batched) {
         int32_t status = read_status(s);
         if (status == 0) {
             read_counters(s);
+            for (int k = 0; k < 6; ++k) {
+                float ms = 0.f;
+                HIPC(hipEventElapsedTime(&ms, s->ev[k], s->ev[k + 1]));
+                s->acc_ms[k] += ms;
+            }
+            for (int k = 0; k < ZDX_N_COUNTERS; ++k) s->acc_ctr[k] += (double)s->last_ctr[k];
+            s->acc_steps += 1;
             s->parity = 1 - par;
             s->n_steps_done += 1;
             return;
@@ -655,51 +680,6 @@ void step_once(zdx_sim* s, bool batched) {
             throw ApiError{ZDX_CAPACITY, "simulation step capacity exceeded (status " + std::to_string(status) + ")"};
     }
     throw ApiError{ZDX_CAPACITY, "simulation step did not fit after repeated growth"};
-}
-
-// plain-mode stage timing with events (one extra synchronising step)
-void time_stages(zdx_sim* s) {
-    hipStream_t st = s->st();
-    hipEvent_t ev[7];
-    for (auto& e : ev) HIPC(hipEventCreate(&e));
-    const int par = s->parity;
-    DBodies B = s->bodies(par);
-    DMeshes M = s->ctx->dmeshes();
-    DMat mat = s->mat();
-    launch_fill_i32(s->d_status.p, 0, 1, st);
-    HIPC(hipEventRecord(ev[0], st));
-    launch_transform(B, M, st);
-    HIPC(hipEventRecord(ev[1], st));
-    launch_broadphase(s->bpbufs(par), st);
-    HIPC(hipEventRecord(ev[2], st));
-    DPairs P = s->dpairs(par, false);
-    launch_fill_u64(s->d_arena_top.p, 0ull, 1, st);
-    NarrowCfg nc = s->ncfg;
-    nc.use_queue = 0;
-    nc.groups = persistent_groups(s->ctx, s->np_groups_per_cu);
-    launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->d_stage.p, s->stage_cap, s->tmax,
-                  s->d_queue.p, st);
-    HIPC(hipEventRecord(ev[3], st));
-    TopoBufs T = s->topobufs();
-    HistBufs H = s->hist(par);
-    launch_topo_sim(P, M, B, mat, H, T, s->sp, st);
-    HIPC(hipEventRecord(ev[4], st));
-    if (s->W > 0)
-        launch_walls(B, M, mat, s->wallbufs(), H, s->d_wcontrib.p, s->d_wnc.p, s->d_wscr.p, s->wall_threads,
-                     s->wall_pts_cap, s->sp, s->d_status.p, st);
-    HIPC(hipEventRecord(ev[5], st));
-    // fold into a scratch state so the timed pass does not advance the run
-    launch_fold_integrate(s->N, s->d_state[par].p, s->d_state[1 - par].p, mat, s->d_adj_off.p, s->d_adj_j.p,
-                          s->d_adj_pair.p, s->adj_cap, T, s->W, s->d_wcontrib.p, s->d_wnc.p, s->max_wall_tris, s->d_F.p,
-                          s->d_T.p, s->d_counts.p, s->sp, st);
-    HIPC(hipEventRecord(ev[6], st));
-    HIPC(hipEventSynchronize(ev[6]));
-    for (int k = 0; k < 6; ++k) {
-        float ms = 0;
-        HIPC(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
-        s->stage_ms[k] = ms;
-    }
-    for (auto& e : ev) (void)hipEventDestroy(e);
 }
 
 }  // namespace
@@ -820,7 +800,7 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
         A.pi.ensure(n); A.pj.ensure(n); A.npairs.ensure(1); A.status.ensure(1); A.order.ensure(n);
         launch_iota_pairs(A.pi.p, A.pj.p, n, st);
         HIPC(hipMemcpyAsync(A.npairs.p, &n, sizeof(int32_t), hipMemcpyHostToDevice, st));
-        A.seg_off.ensure(n); A.tri_checks.ensure(n); A.tri_tests.ensure(n); A.nseg.ensure(n); A.tri_hits.ensure(n);
+        A.seg_off.ensure(n); A.tri_checks.ensure(n); A.tri_tests.ensure(n); A.nseg.ensure(n); A.tri_hits.ensure(n); A.tri_scanned.ensure(n);
         A.arena_top.ensure(1); A.queue.ensure(1); A.bcnt.ensure(64); A.bcur.ensure(64);
         A.ncontrib.ensure(n); A.comps.ensure(n); A.loops_raw.ensure(n); A.tstatus.ensure(n);
         const bool batched = opts->mode == ZDX_MODE_BATCHED;
@@ -834,7 +814,7 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
             DPairs P;
             P.cap = n; P.n = A.npairs.p; P.pi = A.pi.p; P.pj = A.pj.p; P.order = batched ? A.order.p : nullptr;
             P.seg_off = A.seg_off.p; P.nseg = A.nseg.p; P.tri_checks = A.tri_checks.p; P.tri_tests = A.tri_tests.p;
-            P.tri_hits = A.tri_hits.p; P.arena = A.arena.p; P.arena_cap = A.arena_cap; P.arena_top = A.arena_top.p;
+            P.tri_hits = A.tri_hits.p; P.tri_scanned = A.tri_scanned.p; P.arena = A.arena.p; P.arena_cap = A.arena_cap; P.arena_top = A.arena_top.p;
             P.status = A.status.p;
             launch_fill_i32(A.status.p, 0, 1, st);
             launch_fill_u64(A.arena_top.p, 0ull, 1, st);
@@ -1215,12 +1195,18 @@ zdx_status zdx_sim_restore(zdx_sim* s, const char* path) {
     });
 }
 
-zdx_status zdx_sim_stage_times(zdx_sim* s, double* ms, int32_t n) {
-    if (!s || !ms) return ZDX_BAD_ARG;
-    return guard(s->ctx, [&] {
-        time_stages(s);
-        for (int k = 0; k < n && k < 6; ++k) ms[k] = s->stage_ms[k];
-    });
+zdx_status zdx_sim_stage_times(zdx_sim* s, double* out, int32_t n) {
+    if (!s || !out || n < 0) return ZDX_BAD_ARG;
+    double v[7 + ZDX_N_COUNTERS];
+    const double steps = (double)s->acc_steps;
+    for (int k = 0; k < 6; ++k) v[k] = steps > 0 ? s->acc_ms[k] / steps : 0.0;
+    v[6] = steps;
+    for (int k = 0; k < ZDX_N_COUNTERS; ++k) v[7 + k] = s->acc_ctr[k];
+    for (int k = 0; k < n && k < 7 + ZDX_N_COUNTERS; ++k) out[k] = v[k];
+    for (int k = 0; k < 6; ++k) s->acc_ms[k] = 0.0;
+    for (int k = 0; k < ZDX_N_COUNTERS; ++k) s->acc_ctr[k] = 0.0;
+    s->acc_steps = 0;
+    return ZDX_OK;
 }
 
 zdx_status zdx_sim_set_tiles(zdx_sim* s, int32_t block, int32_t groups_per_cu) {

@@ -112,6 +112,7 @@ struct DPairs {
     int64_t* tri_checks;     // per pair (reference counter)
     int64_t* tri_tests;      // per pair (executed AABB tests)
     int32_t* tri_hits;       // per pair
+    int32_t* tri_scanned;    // per pair: TA + TB
     double* arena;           // segments, 6 doubles each
     int64_t arena_cap;       // in segments
     unsigned long long* arena_top;

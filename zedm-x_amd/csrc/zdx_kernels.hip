@@ -608,6 +608,7 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
             P.tri_checks[p] = (int64_t)B.nondeg[i] * TB;
             P.tri_tests[p] = total;
             P.tri_hits[p] = hits;
+            P.tri_scanned[p] = TA + TB;
         }
         __syncthreads();
         if (!use_queue) item += gridDim.x;
@@ -896,7 +897,7 @@ __global__ void __launch_bounds__(256) k_counters(DPairs P, TopoBufs T, int32_t 
                                                   unsigned long long* ctr) {
     const int n = *P.n;
     unsigned long long c_loops = 0, c_checks = 0, c_seg = 0, c_comps = 0, c_raw = 0, c_tests = 0, c_hits = 0,
-                       c_wall = 0;
+                       c_wall = 0, c_scan = 0;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
         c_loops += T.ncontrib[p];
         c_checks += P.tri_checks[p];
@@ -905,10 +906,11 @@ __global__ void __launch_bounds__(256) k_counters(DPairs P, TopoBufs T, int32_t 
         c_raw += T.loops_raw[p];
         c_tests += P.tri_tests[p];
         c_hits += P.tri_hits[p];
+        c_scan += P.tri_scanned[p];
     }
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n_pw; k += gridDim.x * blockDim.x) c_wall += wnc[k];
-    unsigned long long v[8] = {c_loops, c_checks, c_seg, c_comps, c_raw, c_tests, c_hits, c_wall};
-    for (int k = 0; k < 8; ++k)
+    unsigned long long v[9] = {c_loops, c_checks, c_seg, c_comps, c_raw, c_tests, c_hits, c_wall, c_scan};
+    for (int k = 0; k < 9; ++k)
         for (int d = 32; d >= 1; d >>= 1) v[k] += __shfl_xor(v[k], d, WAVE);
     if (lane_id() == 0) {
         atomicAdd(&ctr[0], v[0] + v[7]);   // contacts = accepted loops + wall contacts
@@ -919,6 +921,7 @@ __global__ void __launch_bounds__(256) k_counters(DPairs P, TopoBufs T, int32_t 
         atomicAdd(&ctr[6], v[7]);
         atomicAdd(&ctr[7], v[5]);
         atomicAdd(&ctr[8], v[6]);
+        atomicAdd(&ctr[9], v[8]);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctr[5], (unsigned long long)n);
 }

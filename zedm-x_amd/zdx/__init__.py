@@ -19,7 +19,8 @@ CLI_PATH = os.path.join(PKG, "bin", "zdx")
 
 OK, BAD_ARG, IO, CAPACITY, HIP_ERROR, INTERNAL = range(6)
 MODE_PLAIN, MODE_BATCHED = 0, 1
-COUNTERS = ["contacts", "tri_checks", "segments", "comps", "loops", "pairs", "wall_contacts", "tri_tests", "tri_hits"]
+COUNTERS = ["contacts", "tri_checks", "segments", "comps", "loops", "pairs", "wall_contacts", "tri_tests", "tri_hits",
+            "tri_scanned"]
 EXPORTS = ["zdx_create", "zdx_destroy", "zdx_last_error", "zdx_load_stl", "zdx_mesh_register", "zdx_contact_pairs",
            "zdx_sim_load_config", "zdx_sim_create", "zdx_sim_destroy", "zdx_sim_num_particles", "zdx_sim_step",
            "zdx_sim_get_state", "zdx_sim_set_state", "zdx_sim_get_forces", "zdx_sim_checkpoint", "zdx_sim_restore",
@@ -263,10 +264,17 @@ class Sim:
     def restore(self, path: str) -> None:
         self.ctx._check(self.ctx._L.zdx_sim_restore(self.h, path.encode()))
 
-    def stage_times(self) -> np.ndarray:
-        ms = np.zeros(6)
-        self.ctx._check(self.ctx._L.zdx_sim_stage_times(self.h, ms.ctypes.data, 6))
-        return ms
+    STAGES = ["transform", "broadphase", "narrow", "topology", "walls", "fold_integrate"]
+
+    def stage_times(self) -> dict:
+        """Mean per-stage device ms and counter sums over the steps since the
+        previous call (then reset)."""
+        v = np.zeros(7 + len(COUNTERS))
+        self.ctx._check(self.ctx._L.zdx_sim_stage_times(self.h, v.ctypes.data, len(v)))
+        out = {k: float(v[i]) for i, k in enumerate(self.STAGES)}
+        out["steps"] = int(v[6])
+        out["counters"] = {k: int(v[7 + i]) for i, k in enumerate(COUNTERS)}
+        return out
 
     def set_tiles(self, block: int = 0, groups_per_cu: int = 0) -> None:
         self.ctx._check(self.ctx._L.zdx_sim_set_tiles(self.h, int(block), int(groups_per_cu)))
