@@ -165,8 +165,9 @@ zdx_status zdx_sim_get_forces(zdx_sim* sim, double* F, double* T, int32_t* conta
 zdx_status zdx_sim_checkpoint(zdx_sim* sim, const char* path);
 zdx_status zdx_sim_restore(zdx_sim* sim, const char* path);
 /* Stage timing and counters accumulated over the steps since the previous
- * call (HIP events recorded on the context stream around each stage of every
- * step, inside the captured graph in batched mode), then reset.  out[0..5]:
+ * call, then reset.  Each stage's kernels stamp their span (earliest
+ * workgroup start, latest wave end) on the device's constant-rate clock, in
+ * both modes and inside the captured graph.  out[0..5]:
  * mean ms per step of transform, broad phase, narrow phase, topology+forces,
  * walls, fold+integrate+counters; out[6]: steps; out[7..7+ZDX_N_COUNTERS):
  * counter sums.  Writes min(n, 7 + ZDX_N_COUNTERS) values. */

@@ -124,7 +124,8 @@ int ht_pair(const double* trisA, int TA, const double* trisB, int TB, const doub
     std::vector<int> Iv(caps.n_ints());
     std::vector<double> Dv(caps.n_dbls());
     RecSink sink{rec, rec_cap, vtx, vcap, 0};
-    TopoOut t = topology(seg.data(), S, tol, area_eps, split != 0, sub(pb, pa), SArr<int>{Iv.data(), 1},
+    LaneHost wp;
+    TopoOut t = topology(wp, seg.data(), S, tol, area_eps, split != 0, sub(pb, pa), SArr<int>{Iv.data(), 1},
                          SArr<double>{Dv.data(), 1}, caps, sink);
     counts[0] = nondeg * TB;
     counts[1] = S;

@@ -88,13 +88,13 @@ struct zdx_ctx {
     DBuf<int32_t> m_tri;
     DBuf<int32_t> m_voff, m_nv, m_toff, m_nt;
     DBuf<double> m_tol, m_aeps;
+    DBuf<int32_t> m_soff, m_ns, m_susp;
     // per-pair API buffers
     struct Api {
         DBuf<int32_t> mesh;
         DBuf<double> state;
         DBuf<int64_t> wv_off, wt_off;
         DBuf<double> wv, aabb;
-        DBuf<uint8_t> degen;
         DBuf<int32_t> nondeg;
         DBuf<int32_t> pi, pj, order, npairs, status;
         DBuf<int64_t> seg_off, tri_checks, tri_tests;
@@ -119,6 +119,7 @@ struct zdx_ctx {
         DMeshes M;
         M.verts = m_verts.p; M.tri_idx = m_tri.p; M.vert_off = m_voff.p; M.nverts = m_nv.p;
         M.tri_off = m_toff.p; M.ntris = m_nt.p; M.tol = m_tol.p; M.area_eps = m_aeps.p;
+        M.susp_off = m_soff.p; M.nsusp = m_ns.p; M.susp = m_susp.p;
         return M;
     }
     void sync_meshes() {
@@ -126,7 +127,11 @@ struct zdx_ctx {
         std::vector<double> v;
         std::vector<int32_t> t, voff, nv, toff, nt;
         std::vector<double> tol, aeps;
+        std::vector<int32_t> soff, ns, susp;
         for (auto& m : meshes) {
+            soff.push_back((int32_t)susp.size());
+            ns.push_back((int32_t)m.h.suspect.size());
+            susp.insert(susp.end(), m.h.suspect.begin(), m.h.suspect.end());
             voff.push_back((int32_t)(v.size() / 3));
             nv.push_back((int32_t)(m.h.verts.size() / 3));
             toff.push_back((int32_t)(t.size() / 3));
@@ -144,6 +149,9 @@ struct zdx_ctx {
         m_nt.upload(nt.data(), nt.size(), stream);
         m_tol.upload(tol.data(), tol.size(), stream);
         m_aeps.upload(aeps.data(), aeps.size(), stream);
+        m_soff.upload(soff.data(), soff.size(), stream);
+        m_ns.upload(ns.data(), ns.size(), stream);
+        m_susp.upload(susp.data(), susp.size(), stream);
         HIPC(hipStreamSynchronize(stream));
         mesh_dirty = false;
         ++mesh_version;
@@ -201,8 +209,8 @@ struct zdx_sim {
     DBuf<double> d_mass, d_inv_mass, d_Ibinv, d_eqr, d_young, d_poisson, d_mu, d_rest, d_radius;
     DBuf<int64_t> d_wv_off, d_wt_off;
     DBuf<double> d_wv, d_aabb;
-    DBuf<uint8_t> d_degen;
     DBuf<int32_t> d_nondeg;
+    DBuf<int32_t> d_need_flag, d_need_off, d_need_list, d_n_need;
     DBuf<double> d_state[2];
     DBuf<double> d_F, d_T;
     DBuf<int32_t> d_counts;
@@ -240,7 +248,8 @@ struct zdx_sim {
     DBuf<int32_t> d_scr_i;
     DBuf<double> d_scr_d;
     int32_t lmax = 8;
-    int64_t topo_threads = 8192;
+    int64_t topo_threads = 2048;   // persistent one-wave workgroups
+    int32_t topo_lds = 20 * 1024;  // LDS scratch per workgroup (pairs up to ~50 segments)
     int32_t seg_max = 512;
     // walls
     DBuf<double> d_wtris, d_wnorm, d_wpos, d_wrad, d_wmu;
@@ -258,8 +267,10 @@ struct zdx_sim {
     double stage_ms[6] = {0, 0, 0, 0, 0, 0};
     NarrowCfg ncfg{256, 4, 0, 0};
     int32_t np_groups_per_cu = 8;
-    // per-stage events recorded inside every step (captured into the graphs)
-    hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // per-stage kernel spans stamped on the device clock inside every step
+    // (also inside the captured graphs): 6 stages x (start, end)
+    DBuf<unsigned long long> d_kt;
+    double clock_khz = 100000.0;
     double acc_ms[6] = {0, 0, 0, 0, 0, 0};
     double acc_ctr[ZDX_N_COUNTERS] = {0};
     int64_t acc_steps = 0;
@@ -268,11 +279,7 @@ struct zdx_sim {
     hipGraph_t graph[2] = {nullptr, nullptr};
     bool graphs_valid = false;
 
-    ~zdx_sim() {
-        drop_graphs();
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
-    }
+    ~zdx_sim() { drop_graphs(); }
     void drop_graphs() {
         for (int k = 0; k < 2; ++k) {
             if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
@@ -288,7 +295,7 @@ struct zdx_sim {
     DBodies bodies(int par) {
         DBodies B;
         B.n = N; B.mesh = d_mesh.p; B.state = d_state[par].p; B.wv_off = d_wv_off.p; B.wt_off = d_wt_off.p;
-        B.wv = d_wv.p; B.degen = d_degen.p; B.nondeg = d_nondeg.p; B.aabb = d_aabb.p;
+        B.wv = d_wv.p; B.nondeg = d_nondeg.p; B.aabb = d_aabb.p;
         return B;
     }
     DMat mat() {
@@ -330,7 +337,7 @@ struct zdx_sim {
         TopoBufs T{};
         T.lmax = lmax; T.contrib = d_contrib.p; T.ncontrib = d_ncontrib.p; T.comps = d_comps.p;
         T.loops_raw = d_loops_raw.p; T.tstatus = d_tstatus.p; T.scr_i = d_scr_i.p; T.scr_d = d_scr_d.p;
-        T.nthreads = topo_threads; T.seg_max = seg_max;
+        T.nthreads = topo_threads; T.seg_max = seg_max; T.lds_bytes = topo_lds;
         return T;
     }
     WallBufs wallbufs() {
@@ -406,6 +413,7 @@ struct zdx_sim {
         d_scr_i.ensure((size_t)c.n_ints() * topo_threads);
         d_scr_d.ensure((size_t)c.n_dbls() * topo_threads);
     }
+    int32_t transform_groups() const { return std::max(1, ctx->n_cu * 8); }
 };
 
 namespace {
@@ -434,8 +442,9 @@ void sim_setup_device(zdx_sim* s) {
     s->d_young.upload(young.data(), N, st); s->d_poisson.upload(pois.data(), N, st);
     s->d_mu.upload(mu.data(), N, st); s->d_rest.upload(rest.data(), N, st); s->d_radius.upload(rad.data(), N, st);
     s->d_wv_off.upload(wvo.data(), N + 1, st); s->d_wt_off.upload(wto.data(), N + 1, st);
-    s->d_wv.ensure(3 * (size_t)av); s->d_aabb.ensure(6 * (size_t)N); s->d_degen.ensure((size_t)at);
+    s->d_wv.ensure(3 * (size_t)av); s->d_aabb.ensure(6 * (size_t)N);
     s->d_nondeg.ensure(N);
+    s->d_need_flag.ensure(N); s->d_need_off.ensure(N + 1); s->d_need_list.ensure(N); s->d_n_need.ensure(1);
     s->d_state[0].ensure(16 * (size_t)N); s->d_state[1].ensure(16 * (size_t)N);
     s->d_F.ensure(3 * (size_t)N); s->d_T.ensure(3 * (size_t)N); s->d_counts.ensure(N);
     // broad phase
@@ -449,15 +458,19 @@ void sim_setup_device(zdx_sim* s) {
     s->d_status.ensure(1); s->d_ctr.ensure(16); s->d_queue.ensure(1);
     s->d_bcnt.ensure(64); s->d_bcur.ensure(64);
     s->d_arena_top.ensure(1);
-    for (auto& e : s->ev)
-        if (!e) HIPC(hipEventCreate(&e));
+    s->d_kt.ensure(12);
+    {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) == hipSuccess && khz > 0)
+            s->clock_khz = (double)khz;
+    }
     s->alloc_pairs(std::max(1024, 8 * N));
     s->alloc_arena(std::max<int64_t>(1 << 16, 64 * (int64_t)N));
     size_t groups = (size_t)persistent_groups(c, 16);
     s->tmax = c->max_tris;
     s->d_list_i.ensure(groups * 2 * (size_t)s->tmax);
     s->d_list_f.ensure(groups * 12 * (size_t)s->tmax);
-    s->topo_threads = std::min<int64_t>(8192, std::max<int64_t>(256, ((int64_t)N + 255) / 256 * 256));
+    s->topo_threads = std::min<int64_t>(2048, std::max<int64_t>(64, (int64_t)N));
     s->alloc_stage(s->stage_cap);
     // particle/wall histories start empty
     for (int k = 0; k < 2; ++k) {
@@ -539,34 +552,31 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     DBodies B = s->bodies(par);
     DMeshes M = s->ctx->dmeshes();
     DMat mat = s->mat();
-    HIPC(hipEventRecord(s->ev[0], st));
-    launch_transform(B, M, st);
-    HIPC(hipEventRecord(s->ev[1], st));
-    launch_broadphase(s->bpbufs(par), st);
+    unsigned long long* kt = s->d_kt.p;   // stage slots: 0 transform 1 broad 2 narrow 3 topo 4 walls 5 fold
+    launch_kt_reset(kt, 6, st);
+    launch_broadphase(s->bpbufs(par), st, kt + 2);
+    launch_need_list(s->N, s->d_state[par].p, s->d_radius.p, s->d_cnt_all.p, s->wallbufs(), s->d_need_flag.p,
+                     s->d_need_off.p, s->d_scan_tmp.p, s->d_need_list.p, s->d_n_need.p, st);
+    launch_transform(B, M, st, kt + 0, s->d_need_list.p, s->d_n_need.p, s->transform_groups());
     DPairs P = s->dpairs(par, batched);
     if (batched) launch_lpt_order(P, M, s->d_mesh.p, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st);
     launch_fill_u64(s->d_arena_top.p, 0ull, 1, st);
     launch_fill_i32((int32_t*)s->d_queue.p, 0, 1, st);
-    HIPC(hipEventRecord(s->ev[2], st));
     NarrowCfg nc = s->ncfg;
     nc.use_queue = batched ? 1 : 0;
     nc.groups = persistent_groups(s->ctx, s->np_groups_per_cu);
     launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->d_stage.p, s->stage_cap, s->tmax,
-                  s->d_queue.p, st);
-    HIPC(hipEventRecord(s->ev[3], st));
+                  s->d_queue.p, st, kt + 4);
     TopoBufs T = s->topobufs();
     HistBufs H = s->hist(par);
-    launch_topo_sim(P, M, B, mat, H, T, s->sp, st);
-    HIPC(hipEventRecord(s->ev[4], st));
+    launch_topo_sim(P, M, B, mat, H, T, s->sp, st, kt + 6);
     if (s->W > 0)
         launch_walls(B, M, mat, s->wallbufs(), H, s->d_wcontrib.p, s->d_wnc.p, s->d_wscr.p, s->wall_threads,
-                     s->wall_pts_cap, s->sp, s->d_status.p, st);
-    HIPC(hipEventRecord(s->ev[5], st));
+                     s->wall_pts_cap, s->sp, s->d_status.p, st, kt + 8);
     launch_fold_integrate(s->N, s->d_state[par].p, s->d_state[nxt].p, mat, s->d_adj_off.p, s->d_adj_j.p,
-                          s->d_adj_pair.p, s->adj_cap, T, s->W, s->d_wcontrib.p, s->d_wnc.p, s->max_wall_tris, s->d_F.p, s->d_T.p,
-                          s->d_counts.p, s->sp, st);
+                          s->d_adj_pair.p, s->adj_cap, T, s->W, s->d_wcontrib.p, s->d_wnc.p, s->max_wall_tris,
+                          s->d_F.p, s->d_T.p, s->d_counts.p, s->sp, st, kt + 10);
     launch_counters(P, T, s->N * s->W, s->d_wnc.p, s->d_ctr.p, st);
-    HIPC(hipEventRecord(s->ev[6], st));
 }
 
 int32_t read_status(zdx_sim* s) {
@@ -663,11 +673,11 @@ void step_once(zdx_sim* s, bool batched) {
         int32_t status = read_status(s);
         if (status == 0) {
             read_counters(s);
-            for (int k = 0; k < 6; ++k) {
-                float ms = 0.f;
-                HIPC(hipEventElapsedTime(&ms, s->ev[k], s->ev[k + 1]));
-                s->acc_ms[k] += ms;
-            }
+            unsigned long long kt[12];
+            HIPC(hipMemcpy(kt, s->d_kt.p, sizeof(kt), hipMemcpyDeviceToHost));
+            for (int k = 0; k < 6; ++k)
+                if (kt[2 * k + 1] > kt[2 * k] && kt[2 * k] != ~0ull)
+                    s->acc_ms[k] += (double)(kt[2 * k + 1] - kt[2 * k]) / s->clock_khz;
             for (int k = 0; k < ZDX_N_COUNTERS; ++k) s->acc_ctr[k] += (double)s->last_ctr[k];
             s->acc_steps += 1;
             s->parity = 1 - par;
@@ -791,10 +801,10 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
         A.state.upload(state.data(), state.size(), st);
         A.wv_off.upload(wvo.data(), nb + 1, st);
         A.wt_off.upload(wto.data(), nb + 1, st);
-        A.wv.ensure(3 * (size_t)av); A.aabb.ensure(6 * (size_t)nb); A.degen.ensure((size_t)at); A.nondeg.ensure(nb);
+        A.wv.ensure(3 * (size_t)av); A.aabb.ensure(6 * (size_t)nb); A.nondeg.ensure(nb);
         DBodies B;
         B.n = nb; B.mesh = A.mesh.p; B.state = A.state.p; B.wv_off = A.wv_off.p; B.wt_off = A.wt_off.p;
-        B.wv = A.wv.p; B.degen = A.degen.p; B.nondeg = A.nondeg.p; B.aabb = A.aabb.p;
+        B.wv = A.wv.p; B.nondeg = A.nondeg.p; B.aabb = A.aabb.p;
         DMeshes M = c->dmeshes();
         launch_transform(B, M, st);
         A.pi.ensure(n); A.pj.ensure(n); A.npairs.ensure(1); A.status.ensure(1); A.order.ensure(n);
@@ -830,8 +840,9 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
             // topology with the minimal-energy force
             TopoBufs T{};
             T.lmax = A.lmax; T.ncontrib = A.ncontrib.p; T.comps = A.comps.p; T.loops_raw = A.loops_raw.p;
-            T.tstatus = A.tstatus.p; T.nthreads = std::min<int64_t>(8192, ((int64_t)n + 63) / 64 * 64);
+            T.tstatus = A.tstatus.p; T.nthreads = std::min<int64_t>(4096, (int64_t)n);
             T.seg_max = A.stage_cap;
+            T.lds_bytes = 20 * 1024;
             T.vmax = 4 * A.stage_cap + 16;
             T.k_stiff = opts->k;
             TopoCaps caps; caps.S = T.seg_max; caps.E2 = 4 * T.seg_max;
@@ -1235,7 +1246,6 @@ zdx_status zdx_sim_autotune(zdx_sim* s, int32_t* out_block, int32_t* out_groups)
         DBodies B = s->bodies(par);
         DMeshes M = s->ctx->dmeshes();
         launch_fill_i32(s->d_status.p, 0, 1, st);
-        launch_transform(B, M, st);
         launch_broadphase(s->bpbufs(par), st);
         HIPC(hipStreamSynchronize(st));
         if (read_status(s) & ST_PAIRS) {
@@ -1243,6 +1253,9 @@ zdx_status zdx_sim_autotune(zdx_sim* s, int32_t* out_block, int32_t* out_groups)
             launch_fill_i32(s->d_status.p, 0, 1, st);
             launch_broadphase(s->bpbufs(par), st);
         }
+        launch_need_list(s->N, s->d_state[par].p, s->d_radius.p, s->d_cnt_all.p, s->wallbufs(), s->d_need_flag.p,
+                         s->d_need_off.p, s->d_scan_tmp.p, s->d_need_list.p, s->d_n_need.p, st);
+        launch_transform(B, M, st, nullptr, s->d_need_list.p, s->d_n_need.p, s->transform_groups());
         DPairs P = s->dpairs(par, true);
         launch_lpt_order(P, M, s->d_mesh.p, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st);
         const int cand[][2] = {{128, 4}, {128, 8}, {256, 1}, {256, 2}, {256, 4}, {256, 8}, {512, 2}, {512, 4}};

@@ -347,6 +347,43 @@ struct TopoOut {
     int status;
 };
 
+// Lane policies for topology(): W lanes run the pair's algorithm together.
+// Control flow and every serial step are executed redundantly by all lanes
+// with identical values (each lane reads back its own identical writes), and
+// the O(S^2) scans are split across lanes, their results combined by a ballot
+// mask that the lanes then walk in ascending order.  sync() is needed only
+// after a lane-parallel phase whose writes other lanes read.
+struct LaneHost {
+    static constexpr int W = 1;
+    int lane() const { return 0; }
+    uint64_t ballot(bool p) const { return p ? 1ull : 0ull; }
+    void sync() const {}
+};
+
+ZDX_HD inline int ctz64(uint64_t m) {
+    int k = 0;
+    while (!(m & 1ull)) { m >>= 1; ++k; }
+    return k;
+}
+
+// Reference scan-order key of candidate j for segment i in
+// label_segments_by_contact_greedy (main.cpp:834-857): rep point r of i (p0,
+// p1, mid), neighbour cell (dx,dy,dz) lexicographic; smallest match wins.
+ZDX_HD inline int cand_key(SArr<int> cell9, int i, int j) {
+    int key = 1 << 30;
+    for (int r = 0; r < 3; ++r) {
+        int ax = cell9[9 * i + 3 * r], ay = cell9[9 * i + 3 * r + 1], az = cell9[9 * i + 3 * r + 2];
+        for (int q = 0; q < 3; ++q) {
+            int bx = cell9[9 * j + 3 * q], by = cell9[9 * j + 3 * q + 1], bz = cell9[9 * j + 3 * q + 2];
+            if (near1(ax, ay, az, bx, by, bz)) {
+                int k = r * 27 + (bx - ax + 1) * 9 + (by - ay + 1) * 3 + (bz - az + 1);
+                if (k < key) key = k;
+            }
+        }
+    }
+    return key;
+}
+
 // Per-pair pipeline after the narrow phase (main.cpp:2152-2219):
 // components (label_segments_by_contact_greedy 804-898, when split), then per
 // component snap_endpoints (680-746), split_t_junctions (900-961),
@@ -355,13 +392,16 @@ struct TopoOut {
 // filters and the flip of nA toward B (2188-2219).
 //
 // seg: S segments, 6 doubles each (p0, p1), in narrow-phase order.
+// I / D: int and double scratch of TopoCaps::n_ints() / n_dbls() entries.
 // For every accepted loop, sink(loop_index, comp, raw_index, Sn, Gn, nA, xc0,
-// area, nverts, vids, V) is called in the reference's order.
-template <class Sink>
-ZDX_HD inline TopoOut topology(const double* seg, int S, double tol, double area_eps, bool split,
+// area, nverts, vids, V) is called (by every lane) in the reference's order.
+template <class WP, class Sink>
+ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double tol, double area_eps, bool split,
                                const V3& dirAB, SArr<int> I, SArr<double> D, const TopoCaps& cap, Sink& sink) {
     TopoOut out; out.comps = 0; out.loops_raw = 0; out.loops = 0; out.status = TOPO_OK;
     if (S <= 0) return out;
+    const int W = WP::W;
+    const int ln = wp.lane();
     const double h = tol;
     const int S2 = 2 * S;
     // ---- scratch layout
@@ -403,7 +443,7 @@ ZDX_HD inline TopoOut topology(const double* seg, int S, double tol, double area
     // ---- components
     int ncomp = 0;
     if (split) {
-        for (int i = 0; i < S; ++i) {
+        for (int i = ln; i < S; i += W) {
             V3 p0 = P0(i), p1 = P1(i);
             V3 mid = mul(add(p0, p1), 0.5);
             I3 a = cell_of(p0, h), b = cell_of(p1, h), c = cell_of(mid, h);
@@ -412,6 +452,7 @@ ZDX_HD inline TopoOut topology(const double* seg, int S, double tol, double area
             cell9[9 * i + 6] = c.x; cell9[9 * i + 7] = c.y; cell9[9 * i + 8] = c.z;
             vis[i] = 0;
         }
+        wp.sync();
         const double t2 = h * h;
         int nord = 0;
         for (int seed = 0; seed < S; ++seed) {
@@ -426,41 +467,33 @@ ZDX_HD inline TopoOut topology(const double* seg, int S, double tol, double area
                 V3 p1 = P0(i), q1 = P1(i);
                 V3 amn = v3(dmin(p1.x, q1.x) - h, dmin(p1.y, q1.y) - h, dmin(p1.z, q1.z) - h);
                 V3 amx = v3(dmax(p1.x, q1.x) + h, dmax(p1.y, q1.y) + h, dmax(p1.z, q1.z) + h);
-                // candidates in the reference's scan order: for rep r of i (p0,
-                // p1, mid), neighbour cell (dx,dy,dz) lexicographic, then
-                // segment id; first appearance wins.  Accepted ones are pushed
-                // in that order; the order key is r*27 + cell offset index.
-                int first_acc = sp;
-                for (int j = 0; j < S; ++j) {
-                    if (j == i || vis[j]) continue;
-                    int key = 1 << 30;
-                    for (int r = 0; r < 3; ++r) {
-                        int ax = cell9[9 * i + 3 * r], ay = cell9[9 * i + 3 * r + 1], az = cell9[9 * i + 3 * r + 2];
-                        for (int q = 0; q < 3; ++q) {
-                            int bx = cell9[9 * j + 3 * q], by = cell9[9 * j + 3 * q + 1], bz = cell9[9 * j + 3 * q + 2];
-                            if (near1(ax, ay, az, bx, by, bz)) {
-                                int k = r * 27 + (bx - ax + 1) * 9 + (by - ay + 1) * 3 + (bz - az + 1);
-                                if (k < key) key = k;
-                            }
-                        }
+                // candidates of i in the reference's scan order: accepted ones
+                // (unvisited, AABB(+tol) overlap, seg_seg_dist2 <= tol^2) are
+                // pushed ordered by (scan key, segment id)
+                const int first_acc = sp;
+                for (int base = 0; base < S; base += W) {
+                    const int j = base + ln;
+                    bool acc = false;
+                    if (j < S && j != i && !vis[j] && cand_key(cell9, i, j) != (1 << 30)) {
+                        V3 p2 = P0(j), q2 = P1(j);
+                        V3 bmn = v3(dmin(p2.x, q2.x) - h, dmin(p2.y, q2.y) - h, dmin(p2.z, q2.z) - h);
+                        V3 bmx = v3(dmax(p2.x, q2.x) + h, dmax(p2.y, q2.y) + h, dmax(p2.z, q2.z) + h);
+                        bool ovl = !(amx.x < bmn.x || amn.x > bmx.x || amx.y < bmn.y || amn.y > bmx.y ||
+                                     amx.z < bmn.z || amn.z > bmx.z);
+                        acc = ovl && seg_seg(p1, q1, p2, q2) <= t2;
                     }
-                    if (key == (1 << 30)) continue;
-                    V3 p2 = P0(j), q2 = P1(j);
-                    V3 bmn = v3(dmin(p2.x, q2.x) - h, dmin(p2.y, q2.y) - h, dmin(p2.z, q2.z) - h);
-                    V3 bmx = v3(dmax(p2.x, q2.x) + h, dmax(p2.y, q2.y) + h, dmax(p2.z, q2.z) + h);
-                    if (amx.x < bmn.x || amn.x > bmx.x || amx.y < bmn.y || amn.y > bmx.y || amx.z < bmn.z ||
-                        amn.z > bmx.z)
-                        continue;
-                    if (seg_seg(p1, q1, p2, q2) <= t2) {
-                        // insertion into the accepted run by (key, j); j ascends
-                        // so only the key decides
+                    uint64_t m = wp.ballot(acc);
+                    while (m) {
+                        const int jj = base + ctz64(m);
+                        m &= m - 1;
+                        const int key = cand_key(cell9, i, jj);
                         int pos = sp;
                         while (pos > first_acc && spk[pos - 1] > key) {
                             stk[pos] = stk[pos - 1];
                             spk[pos] = spk[pos - 1];
                             --pos;
                         }
-                        stk[pos] = j;
+                        stk[pos] = jj;
                         spk[pos] = key;
                         ++sp;
                     }
@@ -482,6 +515,7 @@ ZDX_HD inline TopoOut topology(const double* seg, int S, double tol, double area
         ncomp = 1;
     }
     out.comps = ncomp;
+    wp.sync();
 
     // ---- per component
     for (int ci = 0; ci < ncomp; ++ci) {
@@ -491,24 +525,30 @@ ZDX_HD inline TopoOut topology(const double* seg, int S, double tol, double area
         const int n = 2 * ns;
         auto PT = [&](int k) { int s = cord[base + (k >> 1)]; return (k & 1) ? P1(s) : P0(s); };
         // snap_endpoints: union-find over points within tol in +-1 cells
-        for (int k = 0; k < n; ++k) {
+        for (int k = ln; k < n; k += W) {
             I3 cc = cell_of(PT(k), h);
             pcell[3 * k] = cc.x; pcell[3 * k + 1] = cc.y; pcell[3 * k + 2] = cc.z;
             par[k] = k; rnk[k] = 0; rootvid[k] = -1;
         }
+        wp.sync();
         auto find = [&](int x) {
             int r = x;
             while (par[r] != r) r = par[r];
             while (par[x] != r) { int nx = par[x]; par[x] = r; x = nx; }
             return r;
         };
-        for (int i = 0; i < n; ++i) {
-            V3 pi = PT(i);
-            for (int j = 0; j < i; ++j) {
-                if (!near1(pcell[3 * i], pcell[3 * i + 1], pcell[3 * i + 2], pcell[3 * j], pcell[3 * j + 1], pcell[3 * j + 2]))
-                    continue;
-                if (norm2(sub(pi, PT(j))) <= h * h) {
-                    int ra = find(i), rb = find(j);
+        for (int i = 1; i < n; ++i) {
+            const V3 pi = PT(i);
+            const int cx = pcell[3 * i], cy = pcell[3 * i + 1], cz = pcell[3 * i + 2];
+            for (int jb = 0; jb < i; jb += W) {
+                const int j = jb + ln;
+                bool hit = j < i && near1(cx, cy, cz, pcell[3 * j], pcell[3 * j + 1], pcell[3 * j + 2]) &&
+                           norm2(sub(pi, PT(j))) <= h * h;
+                uint64_t m = wp.ballot(hit);
+                while (m) {
+                    const int jj = jb + ctz64(m);
+                    m &= m - 1;
+                    int ra = find(i), rb = find(jj);
                     if (ra != rb) {
                         if (rnk[ra] < rnk[rb]) par[ra] = rb;
                         else if (rnk[ra] > rnk[rb]) par[rb] = ra;
@@ -551,35 +591,46 @@ ZDX_HD inline TopoOut topology(const double* seg, int S, double tol, double area
         const double tol2 = h * h;
         int ne2 = 0;
         for (int e = 0; e < ne; ++e) {
-            int a = E[2 * e], b = E[2 * e + 1];
-            V3 pa = VV(a), pb = VV(b);
+            const int a = E[2 * e], b = E[2 * e + 1];
+            const V3 pa = VV(a), pb = VV(b);
             V3 mn = v3(dmin(pa.x, pb.x) - h, dmin(pa.y, pb.y) - h, dmin(pa.z, pb.z) - h);
             V3 mx = v3(dmax(pa.x, pb.x) + h, dmax(pa.y, pb.y) + h, dmax(pa.z, pb.z) + h);
-            I3 c0 = cell_of(mn, h), c1 = cell_of(mx, h);
+            const I3 c0 = cell_of(mn, h), c1 = cell_of(mx, h);
             int nsp = 0;
-            for (int v = 0; v < nv; ++v) {
-                if (v == a || v == b) continue;
-                int cx = vcell3[3 * v], cy = vcell3[3 * v + 1], cz = vcell3[3 * v + 2];
-                if (cx < c0.x || cx > c1.x || cy < c0.y || cy > c1.y || cz < c0.z || cz > c1.z) continue;
-                double d2, t;
-                pt_seg(VV(v), pa, pb, d2, t);
-                if (d2 <= tol2 && t > 1e-6 && t < 1.0 - 1e-6) {
+            for (int vb = 0; vb < nv; vb += W) {
+                const int v = vb + ln;
+                bool hit = false;
+                if (v < nv && v != a && v != b) {
+                    int cx = vcell3[3 * v], cy = vcell3[3 * v + 1], cz = vcell3[3 * v + 2];
+                    if (!(cx < c0.x || cx > c1.x || cy < c0.y || cy > c1.y || cz < c0.z || cz > c1.z)) {
+                        double d2, t;
+                        pt_seg(VV(v), pa, pb, d2, t);
+                        hit = d2 <= tol2 && t > 1e-6 && t < 1.0 - 1e-6;
+                    }
+                }
+                uint64_t m = wp.ballot(hit);
+                while (m) {
+                    const int vv = vb + ctz64(m);
+                    m &= m - 1;
+                    double d2, t;
+                    pt_seg(VV(vv), pa, pb, d2, t);
                     // the reference scans cells ix, iy, iz over [c0, c1]; key = scan index
+                    int cx = vcell3[3 * vv], cy = vcell3[3 * vv + 1], cz = vcell3[3 * vv + 2];
                     int nyz = (c1.y - c0.y + 1) * (c1.z - c0.z + 1);
-                    int k = (cx - c0.x) * nyz + (cy - c0.y) * (c1.z - c0.z + 1) + (cz - c0.z);
-                    spt[nsp] = t; spv[nsp] = v; spk[nsp] = k;
+                    spt[nsp] = t; spv[nsp] = vv;
+                    spk[nsp] = (cx - c0.x) * nyz + (cy - c0.y) * (c1.z - c0.z + 1) + (cz - c0.z);
                     ++nsp;
                 }
             }
             if (nsp == 0) {
-                if (ne2 + 1 > cap.E2) { out.status |= TOPO_E2_OVERFLOW; return out; }
+                if (ne2 + 1 > E2c) { out.status |= TOPO_E2_OVERFLOW; return out; }
                 E2[2 * ne2] = a; E2[2 * ne2 + 1] = b; ++ne2;
                 continue;
             }
             SplitArr sa; sa.t = spt; sa.v = spv; sa.k = spk;
             // bring candidates into the reference's scan order, then its std::sort by t
             SplitByScan byscan;
-            for (int i2 = 1; i2 < nsp; ++i2) {   // stable insertion sort (small lists)
+            for (int i2 = 1; i2 < nsp; ++i2) {
                 Split x = sa.get(i2);
                 int j2 = i2 - 1;
                 while (j2 >= 0 && byscan(x, sa.get(j2))) { sa.set(j2 + 1, sa.get(j2)); --j2; }
@@ -593,13 +644,13 @@ ZDX_HD inline TopoOut topology(const double* seg, int S, double tol, double area
                 if (v == last) continue;
                 last = v;
                 if (prevv != v) {
-                    if (ne2 + 1 > cap.E2) { out.status |= TOPO_E2_OVERFLOW; return out; }
+                    if (ne2 + 1 > E2c) { out.status |= TOPO_E2_OVERFLOW; return out; }
                     E2[2 * ne2] = prevv; E2[2 * ne2 + 1] = v; ++ne2;
                 }
                 prevv = v;
             }
             if (prevv != b) {
-                if (ne2 + 1 > cap.E2) { out.status |= TOPO_E2_OVERFLOW; return out; }
+                if (ne2 + 1 > E2c) { out.status |= TOPO_E2_OVERFLOW; return out; }
                 E2[2 * ne2] = prevv; E2[2 * ne2 + 1] = b; ++ne2;
             }
         }
@@ -676,6 +727,7 @@ ZDX_HD inline TopoOut topology(const double* seg, int S, double tol, double area
             }
         }
         out.loops_raw += nloops_here;
+        wp.sync();
     }
     return out;
 }

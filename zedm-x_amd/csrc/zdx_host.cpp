@@ -203,6 +203,28 @@ HostMesh build_mesh(const std::vector<double>& in, bool center) {
         ec += 3;
     }
     if (ec > 0) m.mean_edge = es / static_cast<double>(ec);
+    // Triangles whose world-frame plane test (plane_from_tri, |n| < 1e-30) is
+    // not settled by the body frame.  A world corner is R(q) v + p with
+    // |p| <= 1e3 (checked per body on the device); its rounding error is below
+    // d = 64 eps (R + 1e3), so an edge moves by at most 2 d and |e1 x e2| by at
+    // most 4 d Lmax + 4 d^2 (rotation by a unit quaternion keeps |e1 x e2| up
+    // to a relative 1e-15).  Any triangle with 0.99 |e1 x e2| above that bound
+    // (plus 1e-30) stays non-degenerate; the rest are tested on the device.
+    {
+        const double eps = 1.1102230246251565e-16;
+        const double dd = 64.0 * eps * (m.radius + 1e3);
+        for (size_t t = 0; t < T; ++t) {
+            const double* v = &m.tris[9 * t];
+            double e1[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]};
+            double e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+            double e3[3] = {v[6] - v[3], v[7] - v[4], v[8] - v[5]};
+            double c[3];
+            cross3(e1, e2, c);
+            double ln = std::sqrt(d3(c, c));
+            double L = std::sqrt(std::max(d3(e1, e1), std::max(d3(e2, e2), d3(e3, e3))));
+            if (!(0.99 * ln > 4.0 * dd * L + 4.0 * dd * dd + 1e-30)) m.suspect.push_back((int32_t)t);
+        }
+    }
     // indexed form: bitwise-equal corners share a slot
     {
         std::unordered_map<Key3, int, Key3H> slot;

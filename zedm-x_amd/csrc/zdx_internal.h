@@ -21,6 +21,7 @@ struct HostMesh {
     double center[3] = {0, 0, 0};
     double inertia[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int n_unique_ref = 0;          // the reference's unique_vertices count (tolerance merge)
+    std::vector<int32_t> suspect;  // triangles whose world-frame plane test is not certain
 };
 
 bool load_stl(const std::string& path, std::vector<double>& tris);
@@ -73,6 +74,9 @@ struct DMeshes {
     const int32_t* ntris;
     const double* tol;       // per mesh: mean_edge*0.1 rule (main.cpp:2118-2121)
     const double* area_eps;  // per mesh (main.cpp:2205)
+    const int32_t* susp_off; // per mesh: first suspect triangle in susp
+    const int32_t* nsusp;    // per mesh: triangles whose world plane may degenerate
+    const int32_t* susp;     // suspect triangle indices
 };
 
 // Bodies whose world geometry is evaluated each step.
@@ -83,7 +87,6 @@ struct DBodies {
     const int64_t* wv_off;   // per body: first world vertex
     const int64_t* wt_off;   // per body: first triangle flag
     double* wv;              // world vertices (3 doubles each)
-    uint8_t* degen;          // per triangle: 1 if plane_from_tri fails on the world triangle
     int32_t* nondeg;         // per body: non-degenerate triangle count
     double* aabb;            // per body: min(3) max(3) over world vertices
 };

@@ -76,8 +76,9 @@ struct TopoBufs {
     int32_t* tstatus;          // per pair
     int32_t* scr_i;
     double* scr_d;
-    int64_t nthreads;          // persistent threads (scratch stride)
-    int32_t seg_max;           // scratch sized for this many segments
+    int64_t nthreads;          // persistent workgroups (one wave each); global scratch slices
+    int32_t seg_max;           // global scratch slice sized for this many segments
+    int32_t lds_bytes;         // dynamic LDS per workgroup for pairs that fit
     // API mode outputs
     double* rec;               // pair * lmax * 28
     double* vtx;               // pair * vmax * 3
@@ -98,25 +99,33 @@ struct NarrowCfg {
     int32_t use_queue;         // 1: atomic work queue (batched)
 };
 
-void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s);
-void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s);      // all stages, device-sized
+// kt: optional kernel-span stamps (2 x unsigned long long per stage: earliest
+// start, latest end on the device clock); NULL disables them.
+void launch_kt_reset(unsigned long long* kt, int n_stages, hipStream_t s);
+// list/n_list: bodies to transform (device list and count; NULL = all bodies)
+void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s, unsigned long long* kt = nullptr,
+                      const int32_t* list = nullptr, const int32_t* n_list = nullptr, int32_t groups = 0);
+// bodies that are in a candidate pair or near a wall triangle -> list
+void launch_need_list(int32_t n, const double* state, const double* radius, const int32_t* cnt_all, const WallBufs& W,
+                      int32_t* flag, int32_t* off, int32_t* scan_tmp, int32_t* list, int32_t* n_list, hipStream_t s);
+void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s, unsigned long long* kt = nullptr);
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* list_i,
                    float* list_f, double* stage, int32_t stage_cap, int32_t tmax, unsigned int* queue,
-                   hipStream_t s);
+                   hipStream_t s, unsigned long long* kt = nullptr);
 void launch_lpt_order(const DPairs& P, const DMeshes& M, const int32_t* body_mesh, int32_t* order, int32_t* bucket_cnt,
                       int32_t* bucket_cur, int32_t pair_cap, hipStream_t s);
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
-                     const TopoBufs& T, const SimParams& sp, hipStream_t s);
+                     const TopoBufs& T, const SimParams& sp, hipStream_t s, unsigned long long* kt = nullptr);
 void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int32_t split,
                      hipStream_t s);
 void launch_walls(const DBodies& B, const DMeshes& M, const DMat& mat, const WallBufs& W, const HistBufs& H,
                   double* wcontrib, int32_t* wncontrib, double* scr, int64_t nthreads, int32_t pts_cap,
-                  const SimParams& sp, int32_t* status, hipStream_t s);
+                  const SimParams& sp, int32_t* status, hipStream_t s, unsigned long long* kt = nullptr);
 void launch_fold_integrate(int32_t n, const double* state_in, double* state_out, const DMat& mat,
                            const int32_t* adj_off, const int32_t* adj_j, const int32_t* adj_pair,
                            int32_t adj_cap, const TopoBufs& T, int32_t n_walls, const double* wcontrib, const int32_t* wncontrib,
                            int32_t max_wall_tris, double* F, double* Tq, int32_t* counts, const SimParams& sp,
-                           hipStream_t s);
+                           hipStream_t s, unsigned long long* kt = nullptr);
 // counters[ZDX_N_COUNTERS] (int64) reduced from per-pair / per-wall arrays
 void launch_counters(const DPairs& P, const TopoBufs& T, int32_t n_pw, const int32_t* wncontrib,
                      unsigned long long* counters, hipStream_t s);

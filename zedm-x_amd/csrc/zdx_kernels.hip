@@ -87,62 +87,94 @@ __device__ inline int block_excl(int v, int* s_wave, int& total) {
 
 __device__ inline V3 ld3(const double* p) { return v3(p[0], p[1], p[2]); }
 
+// Kernel span stamps: kt[0] = earliest start (first workgroup), kt[1] =
+// latest end (last wave), read from the constant-rate device clock; vector
+// atomics on global memory, so they work the same inside a captured graph.
+__device__ inline void kt_start(unsigned long long* kt) {
+    if (kt && threadIdx.x == 0) atomicMin(kt, (unsigned long long)wall_clock64());
+}
+__device__ inline void kt_end(unsigned long long* kt) {
+    if (kt && lane_id() == 0) atomicMax(kt + 1, (unsigned long long)wall_clock64());
+}
+__global__ void k_kt_reset(unsigned long long* kt, int n) {
+    int k = threadIdx.x;
+    if (k < n) { kt[2 * k] = ~0ull; kt[2 * k + 1] = 0ull; }
+}
+
 // ------------------------------------------------------------ transform
-__global__ void __launch_bounds__(256) k_transform(DBodies B, DMeshes M) {
+// World vertices of the listed bodies.  quat_rotate (main.cpp:93-98)
+// normalises q for every vertex; the normalised q is the same for all of a
+// body's vertices, so it is computed once (bitwise the same doubles).
+// Degeneracy of world triangles (plane_from_tri failing) only feeds the
+// tri_checks counter: triangles whose body-frame plane is far from degenerate
+// cannot become degenerate under a rotation plus a translation of magnitude
+// <= 1e3 (rounding error bound, see build_mesh on the host), so only the
+// mesh's "suspect" triangles are tested, or all of them for far-away bodies.
+__global__ void __launch_bounds__(256) k_transform(DBodies B, DMeshes M, const int32_t* list, const int32_t* n_list,
+                                                   unsigned long long* kt) {
+    kt_start(kt);
     __shared__ double s_red[6][256 / WAVE];
-    __shared__ int s_cnt;
-    const int b = blockIdx.x;
-    if (b >= B.n) return;
-    const int m = B.mesh[b];
-    const double* st = B.state + 16 * (int64_t)b;
-    const V3 pos = v3(st[0], st[1], st[2]);
-    Qt q; q.w = st[3]; q.x = st[4]; q.y = st[5]; q.z = st[6];
-    const int nv = M.nverts[m];
-    const double* bv = M.verts + 3 * (int64_t)M.vert_off[m];
-    double* wv = B.wv + 3 * B.wv_off[b];
-    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int v = threadIdx.x; v < nv; v += blockDim.x) {
-        V3 w = add(qrotate(q, ld3(bv + 3 * v)), pos);
-        wv[3 * v] = w.x; wv[3 * v + 1] = w.y; wv[3 * v + 2] = w.z;
-        mn[0] = fmin(mn[0], w.x); mn[1] = fmin(mn[1], w.y); mn[2] = fmin(mn[2], w.z);
-        mx[0] = fmax(mx[0], w.x); mx[1] = fmax(mx[1], w.y); mx[2] = fmax(mx[2], w.z);
-    }
-    for (int k = 0; k < 3; ++k) {
-        for (int d = 32; d >= 1; d >>= 1) {
-            mn[k] = fmin(mn[k], __shfl_xor(mn[k], d, WAVE));
-            mx[k] = fmax(mx[k], __shfl_xor(mx[k], d, WAVE));
+    __shared__ int s_bad;
+    const int nb = list ? *n_list : B.n;
+    for (int it = blockIdx.x; it < nb; it += gridDim.x) {
+        const int b = list ? list[it] : it;
+        const int m = B.mesh[b];
+        const double* st = B.state + 16 * (int64_t)b;
+        const V3 pos = v3(st[0], st[1], st[2]);
+        Qt q; q.w = st[3]; q.x = st[4]; q.y = st[5]; q.z = st[6];
+        const Qt qn = qnormalize(q);
+        Qt qc; qc.w = qn.w; qc.x = -qn.x; qc.y = -qn.y; qc.z = -qn.z;
+        const int nv = M.nverts[m];
+        const double* bv = M.verts + 3 * (int64_t)M.vert_off[m];
+        double* wv = B.wv + 3 * B.wv_off[b];
+        double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int v = threadIdx.x; v < nv; v += blockDim.x) {
+            Qt pv; pv.w = 0.0; pv.x = bv[3 * v]; pv.y = bv[3 * v + 1]; pv.z = bv[3 * v + 2];
+            Qt r = qmul(qmul(qn, pv), qc);
+            V3 w = add(v3(r.x, r.y, r.z), pos);
+            wv[3 * v] = w.x; wv[3 * v + 1] = w.y; wv[3 * v + 2] = w.z;
+            mn[0] = fmin(mn[0], w.x); mn[1] = fmin(mn[1], w.y); mn[2] = fmin(mn[2], w.z);
+            mx[0] = fmax(mx[0], w.x); mx[1] = fmax(mx[1], w.y); mx[2] = fmax(mx[2], w.z);
         }
-    }
-    const int wid = threadIdx.x / WAVE;
-    if (lane_id() == 0) {
-        for (int k = 0; k < 3; ++k) { s_red[k][wid] = mn[k]; s_red[3 + k][wid] = mx[k]; }
-    }
-    if (threadIdx.x == 0) s_cnt = 0;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double* box = B.aabb + 6 * (int64_t)b;
         for (int k = 0; k < 3; ++k) {
-            double a = s_red[k][0], c = s_red[3 + k][0];
-            for (int w = 1; w < (int)(blockDim.x / WAVE); ++w) { a = fmin(a, s_red[k][w]); c = fmax(c, s_red[3 + k][w]); }
-            box[k] = a; box[3 + k] = c;
+            for (int d = 32; d >= 1; d >>= 1) {
+                mn[k] = fmin(mn[k], __shfl_xor(mn[k], d, WAVE));
+                mx[k] = fmax(mx[k], __shfl_xor(mx[k], d, WAVE));
+            }
         }
+        const int wid = threadIdx.x / WAVE;
+        if (lane_id() == 0) {
+            for (int k = 0; k < 3; ++k) { s_red[k][wid] = mn[k]; s_red[3 + k][wid] = mx[k]; }
+        }
+        if (threadIdx.x == 0) s_bad = 0;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double* box = B.aabb + 6 * (int64_t)b;
+            for (int k = 0; k < 3; ++k) {
+                double a = s_red[k][0], c = s_red[3 + k][0];
+                for (int w = 1; w < (int)(blockDim.x / WAVE); ++w) { a = fmin(a, s_red[k][w]); c = fmax(c, s_red[3 + k][w]); }
+                box[k] = a; box[3 + k] = c;
+            }
+        }
+        const int T = M.ntris[m];
+        const int32_t* ti = M.tri_idx + 3 * (int64_t)M.tri_off[m];
+        const bool far = !(fabs(pos.x) <= 1e3 && fabs(pos.y) <= 1e3 && fabs(pos.z) <= 1e3);
+        const int nchk = far ? T : M.nsusp[m];
+        const int32_t* sl = M.susp + M.susp_off[m];
+        int bad = 0;
+        for (int k = threadIdx.x; k < nchk; k += blockDim.x) {
+            const int t = far ? k : sl[k];
+            V3 a0 = ld3(wv + 3 * ti[3 * t]), a1 = ld3(wv + 3 * ti[3 * t + 1]), a2 = ld3(wv + 3 * ti[3 * t + 2]);
+            V3 n; double d;
+            bad += plane_of(a0, a1, a2, n, d) ? 0 : 1;
+        }
+        for (int d = 32; d >= 1; d >>= 1) bad += __shfl_xor(bad, d, WAVE);
+        if (nchk > 0 && lane_id() == 0) atomicAdd(&s_bad, bad);
+        __syncthreads();
+        if (threadIdx.x == 0) B.nondeg[b] = T - s_bad;
+        __syncthreads();
     }
-    // degenerate flags on the world triangles (plane_from_tri, main.cpp:518-527)
-    const int T = M.ntris[m];
-    const int32_t* ti = M.tri_idx + 3 * (int64_t)M.tri_off[m];
-    uint8_t* dg = B.degen + B.wt_off[b];
-    int good = 0;
-    for (int t = threadIdx.x; t < T; t += blockDim.x) {
-        V3 a = ld3(wv + 3 * ti[3 * t]), c1 = ld3(wv + 3 * ti[3 * t + 1]), c2 = ld3(wv + 3 * ti[3 * t + 2]);
-        V3 n; double d;
-        bool ok = plane_of(a, c1, c2, n, d);
-        dg[t] = ok ? 0 : 1;
-        good += ok ? 1 : 0;
-    }
-    for (int d = 32; d >= 1; d >>= 1) good += __shfl_xor(good, d, WAVE);
-    if (lane_id() == 0) atomicAdd(&s_cnt, good);
-    __syncthreads();
-    if (threadIdx.x == 0) B.nondeg[b] = s_cnt;
+    kt_end(kt);
 }
 
 // ------------------------------------------------------------ scan
@@ -227,7 +259,8 @@ void scan_i32(const int32_t* in, const int32_t* n_ptr, int32_t n_max, int32_t* t
 constexpr int NEIGH_CAP = 128;
 
 __global__ void __launch_bounds__(1024) k_bp_bounds(const double* state, const double* radius, int n, DGrid* g,
-                                                    int max_cells, int32_t* status) {
+                                                    int max_cells, int32_t* status, unsigned long long* kt) {
+    kt_start(kt);
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY}, rm = 0.0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const double* p = state + 16 * (int64_t)i;
@@ -412,7 +445,8 @@ __global__ void __launch_bounds__(256) k_bp_adjfix(BroadPhaseBufs bp) {
     }
 }
 
-__global__ void k_bp_totals(BroadPhaseBufs bp) {
+__global__ void k_bp_totals(BroadPhaseBufs bp, unsigned long long* kt) {
+    kt_end(kt);
     int np = bp.pair_off[bp.n], na = bp.adj_off[bp.n];
     // on overflow the pair and adjacency lists were not written: later stages
     // see no pairs (the fold checks adj_off[n] against the capacity) and the
@@ -424,6 +458,41 @@ __global__ void k_bp_totals(BroadPhaseBufs bp) {
 }
 
 __global__ void k_grid_cells(const DGrid* g, int32_t* ncells) { *ncells = g->ncells; }
+
+// Sphere test and triangle AABB test of the particle-wall path
+// (main.cpp:2318-2323, 2338-2353): true if any wall triangle of wall w needs
+// the plane section of particle i this step.
+__device__ inline bool wall_tri_candidate(const double* wt9, const V3& pmn, const V3& pmx);
+__device__ inline bool near_wall(const WallBufs& W, int w, const V3& pos, double r) {
+    V3 dp = sub(pos, v3(W.pos[3 * w], W.pos[3 * w + 1], W.pos[3 * w + 2]));
+    double d2 = dot(dp, dp);
+    double rs = r + W.radius[w];
+    if (d2 > rs * rs) return false;
+    V3 pmn = sub(pos, v3(r, r, r)), pmx = add(pos, v3(r, r, r));
+    const int t0 = W.tri_off[w], nt = W.ntri[w];
+    for (int t = 0; t < nt; ++t)
+        if (wall_tri_candidate(W.tris + 9 * (int64_t)(t0 + t), pmn, pmx)) return true;
+    return false;
+}
+
+__global__ void __launch_bounds__(256) k_need(int32_t n, const double* state, const double* radius,
+                                              const int32_t* cnt_all, WallBufs W, int32_t* flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bool need = cnt_all[i] > 0;
+    if (!need && W.n_walls > 0) {
+        const double* p = state + 16 * (int64_t)i;
+        V3 pos = v3(p[0], p[1], p[2]);
+        for (int w = 0; w < W.n_walls && !need; ++w) need = near_wall(W, w, pos, radius[i]);
+    }
+    flag[i] = need ? 1 : 0;
+}
+__global__ void __launch_bounds__(256) k_need_list(int32_t n, const int32_t* flag, const int32_t* off, int32_t* list,
+                                                   int32_t* n_list) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) list[off[i]] = i;
+    if (i == 0) *n_list = off[n];
+}
 
 // ------------------------------------------------------------ narrow phase
 // outward-rounded float AABB of a double box
@@ -447,7 +516,8 @@ __device__ inline void load_tri(const double* wv, const int32_t* ti, int t, V3* 
 template <int BS, int K>
 __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, int32_t* list_i, float* list_f,
                                                double* stage, int32_t stage_cap, int32_t tmax, int32_t use_queue,
-                                               unsigned int* queue) {
+                                               unsigned int* queue, unsigned long long* kt) {
+    kt_start(kt);
     __shared__ int s_wave[BS / WAVE];
     __shared__ int s_item;
     __shared__ int s_ha[BS * K];
@@ -477,8 +547,6 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
         const int32_t* tiB = M.tri_idx + 3 * (int64_t)M.tri_off[mB];
         const double* wvA = B.wv + 3 * B.wv_off[i];
         const double* wvB = B.wv + 3 * B.wv_off[j];
-        const uint8_t* dgA = B.degen + B.wt_off[i];
-        const uint8_t* dgB = B.degen + B.wt_off[j];
         float boxA[6], boxB[6];
         box_f(B.aabb + 6 * (int64_t)i, B.aabb + 6 * (int64_t)i + 3, boxA);
         box_f(B.aabb + 6 * (int64_t)j, B.aabb + 6 * (int64_t)j + 3, boxB);
@@ -488,7 +556,7 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
             int t = base + tid;
             bool keep = false;
             float f[6];
-            if (t < TA && !dgA[t]) {
+            if (t < TA) {
                 V3 tr[3]; double lo[3], hi[3];
                 load_tri(wvA, tiA, t, tr);
                 tri_box(tr, lo, hi);
@@ -507,7 +575,7 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
             int t = base + tid;
             bool keep = false;
             float f[6];
-            if (t < TB && !dgB[t]) {
+            if (t < TB) {
                 V3 tr[3]; double lo[3], hi[3];
                 load_tri(wvB, tiB, t, tr);
                 tri_box(tr, lo, hi);
@@ -613,6 +681,7 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
         __syncthreads();
         if (!use_queue) item += gridDim.x;
     }
+    kt_end(kt);
 }
 
 // LPT order: pairs by descending cost bucket floor(log2(TA * TB)), so the
@@ -666,16 +735,42 @@ __device__ inline Mat load_mat(const DMat& m, int i) {
     return r;
 }
 
-__global__ void __launch_bounds__(256) k_topo_sim(DPairs P, DMeshes M, DBodies B, DMat mat, HistBufs H, TopoBufs T,
-                                                  SimParams sp) {
-    const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (tid >= T.nthreads) return;
+// One wave per pair (workgroup = 64 lanes); the pair's scratch lives in LDS
+// when it fits (dynamic shared memory of lds_bytes), otherwise in this
+// workgroup's slice of the global scratch.
+struct LaneDev {
+    static constexpr int W = WAVE;
+    __device__ int lane() const { return lane_id(); }
+    __device__ uint64_t ballot(bool p) const { return __ballot(p); }
+    __device__ void sync() const { __syncthreads(); }
+};
+
+__device__ inline void topo_scratch(const TopoBufs& T, int S, char* lds, int lds_bytes, TopoCaps& caps, SArr<int>& I,
+                                    SArr<double>& D) {
+    caps.S = S > 0 ? S : 1;
+    caps.E2 = 4 * caps.S;
+    const int64_t ni = caps.n_ints(), nd = caps.n_dbls();
+    const int64_t ibytes = (ni * 4 + 7) & ~(int64_t)7;
+    if (ibytes + nd * 8 <= lds_bytes) {
+        I.p = (int*)lds;
+        D.p = (double*)(lds + ibytes);
+    } else {
+        TopoCaps full; full.S = T.seg_max; full.E2 = 4 * T.seg_max;
+        I.p = T.scr_i + (int64_t)blockIdx.x * full.n_ints();
+        D.p = T.scr_d + (int64_t)blockIdx.x * full.n_dbls();
+    }
+    I.s = 1;
+    D.s = 1;
+}
+
+__global__ void __launch_bounds__(64) k_topo_sim(DPairs P, DMeshes M, DBodies B, DMat mat, HistBufs H, TopoBufs T,
+                                                 SimParams sp, int32_t lds_bytes, unsigned long long* kt) {
+    extern __shared__ char s_topo[];
+    kt_start(kt);
     const int n = *P.n;
-    TopoCaps caps; caps.S = T.seg_max; caps.E2 = 4 * T.seg_max;
-    SArr<int> I; I.p = T.scr_i + tid; I.s = T.nthreads;
-    SArr<double> D; D.p = T.scr_d + tid; D.s = T.nthreads;
     const int nprev = *H.prev_n;
-    for (int p = (int)tid; p < n; p += (int)T.nthreads) {
+    LaneDev wp;
+    for (int p = blockIdx.x; p < n; p += gridDim.x) {
         const int i = P.pi[p], j = P.pj[p];
         const int S = P.nseg[p];
         const int mA = B.mesh[i];
@@ -702,17 +797,26 @@ __global__ void __launch_bounds__(256) k_topo_sim(DPairs P, DMeshes M, DBodies B
         TopoOut to;
         to.comps = 0; to.loops_raw = 0; to.loops = 0; to.status = 0;
         if (S > T.seg_max) to.status = TOPO_E2_OVERFLOW;
-        else if (S > 0)
-            to = topology(P.arena + 6 * P.seg_off[p], S, M.tol[mA], M.area_eps[mA], sp.split != 0, sub(b.pos, a.pos),
-                          I, D, caps, sink);
-        T.ncontrib[p] = to.loops < T.lmax ? to.loops : T.lmax;
-        T.comps[p] = to.comps;
-        T.loops_raw[p] = to.loops_raw;
-        T.tstatus[p] = to.status;
-        H.cur_ds[3 * (int64_t)p] = ds.x; H.cur_ds[3 * (int64_t)p + 1] = ds.y; H.cur_ds[3 * (int64_t)p + 2] = ds.z;
-        H.cur_active[p] = to.loops > 0 ? 1 : 0;
-        if (to.status) atomicOr(P.status, (int)ST_TOPO);
+        else if (S > 0) {
+            TopoCaps caps;
+            SArr<int> I;
+            SArr<double> D;
+            topo_scratch(T, S, s_topo, lds_bytes, caps, I, D);
+            to = topology(wp, P.arena + 6 * P.seg_off[p], S, M.tol[mA], M.area_eps[mA], sp.split != 0,
+                          sub(b.pos, a.pos), I, D, caps, sink);
+        }
+        if (lane_id() == 0) {
+            T.ncontrib[p] = to.loops < T.lmax ? to.loops : T.lmax;
+            T.comps[p] = to.comps;
+            T.loops_raw[p] = to.loops_raw;
+            T.tstatus[p] = to.status;
+            H.cur_ds[3 * (int64_t)p] = ds.x; H.cur_ds[3 * (int64_t)p + 1] = ds.y; H.cur_ds[3 * (int64_t)p + 2] = ds.z;
+            H.cur_active[p] = to.loops > 0 ? 1 : 0;
+            if (to.status) atomicOr(P.status, (int)ST_TOPO);
+        }
+        __syncthreads();
     }
+    kt_end(kt);
 }
 
 struct ApiSink {
@@ -750,14 +854,12 @@ struct ApiSink {
     static constexpr int ZDX_LOOP_REC_D = 28;
 };
 
-__global__ void __launch_bounds__(256) k_topo_api(DPairs P, DMeshes M, DBodies B, TopoBufs T, int32_t split) {
-    const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (tid >= T.nthreads) return;
+__global__ void __launch_bounds__(64) k_topo_api(DPairs P, DMeshes M, DBodies B, TopoBufs T, int32_t split,
+                                                 int32_t lds_bytes) {
+    extern __shared__ char s_topo[];
     const int n = *P.n;
-    TopoCaps caps; caps.S = T.seg_max; caps.E2 = 4 * T.seg_max;
-    SArr<int> I; I.p = T.scr_i + tid; I.s = T.nthreads;
-    SArr<double> D; D.p = T.scr_d + tid; D.s = T.nthreads;
-    for (int p = (int)tid; p < n; p += (int)T.nthreads) {
+    LaneDev wp;
+    for (int p = blockIdx.x; p < n; p += gridDim.x) {
         const int i = P.pi[p], j = P.pj[p];
         const int S = P.nseg[p];
         const int mA = B.mesh[i];
@@ -767,25 +869,43 @@ __global__ void __launch_bounds__(256) k_topo_api(DPairs P, DMeshes M, DBodies B
         TopoOut to;
         to.comps = 0; to.loops_raw = 0; to.loops = 0; to.status = 0;
         if (S > T.seg_max) to.status = TOPO_E2_OVERFLOW;
-        else if (S > 0)
-            to = topology(P.arena + 6 * P.seg_off[p], S, M.tol[mA], M.area_eps[mA], split != 0, sub(pb, pa), I, D,
-                          caps, sink);
-        T.ncontrib[p] = to.loops;
-        T.comps[p] = to.comps;
-        T.loops_raw[p] = to.loops_raw;
-        T.tstatus[p] = to.status;
-        if (to.status) atomicOr(P.status, (int)ST_TOPO);
+        else if (S > 0) {
+            TopoCaps caps;
+            SArr<int> I;
+            SArr<double> D;
+            topo_scratch(T, S, s_topo, lds_bytes, caps, I, D);
+            to = topology(wp, P.arena + 6 * P.seg_off[p], S, M.tol[mA], M.area_eps[mA], split != 0, sub(pb, pa), I,
+                          D, caps, sink);
+        }
+        if (lane_id() == 0) {
+            T.ncontrib[p] = to.loops;
+            T.comps[p] = to.comps;
+            T.loops_raw[p] = to.loops_raw;
+            T.tstatus[p] = to.status;
+            if (to.status) atomicOr(P.status, (int)ST_TOPO);
+        }
+        __syncthreads();
     }
 }
 
 // ------------------------------------------------------------ walls
+__device__ inline bool wall_tri_candidate(const double* wt9, const V3& pmn, const V3& pmx) {
+    V3 wt[3] = {ld3(wt9), ld3(wt9 + 3), ld3(wt9 + 6)};
+    double lo[3], hi[3];
+    tri_box(wt, lo, hi);
+    return !(pmx.x < lo[0] || pmn.x > hi[0] || pmx.y < lo[1] || pmn.y > hi[1] || pmx.z < lo[2] || pmn.z > hi[2]);
+}
+
 __global__ void __launch_bounds__(256) k_walls(DBodies B, DMeshes M, DMat mat, WallBufs W, HistBufs H, double* wcontrib,
                                                int32_t* wnc, double* scr, int64_t nthreads, int32_t pts_cap,
-                                               SimParams sp, int32_t* status) {
+                                               SimParams sp, int32_t* status, unsigned long long* kt) {
+    kt_start(kt);
     const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (tid >= nthreads) return;
+    if (tid >= nthreads) { kt_end(kt); return; }
     const int64_t total = (int64_t)B.n * W.n_walls;
-    SArr<double> PTS; PTS.p = scr + tid; PTS.s = nthreads;
+    // contiguous per-thread scratch: section points, then the two clip polygons
+    const int64_t per = 3 * (int64_t)pts_cap + 4 * (int64_t)(pts_cap + 8);
+    SArr<double> PTS; PTS.p = scr + tid * per; PTS.s = 1;
     SArr<double> PA = PTS.at(3 * (int64_t)pts_cap);
     SArr<double> PB = PA.at(2 * (int64_t)(pts_cap + 8));
     for (int64_t idx = tid; idx < total; idx += nthreads) {
@@ -809,11 +929,8 @@ __global__ void __launch_bounds__(256) k_walls(DBodies B, DMeshes M, DMat mat, W
             const int t0 = W.tri_off[w], nt = W.ntri[w];
             for (int t = 0; t < nt; ++t) {
                 const double* wt9 = W.tris + 9 * (int64_t)(t0 + t);
+                if (!wall_tri_candidate(wt9, pmn, pmx)) continue;
                 V3 wt[3] = {ld3(wt9), ld3(wt9 + 3), ld3(wt9 + 6)};
-                double lo[3], hi[3];
-                tri_box(wt, lo, hi);
-                if (pmx.x < lo[0] || pmn.x > hi[0] || pmx.y < lo[1] || pmn.y > hi[1] || pmx.z < lo[2] || pmn.z > hi[2])
-                    continue;
                 double area;
                 V3 xc, cn;
                 int st = 0;
@@ -834,6 +951,7 @@ __global__ void __launch_bounds__(256) k_walls(DBodies B, DMeshes M, DMat mat, W
         H.wcur_ds[3 * idx] = ds.x; H.wcur_ds[3 * idx + 1] = ds.y; H.wcur_ds[3 * idx + 2] = ds.z;
         H.wcur_active[idx] = nc > 0 ? 1 : 0;
     }
+    kt_end(kt);
 }
 
 // ------------------------------------------------------------ fold + integrate
@@ -842,9 +960,11 @@ __global__ void __launch_bounds__(256) k_fold_integrate(int32_t n, const double*
                                                         const int32_t* adj_pair, int32_t adj_cap, TopoBufs T,
                                                         int32_t n_walls,
                                                         const double* wcontrib, const int32_t* wnc, int32_t mwt,
-                                                        double* Fo, double* To, int32_t* counts, SimParams sp) {
+                                                        double* Fo, double* To, int32_t* counts, SimParams sp,
+                                                        unsigned long long* kt) {
+    kt_start(kt);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n) { kt_end(kt); return; }
     // forces[i] and torques[i] start at zero and receive, in the reference's
     // order (main.cpp:2049-2050 pair loop, then walls 2310-2447): pairs with
     // partners in ascending index, loops in order, then walls and their tris
@@ -890,6 +1010,7 @@ __global__ void __launch_bounds__(256) k_fold_integrate(int32_t n, const double*
     for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) Ib.m[r][c] = mat.Ibinv[9 * (int64_t)i + 3 * r + c];
     integrate(b, Ib, mat.inv_mass[i], F, Tq, v3(sp.g[0], sp.g[1], sp.g[2]), sp.dt);
     store_body(sout + 16 * (int64_t)i, b);
+    kt_end(kt);
 }
 
 // ------------------------------------------------------------ counters
@@ -936,15 +1057,29 @@ inline int nblk(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 }  // namespace
 
 // ------------------------------------------------------------ launchers
-void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s) {
-    if (B.n <= 0) return;
-    hipLaunchKernelGGL(k_transform, dim3(B.n), dim3(256), 0, s, B, M);
+void launch_kt_reset(unsigned long long* kt, int n, hipStream_t s) {
+    if (kt) hipLaunchKernelGGL(k_kt_reset, dim3(1), dim3(64), 0, s, kt, n);
 }
 
-void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s) {
+void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s, unsigned long long* kt, const int32_t* list,
+                      const int32_t* n_list, int32_t groups) {
+    if (B.n <= 0) return;
+    int g = list ? groups : B.n;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_transform, dim3(g), dim3(256), 0, s, B, M, list, n_list, kt);
+}
+
+void launch_need_list(int32_t n, const double* state, const double* radius, const int32_t* cnt_all, const WallBufs& W,
+                      int32_t* flag, int32_t* off, int32_t* scan_tmp, int32_t* list, int32_t* n_list, hipStream_t s) {
+    hipLaunchKernelGGL(k_need, dim3(nblk(n, 256)), dim3(256), 0, s, n, state, radius, cnt_all, W, flag);
+    scan_i32(flag, nullptr, n, scan_tmp, off, nullptr, s);
+    hipLaunchKernelGGL(k_need_list, dim3(nblk(n, 256)), dim3(256), 0, s, n, flag, off, list, n_list);
+}
+
+void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s, unsigned long long* kt) {
     const int n = bp.n;
     hipLaunchKernelGGL(k_bp_bounds, dim3(1), dim3(1024), 0, s, bp.state, bp.radius, n, bp.grid, bp.max_cells,
-                       bp.status);
+                       bp.status, kt);
     // cell counters for the grid actually in use
     int32_t* ncells = bp.n_adj;  // reuse as a scalar before it is written below
     hipLaunchKernelGGL(k_grid_cells, dim3(1), dim3(1), 0, s, bp.grid, ncells);
@@ -958,29 +1093,29 @@ void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s) {
     scan_i32(bp.cnt_all, nullptr, n, bp.scan_tmp, bp.adj_off, nullptr, s);
     hipLaunchKernelGGL(k_bp_fill, dim3(nblk(n, 128)), dim3(128), 0, s, bp);
     hipLaunchKernelGGL(k_bp_adjfix, dim3(nblk(n, 256)), dim3(256), 0, s, bp);
-    hipLaunchKernelGGL(k_bp_totals, dim3(1), dim3(1), 0, s, bp);
+    hipLaunchKernelGGL(k_bp_totals, dim3(1), dim3(64), 0, s, bp, kt);
 }
 
 template <int BS, int K>
 static void narrow_t(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
-                     double* st, int32_t sc, int32_t tmax, unsigned int* q, hipStream_t s) {
+                     double* st, int32_t sc, int32_t tmax, unsigned int* q, hipStream_t s, unsigned long long* kt) {
     hipLaunchKernelGGL((k_narrow<BS, K>), dim3(cfg.groups), dim3(BS), 0, s, B, M, P, li, lf, st, sc, tmax,
-                       cfg.use_queue, q);
+                       cfg.use_queue, q, kt);
 }
 
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
-                   double* st, int32_t sc, int32_t tmax, unsigned int* q, hipStream_t s) {
+                   double* st, int32_t sc, int32_t tmax, unsigned int* q, hipStream_t s, unsigned long long* kt) {
     const int key = cfg.block * 100 + cfg.items;
     switch (key) {
-        case 12804: narrow_t<128, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s); break;
-        case 12808: narrow_t<128, 8>(B, M, P, cfg, li, lf, st, sc, tmax, q, s); break;
-        case 25601: narrow_t<256, 1>(B, M, P, cfg, li, lf, st, sc, tmax, q, s); break;
-        case 25602: narrow_t<256, 2>(B, M, P, cfg, li, lf, st, sc, tmax, q, s); break;
-        case 25604: narrow_t<256, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s); break;
-        case 25608: narrow_t<256, 8>(B, M, P, cfg, li, lf, st, sc, tmax, q, s); break;
-        case 51202: narrow_t<512, 2>(B, M, P, cfg, li, lf, st, sc, tmax, q, s); break;
-        case 51204: narrow_t<512, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s); break;
-        default: narrow_t<256, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s); break;
+        case 12804: narrow_t<128, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
+        case 12808: narrow_t<128, 8>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
+        case 25601: narrow_t<256, 1>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
+        case 25602: narrow_t<256, 2>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
+        case 25604: narrow_t<256, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
+        case 25608: narrow_t<256, 8>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
+        case 51202: narrow_t<512, 2>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
+        case 51204: narrow_t<512, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
+        default: narrow_t<256, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
     }
 }
 
@@ -993,27 +1128,29 @@ void launch_lpt_order(const DPairs& P, const DMeshes& M, const int32_t* body_mes
 }
 
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
-                     const TopoBufs& T, const SimParams& sp, hipStream_t s) {
-    hipLaunchKernelGGL(k_topo_sim, dim3(nblk(T.nthreads, 256)), dim3(256), 0, s, P, M, B, mat, H, T, sp);
+                     const TopoBufs& T, const SimParams& sp, hipStream_t s, unsigned long long* kt) {
+    hipLaunchKernelGGL(k_topo_sim, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, mat, H, T, sp,
+                       T.lds_bytes, kt);
 }
 void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int32_t split,
                      hipStream_t s) {
-    hipLaunchKernelGGL(k_topo_api, dim3(nblk(T.nthreads, 256)), dim3(256), 0, s, P, M, B, T, split);
+    hipLaunchKernelGGL(k_topo_api, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, T, split,
+                       T.lds_bytes);
 }
 void launch_walls(const DBodies& B, const DMeshes& M, const DMat& mat, const WallBufs& W, const HistBufs& H,
                   double* wcontrib, int32_t* wnc, double* scr, int64_t nthreads, int32_t pts_cap, const SimParams& sp,
-                  int32_t* status, hipStream_t s) {
+                  int32_t* status, hipStream_t s, unsigned long long* kt) {
     if (W.n_walls <= 0) return;
     hipLaunchKernelGGL(k_walls, dim3(nblk(nthreads, 256)), dim3(256), 0, s, B, M, mat, W, H, wcontrib, wnc, scr,
-                       nthreads, pts_cap, sp, status);
+                       nthreads, pts_cap, sp, status, kt);
 }
 void launch_fold_integrate(int32_t n, const double* sin, double* sout, const DMat& mat, const int32_t* adj_off,
                            const int32_t* adj_j, const int32_t* adj_pair, int32_t adj_cap, const TopoBufs& T,
                            int32_t n_walls,
                            const double* wcontrib, const int32_t* wnc, int32_t mwt, double* F, double* Tq,
-                           int32_t* counts, const SimParams& sp, hipStream_t s) {
+                           int32_t* counts, const SimParams& sp, hipStream_t s, unsigned long long* kt) {
     hipLaunchKernelGGL(k_fold_integrate, dim3(nblk(n, 256)), dim3(256), 0, s, n, sin, sout, mat, adj_off, adj_j,
-                       adj_pair, adj_cap, T, n_walls, wcontrib, wnc, mwt, F, Tq, counts, sp);
+                       adj_pair, adj_cap, T, n_walls, wcontrib, wnc, mwt, F, Tq, counts, sp, kt);
 }
 void launch_counters(const DPairs& P, const TopoBufs& T, int32_t n_pw, const int32_t* wnc, unsigned long long* ctr,
                      hipStream_t s) {
