@@ -211,6 +211,7 @@ struct zdx_sim {
     DBuf<double> d_wv, d_aabb;
     DBuf<int32_t> d_nondeg;
     DBuf<int32_t> d_need_flag, d_need_off, d_need_list, d_n_need;
+    DBuf<int32_t> d_wall_flag, d_wall_off, d_wall_list, d_n_wall;
     DBuf<double> d_state[2];
     DBuf<double> d_F, d_T;
     DBuf<int32_t> d_counts;
@@ -445,6 +446,7 @@ void sim_setup_device(zdx_sim* s) {
     s->d_wv.ensure(3 * (size_t)av); s->d_aabb.ensure(6 * (size_t)N);
     s->d_nondeg.ensure(N);
     s->d_need_flag.ensure(N); s->d_need_off.ensure(N + 1); s->d_need_list.ensure(N); s->d_n_need.ensure(1);
+    s->d_wall_flag.ensure(N); s->d_wall_off.ensure(N + 1); s->d_wall_list.ensure(N); s->d_n_wall.ensure(1);
     s->d_state[0].ensure(16 * (size_t)N); s->d_state[1].ensure(16 * (size_t)N);
     s->d_F.ensure(3 * (size_t)N); s->d_T.ensure(3 * (size_t)N); s->d_counts.ensure(N);
     // broad phase
@@ -517,7 +519,7 @@ void sim_setup_walls(zdx_sim* s, const std::vector<WallHost>& walls) {
         HIPC(hipMemsetAsync(s->ps[k].wactive.p, 0, npw, st));
     }
     s->wall_pts_cap = std::max<int32_t>(64, s->ctx->max_tris);
-    s->wall_threads = std::min<int64_t>(8192, std::max<int64_t>(256, ((int64_t)npw + 255) / 256 * 256));
+    s->wall_threads = std::min<int64_t>(2048, std::max<int64_t>(1, (int64_t)s->N));   // one-wave workgroups
     s->d_wscr.ensure((size_t)s->wall_threads * (size_t)(3 * s->wall_pts_cap + 4 * (s->wall_pts_cap + 8)));
     HIPC(hipStreamSynchronize(st));
 }
@@ -556,7 +558,8 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     launch_kt_reset(kt, 6, st);
     launch_broadphase(s->bpbufs(par), st, kt + 2);
     launch_need_list(s->N, s->d_state[par].p, s->d_radius.p, s->d_cnt_all.p, s->wallbufs(), s->d_need_flag.p,
-                     s->d_need_off.p, s->d_scan_tmp.p, s->d_need_list.p, s->d_n_need.p, st);
+                     s->d_need_off.p, s->d_scan_tmp.p, s->d_need_list.p, s->d_n_need.p, s->d_wall_flag.p,
+                     s->d_wall_off.p, s->d_wall_list.p, s->d_n_wall.p, st);
     launch_transform(B, M, st, kt + 0, s->d_need_list.p, s->d_n_need.p, s->transform_groups());
     DPairs P = s->dpairs(par, batched);
     if (batched) launch_lpt_order(P, M, s->d_mesh.p, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st);
@@ -571,8 +574,8 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     HistBufs H = s->hist(par);
     launch_topo_sim(P, M, B, mat, H, T, s->sp, st, kt + 6);
     if (s->W > 0)
-        launch_walls(B, M, mat, s->wallbufs(), H, s->d_wcontrib.p, s->d_wnc.p, s->d_wscr.p, s->wall_threads,
-                     s->wall_pts_cap, s->sp, s->d_status.p, st, kt + 8);
+        launch_walls(B, M, mat, s->wallbufs(), H, s->d_wcontrib.p, s->d_wnc.p, s->d_wall_list.p, s->d_n_wall.p,
+                     s->d_wscr.p, s->wall_threads, s->wall_pts_cap, s->sp, s->d_status.p, st, kt + 8);
     launch_fold_integrate(s->N, s->d_state[par].p, s->d_state[nxt].p, mat, s->d_adj_off.p, s->d_adj_j.p,
                           s->d_adj_pair.p, s->adj_cap, T, s->W, s->d_wcontrib.p, s->d_wnc.p, s->max_wall_tris,
                           s->d_F.p, s->d_T.p, s->d_counts.p, s->sp, st, kt + 10);
@@ -1254,7 +1257,8 @@ zdx_status zdx_sim_autotune(zdx_sim* s, int32_t* out_block, int32_t* out_groups)
             launch_broadphase(s->bpbufs(par), st);
         }
         launch_need_list(s->N, s->d_state[par].p, s->d_radius.p, s->d_cnt_all.p, s->wallbufs(), s->d_need_flag.p,
-                         s->d_need_off.p, s->d_scan_tmp.p, s->d_need_list.p, s->d_n_need.p, st);
+                         s->d_need_off.p, s->d_scan_tmp.p, s->d_need_list.p, s->d_n_need.p, s->d_wall_flag.p,
+                     s->d_wall_off.p, s->d_wall_list.p, s->d_n_wall.p, st);
         launch_transform(B, M, st, nullptr, s->d_need_list.p, s->d_n_need.p, s->transform_groups());
         DPairs P = s->dpairs(par, true);
         launch_lpt_order(P, M, s->d_mesh.p, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st);

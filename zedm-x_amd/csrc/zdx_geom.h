@@ -357,6 +357,7 @@ struct LaneHost {
     static constexpr int W = 1;
     int lane() const { return 0; }
     uint64_t ballot(bool p) const { return p ? 1ull : 0ull; }
+    double shfl(double v, int) const { return v; }
     void sync() const {}
 };
 
@@ -752,44 +753,62 @@ ZDX_HD inline void basis(const V3& n, V3& u, V3& v) {
     v = cross(n, u);
 }
 
-// compute_wall_contact main.cpp:1205-1258 for one wall triangle, with the
-// particle's world vertices already computed (the reference re-transforms the
-// mesh per wall triangle; quat_rotate is a pure function of the vertex, so the
-// cached world vertices are the identical doubles).
+// compute_wall_contact main.cpp:1205-1258 for one wall triangle, run by the
+// W lanes of a lane policy: the particle's world vertices are already
+// computed (the reference re-transforms the mesh per wall triangle;
+// quat_rotate is a pure function of the vertex, so the cached world vertices
+// are the identical doubles).  The plane section (get_particle_plane_section
+// 1172-1201) tests the (triangle, edge) slots lane-parallel; crossings are
+// taken in slot order from the ballot mask and kept unless an earlier kept
+// point lies within tol (unique_points 561-577), the kept-list check again
+// lane-parallel.  Clip, area and centroid run redundantly on every lane.
 // wv: world vertex array (3 doubles each); tri: vertex indices (3 per tri).
-// Scratch: pts (3*3*T doubles), poly buffers A/B (2*(3*T+8) doubles each).
-ZDX_HD inline bool wall_contact(const double* wv, const int* tri, int T, const V3* wt, const V3& wn, double tol,
-                                SArr<double> pts, SArr<double> PA, SArr<double> PB, int cap_pts,
+// Scratch (shared by the lanes): pts (3 cap_pts doubles), poly buffers A/B
+// (2 (cap_pts + 8) doubles each).
+template <class WP>
+ZDX_HD inline bool wall_contact(const WP& wp, const double* wv, const int* tri, int T, const V3* wt, const V3& wn,
+                                double tol, SArr<double> pts, SArr<double> PA, SArr<double> PB, int cap_pts,
                                 double& area_out, V3& xc_out, V3& n_out, int& status) {
+    const int W = WP::W;
+    const int ln = wp.lane();
     V3 pn = wn;
     double pd = -dot(pn, wt[0]);
-    // get_particle_plane_section 1172-1201: crossings in (tri, edge) order,
-    // then unique_points (keep first within tol)
     const double t2 = tol * tol;
     int np = 0;
-    for (int t = 0; t < T; ++t) {
-        V3 q[3];
-        for (int k = 0; k < 3; ++k) {
-            int vi = tri[3 * t + k];
-            q[k] = v3(wv[3 * vi], wv[3 * vi + 1], wv[3 * vi + 2]);
-        }
-        for (int i = 0; i < 3; ++i) {
-            V3 p0 = q[i], p1 = q[(i + 1) % 3];
+    const int nslots = 3 * T;
+    for (int base = 0; base < nslots; base += W) {
+        const int f = base + ln;
+        bool hit = false;
+        V3 ip = v3(0.0, 0.0, 0.0);
+        if (f < nslots) {
+            const int t = f / 3, i = f - 3 * t;
+            const int i0 = tri[3 * t + i], i1 = tri[3 * t + (i + 1) % 3];
+            V3 p0 = v3(wv[3 * i0], wv[3 * i0 + 1], wv[3 * i0 + 2]);
+            V3 p1 = v3(wv[3 * i1], wv[3 * i1 + 1], wv[3 * i1 + 2]);
             double d0 = dot(pn, p0) + pd, d1 = dot(pn, p1) + pd;
             if (d0 * d1 < 0.0) {
                 double tt = d0 / (d0 - d1);
                 tt = dmin(1.0, dmax(0.0, tt));
-                V3 ip = add(p0, mul(sub(p1, p0), tt));
-                bool keep = true;
-                for (int m = 0; m < np; ++m) {
-                    V3 qq = v3(pts[3 * m], pts[3 * m + 1], pts[3 * m + 2]);
-                    if (norm2(sub(ip, qq)) <= t2) { keep = false; break; }
-                }
-                if (keep) {
-                    if (np >= cap_pts) { status = 1; return false; }
-                    pts[3 * np] = ip.x; pts[3 * np + 1] = ip.y; pts[3 * np + 2] = ip.z;
-                    ++np;
-                }
+                ip = add(p0, mul(sub(p1, p0), tt));
+                hit = true;
+            }
+        }
+        uint64_t m = wp.ballot(hit);
+        while (m) {
+            const int src = ctz64(m);
+            m &= m - 1;
+            const V3 q = v3(wp.shfl(ip.x, src), wp.shfl(ip.y, src), wp.shfl(ip.z, src));
+            bool dup = false;
+            for (int kb = 0; kb < np && !dup; kb += W) {
+                const int k = kb + ln;
+                bool near = false;
+                if (k < np) near = norm2(sub(q, v3(pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]))) <= t2;
+                dup = wp.ballot(near) != 0ull;
+            }
+            if (!dup) {
+                if (np >= cap_pts) { status = 1; return false; }
+                pts[3 * np] = q.x; pts[3 * np + 1] = q.y; pts[3 * np + 2] = q.z;
+                ++np;
             }
         }
     }
@@ -812,23 +831,23 @@ ZDX_HD inline bool wall_contact(const double* wv, const int* tri, int T, const V
         V2 c2 = clip[ck];
         if (nin == 0) break;
         int nout = 0;
-        V2 s = v2(in[2 * (nin - 1)], in[2 * (nin - 1) + 1]);
+        V2 sv = v2(in[2 * (nin - 1)], in[2 * (nin - 1) + 1]);
         for (int m = 0; m < nin; ++m) {
             V2 e = v2(in[2 * m], in[2 * m + 1]);
             if (inside_clip(e, c1, c2)) {
-                if (!inside_clip(s, c1, c2)) {
-                    V2 x = isect_clip(s, e, c1, c2);
+                if (!inside_clip(sv, c1, c2)) {
+                    V2 x = isect_clip(sv, e, c1, c2);
                     if (nout >= cap_pts + 6) { status = 1; return false; }
                     outp[2 * nout] = x.x; outp[2 * nout + 1] = x.y; ++nout;
                 }
                 if (nout >= cap_pts + 6) { status = 1; return false; }
                 outp[2 * nout] = e.x; outp[2 * nout + 1] = e.y; ++nout;
-            } else if (inside_clip(s, c1, c2)) {
-                V2 x = isect_clip(s, e, c1, c2);
+            } else if (inside_clip(sv, c1, c2)) {
+                V2 x = isect_clip(sv, e, c1, c2);
                 if (nout >= cap_pts + 6) { status = 1; return false; }
                 outp[2 * nout] = x.x; outp[2 * nout + 1] = x.y; ++nout;
             }
-            s = e;
+            sv = e;
         }
         SArr<double> tmp = in; in = outp; outp = tmp;
         nin = nout;

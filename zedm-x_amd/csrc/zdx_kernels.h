@@ -106,8 +106,10 @@ void launch_kt_reset(unsigned long long* kt, int n_stages, hipStream_t s);
 void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s, unsigned long long* kt = nullptr,
                       const int32_t* list = nullptr, const int32_t* n_list = nullptr, int32_t groups = 0);
 // bodies that are in a candidate pair or near a wall triangle -> list
+// (and the particles near a wall triangle -> wlist)
 void launch_need_list(int32_t n, const double* state, const double* radius, const int32_t* cnt_all, const WallBufs& W,
-                      int32_t* flag, int32_t* off, int32_t* scan_tmp, int32_t* list, int32_t* n_list, hipStream_t s);
+                      int32_t* flag, int32_t* off, int32_t* scan_tmp, int32_t* list, int32_t* n_list, int32_t* wflag,
+                      int32_t* woff, int32_t* wlist, int32_t* n_wlist, hipStream_t s);
 void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s, unsigned long long* kt = nullptr);
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* list_i,
                    float* list_f, double* stage, int32_t stage_cap, int32_t tmax, unsigned int* queue,
@@ -118,9 +120,12 @@ void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const 
                      const TopoBufs& T, const SimParams& sp, hipStream_t s, unsigned long long* kt = nullptr);
 void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int32_t split,
                      hipStream_t s);
+// one wave per particle of `list` (particles near a wall triangle); scr holds
+// `groups` slices of 3 pts_cap + 4 (pts_cap + 8) doubles
 void launch_walls(const DBodies& B, const DMeshes& M, const DMat& mat, const WallBufs& W, const HistBufs& H,
-                  double* wcontrib, int32_t* wncontrib, double* scr, int64_t nthreads, int32_t pts_cap,
-                  const SimParams& sp, int32_t* status, hipStream_t s, unsigned long long* kt = nullptr);
+                  double* wcontrib, int32_t* wncontrib, const int32_t* list, const int32_t* n_list, double* scr,
+                  int64_t groups, int32_t pts_cap, const SimParams& sp, int32_t* status, hipStream_t s,
+                  unsigned long long* kt = nullptr);
 void launch_fold_integrate(int32_t n, const double* state_in, double* state_out, const DMat& mat,
                            const int32_t* adj_off, const int32_t* adj_j, const int32_t* adj_pair,
                            int32_t adj_cap, const TopoBufs& T, int32_t n_walls, const double* wcontrib, const int32_t* wncontrib,

@@ -476,16 +476,17 @@ __device__ inline bool near_wall(const WallBufs& W, int w, const V3& pos, double
 }
 
 __global__ void __launch_bounds__(256) k_need(int32_t n, const double* state, const double* radius,
-                                              const int32_t* cnt_all, WallBufs W, int32_t* flag) {
+                                              const int32_t* cnt_all, WallBufs W, int32_t* flag, int32_t* wflag) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    bool need = cnt_all[i] > 0;
-    if (!need && W.n_walls > 0) {
+    bool wall = false;
+    if (W.n_walls > 0) {
         const double* p = state + 16 * (int64_t)i;
         V3 pos = v3(p[0], p[1], p[2]);
-        for (int w = 0; w < W.n_walls && !need; ++w) need = near_wall(W, w, pos, radius[i]);
+        for (int w = 0; w < W.n_walls && !wall; ++w) wall = near_wall(W, w, pos, radius[i]);
     }
-    flag[i] = need ? 1 : 0;
+    flag[i] = (cnt_all[i] > 0 || wall) ? 1 : 0;
+    wflag[i] = wall ? 1 : 0;
 }
 __global__ void __launch_bounds__(256) k_need_list(int32_t n, const int32_t* flag, const int32_t* off, int32_t* list,
                                                    int32_t* n_list) {
@@ -742,6 +743,7 @@ struct LaneDev {
     static constexpr int W = WAVE;
     __device__ int lane() const { return lane_id(); }
     __device__ uint64_t ballot(bool p) const { return __ballot(p); }
+    __device__ double shfl(double v, int src) const { return __shfl(v, src, WAVE); }
     __device__ void sync() const { __syncthreads(); }
 };
 
@@ -896,62 +898,76 @@ __device__ inline bool wall_tri_candidate(const double* wt9, const V3& pmn, cons
     return !(pmx.x < lo[0] || pmn.x > hi[0] || pmx.y < lo[1] || pmn.y > hi[1] || pmx.z < lo[2] || pmn.z > hi[2]);
 }
 
-__global__ void __launch_bounds__(256) k_walls(DBodies B, DMeshes M, DMat mat, WallBufs W, HistBufs H, double* wcontrib,
-                                               int32_t* wnc, double* scr, int64_t nthreads, int32_t pts_cap,
-                                               SimParams sp, int32_t* status, unsigned long long* kt) {
+// One wave per listed particle (a particle near at least one wall triangle).
+// Entries of particles not in the list were zeroed before this kernel.
+__global__ void __launch_bounds__(64) k_walls(DBodies B, DMeshes M, DMat mat, WallBufs W, HistBufs H, double* wcontrib,
+                                              int32_t* wnc, const int32_t* list, const int32_t* n_list, double* scr,
+                                              int32_t pts_cap, SimParams sp, int32_t* status, unsigned long long* kt) {
     kt_start(kt);
-    const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (tid >= nthreads) { kt_end(kt); return; }
-    const int64_t total = (int64_t)B.n * W.n_walls;
-    // contiguous per-thread scratch: section points, then the two clip polygons
+    LaneDev wp;
+    const int nl = *n_list;
     const int64_t per = 3 * (int64_t)pts_cap + 4 * (int64_t)(pts_cap + 8);
-    SArr<double> PTS; PTS.p = scr + tid * per; PTS.s = 1;
+    SArr<double> PTS; PTS.p = scr + (int64_t)blockIdx.x * per; PTS.s = 1;
     SArr<double> PA = PTS.at(3 * (int64_t)pts_cap);
     SArr<double> PB = PA.at(2 * (int64_t)(pts_cap + 8));
-    for (int64_t idx = tid; idx < total; idx += nthreads) {
-        const int i = (int)(idx / W.n_walls), w = (int)(idx % W.n_walls);
+    for (int it = blockIdx.x; it < nl; it += gridDim.x) {
+        const int i = list[it];
         Body p = load_body(B.state + 16 * (int64_t)i);
         const double r = mat.radius[i];
-        // sphere test main.cpp:2318-2323
-        V3 dp = sub(p.pos, ld3(W.pos + 3 * w));
-        double d2 = dot(dp, dp);
-        double rs = r + W.radius[w];
-        int nc = 0;
-        V3 ds = H.wprev_active[idx] ? ld3(H.wprev_ds + 3 * idx) : v3(0.0, 0.0, 0.0);
-        if (!(d2 > rs * rs)) {
-            const int m = B.mesh[i];
-            const double tol = M.tol[m];
-            const double* wv = B.wv + 3 * B.wv_off[i];
-            const int32_t* ti = M.tri_idx + 3 * (int64_t)M.tri_off[m];
-            const int T = M.ntris[m];
-            Mat mt = load_mat(mat, i);
-            V3 pmn = sub(p.pos, v3(r, r, r)), pmx = add(p.pos, v3(r, r, r));
-            const int t0 = W.tri_off[w], nt = W.ntri[w];
-            for (int t = 0; t < nt; ++t) {
-                const double* wt9 = W.tris + 9 * (int64_t)(t0 + t);
-                if (!wall_tri_candidate(wt9, pmn, pmx)) continue;
-                V3 wt[3] = {ld3(wt9), ld3(wt9 + 3), ld3(wt9 + 6)};
-                double area;
-                V3 xc, cn;
-                int st = 0;
-                if (!wall_contact(wv, ti, T, wt, ld3(W.normals + 3 * (int64_t)(t0 + t)), tol, PTS, PA, PB, pts_cap,
-                                  area, xc, cn, st)) {
-                    if (st) atomicOr(status, (int)ST_WALL);
-                    continue;
+        const int m = B.mesh[i];
+        const double tol = M.tol[m];
+        const double* wv = B.wv + 3 * B.wv_off[i];
+        const int32_t* ti = M.tri_idx + 3 * (int64_t)M.tri_off[m];
+        const int T = M.ntris[m];
+        Mat mt = load_mat(mat, i);
+        V3 pmn = sub(p.pos, v3(r, r, r)), pmx = add(p.pos, v3(r, r, r));
+        for (int w = 0; w < W.n_walls; ++w) {
+            const int64_t idx = (int64_t)i * W.n_walls + w;
+            // sphere test main.cpp:2318-2323
+            V3 dp = sub(p.pos, ld3(W.pos + 3 * w));
+            double d2 = dot(dp, dp);
+            double rs = r + W.radius[w];
+            int nc = 0;
+            V3 ds = H.wprev_active[idx] ? ld3(H.wprev_ds + 3 * idx) : v3(0.0, 0.0, 0.0);
+            if (!(d2 > rs * rs)) {
+                const int t0 = W.tri_off[w], nt = W.ntri[w];
+                for (int t = 0; t < nt; ++t) {
+                    const double* wt9 = W.tris + 9 * (int64_t)(t0 + t);
+                    if (!wall_tri_candidate(wt9, pmn, pmx)) continue;
+                    V3 wt[3] = {ld3(wt9), ld3(wt9 + 3), ld3(wt9 + 6)};
+                    double area;
+                    V3 xc, cn;
+                    int st = 0;
+                    bool ok = wall_contact(wp, wv, ti, T, wt, ld3(W.normals + 3 * (int64_t)(t0 + t)), tol, PTS, PA,
+                                           PB, pts_cap, area, xc, cn, st);
+                    if (!ok) {
+                        if (st && lane_id() == 0) atomicOr(status, (int)ST_WALL);
+                        continue;
+                    }
+                    Contact c = pw_force(cn, area, xc, p, mt, W.mu[w], sp.dt, ds);
+                    if (nc < W.max_wall_tris && lane_id() == 0) {
+                        double* o = wcontrib + 6 * (idx * W.max_wall_tris + nc);
+                        o[0] = c.f.x; o[1] = c.f.y; o[2] = c.f.z; o[3] = c.r1.x; o[4] = c.r1.y; o[5] = c.r1.z;
+                    }
+                    ++nc;
                 }
-                Contact c = pw_force(cn, area, xc, p, mt, W.mu[w], sp.dt, ds);
-                if (nc < W.max_wall_tris) {
-                    double* o = wcontrib + 6 * (idx * W.max_wall_tris + nc);
-                    o[0] = c.f.x; o[1] = c.f.y; o[2] = c.f.z; o[3] = c.r1.x; o[4] = c.r1.y; o[5] = c.r1.z;
-                }
-                ++nc;
+            }
+            if (lane_id() == 0) {
+                wnc[idx] = nc;
+                H.wcur_ds[3 * idx] = ds.x; H.wcur_ds[3 * idx + 1] = ds.y; H.wcur_ds[3 * idx + 2] = ds.z;
+                H.wcur_active[idx] = nc > 0 ? 1 : 0;
             }
         }
-        wnc[idx] = nc;
-        H.wcur_ds[3 * idx] = ds.x; H.wcur_ds[3 * idx + 1] = ds.y; H.wcur_ds[3 * idx + 2] = ds.z;
-        H.wcur_active[idx] = nc > 0 ? 1 : 0;
+        __syncthreads();
     }
     kt_end(kt);
+}
+
+__global__ void k_walls_clear(int64_t n, int32_t* wnc, uint8_t* wact) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        wnc[k] = 0;
+        wact[k] = 0;
+    }
 }
 
 // ------------------------------------------------------------ fold + integrate
@@ -1070,10 +1086,13 @@ void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s, unsigne
 }
 
 void launch_need_list(int32_t n, const double* state, const double* radius, const int32_t* cnt_all, const WallBufs& W,
-                      int32_t* flag, int32_t* off, int32_t* scan_tmp, int32_t* list, int32_t* n_list, hipStream_t s) {
-    hipLaunchKernelGGL(k_need, dim3(nblk(n, 256)), dim3(256), 0, s, n, state, radius, cnt_all, W, flag);
+                      int32_t* flag, int32_t* off, int32_t* scan_tmp, int32_t* list, int32_t* n_list, int32_t* wflag,
+                      int32_t* woff, int32_t* wlist, int32_t* n_wlist, hipStream_t s) {
+    hipLaunchKernelGGL(k_need, dim3(nblk(n, 256)), dim3(256), 0, s, n, state, radius, cnt_all, W, flag, wflag);
     scan_i32(flag, nullptr, n, scan_tmp, off, nullptr, s);
     hipLaunchKernelGGL(k_need_list, dim3(nblk(n, 256)), dim3(256), 0, s, n, flag, off, list, n_list);
+    scan_i32(wflag, nullptr, n, scan_tmp, woff, nullptr, s);
+    hipLaunchKernelGGL(k_need_list, dim3(nblk(n, 256)), dim3(256), 0, s, n, wflag, woff, wlist, n_wlist);
 }
 
 void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s, unsigned long long* kt) {
@@ -1138,11 +1157,15 @@ void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const 
                        T.lds_bytes);
 }
 void launch_walls(const DBodies& B, const DMeshes& M, const DMat& mat, const WallBufs& W, const HistBufs& H,
-                  double* wcontrib, int32_t* wnc, double* scr, int64_t nthreads, int32_t pts_cap, const SimParams& sp,
-                  int32_t* status, hipStream_t s, unsigned long long* kt) {
+                  double* wcontrib, int32_t* wnc, const int32_t* list, const int32_t* n_list, double* scr,
+                  int64_t groups, int32_t pts_cap, const SimParams& sp, int32_t* status, hipStream_t s,
+                  unsigned long long* kt) {
     if (W.n_walls <= 0) return;
-    hipLaunchKernelGGL(k_walls, dim3(nblk(nthreads, 256)), dim3(256), 0, s, B, M, mat, W, H, wcontrib, wnc, scr,
-                       nthreads, pts_cap, sp, status, kt);
+    const int64_t n = (int64_t)B.n * W.n_walls;
+    hipLaunchKernelGGL(k_walls_clear, dim3(nblk(n, 256) < 4096 ? nblk(n, 256) : 4096), dim3(256), 0, s, n, wnc,
+                       H.wcur_active);
+    hipLaunchKernelGGL(k_walls, dim3((unsigned)groups), dim3(64), 0, s, B, M, mat, W, H, wcontrib, wnc, list, n_list,
+                       scr, pts_cap, sp, status, kt);
 }
 void launch_fold_integrate(int32_t n, const double* sin, double* sout, const DMat& mat, const int32_t* adj_off,
                            const int32_t* adj_j, const int32_t* adj_pair, int32_t adj_cap, const TopoBufs& T,
