@@ -106,7 +106,7 @@ struct zdx_ctx {
         DBuf<double> stage;
         DBuf<unsigned int> queue;
         DBuf<int32_t> bcnt, bcur;
-        DBuf<int32_t> ncontrib, comps, loops_raw, tstatus;
+        DBuf<int32_t> ncontrib, comps, loops_raw, tstatus, topo_list, topo_n;
         DBuf<int32_t> scr_i;
         DBuf<double> scr_d;
         DBuf<double> rec, vtx;
@@ -230,7 +230,7 @@ struct zdx_sim {
         DBuf<double> wds;
         DBuf<uint8_t> wactive;
     } ps[2];
-    DBuf<int32_t> d_order, d_bcnt, d_bcur;
+    DBuf<int32_t> d_order, d_bcnt, d_bcur, d_topo_list, d_topo_n;
     DBuf<int64_t> d_seg_off, d_tri_checks, d_tri_tests;
     DBuf<int32_t> d_nseg, d_tri_hits, d_tri_scanned;
     DBuf<double> d_arena;
@@ -323,6 +323,7 @@ struct zdx_sim {
         P.seg_off = d_seg_off.p; P.nseg = d_nseg.p; P.tri_checks = d_tri_checks.p; P.tri_tests = d_tri_tests.p;
         P.tri_hits = d_tri_hits.p; P.tri_scanned = d_tri_scanned.p; P.arena = d_arena.p; P.arena_cap = arena_cap; P.arena_top = d_arena_top.p;
         P.status = d_status.p;
+        P.topo_list = d_topo_list.p; P.topo_n = d_topo_n.p;
         return P;
     }
     HistBufs hist(int par) {
@@ -380,7 +381,7 @@ struct zdx_sim {
             HIPC(hipMemcpy(ps[k].n.p, &n, sizeof(int32_t), hipMemcpyHostToDevice));
         }
         d_adj_j.ensure(adj_cap); d_adj_pair.ensure(adj_cap);
-        d_order.ensure(cap);
+        d_order.ensure(cap); d_topo_list.ensure(cap); d_topo_n.ensure(1);
         d_seg_off.ensure(cap); d_tri_checks.ensure(cap); d_tri_tests.ensure(cap);
         d_nseg.ensure(cap); d_tri_hits.ensure(cap); d_tri_scanned.ensure(cap);
         d_ncontrib.ensure(cap); d_comps.ensure(cap); d_loops_raw.ensure(cap); d_tstatus.ensure(cap);
@@ -565,6 +566,7 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     if (batched) launch_lpt_order(P, M, s->d_mesh.p, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st);
     launch_fill_u64(s->d_arena_top.p, 0ull, 1, st);
     launch_fill_i32((int32_t*)s->d_queue.p, 0, 1, st);
+    launch_fill_i32(s->d_topo_n.p, 0, 1, st);
     NarrowCfg nc = s->ncfg;
     nc.use_queue = batched ? 1 : 0;
     nc.groups = persistent_groups(s->ctx, s->np_groups_per_cu);
@@ -572,6 +574,7 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
                   s->d_queue.p, st, kt + 4);
     TopoBufs T = s->topobufs();
     HistBufs H = s->hist(par);
+    launch_topo_zero(P, T, H.cur_active, s->pair_cap, st);
     launch_topo_sim(P, M, B, mat, H, T, s->sp, st, kt + 6);
     if (s->W > 0)
         launch_walls(B, M, mat, s->wallbufs(), H, s->d_wcontrib.p, s->d_wnc.p, s->d_wall_list.p, s->d_n_wall.p,
@@ -816,6 +819,7 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
         A.seg_off.ensure(n); A.tri_checks.ensure(n); A.tri_tests.ensure(n); A.nseg.ensure(n); A.tri_hits.ensure(n); A.tri_scanned.ensure(n);
         A.arena_top.ensure(1); A.queue.ensure(1); A.bcnt.ensure(64); A.bcur.ensure(64);
         A.ncontrib.ensure(n); A.comps.ensure(n); A.loops_raw.ensure(n); A.tstatus.ensure(n);
+        A.topo_list.ensure(n); A.topo_n.ensure(1);
         const bool batched = opts->mode == ZDX_MODE_BATCHED;
         const int groups = std::min<int>(persistent_groups(c, 8), std::max(1, n));
         A.list_i.ensure((size_t)groups * 2 * c->max_tris);
@@ -829,7 +833,9 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
             P.seg_off = A.seg_off.p; P.nseg = A.nseg.p; P.tri_checks = A.tri_checks.p; P.tri_tests = A.tri_tests.p;
             P.tri_hits = A.tri_hits.p; P.tri_scanned = A.tri_scanned.p; P.arena = A.arena.p; P.arena_cap = A.arena_cap; P.arena_top = A.arena_top.p;
             P.status = A.status.p;
+            P.topo_list = A.topo_list.p; P.topo_n = A.topo_n.p;
             launch_fill_i32(A.status.p, 0, 1, st);
+            launch_fill_i32(A.topo_n.p, 0, 1, st);
             launch_fill_u64(A.arena_top.p, 0ull, 1, st);
             launch_fill_i32((int32_t*)A.queue.p, 0, 1, st);
             if (batched) launch_lpt_order(P, M, A.mesh.p, A.order.p, A.bcnt.p, A.bcur.p, n, st);
@@ -854,6 +860,7 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
             A.rec.ensure((size_t)n * T.lmax * 28);
             if (want_vtx) A.vtx.ensure((size_t)n * T.vmax * 3);
             T.scr_i = A.scr_i.p; T.scr_d = A.scr_d.p; T.rec = A.rec.p; T.vtx = want_vtx ? A.vtx.p : nullptr;
+            launch_topo_zero(P, T, nullptr, n, st);
             launch_topo_api(P, M, B, T, opts->split_contacts, st);
             HIPC(hipMemcpyAsync(&status, A.status.p, 4, hipMemcpyDeviceToHost, st));
             HIPC(hipStreamSynchronize(st));
@@ -1277,6 +1284,7 @@ zdx_status zdx_sim_autotune(zdx_sim* s, int32_t* out_block, int32_t* out_groups)
                 for (int rep = 0; rep < 2; ++rep) {
                     launch_fill_u64(s->d_arena_top.p, 0ull, 1, st);
                     launch_fill_i32((int32_t*)s->d_queue.p, 0, 1, st);
+                    launch_fill_i32(s->d_topo_n.p, 0, 1, st);
                     HIPC(hipEventRecord(e0, st));
                     launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->d_stage.p, s->stage_cap,
                                   s->tmax, s->d_queue.p, st);

@@ -304,10 +304,11 @@ struct TopoCaps {
     // sizes of the int / double scratch regions laid out in topology()
     ZDX_HD int n_ints() const {
         const int S2 = 2 * S;
-        return 9 * S + 4 * S + 2 * (S + 1) + 3 * S2 + 5 * S2 + 2 * S2 + 3 * S2 + 2 * E2 + 2 * S2 + 4 * E2 +
-               4 * S2 + E2 + E2 + 1 + 8;
+        const int adj = (S * S + 3) / 4;            // S x S candidate bytes
+        return 9 * S + 4 * S + 2 * (S + 1) + 3 * S2 + 5 * S2 + 2 * S2 + 3 * S2 + 2 * E2 + 3 * (2 * S2) + 4 * E2 +
+               4 * S2 + E2 + E2 + 1 + adj + 8;
     }
-    ZDX_HD int n_dbls() const { return 3 * (2 * S) + 2 * S + 8; }
+    ZDX_HD int n_dbls() const { return 3 * (2 * S) + 2 * (2 * S) + 8; }
 };
 
 struct Split { double t; int v; int k; };   // k = reference scan-order key
@@ -392,8 +393,16 @@ ZDX_HD inline int cand_key(SArr<int> cell9, int i, int j) {
 // (1027-1040), contact_point_xc0 (1042-1052), the <3-vertex and area_eps
 // filters and the flip of nA toward B (2188-2219).
 //
+// The quadratic searches run as flat lane-parallel passes whose iterations do
+// not depend on each other: the component test of every ordered segment pair
+// (popped i, candidate j -- the order the reference evaluates it in), the
+// snap-distance test of every point pair, and the T-junction test of every
+// (edge, vertex) pair.  The order-dependent walks (DFS, vertex numbering,
+// split chains, loop tracing, sums) then run on their results.
+//
 // seg: S segments, 6 doubles each (p0, p1), in narrow-phase order.
-// I / D: int and double scratch of TopoCaps::n_ints() / n_dbls() entries.
+// I / D: int and double scratch of TopoCaps::n_ints() / n_dbls() entries,
+// contiguous (stride 1).
 // For every accepted loop, sink(loop_index, comp, raw_index, Sn, Gn, nA, xc0,
 // area, nverts, vids, V) is called (by every lane) in the reference's order.
 template <class WP, class Sink>
@@ -404,9 +413,11 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
     const int W = WP::W;
     const int ln = wp.lane();
     const double h = tol;
+    const double t2 = h * h;
     const int S2 = 2 * S;
     // ---- scratch layout
     const int E2c = cap.E2;
+    const int HC = 2 * S2;                  // T-junction hit capacity
     int o = 0;
     SArr<int> cell9 = I.at(o); o += 9 * S;
     SArr<int> vis = I.at(o); o += S;
@@ -424,8 +435,9 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
     SArr<int> E = I.at(o); o += 2 * S2;     // snapped edges (a, b)
     SArr<int> vcell3 = I.at(o); o += 3 * S2;
     SArr<int> E2 = I.at(o); o += 2 * E2c;   // split edges (a, b)
-    SArr<int> spv = I.at(o); o += S2;
-    SArr<int> spk = I.at(o); o += S2;
+    SArr<int> spv = I.at(o); o += HC;       // T-junction hits: vertex, scan key, edge
+    SArr<int> spk = I.at(o); o += HC;
+    SArr<int> spe = I.at(o); o += HC;
     SArr<int> adjn = I.at(o); o += 2 * E2c;
     SArr<int> adje = I.at(o); o += 2 * E2c;
     SArr<int> deg = I.at(o); o += S2;
@@ -434,9 +446,10 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
     SArr<int> sord = I.at(o); o += S2;
     SArr<int> used = I.at(o); o += E2c;
     SArr<int> path = I.at(o); o += E2c + 1;
+    uint8_t* adjm = (uint8_t*)(I.p + o);    // S x S component-candidate bytes
     int od = 0;
     SArr<double> Vs = D.at(od); od += 3 * S2;
-    SArr<double> spt = D.at(od); od += S2;
+    SArr<double> spt = D.at(od); od += HC;
 
     auto P0 = [&](int s) { return v3(seg[6 * s + 0], seg[6 * s + 1], seg[6 * s + 2]); };
     auto P1 = [&](int s) { return v3(seg[6 * s + 3], seg[6 * s + 4], seg[6 * s + 5]); };
@@ -454,7 +467,25 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
             vis[i] = 0;
         }
         wp.sync();
-        const double t2 = h * h;
+        // candidate test of every ordered pair (i popped, j candidate): in the
+        // grid neighbourhood, AABB(+tol) overlap, seg_seg_dist2 <= tol^2
+        const int SS = S * S;
+        for (int k = ln; k < SS; k += W) {
+            const int i = k / S, j = k - (k / S) * S;
+            bool acc = false;
+            if (j != i && cand_key(cell9, i, j) != (1 << 30)) {
+                V3 p1 = P0(i), q1 = P1(i), p2 = P0(j), q2 = P1(j);
+                V3 amn = v3(dmin(p1.x, q1.x) - h, dmin(p1.y, q1.y) - h, dmin(p1.z, q1.z) - h);
+                V3 amx = v3(dmax(p1.x, q1.x) + h, dmax(p1.y, q1.y) + h, dmax(p1.z, q1.z) + h);
+                V3 bmn = v3(dmin(p2.x, q2.x) - h, dmin(p2.y, q2.y) - h, dmin(p2.z, q2.z) - h);
+                V3 bmx = v3(dmax(p2.x, q2.x) + h, dmax(p2.y, q2.y) + h, dmax(p2.z, q2.z) + h);
+                bool ovl = !(amx.x < bmn.x || amn.x > bmx.x || amx.y < bmn.y || amn.y > bmx.y || amx.z < bmn.z ||
+                             amn.z > bmx.z);
+                acc = ovl && seg_seg(p1, q1, p2, q2) <= t2;
+            }
+            adjm[k] = acc ? 1 : 0;
+        }
+        wp.sync();
         int nord = 0;
         for (int seed = 0; seed < S; ++seed) {
             if (vis[seed]) continue;
@@ -464,25 +495,12 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
             vis[seed] = 1;
             cord[nord++] = seed;
             while (sp > 0) {
-                int i = stk[--sp];
-                V3 p1 = P0(i), q1 = P1(i);
-                V3 amn = v3(dmin(p1.x, q1.x) - h, dmin(p1.y, q1.y) - h, dmin(p1.z, q1.z) - h);
-                V3 amx = v3(dmax(p1.x, q1.x) + h, dmax(p1.y, q1.y) + h, dmax(p1.z, q1.z) + h);
-                // candidates of i in the reference's scan order: accepted ones
-                // (unvisited, AABB(+tol) overlap, seg_seg_dist2 <= tol^2) are
-                // pushed ordered by (scan key, segment id)
+                const int i = stk[--sp];
+                // accepted candidates of i are pushed ordered by (scan key, id)
                 const int first_acc = sp;
                 for (int base = 0; base < S; base += W) {
                     const int j = base + ln;
-                    bool acc = false;
-                    if (j < S && j != i && !vis[j] && cand_key(cell9, i, j) != (1 << 30)) {
-                        V3 p2 = P0(j), q2 = P1(j);
-                        V3 bmn = v3(dmin(p2.x, q2.x) - h, dmin(p2.y, q2.y) - h, dmin(p2.z, q2.z) - h);
-                        V3 bmx = v3(dmax(p2.x, q2.x) + h, dmax(p2.y, q2.y) + h, dmax(p2.z, q2.z) + h);
-                        bool ovl = !(amx.x < bmn.x || amn.x > bmx.x || amx.y < bmn.y || amn.y > bmx.y ||
-                                     amx.z < bmn.z || amn.z > bmx.z);
-                        acc = ovl && seg_seg(p1, q1, p2, q2) <= t2;
-                    }
+                    const bool acc = j < S && adjm[i * S + j] && !vis[j];
                     uint64_t m = wp.ballot(acc);
                     while (m) {
                         const int jj = base + ctz64(m);
@@ -525,7 +543,8 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
         const int ns = csz[c];
         const int n = 2 * ns;
         auto PT = [&](int k) { int s = cord[base + (k >> 1)]; return (k & 1) ? P1(s) : P0(s); };
-        // snap_endpoints: union-find over points within tol in +-1 cells
+        // snap_endpoints: union-find over points within tol in +-1 cells; the
+        // partition does not depend on the order of the unions
         for (int k = ln; k < n; k += W) {
             I3 cc = cell_of(PT(k), h);
             pcell[3 * k] = cc.x; pcell[3 * k + 1] = cc.y; pcell[3 * k + 2] = cc.z;
@@ -538,25 +557,39 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
             while (par[x] != r) { int nx = par[x]; par[x] = r; x = nx; }
             return r;
         };
-        for (int i = 1; i < n; ++i) {
-            const V3 pi = PT(i);
-            const int cx = pcell[3 * i], cy = pcell[3 * i + 1], cz = pcell[3 * i + 2];
-            for (int jb = 0; jb < i; jb += W) {
-                const int j = jb + ln;
-                bool hit = j < i && near1(cx, cy, cz, pcell[3 * j], pcell[3 * j + 1], pcell[3 * j + 2]) &&
-                           norm2(sub(pi, PT(j))) <= h * h;
-                uint64_t m = wp.ballot(hit);
-                while (m) {
-                    const int jj = jb + ctz64(m);
-                    m &= m - 1;
-                    int ra = find(i), rb = find(jj);
-                    if (ra != rb) {
-                        if (rnk[ra] < rnk[rb]) par[ra] = rb;
-                        else if (rnk[ra] > rnk[rb]) par[rb] = ra;
-                        else { par[rb] = ra; rnk[ra] += 1; }
-                    }
+        const int npairs = n * (n - 1) / 2;
+        for (int kb = 0; kb < npairs; kb += W) {
+            const int k = kb + ln;
+            bool hit = false;
+            int pi = 0, pj = 0;
+            if (k < npairs) {
+                // k -> (i, j), j < i, k = i (i - 1) / 2 + j
+                int i = (int)((1.0 + sqrt(1.0 + 8.0 * (double)k)) * 0.5);
+                while (i * (i - 1) / 2 > k) --i;
+                while ((i + 1) * i / 2 <= k) ++i;
+                pi = i;
+                pj = k - i * (i - 1) / 2;
+                hit = near1(pcell[3 * pi], pcell[3 * pi + 1], pcell[3 * pi + 2], pcell[3 * pj], pcell[3 * pj + 1],
+                            pcell[3 * pj + 2]) &&
+                      norm2(sub(PT(pi), PT(pj))) <= h * h;
+            }
+            uint64_t m = wp.ballot(hit);
+            while (m) {
+                const int src = ctz64(m);
+                m &= m - 1;
+                const int kk = kb + src;
+                int i = (int)((1.0 + sqrt(1.0 + 8.0 * (double)kk)) * 0.5);
+                while (i * (i - 1) / 2 > kk) --i;
+                while ((i + 1) * i / 2 <= kk) ++i;
+                const int j = kk - i * (i - 1) / 2;
+                int ra = find(i), rb = find(j);
+                if (ra != rb) {
+                    if (rnk[ra] < rnk[rb]) par[ra] = rb;
+                    else if (rnk[ra] > rnk[rb]) par[rb] = ra;
+                    else { par[rb] = ra; rnk[ra] += 1; }
                 }
             }
+            (void)pi; (void)pj;
         }
         int nv = 0;
         for (int i = 0; i < n; ++i) {
@@ -584,24 +617,26 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
             int a = ptv[2 * s], b = ptv[2 * s + 1];
             if (a != b) { E[2 * ne] = a; E[2 * ne + 1] = b; ++ne; }
         }
-        // split_t_junctions
+        // split_t_junctions: candidate test of every (edge, vertex) pair, hits
+        // collected edge-major in vertex order
         for (int v = 0; v < nv; ++v) {
             I3 cc = cell_of(VV(v), h);
             vcell3[3 * v] = cc.x; vcell3[3 * v + 1] = cc.y; vcell3[3 * v + 2] = cc.z;
         }
         const double tol2 = h * h;
-        int ne2 = 0;
-        for (int e = 0; e < ne; ++e) {
-            const int a = E[2 * e], b = E[2 * e + 1];
-            const V3 pa = VV(a), pb = VV(b);
-            V3 mn = v3(dmin(pa.x, pb.x) - h, dmin(pa.y, pb.y) - h, dmin(pa.z, pb.z) - h);
-            V3 mx = v3(dmax(pa.x, pb.x) + h, dmax(pa.y, pb.y) + h, dmax(pa.z, pb.z) + h);
-            const I3 c0 = cell_of(mn, h), c1 = cell_of(mx, h);
-            int nsp = 0;
-            for (int vb = 0; vb < nv; vb += W) {
-                const int v = vb + ln;
-                bool hit = false;
-                if (v < nv && v != a && v != b) {
+        int nh = 0;
+        const int ntest = ne * nv;
+        for (int kb = 0; kb < ntest; kb += W) {
+            const int k = kb + ln;
+            bool hit = false;
+            if (k < ntest) {
+                const int e = k / nv, v = k - (k / nv) * nv;
+                const int a = E[2 * e], b = E[2 * e + 1];
+                if (v != a && v != b) {
+                    const V3 pa = VV(a), pb = VV(b);
+                    V3 mn = v3(dmin(pa.x, pb.x) - h, dmin(pa.y, pb.y) - h, dmin(pa.z, pb.z) - h);
+                    V3 mx = v3(dmax(pa.x, pb.x) + h, dmax(pa.y, pb.y) + h, dmax(pa.z, pb.z) + h);
+                    const I3 c0 = cell_of(mn, h), c1 = cell_of(mx, h);
                     int cx = vcell3[3 * v], cy = vcell3[3 * v + 1], cz = vcell3[3 * v + 2];
                     if (!(cx < c0.x || cx > c1.x || cy < c0.y || cy > c1.y || cz < c0.z || cz > c1.z)) {
                         double d2, t;
@@ -609,26 +644,41 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
                         hit = d2 <= tol2 && t > 1e-6 && t < 1.0 - 1e-6;
                     }
                 }
-                uint64_t m = wp.ballot(hit);
-                while (m) {
-                    const int vv = vb + ctz64(m);
-                    m &= m - 1;
-                    double d2, t;
-                    pt_seg(VV(vv), pa, pb, d2, t);
-                    // the reference scans cells ix, iy, iz over [c0, c1]; key = scan index
-                    int cx = vcell3[3 * vv], cy = vcell3[3 * vv + 1], cz = vcell3[3 * vv + 2];
-                    int nyz = (c1.y - c0.y + 1) * (c1.z - c0.z + 1);
-                    spt[nsp] = t; spv[nsp] = vv;
-                    spk[nsp] = (cx - c0.x) * nyz + (cy - c0.y) * (c1.z - c0.z + 1) + (cz - c0.z);
-                    ++nsp;
-                }
             }
+            uint64_t m = wp.ballot(hit);
+            while (m) {
+                const int kk = kb + ctz64(m);
+                m &= m - 1;
+                const int e = kk / nv, v = kk - (kk / nv) * nv;
+                const int a = E[2 * e], b = E[2 * e + 1];
+                const V3 pa = VV(a), pb = VV(b);
+                V3 mn = v3(dmin(pa.x, pb.x) - h, dmin(pa.y, pb.y) - h, dmin(pa.z, pb.z) - h);
+                V3 mx = v3(dmax(pa.x, pb.x) + h, dmax(pa.y, pb.y) + h, dmax(pa.z, pb.z) + h);
+                const I3 c0 = cell_of(mn, h), c1 = cell_of(mx, h);
+                double d2, t;
+                pt_seg(VV(v), pa, pb, d2, t);
+                if (nh >= HC) { out.status |= TOPO_E2_OVERFLOW; return out; }
+                // the reference scans cells ix, iy, iz over [c0, c1]; key = scan index
+                int cx = vcell3[3 * v], cy = vcell3[3 * v + 1], cz = vcell3[3 * v + 2];
+                int nyz = (c1.y - c0.y + 1) * (c1.z - c0.z + 1);
+                spt[nh] = t; spv[nh] = v; spe[nh] = e;
+                spk[nh] = (cx - c0.x) * nyz + (cy - c0.y) * (c1.z - c0.z + 1) + (cz - c0.z);
+                ++nh;
+            }
+        }
+        int ne2 = 0;
+        int h0 = 0;
+        for (int e = 0; e < ne; ++e) {
+            const int a = E[2 * e], b = E[2 * e + 1];
+            int h1 = h0;
+            while (h1 < nh && spe[h1] == e) ++h1;
+            const int nsp = h1 - h0;
             if (nsp == 0) {
                 if (ne2 + 1 > E2c) { out.status |= TOPO_E2_OVERFLOW; return out; }
                 E2[2 * ne2] = a; E2[2 * ne2 + 1] = b; ++ne2;
                 continue;
             }
-            SplitArr sa; sa.t = spt; sa.v = spv; sa.k = spk;
+            SplitArr sa; sa.t = spt.at(h0); sa.v = spv.at(h0); sa.k = spk.at(h0);
             // bring candidates into the reference's scan order, then its std::sort by t
             SplitByScan byscan;
             for (int i2 = 1; i2 < nsp; ++i2) {
@@ -641,7 +691,7 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
             ref_sort(sa, nsp, byt);
             int prevv = a, last = -1;
             for (int k = 0; k < nsp; ++k) {
-                int v = spv[k];
+                int v = spv[h0 + k];
                 if (v == last) continue;
                 last = v;
                 if (prevv != v) {
@@ -654,6 +704,7 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
                 if (ne2 + 1 > E2c) { out.status |= TOPO_E2_OVERFLOW; return out; }
                 E2[2 * ne2] = prevv; E2[2 * ne2 + 1] = b; ++ne2;
             }
+            h0 = h1;
         }
         // extract_loops
         if (nv > S2) { out.status |= TOPO_E2_OVERFLOW; return out; }

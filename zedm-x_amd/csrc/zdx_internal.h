@@ -118,6 +118,8 @@ struct DPairs {
     int32_t* tri_scanned;    // per pair: TA + TB
     double* arena;           // segments, 6 doubles each
     int64_t arena_cap;       // in segments
+    int32_t* topo_list;      // pairs with at least one segment (appended by the narrow phase)
+    int32_t* topo_n;         // device count of topo_list
     unsigned long long* arena_top;
     int32_t* status;         // global error bits
 };

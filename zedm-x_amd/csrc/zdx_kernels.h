@@ -116,6 +116,9 @@ void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const Na
                    hipStream_t s, unsigned long long* kt = nullptr);
 void launch_lpt_order(const DPairs& P, const DMeshes& M, const int32_t* body_mesh, int32_t* order, int32_t* bucket_cnt,
                       int32_t* bucket_cur, int32_t pair_cap, hipStream_t s);
+// zero outputs of every pair (the topology kernels then visit only the
+// pairs of P.topo_list); cur_active may be NULL
+void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, int32_t pair_cap, hipStream_t s);
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
                      const TopoBufs& T, const SimParams& sp, hipStream_t s, unsigned long long* kt = nullptr);
 void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int32_t split,

@@ -674,6 +674,7 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
         if (tid == 0) {
             P.seg_off[p] = bse != ~0ull ? (int64_t)bse : 0;
             P.nseg[p] = bse != ~0ull ? nstore : 0;
+            if (bse != ~0ull && nstore > 0) P.topo_list[atomicAdd(P.topo_n, 1)] = p;
             P.tri_checks[p] = (int64_t)B.nondeg[i] * TB;
             P.tri_tests[p] = total;
             P.tri_hits[p] = hits;
@@ -769,10 +770,11 @@ __global__ void __launch_bounds__(64) k_topo_sim(DPairs P, DMeshes M, DBodies B,
                                                  SimParams sp, int32_t lds_bytes, unsigned long long* kt) {
     extern __shared__ char s_topo[];
     kt_start(kt);
-    const int n = *P.n;
+    const int n = *P.topo_n;
     const int nprev = *H.prev_n;
     LaneDev wp;
-    for (int p = blockIdx.x; p < n; p += gridDim.x) {
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const int p = P.topo_list[it];
         const int i = P.pi[p], j = P.pj[p];
         const int S = P.nseg[p];
         const int mA = B.mesh[i];
@@ -821,6 +823,20 @@ __global__ void __launch_bounds__(64) k_topo_sim(DPairs P, DMeshes M, DBodies B,
     kt_end(kt);
 }
 
+// Outputs of pairs the topology kernels do not visit (no segments): no
+// loops, no history entry (the reference's map entry is only created for an
+// accepted loop and erased otherwise).
+__global__ void k_topo_zero(DPairs P, TopoBufs T, uint8_t* cur_active) {
+    const int n = *P.n;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        T.ncontrib[p] = 0;
+        T.comps[p] = 0;
+        T.loops_raw[p] = 0;
+        T.tstatus[p] = 0;
+        if (cur_active) cur_active[p] = 0;
+    }
+}
+
 struct ApiSink {
     double* rec;
     double* vtx;
@@ -859,9 +875,10 @@ struct ApiSink {
 __global__ void __launch_bounds__(64) k_topo_api(DPairs P, DMeshes M, DBodies B, TopoBufs T, int32_t split,
                                                  int32_t lds_bytes) {
     extern __shared__ char s_topo[];
-    const int n = *P.n;
+    const int n = *P.topo_n;
     LaneDev wp;
-    for (int p = blockIdx.x; p < n; p += gridDim.x) {
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const int p = P.topo_list[it];
         const int i = P.pi[p], j = P.pj[p];
         const int S = P.nseg[p];
         const int mA = B.mesh[i];
@@ -1144,6 +1161,13 @@ void launch_lpt_order(const DPairs& P, const DMeshes& M, const int32_t* body_mes
     hipLaunchKernelGGL(k_lpt_count, dim3(nblk(pair_cap, 256)), dim3(256), 0, s, P, M, body_mesh, bcnt);
     hipLaunchKernelGGL(k_lpt_scan, dim3(1), dim3(64), 0, s, bcnt, bcur);
     hipLaunchKernelGGL(k_lpt_scatter, dim3(nblk(pair_cap, 256)), dim3(256), 0, s, P, M, body_mesh, bcur, order);
+}
+
+void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, int32_t pair_cap, hipStream_t s) {
+    int g = nblk(pair_cap, 256);
+    if (g > 2048) g = 2048;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_topo_zero, dim3(g), dim3(256), 0, s, P, T, cur_active);
 }
 
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
