@@ -748,22 +748,25 @@ struct LaneDev {
     __device__ void sync() const { __syncthreads(); }
 };
 
-__device__ inline void topo_scratch(const TopoBufs& T, int S, char* lds, int lds_bytes, TopoCaps& caps, SArr<int>& I,
-                                    SArr<double>& D) {
+// Scratch of one pair: LDS when it fits, else this workgroup's global slice.
+// The two cases call topology() separately so each inlined copy addresses
+// its scratch directly (LDS instructions for the first).
+template <class Body>
+__device__ inline TopoOut with_topo_scratch(const TopoBufs& T, int S, char* lds, int lds_bytes, Body&& body) {
+    TopoCaps caps;
     caps.S = S > 0 ? S : 1;
     caps.E2 = 4 * caps.S;
     const int64_t ni = caps.n_ints(), nd = caps.n_dbls();
     const int64_t ibytes = (ni * 4 + 7) & ~(int64_t)7;
     if (ibytes + nd * 8 <= lds_bytes) {
-        I.p = (int*)lds;
-        D.p = (double*)(lds + ibytes);
-    } else {
-        TopoCaps full; full.S = T.seg_max; full.E2 = 4 * T.seg_max;
-        I.p = T.scr_i + (int64_t)blockIdx.x * full.n_ints();
-        D.p = T.scr_d + (int64_t)blockIdx.x * full.n_dbls();
+        SArr<int> I; I.p = (int*)lds; I.s = 1;
+        SArr<double> D; D.p = (double*)(lds + ibytes); D.s = 1;
+        return body(I, D, caps);
     }
-    I.s = 1;
-    D.s = 1;
+    TopoCaps full; full.S = T.seg_max; full.E2 = 4 * T.seg_max;
+    SArr<int> I; I.p = T.scr_i + (int64_t)blockIdx.x * full.n_ints(); I.s = 1;
+    SArr<double> D; D.p = T.scr_d + (int64_t)blockIdx.x * full.n_dbls(); D.s = 1;
+    return body(I, D, caps);
 }
 
 __global__ void __launch_bounds__(64) k_topo_sim(DPairs P, DMeshes M, DBodies B, DMat mat, HistBufs H, TopoBufs T,
@@ -802,12 +805,12 @@ __global__ void __launch_bounds__(64) k_topo_sim(DPairs P, DMeshes M, DBodies B,
         to.comps = 0; to.loops_raw = 0; to.loops = 0; to.status = 0;
         if (S > T.seg_max) to.status = TOPO_E2_OVERFLOW;
         else if (S > 0) {
-            TopoCaps caps;
-            SArr<int> I;
-            SArr<double> D;
-            topo_scratch(T, S, s_topo, lds_bytes, caps, I, D);
-            to = topology(wp, P.arena + 6 * P.seg_off[p], S, M.tol[mA], M.area_eps[mA], sp.split != 0,
-                          sub(b.pos, a.pos), I, D, caps, sink);
+            const double* sg = P.arena + 6 * P.seg_off[p];
+            const double tl = M.tol[mA], ae = M.area_eps[mA];
+            const V3 dir = sub(b.pos, a.pos);
+            to = with_topo_scratch(T, S, s_topo, lds_bytes, [&](SArr<int> I, SArr<double> D, const TopoCaps& caps) {
+                return topology(wp, sg, S, tl, ae, sp.split != 0, dir, I, D, caps, sink);
+            });
         }
         if (lane_id() == 0) {
             T.ncontrib[p] = to.loops < T.lmax ? to.loops : T.lmax;
@@ -889,12 +892,12 @@ __global__ void __launch_bounds__(64) k_topo_api(DPairs P, DMeshes M, DBodies B,
         to.comps = 0; to.loops_raw = 0; to.loops = 0; to.status = 0;
         if (S > T.seg_max) to.status = TOPO_E2_OVERFLOW;
         else if (S > 0) {
-            TopoCaps caps;
-            SArr<int> I;
-            SArr<double> D;
-            topo_scratch(T, S, s_topo, lds_bytes, caps, I, D);
-            to = topology(wp, P.arena + 6 * P.seg_off[p], S, M.tol[mA], M.area_eps[mA], split != 0, sub(pb, pa), I,
-                          D, caps, sink);
+            const double* sg = P.arena + 6 * P.seg_off[p];
+            const double tl = M.tol[mA], ae = M.area_eps[mA];
+            const V3 dir = sub(pb, pa);
+            to = with_topo_scratch(T, S, s_topo, lds_bytes, [&](SArr<int> I, SArr<double> D, const TopoCaps& caps) {
+                return topology(wp, sg, S, tl, ae, split != 0, dir, I, D, caps, sink);
+            });
         }
         if (lane_id() == 0) {
             T.ncontrib[p] = to.loops;
