@@ -549,6 +549,11 @@ WallHost make_wall(const HostMesh& m, const double* pos, const double* quat, dou
 
 // Launch sequence of one step; `host_sized` inserts the plain mode's syncs and
 // retries, otherwise everything is device-sized (graph-capturable).
+// Launch sequence of one step.  Batched: everything device-sized (persistent
+// grids reading counts from device memory, LPT work queue) so the step can be
+// captured in a graph.  Plain: one launch per stage with the grids sized on
+// the host from the counts read back after the broad phase and after the
+// narrow phase.
 void enqueue_step(zdx_sim* s, int par, bool batched) {
     hipStream_t st = s->st();
     const int nxt = 1 - par;
@@ -556,12 +561,25 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     DMeshes M = s->ctx->dmeshes();
     DMat mat = s->mat();
     unsigned long long* kt = s->d_kt.p;   // stage slots: 0 transform 1 broad 2 narrow 3 topo 4 walls 5 fold
+    const int gmax = persistent_groups(s->ctx, s->np_groups_per_cu);
     launch_kt_reset(kt, 6, st);
     launch_broadphase(s->bpbufs(par), st, kt + 2);
     launch_need_list(s->N, s->d_state[par].p, s->d_radius.p, s->d_cnt_all.p, s->wallbufs(), s->d_need_flag.p,
                      s->d_need_off.p, s->d_scan_tmp.p, s->d_need_list.p, s->d_n_need.p, s->d_wall_flag.p,
                      s->d_wall_off.p, s->d_wall_list.p, s->d_n_wall.p, st);
-    launch_transform(B, M, st, kt + 0, s->d_need_list.p, s->d_n_need.p, s->transform_groups());
+    int g_transform = s->transform_groups(), g_narrow = gmax, g_topo = (int)s->topo_threads,
+        g_walls = (int)s->wall_threads;
+    if (!batched) {
+        int32_t c[3] = {0, 0, 0};
+        HIPC(hipMemcpyAsync(&c[0], s->ps[par].n.p, 4, hipMemcpyDeviceToHost, st));
+        HIPC(hipMemcpyAsync(&c[1], s->d_n_need.p, 4, hipMemcpyDeviceToHost, st));
+        HIPC(hipMemcpyAsync(&c[2], s->d_n_wall.p, 4, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        g_narrow = std::max(1, std::min(c[0], gmax));
+        g_transform = std::max(1, std::min(c[1], g_transform));
+        g_walls = std::max(1, std::min(c[2], g_walls));
+    }
+    launch_transform(B, M, st, kt + 0, s->d_need_list.p, s->d_n_need.p, g_transform);
     DPairs P = s->dpairs(par, batched);
     if (batched) launch_lpt_order(P, M, s->d_mesh.p, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st);
     launch_fill_u64(s->d_arena_top.p, 0ull, 1, st);
@@ -569,16 +587,22 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     launch_fill_i32(s->d_topo_n.p, 0, 1, st);
     NarrowCfg nc = s->ncfg;
     nc.use_queue = batched ? 1 : 0;
-    nc.groups = persistent_groups(s->ctx, s->np_groups_per_cu);
+    nc.groups = g_narrow;
     launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->d_stage.p, s->stage_cap, s->tmax,
                   s->d_queue.p, st, kt + 4);
     TopoBufs T = s->topobufs();
+    if (!batched) {
+        int32_t nt = 0;
+        HIPC(hipMemcpyAsync(&nt, s->d_topo_n.p, 4, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        T.nthreads = std::max(1, std::min<int>(nt, g_topo));
+    }
     HistBufs H = s->hist(par);
     launch_topo_zero(P, T, H.cur_active, s->pair_cap, st);
     launch_topo_sim(P, M, B, mat, H, T, s->sp, st, kt + 6);
     if (s->W > 0)
         launch_walls(B, M, mat, s->wallbufs(), H, s->d_wcontrib.p, s->d_wnc.p, s->d_wall_list.p, s->d_n_wall.p,
-                     s->d_wscr.p, s->wall_threads, s->wall_pts_cap, s->sp, s->d_status.p, st, kt + 8);
+                     s->d_wscr.p, g_walls, s->wall_pts_cap, s->sp, s->d_status.p, st, kt + 8);
     launch_fold_integrate(s->N, s->d_state[par].p, s->d_state[nxt].p, mat, s->d_adj_off.p, s->d_adj_j.p,
                           s->d_adj_pair.p, s->adj_cap, T, s->W, s->d_wcontrib.p, s->d_wnc.p, s->max_wall_tris,
                           s->d_F.p, s->d_T.p, s->d_counts.p, s->sp, st, kt + 10);
