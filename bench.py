@@ -205,7 +205,7 @@ def main():
     # roofline of the dominant kernel of the batched step, from the live events
     stage_ms = {k: st_b[k] for k in zdx.Sim.STAGES}
     dom = max(stage_ms, key=stage_ms.get)
-    per_step = {k: v / args.steps for k, v in st_b["counters"].items()}
+    per_step = {k: (v if k == "seg_max" else v / args.steps) for k, v in st_b["counters"].items()}
     ops, byts = narrow_model(per_step)
     t_np = stage_ms["narrow"] * 1e-3
     ach_tflops = ops / t_np / 1e12 if t_np > 0 else 0.0

@@ -145,7 +145,9 @@ enum {
     ZDX_CTR_TRI_TESTS = 7,    /* triangle AABB tests actually executed after culling */
     ZDX_CTR_TRI_HITS = 8,     /* triangle pairs that reached the exact intersection test */
     ZDX_CTR_TRI_SCANNED = 9,  /* triangles of both meshes scanned by the per-pair cull (sum of TA + TB) */
-    ZDX_N_COUNTERS = 10
+    ZDX_CTR_TOPO_PAIRS = 10,  /* pairs with at least one intersection segment */
+    ZDX_CTR_SEG_MAX = 11,     /* largest segment count of one pair (zdx_sim_stage_times: max over the steps) */
+    ZDX_N_COUNTERS = 12
 };
 /* Runs n_steps steps in mode ZDX_MODE_PLAIN (one kernel per stage,
  * host-sized launches) or ZDX_MODE_BATCHED (persistent work-queue kernels,
@@ -170,7 +172,7 @@ zdx_status zdx_sim_restore(zdx_sim* sim, const char* path);
  * both modes and inside the captured graph.  out[0..5]:
  * mean ms per step of transform, broad phase, narrow phase, topology+forces,
  * walls, fold+integrate+counters; out[6]: steps; out[7..7+ZDX_N_COUNTERS):
- * counter sums.  Writes min(n, 7 + ZDX_N_COUNTERS) values. */
+ * counter sums (ZDX_CTR_SEG_MAX: the maximum).  Writes min(n, 7 + ZDX_N_COUNTERS) values. */
 zdx_status zdx_sim_stage_times(zdx_sim* sim, double* out, int32_t n);
 /* Narrow-phase launch shape; 0 keeps the current value.  The autotuner
  * (zdx_sim_autotune) sets them from timed trial steps. */

@@ -547,8 +547,6 @@ WallHost make_wall(const HostMesh& m, const double* pos, const double* quat, dou
     return w;
 }
 
-// Launch sequence of one step; `host_sized` inserts the plain mode's syncs and
-// retries, otherwise everything is device-sized (graph-capturable).
 // Launch sequence of one step.  Batched: everything device-sized (persistent
 // grids reading counts from device memory, LPT work queue) so the step can be
 // captured in a graph.  Plain: one launch per stage with the grids sized on
@@ -708,7 +706,9 @@ void step_once(zdx_sim* s, bool batched) {
             for (int k = 0; k < 6; ++k)
                 if (kt[2 * k + 1] > kt[2 * k] && kt[2 * k] != ~0ull)
                     s->acc_ms[k] += (double)(kt[2 * k + 1] - kt[2 * k]) / s->clock_khz;
-            for (int k = 0; k < ZDX_N_COUNTERS; ++k) s->acc_ctr[k] += (double)s->last_ctr[k];
+            for (int k = 0; k < ZDX_N_COUNTERS; ++k)
+                s->acc_ctr[k] = k == ZDX_CTR_SEG_MAX ? std::max(s->acc_ctr[k], (double)s->last_ctr[k])
+                                                     : s->acc_ctr[k] + (double)s->last_ctr[k];
             s->acc_steps += 1;
             s->parity = 1 - par;
             s->n_steps_done += 1;

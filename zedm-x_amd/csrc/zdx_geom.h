@@ -354,12 +354,14 @@ struct TopoOut {
 // the O(S^2) scans are split across lanes, their results combined by a ballot
 // mask that the lanes then walk in ascending order.  sync() is needed only
 // after a lane-parallel phase whose writes other lanes read.
+// mark(k) closes the span of phase k (a no-op unless a policy times phases).
 struct LaneHost {
     static constexpr int W = 1;
     int lane() const { return 0; }
     uint64_t ballot(bool p) const { return p ? 1ull : 0ull; }
     double shfl(double v, int) const { return v; }
     void sync() const {}
+    void mark(int) const {}
 };
 
 ZDX_HD inline int ctz64(uint64_t m) {
@@ -486,6 +488,7 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
             adjm[k] = acc ? 1 : 0;
         }
         wp.sync();
+        wp.mark(1);
         int nord = 0;
         for (int seed = 0; seed < S; ++seed) {
             if (vis[seed]) continue;
@@ -535,6 +538,7 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
     }
     out.comps = ncomp;
     wp.sync();
+    wp.mark(2);
 
     // ---- per component
     for (int ci = 0; ci < ncomp; ++ci) {
@@ -557,6 +561,7 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
             while (par[x] != r) { int nx = par[x]; par[x] = r; x = nx; }
             return r;
         };
+        wp.mark(3);
         const int npairs = n * (n - 1) / 2;
         for (int kb = 0; kb < npairs; kb += W) {
             const int k = kb + ln;
@@ -591,6 +596,7 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
             }
             (void)pi; (void)pj;
         }
+        wp.mark(4);
         int nv = 0;
         for (int i = 0; i < n; ++i) {
             int r = find(i);
@@ -623,6 +629,7 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
             I3 cc = cell_of(VV(v), h);
             vcell3[3 * v] = cc.x; vcell3[3 * v + 1] = cc.y; vcell3[3 * v + 2] = cc.z;
         }
+        wp.mark(5);
         const double tol2 = h * h;
         int nh = 0;
         const int ntest = ne * nv;
@@ -666,6 +673,7 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
                 ++nh;
             }
         }
+        wp.mark(6);
         int ne2 = 0;
         int h0 = 0;
         for (int e = 0; e < ne; ++e) {
@@ -706,6 +714,7 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
             }
             h0 = h1;
         }
+        wp.mark(7);
         // extract_loops
         if (nv > S2) { out.status |= TOPO_E2_OVERFLOW; return out; }
         for (int v = 0; v < nv; ++v) deg[v] = 0;
@@ -780,6 +789,7 @@ ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double to
         }
         out.loops_raw += nloops_here;
         wp.sync();
+        wp.mark(8);
     }
     return out;
 }

@@ -740,12 +740,30 @@ __device__ inline Mat load_mat(const DMat& m, int i) {
 // One wave per pair (workgroup = 64 lanes); the pair's scratch lives in LDS
 // when it fits (dynamic shared memory of lds_bytes), otherwise in this
 // workgroup's slice of the global scratch.
+#ifdef ZDX_TOPO_PROFILE
+// Build option for profiling the topology: device clock ticks per phase of
+// topology(), summed over all pairs (phase 0: set-up before the call, 9: the
+// outputs after it), read back with zdx_topo_profile().
+__device__ unsigned long long g_topo_prof[16];
+#endif
 struct LaneDev {
     static constexpr int W = WAVE;
     __device__ int lane() const { return lane_id(); }
     __device__ uint64_t ballot(bool p) const { return __ballot(p); }
     __device__ double shfl(double v, int src) const { return __shfl(v, src, WAVE); }
     __device__ void sync() const { __syncthreads(); }
+#ifdef ZDX_TOPO_PROFILE
+    mutable unsigned long long t0 = 0;
+    __device__ void start() const { t0 = wall_clock64(); }
+    __device__ void mark(int k) const {
+        const unsigned long long t = wall_clock64();
+        if (lane_id() == 0) atomicAdd(&g_topo_prof[k], t - t0);
+        t0 = t;
+    }
+#else
+    __device__ void start() const {}
+    __device__ void mark(int) const {}
+#endif
 };
 
 // Scratch of one pair: LDS when it fits, else this workgroup's global slice.
@@ -777,6 +795,7 @@ __global__ void __launch_bounds__(64) k_topo_sim(DPairs P, DMeshes M, DBodies B,
     const int nprev = *H.prev_n;
     LaneDev wp;
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        wp.start();
         const int p = P.topo_list[it];
         const int i = P.pi[p], j = P.pj[p];
         const int S = P.nseg[p];
@@ -808,6 +827,7 @@ __global__ void __launch_bounds__(64) k_topo_sim(DPairs P, DMeshes M, DBodies B,
             const double* sg = P.arena + 6 * P.seg_off[p];
             const double tl = M.tol[mA], ae = M.area_eps[mA];
             const V3 dir = sub(b.pos, a.pos);
+            wp.mark(0);
             to = with_topo_scratch(T, S, s_topo, lds_bytes, [&](SArr<int> I, SArr<double> D, const TopoCaps& caps) {
                 return topology(wp, sg, S, tl, ae, sp.split != 0, dir, I, D, caps, sink);
             });
@@ -822,6 +842,7 @@ __global__ void __launch_bounds__(64) k_topo_sim(DPairs P, DMeshes M, DBodies B,
             if (to.status) atomicOr(P.status, (int)ST_TOPO);
         }
         __syncthreads();
+        wp.mark(9);
     }
     kt_end(kt);
 }
@@ -881,6 +902,7 @@ __global__ void __launch_bounds__(64) k_topo_api(DPairs P, DMeshes M, DBodies B,
     const int n = *P.topo_n;
     LaneDev wp;
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        wp.start();
         const int p = P.topo_list[it];
         const int i = P.pi[p], j = P.pj[p];
         const int S = P.nseg[p];
@@ -895,6 +917,7 @@ __global__ void __launch_bounds__(64) k_topo_api(DPairs P, DMeshes M, DBodies B,
             const double* sg = P.arena + 6 * P.seg_off[p];
             const double tl = M.tol[mA], ae = M.area_eps[mA];
             const V3 dir = sub(pb, pa);
+            wp.mark(0);
             to = with_topo_scratch(T, S, s_topo, lds_bytes, [&](SArr<int> I, SArr<double> D, const TopoCaps& caps) {
                 return topology(wp, sg, S, tl, ae, split != 0, dir, I, D, caps, sink);
             });
@@ -907,6 +930,7 @@ __global__ void __launch_bounds__(64) k_topo_api(DPairs P, DMeshes M, DBodies B,
             if (to.status) atomicOr(P.status, (int)ST_TOPO);
         }
         __syncthreads();
+        wp.mark(9);
     }
 }
 
@@ -1054,11 +1078,14 @@ __global__ void __launch_bounds__(256) k_counters(DPairs P, TopoBufs T, int32_t 
                                                   unsigned long long* ctr) {
     const int n = *P.n;
     unsigned long long c_loops = 0, c_checks = 0, c_seg = 0, c_comps = 0, c_raw = 0, c_tests = 0, c_hits = 0,
-                       c_wall = 0, c_scan = 0;
+                       c_wall = 0, c_scan = 0, c_tp = 0, c_smax = 0;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const unsigned long long ns = (unsigned long long)P.nseg[p];
         c_loops += T.ncontrib[p];
         c_checks += P.tri_checks[p];
-        c_seg += P.nseg[p];
+        c_seg += ns;
+        c_tp += ns > 0 ? 1ull : 0ull;
+        c_smax = ns > c_smax ? ns : c_smax;
         c_comps += T.comps[p];
         c_raw += T.loops_raw[p];
         c_tests += P.tri_tests[p];
@@ -1066,9 +1093,13 @@ __global__ void __launch_bounds__(256) k_counters(DPairs P, TopoBufs T, int32_t 
         c_scan += P.tri_scanned[p];
     }
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n_pw; k += gridDim.x * blockDim.x) c_wall += wnc[k];
-    unsigned long long v[9] = {c_loops, c_checks, c_seg, c_comps, c_raw, c_tests, c_hits, c_wall, c_scan};
-    for (int k = 0; k < 9; ++k)
+    unsigned long long v[10] = {c_loops, c_checks, c_seg, c_comps, c_raw, c_tests, c_hits, c_wall, c_scan, c_tp};
+    for (int k = 0; k < 10; ++k)
         for (int d = 32; d >= 1; d >>= 1) v[k] += __shfl_xor(v[k], d, WAVE);
+    for (int d = 32; d >= 1; d >>= 1) {
+        const unsigned long long o = __shfl_xor(c_smax, d, WAVE);
+        c_smax = o > c_smax ? o : c_smax;
+    }
     if (lane_id() == 0) {
         atomicAdd(&ctr[0], v[0] + v[7]);   // contacts = accepted loops + wall contacts
         atomicAdd(&ctr[1], v[1]);
@@ -1079,6 +1110,8 @@ __global__ void __launch_bounds__(256) k_counters(DPairs P, TopoBufs T, int32_t 
         atomicAdd(&ctr[7], v[5]);
         atomicAdd(&ctr[8], v[6]);
         atomicAdd(&ctr[9], v[8]);
+        atomicAdd(&ctr[10], v[9]);
+        atomicMax(&ctr[11], c_smax);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctr[5], (unsigned long long)n);
 }
@@ -1091,6 +1124,14 @@ __global__ void k_iota_pairs(int32_t* pi, int32_t* pj, int32_t n) {
 inline int nblk(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
 }  // namespace
+
+#ifdef ZDX_TOPO_PROFILE
+extern "C" int zdx_topo_profile(unsigned long long* out) {
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_topo_prof), 16 * sizeof(unsigned long long)) != hipSuccess) return 1;
+    unsigned long long z[16] = {0};
+    return hipMemcpyToSymbol(HIP_SYMBOL(g_topo_prof), z, sizeof(z)) == hipSuccess ? 0 : 1;
+}
+#endif
 
 // ------------------------------------------------------------ launchers
 void launch_kt_reset(unsigned long long* kt, int n, hipStream_t s) {

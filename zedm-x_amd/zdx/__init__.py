@@ -20,7 +20,7 @@ CLI_PATH = os.path.join(PKG, "bin", "zdx")
 OK, BAD_ARG, IO, CAPACITY, HIP_ERROR, INTERNAL = range(6)
 MODE_PLAIN, MODE_BATCHED = 0, 1
 COUNTERS = ["contacts", "tri_checks", "segments", "comps", "loops", "pairs", "wall_contacts", "tri_tests", "tri_hits",
-            "tri_scanned"]
+            "tri_scanned", "topo_pairs", "seg_max"]
 EXPORTS = ["zdx_create", "zdx_destroy", "zdx_last_error", "zdx_load_stl", "zdx_mesh_register", "zdx_contact_pairs",
            "zdx_sim_load_config", "zdx_sim_create", "zdx_sim_destroy", "zdx_sim_num_particles", "zdx_sim_step",
            "zdx_sim_get_state", "zdx_sim_set_state", "zdx_sim_get_forces", "zdx_sim_checkpoint", "zdx_sim_restore",
