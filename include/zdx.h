@@ -161,6 +161,16 @@ zdx_status zdx_sim_set_state(zdx_sim* sim, const double* state);
 /* Force, torque and contact count of each particle from the last step
  * (main.cpp:2027-2029); any pointer may be NULL. */
 zdx_status zdx_sim_get_forces(zdx_sim* sim, double* F, double* T, int32_t* contact_counts);
+/* write_vtk_particles, src/main.cpp:1710-1840: legacy ASCII VTK polydata of
+ * every particle's world triangles with id, mass, radius, contact_count,
+ * velocity, omega, force, torque and orientation, byte-compatible with the
+ * reference's file for equal state. */
+zdx_status zdx_sim_write_vtk(zdx_sim* sim, const char* path);
+/* write_particle_state_txt, src/main.cpp:1842-1870: particle blocks (stl,
+ * pos, vel, quat, omega, scale, material) that a config file can include.
+ * stl and scale are the config's; sims made with zdx_sim_create write an
+ * empty stl and scale 1. */
+zdx_status zdx_sim_write_state_txt(zdx_sim* sim, const char* path);
 /* Exact binary checkpoint of state and tangential histories (the
  * reference's only restart file, write_particle_state_txt main.cpp:1842-1870,
  * keeps 9 decimals and no history). */

@@ -219,7 +219,40 @@ def gold_sims(R):
     return out
 
 
+# (case, step) whose VTK and state txt files are kept byte for byte
+OUTPUT_FILES = [("wall_drop", 2000), ("cluster_box", 50)]
+
+
+def gold_outputs():
+    """Copies the output files (write_vtk_particles, write_particle_state_txt)
+    of a few snapshots written by the reference's unmodified zdem_cpu program
+    (oracle/_ref/zdem_cpu; the harness build prints more digits) into output/."""
+    import subprocess
+
+    od = os.path.join(HERE, "output")
+    os.makedirs(od, exist_ok=True)
+    want = dict(OUTPUT_FILES)
+    with tempfile.TemporaryDirectory() as td:
+        os.symlink(os.path.join(INPUTS, "geometry"), os.path.join(td, "geometry"))
+        for name, text, files in GI.sim_cases(INPUTS):
+            if name not in want:
+                continue
+            for fn, tris in files.items():
+                synth.write_stl_ascii(os.path.join(td, fn), tris)
+            cfg_path = os.path.join(td, name + ".txt")
+            with open(cfg_path, "w") as f:
+                f.write(text)
+            subprocess.run([O.REF_BIN, "--config", cfg_path], cwd=td, check=True, stdout=subprocess.DEVNULL)
+            prefix = [ln for ln in text.splitlines() if ln.startswith("vtk_prefix")][0].split("=")[1].strip()
+            for ext in ("vtk", "txt"):
+                src = os.path.join(td, "out", f"{prefix}_{want[name]:06d}.{ext}")
+                shutil.copyfile(src, os.path.join(od, f"{name}_{want[name]:06d}.{ext}"))
+
+
 def main():
+    if "--outputs-only" in sys.argv:
+        gold_outputs()
+        return
     if not O.ref_available():
         O.build(ref=True)
     R = O.ref()
@@ -230,6 +263,7 @@ def main():
     sims = gold_sims(R)
     with open(os.path.join(HERE, "golden_sims.json"), "w") as f:
         json.dump(sims, f, separators=(",", ":"))
+    gold_outputs()
     print("pairs:", [(p["name"], p["segments"], len(p["comps"]), len(p["loops"])) for p in data["pairs"]])
     print("sims:", [(s["name"], len(s["snapshots"])) for s in sims])
 

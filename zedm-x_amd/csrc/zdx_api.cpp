@@ -199,6 +199,7 @@ struct zdx_sim {
     zdx_ctx* ctx = nullptr;
     int32_t N = 0, W = 0;
     SimConfig cfg;
+    std::vector<ParticleInit> inits;   // per particle, as configured (state txt output)
     SimParams sp{};
     std::vector<ParticleConst> pc;
     std::vector<int32_t> mesh_of;
@@ -993,6 +994,12 @@ zdx_status zdx_sim_create(zdx_ctx* c, const zdx_sim_desc* d, zdx_sim** out) {
             std::memcpy(x, in, 13 * sizeof(double));
             initial_L(s->pc[i], in + 3, in + 10, x + 13);
         }
+        s->inits.resize(s->N);
+        for (int i = 0; i < s->N; ++i) {
+            const double* m = d->material + 6 * (size_t)i;
+            ParticleInit& pi = s->inits[i];
+            pi.scale = 1.0; pi.density = m[0]; pi.young = m[1]; pi.poisson = m[2]; pi.mu = m[3]; pi.rest = m[4];
+        }
         sim_setup_device(s);
         std::vector<WallHost> walls;
         for (int w = 0; w < d->n_walls; ++w) {
@@ -1020,55 +1027,22 @@ zdx_status zdx_sim_load_config(zdx_ctx* c, const char* path, const char* base_di
         c->err = std::string("failed to parse config ") + path;
         return ZDX_IO;
     }
-    auto resolve = [&](const std::string& p) {
-        if (!base_dir || p.empty() || p[0] == '/') return p;
-        std::string b(base_dir);
-        if (!b.empty() && b.back() != '/') b.push_back('/');
-        return b + p;
-    };
     std::vector<int32_t> mesh_ids;
     std::vector<double> init, mat;
     std::vector<int32_t> wmesh;
     std::vector<double> wpose, wmat;
+    ConfigScene sc;
+    std::string err;
+    if (!build_scene(cfg, base_dir ? base_dir : "", sc, err)) {
+        c->err = err;
+        return err.rfind("failed to load", 0) == 0 ? ZDX_IO : ZDX_BAD_ARG;
+    }
     zdx_status st = guard(c, [&] {
-        // mesh cache keyed by path|to_string(scale), main.cpp:1887-1914
-        std::map<std::string, std::vector<double>> tcache;
-        std::map<std::string, int> mcache;
-        auto load_for = [&](const std::string& p, double scale) -> int {
-            std::string key = p + "|" + std::to_string(scale);
-            auto it = mcache.find(key);
-            if (it != mcache.end()) return it->second;
-            auto jt = tcache.find(p);
-            if (jt == tcache.end()) {
-                std::vector<double> t;
-                if (!load_stl(resolve(p), t)) throw ApiError{ZDX_IO, "failed to load STL " + p};
-                jt = tcache.emplace(p, std::move(t)).first;
-            }
-            std::vector<double> t = jt->second;
-            for (auto& v : t) v *= scale;
-            int id = c->add_mesh(build_mesh(t, cfg.center));
-            mcache[key] = id;
-            return id;
-        };
-        std::vector<ParticleInit> inits = cfg.parts;
-        if (inits.empty()) {
-            inits.resize(cfg.n);
-            for (int i = 0; i < cfg.n; ++i) {
-                inits[i].stl = cfg.stl;
-                inits[i].pos[0] = cfg.spacing * i;
-                const double* v = (i == 0) ? cfg.v0 : (i == 1 ? cfg.v1 : nullptr);
-                for (int k = 0; k < 3; ++k) inits[i].vel[k] = v ? v[k] : 0.0;
-                inits[i].scale = 1.0; inits[i].density = 1.0; inits[i].young = 1e7; inits[i].poisson = 0.25;
-                inits[i].mu = 0.5; inits[i].rest = 0.5;
-            }
-        }
-        for (const auto& in : inits) {
-            std::string sp = in.stl.empty() ? cfg.stl : in.stl;
-            if (sp.empty()) throw ApiError{ZDX_BAD_ARG, "particle without stl"};
-            double sc = in.scale > 0.0 ? in.scale : 1.0;
-            int id = load_for(sp, sc);
-            if (c->meshes[id].h.verts.empty()) throw ApiError{ZDX_BAD_ARG, "mesh has no vertices"};
-            mesh_ids.push_back(id);
+        std::vector<int> ctx_id(sc.meshes.size());
+        for (size_t m = 0; m < sc.meshes.size(); ++m) ctx_id[m] = c->add_mesh(std::move(sc.meshes[m]));
+        for (size_t i = 0; i < sc.inits.size(); ++i) {
+            const ParticleInit& in = sc.inits[i];
+            mesh_ids.push_back(ctx_id[sc.mesh_of[i]]);
             init.insert(init.end(), in.pos, in.pos + 3);
             init.insert(init.end(), in.quat, in.quat + 4);
             init.insert(init.end(), in.vel, in.vel + 3);
@@ -1076,15 +1050,9 @@ zdx_status zdx_sim_load_config(zdx_ctx* c, const char* path, const char* base_di
             double m[6] = {in.density, in.young, in.poisson, in.mu, in.rest, 0.0};
             mat.insert(mat.end(), m, m + 6);
         }
-        // walls: main.cpp:1990-2022 (a wall whose STL fails is skipped)
-        for (const auto& wi : cfg.walls) {
-            if (wi.stl.empty()) continue;
-            std::vector<double> t;
-            if (!load_stl(resolve(wi.stl), t)) continue;
-            double sc = wi.scale > 0.0 ? wi.scale : 1.0;
-            for (auto& v : t) v = v * sc;
-            int id = c->add_mesh(build_mesh(t, cfg.center));
-            wmesh.push_back(id);
+        for (size_t w = 0; w < sc.walls.size(); ++w) {
+            const WallInit& wi = sc.walls[w];
+            wmesh.push_back(c->add_mesh(std::move(sc.wall_meshes[w])));
             wpose.insert(wpose.end(), wi.pos, wi.pos + 3);
             wpose.insert(wpose.end(), wi.quat, wi.quat + 4);
             wmat.push_back(wi.mu);
@@ -1092,6 +1060,7 @@ zdx_status zdx_sim_load_config(zdx_ctx* c, const char* path, const char* base_di
         }
     });
     if (st != ZDX_OK) return st;
+    std::vector<ParticleInit>& inits = sc.inits;
     zdx_sim_desc d{};
     d.n_particles = (int32_t)mesh_ids.size();
     d.mesh_id = mesh_ids.data();
@@ -1105,7 +1074,10 @@ zdx_status zdx_sim_load_config(zdx_ctx* c, const char* path, const char* base_di
     d.dt = cfg.dt;
     d.split_contacts = cfg.split ? 1 : 0;
     st = zdx_sim_create(c, &d, out);
-    if (st == ZDX_OK) (*out)->cfg = cfg;
+    if (st == ZDX_OK) {
+        (*out)->cfg = cfg;
+        (*out)->inits = inits;
+    }
     return st;
 }
 
@@ -1150,6 +1122,40 @@ zdx_status zdx_sim_get_forces(zdx_sim* s, double* F, double* T, int32_t* cnt) {
         if (F) HIPC(hipMemcpy(F, s->d_F.p, 3 * (size_t)s->N * 8, hipMemcpyDeviceToHost));
         if (T) HIPC(hipMemcpy(T, s->d_T.p, 3 * (size_t)s->N * 8, hipMemcpyDeviceToHost));
         if (cnt) HIPC(hipMemcpy(cnt, s->d_counts.p, (size_t)s->N * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+zdx_status zdx_sim_write_vtk(zdx_sim* s, const char* path) {
+    if (!s || !path) return ZDX_BAD_ARG;
+    return guard(s->ctx, [&] {
+        const size_t n = (size_t)s->N;
+        std::vector<double> state(16 * n), F(3 * n), T(3 * n), mass(n), radius(n);
+        std::vector<int32_t> cnt(n);
+        std::vector<const HostMesh*> mesh(n);
+        HIPC(hipStreamSynchronize(s->st()));
+        HIPC(hipMemcpy(state.data(), s->d_state[s->parity].p, state.size() * 8, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(F.data(), s->d_F.p, F.size() * 8, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(T.data(), s->d_T.p, T.size() * 8, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(cnt.data(), s->d_counts.p, cnt.size() * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            mass[i] = s->pc[i].mass;
+            radius[i] = s->pc[i].radius;
+            mesh[i] = &s->ctx->meshes[s->mesh_of[i]].h;
+        }
+        if (!write_vtk_particles(path, (int)n, mesh.data(), state.data(), mass.data(), radius.data(), F.data(),
+                                 T.data(), cnt.data()))
+            throw ApiError{ZDX_IO, std::string("failed to write VTK ") + path};
+    });
+}
+
+zdx_status zdx_sim_write_state_txt(zdx_sim* s, const char* path) {
+    if (!s || !path) return ZDX_BAD_ARG;
+    return guard(s->ctx, [&] {
+        std::vector<double> state(16 * (size_t)s->N);
+        HIPC(hipStreamSynchronize(s->st()));
+        HIPC(hipMemcpy(state.data(), s->d_state[s->parity].p, state.size() * 8, hipMemcpyDeviceToHost));
+        if (!write_particle_state_txt(path, s->N, state.data(), s->inits.data()))
+            throw ApiError{ZDX_IO, std::string("failed to write particle state ") + path};
     });
 }
 

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <map>
 #include <limits>
 #include <sstream>
 #include <unordered_map>
@@ -317,6 +318,66 @@ bool parse_config(const std::string& path, SimConfig& c) {
     if (c.output_interval < 1) c.output_interval = 1;
     if (!c.stl.empty()) return true;
     return !c.parts.empty();
+}
+
+bool build_scene(const SimConfig& cfg, const std::string& base_dir, ConfigScene& sc, std::string& err) {
+    auto resolve = [&](const std::string& p) {
+        if (base_dir.empty() || p.empty() || p[0] == '/') return p;
+        std::string b = base_dir;
+        if (b.back() != '/') b.push_back('/');
+        return b + p;
+    };
+    std::map<std::string, std::vector<double>> tcache;   // per path
+    std::map<std::string, int> mcache;                   // per path|scale
+    auto load_for = [&](const std::string& p, double scale) -> int {
+        const std::string key = p + "|" + std::to_string(scale);
+        auto it = mcache.find(key);
+        if (it != mcache.end()) return it->second;
+        auto jt = tcache.find(p);
+        if (jt == tcache.end()) {
+            std::vector<double> t;
+            if (!load_stl(resolve(p), t)) return -1;
+            jt = tcache.emplace(p, std::move(t)).first;
+        }
+        std::vector<double> t = jt->second;
+        for (auto& v : t) v *= scale;
+        sc.meshes.push_back(build_mesh(t, cfg.center));
+        const int id = (int)sc.meshes.size() - 1;
+        mcache[key] = id;
+        return id;
+    };
+    sc = ConfigScene{};
+    sc.inits = cfg.parts;
+    if (sc.inits.empty()) {
+        sc.inits.resize(cfg.n > 0 ? cfg.n : 0);
+        for (int i = 0; i < cfg.n; ++i) {
+            ParticleInit& in = sc.inits[i];
+            in.stl = cfg.stl;
+            in.pos[0] = cfg.spacing * i;
+            const double* v = (i == 0) ? cfg.v0 : (i == 1 ? cfg.v1 : nullptr);
+            for (int k = 0; k < 3; ++k) in.vel[k] = v ? v[k] : 0.0;
+            in.scale = 1.0; in.density = 1.0; in.young = 1e7; in.poisson = 0.25; in.mu = 0.5; in.rest = 0.5;
+        }
+    }
+    for (const auto& in : sc.inits) {
+        const std::string p = in.stl.empty() ? cfg.stl : in.stl;
+        if (p.empty()) { err = "particle without stl"; return false; }
+        const int id = load_for(p, in.scale > 0.0 ? in.scale : 1.0);
+        if (id < 0) { err = "failed to load STL " + p; return false; }
+        if (sc.meshes[id].verts.empty()) { err = "mesh has no vertices: " + p; return false; }
+        sc.mesh_of.push_back(id);
+    }
+    // walls (main.cpp:1990-2022): a wall whose STL does not load is skipped
+    for (const auto& wi : cfg.walls) {
+        if (wi.stl.empty()) continue;
+        std::vector<double> t;
+        if (!load_stl(resolve(wi.stl), t)) continue;
+        const double s = wi.scale > 0.0 ? wi.scale : 1.0;
+        for (auto& v : t) v = v * s;
+        sc.wall_meshes.push_back(build_mesh(t, cfg.center));
+        sc.walls.push_back(wi);
+    }
+    return true;
 }
 
 namespace {

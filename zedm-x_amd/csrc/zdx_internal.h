@@ -1,6 +1,5 @@
 // zdx_internal.h — data layouts shared by the host code and the HIP kernels.
 #pragma once
-#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include <string>
@@ -52,6 +51,27 @@ struct SimConfig {
     std::vector<WallInit> walls;
 };
 bool parse_config(const std::string& path, SimConfig& cfg);
+
+// The particles and walls a config describes (src/main.cpp:1887-2022): one
+// mesh per (stl path, scale) as the reference's `path|to_string(scale)` cache,
+// the per-particle inits (the n/stl/spacing/v0/v1 form expanded as the
+// reference does), and the walls whose STL loads.  Relative paths are
+// resolved against base_dir when it is not empty.
+struct ConfigScene {
+    std::vector<HostMesh> meshes;
+    std::vector<int32_t> mesh_of;          // per particle, index into meshes
+    std::vector<ParticleInit> inits;       // per particle, as configured
+    std::vector<HostMesh> wall_meshes;     // per loaded wall (scaled, centred like particles)
+    std::vector<WallInit> walls;           // per loaded wall
+};
+bool build_scene(const SimConfig& cfg, const std::string& base_dir, ConfigScene& scene, std::string& err);
+
+// per-step output files (src/main.cpp:1710-1870); mesh[i] is particle i's mesh,
+// state rows are 16 doubles (pos quat vel omega L)
+bool write_vtk_particles(const std::string& path, int n, const HostMesh* const* mesh, const double* state,
+                         const double* mass, const double* radius, const double* F, const double* T,
+                         const int32_t* counts);
+bool write_particle_state_txt(const std::string& path, int n, const double* state, const ParticleInit* init);
 
 // per-particle constants derived at set-up (src/main.cpp:1955-1985)
 struct ParticleConst {

@@ -1,5 +1,7 @@
 // zdx_kernels.h — host-side launchers of the HIP kernels (zdx_kernels.hip).
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include "zdx_internal.h"
 
 namespace zdx {

@@ -23,7 +23,8 @@ COUNTERS = ["contacts", "tri_checks", "segments", "comps", "loops", "pairs", "wa
             "tri_scanned", "topo_pairs", "seg_max"]
 EXPORTS = ["zdx_create", "zdx_destroy", "zdx_last_error", "zdx_load_stl", "zdx_mesh_register", "zdx_contact_pairs",
            "zdx_sim_load_config", "zdx_sim_create", "zdx_sim_destroy", "zdx_sim_num_particles", "zdx_sim_step",
-           "zdx_sim_get_state", "zdx_sim_set_state", "zdx_sim_get_forces", "zdx_sim_checkpoint", "zdx_sim_restore",
+           "zdx_sim_get_state", "zdx_sim_set_state", "zdx_sim_get_forces", "zdx_sim_write_vtk",
+           "zdx_sim_write_state_txt", "zdx_sim_checkpoint", "zdx_sim_restore",
            "zdx_sim_stage_times", "zdx_sim_set_tiles", "zdx_sim_autotune"]
 LOOP_REC = 28
 
@@ -86,6 +87,8 @@ def lib():
         L.zdx_sim_get_state.argtypes = [vp, vp]
         L.zdx_sim_set_state.argtypes = [vp, vp]
         L.zdx_sim_get_forces.argtypes = [vp, vp, vp, vp]
+        L.zdx_sim_write_vtk.argtypes = [vp, C.c_char_p]
+        L.zdx_sim_write_state_txt.argtypes = [vp, C.c_char_p]
         L.zdx_sim_checkpoint.argtypes = [vp, C.c_char_p]
         L.zdx_sim_restore.argtypes = [vp, C.c_char_p]
         L.zdx_sim_stage_times.argtypes = [vp, vp, i32]
@@ -257,6 +260,14 @@ class Sim:
         c = np.zeros(self.n, dtype=np.int32)
         self.ctx._check(self.ctx._L.zdx_sim_get_forces(self.h, F.ctypes.data, T.ctypes.data, c.ctypes.data))
         return F, T, c
+
+    def write_vtk(self, path: str) -> None:
+        """The reference's per-step particle VTK (write_vtk_particles)."""
+        self.ctx._check(self.ctx._L.zdx_sim_write_vtk(self.h, path.encode()))
+
+    def write_state_txt(self, path: str) -> None:
+        """The reference's particle state file (write_particle_state_txt)."""
+        self.ctx._check(self.ctx._L.zdx_sim_write_state_txt(self.h, path.encode()))
 
     def checkpoint(self, path: str) -> None:
         self.ctx._check(self.ctx._L.zdx_sim_checkpoint(self.h, path.encode()))
