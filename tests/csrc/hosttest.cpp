@@ -2,7 +2,7 @@
 // for CPU-side tests: the same source the GPU kernels run, compiled with g++
 // -ffp-contract=off, compared against the oracle in tests/test_host_geom.py.
 #define ZDX_HD
-#include "../../zedm-x_amd/csrc/zdx_geom.h"
+#include "../../zedm-x_amd/csrc/zdx_topo.h"
 
 #include <vector>
 
@@ -118,15 +118,8 @@ int ht_pair(const double* trisA, int TA, const double* trisB, int TB, const doub
     double tol = mean_edge_a > 0.0 ? (mean_edge_a * 0.1) : (bbox_diag_a * 1e-2);
     if (tol <= 0.0) tol = 1e-6;
     double area_eps = mean_edge_a > 0.0 ? (mean_edge_a * mean_edge_a * 1e-4) : 1e-12;
-    TopoCaps caps;
-    caps.S = S > 0 ? S : 1;
-    caps.E2 = 4 * caps.S;
-    std::vector<int> Iv(caps.n_ints());
-    std::vector<double> Dv(caps.n_dbls());
     RecSink sink{rec, rec_cap, vtx, vcap, 0};
-    LaneHost wp;
-    TopoOut t = topology(wp, seg.data(), S, tol, area_eps, split != 0, sub(pb, pa), SArr<int>{Iv.data(), 1},
-                         SArr<double>{Dv.data(), 1}, caps, sink);
+    TopoOut t = tp_topology_host(seg.data(), S, tol, area_eps, split != 0, sub(pb, pa), sink);
     counts[0] = nondeg * TB;
     counts[1] = S;
     counts[2] = S > 0 ? t.comps : 0;

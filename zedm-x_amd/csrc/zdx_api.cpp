@@ -19,6 +19,7 @@
 #include "zdx_geom.h"
 #include "zdx_internal.h"
 #include "zdx_kernels.h"
+#include "zdx_topo.h"
 
 using namespace zdx;
 
@@ -72,6 +73,66 @@ struct MeshRec {
     double tol, area_eps;
 };
 
+// Device buffers of the phased topology (zdx_topo.h), used by the time step
+// and by the per-pair API: per arena segment (cells, components, vertices,
+// edges), per work-list item (offsets, component counts), the bump-allocated
+// adjacency / snap / T-junction buffers and the walks' global scratch slices.
+struct TopoWork {
+    DBuf<int32_t> cell9, comp, vcell, E, ncomp;
+    DBuf<double> V;
+    DBuf<uint8_t> adj;
+    DBuf<uint64_t> snapw, tjw;
+    DBuf<unsigned long long> top;
+    DBuf<int64_t> adj_off, snap_off, tj_off;
+    DBuf<int32_t> scr_i;
+    DBuf<double> scr_d;
+    int64_t adj_cap = 1 << 22, snap_cap = 1 << 17, tj_cap = 1 << 17;
+    int64_t scr_ni = 0, scr_nd = 0;
+
+    void ensure_arena(int64_t segs) {
+        cell9.ensure(9 * (size_t)segs); comp.ensure(TPC_N * (size_t)segs); V.ensure(6 * (size_t)segs);
+        vcell.ensure(6 * (size_t)segs); E.ensure(2 * (size_t)segs);
+    }
+    void ensure_items(int64_t items) {
+        adj_off.ensure(items); snap_off.ensure(items); tj_off.ensure(items); ncomp.ensure(items);
+    }
+    void ensure_slices(int64_t slices, int32_t seg_max) {
+        scr_ni = tp_walk_ints(seg_max);
+        scr_nd = tp_walk_dbls(seg_max);
+        scr_i.ensure((size_t)(scr_ni * slices));
+        scr_d.ensure((size_t)(scr_nd * slices));
+    }
+    void ensure_bump() {
+        adj.ensure((size_t)adj_cap); snapw.ensure((size_t)snap_cap); tjw.ensure((size_t)tj_cap); top.ensure(3);
+    }
+    // After a step whose pass A or walk B ran out of room: the bump counters
+    // hold what every pair asked for.  Returns true if a buffer was grown.
+    bool grow_from_counters(hipStream_t st) {
+        unsigned long long t[3] = {0, 0, 0};
+        HIPC(hipMemcpyAsync(t, top.p, sizeof(t), hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        bool g = false;
+        auto grow = [&](int64_t& cap, unsigned long long need) {
+            if ((int64_t)need > cap) { cap = (int64_t)need + (int64_t)need / 4 + 1024; g = true; }
+        };
+        grow(adj_cap, t[0]);
+        grow(snap_cap, t[1]);
+        grow(tj_cap, t[2]);
+        if (g) {
+            adj.release(); snapw.release(); tjw.release();
+            ensure_bump();
+        }
+        return g;
+    }
+    void fill(TopoBufs& T) {
+        ensure_bump();
+        T.cell9 = cell9.p; T.adj = adj.p; T.adj_cap = adj_cap; T.snapw = snapw.p; T.snap_cap = snap_cap;
+        T.tjw = tjw.p; T.tj_cap = tj_cap; T.top = top.p; T.adj_off = adj_off.p; T.snap_off = snap_off.p;
+        T.tj_off = tj_off.p; T.ncomp = ncomp.p; T.comp = comp.p; T.V = V.p; T.vcell = vcell.p; T.E = E.p;
+        T.scr_i = scr_i.p; T.scr_d = scr_d.p; T.scr_ni = scr_ni; T.scr_nd = scr_nd;
+    }
+};
+
 }  // namespace
 
 struct zdx_ctx {
@@ -107,8 +168,7 @@ struct zdx_ctx {
         DBuf<unsigned int> queue;
         DBuf<int32_t> bcnt, bcur;
         DBuf<int32_t> ncontrib, comps, loops_raw, tstatus, topo_list, topo_n;
-        DBuf<int32_t> scr_i;
-        DBuf<double> scr_d;
+        TopoWork tw;
         DBuf<double> rec, vtx;
         int32_t stage_cap = 1024;
         int64_t arena_cap = 1 << 20;
@@ -247,11 +307,11 @@ struct zdx_sim {
     // topology
     DBuf<double> d_contrib;
     DBuf<int32_t> d_ncontrib, d_comps, d_loops_raw, d_tstatus;
-    DBuf<int32_t> d_scr_i;
-    DBuf<double> d_scr_d;
+    TopoWork tw;
     int32_t lmax = 8;
-    int64_t topo_threads = 2048;   // persistent one-wave workgroups
-    int32_t topo_lds = 20 * 1024;  // LDS scratch per workgroup (pairs up to ~50 segments)
+    int64_t topo_threads = 2048;   // persistent one-wave workgroups of the topology walks
+    int64_t topo_pass_groups = 1024;  // persistent 256-thread workgroups of the topology passes
+    int32_t topo_lds = 32 * 1024;  // LDS scratch per walk workgroup (pairs up to ~120 segments)
     int32_t seg_max = 512;
     // walls
     DBuf<double> d_wtris, d_wnorm, d_wpos, d_wrad, d_wmu;
@@ -339,8 +399,9 @@ struct zdx_sim {
     TopoBufs topobufs() {
         TopoBufs T{};
         T.lmax = lmax; T.contrib = d_contrib.p; T.ncontrib = d_ncontrib.p; T.comps = d_comps.p;
-        T.loops_raw = d_loops_raw.p; T.tstatus = d_tstatus.p; T.scr_i = d_scr_i.p; T.scr_d = d_scr_d.p;
-        T.nthreads = topo_threads; T.seg_max = seg_max; T.lds_bytes = topo_lds;
+        T.loops_raw = d_loops_raw.p; T.tstatus = d_tstatus.p;
+        tw.fill(T);
+        T.nthreads = topo_threads; T.pass_groups = topo_pass_groups; T.seg_max = seg_max; T.lds_bytes = topo_lds;
         return T;
     }
     WallBufs wallbufs() {
@@ -387,6 +448,7 @@ struct zdx_sim {
         d_nseg.ensure(cap); d_tri_hits.ensure(cap); d_tri_scanned.ensure(cap);
         d_ncontrib.ensure(cap); d_comps.ensure(cap); d_loops_raw.ensure(cap); d_tstatus.ensure(cap);
         d_contrib.ensure((size_t)cap * lmax * 9);
+        tw.ensure_items(cap);
     }
     void alloc_lmax(int32_t l) {
         drop_graphs();
@@ -399,6 +461,7 @@ struct zdx_sim {
         arena_cap = cap;
         d_arena.release();
         d_arena.ensure((size_t)cap * 6);
+        tw.ensure_arena(cap);
     }
     void alloc_stage(int32_t cap) {
         drop_graphs();
@@ -411,10 +474,8 @@ struct zdx_sim {
     }
     void alloc_topo_scratch() {
         drop_graphs();
-        TopoCaps c; c.S = seg_max; c.E2 = 4 * seg_max;
-        d_scr_i.release(); d_scr_d.release();
-        d_scr_i.ensure((size_t)c.n_ints() * topo_threads);
-        d_scr_d.ensure((size_t)c.n_dbls() * topo_threads);
+        tw.scr_i.release(); tw.scr_d.release();
+        tw.ensure_slices(topo_threads, seg_max);
     }
     int32_t transform_groups() const { return std::max(1, ctx->n_cu * 8); }
 };
@@ -476,6 +537,7 @@ void sim_setup_device(zdx_sim* s) {
     s->d_list_f.ensure(groups * 12 * (size_t)s->tmax);
     s->topo_threads = std::min<int64_t>(2048, std::max<int64_t>(64, (int64_t)N));
     s->alloc_stage(s->stage_cap);
+    s->tw.ensure_bump();   // before any graph capture
     // particle/wall histories start empty
     for (int k = 0; k < 2; ++k) {
         int32_t z = 0;
@@ -595,6 +657,7 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
         HIPC(hipMemcpyAsync(&nt, s->d_topo_n.p, 4, hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
         T.nthreads = std::max(1, std::min<int>(nt, g_topo));
+        T.pass_groups = std::max<int64_t>(1, std::min<int64_t>(nt, T.pass_groups));
     }
     HistBufs H = s->hist(par);
     launch_topo_zero(P, T, H.cur_active, s->pair_cap, st);
@@ -634,7 +697,12 @@ bool grow_for(zdx_sim* s, int32_t status, int par) {
         grown = true;
     }
     if (status & ST_TOPO) {
-        // find what overflowed: segments beyond the scratch, or loops beyond lmax
+        // find what overflowed: the bump-allocated topology buffers, segments
+        // beyond the walk scratch, or loops beyond lmax
+        if (s->tw.grow_from_counters(s->st())) {
+            s->drop_graphs();
+            grown = true;
+        }
         int32_t n = 0;
         HIPC(hipMemcpy(&n, s->ps[par].n.p, sizeof(int32_t), hipMemcpyDeviceToHost));
         std::vector<int32_t> ts(n), ns(n), nl(n);
@@ -649,9 +717,8 @@ bool grow_for(zdx_sim* s, int32_t status, int par) {
             if (ts[p] & TOPO_E2_OVERFLOW) smax = std::max(smax, ns[p]);
         }
         if (loops) { s->alloc_lmax(s->lmax * 2); grown = true; }
-        if (smax > 0) {
-            if (smax > s->seg_max) s->seg_max = smax;
-            else s->seg_max = s->seg_max * 2;
+        if (smax > s->seg_max) {
+            s->seg_max = smax;
             s->alloc_topo_scratch();
             grown = true;
         }
@@ -874,17 +941,20 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
             // topology with the minimal-energy force
             TopoBufs T{};
             T.lmax = A.lmax; T.ncontrib = A.ncontrib.p; T.comps = A.comps.p; T.loops_raw = A.loops_raw.p;
-            T.tstatus = A.tstatus.p; T.nthreads = std::min<int64_t>(4096, (int64_t)n);
+            T.tstatus = A.tstatus.p;
+            T.nthreads = std::min<int64_t>(4096, (int64_t)n);
+            T.pass_groups = std::min<int64_t>(2048, (int64_t)n);
             T.seg_max = A.stage_cap;
-            T.lds_bytes = 20 * 1024;
+            T.lds_bytes = 32 * 1024;
             T.vmax = 4 * A.stage_cap + 16;
             T.k_stiff = opts->k;
-            TopoCaps caps; caps.S = T.seg_max; caps.E2 = 4 * T.seg_max;
-            A.scr_i.ensure((size_t)caps.n_ints() * T.nthreads);
-            A.scr_d.ensure((size_t)caps.n_dbls() * T.nthreads);
+            A.tw.ensure_arena(A.arena_cap);
+            A.tw.ensure_items(n);
+            A.tw.ensure_slices(T.nthreads, T.seg_max);
+            A.tw.fill(T);
             A.rec.ensure((size_t)n * T.lmax * 28);
             if (want_vtx) A.vtx.ensure((size_t)n * T.vmax * 3);
-            T.scr_i = A.scr_i.p; T.scr_d = A.scr_d.p; T.rec = A.rec.p; T.vtx = want_vtx ? A.vtx.p : nullptr;
+            T.rec = A.rec.p; T.vtx = want_vtx ? A.vtx.p : nullptr;
             launch_topo_zero(P, T, nullptr, n, st);
             launch_topo_api(P, M, B, T, opts->split_contacts, st);
             HIPC(hipMemcpyAsync(&status, A.status.p, 4, hipMemcpyDeviceToHost, st));
@@ -893,9 +963,11 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
                 std::vector<int32_t> ts(n);
                 HIPC(hipMemcpy(ts.data(), A.tstatus.p, n * 4, hipMemcpyDeviceToHost));
                 bool lo = false, vo = false;
-                for (int p = 0; p < n; ++p) { lo |= (ts[p] & TOPO_LOOP_OVERFLOW) != 0; vo |= (ts[p] & (TOPO_VTX_OVERFLOW | TOPO_E2_OVERFLOW)) != 0; }
+                for (int p = 0; p < n; ++p) { lo |= (ts[p] & TOPO_LOOP_OVERFLOW) != 0; vo |= (ts[p] & TOPO_VTX_OVERFLOW) != 0; }
+                const bool grown = A.tw.grow_from_counters(st);
                 if (lo) A.lmax *= 2;
                 if (vo) A.stage_cap *= 2;
+                if (!lo && !vo && !grown) throw ApiError{ZDX_CAPACITY, "contact topology exceeded its edge capacity"};
                 continue;
             }
             // gather to the host and pack

@@ -297,20 +297,6 @@ ZDX_HD inline void pt_seg(const V3& p, const V3& a, const V3& b, double& d2, dou
 }
 
 // ------------------------------------------------------------ per-pair topology
-// Scratch sizes for a pair with at most S segments (caps checked at run time).
-struct TopoCaps {
-    int S;       // segments
-    int E2;      // edges after T-junction splitting (4*S)
-    // sizes of the int / double scratch regions laid out in topology()
-    ZDX_HD int n_ints() const {
-        const int S2 = 2 * S;
-        const int adj = (S * S + 3) / 4;            // S x S candidate bytes
-        return 9 * S + 4 * S + 2 * (S + 1) + 3 * S2 + 5 * S2 + 2 * S2 + 3 * S2 + 2 * E2 + 3 * (2 * S2) + 4 * E2 +
-               4 * S2 + E2 + E2 + 1 + adj + 8;
-    }
-    ZDX_HD int n_dbls() const { return 3 * (2 * S) + 2 * (2 * S) + 8; }
-};
-
 struct Split { double t; int v; int k; };   // k = reference scan-order key
 
 struct SplitArr {
@@ -360,439 +346,13 @@ struct LaneHost {
     int lane() const { return 0; }
     uint64_t ballot(bool p) const { return p ? 1ull : 0ull; }
     double shfl(double v, int) const { return v; }
+    uint64_t shfl64(uint64_t v, int) const { return v; }
     void sync() const {}
     void mark(int) const {}
 };
 
-ZDX_HD inline int ctz64(uint64_t m) {
-    int k = 0;
-    while (!(m & 1ull)) { m >>= 1; ++k; }
-    return k;
-}
-
-// Reference scan-order key of candidate j for segment i in
-// label_segments_by_contact_greedy (main.cpp:834-857): rep point r of i (p0,
-// p1, mid), neighbour cell (dx,dy,dz) lexicographic; smallest match wins.
-ZDX_HD inline int cand_key(SArr<int> cell9, int i, int j) {
-    int key = 1 << 30;
-    for (int r = 0; r < 3; ++r) {
-        int ax = cell9[9 * i + 3 * r], ay = cell9[9 * i + 3 * r + 1], az = cell9[9 * i + 3 * r + 2];
-        for (int q = 0; q < 3; ++q) {
-            int bx = cell9[9 * j + 3 * q], by = cell9[9 * j + 3 * q + 1], bz = cell9[9 * j + 3 * q + 2];
-            if (near1(ax, ay, az, bx, by, bz)) {
-                int k = r * 27 + (bx - ax + 1) * 9 + (by - ay + 1) * 3 + (bz - az + 1);
-                if (k < key) key = k;
-            }
-        }
-    }
-    return key;
-}
-
-// Per-pair pipeline after the narrow phase (main.cpp:2152-2219):
-// components (label_segments_by_contact_greedy 804-898, when split), then per
-// component snap_endpoints (680-746), split_t_junctions (900-961),
-// extract_loops (963-1025), and per loop accumulate_Sn_Gn_from_polyline
-// (1027-1040), contact_point_xc0 (1042-1052), the <3-vertex and area_eps
-// filters and the flip of nA toward B (2188-2219).
-//
-// The quadratic searches run as flat lane-parallel passes whose iterations do
-// not depend on each other: the component test of every ordered segment pair
-// (popped i, candidate j -- the order the reference evaluates it in), the
-// snap-distance test of every point pair, and the T-junction test of every
-// (edge, vertex) pair.  The order-dependent walks (DFS, vertex numbering,
-// split chains, loop tracing, sums) then run on their results.
-//
-// seg: S segments, 6 doubles each (p0, p1), in narrow-phase order.
-// I / D: int and double scratch of TopoCaps::n_ints() / n_dbls() entries,
-// contiguous (stride 1).
-// For every accepted loop, sink(loop_index, comp, raw_index, Sn, Gn, nA, xc0,
-// area, nverts, vids, V) is called (by every lane) in the reference's order.
-template <class WP, class Sink>
-ZDX_HD inline TopoOut topology(const WP& wp, const double* seg, int S, double tol, double area_eps, bool split,
-                               const V3& dirAB, SArr<int> I, SArr<double> D, const TopoCaps& cap, Sink& sink) {
-    TopoOut out; out.comps = 0; out.loops_raw = 0; out.loops = 0; out.status = TOPO_OK;
-    if (S <= 0) return out;
-    const int W = WP::W;
-    const int ln = wp.lane();
-    const double h = tol;
-    const double t2 = h * h;
-    const int S2 = 2 * S;
-    // ---- scratch layout
-    const int E2c = cap.E2;
-    const int HC = 2 * S2;                  // T-junction hit capacity
-    int o = 0;
-    SArr<int> cell9 = I.at(o); o += 9 * S;
-    SArr<int> vis = I.at(o); o += S;
-    SArr<int> stk = I.at(o); o += S;
-    SArr<int> cord = I.at(o); o += S;       // element order of all comps, concatenated
-    SArr<int> cidx = I.at(o); o += S;       // comp permutation (sorted)
-    SArr<int> cbeg = I.at(o); o += S + 1;   // comp start in cord
-    SArr<int> csz = I.at(o); o += S + 1;    // comp sizes
-    SArr<int> pcell = I.at(o); o += 3 * S2;
-    SArr<int> par = I.at(o); o += S2;
-    SArr<int> rnk = I.at(o); o += S2;
-    SArr<int> rootvid = I.at(o); o += S2;
-    SArr<int> ptv = I.at(o); o += S2;       // vid of each point
-    SArr<int> cnt = I.at(o); o += S2;
-    SArr<int> E = I.at(o); o += 2 * S2;     // snapped edges (a, b)
-    SArr<int> vcell3 = I.at(o); o += 3 * S2;
-    SArr<int> E2 = I.at(o); o += 2 * E2c;   // split edges (a, b)
-    SArr<int> spv = I.at(o); o += HC;       // T-junction hits: vertex, scan key, edge
-    SArr<int> spk = I.at(o); o += HC;
-    SArr<int> spe = I.at(o); o += HC;
-    SArr<int> adjn = I.at(o); o += 2 * E2c;
-    SArr<int> adje = I.at(o); o += 2 * E2c;
-    SArr<int> deg = I.at(o); o += S2;
-    SArr<int> aoff = I.at(o); o += S2;
-    SArr<int> afill = I.at(o); o += S2;
-    SArr<int> sord = I.at(o); o += S2;
-    SArr<int> used = I.at(o); o += E2c;
-    SArr<int> path = I.at(o); o += E2c + 1;
-    uint8_t* adjm = (uint8_t*)(I.p + o);    // S x S component-candidate bytes
-    int od = 0;
-    SArr<double> Vs = D.at(od); od += 3 * S2;
-    SArr<double> spt = D.at(od); od += HC;
-
-    auto P0 = [&](int s) { return v3(seg[6 * s + 0], seg[6 * s + 1], seg[6 * s + 2]); };
-    auto P1 = [&](int s) { return v3(seg[6 * s + 3], seg[6 * s + 4], seg[6 * s + 5]); };
-
-    // ---- components
-    int ncomp = 0;
-    if (split) {
-        for (int i = ln; i < S; i += W) {
-            V3 p0 = P0(i), p1 = P1(i);
-            V3 mid = mul(add(p0, p1), 0.5);
-            I3 a = cell_of(p0, h), b = cell_of(p1, h), c = cell_of(mid, h);
-            cell9[9 * i + 0] = a.x; cell9[9 * i + 1] = a.y; cell9[9 * i + 2] = a.z;
-            cell9[9 * i + 3] = b.x; cell9[9 * i + 4] = b.y; cell9[9 * i + 5] = b.z;
-            cell9[9 * i + 6] = c.x; cell9[9 * i + 7] = c.y; cell9[9 * i + 8] = c.z;
-            vis[i] = 0;
-        }
-        wp.sync();
-        // candidate test of every ordered pair (i popped, j candidate): in the
-        // grid neighbourhood, AABB(+tol) overlap, seg_seg_dist2 <= tol^2
-        const int SS = S * S;
-        for (int k = ln; k < SS; k += W) {
-            const int i = k / S, j = k - (k / S) * S;
-            bool acc = false;
-            if (j != i && cand_key(cell9, i, j) != (1 << 30)) {
-                V3 p1 = P0(i), q1 = P1(i), p2 = P0(j), q2 = P1(j);
-                V3 amn = v3(dmin(p1.x, q1.x) - h, dmin(p1.y, q1.y) - h, dmin(p1.z, q1.z) - h);
-                V3 amx = v3(dmax(p1.x, q1.x) + h, dmax(p1.y, q1.y) + h, dmax(p1.z, q1.z) + h);
-                V3 bmn = v3(dmin(p2.x, q2.x) - h, dmin(p2.y, q2.y) - h, dmin(p2.z, q2.z) - h);
-                V3 bmx = v3(dmax(p2.x, q2.x) + h, dmax(p2.y, q2.y) + h, dmax(p2.z, q2.z) + h);
-                bool ovl = !(amx.x < bmn.x || amn.x > bmx.x || amx.y < bmn.y || amn.y > bmx.y || amx.z < bmn.z ||
-                             amn.z > bmx.z);
-                acc = ovl && seg_seg(p1, q1, p2, q2) <= t2;
-            }
-            adjm[k] = acc ? 1 : 0;
-        }
-        wp.sync();
-        wp.mark(1);
-        int nord = 0;
-        for (int seed = 0; seed < S; ++seed) {
-            if (vis[seed]) continue;
-            cbeg[ncomp] = nord;
-            int sp = 0;
-            stk[sp++] = seed;
-            vis[seed] = 1;
-            cord[nord++] = seed;
-            while (sp > 0) {
-                const int i = stk[--sp];
-                // accepted candidates of i are pushed ordered by (scan key, id)
-                const int first_acc = sp;
-                for (int base = 0; base < S; base += W) {
-                    const int j = base + ln;
-                    const bool acc = j < S && adjm[i * S + j] && !vis[j];
-                    uint64_t m = wp.ballot(acc);
-                    while (m) {
-                        const int jj = base + ctz64(m);
-                        m &= m - 1;
-                        const int key = cand_key(cell9, i, jj);
-                        int pos = sp;
-                        while (pos > first_acc && spk[pos - 1] > key) {
-                            stk[pos] = stk[pos - 1];
-                            spk[pos] = spk[pos - 1];
-                            --pos;
-                        }
-                        stk[pos] = jj;
-                        spk[pos] = key;
-                        ++sp;
-                    }
-                }
-                for (int k = first_acc; k < sp; ++k) {
-                    vis[stk[k]] = 1;
-                    cord[nord++] = stk[k];
-                }
-            }
-            csz[ncomp] = nord - cbeg[ncomp];
-            ++ncomp;
-        }
-        for (int c = 0; c < ncomp; ++c) cidx[c] = c;
-        CompBySizeDesc cmp; cmp.size = csz;
-        ref_sort(cidx, ncomp, cmp);
-    } else {
-        for (int s = 0; s < S; ++s) cord[s] = s;
-        cbeg[0] = 0; csz[0] = S; cidx[0] = 0;
-        ncomp = 1;
-    }
-    out.comps = ncomp;
-    wp.sync();
-    wp.mark(2);
-
-    // ---- per component
-    for (int ci = 0; ci < ncomp; ++ci) {
-        const int c = cidx[ci];
-        const int base = cbeg[c];
-        const int ns = csz[c];
-        const int n = 2 * ns;
-        auto PT = [&](int k) { int s = cord[base + (k >> 1)]; return (k & 1) ? P1(s) : P0(s); };
-        // snap_endpoints: union-find over points within tol in +-1 cells; the
-        // partition does not depend on the order of the unions
-        for (int k = ln; k < n; k += W) {
-            I3 cc = cell_of(PT(k), h);
-            pcell[3 * k] = cc.x; pcell[3 * k + 1] = cc.y; pcell[3 * k + 2] = cc.z;
-            par[k] = k; rnk[k] = 0; rootvid[k] = -1;
-        }
-        wp.sync();
-        auto find = [&](int x) {
-            int r = x;
-            while (par[r] != r) r = par[r];
-            while (par[x] != r) { int nx = par[x]; par[x] = r; x = nx; }
-            return r;
-        };
-        wp.mark(3);
-        const int npairs = n * (n - 1) / 2;
-        for (int kb = 0; kb < npairs; kb += W) {
-            const int k = kb + ln;
-            bool hit = false;
-            int pi = 0, pj = 0;
-            if (k < npairs) {
-                // k -> (i, j), j < i, k = i (i - 1) / 2 + j
-                int i = (int)((1.0 + sqrt(1.0 + 8.0 * (double)k)) * 0.5);
-                while (i * (i - 1) / 2 > k) --i;
-                while ((i + 1) * i / 2 <= k) ++i;
-                pi = i;
-                pj = k - i * (i - 1) / 2;
-                hit = near1(pcell[3 * pi], pcell[3 * pi + 1], pcell[3 * pi + 2], pcell[3 * pj], pcell[3 * pj + 1],
-                            pcell[3 * pj + 2]) &&
-                      norm2(sub(PT(pi), PT(pj))) <= h * h;
-            }
-            uint64_t m = wp.ballot(hit);
-            while (m) {
-                const int src = ctz64(m);
-                m &= m - 1;
-                const int kk = kb + src;
-                int i = (int)((1.0 + sqrt(1.0 + 8.0 * (double)kk)) * 0.5);
-                while (i * (i - 1) / 2 > kk) --i;
-                while ((i + 1) * i / 2 <= kk) ++i;
-                const int j = kk - i * (i - 1) / 2;
-                int ra = find(i), rb = find(j);
-                if (ra != rb) {
-                    if (rnk[ra] < rnk[rb]) par[ra] = rb;
-                    else if (rnk[ra] > rnk[rb]) par[rb] = ra;
-                    else { par[rb] = ra; rnk[ra] += 1; }
-                }
-            }
-            (void)pi; (void)pj;
-        }
-        wp.mark(4);
-        int nv = 0;
-        for (int i = 0; i < n; ++i) {
-            int r = find(i);
-            V3 p = PT(i);
-            if (rootvid[r] < 0) {
-                rootvid[r] = nv;
-                Vs[3 * nv] = p.x; Vs[3 * nv + 1] = p.y; Vs[3 * nv + 2] = p.z;
-                cnt[nv] = 1;
-                ++nv;
-            } else {
-                int v = rootvid[r];
-                Vs[3 * v] += p.x; Vs[3 * v + 1] += p.y; Vs[3 * v + 2] += p.z;
-                cnt[v] += 1;
-            }
-            ptv[i] = rootvid[r];
-        }
-        for (int v = 0; v < nv; ++v) {
-            double cn = (double)cnt[v];
-            Vs[3 * v] = Vs[3 * v] / cn; Vs[3 * v + 1] = Vs[3 * v + 1] / cn; Vs[3 * v + 2] = Vs[3 * v + 2] / cn;
-        }
-        auto VV = [&](int v) { return v3(Vs[3 * v], Vs[3 * v + 1], Vs[3 * v + 2]); };
-        int ne = 0;
-        for (int s = 0; s < ns; ++s) {
-            int a = ptv[2 * s], b = ptv[2 * s + 1];
-            if (a != b) { E[2 * ne] = a; E[2 * ne + 1] = b; ++ne; }
-        }
-        // split_t_junctions: candidate test of every (edge, vertex) pair, hits
-        // collected edge-major in vertex order
-        for (int v = 0; v < nv; ++v) {
-            I3 cc = cell_of(VV(v), h);
-            vcell3[3 * v] = cc.x; vcell3[3 * v + 1] = cc.y; vcell3[3 * v + 2] = cc.z;
-        }
-        wp.mark(5);
-        const double tol2 = h * h;
-        int nh = 0;
-        const int ntest = ne * nv;
-        for (int kb = 0; kb < ntest; kb += W) {
-            const int k = kb + ln;
-            bool hit = false;
-            if (k < ntest) {
-                const int e = k / nv, v = k - (k / nv) * nv;
-                const int a = E[2 * e], b = E[2 * e + 1];
-                if (v != a && v != b) {
-                    const V3 pa = VV(a), pb = VV(b);
-                    V3 mn = v3(dmin(pa.x, pb.x) - h, dmin(pa.y, pb.y) - h, dmin(pa.z, pb.z) - h);
-                    V3 mx = v3(dmax(pa.x, pb.x) + h, dmax(pa.y, pb.y) + h, dmax(pa.z, pb.z) + h);
-                    const I3 c0 = cell_of(mn, h), c1 = cell_of(mx, h);
-                    int cx = vcell3[3 * v], cy = vcell3[3 * v + 1], cz = vcell3[3 * v + 2];
-                    if (!(cx < c0.x || cx > c1.x || cy < c0.y || cy > c1.y || cz < c0.z || cz > c1.z)) {
-                        double d2, t;
-                        pt_seg(VV(v), pa, pb, d2, t);
-                        hit = d2 <= tol2 && t > 1e-6 && t < 1.0 - 1e-6;
-                    }
-                }
-            }
-            uint64_t m = wp.ballot(hit);
-            while (m) {
-                const int kk = kb + ctz64(m);
-                m &= m - 1;
-                const int e = kk / nv, v = kk - (kk / nv) * nv;
-                const int a = E[2 * e], b = E[2 * e + 1];
-                const V3 pa = VV(a), pb = VV(b);
-                V3 mn = v3(dmin(pa.x, pb.x) - h, dmin(pa.y, pb.y) - h, dmin(pa.z, pb.z) - h);
-                V3 mx = v3(dmax(pa.x, pb.x) + h, dmax(pa.y, pb.y) + h, dmax(pa.z, pb.z) + h);
-                const I3 c0 = cell_of(mn, h), c1 = cell_of(mx, h);
-                double d2, t;
-                pt_seg(VV(v), pa, pb, d2, t);
-                if (nh >= HC) { out.status |= TOPO_E2_OVERFLOW; return out; }
-                // the reference scans cells ix, iy, iz over [c0, c1]; key = scan index
-                int cx = vcell3[3 * v], cy = vcell3[3 * v + 1], cz = vcell3[3 * v + 2];
-                int nyz = (c1.y - c0.y + 1) * (c1.z - c0.z + 1);
-                spt[nh] = t; spv[nh] = v; spe[nh] = e;
-                spk[nh] = (cx - c0.x) * nyz + (cy - c0.y) * (c1.z - c0.z + 1) + (cz - c0.z);
-                ++nh;
-            }
-        }
-        wp.mark(6);
-        int ne2 = 0;
-        int h0 = 0;
-        for (int e = 0; e < ne; ++e) {
-            const int a = E[2 * e], b = E[2 * e + 1];
-            int h1 = h0;
-            while (h1 < nh && spe[h1] == e) ++h1;
-            const int nsp = h1 - h0;
-            if (nsp == 0) {
-                if (ne2 + 1 > E2c) { out.status |= TOPO_E2_OVERFLOW; return out; }
-                E2[2 * ne2] = a; E2[2 * ne2 + 1] = b; ++ne2;
-                continue;
-            }
-            SplitArr sa; sa.t = spt.at(h0); sa.v = spv.at(h0); sa.k = spk.at(h0);
-            // bring candidates into the reference's scan order, then its std::sort by t
-            SplitByScan byscan;
-            for (int i2 = 1; i2 < nsp; ++i2) {
-                Split x = sa.get(i2);
-                int j2 = i2 - 1;
-                while (j2 >= 0 && byscan(x, sa.get(j2))) { sa.set(j2 + 1, sa.get(j2)); --j2; }
-                sa.set(j2 + 1, x);
-            }
-            SplitByT byt;
-            ref_sort(sa, nsp, byt);
-            int prevv = a, last = -1;
-            for (int k = 0; k < nsp; ++k) {
-                int v = spv[h0 + k];
-                if (v == last) continue;
-                last = v;
-                if (prevv != v) {
-                    if (ne2 + 1 > E2c) { out.status |= TOPO_E2_OVERFLOW; return out; }
-                    E2[2 * ne2] = prevv; E2[2 * ne2 + 1] = v; ++ne2;
-                }
-                prevv = v;
-            }
-            if (prevv != b) {
-                if (ne2 + 1 > E2c) { out.status |= TOPO_E2_OVERFLOW; return out; }
-                E2[2 * ne2] = prevv; E2[2 * ne2 + 1] = b; ++ne2;
-            }
-            h0 = h1;
-        }
-        wp.mark(7);
-        // extract_loops
-        if (nv > S2) { out.status |= TOPO_E2_OVERFLOW; return out; }
-        for (int v = 0; v < nv; ++v) deg[v] = 0;
-        for (int e = 0; e < ne2; ++e) { deg[E2[2 * e]] += 1; deg[E2[2 * e + 1]] += 1; }
-        int acc = 0;
-        for (int v = 0; v < nv; ++v) { aoff[v] = acc; afill[v] = acc; acc += deg[v]; }
-        for (int e = 0; e < ne2; ++e) {
-            int a = E2[2 * e], b = E2[2 * e + 1];
-            int pa = afill[a]++; adjn[pa] = b; adje[pa] = e;
-            int pb = afill[b]++; adjn[pb] = a; adje[pb] = e;
-        }
-        for (int e = 0; e < ne2; ++e) used[e] = 0;
-        for (int v = 0; v < nv; ++v) sord[v] = v;
-        StartOrder so; so.deg = deg;
-        ref_sort(sord, nv, so);
-        int nloops_here = 0;
-        for (int si = 0; si < nv; ++si) {
-            int sv = sord[si];
-            for (int k = aoff[sv]; k < aoff[sv] + deg[sv]; ++k) {
-                int ei = adje[k];
-                if (used[ei]) continue;
-                int plen = 0;
-                path[plen++] = sv;
-                int prev = sv, cur = adjn[k];
-                used[ei] = 1;
-                path[plen++] = cur;
-                int steps = 0;
-                bool closed = false;
-                while (steps < 100000) {
-                    steps += 1;
-                    if (cur == sv) { closed = plen > 1; break; }
-                    int pick = -1, first = -1;
-                    for (int m = aoff[cur]; m < aoff[cur] + deg[cur]; ++m) {
-                        if (used[adje[m]]) continue;
-                        if (first < 0) first = m;
-                        if (adjn[m] != prev) { pick = m; break; }
-                    }
-                    if (first < 0) break;
-                    if (pick < 0) pick = first;
-                    used[adje[pick]] = 1;
-                    prev = cur;
-                    cur = adjn[pick];
-                    if (plen > E2c) { out.status |= TOPO_E2_OVERFLOW; return out; }
-                    path[plen++] = cur;
-                }
-                if (!closed) continue;
-                --plen;  // drop the repeated start vertex
-                const int raw = out.loops_raw + nloops_here;
-                ++nloops_here;
-                if (plen < 3) continue;
-                // accumulate_Sn_Gn_from_polyline
-                V3 Sn = v3(0.0, 0.0, 0.0), Gn = v3(0.0, 0.0, 0.0);
-                for (int m = 0; m < plen; ++m) {
-                    V3 p0 = VV(path[m]);
-                    V3 p1 = VV(path[(m + 1) % plen]);
-                    Sn = add(Sn, mul(cross(p0, p1), 0.5));
-                    V3 dx = sub(p1, p0);
-                    double s = dot(p0, p1) + (1.0 / 3.0) * dot(dx, dx);
-                    Gn = add(Gn, mul(dx, -(1.0 / 3.0) * s));
-                }
-                // contact_point_xc0
-                double area = norm(Sn);
-                if (area < 1e-14) continue;
-                V3 nA = mul(Sn, -1.0 / area);
-                V3 xc = dvd(cross(nA, Gn), area);
-                if (area < area_eps) continue;
-                if (dot(nA, dirAB) < 0.0) nA = mul(nA, -1.0);
-                int st = sink(out.loops, ci, raw, Sn, Gn, nA, xc, area, plen, path, Vs);
-                out.status |= st;
-                out.loops += 1;
-            }
-        }
-        out.loops_raw += nloops_here;
-        wp.sync();
-        wp.mark(8);
-    }
-    return out;
-}
+// index of the lowest set bit (m != 0)
+ZDX_HD inline int ctz64(uint64_t m) { return __builtin_ctzll(m); }
 
 // ------------------------------------------------------------ walls
 struct V2 { double x, y; };

@@ -76,11 +76,31 @@ struct TopoBufs {
     int32_t* comps;            // per pair
     int32_t* loops_raw;        // per pair
     int32_t* tstatus;          // per pair
+    // phased topology (zdx_topo.h); "item" = index into the pair work list
+    int32_t* cell9;            // 9 per arena segment
+    uint8_t* adj;              // adjacency keys, S*S per pair (bump-allocated)
+    int64_t adj_cap;
+    uint64_t* snapw;           // snap hit words (bump-allocated)
+    int64_t snap_cap;
+    uint64_t* tjw;             // T-junction hit words (bump-allocated)
+    int64_t tj_cap;
+    unsigned long long* top;   // 3 bump counters: adjacency bytes, snap words, T-junction words
+    int64_t* adj_off;          // per item (-1: did not fit)
+    int64_t* snap_off;
+    int64_t* tj_off;
+    int32_t* ncomp;            // per item (-1: not evaluated)
+    int32_t* comp;             // TPC_N per arena segment
+    double* V;                 // 2 vertices (6 doubles) per arena segment
+    int32_t* vcell;            // 2 vertex cells (6 ints) per arena segment
+    int32_t* E;                // 1 edge (2 ints) per arena segment
+    // walk scratch: LDS of lds_bytes when a pair fits, else per-workgroup slices
     int32_t* scr_i;
     double* scr_d;
-    int64_t nthreads;          // persistent workgroups (one wave each); global scratch slices
-    int32_t seg_max;           // global scratch slice sized for this many segments
-    int32_t lds_bytes;         // dynamic LDS per workgroup for pairs that fit
+    int64_t scr_ni, scr_nd;    // slice sizes (for seg_max segments)
+    int64_t nthreads;          // persistent one-wave workgroups of the walks (= slices)
+    int64_t pass_groups;       // persistent 256-thread workgroups of the passes
+    int32_t seg_max;           // largest segment count the slices hold
+    int32_t lds_bytes;
     // API mode outputs
     double* rec;               // pair * lmax * 28
     double* vtx;               // pair * vmax * 3

@@ -26,6 +26,7 @@
 
 #include "zdx_geom.h"
 #include "zdx_kernels.h"
+#include "zdx_topo.h"
 
 namespace zdx {
 
@@ -737,72 +738,194 @@ __device__ inline Mat load_mat(const DMat& m, int i) {
     return r;
 }
 
-// One wave per pair (workgroup = 64 lanes); the pair's scratch lives in LDS
-// when it fits (dynamic shared memory of lds_bytes), otherwise in this
-// workgroup's slice of the global scratch.
-#ifdef ZDX_TOPO_PROFILE
-// Build option for profiling the topology: device clock ticks per phase of
-// topology(), summed over all pairs (phase 0: set-up before the call, 9: the
-// outputs after it), read back with zdx_topo_profile().
-__device__ unsigned long long g_topo_prof[16];
-#endif
+// Lane policy of the walks: one wave, all lanes on the same serial path.
 struct LaneDev {
     static constexpr int W = WAVE;
     __device__ int lane() const { return lane_id(); }
     __device__ uint64_t ballot(bool p) const { return __ballot(p); }
     __device__ double shfl(double v, int src) const { return __shfl(v, src, WAVE); }
-    __device__ void sync() const { __syncthreads(); }
-#ifdef ZDX_TOPO_PROFILE
-    mutable unsigned long long t0 = 0;
-    __device__ void start() const { t0 = wall_clock64(); }
-    __device__ void mark(int k) const {
-        const unsigned long long t = wall_clock64();
-        if (lane_id() == 0) atomicAdd(&g_topo_prof[k], t - t0);
-        t0 = t;
+    __device__ uint64_t shfl64(uint64_t v, int src) const {
+        const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, WAVE);
+        const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, WAVE);
+        return ((uint64_t)hi << 32) | lo;
     }
-#else
-    __device__ void start() const {}
-    __device__ void mark(int) const {}
-#endif
+    __device__ void sync() const { __syncthreads(); }
 };
 
-// Scratch of one pair: LDS when it fits, else this workgroup's global slice.
-// The two cases call topology() separately so each inlined copy addresses
-// its scratch directly (LDS instructions for the first).
-template <class Body>
-__device__ inline TopoOut with_topo_scratch(const TopoBufs& T, int S, char* lds, int lds_bytes, Body&& body) {
-    TopoCaps caps;
-    caps.S = S > 0 ? S : 1;
-    caps.E2 = 4 * caps.S;
-    const int64_t ni = caps.n_ints(), nd = caps.n_dbls();
-    const int64_t ibytes = (ni * 4 + 7) & ~(int64_t)7;
-    if (ibytes + nd * 8 <= lds_bytes) {
-        SArr<int> I; I.p = (int*)lds; I.s = 1;
-        SArr<double> D; D.p = (double*)(lds + ibytes); D.s = 1;
-        return body(I, D, caps);
+// ---- phased topology (zdx_topo.h): pass A, walk B, pass C, walk D.
+// Pass kernels: one 256-thread workgroup per pair of the work list
+// (persistent); walk kernels: one wave per pair, scratch in LDS when it fits,
+// else the workgroup's slice of the global scratch.
+
+// Pass A: representative-point cells, the S x S adjacency keys and the snap
+// hit words of every pair; the pair's adjacency bytes and snap words are
+// bump-allocated (a pair that does not fit gets offset -1 and the step is
+// retried with larger buffers: the counters keep the total needed).
+__global__ void __launch_bounds__(256) k_tp_pass_a(DPairs P, DMeshes M, DBodies B, TopoBufs T,
+                                                  unsigned long long* kt) {
+    __shared__ long long s_off[2];
+    kt_start(kt);
+    const int n = *P.topo_n;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const int p = P.topo_list[it];
+        const int S = P.nseg[p];
+        const int64_t sb = P.seg_off[p];
+        const double h = M.tol[B.mesh[P.pi[p]]];
+        const double* seg = P.arena + 6 * sb;
+        int* c9 = T.cell9 + 9 * sb;
+        const int nadj = S * S;
+        const int npt = S * (2 * S - 1);   // endpoint pairs of 2S points
+        const int nw = (npt + 63) >> 6;
+        if (threadIdx.x == 0) {
+            long long ao = (long long)atomicAdd(T.top + 0, (unsigned long long)nadj);
+            long long so = (long long)atomicAdd(T.top + 1, (unsigned long long)nw);
+            if (ao + nadj > T.adj_cap || so + nw > T.snap_cap) { ao = -1; so = -1; }
+            s_off[0] = ao;
+            s_off[1] = so;
+            T.adj_off[it] = ao;
+            T.snap_off[it] = so;
+        }
+        for (int k = threadIdx.x; k < S; k += blockDim.x) tp_cells(seg, k, h, c9 + 9 * k);
+        __syncthreads();
+        const long long ao = s_off[0], so = s_off[1];
+        if (ao >= 0) {
+            uint8_t* adj = T.adj + ao;
+            for (int k = threadIdx.x; k < nadj; k += blockDim.x) {
+                const int i = k / S, j = k - (k / S) * S;
+                adj[k] = (uint8_t)tp_adj_key(seg, c9, h, i, j);
+            }
+            uint64_t* w = T.snapw + so;
+            // each wave covers 64 consecutive pair numbers: one word per ballot
+            for (int kb = 0; kb < nw * 64; kb += blockDim.x) {
+                const int t = kb + threadIdx.x;
+                bool hit = false;
+                if (t < npt) {
+                    int a, b;
+                    tp_pair_of(t, a, b);
+                    hit = tp_snap_hit(seg, c9, h, a, b);
+                }
+                const uint64_t m = __ballot(hit);
+                if (lane_id() == 0 && (t >> 6) < nw) w[t >> 6] = m;
+            }
+        }
+        __syncthreads();
     }
-    TopoCaps full; full.S = T.seg_max; full.E2 = 4 * T.seg_max;
-    SArr<int> I; I.p = T.scr_i + (int64_t)blockIdx.x * full.n_ints(); I.s = 1;
-    SArr<double> D; D.p = T.scr_d + (int64_t)blockIdx.x * full.n_dbls(); D.s = 1;
-    return body(I, D, caps);
 }
 
-__global__ void __launch_bounds__(64) k_topo_sim(DPairs P, DMeshes M, DBodies B, DMat mat, HistBufs H, TopoBufs T,
-                                                 SimParams sp, int32_t lds_bytes, unsigned long long* kt) {
-    extern __shared__ char s_topo[];
-    kt_start(kt);
+// Walk scratch of one pair: LDS when it fits, else this workgroup's global
+// slice.  The two cases call the walk separately so each inlined copy
+// addresses its scratch directly (LDS instructions for the first).
+template <class Body>
+__device__ inline auto with_walk_scratch(const TopoBufs& T, int64_t ni, int64_t nd, char* lds, int lds_bytes,
+                                         Body&& body) {
+    const int64_t ib = (ni * 4 + 7) & ~(int64_t)7;
+    if (ib + nd * 8 <= lds_bytes) return body((int*)lds, (double*)(lds + ib));
+    return body(T.scr_i + (int64_t)blockIdx.x * T.scr_ni, T.scr_d + (int64_t)blockIdx.x * T.scr_nd);
+}
+
+// Walk B: components, snapped vertices and edges of each pair; allocates the
+// pair's T-junction words.
+__global__ void __launch_bounds__(64) k_tp_walk_b(DPairs P, DMeshes M, DBodies B, TopoBufs T, int32_t split,
+                                                  int32_t lds_bytes) {
+    extern __shared__ char s_walk[];
+    const int n = *P.topo_n;
+    LaneDev wp;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const int p = P.topo_list[it];
+        const int S = P.nseg[p];
+        const int64_t sb = P.seg_off[p];
+        const double h = M.tol[B.mesh[P.pi[p]]];
+        const long long ao = T.adj_off[it], so = T.snap_off[it];
+        int ncomp = -1, tjn = 0;
+        if (ao >= 0 && S <= T.seg_max) {
+            const double* seg = P.arena + 6 * sb;
+            ncomp = with_walk_scratch(T, tp_walk_b_ints(S), tp_walk_b_dbls(S), s_walk, lds_bytes,
+                                      [&](int* I, double* D) {
+                double* sgl = D + tp_b_dbls(S);   // local copy of the segments
+                for (int k = lane_id(); k < 6 * S; k += WAVE) sgl[k] = seg[k];
+                __syncthreads();
+                return tp_components_and_vertices(wp, sgl, S, h, split != 0, T.adj + ao, T.snapw + so, I, D,
+                                                  T.comp + TPC_N * sb, T.V + 6 * sb, T.vcell + 6 * sb,
+                                                  T.E + 2 * sb, &tjn);
+            });
+        }
+        if (lane_id() == 0) {
+            long long to = -1;
+            if (ncomp >= 0) {
+                to = (long long)atomicAdd(T.top + 2, (unsigned long long)tjn);
+                if (to + tjn > T.tj_cap) to = -1;
+            }
+            T.ncomp[it] = ncomp;
+            T.tj_off[it] = ncomp >= 0 ? to : -1;
+        }
+        __syncthreads();
+    }
+}
+
+// Pass C: T-junction hit words of every component; each wave covers 64
+// vertices of one edge row per ballot.
+__global__ void __launch_bounds__(256) k_tp_pass_c(DPairs P, DMeshes M, DBodies B, TopoBufs T) {
+    const int n = *P.topo_n;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const int ncomp = T.ncomp[it];
+        const long long to = T.tj_off[it];
+        if (ncomp <= 0 || to < 0) continue;
+        const int p = P.topo_list[it];
+        const int64_t sb = P.seg_off[p];
+        const double h = M.tol[B.mesh[P.pi[p]]];
+        const int* comp = T.comp + TPC_N * sb;
+        for (int q = 0; q < ncomp; ++q) {
+            const int* cr = comp + TPC_N * q;
+            const int nv = cr[TPC_NV], ne = cr[TPC_NE];
+            const double* V = T.V + 6 * sb + 3 * (int64_t)cr[TPC_VB];
+            const int* vc = T.vcell + 6 * sb + 3 * (int64_t)cr[TPC_VB];
+            const int* E = T.E + 2 * sb + 2 * (int64_t)cr[TPC_EB];
+            uint64_t* w = T.tjw + to + cr[TPC_TW];
+            const int rl = 64 * tp_row_words(nv);
+            const int items = ne * rl;
+            for (int kb = 0; kb < items; kb += blockDim.x) {
+                const int t = kb + threadIdx.x;
+                const int e = t / rl, v = t - (t / rl) * rl;
+                const bool hit = t < items && v < nv && tp_tj_hit(V, vc, E, h, e, v);
+                const uint64_t m = __ballot(hit);
+                if (lane_id() == 0 && t < items) w[t >> 6] = m;
+            }
+        }
+    }
+}
+
+// Walk D with its inputs copied to the scratch: edges, vertex cells.
+template <class Sink>
+__device__ inline TopoOut tp_walk_d(const LaneDev& wp, const TopoBufs& T, int S, int ncomp, int64_t sb, long long to,
+                                    double h, double area_eps, const V3& dir, char* lds, int lds_bytes, Sink& sink) {
+    return with_walk_scratch(T, tp_walk_d_ints(S), tp_walk_d_dbls(S), lds, lds_bytes, [&](int* I, double* D) {
+        int* El = I + tp_d_ints(S);     // 2S
+        int* vcl = El + 2 * S;          // 6S
+        const int* comp = T.comp + TPC_N * sb;
+        const int ne_all = comp[TPC_N * (ncomp - 1) + TPC_EB] + comp[TPC_N * (ncomp - 1) + TPC_NE];
+        const int nv_all = comp[TPC_N * (ncomp - 1) + TPC_VB] + comp[TPC_N * (ncomp - 1) + TPC_NV];
+        for (int k = lane_id(); k < 2 * ne_all; k += WAVE) El[k] = T.E[2 * sb + k];
+        for (int k = lane_id(); k < 3 * nv_all; k += WAVE) vcl[k] = T.vcell[6 * sb + k];
+        __syncthreads();
+        return tp_loops(wp, S, ncomp, h, area_eps, dir, comp, T.V + 6 * sb, vcl, El, T.tjw + to, I, D, sink);
+    });
+}
+
+__global__ void __launch_bounds__(64) k_tp_walk_d_sim(DPairs P, DMeshes M, DBodies B, DMat mat, HistBufs H,
+                                                      TopoBufs T, SimParams sp, int32_t lds_bytes,
+                                                      unsigned long long* kt) {
+    extern __shared__ char s_walk[];
     const int n = *P.topo_n;
     const int nprev = *H.prev_n;
     LaneDev wp;
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
-        wp.start();
         const int p = P.topo_list[it];
         const int i = P.pi[p], j = P.pj[p];
         const int S = P.nseg[p];
         const int mA = B.mesh[i];
-        Body a = load_body(B.state + 16 * (int64_t)i);
-        Body b = load_body(B.state + 16 * (int64_t)j);
-        Mat ma = load_mat(mat, i), mb = load_mat(mat, j);
+        const Body a = load_body(B.state + 16 * (int64_t)i);
+        const Body b = load_body(B.state + 16 * (int64_t)j);
+        const Mat ma = load_mat(mat, i), mb = load_mat(mat, j);
         // previous tangential spring of this pair (the reference's map entry,
         // erased when the pair was not in contact on the previous step)
         const int64_t key = ((int64_t)i << 32) | (int64_t)(uint32_t)j;
@@ -810,8 +933,8 @@ __global__ void __launch_bounds__(64) k_topo_sim(DPairs P, DMeshes M, DBodies B,
         {
             int lo = 0, hi = nprev - 1;
             while (lo <= hi) {
-                int mid = (lo + hi) >> 1;
-                int64_t k = H.prev_key[mid];
+                const int mid = (lo + hi) >> 1;
+                const int64_t k = H.prev_key[mid];
                 if (k == key) {
                     if (H.prev_active[mid]) ds = ld3(H.prev_ds + 3 * (int64_t)mid);
                     break;
@@ -820,29 +943,24 @@ __global__ void __launch_bounds__(64) k_topo_sim(DPairs P, DMeshes M, DBodies B,
             }
         }
         SimSink sink{T.contrib + 9 * (int64_t)p * T.lmax, T.lmax, &a, &b, &ma, &mb, sp.dt, &ds};
-        TopoOut to;
-        to.comps = 0; to.loops_raw = 0; to.loops = 0; to.status = 0;
-        if (S > T.seg_max) to.status = TOPO_E2_OVERFLOW;
-        else if (S > 0) {
-            const double* sg = P.arena + 6 * P.seg_off[p];
-            const double tl = M.tol[mA], ae = M.area_eps[mA];
-            const V3 dir = sub(b.pos, a.pos);
-            wp.mark(0);
-            to = with_topo_scratch(T, S, s_topo, lds_bytes, [&](SArr<int> I, SArr<double> D, const TopoCaps& caps) {
-                return topology(wp, sg, S, tl, ae, sp.split != 0, dir, I, D, caps, sink);
-            });
-        }
+        const int ncomp = T.ncomp[it];
+        const long long to = T.tj_off[it];
+        TopoOut o;
+        o.comps = 0; o.loops_raw = 0; o.loops = 0; o.status = 0;
+        if (ncomp < 0 || to < 0) o.status = TOPO_E2_OVERFLOW;
+        else if (ncomp > 0)
+            o = tp_walk_d(wp, T, S, ncomp, P.seg_off[p], to, M.tol[mA], M.area_eps[mA], sub(b.pos, a.pos), s_walk,
+                          lds_bytes, sink);
         if (lane_id() == 0) {
-            T.ncontrib[p] = to.loops < T.lmax ? to.loops : T.lmax;
-            T.comps[p] = to.comps;
-            T.loops_raw[p] = to.loops_raw;
-            T.tstatus[p] = to.status;
+            T.ncontrib[p] = o.loops < T.lmax ? o.loops : T.lmax;
+            T.comps[p] = o.comps;
+            T.loops_raw[p] = o.loops_raw;
+            T.tstatus[p] = o.status;
             H.cur_ds[3 * (int64_t)p] = ds.x; H.cur_ds[3 * (int64_t)p + 1] = ds.y; H.cur_ds[3 * (int64_t)p + 2] = ds.z;
-            H.cur_active[p] = to.loops > 0 ? 1 : 0;
-            if (to.status) atomicOr(P.status, (int)ST_TOPO);
+            H.cur_active[p] = o.loops > 0 ? 1 : 0;
+            if (o.status) atomicOr(P.status, (int)ST_TOPO);
         }
         __syncthreads();
-        wp.mark(9);
     }
     kt_end(kt);
 }
@@ -896,41 +1014,34 @@ struct ApiSink {
     static constexpr int ZDX_LOOP_REC_D = 28;
 };
 
-__global__ void __launch_bounds__(64) k_topo_api(DPairs P, DMeshes M, DBodies B, TopoBufs T, int32_t split,
-                                                 int32_t lds_bytes) {
-    extern __shared__ char s_topo[];
+__global__ void __launch_bounds__(64) k_tp_walk_d_api(DPairs P, DMeshes M, DBodies B, TopoBufs T, int32_t lds_bytes) {
+    extern __shared__ char s_walk[];
     const int n = *P.topo_n;
     LaneDev wp;
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
-        wp.start();
         const int p = P.topo_list[it];
         const int i = P.pi[p], j = P.pj[p];
         const int S = P.nseg[p];
         const int mA = B.mesh[i];
-        V3 pa = ld3(B.state + 16 * (int64_t)i), pb = ld3(B.state + 16 * (int64_t)j);
+        const V3 pa = ld3(B.state + 16 * (int64_t)i), pb = ld3(B.state + 16 * (int64_t)j);
         ApiSink sink{T.rec + 28 * (int64_t)p * T.lmax, T.vtx ? T.vtx + 3 * (int64_t)p * T.vmax : nullptr,
                      T.lmax, T.vmax, 0, pa, pb, T.k_stiff};
-        TopoOut to;
-        to.comps = 0; to.loops_raw = 0; to.loops = 0; to.status = 0;
-        if (S > T.seg_max) to.status = TOPO_E2_OVERFLOW;
-        else if (S > 0) {
-            const double* sg = P.arena + 6 * P.seg_off[p];
-            const double tl = M.tol[mA], ae = M.area_eps[mA];
-            const V3 dir = sub(pb, pa);
-            wp.mark(0);
-            to = with_topo_scratch(T, S, s_topo, lds_bytes, [&](SArr<int> I, SArr<double> D, const TopoCaps& caps) {
-                return topology(wp, sg, S, tl, ae, split != 0, dir, I, D, caps, sink);
-            });
-        }
+        const int ncomp = T.ncomp[it];
+        const long long to = T.tj_off[it];
+        TopoOut o;
+        o.comps = 0; o.loops_raw = 0; o.loops = 0; o.status = 0;
+        if (ncomp < 0 || to < 0) o.status = TOPO_E2_OVERFLOW;
+        else if (ncomp > 0)
+            o = tp_walk_d(wp, T, S, ncomp, P.seg_off[p], to, M.tol[mA], M.area_eps[mA], sub(pb, pa), s_walk,
+                          lds_bytes, sink);
         if (lane_id() == 0) {
-            T.ncontrib[p] = to.loops;
-            T.comps[p] = to.comps;
-            T.loops_raw[p] = to.loops_raw;
-            T.tstatus[p] = to.status;
-            if (to.status) atomicOr(P.status, (int)ST_TOPO);
+            T.ncontrib[p] = o.loops;
+            T.comps[p] = o.comps;
+            T.loops_raw[p] = o.loops_raw;
+            T.tstatus[p] = o.status;
+            if (o.status) atomicOr(P.status, (int)ST_TOPO);
         }
         __syncthreads();
-        wp.mark(9);
     }
 }
 
@@ -1125,13 +1236,7 @@ inline int nblk(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
 }  // namespace
 
-#ifdef ZDX_TOPO_PROFILE
-extern "C" int zdx_topo_profile(unsigned long long* out) {
-    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_topo_prof), 16 * sizeof(unsigned long long)) != hipSuccess) return 1;
-    unsigned long long z[16] = {0};
-    return hipMemcpyToSymbol(HIP_SYMBOL(g_topo_prof), z, sizeof(z)) == hipSuccess ? 0 : 1;
-}
-#endif
+
 
 // ------------------------------------------------------------ launchers
 void launch_kt_reset(unsigned long long* kt, int n, hipStream_t s) {
@@ -1216,12 +1321,22 @@ void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, i
 
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
                      const TopoBufs& T, const SimParams& sp, hipStream_t s, unsigned long long* kt) {
-    hipLaunchKernelGGL(k_topo_sim, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, mat, H, T, sp,
+    launch_fill_u64(T.top, 0ull, 3, s);
+    hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T, kt);
+    hipLaunchKernelGGL(k_tp_walk_b, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, T, sp.split,
+                       T.lds_bytes);
+    hipLaunchKernelGGL(k_tp_pass_c, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T);
+    hipLaunchKernelGGL(k_tp_walk_d_sim, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, mat, H, T, sp,
                        T.lds_bytes, kt);
 }
 void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int32_t split,
                      hipStream_t s) {
-    hipLaunchKernelGGL(k_topo_api, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, T, split,
+    launch_fill_u64(T.top, 0ull, 3, s);
+    hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T, nullptr);
+    hipLaunchKernelGGL(k_tp_walk_b, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, T, split,
+                       T.lds_bytes);
+    hipLaunchKernelGGL(k_tp_pass_c, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T);
+    hipLaunchKernelGGL(k_tp_walk_d_api, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, T,
                        T.lds_bytes);
 }
 void launch_walls(const DBodies& B, const DMeshes& M, const DMat& mat, const WallBufs& W, const HistBufs& H,

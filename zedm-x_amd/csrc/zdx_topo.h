@@ -1,0 +1,541 @@
+// zdx_topo.h — segment topology of one contact pair, split into phases.
+//
+// The reference (src/main.cpp:2152-2219) runs, per pair: components
+// (label_segments_by_contact_greedy 804-898, when split), then per component
+// snap_endpoints (680-746), split_t_junctions (900-961), extract_loops
+// (963-1025), accumulate_Sn_Gn_from_polyline (1027-1040), contact_point_xc0
+// (1042-1052), the <3-vertex and area_eps filters and the flip of nA toward B
+// (2188-2219).
+//
+// Here the work is cut where the quadratic searches are: every test that
+// does not depend on another (segment pair adjacency, endpoint pair snap
+// distance, (edge, vertex) T-junction test) is a pure function of one item,
+// so the GPU runs all items of all pairs as flat passes over the device, and
+// only the order-dependent walks run serially per pair:
+//   pass A  (items)  tp_cells, tp_adj_key, tp_snap_hit
+//   walk B  (pair)   tp_components_and_vertices: DFS components, union-find
+//                    over the snap hits of each component, first-occurrence
+//                    vertex ids, sequential vertex means, edges
+//   pass C  (items)  tp_tj_hit
+//   walk D  (pair)   tp_loops: T-junction chains, loop tracing, Sn/Gn/xc0,
+//                    filters, sink
+// tp_topology_host() runs the same functions in sequence on the host (tests).
+//
+// Walks B and D execute "uniformly redundantly": every lane of a wave runs
+// the same serial code on identical values; WP::ballot spreads a scan of up
+// to 64 items over the lanes.  With LaneHost (W = 1) the same code is the
+// plain serial algorithm.
+#pragma once
+#include <vector>
+
+#include "zdx_geom.h"
+
+namespace zdx {
+
+constexpr int TP_NOADJ = 255;   // adjacency byte of a pair that is not joined
+
+// ---------------------------------------------------------------- pass A
+// Cells of a segment's representative points p0, p1, mid (cell = tol).
+ZDX_HD inline void tp_cells(const double* seg, int s, double h, int* c9) {
+    const V3 p0 = v3(seg[6 * s + 0], seg[6 * s + 1], seg[6 * s + 2]);
+    const V3 p1 = v3(seg[6 * s + 3], seg[6 * s + 4], seg[6 * s + 5]);
+    const V3 mid = mul(add(p0, p1), 0.5);
+    const I3 a = cell_of(p0, h), b = cell_of(p1, h), c = cell_of(mid, h);
+    c9[0] = a.x; c9[1] = a.y; c9[2] = a.z;
+    c9[3] = b.x; c9[4] = b.y; c9[5] = b.z;
+    c9[6] = c.x; c9[7] = c.y; c9[8] = c.z;
+}
+
+// Component candidate test of segment j found from popped segment i
+// (main.cpp:834-879): j in the +-1 cell neighbourhood of one of i's
+// representative points, AABB(+tol) overlap, seg_seg_dist2 <= tol^2.
+// Returns the reference's scan-order key (rep point r of i, neighbour cell
+// (dx,dy,dz) lexicographic: r*27 + cell offset, 0..80) or TP_NOADJ.
+ZDX_HD inline int tp_adj_key(const double* seg, const int* cell9, double h, int i, int j) {
+    if (i == j) return TP_NOADJ;
+    int key = TP_NOADJ;
+    for (int r = 0; r < 3; ++r) {
+        const int ax = cell9[9 * i + 3 * r], ay = cell9[9 * i + 3 * r + 1], az = cell9[9 * i + 3 * r + 2];
+        for (int q = 0; q < 3; ++q) {
+            const int bx = cell9[9 * j + 3 * q], by = cell9[9 * j + 3 * q + 1], bz = cell9[9 * j + 3 * q + 2];
+            if (near1(ax, ay, az, bx, by, bz)) {
+                const int k = r * 27 + (bx - ax + 1) * 9 + (by - ay + 1) * 3 + (bz - az + 1);
+                if (k < key) key = k;
+            }
+        }
+    }
+    if (key == TP_NOADJ) return TP_NOADJ;
+    const V3 p1 = v3(seg[6 * i + 0], seg[6 * i + 1], seg[6 * i + 2]), q1 = v3(seg[6 * i + 3], seg[6 * i + 4], seg[6 * i + 5]);
+    const V3 p2 = v3(seg[6 * j + 0], seg[6 * j + 1], seg[6 * j + 2]), q2 = v3(seg[6 * j + 3], seg[6 * j + 4], seg[6 * j + 5]);
+    const V3 amn = v3(dmin(p1.x, q1.x) - h, dmin(p1.y, q1.y) - h, dmin(p1.z, q1.z) - h);
+    const V3 amx = v3(dmax(p1.x, q1.x) + h, dmax(p1.y, q1.y) + h, dmax(p1.z, q1.z) + h);
+    const V3 bmn = v3(dmin(p2.x, q2.x) - h, dmin(p2.y, q2.y) - h, dmin(p2.z, q2.z) - h);
+    const V3 bmx = v3(dmax(p2.x, q2.x) + h, dmax(p2.y, q2.y) + h, dmax(p2.z, q2.z) + h);
+    if (amx.x < bmn.x || amn.x > bmx.x || amx.y < bmn.y || amn.y > bmx.y || amx.z < bmn.z || amn.z > bmx.z)
+        return TP_NOADJ;
+    return seg_seg(p1, q1, p2, q2) <= h * h ? key : TP_NOADJ;
+}
+
+// Endpoint pairs are numbered t = a(a-1)/2 + b, b < a; point k is end k&1 of
+// segment k>>1.  The snap test of snap_endpoints (main.cpp:697-723): cells
+// within +-1 and squared distance <= tol^2.
+ZDX_HD inline void tp_pair_of(int64_t t, int& a, int& b) {
+    int i = (int)((1.0 + sqrt(1.0 + 8.0 * (double)t)) * 0.5);
+    while ((int64_t)i * (i - 1) / 2 > t) --i;
+    while ((int64_t)(i + 1) * i / 2 <= t) ++i;
+    a = i;
+    b = (int)(t - (int64_t)i * (i - 1) / 2);
+}
+ZDX_HD inline bool tp_snap_hit(const double* seg, const int* cell9, double h, int a, int b) {
+    const int* ca = cell9 + 9 * (a >> 1) + 3 * (a & 1);
+    const int* cb = cell9 + 9 * (b >> 1) + 3 * (b & 1);
+    if (!near1(ca[0], ca[1], ca[2], cb[0], cb[1], cb[2])) return false;
+    const double* pa = seg + 6 * (a >> 1) + 3 * (a & 1);
+    const double* pb = seg + 6 * (b >> 1) + 3 * (b & 1);
+    return norm2(sub(v3(pa[0], pa[1], pa[2]), v3(pb[0], pb[1], pb[2]))) <= h * h;
+}
+
+// ---------------------------------------------------------------- walk B
+// Per processed component q (in the reference's processing order: size
+// descending, stable), 5 ints: vertex count, edge count, first vertex, first
+// edge (both within the pair), first T-junction bit word (within the pair).
+enum { TPC_NV = 0, TPC_NE = 1, TPC_VB = 2, TPC_EB = 3, TPC_TW = 4, TPC_N = 5 };
+
+// Scratch of walk B for a pair of S segments: ints, doubles.
+ZDX_HD inline int64_t tp_b_ints(int S) { return 18 * (int64_t)S + 4; }
+ZDX_HD inline int64_t tp_b_dbls(int S) { return 6 * (int64_t)S; }
+// T-junction words of a component: one row of ceil(nv/64) words per edge.
+ZDX_HD inline int tp_row_words(int nv) { return (nv + 63) >> 6; }
+
+// adj: S x S adjacency keys (row = popped segment); snapw: bit t of word t/64
+// is the snap hit of endpoint pair t.  Outputs: comp (ncomp x TPC_N), V (3 per
+// vertex), vcell (3 per vertex), E (2 per edge, vertex ids local to the
+// component).  Returns ncomp; *tj_words = total T-junction words.
+template <class WP>
+ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, int S, double h, bool split,
+                                             const uint8_t* adj, const uint64_t* snapw, int* I, double* D,
+                                             int* comp, double* V, int* vcell, int* E, int* tj_words) {
+    const int W = WP::W;
+    const int ln = wp.lane();
+    const int n = 2 * S;
+    int* vis = I;            // S
+    int* stk = vis + S;      // S
+    int* spk = stk + S;      // S
+    int* cord = spk + S;     // S   segment order of all comps, concatenated
+    int* cbeg = cord + S;    // S+1
+    int* csz = cbeg + S + 1; // S+1
+    int* cidx = csz + S + 1; // S   comp processing order
+    int* compof = cidx + S;  // S   comp of each segment
+    int* par = compof + S;   // 2S
+    int* rnk = par + n;      // 2S
+    int* rootvid = rnk + n;  // 2S
+    int* ptv = rootvid + n;  // 2S
+    int* cnt = ptv + n;      // 2S
+    double* Vs = D;          // 3 x 2S
+    // ---- components (DFS in the reference's candidate order)
+    int ncomp = 0;
+    if (split) {
+        for (int s = ln; s < S; s += W) vis[s] = 0;
+        wp.sync();
+        int nord = 0;
+        for (int seed = 0; seed < S; ++seed) {
+            if (vis[seed]) continue;
+            cbeg[ncomp] = nord;
+            int sp = 0;
+            stk[sp++] = seed;
+            vis[seed] = 1;
+            cord[nord++] = seed;
+            while (sp > 0) {
+                const int i = stk[--sp];
+                const int first = sp;
+                const uint8_t* row = adj + (int64_t)i * S;
+                for (int base = 0; base < S; base += W) {
+                    const int j = base + ln;
+                    const int key = j < S ? (int)row[j] : TP_NOADJ;
+                    uint64_t m = wp.ballot(key != TP_NOADJ && !vis[j < S ? j : 0]);
+                    while (m) {
+                        const int jj = base + ctz64(m);
+                        m &= m - 1;
+                        const int kj = (int)row[jj];
+                        // accepted candidates of i are pushed ordered by (scan key, id)
+                        int pos = sp;
+                        while (pos > first && spk[pos - 1] > kj) {
+                            stk[pos] = stk[pos - 1];
+                            spk[pos] = spk[pos - 1];
+                            --pos;
+                        }
+                        stk[pos] = jj;
+                        spk[pos] = kj;
+                        ++sp;
+                    }
+                }
+                for (int k = first; k < sp; ++k) {
+                    vis[stk[k]] = 1;
+                    cord[nord++] = stk[k];
+                }
+            }
+            csz[ncomp] = nord - cbeg[ncomp];
+            ++ncomp;
+        }
+        for (int c = 0; c < ncomp; ++c) cidx[c] = c;
+        CompBySizeDesc cmp; cmp.size = SArr<int>{csz, 1};
+        ref_sort(SArr<int>{cidx, 1}, ncomp, cmp);
+        for (int c = 0; c < ncomp; ++c)
+            for (int k = 0; k < csz[c]; ++k) compof[cord[cbeg[c] + k]] = c;
+    } else {
+        for (int s = 0; s < S; ++s) { cord[s] = s; compof[s] = 0; }
+        cbeg[0] = 0; csz[0] = S; cidx[0] = 0;
+        ncomp = 1;
+    }
+    // ---- snap: union-find over the hits inside each component; the
+    // partition does not depend on the order of the unions
+    for (int k = ln; k < n; k += W) { par[k] = k; rnk[k] = 0; rootvid[k] = -1; }
+    wp.sync();
+    auto find = [&](int x) {
+        int r = x;
+        while (par[r] != r) r = par[r];
+        while (par[x] != r) { const int nx = par[x]; par[x] = r; x = nx; }
+        return r;
+    };
+    const int64_t npairs = (int64_t)n * (n - 1) / 2;
+    const int64_t nwords = (npairs + 63) >> 6;
+    for (int64_t wb = 0; wb < nwords; wb += W) {
+        const int64_t w = wb + ln;
+        const uint64_t word = w < nwords ? snapw[w] : 0ull;
+        uint64_t m = wp.ballot(word != 0ull);
+        while (m) {
+            const int src = ctz64(m);
+            m &= m - 1;
+            uint64_t bits = wp.shfl64(word, src);
+            while (bits) {
+                const int64_t t = ((wb + src) << 6) + ctz64(bits);
+                bits &= bits - 1;
+                int a, b;
+                tp_pair_of(t, a, b);
+                if (compof[a >> 1] != compof[b >> 1]) continue;
+                const int ra = find(a), rb = find(b);
+                if (ra != rb) {
+                    if (rnk[ra] < rnk[rb]) par[ra] = rb;
+                    else if (rnk[ra] > rnk[rb]) par[rb] = ra;
+                    else { par[rb] = ra; rnk[ra] += 1; }
+                }
+            }
+        }
+    }
+    // ---- per component: first-occurrence vertex ids, sequential means, edges
+    int vb = 0, eb = 0, tw = 0;
+    for (int q = 0; q < ncomp; ++q) {
+        const int c = cidx[q];
+        const int base = cbeg[c], ns = csz[c];
+        int nv = 0;
+        for (int k = 0; k < 2 * ns; ++k) {
+            const int s = cord[base + (k >> 1)];
+            const int pt = 2 * s + (k & 1);
+            const int r = find(pt);
+            const double* p = seg + 6 * s + 3 * (k & 1);
+            if (rootvid[r] < 0) {
+                rootvid[r] = nv;
+                Vs[3 * nv] = p[0]; Vs[3 * nv + 1] = p[1]; Vs[3 * nv + 2] = p[2];
+                cnt[nv] = 1;
+                ++nv;
+            } else {
+                const int v = rootvid[r];
+                Vs[3 * v] += p[0]; Vs[3 * v + 1] += p[1]; Vs[3 * v + 2] += p[2];
+                cnt[v] += 1;
+            }
+            ptv[pt] = rootvid[r];
+        }
+        wp.sync();
+        for (int v = ln; v < nv; v += W) {
+            const double cn = (double)cnt[v];
+            const V3 m = v3(Vs[3 * v] / cn, Vs[3 * v + 1] / cn, Vs[3 * v + 2] / cn);
+            double* o = V + 3 * (int64_t)(vb + v);
+            o[0] = m.x; o[1] = m.y; o[2] = m.z;
+            const I3 cc = cell_of(m, h);
+            int* oc = vcell + 3 * (int64_t)(vb + v);
+            oc[0] = cc.x; oc[1] = cc.y; oc[2] = cc.z;
+        }
+        int ne = 0;
+        for (int k = 0; k < ns; ++k) {
+            const int s = cord[base + k];
+            const int a = ptv[2 * s], b = ptv[2 * s + 1];
+            if (a != b) {
+                if (ln == 0) { E[2 * (int64_t)(eb + ne)] = a; E[2 * (int64_t)(eb + ne) + 1] = b; }
+                ++ne;
+            }
+        }
+        if (ln == 0) {
+            int* cr = comp + TPC_N * q;
+            cr[TPC_NV] = nv; cr[TPC_NE] = ne; cr[TPC_VB] = vb; cr[TPC_EB] = eb; cr[TPC_TW] = tw;
+        }
+        vb += nv;
+        eb += ne;
+        tw += ne * tp_row_words(nv);
+        wp.sync();
+    }
+    *tj_words = tw;
+    return ncomp;
+}
+
+// ---------------------------------------------------------------- pass C
+// split_t_junctions candidate test (main.cpp:908-935) of vertex v against
+// edge e of one component (V, vcell: that component's vertices).
+ZDX_HD inline bool tp_tj_hit(const double* V, const int* vcell, const int* E, double h, int e, int v) {
+    const int a = E[2 * e], b = E[2 * e + 1];
+    if (v == a || v == b) return false;
+    const V3 pa = v3(V[3 * a], V[3 * a + 1], V[3 * a + 2]), pb = v3(V[3 * b], V[3 * b + 1], V[3 * b + 2]);
+    const V3 mn = v3(dmin(pa.x, pb.x) - h, dmin(pa.y, pb.y) - h, dmin(pa.z, pb.z) - h);
+    const V3 mx = v3(dmax(pa.x, pb.x) + h, dmax(pa.y, pb.y) + h, dmax(pa.z, pb.z) + h);
+    const I3 c0 = cell_of(mn, h), c1 = cell_of(mx, h);
+    const int cx = vcell[3 * v], cy = vcell[3 * v + 1], cz = vcell[3 * v + 2];
+    if (cx < c0.x || cx > c1.x || cy < c0.y || cy > c1.y || cz < c0.z || cz > c1.z) return false;
+    double d2, t;
+    pt_seg(v3(V[3 * v], V[3 * v + 1], V[3 * v + 2]), pa, pb, d2, t);
+    return d2 <= h * h && t > 1e-6 && t < 1.0 - 1e-6;
+}
+
+// ---------------------------------------------------------------- walk D
+// Scratch of walk D for a pair of S segments (vertices <= 2S, split edges <= 4S).
+ZDX_HD inline int64_t tp_d_ints(int S) { return 4 * (int64_t)(2 * S) + 2 * (4 * (int64_t)S) * 3 + 2 * (4 * (int64_t)S) + 2 * (2 * (int64_t)S) + 8; }
+ZDX_HD inline int64_t tp_d_dbls(int S) { return 3 * (2 * (int64_t)S) + 2 * (int64_t)S + 8; }
+
+// comp/V/vcell/E as written by walk B, tjw: the T-junction bit words of the
+// pair.  For every accepted loop, sink(loop_index, comp, raw_index, Sn, Gn,
+// nA, xc0, area, nverts, path, Vs) is called (by every lane) in the
+// reference's order.
+template <class WP, class Sink>
+ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double area_eps, const V3& dirAB,
+                               const int* comp, const double* V, const int* vcell, const int* E, const uint64_t* tjw,
+                               int* I, double* D, Sink& sink) {
+    TopoOut out; out.comps = ncomp; out.loops_raw = 0; out.loops = 0; out.status = TOPO_OK;
+    const int W = WP::W;
+    const int ln = wp.lane();
+    const int NV = 2 * S, E2c = 4 * S;
+    int* deg = I;                // NV
+    int* aoff = deg + NV;        // NV
+    int* afill = aoff + NV;      // NV
+    int* sord = afill + NV;      // NV
+    int* E2 = sord + NV;         // 2 E2c
+    int* adjn = E2 + 2 * E2c;    // 2 E2c
+    int* adje = adjn + 2 * E2c;  // 2 E2c
+    int* used = adje + 2 * E2c;  // E2c
+    int* path = used + E2c;      // E2c + 1
+    int* spv = path + E2c + 1;   // NV  hits of one edge: vertex
+    int* spk = spv + NV;         // NV  scan key
+    double* Vs = D;              // 3 NV  this component's vertices
+    double* spt = Vs + 3 * NV;   // NV? (2S) hit parameter t
+    for (int q = 0; q < ncomp; ++q) {
+        const int* cr = comp + TPC_N * q;
+        const int nv = cr[TPC_NV], ne = cr[TPC_NE];
+        const double* Vq = V + 3 * (int64_t)cr[TPC_VB];
+        const int* vcq = vcell + 3 * (int64_t)cr[TPC_VB];
+        const int* Eq = E + 2 * (int64_t)cr[TPC_EB];
+        const uint64_t* Tq = tjw + cr[TPC_TW];
+        const int rw = tp_row_words(nv);
+        for (int k = ln; k < 3 * nv; k += W) Vs[k] = Vq[k];
+        wp.sync();
+        auto VV = [&](int v) { return v3(Vs[3 * v], Vs[3 * v + 1], Vs[3 * v + 2]); };
+        // ---- split_t_junctions: per edge, its hits in vertex order, put in
+        // the reference's cell-scan order, std::sort by t, chained
+        int ne2 = 0;
+        for (int e = 0; e < ne; ++e) {
+            const int a = Eq[2 * e], b = Eq[2 * e + 1];
+            const uint64_t* row = Tq + (int64_t)e * rw;
+            int nsp = 0;
+            for (int wb = 0; wb < rw; wb += W) {
+                const int w = wb + ln;
+                const uint64_t word = w < rw ? row[w] : 0ull;
+                uint64_t m = wp.ballot(word != 0ull);
+                while (m) {
+                    const int src = ctz64(m);
+                    m &= m - 1;
+                    uint64_t bits = wp.shfl64(word, src);
+                    while (bits) {
+                        const int v = ((wb + src) << 6) + ctz64(bits);
+                        bits &= bits - 1;
+                        spv[nsp] = v;
+                        ++nsp;
+                    }
+                }
+            }
+            if (nsp == 0) {
+                if (ne2 + 1 > E2c) { out.status |= TOPO_E2_OVERFLOW; return out; }
+                E2[2 * ne2] = a; E2[2 * ne2 + 1] = b; ++ne2;
+                continue;
+            }
+            const V3 pa = VV(a), pb = VV(b);
+            const V3 mn = v3(dmin(pa.x, pb.x) - h, dmin(pa.y, pb.y) - h, dmin(pa.z, pb.z) - h);
+            const V3 mx = v3(dmax(pa.x, pb.x) + h, dmax(pa.y, pb.y) + h, dmax(pa.z, pb.z) + h);
+            const I3 c0 = cell_of(mn, h), c1 = cell_of(mx, h);
+            const int nyz = (c1.y - c0.y + 1) * (c1.z - c0.z + 1);
+            for (int k = 0; k < nsp; ++k) {
+                const int v = spv[k];
+                double d2, t;
+                pt_seg(VV(v), pa, pb, d2, t);
+                spt[k] = t;
+                // the reference scans cells ix, iy, iz over [c0, c1]; key = scan index
+                spk[k] = (vcq[3 * v] - c0.x) * nyz + (vcq[3 * v + 1] - c0.y) * (c1.z - c0.z + 1) + (vcq[3 * v + 2] - c0.z);
+            }
+            SplitArr sa; sa.t = SArr<double>{spt, 1}; sa.v = SArr<int>{spv, 1}; sa.k = SArr<int>{spk, 1};
+            SplitByScan byscan;
+            for (int i2 = 1; i2 < nsp; ++i2) {
+                Split x = sa.get(i2);
+                int j2 = i2 - 1;
+                while (j2 >= 0 && byscan(x, sa.get(j2))) { sa.set(j2 + 1, sa.get(j2)); --j2; }
+                sa.set(j2 + 1, x);
+            }
+            SplitByT byt;
+            ref_sort(sa, nsp, byt);
+            int prevv = a, last = -1;
+            for (int k = 0; k < nsp; ++k) {
+                const int v = spv[k];
+                if (v == last) continue;
+                last = v;
+                if (prevv != v) {
+                    if (ne2 + 1 > E2c) { out.status |= TOPO_E2_OVERFLOW; return out; }
+                    E2[2 * ne2] = prevv; E2[2 * ne2 + 1] = v; ++ne2;
+                }
+                prevv = v;
+            }
+            if (prevv != b) {
+                if (ne2 + 1 > E2c) { out.status |= TOPO_E2_OVERFLOW; return out; }
+                E2[2 * ne2] = prevv; E2[2 * ne2 + 1] = b; ++ne2;
+            }
+        }
+        // ---- extract_loops
+        for (int v = 0; v < nv; ++v) deg[v] = 0;
+        for (int e = 0; e < ne2; ++e) { deg[E2[2 * e]] += 1; deg[E2[2 * e + 1]] += 1; }
+        int acc = 0;
+        for (int v = 0; v < nv; ++v) { aoff[v] = acc; afill[v] = acc; acc += deg[v]; }
+        for (int e = 0; e < ne2; ++e) {
+            const int a = E2[2 * e], b = E2[2 * e + 1];
+            const int pa = afill[a]++; adjn[pa] = b; adje[pa] = e;
+            const int pb = afill[b]++; adjn[pb] = a; adje[pb] = e;
+        }
+        for (int e = 0; e < ne2; ++e) used[e] = 0;
+        for (int v = 0; v < nv; ++v) sord[v] = v;
+        StartOrder so; so.deg = SArr<int>{deg, 1};
+        ref_sort(SArr<int>{sord, 1}, nv, so);
+        int nloops_here = 0;
+        for (int si = 0; si < nv; ++si) {
+            const int sv = sord[si];
+            for (int k = aoff[sv]; k < aoff[sv] + deg[sv]; ++k) {
+                const int ei = adje[k];
+                if (used[ei]) continue;
+                int plen = 0;
+                path[plen++] = sv;
+                int prev = sv, cur = adjn[k];
+                used[ei] = 1;
+                path[plen++] = cur;
+                int steps = 0;
+                bool closed = false;
+                while (steps < 100000) {
+                    steps += 1;
+                    if (cur == sv) { closed = plen > 1; break; }
+                    int pick = -1, first = -1;
+                    for (int m = aoff[cur]; m < aoff[cur] + deg[cur]; ++m) {
+                        if (used[adje[m]]) continue;
+                        if (first < 0) first = m;
+                        if (adjn[m] != prev) { pick = m; break; }
+                    }
+                    if (first < 0) break;
+                    if (pick < 0) pick = first;
+                    used[adje[pick]] = 1;
+                    prev = cur;
+                    cur = adjn[pick];
+                    if (plen > E2c) { out.status |= TOPO_E2_OVERFLOW; return out; }
+                    path[plen++] = cur;
+                }
+                if (!closed) continue;
+                --plen;  // drop the repeated start vertex
+                const int raw = out.loops_raw + nloops_here;
+                ++nloops_here;
+                if (plen < 3) continue;
+                // accumulate_Sn_Gn_from_polyline
+                V3 Sn = v3(0.0, 0.0, 0.0), Gn = v3(0.0, 0.0, 0.0);
+                for (int m = 0; m < plen; ++m) {
+                    const V3 p0 = VV(path[m]);
+                    const V3 p1 = VV(path[(m + 1) % plen]);
+                    Sn = add(Sn, mul(cross(p0, p1), 0.5));
+                    const V3 dx = sub(p1, p0);
+                    const double s = dot(p0, p1) + (1.0 / 3.0) * dot(dx, dx);
+                    Gn = add(Gn, mul(dx, -(1.0 / 3.0) * s));
+                }
+                // contact_point_xc0
+                const double area = norm(Sn);
+                if (area < 1e-14) continue;
+                V3 nA = mul(Sn, -1.0 / area);
+                const V3 xc = dvd(cross(nA, Gn), area);
+                if (area < area_eps) continue;
+                if (dot(nA, dirAB) < 0.0) nA = mul(nA, -1.0);
+                out.status |= sink(out.loops, q, raw, Sn, Gn, nA, xc, area, plen, SArr<int>{path, 1},
+                                   SArr<double>{Vs, 1});
+                out.loops += 1;
+            }
+        }
+        out.loops_raw += nloops_here;
+        wp.sync();
+    }
+    return out;
+}
+
+// Scratch of the device walks, including their copies of the pair's inputs
+// (walk B: the segments, 6S doubles; walk D: edges 2S and vertex cells 6S ints).
+ZDX_HD inline int64_t tp_walk_b_ints(int S) { return tp_b_ints(S); }
+ZDX_HD inline int64_t tp_walk_b_dbls(int S) { return tp_b_dbls(S) + 6 * (int64_t)S; }
+ZDX_HD inline int64_t tp_walk_d_ints(int S) { return tp_d_ints(S) + 8 * (int64_t)S; }
+ZDX_HD inline int64_t tp_walk_d_dbls(int S) { return tp_d_dbls(S); }
+ZDX_HD inline int64_t tp_walk_ints(int S) {
+    const int64_t b = tp_walk_b_ints(S), d = tp_walk_d_ints(S);
+    return b > d ? b : d;
+}
+ZDX_HD inline int64_t tp_walk_dbls(int S) {
+    const int64_t b = tp_walk_b_dbls(S), d = tp_walk_d_dbls(S);
+    return b > d ? b : d;
+}
+// bytes of a walk's scratch (ints first, doubles 8-aligned)
+ZDX_HD inline int64_t tp_scratch_bytes(int64_t ni, int64_t nd) { return ((ni * 4 + 7) & ~(int64_t)7) + nd * 8; }
+
+// ---------------------------------------------------------------- host
+// All phases of one pair in sequence (LaneHost), for host-side tests.
+template <class Sink>
+inline TopoOut tp_topology_host(const double* seg, int S, double h, double area_eps, bool split, const V3& dirAB,
+                                Sink& sink) {
+    TopoOut out; out.comps = 0; out.loops_raw = 0; out.loops = 0; out.status = TOPO_OK;
+    if (S <= 0) return out;
+    LaneHost wp;
+    std::vector<int> cell9(9 * (size_t)S);
+    for (int s = 0; s < S; ++s) tp_cells(seg, s, h, &cell9[9 * (size_t)s]);
+    std::vector<uint8_t> adj((size_t)S * S);
+    for (int i = 0; i < S; ++i)
+        for (int j = 0; j < S; ++j) adj[(size_t)i * S + j] = (uint8_t)tp_adj_key(seg, cell9.data(), h, i, j);
+    const int n = 2 * S;
+    const int64_t npairs = (int64_t)n * (n - 1) / 2;
+    std::vector<uint64_t> snapw((size_t)((npairs + 63) >> 6), 0ull);
+    for (int64_t t = 0; t < npairs; ++t) {
+        int a, b;
+        tp_pair_of(t, a, b);
+        if (tp_snap_hit(seg, cell9.data(), h, a, b)) snapw[(size_t)(t >> 6)] |= 1ull << (t & 63);
+    }
+    std::vector<int> IB((size_t)tp_b_ints(S)), comp(TPC_N * (size_t)S), vcell(3 * (size_t)n), E(2 * (size_t)S);
+    std::vector<double> DB((size_t)tp_b_dbls(S)), V(3 * (size_t)n);
+    int tjn = 0;
+    const int ncomp = tp_components_and_vertices(wp, seg, S, h, split, adj.data(), snapw.data(), IB.data(), DB.data(),
+                                                 comp.data(), V.data(), vcell.data(), E.data(), &tjn);
+    std::vector<uint64_t> tjw((size_t)tjn + 1, 0ull);
+    for (int q = 0; q < ncomp; ++q) {
+        const int* cr = &comp[TPC_N * (size_t)q];
+        const int rw = tp_row_words(cr[TPC_NV]);
+        for (int e = 0; e < cr[TPC_NE]; ++e)
+            for (int v = 0; v < cr[TPC_NV]; ++v)
+                if (tp_tj_hit(&V[3 * (size_t)cr[TPC_VB]], &vcell[3 * (size_t)cr[TPC_VB]], &E[2 * (size_t)cr[TPC_EB]], h, e,
+                              v))
+                    tjw[(size_t)cr[TPC_TW] + (size_t)e * rw + (v >> 6)] |= 1ull << (v & 63);
+    }
+    std::vector<int> ID((size_t)tp_d_ints(S));
+    std::vector<double> DD((size_t)tp_d_dbls(S));
+    return tp_loops(wp, S, ncomp, h, area_eps, dirAB, comp.data(), V.data(), vcell.data(), E.data(), tjw.data(),
+                    ID.data(), DD.data(), sink);
+}
+
+}  // namespace zdx
