@@ -191,3 +191,24 @@ def test_sim_checkpoint_restore_is_exact(ctx, golden_sims, ref_inputs):
         b.restore(ck)
         b.step(60, zdx.MODE_BATCHED)
         assert np.array_equal(sa, b.state())
+
+
+@pytest.mark.parametrize("depth,off,pa", [(0.2, (0.0, 0.0), (0.0, 0.0)), (0.3, (0.2, 0.2), (0.3, -0.7)),
+                                           (0.45, (0.3, -0.3), (-1.1, 0.25))])
+def test_box_on_box_closed_form(ctx, depth, off, pa):
+    """|Sn| = overlap area, Fn = k/2 |Sn| (= k/2 dOmega/ddelta with Omega =
+    delta A), n from A to B, xc0 = -2/3 of the rectangle centre's in-plane
+    part -- the closed form the oracle is pinned with, on the device path."""
+    k = 3.0
+    a, b = ctx.register_mesh(synth.box_wall(1.0, 1.2, 0.8)), ctx.register_mesh(synth.box_wall(2, 2, 2))
+    pA = np.array([0.0, pa[0], pa[1]])
+    pB = pA + np.array([0.5 - depth + 1.0, off[0], off[1]])
+    for mode in (zdx.MODE_PLAIN, zdx.MODE_BATCHED):
+        r = ctx.contact_pairs([a], [b], [pA], [[1, 0, 0, 0]], [pB], [[1, 0, 0, 0]], mode=mode, k=k)
+        assert len(r["loops"]) == 1
+        L = r["loops"][0]
+        A = 1.2 * 0.8
+        assert L[12] == pytest.approx(A, rel=1e-13)
+        assert np.allclose(L[6:9], [1.0, 0.0, 0.0], atol=1e-13)
+        assert L[13] == pytest.approx(0.5 * k * A, rel=1e-13)
+        assert np.allclose(L[9:12], -2.0 / 3.0 * np.array([0.0, pa[0], pa[1]]), atol=1e-13)
