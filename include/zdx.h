@@ -69,6 +69,8 @@ typedef struct {
     int32_t split_contacts; /* 1: split segments into components (main.cpp:2154-2162) */
     int32_t mode;           /* ZDX_MODE_PLAIN or ZDX_MODE_BATCHED */
     double k;               /* stiffness of Fn = 0.5 k |Sn| */
+    int32_t sensitivities;  /* 1: fill zdx_pair_out.loop_jac (no reference counterpart) */
+    int32_t reserved;
 } zdx_contact_opts;
 
 enum { ZDX_MODE_PLAIN = 0, ZDX_MODE_BATCHED = 1 };
@@ -98,6 +100,11 @@ typedef struct {
     int64_t seg_cap;
     /* out: totals (also filled on ZDX_CAPACITY) */
     int64_t n_loops_total, n_vtx_total, n_seg_total;
+    /* per accepted loop, with opts->sensitivities: loop_cap x 72 doubles,
+     * dSn/dxi (3 x 12, row-major) then dGn/dxi (3 x 12), where xi = (dA, wA,
+     * dB, wB) is a rigid motion of the pair's bodies: body X moves its points
+     * x by dX + wX x (x - posX).  The loop's topology is held fixed. */
+    double* loop_jac;
 } zdx_pair_out;
 
 zdx_status zdx_contact_pairs(zdx_ctx* ctx, const zdx_pair_in* pairs, int32_t n_pairs, const zdx_contact_opts* opts,

@@ -60,9 +60,12 @@ struct RecSink {
     double* vtx;
     int vcap;
     int nv;
+    double* ljac = nullptr;   // 72 per loop (dSn, dGn) when sensitivities are computed
     int operator()(int li, int comp, int raw, const V3& Sn, const V3& Gn, const V3& nA, const V3& xc, double area,
-                   int plen, SArr<int> path, SArr<double> Vs) {
+                   int plen, SArr<int> path, SArr<double> Vs, const double* jac) {
         if (li >= cap) return TOPO_LOOP_OVERFLOW;
+        if (ljac && jac)
+            for (int c = 0; c < 72; ++c) ljac[72 * li + c] = jac[c];
         double* r = rec + 28 * li;
         r[0] = Sn.x; r[1] = Sn.y; r[2] = Sn.z; r[3] = Gn.x; r[4] = Gn.y; r[5] = Gn.z;
         r[6] = nA.x; r[7] = nA.y; r[8] = nA.z; r[9] = xc.x; r[10] = xc.y; r[11] = xc.z;
@@ -128,6 +131,118 @@ int ht_pair(const double* trisA, int TA, const double* trisB, int TB, const doub
     counts[5] = sink.nv;
     counts[6] = t.status;
     return 0;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// Endpoints of triangle A (body-frame corners a9 about centre cA) and B after
+// the rigid motion xi = (dA, wA, dB, wB) (rotation by the vector w about the
+// body centre, Rodrigues), with their provenance codes and, at xi, the
+// endpoint Jacobians (3 x 12 each).  Returns 1 when the triangles intersect.
+int ht_endpoint(const double* a9, const double* b9, const double* cA, const double* cB, const double* xi,
+                double* p01, int* codes, double* J0, double* J1) {
+    auto rot = [](const V3& w, const V3& r) {
+        const double th = norm(w);
+        if (th == 0.0) return r;
+        const V3 k = dvd(w, th);
+        return add(add(mul(r, cos(th)), mul(cross(k, r), sin(th))), mul(k, dot(k, r) * (1.0 - cos(th))));
+    };
+    V3 A[3], B[3];
+    const V3 ca = v3(cA[0], cA[1], cA[2]), cb = v3(cB[0], cB[1], cB[2]);
+    const V3 dA = v3(xi[0], xi[1], xi[2]), wA = v3(xi[3], xi[4], xi[5]);
+    const V3 dB = v3(xi[6], xi[7], xi[8]), wB = v3(xi[9], xi[10], xi[11]);
+    for (int k = 0; k < 3; ++k) {
+        const V3 a = v3(a9[3 * k], a9[3 * k + 1], a9[3 * k + 2]);
+        const V3 b = v3(b9[3 * k], b9[3 * k + 1], b9[3 * k + 2]);
+        A[k] = add(add(ca, rot(wA, sub(a, ca))), dA);
+        B[k] = add(add(cb, rot(wB, sub(b, cb))), dB);
+    }
+    V3 nA, nB;
+    double d1, d2;
+    if (!plane_of(A[0], A[1], A[2], nA, d1) || !plane_of(B[0], B[1], B[2], nB, d2)) return 0;
+    V3 p0, p1;
+    int c0 = 0, c1 = 0;
+    if (!tri_tri_prov(A, nA, d1, B, nB, d2, p0, p1, c0, c1)) return 0;
+    V3 q0, q1;
+    if (!tri_tri(A, nA, d1, B, nB, d2, q0, q1)) return -1;
+    if (norm2(sub(p0, q0)) != 0.0 || norm2(sub(p1, q1)) != 0.0) return -2;   // same arithmetic as tri_tri
+    p01[0] = p0.x; p01[1] = p0.y; p01[2] = p0.z; p01[3] = p1.x; p01[4] = p1.y; p01[5] = p1.z;
+    codes[0] = c0; codes[1] = c1;
+    const V3 cA2 = add(ca, dA), cB2 = add(cb, dB);
+    endpoint_jacobian(c0, A, B, nA, nB, p0, cA2, cB2, J0);
+    endpoint_jacobian(c1, A, B, nA, nB, p1, cA2, cB2, J1);
+    return 1;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// Whole per-pair pipeline with sensitivities: bodies A/B (body-frame
+// triangles, centre at the origin of their frames) posed by pos/quat and then
+// moved by the rigid motion xi (rotation vector about the body position,
+// Rodrigues); loop records as ht_pair plus, at xi, dSn/dxi and dGn/dxi per
+// loop (72 doubles).  Returns the number of accepted loops (or -1).
+int ht_pair_jac(const double* trisA, int TA, const double* trisB, int TB, const double* pA, const double* qA,
+                const double* pB, const double* qB, const double* xi, double mean_edge_a, double* rec, int rec_cap,
+                double* ljac) {
+    auto rot = [](const V3& w, const V3& r) {
+        const double th = norm(w);
+        if (th == 0.0) return r;
+        const V3 k = dvd(w, th);
+        return add(add(mul(r, cos(th)), mul(cross(k, r), sin(th))), mul(k, dot(k, r) * (1.0 - cos(th))));
+    };
+    Qt qa; qa.w = qA[0]; qa.x = qA[1]; qa.y = qA[2]; qa.z = qA[3];
+    Qt qb; qb.w = qB[0]; qb.x = qB[1]; qb.y = qB[2]; qb.z = qB[3];
+    const V3 pa = v3(pA[0], pA[1], pA[2]), pb = v3(pB[0], pB[1], pB[2]);
+    const V3 dA = v3(xi[0], xi[1], xi[2]), wA = v3(xi[3], xi[4], xi[5]);
+    const V3 dB = v3(xi[6], xi[7], xi[8]), wB = v3(xi[9], xi[10], xi[11]);
+    std::vector<V3> wa(3 * TA), wb(3 * TB);
+    for (int i = 0; i < 3 * TA; ++i)
+        wa[i] = add(add(pa, rot(wA, qrotate(qa, v3(trisA[3 * i], trisA[3 * i + 1], trisA[3 * i + 2])))), dA);
+    for (int i = 0; i < 3 * TB; ++i)
+        wb[i] = add(add(pb, rot(wB, qrotate(qb, v3(trisB[3 * i], trisB[3 * i + 1], trisB[3 * i + 2])))), dB);
+    const V3 cA = add(pa, dA), cB = add(pb, dB);
+    std::vector<V3> nA(TA), nB(TB);
+    std::vector<double> dAp(TA), dBp(TB);
+    for (int i = 0; i < TA; ++i) plane_of(wa[3 * i], wa[3 * i + 1], wa[3 * i + 2], nA[i], dAp[i]);
+    for (int i = 0; i < TB; ++i) plane_of(wb[3 * i], wb[3 * i + 1], wb[3 * i + 2], nB[i], dBp[i]);
+    std::vector<double> seg, EJ;
+    for (int i = 0; i < TA; ++i) {
+        if (norm2(nA[i]) < 1e-30) continue;
+        const V3* A = &wa[3 * i];
+        double a0x = min3(A[0].x, A[1].x, A[2].x), a0y = min3(A[0].y, A[1].y, A[2].y), a0z = min3(A[0].z, A[1].z, A[2].z);
+        double a1x = max3(A[0].x, A[1].x, A[2].x), a1y = max3(A[0].y, A[1].y, A[2].y), a1z = max3(A[0].z, A[1].z, A[2].z);
+        for (int j = 0; j < TB; ++j) {
+            const V3* B = &wb[3 * j];
+            double b0x = min3(B[0].x, B[1].x, B[2].x), b0y = min3(B[0].y, B[1].y, B[2].y), b0z = min3(B[0].z, B[1].z, B[2].z);
+            double b1x = max3(B[0].x, B[1].x, B[2].x), b1y = max3(B[0].y, B[1].y, B[2].y), b1z = max3(B[0].z, B[1].z, B[2].z);
+            if (a1x < b0x || a0x > b1x || a1y < b0y || a0y > b1y || a1z < b0z || a0z > b1z) continue;
+            if (norm2(nB[j]) < 1e-30) continue;
+            V3 p0, p1;
+            int c0, c1;
+            if (!tri_tri_prov(A, nA[i], dAp[i], B, nB[j], dBp[j], p0, p1, c0, c1)) continue;
+            seg.push_back(p0.x); seg.push_back(p0.y); seg.push_back(p0.z);
+            seg.push_back(p1.x); seg.push_back(p1.y); seg.push_back(p1.z);
+            double J[36];
+            endpoint_jacobian(c0, A, B, nA[i], nB[j], p0, cA, cB, J);
+            EJ.insert(EJ.end(), J, J + 36);
+            endpoint_jacobian(c1, A, B, nA[i], nB[j], p1, cA, cB, J);
+            EJ.insert(EJ.end(), J, J + 36);
+        }
+    }
+    const int S = (int)(seg.size() / 6);
+    const double tol = mean_edge_a * 0.1, area_eps = mean_edge_a * mean_edge_a * 1e-4;
+    std::vector<double> vtx(1);
+    RecSink sink{rec, rec_cap, vtx.data(), 0, 0, ljac};
+    std::vector<double> vdummy(3 * 4 * (size_t)(S + 1));
+    sink.vtx = vdummy.data();
+    sink.vcap = (int)(vdummy.size() / 3);
+    TopoOut t = tp_topology_host(seg.data(), S, tol, area_eps, true, sub(cB, cA), sink, EJ.empty() ? nullptr : EJ.data());
+    if (t.status) return -1;
+    return t.loops;
 }
 
 }  // extern "C"

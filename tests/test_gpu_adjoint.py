@@ -1,0 +1,172 @@
+"""GPU: sensitivities of the contact pipeline and the differentiable
+minimal-energy step, against central finite differences (relative step
+1e-6 on parameters, small absolute steps on poses) of the same device
+forward."""
+import numpy as np
+import pytest
+
+import golden_inputs as GI
+import oracle as O
+from zdx import synth
+
+zdx = pytest.importorskip("zdx")
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = zdx.Context(0)
+    yield c
+    c.close()
+
+
+def _qexp(w):
+    th = np.linalg.norm(w)
+    if th == 0.0:
+        return np.array([1.0, 0.0, 0.0, 0.0])
+    return np.concatenate([[np.cos(th / 2)], np.sin(th / 2) * w / th])
+
+
+def _qmul(a, b):
+    w1, x1, y1, z1 = a
+    w2, x2, y2, z2 = b
+    return np.array([w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                     w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2])
+
+
+CASES = [("ellipsoid200", "ellipsoid200", [1.95, 0.0, 0.0], [1, 0, 0, 0]),
+         ("hull128", "star2a", [1.5, 0.3, -0.1], [0.9, 0.1, 0.3, -0.2]),
+         ("star2a", "star2b", [1.45, 0.1, -0.05], [0.8, -0.3, 0.2, 0.4])]
+
+
+def test_loop_jacobians_match_finite_differences(ctx):
+    m = GI.synth_meshes()
+    for na, nb, pB, qB in CASES:
+        a, b = ctx.register_mesh(m[na]), ctx.register_mesh(m[nb])
+        pA, qA = np.zeros(3), np.array([1.0, 0.0, 0.0, 0.0])
+        pB, qB = np.asarray(pB, float), np.asarray(qB, float) / np.linalg.norm(qB)
+        r0 = ctx.contact_pairs([a], [b], [pA], [qA], [pB], [qB], sensitivities=True, vertices=False)
+        n0 = len(r0["loops"])
+        assert n0 >= 1
+        h = 1e-7
+        num = np.zeros((n0, 72))
+        for k in range(12):
+            out = []
+            for sgn in (1, -1):
+                xi = np.zeros(12)
+                xi[k] = sgn * h
+                pa, pb = pA + xi[0:3], pB + xi[6:9]
+                qa, qb = _qmul(_qexp(xi[3:6]), qA), _qmul(_qexp(xi[9:12]), qB)
+                r = ctx.contact_pairs([a], [b], [pa], [qa], [pb], [qb], vertices=False)
+                assert len(r["loops"]) == n0
+                out.append(r["loops"])
+            d = (out[0] - out[1]) / (2 * h)
+            num[:, k:36:12] = d[:, 0:3]
+            num[:, 36 + k:72:12] = d[:, 3:6]
+        ana = r0["loop_jac"]
+        for l in range(n0):
+            for part in (slice(0, 36), slice(36, 72)):
+                scale = np.max(np.abs(ana[l, part]))
+                assert np.max(np.abs(ana[l, part] - num[l, part])) <= 2e-5 * scale, (na, nb, l)
+
+
+def test_pair_contact_backward_matches_finite_differences(ctx):
+    from zdx.autograd import PairContact
+
+    m = GI.synth_meshes()
+    ids = [(ctx.register_mesh(m[na]), ctx.register_mesh(m[nb])) for na, nb, _, _ in CASES]
+    pA = torch.zeros(3, 3, dtype=torch.float64)
+    qA = torch.tensor([[1.0, 0, 0, 0], [0.9, 0.2, -0.1, 0.3], [1.0, 0, 0, 0]], dtype=torch.float64)
+    pB = torch.tensor([c[2] for c in CASES], dtype=torch.float64)
+    qB = torch.tensor([c[3] for c in CASES], dtype=torch.float64)
+    ma = np.array([i[0] for i in ids])
+    mb = np.array([i[1] for i in ids])
+    rng = np.random.default_rng(4)
+
+    def f(pA, qA, pB, qB, w):
+        Sc, xc, lp = PairContact.apply(pA, qA, pB, qB, ctx, ma, mb, True)
+        return (Sc * w[0][: len(Sc)]).sum() + (xc * w[1][: len(xc)]).sum()
+
+    w = (torch.from_numpy(rng.normal(size=(64, 3))), torch.from_numpy(rng.normal(size=(64, 3))))
+    X = [t.clone().requires_grad_(True) for t in (pA, qA, pB, qB)]
+    f(*X, w).backward()
+    for idx, name in enumerate(("posA", "quatA", "posB", "quatB")):
+        g = X[idx].grad.numpy()
+        num = np.zeros_like(g)
+        base = [t.detach().clone() for t in (pA, qA, pB, qB)]
+        h = 1e-7
+        for pos in np.ndindex(*g.shape):
+            vals = []
+            for sgn in (1, -1):
+                Y = [t.clone() for t in base]
+                Y[idx][pos] += sgn * h
+                with torch.no_grad():
+                    vals.append(float(f(*Y, w)))
+            num[pos] = (vals[0] - vals[1]) / (2 * h)
+        scale = np.max(np.abs(num)) + 1e-12
+        assert np.max(np.abs(g - num)) <= 1e-5 * scale, (name, g, num)
+
+
+def _drop_scene(ctx, n_side=2, seed=4):
+    """Hulls of 128 sphere points (r = 5 mm) just above a floor slab."""
+    from zdx.autograd import Scene
+
+    rng = np.random.default_rng(seed)
+    hull = synth.sphere_hull(128, 7) * 0.005
+    mid = ctx.register_mesh(hull)
+    _, props, _ = O.build_mesh(hull)
+    V = props[0]
+    dens = 2500.0
+    I = props[7:16].reshape(3, 3) * dens
+    floor = synth.box_wall(0.2, 0.2, 0.02)
+    fid = ctx.register_mesh(floor)
+    n = n_side * n_side
+    pos = np.zeros((n, 3))
+    for k in range(n):
+        pos[k] = [(k % n_side) * 0.012, (k // n_side) * 0.012, 0.0052 + 0.0003 * rng.random()]
+    quat = rng.normal(size=(n, 4))
+    quat /= np.linalg.norm(quat, axis=1, keepdims=True)
+    rad = np.full(n, props[4])
+    scene = Scene(zctx=ctx, mesh=np.full(n, mid), radius=rad, mass=torch.full((n,), dens * V, dtype=torch.float64),
+                  Ibinv=torch.from_numpy(np.linalg.inv(I)).expand(n, 3, 3).clone(),
+                  req=torch.full((n,), (3 * V / (4 * np.pi)) ** (1 / 3), dtype=torch.float64), floor_mesh=fid,
+                  floor_pos=np.array([0.0, 0.0, -0.01]), floor_top=0.0,
+                  gravity=torch.tensor([0.0, 0.0, -9.81], dtype=torch.float64), dt=1e-5)
+    return scene, torch.from_numpy(pos), torch.from_numpy(quat)
+
+
+def test_minimal_model_gradients_match_finite_differences(ctx):
+    """d loss / d (k, e, v0) through 300 steps with checkpoints every 50
+    (adjoint) against central differences with relative step 1e-6."""
+    from zdx.autograd import simulate
+
+    scene, pos0, quat0 = _drop_scene(ctx)
+    n = pos0.shape[0]
+    target = pos0.clone() + torch.tensor([0.0, 0.0, 0.0002])
+    v0 = torch.zeros(n, 3, dtype=torch.float64)
+    v0[:, 2] = -0.4
+    k0 = torch.tensor(8e6, dtype=torch.float64)
+    e0 = torch.tensor(0.5, dtype=torch.float64)
+
+    def loss(k, e, v):
+        p, q, vv, L = simulate(scene, k, e, pos0, quat0, v, torch.zeros(n, 3, dtype=torch.float64), 300, 50)
+        return ((p - target) ** 2).sum()
+
+    k, e, v = k0.clone().requires_grad_(True), e0.clone().requires_grad_(True), v0.clone().requires_grad_(True)
+    L0 = loss(k, e, v)
+    L0.backward()
+    with torch.no_grad():
+        assert loss(k0, e0, v0) == L0.detach()     # deterministic forward
+        rel = 1e-6
+        checks = [("k", k.grad.item(), lambda s: loss(k0 * (1 + s * rel), e0, v0), float(k0) * rel),
+                  ("e", e.grad.item(), lambda s: loss(k0, e0 * (1 + s * rel), v0), float(e0) * rel)]
+        for i in range(n):
+            def fv(s, i=i):
+                vv = v0.clone()
+                vv[i, 2] *= (1 + s * rel)
+                return loss(k0, e0, vv)
+            checks.append((f"v0[{i}].z", v.grad[i, 2].item(), fv, abs(float(v0[i, 2])) * rel))
+        for name, g, fn, hstep in checks:
+            fd = (float(fn(1)) - float(fn(-1))) / (2 * hstep)
+            assert abs(g - fd) <= 1e-4 * max(abs(fd), abs(g)) + 1e-12, (name, g, fd)

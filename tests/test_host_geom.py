@@ -133,3 +133,95 @@ def test_pair_pipeline_host_random(H):
         ns, nl = _check_pair(H, ta, tb, pose, pa, split, f"{a}-{b}-{it}")
         n_multi += nl > 1
     assert n_multi >= 2
+
+
+def test_endpoint_jacobian_matches_finite_differences(H):
+    """d(endpoint)/d(rigid motions of the two bodies) from the edge-plane
+    provenance (zdx_geom.h endpoint_jacobian) against central differences of
+    the product's own tri-tri on rotated/translated triangles."""
+    dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    H.ht_endpoint.argtypes = [dp, dp, dp, dp, dp, dp, np.ctypeslib.ndpointer(dtype=np.int32), dp, dp]
+    rng = np.random.default_rng(17)
+    checked = 0
+    for _ in range(400):
+        a9 = rng.normal(size=9)
+        b9 = rng.normal(size=9) * 0.8 + 0.1
+        cA = a9.reshape(3, 3).mean(0) + rng.normal(size=3) * 0.3
+        cB = b9.reshape(3, 3).mean(0) + rng.normal(size=3) * 0.3
+        p = np.zeros(6)
+        codes = np.zeros(2, dtype=np.int32)
+        J0, J1 = np.zeros(36), np.zeros(36)
+        xi0 = np.zeros(12)
+        if H.ht_endpoint(a9, b9, cA, cB, xi0, p, codes, J0, J1) != 1:
+            continue
+        if codes[0] & 24 or codes[1] & 24:
+            continue
+        h = 1e-6
+        num = np.zeros((6, 12))
+        ok = True
+        for k in range(12):
+            for sgn, store in ((1, 0), (-1, 1)):
+                xi = np.zeros(12)
+                xi[k] = sgn * h
+                q = np.zeros(6)
+                c2 = np.zeros(2, dtype=np.int32)
+                if H.ht_endpoint(a9, b9, cA, cB, xi, q, c2, np.zeros(36), np.zeros(36)) != 1 or not np.array_equal(c2, codes):
+                    ok = False
+                    break
+                if sgn == 1:
+                    qp = q
+                else:
+                    num[:, k] = (qp - q) / (2 * h)
+            if not ok:
+                break
+        if not ok:
+            continue
+        ana = np.vstack([J0.reshape(3, 12), J1.reshape(3, 12)])
+        scale = np.max(np.abs(ana)) + 1e-12
+        assert np.max(np.abs(ana - num)) <= 1e-6 * scale, (codes, np.max(np.abs(ana - num)), scale)
+        checked += 1
+    assert checked >= 50
+
+
+def test_loop_sensitivities_match_finite_differences(H):
+    """dSn/dxi and dGn/dxi of every loop (vertex Jacobians = mean of the
+    snapped endpoints' Jacobians, then the polyline sums) against central
+    differences of the whole per-pair pipeline under rigid motions."""
+    dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    H.ht_pair_jac.argtypes = [dp, C.c_int, dp, C.c_int, dp, dp, dp, dp, dp, C.c_double, dp, C.c_int, dp]
+    m = GI.synth_meshes()
+    cases = [("ellipsoid200", "ellipsoid200", [1.95, 0.0, 0.0], [1, 0, 0, 0]),
+             ("hull128", "star2a", [1.5, 0.3, -0.1], [0.9, 0.1, 0.3, -0.2]),
+             ("star2a", "star2b", [1.45, 0.1, -0.05], [0.8, -0.3, 0.2, 0.4])]
+    done = 0
+    for na, nb, pB, qB in cases:
+        ta, pa, _ = O.build_mesh(m[na])
+        tb, _, _ = O.build_mesh(m[nb])
+        args = lambda xi, rec, lj: H.ht_pair_jac(np.ascontiguousarray(ta.reshape(-1)), len(ta),
+                                                 np.ascontiguousarray(tb.reshape(-1)), len(tb), np.zeros(3),
+                                                 np.array([1.0, 0, 0, 0]), np.asarray(pB, float),
+                                                 np.asarray(qB, float) / np.linalg.norm(qB), xi, pa[5], rec, 64, lj)
+        rec0, lj0 = np.zeros((64, 28)), np.zeros((64, 72))
+        n0 = args(np.zeros(12), rec0, lj0)
+        assert n0 >= 1, na
+        h = 1e-7
+        num = np.zeros((n0, 72))
+        for k in range(12):
+            out = []
+            for sgn in (1, -1):
+                xi = np.zeros(12)
+                xi[k] = sgn * h
+                rec, lj = np.zeros((64, 28)), np.zeros((64, 72))
+                assert args(xi, rec, lj) == n0
+                out.append(rec[:n0])
+            d = (out[0] - out[1]) / (2 * h)
+            for l in range(n0):
+                num[l, 0 + k:36:12] = d[l, 0:3]
+                num[l, 36 + k:72:12] = d[l, 3:6]
+        ana = lj0[:n0]
+        for l in range(n0):
+            for part in (slice(0, 36), slice(36, 72)):
+                scale = np.max(np.abs(ana[l, part])) + 1e-300
+                assert np.max(np.abs(ana[l, part] - num[l, part])) <= 2e-5 * scale, (na, nb, l, part)
+        done += 1
+    assert done == len(cases)

@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -170,6 +171,8 @@ struct zdx_ctx {
         DBuf<int32_t> ncontrib, comps, loops_raw, tstatus, topo_list, topo_n;
         TopoWork tw;
         DBuf<double> rec, vtx;
+        DBuf<int32_t> prov;            // sensitivity runs
+        DBuf<double> ej, vj, ljac;
         int32_t stage_cap = 1024;
         int64_t arena_cap = 1 << 20;
         int32_t lmax = 32;
@@ -378,7 +381,7 @@ struct zdx_sim {
         return b;
     }
     DPairs dpairs(int par, bool lpt) {
-        DPairs P;
+        DPairs P{};
         P.cap = pair_cap; P.n = ps[par].n.p; P.pi = ps[par].pi.p; P.pj = ps[par].pj.p;
         P.order = lpt ? d_order.p : nullptr;
         P.seg_off = d_seg_off.p; P.nseg = d_nseg.p; P.tri_checks = d_tri_checks.p; P.tri_tests = d_tri_tests.p;
@@ -752,7 +755,15 @@ void step_once(zdx_sim* s, bool batched) {
     for (int attempt = 0; attempt < 12; ++attempt) {
         const int par = s->parity;
         launch_fill_i32(s->d_status.p, 0, 1, st);
-        if (batched) {
+        // ZDX_GRAPHS=0 runs the batched sequence without graph capture (same
+        // kernels and launch shapes), for tools that cannot trace graphs
+        static const bool use_graphs = [] {
+            const char* e = std::getenv("ZDX_GRAPHS");
+            return !(e && e[0] == '0');
+        }();
+        if (batched && !use_graphs) {
+            enqueue_step(s, par, true);
+        } else if (batched) {
             if (!s->graphs_valid) {
                 for (int k = 0; k < 2; ++k) {
                     HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -919,13 +930,18 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
         const bool want_vtx = out->loop_vtx != nullptr;
         for (int attempt = 0; attempt < 10; ++attempt) {
             A.arena.ensure((size_t)A.arena_cap * 6);
-            A.stage.ensure((size_t)groups * 6 * A.stage_cap);
-            DPairs P;
+            const bool sens = opts->sensitivities != 0;
+            A.stage.ensure((size_t)groups * (sens ? 8 : 6) * A.stage_cap);
+            DPairs P{};
             P.cap = n; P.n = A.npairs.p; P.pi = A.pi.p; P.pj = A.pj.p; P.order = batched ? A.order.p : nullptr;
             P.seg_off = A.seg_off.p; P.nseg = A.nseg.p; P.tri_checks = A.tri_checks.p; P.tri_tests = A.tri_tests.p;
             P.tri_hits = A.tri_hits.p; P.tri_scanned = A.tri_scanned.p; P.arena = A.arena.p; P.arena_cap = A.arena_cap; P.arena_top = A.arena_top.p;
             P.status = A.status.p;
             P.topo_list = A.topo_list.p; P.topo_n = A.topo_n.p;
+            if (sens) {
+                A.prov.ensure((size_t)A.arena_cap * 4);
+                P.prov = A.prov.p;
+            }
             launch_fill_i32(A.status.p, 0, 1, st);
             launch_fill_i32(A.topo_n.p, 0, 1, st);
             launch_fill_u64(A.arena_top.p, 0ull, 1, st);
@@ -954,6 +970,12 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
             A.tw.fill(T);
             A.rec.ensure((size_t)n * T.lmax * 28);
             if (want_vtx) A.vtx.ensure((size_t)n * T.vmax * 3);
+            if (sens) {
+                A.ej.ensure((size_t)A.arena_cap * 72);
+                A.vj.ensure((size_t)A.arena_cap * 72);
+                A.ljac.ensure((size_t)n * T.lmax * 72);
+                T.ej = A.ej.p; T.vj = A.vj.p; T.ljac = A.ljac.p;
+            }
             T.rec = A.rec.p; T.vtx = want_vtx ? A.vtx.p : nullptr;
             launch_topo_zero(P, T, nullptr, n, st);
             launch_topo_api(P, M, B, T, opts->split_contacts, st);
@@ -981,6 +1003,11 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
             HIPC(hipMemcpy(tc.data(), A.tri_checks.p, n * 8, hipMemcpyDeviceToHost));
             std::vector<double> rec((size_t)n * T.lmax * 28);
             HIPC(hipMemcpy(rec.data(), A.rec.p, rec.size() * 8, hipMemcpyDeviceToHost));
+            std::vector<double> ljac;
+            if (sens && out->loop_jac) {
+                ljac.resize((size_t)n * T.lmax * 72);
+                HIPC(hipMemcpy(ljac.data(), A.ljac.p, ljac.size() * 8, hipMemcpyDeviceToHost));
+            }
             int64_t nloops = 0, nseg = 0, nvtx = 0;
             for (int p = 0; p < n; ++p) {
                 nloops += nl[p];
@@ -1016,6 +1043,8 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
                 for (int l = 0; l < nl[p]; ++l) {
                     const double* r = &rec[((size_t)p * T.lmax + l) * 28];
                     if (out->loop_rec) std::memcpy(out->loop_rec + 28 * (lo + l), r, 28 * sizeof(double));
+                    if (!ljac.empty())
+                        std::memcpy(out->loop_jac + 72 * (lo + l), &ljac[((size_t)p * T.lmax + l) * 72], 72 * sizeof(double));
                     int nv = (int)r[24];
                     if (want_vtx) {
                         out->vtx_offset[lo + l] = (int32_t)vo;

@@ -228,6 +228,116 @@ ZDX_HD inline bool tri_tri(const V3* A, const V3& nA, double dA, const V3* B, co
     return true;
 }
 
+// tri_tri with the provenance of the two endpoints: code r = candidate index
+// (0..2: edge r of A crossing B's plane, 3..5: edge r-3 of B crossing A's
+// plane), + 8 when the edge parameter was clamped to 0, + 16 when to 1.  The
+// arithmetic is tri_tri's, so the endpoints are identical.
+ZDX_HD inline bool seg_plane_c(const V3& p0, const V3& p1, const V3& n, double d, V3& out, int& clamp) {
+    V3 u = sub(p1, p0);
+    double den = dot(n, u);
+    double num = -(dot(n, p0) + d);
+    if (fabs(den) < 1e-30) return false;
+    double t = num / den;
+    if (t < -1e-10 || t > 1.0 + 1e-10) return false;
+    clamp = t < 0.0 ? 8 : (t > 1.0 ? 16 : 0);
+    t = dmin(1.0, dmax(0.0, t));
+    out = add(p0, mul(u, t));
+    return true;
+}
+ZDX_HD inline bool tri_tri_prov(const V3* A, const V3& nA, double dA, const V3* B, const V3& nB, double dB,
+                                V3& o0, V3& o1, int& code0, int& code1) {
+    V3 tau = cross(nA, nB);
+    double lt = norm(tau);
+    if (lt < 1e-20) return false;
+    tau = dvd(tau, lt);
+    V3 raw[6];
+    int rc[6];
+    int nr = 0;
+    for (int i = 0; i < 3; ++i) {
+        V3 p;
+        int cl = 0;
+        if (seg_plane_c(A[i], A[(i + 1) % 3], nB, dB, p, cl) && point_in_tri(p, B, nB)) { rc[nr] = i | cl; raw[nr++] = p; }
+    }
+    for (int i = 0; i < 3; ++i) {
+        V3 p;
+        int cl = 0;
+        if (seg_plane_c(B[i], B[(i + 1) % 3], nA, dA, p, cl) && point_in_tri(p, A, nA)) { rc[nr] = (3 + i) | cl; raw[nr++] = p; }
+    }
+    V3 c[6];
+    int cc[6];
+    int nc = 0;
+    const double t2 = 1e-9 * 1e-9;
+    for (int i = 0; i < nr; ++i) {
+        bool keep = true;
+        for (int j = 0; j < nc; ++j)
+            if (norm2(sub(raw[i], c[j])) <= t2) { keep = false; break; }
+        if (keep) { cc[nc] = rc[i]; c[nc++] = raw[i]; }
+    }
+    if (nc < 2) return false;
+    double mn = INFINITY, mx = -INFINITY;
+    int i0 = -1, i1 = -1;
+    for (int i = 0; i < nc; ++i) {
+        double t = dot(c[i], tau);
+        if (t < mn) { mn = t; i0 = i; }
+        if (t > mx) { mx = t; i1 = i; }
+    }
+    if (i0 == i1) return false;
+    V3 p0 = c[i0], p1 = c[i1];
+    int k0 = cc[i0], k1 = cc[i1];
+    if (norm2(sub(p1, p0)) < 1e-18) return false;
+    if (dot(sub(p1, p0), tau) < 0.0) { V3 t = p0; p0 = p1; p1 = t; int k = k0; k0 = k1; k1 = k; }
+    o0 = p0; o1 = p1;
+    code0 = k0; code1 = k1;
+    return true;
+}
+
+// Sensitivity of a segment endpoint to rigid motions of the two bodies.
+// Motion parameters xi = (dA, wA, dB, wB): body X moves its material points
+// by u_X(x) = d_X + w_X x (x - c_X).  An endpoint on edge (a, b) of body X
+// crossing the plane (n) of a triangle of body Y moves by
+//   dp = u_X(p) + e [n . (u_Y(p) - u_X(p))] / (n . e),   e = b - a
+// (a clamped endpoint is the edge's vertex and moves with X).  J: 3 x 12,
+// row-major.
+ZDX_HD inline void endpoint_jacobian(int code, const V3* A, const V3* B, const V3& nA, const V3& nB, const V3& p,
+                                     const V3& cA, const V3& cB, double* J) {
+    const int r = code & 7;
+    const bool onA = r < 3;                   // the edge belongs to A
+    const int ei = onA ? r : r - 3;
+    const V3* T = onA ? A : B;
+    const V3 e = sub(T[(ei + 1) % 3], T[ei]);
+    const V3 n = onA ? nB : nA;               // plane of the other body's triangle
+    const int cx = onA ? 0 : 6, cy = onA ? 6 : 0;
+    const V3 rx = sub(p, onA ? cA : cB), ry = sub(p, onA ? cB : cA);
+    for (int k = 0; k < 36; ++k) J[k] = 0.0;
+    // u_X(p) = d_X - [rx]x w_X
+    auto put_rigid = [&](int col, const V3& rr, double s) {
+        // rows: x, y, z; translation block I, rotation block -[rr]x
+        J[0 * 12 + col + 0] += s; J[1 * 12 + col + 1] += s; J[2 * 12 + col + 2] += s;
+        J[0 * 12 + col + 4] += s * rr.z;  J[0 * 12 + col + 5] -= s * rr.y;
+        J[1 * 12 + col + 3] -= s * rr.z;  J[1 * 12 + col + 5] += s * rr.x;
+        J[2 * 12 + col + 3] += s * rr.y;  J[2 * 12 + col + 4] -= s * rr.x;
+    };
+    put_rigid(cx, rx, 1.0);
+    if (code & 24) return;
+    const double ne = dot(n, e);
+    // row vector g = n^T (J_Y - J_X) / (n . e); J += e g
+    double g[12];
+    for (int k = 0; k < 12; ++k) g[k] = 0.0;
+    auto row_rigid = [&](int col, const V3& rr, double s) {
+        // n^T [I, -[rr]x] = [n, (rr x n)^T]  (n . (w x rr) = w . (rr x n))
+        const V3 rn = cross(rr, n);
+        g[col + 0] += s * n.x; g[col + 1] += s * n.y; g[col + 2] += s * n.z;
+        g[col + 3] += s * rn.x; g[col + 4] += s * rn.y; g[col + 5] += s * rn.z;
+    };
+    row_rigid(cy, ry, 1.0 / ne);
+    row_rigid(cx, rx, -1.0 / ne);
+    for (int k = 0; k < 12; ++k) {
+        J[0 * 12 + k] += e.x * g[k];
+        J[1 * 12 + k] += e.y * g[k];
+        J[2 * 12 + k] += e.z * g[k];
+    }
+}
+
 // ------------------------------------------------------------ strided scratch
 // Per-thread scratch arrays are interleaved across the threads of a launch
 // (element k of thread t at base[k * stride + t]) so that lanes of a wave that

@@ -142,6 +142,7 @@ struct DPairs {
     int32_t* topo_n;         // device count of topo_list
     unsigned long long* arena_top;
     int32_t* status;         // global error bits
+    int32_t* prov;           // optional (sensitivity runs): 4 ints per arena segment: ta, tb, endpoint codes
 };
 
 enum : int32_t {

@@ -93,6 +93,10 @@ struct TopoBufs {
     double* V;                 // 2 vertices (6 doubles) per arena segment
     int32_t* vcell;            // 2 vertex cells (6 ints) per arena segment
     int32_t* E;                // 1 edge (2 ints) per arena segment
+    // sensitivities (API runs with provenance; nullptr otherwise)
+    double* ej;                // 2 endpoint Jacobians (72 doubles) per arena segment
+    double* vj;                // 2 vertex Jacobians (72 doubles) per arena segment
+    double* ljac;              // pair * lmax * 72: dSn/dxi, dGn/dxi per loop
     // walk scratch: LDS of lds_bytes when a pair fits, else per-workgroup slices
     int32_t* scr_i;
     double* scr_d;

@@ -515,7 +515,9 @@ __device__ inline void load_tri(const double* wv, const int32_t* ti, int t, V3* 
     out[2] = ld3(wv + 3 * ti[3 * t + 2]);
 }
 
-template <int BS, int K>
+// PROV: also record each segment's provenance (ta, tb, endpoint codes of
+// tri_tri_prov) in P.prov, staged per workgroup after its segments.
+template <int BS, int K, bool PROV>
 __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, int32_t* list_i, float* list_f,
                                                double* stage, int32_t stage_cap, int32_t tmax, int32_t use_queue,
                                                unsigned int* queue, unsigned long long* kt) {
@@ -530,7 +532,8 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
     int32_t* LB = LA + tmax;
     float* FA = list_f + (int64_t)blockIdx.x * 12 * tmax;
     float* FB = FA + 6 * tmax;
-    double* ST = stage + (int64_t)blockIdx.x * 6 * stage_cap;
+    double* ST = stage + (int64_t)blockIdx.x * (PROV ? 8 : 6) * stage_cap;
+    int32_t* STP = (int32_t*)(ST + 6 * (int64_t)stage_cap);   // PROV: 4 ints per staged segment
     const int npairs = *P.n;
     int item = blockIdx.x;
     while (true) {
@@ -624,6 +627,7 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
                 const int h = hb + tid;
                 bool sg = false;
                 V3 p0, p1;
+                int pv[4] = {0, 0, 0, 0};
                 if (h < nh) {
                     const int ta = LA[s_ha[h]], tb = LB[s_hb[h]];
                     V3 A[3], Bt[3];
@@ -639,7 +643,12 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
                         V3 nA3, nB3; double dA, dB;
                         plane_of(A[0], A[1], A[2], nA3, dA);
                         plane_of(Bt[0], Bt[1], Bt[2], nB3, dB);
-                        sg = tri_tri(A, nA3, dA, Bt, nB3, dB, p0, p1);
+                        if (PROV) {
+                            pv[0] = ta; pv[1] = tb;
+                            sg = tri_tri_prov(A, nA3, dA, Bt, nB3, dB, p0, p1, pv[2], pv[3]);
+                        } else {
+                            sg = tri_tri(A, nA3, dA, Bt, nB3, dB, p0, p1);
+                        }
                     }
                 }
                 int ns;
@@ -647,6 +656,8 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
                 if (sg && nseg + r < stage_cap) {
                     double* o = ST + 6 * (int64_t)(nseg + r);
                     o[0] = p0.x; o[1] = p0.y; o[2] = p0.z; o[3] = p1.x; o[4] = p1.y; o[5] = p1.z;
+                    if (PROV)
+                        for (int q = 0; q < 4; ++q) STP[4 * (nseg + r) + q] = pv[q];
                 }
                 nseg += ns;
             }
@@ -671,6 +682,8 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
         if (bse != ~0ull) {
             double* dst = P.arena + 6 * (int64_t)bse;
             for (int k = tid; k < 6 * nstore; k += BS) dst[k] = ST[k];
+            if (PROV)
+                for (int k = tid; k < 4 * nstore; k += BS) P.prov[4 * (int64_t)bse + k] = STP[k];
         }
         if (tid == 0) {
             P.seg_off[p] = bse != ~0ull ? (int64_t)bse : 0;
@@ -720,7 +733,7 @@ struct SimSink {
     double dt;
     V3* ds;
     __device__ int operator()(int li, int, int, const V3&, const V3&, const V3& nA, const V3& xc, double area, int,
-                              SArr<int>, SArr<double>) {
+                              SArr<int>, SArr<double>, const double*) {
         Contact c = pp_force(nA, area, xc, *a, *b, *ma, *mb, dt, *ds);
         if (li >= lmax) return TOPO_LOOP_OVERFLOW;
         double* o = contrib + 9 * (int64_t)li;
@@ -812,6 +825,37 @@ __global__ void __launch_bounds__(256) k_tp_pass_a(DPairs P, DMeshes M, DBodies 
     }
 }
 
+// Sensitivity runs: Jacobian of every segment endpoint (endpoint_jacobian)
+// from the narrow phase's provenance; body A of a pair is pi, B is pj.
+__global__ void __launch_bounds__(256) k_tp_endpoint_jac(DPairs P, DMeshes M, DBodies B, TopoBufs T) {
+    const int n = *P.topo_n;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const int p = P.topo_list[it];
+        const int i = P.pi[p], j = P.pj[p];
+        const int S = P.nseg[p];
+        const int64_t sb = P.seg_off[p];
+        const int mA = B.mesh[i], mB = B.mesh[j];
+        const int32_t* tiA = M.tri_idx + 3 * (int64_t)M.tri_off[mA];
+        const int32_t* tiB = M.tri_idx + 3 * (int64_t)M.tri_off[mB];
+        const double* wvA = B.wv + 3 * B.wv_off[i];
+        const double* wvB = B.wv + 3 * B.wv_off[j];
+        const V3 cA = ld3(B.state + 16 * (int64_t)i), cB = ld3(B.state + 16 * (int64_t)j);
+        for (int k = threadIdx.x; k < 2 * S; k += blockDim.x) {
+            const int s = k >> 1, e = k & 1;
+            const int32_t* pv = P.prov + 4 * (sb + s);
+            V3 A[3], Bt[3];
+            load_tri(wvA, tiA, pv[0], A);
+            load_tri(wvB, tiB, pv[1], Bt);
+            V3 nA, nB;
+            double dA, dB;
+            plane_of(A[0], A[1], A[2], nA, dA);
+            plane_of(Bt[0], Bt[1], Bt[2], nB, dB);
+            const V3 q = ld3(P.arena + 6 * (sb + s) + 3 * e);
+            endpoint_jacobian(pv[2 + e], A, Bt, nA, nB, q, cA, cB, T.ej + 72 * (sb + s) + 36 * e);
+        }
+    }
+}
+
 // Walk scratch of one pair: LDS when it fits, else this workgroup's global
 // slice.  The two cases call the walk separately so each inlined copy
 // addresses its scratch directly (LDS instructions for the first).
@@ -846,7 +890,8 @@ __global__ void __launch_bounds__(64) k_tp_walk_b(DPairs P, DMeshes M, DBodies B
                 __syncthreads();
                 return tp_components_and_vertices(wp, sgl, S, h, split != 0, T.adj + ao, T.snapw + so, I, D,
                                                   T.comp + TPC_N * sb, T.V + 6 * sb, T.vcell + 6 * sb,
-                                                  T.E + 2 * sb, &tjn);
+                                                  T.E + 2 * sb, &tjn, T.ej ? T.ej + 72 * sb : nullptr,
+                                                  T.vj ? T.vj + 72 * sb : nullptr);
             });
         }
         if (lane_id() == 0) {
@@ -907,7 +952,8 @@ __device__ inline TopoOut tp_walk_d(const LaneDev& wp, const TopoBufs& T, int S,
         for (int k = lane_id(); k < 2 * ne_all; k += WAVE) El[k] = T.E[2 * sb + k];
         for (int k = lane_id(); k < 3 * nv_all; k += WAVE) vcl[k] = T.vcell[6 * sb + k];
         __syncthreads();
-        return tp_loops(wp, S, ncomp, h, area_eps, dir, comp, T.V + 6 * sb, vcl, El, T.tjw + to, I, D, sink);
+        return tp_loops(wp, S, ncomp, h, area_eps, dir, comp, T.V + 6 * sb, vcl, El, T.tjw + to, I, D, sink,
+                        T.vj ? T.vj + 72 * sb : nullptr, D + tp_d_dbls(S));
     });
 }
 
@@ -985,9 +1031,12 @@ struct ApiSink {
     int lmax, vmax, nv;
     V3 posA, posB;
     double k;
+    double* ljac;   // lmax * 72 of this pair, or nullptr
     __device__ int operator()(int li, int comp, int raw, const V3& Sn, const V3& Gn, const V3& nA, const V3& xc,
-                              double area, int plen, SArr<int> path, SArr<double> Vs) {
+                              double area, int plen, SArr<int> path, SArr<double> Vs, const double* jac) {
         if (li >= lmax) return TOPO_LOOP_OVERFLOW;
+        if (ljac && jac)
+            for (int c = lane_id(); c < 72; c += WAVE) ljac[72 * (int64_t)li + c] = jac[c];
         double* r = rec + ZDX_LOOP_REC_D * (int64_t)li;
         r[0] = Sn.x; r[1] = Sn.y; r[2] = Sn.z; r[3] = Gn.x; r[4] = Gn.y; r[5] = Gn.z;
         r[6] = nA.x; r[7] = nA.y; r[8] = nA.z; r[9] = xc.x; r[10] = xc.y; r[11] = xc.z;
@@ -1025,7 +1074,7 @@ __global__ void __launch_bounds__(64) k_tp_walk_d_api(DPairs P, DMeshes M, DBodi
         const int mA = B.mesh[i];
         const V3 pa = ld3(B.state + 16 * (int64_t)i), pb = ld3(B.state + 16 * (int64_t)j);
         ApiSink sink{T.rec + 28 * (int64_t)p * T.lmax, T.vtx ? T.vtx + 3 * (int64_t)p * T.vmax : nullptr,
-                     T.lmax, T.vmax, 0, pa, pb, T.k_stiff};
+                     T.lmax, T.vmax, 0, pa, pb, T.k_stiff, T.ljac ? T.ljac + 72 * (int64_t)p * T.lmax : nullptr};
         const int ncomp = T.ncomp[it];
         const long long to = T.tj_off[it];
         TopoOut o;
@@ -1284,12 +1333,20 @@ void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s, unsigned long lo
 template <int BS, int K>
 static void narrow_t(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
                      double* st, int32_t sc, int32_t tmax, unsigned int* q, hipStream_t s, unsigned long long* kt) {
-    hipLaunchKernelGGL((k_narrow<BS, K>), dim3(cfg.groups), dim3(BS), 0, s, B, M, P, li, lf, st, sc, tmax,
-                       cfg.use_queue, q, kt);
+    if (P.prov)
+        hipLaunchKernelGGL((k_narrow<BS, K, true>), dim3(cfg.groups), dim3(BS), 0, s, B, M, P, li, lf, st, sc, tmax,
+                           cfg.use_queue, q, kt);
+    else
+        hipLaunchKernelGGL((k_narrow<BS, K, false>), dim3(cfg.groups), dim3(BS), 0, s, B, M, P, li, lf, st, sc, tmax,
+                           cfg.use_queue, q, kt);
 }
 
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
                    double* st, int32_t sc, int32_t tmax, unsigned int* q, hipStream_t s, unsigned long long* kt) {
+    if (P.prov) {   // sensitivity runs: one shape
+        narrow_t<256, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt);
+        return;
+    }
     const int key = cfg.block * 100 + cfg.items;
     switch (key) {
         case 12804: narrow_t<128, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
@@ -1332,6 +1389,7 @@ void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const 
 void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int32_t split,
                      hipStream_t s) {
     launch_fill_u64(T.top, 0ull, 3, s);
+    if (T.ej) hipLaunchKernelGGL(k_tp_endpoint_jac, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T);
     hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T, nullptr);
     hipLaunchKernelGGL(k_tp_walk_b, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, T, split,
                        T.lds_bytes);

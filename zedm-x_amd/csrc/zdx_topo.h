@@ -111,10 +111,14 @@ ZDX_HD inline int tp_row_words(int nv) { return (nv + 63) >> 6; }
 // is the snap hit of endpoint pair t.  Outputs: comp (ncomp x TPC_N), V (3 per
 // vertex), vcell (3 per vertex), E (2 per edge, vertex ids local to the
 // component).  Returns ncomp; *tj_words = total T-junction words.
+// Optional sensitivities: EJ (36 per endpoint, point order 2s+e) gives VJ (36
+// per vertex, same numbering as V): the mean of its endpoints' Jacobians,
+// summed in point order.
 template <class WP>
 ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, int S, double h, bool split,
                                              const uint8_t* adj, const uint64_t* snapw, int* I, double* D,
-                                             int* comp, double* V, int* vcell, int* E, int* tj_words) {
+                                             int* comp, double* V, int* vcell, int* E, int* tj_words,
+                                             const double* EJ = nullptr, double* VJ = nullptr) {
     const int W = WP::W;
     const int ln = wp.lane();
     const int n = 2 * S;
@@ -237,15 +241,22 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
                 rootvid[r] = nv;
                 Vs[3 * nv] = p[0]; Vs[3 * nv + 1] = p[1]; Vs[3 * nv + 2] = p[2];
                 cnt[nv] = 1;
+                if (VJ)
+                    for (int c = ln; c < 36; c += W) VJ[36 * (int64_t)(vb + nv) + c] = EJ[36 * (int64_t)pt + c];
                 ++nv;
             } else {
                 const int v = rootvid[r];
                 Vs[3 * v] += p[0]; Vs[3 * v + 1] += p[1]; Vs[3 * v + 2] += p[2];
                 cnt[v] += 1;
+                if (VJ)
+                    for (int c = ln; c < 36; c += W) VJ[36 * (int64_t)(vb + v) + c] += EJ[36 * (int64_t)pt + c];
             }
             ptv[pt] = rootvid[r];
         }
         wp.sync();
+        if (VJ)
+            for (int v = 0; v < nv; ++v)
+                for (int c = ln; c < 36; c += W) VJ[36 * (int64_t)(vb + v) + c] /= (double)cnt[v];
         for (int v = ln; v < nv; v += W) {
             const double cn = (double)cnt[v];
             const V3 m = v3(Vs[3 * v] / cn, Vs[3 * v + 1] / cn, Vs[3 * v + 2] / cn);
@@ -299,14 +310,57 @@ ZDX_HD inline bool tp_tj_hit(const double* V, const int* vcell, const int* E, do
 ZDX_HD inline int64_t tp_d_ints(int S) { return 4 * (int64_t)(2 * S) + 2 * (4 * (int64_t)S) * 3 + 2 * (4 * (int64_t)S) + 2 * (2 * (int64_t)S) + 8; }
 ZDX_HD inline int64_t tp_d_dbls(int S) { return 3 * (2 * (int64_t)S) + 2 * (int64_t)S + 8; }
 
+// Sensitivities of a loop's Sn and Gn (accumulate_Sn_Gn_from_polyline) to
+// the motion parameters, from the vertex Jacobians VJ (36 per vertex):
+//   dSn = -1/2 sum_m [p_{m+1} - p_{m-1}]x J_m
+//   dGn = sum_m dg/dp0 J_m + dg/dp1 J_{m+1},  g = -1/3 dx (p0.p1 + 1/3 |dx|^2)
+// out: 72 doubles (dSn 3 x 12, then dGn 3 x 12); entry c computed by lane c mod W.
+template <class WP>
+ZDX_HD inline void tp_loop_jacobian(const WP& wp, const int* path, int plen, const double* Vs, const double* VJ,
+                                    double* out) {
+    const int W = WP::W;
+    for (int c = wp.lane(); c < 72; c += W) {
+        const int row = (c % 36) / 12, col = c % 12;
+        double acc = 0.0;
+        for (int m = 0; m < plen; ++m) {
+            const int vm = path[m], vn = path[(m + 1) % plen], vp = path[(m + plen - 1) % plen];
+            const double* Jm = VJ + 36 * (int64_t)vm;
+            if (c < 36) {
+                // -1/2 [w]x J,  w = p_{m+1} - p_{m-1}
+                const double w0 = Vs[3 * vn] - Vs[3 * vp], w1 = Vs[3 * vn + 1] - Vs[3 * vp + 1],
+                             w2 = Vs[3 * vn + 2] - Vs[3 * vp + 2];
+                const double j0 = Jm[col], j1 = Jm[12 + col], j2 = Jm[24 + col];
+                const double cr = row == 0 ? (w1 * j2 - w2 * j1) : (row == 1 ? (w2 * j0 - w0 * j2) : (w0 * j1 - w1 * j0));
+                acc += -0.5 * cr;
+            } else {
+                const double* Jn = VJ + 36 * (int64_t)vn;
+                const V3 p0 = v3(Vs[3 * vm], Vs[3 * vm + 1], Vs[3 * vm + 2]);
+                const V3 p1 = v3(Vs[3 * vn], Vs[3 * vn + 1], Vs[3 * vn + 2]);
+                const V3 dx = sub(p1, p0);
+                const double sv = dot(p0, p1) + (1.0 / 3.0) * dot(dx, dx);
+                const V3 a0 = sub(p1, mul(dx, 2.0 / 3.0));   // ds/dp0
+                const V3 a1 = add(p0, mul(dx, 2.0 / 3.0));   // ds/dp1
+                const double dxr = row == 0 ? dx.x : (row == 1 ? dx.y : dx.z);
+                // (dg/dp0 J_m)[row][col] = 1/3 s J_m[row] - 1/3 dx_row (a0 . J_m[:,col])
+                const double a0J = a0.x * Jm[col] + a0.y * Jm[12 + col] + a0.z * Jm[24 + col];
+                const double a1J = a1.x * Jn[col] + a1.y * Jn[12 + col] + a1.z * Jn[24 + col];
+                acc += (1.0 / 3.0) * sv * Jm[12 * row + col] - (1.0 / 3.0) * dxr * a0J;
+                acc += -(1.0 / 3.0) * sv * Jn[12 * row + col] - (1.0 / 3.0) * dxr * a1J;
+            }
+        }
+        out[c] = acc;
+    }
+}
+
 // comp/V/vcell/E as written by walk B, tjw: the T-junction bit words of the
 // pair.  For every accepted loop, sink(loop_index, comp, raw_index, Sn, Gn,
-// nA, xc0, area, nverts, path, Vs) is called (by every lane) in the
-// reference's order.
+// nA, xc0, area, nverts, path, Vs, jac) is called (by every lane) in the
+// reference's order; jac (72 doubles, tp_loop_jacobian) when VJ (vertex
+// Jacobians of the pair, walk B) is given, else nullptr.
 template <class WP, class Sink>
 ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double area_eps, const V3& dirAB,
                                const int* comp, const double* V, const int* vcell, const int* E, const uint64_t* tjw,
-                               int* I, double* D, Sink& sink) {
+                               int* I, double* D, Sink& sink, const double* VJ = nullptr, double* jac = nullptr) {
     TopoOut out; out.comps = ncomp; out.loops_raw = 0; out.loops = 0; out.status = TOPO_OK;
     const int W = WP::W;
     const int ln = wp.lane();
@@ -468,8 +522,12 @@ ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double 
                 const V3 xc = dvd(cross(nA, Gn), area);
                 if (area < area_eps) continue;
                 if (dot(nA, dirAB) < 0.0) nA = mul(nA, -1.0);
+                if (VJ) {
+                    tp_loop_jacobian(wp, path, plen, Vs, VJ + 36 * (int64_t)cr[TPC_VB], jac);
+                    wp.sync();
+                }
                 out.status |= sink(out.loops, q, raw, Sn, Gn, nA, xc, area, plen, SArr<int>{path, 1},
-                                   SArr<double>{Vs, 1});
+                                   SArr<double>{Vs, 1}, VJ ? jac : nullptr);
                 out.loops += 1;
             }
         }
@@ -484,7 +542,7 @@ ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double 
 ZDX_HD inline int64_t tp_walk_b_ints(int S) { return tp_b_ints(S); }
 ZDX_HD inline int64_t tp_walk_b_dbls(int S) { return tp_b_dbls(S) + 6 * (int64_t)S; }
 ZDX_HD inline int64_t tp_walk_d_ints(int S) { return tp_d_ints(S) + 8 * (int64_t)S; }
-ZDX_HD inline int64_t tp_walk_d_dbls(int S) { return tp_d_dbls(S); }
+ZDX_HD inline int64_t tp_walk_d_dbls(int S) { return tp_d_dbls(S) + 72; }   // + one loop's sensitivities
 ZDX_HD inline int64_t tp_walk_ints(int S) {
     const int64_t b = tp_walk_b_ints(S), d = tp_walk_d_ints(S);
     return b > d ? b : d;
@@ -500,7 +558,7 @@ ZDX_HD inline int64_t tp_scratch_bytes(int64_t ni, int64_t nd) { return ((ni * 4
 // All phases of one pair in sequence (LaneHost), for host-side tests.
 template <class Sink>
 inline TopoOut tp_topology_host(const double* seg, int S, double h, double area_eps, bool split, const V3& dirAB,
-                                Sink& sink) {
+                                Sink& sink, const double* EJ = nullptr) {
     TopoOut out; out.comps = 0; out.loops_raw = 0; out.loops = 0; out.status = TOPO_OK;
     if (S <= 0) return out;
     LaneHost wp;
@@ -520,8 +578,10 @@ inline TopoOut tp_topology_host(const double* seg, int S, double h, double area_
     std::vector<int> IB((size_t)tp_b_ints(S)), comp(TPC_N * (size_t)S), vcell(3 * (size_t)n), E(2 * (size_t)S);
     std::vector<double> DB((size_t)tp_b_dbls(S)), V(3 * (size_t)n);
     int tjn = 0;
+    std::vector<double> VJ(EJ ? 36 * (size_t)n : 0), jac(72);
     const int ncomp = tp_components_and_vertices(wp, seg, S, h, split, adj.data(), snapw.data(), IB.data(), DB.data(),
-                                                 comp.data(), V.data(), vcell.data(), E.data(), &tjn);
+                                                 comp.data(), V.data(), vcell.data(), E.data(), &tjn, EJ,
+                                                 EJ ? VJ.data() : nullptr);
     std::vector<uint64_t> tjw((size_t)tjn + 1, 0ull);
     for (int q = 0; q < ncomp; ++q) {
         const int* cr = &comp[TPC_N * (size_t)q];
@@ -535,7 +595,7 @@ inline TopoOut tp_topology_host(const double* seg, int S, double h, double area_
     std::vector<int> ID((size_t)tp_d_ints(S));
     std::vector<double> DD((size_t)tp_d_dbls(S));
     return tp_loops(wp, S, ncomp, h, area_eps, dirAB, comp.data(), V.data(), vcell.data(), E.data(), tjw.data(),
-                    ID.data(), DD.data(), sink);
+                    ID.data(), DD.data(), sink, EJ ? VJ.data() : nullptr, jac.data());
 }
 
 }  // namespace zdx

@@ -41,7 +41,8 @@ class PairIn(C.Structure):
 
 
 class ContactOpts(C.Structure):
-    _fields_ = [("split_contacts", C.c_int32), ("mode", C.c_int32), ("k", C.c_double)]
+    _fields_ = [("split_contacts", C.c_int32), ("mode", C.c_int32), ("k", C.c_double), ("sensitivities", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class PairOut(C.Structure):
@@ -49,7 +50,8 @@ class PairOut(C.Structure):
                 ("n_loops_raw", C.c_void_p), ("loop_offset", C.c_void_p), ("seg_offset", C.c_void_p),
                 ("loop_rec", C.c_void_p), ("loop_cap", C.c_int64), ("vtx_offset", C.c_void_p),
                 ("loop_vtx", C.c_void_p), ("vtx_cap", C.c_int64), ("segments", C.c_void_p), ("seg_cap", C.c_int64),
-                ("n_loops_total", C.c_int64), ("n_vtx_total", C.c_int64), ("n_seg_total", C.c_int64)]
+                ("n_loops_total", C.c_int64), ("n_vtx_total", C.c_int64), ("n_seg_total", C.c_int64),
+                ("loop_jac", C.c_void_p)]
 
 
 class SimDesc(C.Structure):
@@ -154,7 +156,7 @@ class Context:
         return mid.value
 
     def contact_pairs(self, mesh_a, mesh_b, pos_a, quat_a, pos_b, quat_b, split=True, mode=MODE_BATCHED, k=1.0,
-                      segments=False, vertices=True):
+                      segments=False, vertices=True, sensitivities=False):
         """Batched per-pair contact evaluation.  Arrays are per pair; returns a
         dict with per-pair counters, per-loop records (ZDX_LOOP_REC layout) and,
         optionally, loop vertices and narrow-phase segments."""
@@ -166,7 +168,7 @@ class Context:
             arr[p].mesh_a = int(mesh_a[p]); arr[p].mesh_b = int(mesh_b[p])
             arr[p].pos_a[:] = pa[p].tolist(); arr[p].quat_a[:] = qa[p].tolist()
             arr[p].pos_b[:] = pb[p].tolist(); arr[p].quat_b[:] = qb[p].tolist()
-        opts = ContactOpts(int(split), int(mode), float(k))
+        opts = ContactOpts(int(split), int(mode), float(k), int(bool(sensitivities)), 0)
         loop_cap, vtx_cap, seg_cap = max(64, 4 * n), max(1024, 64 * n), max(1024, 64 * n)
         for _ in range(4):
             tc = np.zeros(n, dtype=np.int64)
@@ -179,8 +181,10 @@ class Context:
             vo = np.zeros(loop_cap + 1, dtype=np.int32) if vertices else None
             vtx = np.zeros((vtx_cap, 3)) if vertices else None
             seg = np.zeros((seg_cap, 6)) if segments else None
+            jac = np.zeros((loop_cap, 72)) if sensitivities else None
             out = PairOut(tc.ctypes.data, ns.ctypes.data, nc.ctypes.data, nr.ctypes.data, lo.ctypes.data, _ptr(so),
-                          rec.ctypes.data, loop_cap, _ptr(vo), _ptr(vtx), vtx_cap, _ptr(seg), seg_cap, 0, 0, 0)
+                          rec.ctypes.data, loop_cap, _ptr(vo), _ptr(vtx), vtx_cap, _ptr(seg), seg_cap, 0, 0, 0,
+                          _ptr(jac))
             st = self._L.zdx_contact_pairs(self.h, C.cast(arr, C.c_void_p), n, C.byref(opts), C.byref(out))
             if st == CAPACITY:
                 loop_cap = max(loop_cap, int(out.n_loops_total) + 1)
@@ -190,6 +194,8 @@ class Context:
             self._check(st)
             nl = int(out.n_loops_total)
             res = dict(tri_checks=tc, n_segments=ns, n_comps=nc, n_loops_raw=nr, loop_offset=lo, loops=rec[:nl])
+            if sensitivities:
+                res["loop_jac"] = jac[:nl]
             if vertices:
                 res["vtx_offset"] = vo[:nl + 1]
                 res["loop_vtx"] = vtx[:int(out.n_vtx_total)]

@@ -1,0 +1,220 @@
+"""Differentiable contact step of the minimal-energy model (polyline.py:655-701)
+on the device contact pipeline, for calibration.
+
+* ``PairContact`` (a ``torch.autograd.Function``): per contact loop of a batch
+  of pairs, the canonical area vector ``Sc = -|Sn| n`` (n: unit normal from A
+  to B) and the contact point ``xc0`` (src/main.cpp:1042-1052).  Forward is
+  ``zdx_contact_pairs``; backward is the hand-written adjoint: the device
+  returns, per loop, dSn/dxi and dGn/dxi for rigid motions xi of the two
+  bodies (each loop vertex is the mean of its snapped segment endpoints, each
+  endpoint an edge of one body crossing a triangle plane of the other), and
+  the chain to Sc, xc0 and to positions/quaternions is done here.  The
+  loop topology is held fixed, as a derivative must.
+* ``MinimalModel``: forces from loops -- elastic ``F_B = k/2 |Sn| n`` (the
+  energy identity f = -dPsi/dlambda n with Psi = k/2 Omega, dOmega/dlambda =
+  |Sn|) plus normal damping ``-c (v_rel . n) n`` with ``c = 2 zeta(e)
+  sqrt(m_eff k_c)``, ``zeta(e) = -ln e / sqrt(pi^2 + ln^2 e)`` and the contact
+  stiffness of a sphere of the equivalent radius R, ``k_c = pi k R`` -- and
+  the rigid-body update of ``integrate_particle`` (src/main.cpp:1494-1510,
+  old rotation for omega) in torch.
+* ``simulate``: K-step segments under ``torch.utils.checkpoint`` so memory is
+  O(N steps / K) states, the contact calls of a segment being re-run in
+  backward (they are deterministic).
+
+The static floor is a registered mesh with infinite mass (a fixed body B).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+from torch.utils.checkpoint import checkpoint
+
+
+def _skew(v):
+    z = np.zeros(v.shape[:-1])
+    return np.stack([np.stack([z, -v[..., 2], v[..., 1]], -1), np.stack([v[..., 2], z, -v[..., 0]], -1),
+                     np.stack([-v[..., 1], v[..., 0], z], -1)], -2)
+
+
+def _quat_grad(q, g_w):
+    """dL/dq for the world-frame rotation increment w (q' = exp(w/2) q, q
+    normalised inside the rotation): 2 ((0, g_w) x q_hat) / |q|."""
+    nq = np.linalg.norm(q, axis=-1, keepdims=True)
+    qh = q / nq
+    w, v = qh[..., :1], qh[..., 1:]
+    gw = -(g_w * v).sum(-1, keepdims=True)
+    gv = w * g_w + np.cross(g_w, v)
+    return 2.0 * np.concatenate([gw, gv], -1) / nq
+
+
+class PairContact(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, posA, quatA, posB, quatB, zctx, mesh_a, mesh_b, split):
+        r = zctx.contact_pairs(mesh_a, mesh_b, posA.detach().cpu().numpy(), quatA.detach().cpu().numpy(),
+                               posB.detach().cpu().numpy(), quatB.detach().cpu().numpy(), split=split, k=1.0,
+                               vertices=False, sensitivities=True)
+        rec, jac, lo = r["loops"], r["loop_jac"], r["loop_offset"]
+        nl = len(rec)
+        pair = np.repeat(np.arange(len(mesh_a)), np.diff(lo)).astype(np.int64)
+        Sn, Gn, n, xc, area = rec[:, 0:3], rec[:, 3:6], rec[:, 6:9], rec[:, 9:12], rec[:, 12]
+        s = np.sign((n * (-Sn)).sum(-1)) if nl else np.zeros(0)   # n = s (-Sn/|Sn|)
+        Sc = s[:, None] * Sn
+        dSn = jac[:, 0:36].reshape(nl, 3, 12)
+        dGn = jac[:, 36:72].reshape(nl, 3, 12)
+        # xc = -(Sn x Gn) / |Sn|^2
+        a2 = (area ** 2)[:, None, None]
+        c = np.cross(Sn, Gn)
+        dc = -_skew(Gn) @ dSn + _skew(Sn) @ dGn
+        dxc = -dc / a2 + 2.0 * c[:, :, None] * np.einsum("li,lij->lj", Sn, dSn)[:, None, :] / (a2 * a2)
+        ctx.save_for_backward(torch.from_numpy(s[:, None, None] * dSn), torch.from_numpy(dxc),
+                              torch.from_numpy(pair), quatA.detach().cpu(), quatB.detach().cpu())
+        ctx.n_pairs = len(mesh_a)
+        ctx.dev = posA.device
+        lp = torch.from_numpy(pair).to(posA.device)
+        ctx.mark_non_differentiable(lp)
+        return torch.from_numpy(Sc).to(posA), torch.from_numpy(xc.copy()).to(posA), lp
+
+    @staticmethod
+    def backward(ctx, gSc, gxc, _gpair):
+        dSc, dxc, pair, qA, qB = ctx.saved_tensors
+        npair = ctx.n_pairs
+        gxi = np.einsum("li,lij->lj", gSc.detach().cpu().numpy(), dSc.numpy())
+        gxi += np.einsum("li,lij->lj", gxc.detach().cpu().numpy(), dxc.numpy())
+        G = np.zeros((npair, 12))
+        np.add.at(G, pair.numpy(), gxi)
+        gpA = torch.from_numpy(G[:, 0:3]).to(ctx.dev)
+        gqA = torch.from_numpy(_quat_grad(qA.numpy(), G[:, 3:6])).to(ctx.dev)
+        gpB = torch.from_numpy(G[:, 6:9]).to(ctx.dev)
+        gqB = torch.from_numpy(_quat_grad(qB.numpy(), G[:, 9:12])).to(ctx.dev)
+        return gpA, gqA, gpB, gqB, None, None, None, None
+
+
+def qmul(a, b):
+    w1, x1, y1, z1 = a.unbind(-1)
+    w2, x2, y2, z2 = b.unbind(-1)
+    return torch.stack([w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                        w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2], -1)
+
+
+def qmat(q):
+    q = q / q.norm(dim=-1, keepdim=True)
+    w, x, y, z = q.unbind(-1)
+    return torch.stack([torch.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+                        torch.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+                        torch.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], -2)
+
+
+@dataclass
+class Scene:
+    """Particles (registered meshes) above a static floor mesh."""
+    zctx: object
+    mesh: np.ndarray            # per particle mesh id
+    radius: np.ndarray          # bounding radius per particle (candidate test)
+    mass: torch.Tensor          # (N,)
+    Ibinv: torch.Tensor         # (N, 3, 3) body-frame inverse inertia
+    req: torch.Tensor           # (N,) equivalent-sphere radius (cbrt(3V/4pi))
+    floor_mesh: int
+    floor_pos: np.ndarray
+    floor_top: float            # z of the floor's top face
+    gravity: torch.Tensor
+    dt: float
+    split: bool = True
+
+
+def _candidates(scene: Scene, pos: np.ndarray):
+    """Particle pairs whose bounding spheres overlap (the reference's broad
+    phase test) and particles whose sphere reaches the floor."""
+    n = len(pos)
+    d = pos[:, None, :] - pos[None, :, :]
+    rs = scene.radius[:, None] + scene.radius[None, :]
+    ii, jj = np.nonzero(np.triu((d * d).sum(-1) <= rs * rs, 1))
+    fl = np.nonzero(pos[:, 2] - scene.radius <= scene.floor_top)[0]
+    return ii, jj, fl
+
+
+def step(scene: Scene, k, e, pos, quat, vel, L):
+    """One step of the minimal model; all tensors float64, differentiable in
+    k, e and the state."""
+    n = pos.shape[0]
+    ii, jj, fl = _candidates(scene, pos.detach().cpu().numpy())
+    ia = np.concatenate([ii, fl]).astype(np.int64)
+    ib = np.concatenate([jj, np.full(len(fl), -1)]).astype(np.int64)
+    F = torch.zeros_like(pos)
+    T = torch.zeros_like(pos)
+    if len(ia):
+        ident = torch.tensor([1.0, 0.0, 0.0, 0.0], dtype=pos.dtype, device=pos.device)
+        fpos = torch.tensor(scene.floor_pos, dtype=pos.dtype, device=pos.device)
+        tia = torch.from_numpy(ia).to(pos.device)
+        tib = torch.from_numpy(np.where(ib >= 0, ib, 0)).to(pos.device)
+        isfloor = torch.from_numpy(ib < 0).to(pos.device)
+        pA, qA = pos[tia], quat[tia]
+        pB = torch.where(isfloor[:, None], fpos.expand(len(ia), 3), pos[tib])
+        qB = torch.where(isfloor[:, None], ident.expand(len(ia), 4), quat[tib])
+        mb = np.where(ib >= 0, scene.mesh[np.where(ib >= 0, ib, 0)], scene.floor_mesh)
+        Sc, xc, lp = PairContact.apply(pA, qA, pB, qB, scene.zctx, scene.mesh[ia], mb, scene.split)
+        if Sc.shape[0]:
+            lp = lp.long()
+            a = tia[lp]
+            b = tib[lp]
+            fb = isfloor[lp]
+            area = Sc.norm(dim=-1, keepdim=True)
+            nrm = -Sc / area
+            # velocities of the material points at xc
+            R_a = qmat(quat[a])
+            Iinv_a = R_a @ scene.Ibinv[a] @ R_a.transpose(-1, -2)
+            wa = (Iinv_a @ L[a].unsqueeze(-1)).squeeze(-1)
+            rA = xc - pos[a]
+            va = vel[a] + torch.cross(wa, rA, dim=-1)
+            R_b = qmat(quat[b])
+            Iinv_b = R_b @ scene.Ibinv[b] @ R_b.transpose(-1, -2)
+            wb = (Iinv_b @ L[b].unsqueeze(-1)).squeeze(-1)
+            rB = xc - torch.where(fb[:, None], fpos.expand_as(xc), pos[b])
+            vb = torch.where(fb[:, None], torch.zeros_like(xc), vel[b] + torch.cross(wb, rB, dim=-1))
+            vn = ((vb - va) * nrm).sum(-1, keepdim=True)
+            ma = scene.mass[a][:, None]
+            mbm = torch.where(fb[:, None], torch.full_like(ma, math.inf), scene.mass[b][:, None])
+            meff = torch.where(fb[:, None], ma, ma * mbm / (ma + torch.where(fb[:, None], ma, mbm)))
+            req = torch.where(fb, scene.req[a], 0.5 * (scene.req[a] + scene.req[b]))[:, None]
+            kc = math.pi * k * req
+            le = torch.log(e)
+            zeta = -le / torch.sqrt(math.pi ** 2 + le * le)
+            c = 2.0 * zeta * torch.sqrt(meff * kc)
+            fB = 0.5 * k * area * nrm - c * vn * nrm          # force on B
+            F = F.index_add(0, a, -fB)
+            T = T.index_add(0, a, torch.cross(rA, -fB, dim=-1))
+            keep = ~fb
+            F = F.index_add(0, b[keep], fB[keep])
+            T = T.index_add(0, b[keep], torch.cross(rB[keep], fB[keep], dim=-1))
+    # integrate_particle (old rotation for omega)
+    vel = vel + (F / scene.mass[:, None] + scene.gravity) * scene.dt
+    pos = pos + vel * scene.dt
+    R = qmat(quat)
+    Iinv = R @ scene.Ibinv @ R.transpose(-1, -2)
+    L = L + T * scene.dt
+    om = (Iinv @ L.unsqueeze(-1)).squeeze(-1)
+    dq = qmul(torch.cat([torch.zeros_like(om[:, :1]), om], -1), quat)
+    quat = quat + 0.5 * dq * scene.dt
+    quat = quat / quat.norm(dim=-1, keepdim=True)
+    return pos, quat, vel, L
+
+
+def simulate(scene: Scene, k, e, pos, quat, vel, L, steps: int, ckpt_every: int = 50):
+    """`steps` steps; every `ckpt_every` steps form one checkpointed segment."""
+    s = 0
+    while s < steps:
+        m = min(ckpt_every, steps - s)
+
+        def seg(k_, e_, p_, q_, v_, l_, m=m):
+            for _ in range(m):
+                p_, q_, v_, l_ = step(scene, k_, e_, p_, q_, v_, l_)
+            return p_, q_, v_, l_
+
+        if torch.is_grad_enabled():
+            pos, quat, vel, L = checkpoint(seg, k, e, pos, quat, vel, L, use_reentrant=False)
+        else:
+            pos, quat, vel, L = seg(k, e, pos, quat, vel, L)
+        s += m
+    return pos, quat, vel, L
