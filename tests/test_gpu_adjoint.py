@@ -166,7 +166,7 @@ def test_minimal_model_gradients_match_finite_differences(ctx):
                 vv = v0.clone()
                 vv[i, 2] *= (1 + s * rel)
                 return loss(k0, e0, vv)
-            checks.append((f"v0[{i}].z", v.grad[i, 2].item(), fv, abs(float(v0[i, 2])) * rel))
+            checks.append((f"v0[{i}].z", v.grad[i, 2].item(), fv, float(v0[i, 2]) * rel))
         for name, g, fn, hstep in checks:
             fd = (float(fn(1)) - float(fn(-1))) / (2 * hstep)
             assert abs(g - fd) <= 1e-4 * max(abs(fd), abs(g)) + 1e-12, (name, g, fd)

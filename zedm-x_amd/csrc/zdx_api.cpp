@@ -81,14 +81,17 @@ struct MeshRec {
 struct TopoWork {
     DBuf<int32_t> cell9, comp, vcell, E, ncomp;
     DBuf<double> V;
-    DBuf<uint8_t> adj;
+    DBuf<int32_t> adj;
     DBuf<uint64_t> snapw, tjw;
     DBuf<unsigned long long> top;
     DBuf<int64_t> adj_off, snap_off, tj_off;
     DBuf<int32_t> scr_i;
     DBuf<double> scr_d;
-    int64_t adj_cap = 1 << 22, snap_cap = 1 << 17, tj_cap = 1 << 17;
+    int64_t adj_cap = 1 << 20, snap_cap = 1 << 17, tj_cap = 1 << 17;
+    int32_t adj_row = 16;
     int64_t scr_ni = 0, scr_nd = 0;
+    int64_t slices = 0;
+    int32_t slice_seg_max = 0;
 
     void ensure_arena(int64_t segs) {
         cell9.ensure(9 * (size_t)segs); comp.ensure(TPC_N * (size_t)segs); V.ensure(6 * (size_t)segs);
@@ -97,25 +100,36 @@ struct TopoWork {
     void ensure_items(int64_t items) {
         adj_off.ensure(items); snap_off.ensure(items); tj_off.ensure(items); ncomp.ensure(items);
     }
-    void ensure_slices(int64_t slices, int32_t seg_max) {
-        scr_ni = tp_walk_ints(seg_max);
+    void ensure_slices(int64_t nslices, int32_t seg_max) {
+        slices = nslices;
+        slice_seg_max = seg_max;
+        scr_ni = tp_walk_ints(seg_max, adj_row);
         scr_nd = tp_walk_dbls(seg_max);
         scr_i.ensure((size_t)(scr_ni * slices));
         scr_d.ensure((size_t)(scr_nd * slices));
     }
     void ensure_bump() {
-        adj.ensure((size_t)adj_cap); snapw.ensure((size_t)snap_cap); tjw.ensure((size_t)tj_cap); top.ensure(3);
+        adj.ensure((size_t)adj_cap); snapw.ensure((size_t)snap_cap); tjw.ensure((size_t)tj_cap); top.ensure(4);
     }
     // After a step whose pass A or walk B ran out of room: the bump counters
     // hold what every pair asked for.  Returns true if a buffer was grown.
     bool grow_from_counters(hipStream_t st) {
-        unsigned long long t[3] = {0, 0, 0};
+        unsigned long long t[4] = {0, 0, 0, 0};
         HIPC(hipMemcpyAsync(t, top.p, sizeof(t), hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
         bool g = false;
         auto grow = [&](int64_t& cap, unsigned long long need) {
             if ((int64_t)need > cap) { cap = (int64_t)need + (int64_t)need / 4 + 1024; g = true; }
         };
+        if ((int64_t)t[3] > adj_row) {
+            // longer rows: the rows buffer and the walk slices scale with them
+            const int32_t r = (int32_t)t[3] + 4;
+            t[0] = t[0] / (unsigned long long)(adj_row + 1) * (unsigned long long)(r + 1);
+            adj_row = r;
+            scr_i.release(); scr_d.release();
+            ensure_slices(slices, slice_seg_max);
+            g = true;
+        }
         grow(adj_cap, t[0]);
         grow(snap_cap, t[1]);
         grow(tj_cap, t[2]);
@@ -127,7 +141,8 @@ struct TopoWork {
     }
     void fill(TopoBufs& T) {
         ensure_bump();
-        T.cell9 = cell9.p; T.adj = adj.p; T.adj_cap = adj_cap; T.snapw = snapw.p; T.snap_cap = snap_cap;
+        T.cell9 = cell9.p; T.adj = adj.p; T.adj_cap = adj_cap; T.adj_row = adj_row; T.snapw = snapw.p;
+        T.snap_cap = snap_cap;
         T.tjw = tjw.p; T.tj_cap = tj_cap; T.top = top.p; T.adj_off = adj_off.p; T.snap_off = snap_off.p;
         T.tj_off = tj_off.p; T.ncomp = ncomp.p; T.comp = comp.p; T.V = V.p; T.vcell = vcell.p; T.E = E.p;
         T.scr_i = scr_i.p; T.scr_d = scr_d.p; T.scr_ni = scr_ni; T.scr_nd = scr_nd;
@@ -281,6 +296,7 @@ struct zdx_sim {
     DBuf<int32_t> d_counts;
     // broad phase
     DBuf<DGrid> d_grid;
+    DBuf<double> d_bp_part;
     int32_t max_cells = 1 << 24;
     DBuf<int32_t> d_cell_of, d_cell_count, d_cell_start, d_cell_cursor, d_sorted, d_cnt_up, d_cnt_all;
     DBuf<int32_t> d_pair_off, d_adj_off, d_adj_j, d_adj_pair, d_n_adj, d_scan_tmp;
@@ -372,6 +388,7 @@ struct zdx_sim {
     BroadPhaseBufs bpbufs(int par) {
         BroadPhaseBufs b;
         b.n = N; b.state = d_state[par].p; b.radius = d_radius.p; b.grid = d_grid.p; b.max_cells = max_cells;
+        b.part = d_bp_part.p;
         b.cell_of = d_cell_of.p; b.cell_count = d_cell_count.p; b.cell_start = d_cell_start.p;
         b.cell_cursor = d_cell_cursor.p; b.sorted = d_sorted.p; b.cnt_up = d_cnt_up.p; b.cnt_all = d_cnt_all.p;
         b.pair_off = d_pair_off.p; b.adj_off = d_adj_off.p; b.pi = ps[par].pi.p; b.pj = ps[par].pj.p;
@@ -517,6 +534,7 @@ void sim_setup_device(zdx_sim* s) {
     s->d_F.ensure(3 * (size_t)N); s->d_T.ensure(3 * (size_t)N); s->d_counts.ensure(N);
     // broad phase
     s->d_grid.ensure(1);
+    s->d_bp_part.ensure(7 * (size_t)BP_PARTS);
     int64_t cells = std::min<int64_t>((int64_t)1 << 24, std::max<int64_t>(64, 8 * (int64_t)N));
     s->max_cells = (int32_t)cells;
     s->d_cell_of.ensure(N); s->d_cell_count.ensure(cells); s->d_cell_start.ensure(cells + 1);

@@ -13,10 +13,13 @@ struct DGrid {
     int32_t ncells;
 };
 
+constexpr int BP_PARTS = 256;   // workgroups of the bounds reduction
+
 struct BroadPhaseBufs {
     int32_t n;                 // particles
     const double* state;       // 16 per particle
     const double* radius;
+    double* part;              // BP_PARTS x 7 partial bounds (min xyz, max xyz, max radius)
     DGrid* grid;               // device
     int32_t max_cells;
     int32_t* cell_of;          // per particle
@@ -78,13 +81,14 @@ struct TopoBufs {
     int32_t* tstatus;          // per pair
     // phased topology (zdx_topo.h); "item" = index into the pair work list
     int32_t* cell9;            // 9 per arena segment
-    uint8_t* adj;              // adjacency keys, S*S per pair (bump-allocated)
+    int32_t* adj;              // adjacency rows: per pair S counts + S x adj_row entries (bump-allocated)
     int64_t adj_cap;
+    int32_t adj_row;           // entries per adjacency row
     uint64_t* snapw;           // snap hit words (bump-allocated)
     int64_t snap_cap;
     uint64_t* tjw;             // T-junction hit words (bump-allocated)
     int64_t tj_cap;
-    unsigned long long* top;   // 3 bump counters: adjacency bytes, snap words, T-junction words
+    unsigned long long* top;   // bump counters: adjacency ints, snap words, T-junction words; longest row
     int64_t* adj_off;          // per item (-1: did not fit)
     int64_t* snap_off;
     int64_t* tj_off;

@@ -259,35 +259,45 @@ void scan_i32(const int32_t* in, const int32_t* n_ptr, int32_t n_max, int32_t* t
 // ------------------------------------------------------------ broad phase
 constexpr int NEIGH_CAP = 128;
 
-__global__ void __launch_bounds__(1024) k_bp_bounds(const double* state, const double* radius, int n, DGrid* g,
-                                                    int max_cells, int32_t* status, unsigned long long* kt) {
+// Bounds of the particle positions and the largest radius: BP_PARTS
+// workgroups reduce a strided share each, then one workgroup combines them.
+__global__ void __launch_bounds__(256) k_bp_bounds_part(const double* state, const double* radius, int n,
+                                                        double* part, unsigned long long* kt) {
     kt_start(kt);
-    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY}, rm = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double* p = state + 16 * (int64_t)i;
-        for (int k = 0; k < 3; ++k) { mn[k] = fmin(mn[k], p[k]); mx[k] = fmax(mx[k], p[k]); }
-        rm = fmax(rm, radius[i]);
+        for (int k = 0; k < 3; ++k) { v[k] = fmin(v[k], p[k]); v[3 + k] = fmax(v[3 + k], p[k]); }
+        v[6] = fmax(v[6], radius[i]);
     }
-    __shared__ double s[7][16];
-    for (int d = 32; d >= 1; d >>= 1) {
-        for (int k = 0; k < 3; ++k) {
-            mn[k] = fmin(mn[k], __shfl_xor(mn[k], d, WAVE));
-            mx[k] = fmax(mx[k], __shfl_xor(mx[k], d, WAVE));
+    for (int d = 32; d >= 1; d >>= 1)
+        for (int k = 0; k < 7; ++k) {
+            const double o = __shfl_xor(v[k], d, WAVE);
+            v[k] = k < 3 ? fmin(v[k], o) : fmax(v[k], o);
         }
-        rm = fmax(rm, __shfl_xor(rm, d, WAVE));
+    __shared__ double s[7][4];
+    if (lane_id() == 0)
+        for (int k = 0; k < 7; ++k) s[k][threadIdx.x / WAVE] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        const int k = threadIdx.x;
+        double r = s[k][0];
+        for (int w = 1; w < (int)(blockDim.x / WAVE); ++w) r = k < 3 ? fmin(r, s[k][w]) : fmax(r, s[k][w]);
+        part[7 * blockIdx.x + k] = r;
     }
-    if (lane_id() == 0) {
-        int w = threadIdx.x / WAVE;
-        for (int k = 0; k < 3; ++k) { s[k][w] = mn[k]; s[3 + k][w] = mx[k]; }
-        s[6][w] = rm;
+}
+
+__global__ void __launch_bounds__(64) k_bp_bounds(const double* part, int nparts, DGrid* g, int max_cells,
+                                                  int32_t* status) {
+    __shared__ double s[7][1];
+    if (threadIdx.x < 7) {
+        const int k = threadIdx.x;
+        double r = part[k];
+        for (int b = 1; b < nparts; ++b) r = k < 3 ? fmin(r, part[7 * b + k]) : fmax(r, part[7 * b + k]);
+        s[k][0] = r;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    int nw = blockDim.x / WAVE;
-    for (int w = 1; w < nw; ++w) {
-        for (int k = 0; k < 3; ++k) { s[k][0] = fmin(s[k][0], s[k][w]); s[3 + k][0] = fmax(s[3 + k][0], s[3 + k][w]); }
-        s[6][0] = fmax(s[6][0], s[6][w]);
-    }
     DGrid G;
     G.nx = G.ny = G.nz = 1;
     G.ox = s[0][0]; G.oy = s[1][0]; G.oz = s[2][0];
@@ -770,15 +780,17 @@ struct LaneDev {
 // (persistent); walk kernels: one wave per pair, scratch in LDS when it fits,
 // else the workgroup's slice of the global scratch.
 
-// Pass A: representative-point cells, the S x S adjacency keys and the snap
-// hit words of every pair; the pair's adjacency bytes and snap words are
-// bump-allocated (a pair that does not fit gets offset -1 and the step is
-// retried with larger buffers: the counters keep the total needed).
+// Pass A: representative-point cells, the adjacency rows (per segment the
+// count and up to R accepted candidates, one thread per row) and the snap hit
+// words of every pair; each pair's rows and snap words are bump-allocated (a
+// pair that does not fit gets offset -1 and the step is retried with larger
+// buffers: the counters keep the total needed, top[3] the longest row).
 __global__ void __launch_bounds__(256) k_tp_pass_a(DPairs P, DMeshes M, DBodies B, TopoBufs T,
                                                   unsigned long long* kt) {
     __shared__ long long s_off[2];
     kt_start(kt);
     const int n = *P.topo_n;
+    const int R = T.adj_row;
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
         const int p = P.topo_list[it];
         const int S = P.nseg[p];
@@ -786,7 +798,7 @@ __global__ void __launch_bounds__(256) k_tp_pass_a(DPairs P, DMeshes M, DBodies 
         const double h = M.tol[B.mesh[P.pi[p]]];
         const double* seg = P.arena + 6 * sb;
         int* c9 = T.cell9 + 9 * sb;
-        const int nadj = S * S;
+        const int nadj = S * (R + 1);
         const int npt = S * (2 * S - 1);   // endpoint pairs of 2S points
         const int nw = (npt + 63) >> 6;
         if (threadIdx.x == 0) {
@@ -802,10 +814,12 @@ __global__ void __launch_bounds__(256) k_tp_pass_a(DPairs P, DMeshes M, DBodies 
         __syncthreads();
         const long long ao = s_off[0], so = s_off[1];
         if (ao >= 0) {
-            uint8_t* adj = T.adj + ao;
-            for (int k = threadIdx.x; k < nadj; k += blockDim.x) {
-                const int i = k / S, j = k - (k / S) * S;
-                adj[k] = (uint8_t)tp_adj_key(seg, c9, h, i, j);
+            int* cntp = T.adj + ao;
+            int* rows = cntp + S;
+            for (int i = threadIdx.x; i < S; i += blockDim.x) {
+                const int m = tp_adj_row(seg, c9, h, S, i, rows + (int64_t)i * R, R);
+                cntp[i] = m;
+                if (m > R) atomicMax(T.top + 3, (unsigned long long)m);
             }
             uint64_t* w = T.snapw + so;
             // each wave covers 64 consecutive pair numbers: one word per ballot
@@ -868,11 +882,12 @@ __device__ inline auto with_walk_scratch(const TopoBufs& T, int64_t ni, int64_t 
 }
 
 // Walk B: components, snapped vertices and edges of each pair; allocates the
-// pair's T-junction words.
+// pair's T-junction words.  The adjacency rows are copied into the scratch.
 __global__ void __launch_bounds__(64) k_tp_walk_b(DPairs P, DMeshes M, DBodies B, TopoBufs T, int32_t split,
                                                   int32_t lds_bytes) {
     extern __shared__ char s_walk[];
     const int n = *P.topo_n;
+    const int R = T.adj_row;
     LaneDev wp;
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
         const int p = P.topo_list[it];
@@ -883,12 +898,16 @@ __global__ void __launch_bounds__(64) k_tp_walk_b(DPairs P, DMeshes M, DBodies B
         int ncomp = -1, tjn = 0;
         if (ao >= 0 && S <= T.seg_max) {
             const double* seg = P.arena + 6 * sb;
-            ncomp = with_walk_scratch(T, tp_walk_b_ints(S), tp_walk_b_dbls(S), s_walk, lds_bytes,
-                                      [&](int* I, double* D) {
-                double* sgl = D + tp_b_dbls(S);   // local copy of the segments
-                for (int k = lane_id(); k < 6 * S; k += WAVE) sgl[k] = seg[k];
+            ncomp = with_walk_scratch(T, tp_walk_b_ints(S, R), tp_walk_b_dbls(S), s_walk, lds_bytes,
+                                      [&](int* I, double*) {
+                int* adj = I + tp_b_ints(S);   // counts (S) then rows (S x R)
+                const int* src = T.adj + ao;
+                for (int k = lane_id(); k < S * (R + 1); k += WAVE) adj[k] = src[k];
                 __syncthreads();
-                return tp_components_and_vertices(wp, sgl, S, h, split != 0, T.adj + ao, T.snapw + so, I, D,
+                int ok = 1;
+                for (int k = lane_id(); k < S; k += WAVE) ok &= adj[k] <= R;
+                if (!__all(ok)) return -1;
+                return tp_components_and_vertices(wp, seg, S, h, split != 0, adj, adj + S, R, T.snapw + so, I,
                                                   T.comp + TPC_N * sb, T.V + 6 * sb, T.vcell + 6 * sb,
                                                   T.E + 2 * sb, &tjn, T.ej ? T.ej + 72 * sb : nullptr,
                                                   T.vj ? T.vj + 72 * sb : nullptr);
@@ -1253,25 +1272,37 @@ __global__ void __launch_bounds__(256) k_counters(DPairs P, TopoBufs T, int32_t 
         c_scan += P.tri_scanned[p];
     }
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n_pw; k += gridDim.x * blockDim.x) c_wall += wnc[k];
-    unsigned long long v[10] = {c_loops, c_checks, c_seg, c_comps, c_raw, c_tests, c_hits, c_wall, c_scan, c_tp};
+    unsigned long long v[11] = {c_loops, c_checks, c_seg, c_comps, c_raw, c_tests, c_hits, c_wall, c_scan, c_tp, c_smax};
     for (int k = 0; k < 10; ++k)
         for (int d = 32; d >= 1; d >>= 1) v[k] += __shfl_xor(v[k], d, WAVE);
     for (int d = 32; d >= 1; d >>= 1) {
-        const unsigned long long o = __shfl_xor(c_smax, d, WAVE);
-        c_smax = o > c_smax ? o : c_smax;
+        const unsigned long long o = __shfl_xor(v[10], d, WAVE);
+        v[10] = o > v[10] ? o : v[10];
     }
-    if (lane_id() == 0) {
-        atomicAdd(&ctr[0], v[0] + v[7]);   // contacts = accepted loops + wall contacts
-        atomicAdd(&ctr[1], v[1]);
-        atomicAdd(&ctr[2], v[2]);
-        atomicAdd(&ctr[3], v[3]);
-        atomicAdd(&ctr[4], v[4]);
-        atomicAdd(&ctr[6], v[7]);
-        atomicAdd(&ctr[7], v[5]);
-        atomicAdd(&ctr[8], v[6]);
-        atomicAdd(&ctr[9], v[8]);
-        atomicAdd(&ctr[10], v[9]);
-        atomicMax(&ctr[11], c_smax);
+    // combine the waves of the workgroup, then one atomic per value
+    __shared__ unsigned long long s[11][16];
+    __shared__ unsigned long long tot[11];
+    const int w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    if (lane_id() == 0)
+        for (int k = 0; k < 11; ++k) s[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 11) {
+        const int k = threadIdx.x;
+        unsigned long long r = s[k][0];
+        for (int q = 1; q < nw; ++q) r = k == 10 ? (s[k][q] > r ? s[k][q] : r) : r + s[k][q];
+        tot[k] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 11) {
+        // v order: loops, checks, segments, comps, raw loops, tests, hits, wall contacts, scanned, pairs with
+        // segments, largest segment count.  ctr: 0 contacts (accepted loops + wall contacts), 1 tri_checks,
+        // 2 segments, 3 comps, 4 loops, 5 pairs, 6 wall contacts, 7 tri_tests, 8 tri_hits, 9 tri_scanned,
+        // 10 pairs with segments, 11 largest segment count
+        const int k = threadIdx.x;
+        const int dst[11] = {0, 1, 2, 3, 4, 7, 8, 6, 9, 10, 11};
+        if (k == 0) atomicAdd(&ctr[0], tot[0] + tot[7]);
+        else if (k == 10) atomicMax(&ctr[11], tot[10]);
+        else atomicAdd(&ctr[dst[k]], tot[k]);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctr[5], (unsigned long long)n);
 }
@@ -1312,8 +1343,9 @@ void launch_need_list(int32_t n, const double* state, const double* radius, cons
 
 void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s, unsigned long long* kt) {
     const int n = bp.n;
-    hipLaunchKernelGGL(k_bp_bounds, dim3(1), dim3(1024), 0, s, bp.state, bp.radius, n, bp.grid, bp.max_cells,
-                       bp.status, kt);
+    hipLaunchKernelGGL(k_bp_bounds_part, dim3(BP_PARTS), dim3(256), 0, s, bp.state, bp.radius, n, bp.part, kt);
+    hipLaunchKernelGGL(k_bp_bounds, dim3(1), dim3(64), 0, s, bp.part, (int)BP_PARTS, bp.grid, bp.max_cells,
+                       bp.status);
     // cell counters for the grid actually in use
     int32_t* ncells = bp.n_adj;  // reuse as a scalar before it is written below
     hipLaunchKernelGGL(k_grid_cells, dim3(1), dim3(1), 0, s, bp.grid, ncells);
@@ -1378,7 +1410,7 @@ void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, i
 
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
                      const TopoBufs& T, const SimParams& sp, hipStream_t s, unsigned long long* kt) {
-    launch_fill_u64(T.top, 0ull, 3, s);
+    launch_fill_u64(T.top, 0ull, 4, s);
     hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T, kt);
     hipLaunchKernelGGL(k_tp_walk_b, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, T, sp.split,
                        T.lds_bytes);
@@ -1388,7 +1420,7 @@ void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const 
 }
 void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int32_t split,
                      hipStream_t s) {
-    launch_fill_u64(T.top, 0ull, 3, s);
+    launch_fill_u64(T.top, 0ull, 4, s);
     if (T.ej) hipLaunchKernelGGL(k_tp_endpoint_jac, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T);
     hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T, nullptr);
     hipLaunchKernelGGL(k_tp_walk_b, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, T, split,
@@ -1419,7 +1451,7 @@ void launch_fold_integrate(int32_t n, const double* sin, double* sout, const DMa
 void launch_counters(const DPairs& P, const TopoBufs& T, int32_t n_pw, const int32_t* wnc, unsigned long long* ctr,
                      hipStream_t s) {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctr, 0ull, (int64_t)16);
-    hipLaunchKernelGGL(k_counters, dim3(256), dim3(256), 0, s, P, T, n_pw, wnc, ctr);
+    hipLaunchKernelGGL(k_counters, dim3(32), dim3(256), 0, s, P, T, n_pw, wnc, ctr);
 }
 void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s) {
     if (n <= 0) return;

@@ -76,6 +76,20 @@ ZDX_HD inline int tp_adj_key(const double* seg, const int* cell9, double h, int 
     return seg_seg(p1, q1, p2, q2) <= h * h ? key : TP_NOADJ;
 }
 
+// Row i of the adjacency as a list: the accepted candidates j of popped
+// segment i in ascending j, each as j | key << 24; writes at most R entries
+// and returns the true count (a count above R means the list did not fit).
+ZDX_HD inline int tp_adj_row(const double* seg, const int* cell9, double h, int S, int i, int* out, int R) {
+    int m = 0;
+    for (int j = 0; j < S; ++j) {
+        const int key = tp_adj_key(seg, cell9, h, i, j);
+        if (key == TP_NOADJ) continue;
+        if (m < R) out[m] = j | (key << 24);
+        ++m;
+    }
+    return m;
+}
+
 // Endpoint pairs are numbered t = a(a-1)/2 + b, b < a; point k is end k&1 of
 // segment k>>1.  The snap test of snap_endpoints (main.cpp:697-723): cells
 // within +-1 and squared distance <= tol^2.
@@ -101,22 +115,23 @@ ZDX_HD inline bool tp_snap_hit(const double* seg, const int* cell9, double h, in
 // edge (both within the pair), first T-junction bit word (within the pair).
 enum { TPC_NV = 0, TPC_NE = 1, TPC_VB = 2, TPC_EB = 3, TPC_TW = 4, TPC_N = 5 };
 
-// Scratch of walk B for a pair of S segments: ints, doubles.
-ZDX_HD inline int64_t tp_b_ints(int S) { return 18 * (int64_t)S + 4; }
-ZDX_HD inline int64_t tp_b_dbls(int S) { return 6 * (int64_t)S; }
+// Scratch of walk B for a pair of S segments and adjacency rows of R entries
+// (the rows themselves live in the caller's buffer): ints.
+ZDX_HD inline int64_t tp_b_ints(int S) { return 24 * (int64_t)S + 4; }
 // T-junction words of a component: one row of ceil(nv/64) words per edge.
 ZDX_HD inline int tp_row_words(int nv) { return (nv + 63) >> 6; }
 
-// adj: S x S adjacency keys (row = popped segment); snapw: bit t of word t/64
-// is the snap hit of endpoint pair t.  Outputs: comp (ncomp x TPC_N), V (3 per
-// vertex), vcell (3 per vertex), E (2 per edge, vertex ids local to the
+// adjn/adjl: per segment the count and the list (R entries, j | key << 24,
+// ascending j) of its accepted component candidates; snapw: bit t of word
+// t/64 is the snap hit of endpoint pair t.  Outputs: comp (ncomp x TPC_N), V
+// (3 per vertex), vcell (3 per vertex), E (2 per edge, vertex ids local to the
 // component).  Returns ncomp; *tj_words = total T-junction words.
 // Optional sensitivities: EJ (36 per endpoint, point order 2s+e) gives VJ (36
 // per vertex, same numbering as V): the mean of its endpoints' Jacobians,
 // summed in point order.
 template <class WP>
 ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, int S, double h, bool split,
-                                             const uint8_t* adj, const uint64_t* snapw, int* I, double* D,
+                                             const int* adjn, const int* adjl, int R, const uint64_t* snapw, int* I,
                                              int* comp, double* V, int* vcell, int* E, int* tj_words,
                                              const double* EJ = nullptr, double* VJ = nullptr) {
     const int W = WP::W;
@@ -135,7 +150,9 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
     int* rootvid = rnk + n;  // 2S
     int* ptv = rootvid + n;  // 2S
     int* cnt = ptv + n;      // 2S
-    double* Vs = D;          // 3 x 2S
+    int* voff = cnt + n;     // 2S+1  members of vertex v: mem[voff[v] .. voff[v+1])
+    int* mem = voff + n + 1; // 2S    points of the component, grouped by vertex, in point order
+    int* vfill = mem + n;    // 2S
     // ---- components (DFS in the reference's candidate order)
     int ncomp = 0;
     if (split) {
@@ -152,26 +169,22 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
             while (sp > 0) {
                 const int i = stk[--sp];
                 const int first = sp;
-                const uint8_t* row = adj + (int64_t)i * S;
-                for (int base = 0; base < S; base += W) {
-                    const int j = base + ln;
-                    const int key = j < S ? (int)row[j] : TP_NOADJ;
-                    uint64_t m = wp.ballot(key != TP_NOADJ && !vis[j < S ? j : 0]);
-                    while (m) {
-                        const int jj = base + ctz64(m);
-                        m &= m - 1;
-                        const int kj = (int)row[jj];
-                        // accepted candidates of i are pushed ordered by (scan key, id)
-                        int pos = sp;
-                        while (pos > first && spk[pos - 1] > kj) {
-                            stk[pos] = stk[pos - 1];
-                            spk[pos] = spk[pos - 1];
-                            --pos;
-                        }
-                        stk[pos] = jj;
-                        spk[pos] = kj;
-                        ++sp;
+                const int* row = adjl + (int64_t)i * R;
+                const int m = adjn[i];
+                for (int q = 0; q < m; ++q) {
+                    const int e = row[q];
+                    const int jj = e & 0xFFFFFF, kj = e >> 24;
+                    if (vis[jj]) continue;
+                    // accepted candidates of i are pushed ordered by (scan key, id)
+                    int pos = sp;
+                    while (pos > first && spk[pos - 1] > kj) {
+                        stk[pos] = stk[pos - 1];
+                        spk[pos] = spk[pos - 1];
+                        --pos;
                     }
+                    stk[pos] = jj;
+                    spk[pos] = kj;
+                    ++sp;
                 }
                 for (int k = first; k < sp; ++k) {
                     vis[stk[k]] = 1;
@@ -226,46 +239,52 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
             }
         }
     }
-    // ---- per component: first-occurrence vertex ids, sequential means, edges
+    // ---- per component: first-occurrence vertex ids, then each vertex's
+    // mean over its points summed in point order (one lane per vertex), edges
     int vb = 0, eb = 0, tw = 0;
     for (int q = 0; q < ncomp; ++q) {
         const int c = cidx[q];
         const int base = cbeg[c], ns = csz[c];
         int nv = 0;
         for (int k = 0; k < 2 * ns; ++k) {
-            const int s = cord[base + (k >> 1)];
-            const int pt = 2 * s + (k & 1);
+            const int pt = 2 * cord[base + (k >> 1)] + (k & 1);
             const int r = find(pt);
-            const double* p = seg + 6 * s + 3 * (k & 1);
-            if (rootvid[r] < 0) {
-                rootvid[r] = nv;
-                Vs[3 * nv] = p[0]; Vs[3 * nv + 1] = p[1]; Vs[3 * nv + 2] = p[2];
-                cnt[nv] = 1;
-                if (VJ)
-                    for (int c = ln; c < 36; c += W) VJ[36 * (int64_t)(vb + nv) + c] = EJ[36 * (int64_t)pt + c];
-                ++nv;
-            } else {
-                const int v = rootvid[r];
-                Vs[3 * v] += p[0]; Vs[3 * v + 1] += p[1]; Vs[3 * v + 2] += p[2];
-                cnt[v] += 1;
-                if (VJ)
-                    for (int c = ln; c < 36; c += W) VJ[36 * (int64_t)(vb + v) + c] += EJ[36 * (int64_t)pt + c];
-            }
-            ptv[pt] = rootvid[r];
+            if (rootvid[r] < 0) { rootvid[r] = nv; cnt[nv] = 0; ++nv; }
+            const int v = rootvid[r];
+            cnt[v] += 1;
+            ptv[pt] = v;
+        }
+        int acc = 0;
+        for (int v = 0; v < nv; ++v) { voff[v] = acc; vfill[v] = acc; acc += cnt[v]; }
+        voff[nv] = acc;
+        for (int k = 0; k < 2 * ns; ++k) {
+            const int pt = 2 * cord[base + (k >> 1)] + (k & 1);
+            mem[vfill[ptv[pt]]++] = pt;
         }
         wp.sync();
-        if (VJ)
-            for (int v = 0; v < nv; ++v)
-                for (int c = ln; c < 36; c += W) VJ[36 * (int64_t)(vb + v) + c] /= (double)cnt[v];
         for (int v = ln; v < nv; v += W) {
-            const double cn = (double)cnt[v];
-            const V3 m = v3(Vs[3 * v] / cn, Vs[3 * v + 1] / cn, Vs[3 * v + 2] / cn);
+            const int m0 = voff[v], m1 = voff[v + 1];
+            const double* p = seg + 3 * mem[m0];     // point pt at seg + 6 (pt >> 1) + 3 (pt & 1) = seg + 3 pt
+            double sx = p[0], sy = p[1], sz = p[2];
+            for (int m = m0 + 1; m < m1; ++m) {
+                const double* pm = seg + 3 * mem[m];
+                sx += pm[0]; sy += pm[1]; sz += pm[2];
+            }
+            const double cn = (double)(m1 - m0);
+            const V3 mv = v3(sx / cn, sy / cn, sz / cn);
             double* o = V + 3 * (int64_t)(vb + v);
-            o[0] = m.x; o[1] = m.y; o[2] = m.z;
-            const I3 cc = cell_of(m, h);
+            o[0] = mv.x; o[1] = mv.y; o[2] = mv.z;
+            const I3 cc = cell_of(mv, h);
             int* oc = vcell + 3 * (int64_t)(vb + v);
             oc[0] = cc.x; oc[1] = cc.y; oc[2] = cc.z;
         }
+        if (VJ)
+            for (int v = 0; v < nv; ++v)
+                for (int cj = ln; cj < 36; cj += W) {
+                    double a = EJ[36 * (int64_t)mem[voff[v]] + cj];
+                    for (int m = voff[v] + 1; m < voff[v + 1]; ++m) a += EJ[36 * (int64_t)mem[m] + cj];
+                    VJ[36 * (int64_t)(vb + v) + cj] = a / (double)(voff[v + 1] - voff[v]);
+                }
         int ne = 0;
         for (int k = 0; k < ns; ++k) {
             const int s = cord[base + k];
@@ -539,12 +558,12 @@ ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double 
 
 // Scratch of the device walks, including their copies of the pair's inputs
 // (walk B: the segments, 6S doubles; walk D: edges 2S and vertex cells 6S ints).
-ZDX_HD inline int64_t tp_walk_b_ints(int S) { return tp_b_ints(S); }
-ZDX_HD inline int64_t tp_walk_b_dbls(int S) { return tp_b_dbls(S) + 6 * (int64_t)S; }
+ZDX_HD inline int64_t tp_walk_b_ints(int S, int R) { return tp_b_ints(S) + (int64_t)S * (R + 1); }   // + the rows
+ZDX_HD inline int64_t tp_walk_b_dbls(int) { return 0; }
 ZDX_HD inline int64_t tp_walk_d_ints(int S) { return tp_d_ints(S) + 8 * (int64_t)S; }
 ZDX_HD inline int64_t tp_walk_d_dbls(int S) { return tp_d_dbls(S) + 72; }   // + one loop's sensitivities
-ZDX_HD inline int64_t tp_walk_ints(int S) {
-    const int64_t b = tp_walk_b_ints(S), d = tp_walk_d_ints(S);
+ZDX_HD inline int64_t tp_walk_ints(int S, int R) {
+    const int64_t b = tp_walk_b_ints(S, R), d = tp_walk_d_ints(S);
     return b > d ? b : d;
 }
 ZDX_HD inline int64_t tp_walk_dbls(int S) {
@@ -564,9 +583,8 @@ inline TopoOut tp_topology_host(const double* seg, int S, double h, double area_
     LaneHost wp;
     std::vector<int> cell9(9 * (size_t)S);
     for (int s = 0; s < S; ++s) tp_cells(seg, s, h, &cell9[9 * (size_t)s]);
-    std::vector<uint8_t> adj((size_t)S * S);
-    for (int i = 0; i < S; ++i)
-        for (int j = 0; j < S; ++j) adj[(size_t)i * S + j] = (uint8_t)tp_adj_key(seg, cell9.data(), h, i, j);
+    std::vector<int> adjn((size_t)S), adjl((size_t)S * S);
+    for (int i = 0; i < S; ++i) adjn[i] = tp_adj_row(seg, cell9.data(), h, S, i, &adjl[(size_t)i * S], S);
     const int n = 2 * S;
     const int64_t npairs = (int64_t)n * (n - 1) / 2;
     std::vector<uint64_t> snapw((size_t)((npairs + 63) >> 6), 0ull);
@@ -576,11 +594,11 @@ inline TopoOut tp_topology_host(const double* seg, int S, double h, double area_
         if (tp_snap_hit(seg, cell9.data(), h, a, b)) snapw[(size_t)(t >> 6)] |= 1ull << (t & 63);
     }
     std::vector<int> IB((size_t)tp_b_ints(S)), comp(TPC_N * (size_t)S), vcell(3 * (size_t)n), E(2 * (size_t)S);
-    std::vector<double> DB((size_t)tp_b_dbls(S)), V(3 * (size_t)n);
+    std::vector<double> V(3 * (size_t)n);
     int tjn = 0;
     std::vector<double> VJ(EJ ? 36 * (size_t)n : 0), jac(72);
-    const int ncomp = tp_components_and_vertices(wp, seg, S, h, split, adj.data(), snapw.data(), IB.data(), DB.data(),
-                                                 comp.data(), V.data(), vcell.data(), E.data(), &tjn, EJ,
+    const int ncomp = tp_components_and_vertices(wp, seg, S, h, split, adjn.data(), adjl.data(), S, snapw.data(),
+                                                 IB.data(), comp.data(), V.data(), vcell.data(), E.data(), &tjn, EJ,
                                                  EJ ? VJ.data() : nullptr);
     std::vector<uint64_t> tjw((size_t)tjn + 1, 0ull);
     for (int q = 0; q < ncomp; ++q) {
