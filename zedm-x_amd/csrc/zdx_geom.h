@@ -458,7 +458,7 @@ struct LaneHost {
     double shfl(double v, int) const { return v; }
     uint64_t shfl64(uint64_t v, int) const { return v; }
     void sync() const {}
-    void mark(int) const {}
+    void mark(int) const {}   // phase boundary (timed by profiling builds of the device policy)
 };
 
 // index of the lowest set bit (m != 0)

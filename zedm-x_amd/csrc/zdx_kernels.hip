@@ -761,6 +761,12 @@ __device__ inline Mat load_mat(const DMat& m, int i) {
     return r;
 }
 
+#ifdef ZDX_WALK_PROFILE
+// Profiling build: device clock ticks per phase of the walks, summed over all
+// pairs (slot 0: from the start of a pair to its first mark), read back with
+// zdx_walk_profile().
+__device__ unsigned long long g_walk_prof[16];
+#endif
 // Lane policy of the walks: one wave, all lanes on the same serial path.
 struct LaneDev {
     static constexpr int W = WAVE;
@@ -773,6 +779,18 @@ struct LaneDev {
         return ((uint64_t)hi << 32) | lo;
     }
     __device__ void sync() const { __syncthreads(); }
+#ifdef ZDX_WALK_PROFILE
+    mutable unsigned long long t0 = 0;
+    __device__ void start() const { t0 = wall_clock64(); }
+    __device__ void mark(int k) const {
+        const unsigned long long t = wall_clock64();
+        if (lane_id() == 0) atomicAdd(&g_walk_prof[k], t - t0);
+        t0 = t;
+    }
+#else
+    __device__ void start() const {}
+    __device__ void mark(int) const {}
+#endif
 };
 
 // ---- phased topology (zdx_topo.h): pass A, walk B, pass C, walk D.
@@ -896,6 +914,7 @@ __global__ void __launch_bounds__(64) k_tp_walk_b(DPairs P, DMeshes M, DBodies B
         const double h = M.tol[B.mesh[P.pi[p]]];
         const long long ao = T.adj_off[it], so = T.snap_off[it];
         int ncomp = -1, tjn = 0;
+        wp.start();
         if (ao >= 0 && S <= T.seg_max) {
             const double* seg = P.arena + 6 * sb;
             ncomp = with_walk_scratch(T, tp_walk_b_ints(S, R), tp_walk_b_dbls(S), s_walk, lds_bytes,
@@ -904,6 +923,7 @@ __global__ void __launch_bounds__(64) k_tp_walk_b(DPairs P, DMeshes M, DBodies B
                 const int* src = T.adj + ao;
                 for (int k = lane_id(); k < S * (R + 1); k += WAVE) adj[k] = src[k];
                 __syncthreads();
+                wp.mark(0);
                 int ok = 1;
                 for (int k = lane_id(); k < S; k += WAVE) ok &= adj[k] <= R;
                 if (!__all(ok)) return -1;
@@ -923,6 +943,7 @@ __global__ void __launch_bounds__(64) k_tp_walk_b(DPairs P, DMeshes M, DBodies B
             T.tj_off[it] = ncomp >= 0 ? to : -1;
         }
         __syncthreads();
+        wp.mark(4);
     }
 }
 
@@ -1315,6 +1336,14 @@ __global__ void k_iota_pairs(int32_t* pi, int32_t* pj, int32_t n) {
 inline int nblk(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
 }  // namespace
+
+#ifdef ZDX_WALK_PROFILE
+extern "C" int zdx_walk_profile(unsigned long long* out) {
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_walk_prof), 16 * sizeof(unsigned long long)) != hipSuccess) return 1;
+    unsigned long long z[16] = {0};
+    return hipMemcpyToSymbol(HIP_SYMBOL(g_walk_prof), z, sizeof(z)) == hipSuccess ? 0 : 1;
+}
+#endif
 
 
 

@@ -112,6 +112,7 @@ struct IntroSorter {
     }
 
     ZDX_HD void introsort_loop(int first, int last, int depth) {
+        if (last - first <= 16) return;   // left to the final insertion sort
         // explicit stack for the right-hand recursion (depth <= 2*log2(n))
         int st_first[64], st_last[64], st_depth[64];
         int sp = 0;

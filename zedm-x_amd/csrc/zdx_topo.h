@@ -204,6 +204,7 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
         cbeg[0] = 0; csz[0] = S; cidx[0] = 0;
         ncomp = 1;
     }
+    wp.mark(1);
     // ---- snap: union-find over the hits inside each component; the
     // partition does not depend on the order of the unions
     for (int k = ln; k < n; k += W) { par[k] = k; rnk[k] = 0; rootvid[k] = -1; }
@@ -239,6 +240,7 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
             }
         }
     }
+    wp.mark(2);
     // ---- per component: first-occurrence vertex ids, then each vertex's
     // mean over its points summed in point order (one lane per vertex), edges
     int vb = 0, eb = 0, tw = 0;
@@ -303,6 +305,7 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
         tw += ne * tp_row_words(nv);
         wp.sync();
     }
+    wp.mark(3);
     *tj_words = tw;
     return ncomp;
 }
