@@ -457,6 +457,11 @@ struct LaneHost {
     uint64_t ballot(bool p) const { return p ? 1ull : 0ull; }
     double shfl(double v, int) const { return v; }
     uint64_t shfl64(uint64_t v, int) const { return v; }
+    int atomic_cas(int* p, int expected, int desired) const {
+        const int old = *p;
+        if (old == expected) *p = desired;
+        return old;
+    }
     void sync() const {}
     void mark(int) const {}   // phase boundary (timed by profiling builds of the device policy)
 };

@@ -778,6 +778,7 @@ struct LaneDev {
         const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, WAVE);
         return ((uint64_t)hi << 32) | lo;
     }
+    __device__ int atomic_cas(int* p, int expected, int desired) const { return atomicCAS(p, expected, desired); }
     __device__ void sync() const { __syncthreads(); }
 #ifdef ZDX_WALK_PROFILE
     mutable unsigned long long t0 = 0;

@@ -205,41 +205,45 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
         ncomp = 1;
     }
     wp.mark(1);
-    // ---- snap: union-find over the hits inside each component; the
-    // partition does not depend on the order of the unions
-    for (int k = ln; k < n; k += W) { par[k] = k; rnk[k] = 0; rootvid[k] = -1; }
+    // ---- snap: connected components of the snap hits inside each component
+    // (the reference's union-find partition, which does not depend on the
+    // order of the unions).  Lanes take the hit words in parallel and link
+    // roots lock-free: a root only ever points to a smaller index (compare and
+    // swap), so lab[] stays a forest; then every point is pointed at its root.
+    int* lab = par;            // 2S
+    for (int k = ln; k < n; k += W) lab[k] = k;
     wp.sync();
-    auto find = [&](int x) {
-        int r = x;
-        while (par[r] != r) r = par[r];
-        while (par[x] != r) { const int nx = par[x]; par[x] = r; x = nx; }
-        return r;
+    auto root = [&](int x) {
+        int y = lab[x];
+        while (y != x) { x = y; y = lab[x]; }
+        return x;
     };
     const int64_t npairs = (int64_t)n * (n - 1) / 2;
     const int64_t nwords = (npairs + 63) >> 6;
     for (int64_t wb = 0; wb < nwords; wb += W) {
         const int64_t w = wb + ln;
-        const uint64_t word = w < nwords ? snapw[w] : 0ull;
-        uint64_t m = wp.ballot(word != 0ull);
-        while (m) {
-            const int src = ctz64(m);
-            m &= m - 1;
-            uint64_t bits = wp.shfl64(word, src);
-            while (bits) {
-                const int64_t t = ((wb + src) << 6) + ctz64(bits);
-                bits &= bits - 1;
-                int a, b;
-                tp_pair_of(t, a, b);
-                if (compof[a >> 1] != compof[b >> 1]) continue;
-                const int ra = find(a), rb = find(b);
-                if (ra != rb) {
-                    if (rnk[ra] < rnk[rb]) par[ra] = rb;
-                    else if (rnk[ra] > rnk[rb]) par[rb] = ra;
-                    else { par[rb] = ra; rnk[ra] += 1; }
-                }
+        uint64_t word = w < nwords ? snapw[w] : 0ull;
+        while (word) {
+            const int64_t t = (w << 6) + ctz64(word);
+            word &= word - 1;
+            int a, b;
+            tp_pair_of(t, a, b);
+            if (compof[a >> 1] != compof[b >> 1]) continue;
+            int ra = root(a), rb = root(b);
+            while (ra != rb) {
+                if (ra < rb) { const int x = ra; ra = rb; rb = x; }   // link the larger root under the smaller
+                const int old = wp.atomic_cas(&lab[ra], ra, rb);
+                if (old == ra) break;
+                ra = root(old);
+                rb = root(rb);
             }
         }
     }
+    wp.sync();
+    for (int k = ln; k < n; k += W) lab[k] = root(k);
+    for (int k = ln; k < n; k += W) rootvid[k] = -1;
+    wp.sync();
+    auto find = [&](int x) { return lab[x]; };
     wp.mark(2);
     // ---- per component: first-occurrence vertex ids, then each vertex's
     // mean over its points summed in point order (one lane per vertex), edges
