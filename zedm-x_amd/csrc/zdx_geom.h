@@ -185,53 +185,15 @@ ZDX_HD inline bool seg_plane(const V3& p0, const V3& p1, const V3& n, double d, 
     out = add(p0, mul(u, t));
     return true;
 }
-// tri_tri_intersection_segment main.cpp:579-631 (unique_points 561-577 inlined
-// on a fixed array of at most 6 candidates)
-ZDX_HD inline bool tri_tri(const V3* A, const V3& nA, double dA, const V3* B, const V3& nB, double dB,
-                           V3& o0, V3& o1) {
-    V3 tau = cross(nA, nB);
-    double lt = norm(tau);
-    if (lt < 1e-20) return false;
-    tau = dvd(tau, lt);
-    V3 raw[6];
-    int nr = 0;
-    for (int i = 0; i < 3; ++i) {
-        V3 p;
-        if (seg_plane(A[i], A[(i + 1) % 3], nB, dB, p) && point_in_tri(p, B, nB)) raw[nr++] = p;
-    }
-    for (int i = 0; i < 3; ++i) {
-        V3 p;
-        if (seg_plane(B[i], B[(i + 1) % 3], nA, dA, p) && point_in_tri(p, A, nA)) raw[nr++] = p;
-    }
-    V3 c[6];
-    int nc = 0;
-    const double t2 = 1e-9 * 1e-9;
-    for (int i = 0; i < nr; ++i) {
-        bool keep = true;
-        for (int j = 0; j < nc; ++j)
-            if (norm2(sub(raw[i], c[j])) <= t2) { keep = false; break; }
-        if (keep) c[nc++] = raw[i];
-    }
-    if (nc < 2) return false;
-    double mn = INFINITY, mx = -INFINITY;
-    int i0 = -1, i1 = -1;
-    for (int i = 0; i < nc; ++i) {
-        double t = dot(c[i], tau);
-        if (t < mn) { mn = t; i0 = i; }
-        if (t > mx) { mx = t; i1 = i; }
-    }
-    if (i0 == i1) return false;
-    V3 p0 = c[i0], p1 = c[i1];
-    if (norm2(sub(p1, p0)) < 1e-18) return false;
-    if (dot(sub(p1, p0), tau) < 0.0) { V3 t = p0; p0 = p1; p1 = t; }
-    o0 = p0; o1 = p1;
-    return true;
-}
-
-// tri_tri with the provenance of the two endpoints: code r = candidate index
-// (0..2: edge r of A crossing B's plane, 3..5: edge r-3 of B crossing A's
-// plane), + 8 when the edge parameter was clamped to 0, + 16 when to 1.  The
-// arithmetic is tri_tri's, so the endpoints are identical.
+// tri_tri_intersection_segment main.cpp:579-631, unique_points 561-577 inlined.
+// The candidates are deduplicated as they come (a candidate is kept unless it
+// lies within 1e-9 of an already kept one -- what unique_points does over the
+// candidate list in order), and the kept points live in a fixed array that is
+// only ever indexed by unrolled constants, so it stays in registers.  With
+// codes != nullptr the provenance of the two endpoints is returned: r =
+// candidate index (0..2: edge r of A crossing B's plane, 3..5: edge r-3 of B
+// crossing A's plane), + 8 when the edge parameter was clamped to 0, + 16 when
+// to 1.
 ZDX_HD inline bool seg_plane_c(const V3& p0, const V3& p1, const V3& n, double d, V3& out, int& clamp) {
     V3 u = sub(p1, p0);
     double den = dot(n, u);
@@ -244,51 +206,69 @@ ZDX_HD inline bool seg_plane_c(const V3& p0, const V3& p1, const V3& n, double d
     out = add(p0, mul(u, t));
     return true;
 }
-ZDX_HD inline bool tri_tri_prov(const V3* A, const V3& nA, double dA, const V3* B, const V3& nB, double dB,
-                                V3& o0, V3& o1, int& code0, int& code1) {
+ZDX_HD inline bool tri_tri_impl(const V3* A, const V3& nA, double dA, const V3* B, const V3& nB, double dB, V3& o0,
+                                V3& o1, int* codes) {
     V3 tau = cross(nA, nB);
     double lt = norm(tau);
     if (lt < 1e-20) return false;
     tau = dvd(tau, lt);
-    V3 raw[6];
-    int rc[6];
-    int nr = 0;
-    for (int i = 0; i < 3; ++i) {
-        V3 p;
-        int cl = 0;
-        if (seg_plane_c(A[i], A[(i + 1) % 3], nB, dB, p, cl) && point_in_tri(p, B, nB)) { rc[nr] = i | cl; raw[nr++] = p; }
-    }
-    for (int i = 0; i < 3; ++i) {
-        V3 p;
-        int cl = 0;
-        if (seg_plane_c(B[i], B[(i + 1) % 3], nA, dA, p, cl) && point_in_tri(p, A, nA)) { rc[nr] = (3 + i) | cl; raw[nr++] = p; }
-    }
     V3 c[6];
     int cc[6];
     int nc = 0;
     const double t2 = 1e-9 * 1e-9;
-    for (int i = 0; i < nr; ++i) {
+    auto take = [&](const V3& p, int code) {
         bool keep = true;
-        for (int j = 0; j < nc; ++j)
-            if (norm2(sub(raw[i], c[j])) <= t2) { keep = false; break; }
-        if (keep) { cc[nc] = rc[i]; c[nc++] = raw[i]; }
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+            if (j < nc && keep && norm2(sub(p, c[j])) <= t2) keep = false;
+        if (!keep) return;
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+            if (j == nc) { c[j] = p; cc[j] = code; }
+        ++nc;
+    };
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        V3 p;
+        int cl = 0;
+        if (seg_plane_c(A[i], A[(i + 1) % 3], nB, dB, p, cl) && point_in_tri(p, B, nB)) take(p, i | cl);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        V3 p;
+        int cl = 0;
+        if (seg_plane_c(B[i], B[(i + 1) % 3], nA, dA, p, cl) && point_in_tri(p, A, nA)) take(p, (3 + i) | cl);
     }
     if (nc < 2) return false;
     double mn = INFINITY, mx = -INFINITY;
-    int i0 = -1, i1 = -1;
-    for (int i = 0; i < nc; ++i) {
-        double t = dot(c[i], tau);
-        if (t < mn) { mn = t; i0 = i; }
-        if (t > mx) { mx = t; i1 = i; }
+    int i0 = -1, i1 = -1, k0 = 0, k1 = 0;
+    V3 p0 = v3(0.0, 0.0, 0.0), p1 = v3(0.0, 0.0, 0.0);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        if (j < nc) {
+            const double t = dot(c[j], tau);
+            if (t < mn) { mn = t; i0 = j; p0 = c[j]; k0 = cc[j]; }
+            if (t > mx) { mx = t; i1 = j; p1 = c[j]; k1 = cc[j]; }
+        }
     }
     if (i0 == i1) return false;
-    V3 p0 = c[i0], p1 = c[i1];
-    int k0 = cc[i0], k1 = cc[i1];
     if (norm2(sub(p1, p0)) < 1e-18) return false;
     if (dot(sub(p1, p0), tau) < 0.0) { V3 t = p0; p0 = p1; p1 = t; int k = k0; k0 = k1; k1 = k; }
     o0 = p0; o1 = p1;
-    code0 = k0; code1 = k1;
+    if (codes) { codes[0] = k0; codes[1] = k1; }
     return true;
+}
+ZDX_HD inline bool tri_tri(const V3* A, const V3& nA, double dA, const V3* B, const V3& nB, double dB, V3& o0,
+                           V3& o1) {
+    return tri_tri_impl(A, nA, dA, B, nB, dB, o0, o1, nullptr);
+}
+ZDX_HD inline bool tri_tri_prov(const V3* A, const V3& nA, double dA, const V3* B, const V3& nB, double dB,
+                                V3& o0, V3& o1, int& code0, int& code1) {
+    int k[2] = {0, 0};
+    const bool r = tri_tri_impl(A, nA, dA, B, nB, dB, o0, o1, k);
+    code0 = k[0];
+    code1 = k[1];
+    return r;
 }
 
 // Sensitivity of a segment endpoint to rigid motions of the two bodies.

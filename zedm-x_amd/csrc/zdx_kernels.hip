@@ -610,27 +610,38 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
         const int64_t total = (int64_t)nA * nB;
         int nseg = 0, hits = 0;
         for (int64_t base = 0; base < total; base += (int64_t)BS * K) {
-            int a[K], b[K];
-            int myh = 0;
-            int64_t g0 = base + (int64_t)tid * K;
-            int ca = 0, cb = 0;
-            if (g0 < total) { ca = (int)(g0 / nB); cb = (int)(g0 - (int64_t)ca * nB); }
-            float fa[6];
-            for (int k = 0; k < 6; ++k) fa[k] = FA[6 * ca + k];
+            // K consecutive items of the flattened product per thread; hits
+            // as a bit mask (no per-thread arrays), written in item order
+            unsigned hm = 0u;
+            const int64_t g0 = base + (int64_t)tid * K;
+            int ca0 = 0, cb0 = 0;
+            if (g0 < total) { ca0 = (int)(g0 / nB); cb0 = (int)(g0 - (int64_t)ca0 * nB); }
+            {
+                int ca = ca0, cb = cb0;
+                float fa[6];
+                for (int k = 0; k < 6; ++k) fa[k] = FA[6 * ca + k];
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                int64_t g = g0 + k;
-                if (g < total) {
-                    if (ovl_f(fa, FB + 6 * cb)) { a[myh] = ca; b[myh] = cb; ++myh; }
-                    if (++cb == nB) {
-                        cb = 0; ++ca;
-                        if (ca < nA) for (int q = 0; q < 6; ++q) fa[q] = FA[6 * ca + q];
+                for (int k = 0; k < K; ++k) {
+                    if (g0 + k < total) {
+                        if (ovl_f(fa, FB + 6 * cb)) hm |= 1u << k;
+                        if (++cb == nB) {
+                            cb = 0; ++ca;
+                            if (ca < nA) for (int q = 0; q < 6; ++q) fa[q] = FA[6 * ca + q];
+                        }
                     }
                 }
             }
+            const int myh = __popc(hm);
             int nh;
             int off = block_excl<BS>(myh, s_wave, nh);
-            for (int k = 0; k < myh; ++k) { s_ha[off + k] = a[k]; s_hb[off + k] = b[k]; }
+            {
+                int ca = ca0, cb = cb0;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    if (hm & (1u << k)) { s_ha[off] = ca; s_hb[off] = cb; ++off; }
+                    if (++cb == nB) { cb = 0; ++ca; }
+                }
+            }
             __syncthreads();
             hits += nh;
             for (int hb = 0; hb < nh; hb += BS) {
