@@ -111,13 +111,16 @@ __global__ void k_kt_reset(unsigned long long* kt, int n) {
 // cannot become degenerate under a rotation plus a translation of magnitude
 // <= 1e3 (rounding error bound, see build_mesh on the host), so only the
 // mesh's "suspect" triangles are tested, or all of them for far-away bodies.
+// One wave per body of the list (4 bodies per 256-thread workgroup): world
+// vertices, AABB, and the count of non-degenerate triangles (only the mesh's
+// suspect triangles can degenerate, unless the body is far from the origin).
 __global__ void __launch_bounds__(256) k_transform(DBodies B, DMeshes M, const int32_t* list, const int32_t* n_list,
                                                    unsigned long long* kt) {
     kt_start(kt);
-    __shared__ double s_red[6][256 / WAVE];
-    __shared__ int s_bad;
     const int nb = list ? *n_list : B.n;
-    for (int it = blockIdx.x; it < nb; it += gridDim.x) {
+    const int wpb = blockDim.x / WAVE;
+    const int ln = lane_id();
+    for (int it = blockIdx.x * wpb + threadIdx.x / WAVE; it < nb; it += gridDim.x * wpb) {
         const int b = list ? list[it] : it;
         const int m = B.mesh[b];
         const double* st = B.state + 16 * (int64_t)b;
@@ -129,7 +132,7 @@ __global__ void __launch_bounds__(256) k_transform(DBodies B, DMeshes M, const i
         const double* bv = M.verts + 3 * (int64_t)M.vert_off[m];
         double* wv = B.wv + 3 * B.wv_off[b];
         double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (int v = threadIdx.x; v < nv; v += blockDim.x) {
+        for (int v = ln; v < nv; v += WAVE) {
             Qt pv; pv.w = 0.0; pv.x = bv[3 * v]; pv.y = bv[3 * v + 1]; pv.z = bv[3 * v + 2];
             Qt r = qmul(qmul(qn, pv), qc);
             V3 w = add(v3(r.x, r.y, r.z), pos);
@@ -143,19 +146,9 @@ __global__ void __launch_bounds__(256) k_transform(DBodies B, DMeshes M, const i
                 mx[k] = fmax(mx[k], __shfl_xor(mx[k], d, WAVE));
             }
         }
-        const int wid = threadIdx.x / WAVE;
-        if (lane_id() == 0) {
-            for (int k = 0; k < 3; ++k) { s_red[k][wid] = mn[k]; s_red[3 + k][wid] = mx[k]; }
-        }
-        if (threadIdx.x == 0) s_bad = 0;
-        __syncthreads();
-        if (threadIdx.x == 0) {
+        if (ln == 0) {
             double* box = B.aabb + 6 * (int64_t)b;
-            for (int k = 0; k < 3; ++k) {
-                double a = s_red[k][0], c = s_red[3 + k][0];
-                for (int w = 1; w < (int)(blockDim.x / WAVE); ++w) { a = fmin(a, s_red[k][w]); c = fmax(c, s_red[3 + k][w]); }
-                box[k] = a; box[3 + k] = c;
-            }
+            for (int k = 0; k < 3; ++k) { box[k] = mn[k]; box[3 + k] = mx[k]; }
         }
         const int T = M.ntris[m];
         const int32_t* ti = M.tri_idx + 3 * (int64_t)M.tri_off[m];
@@ -163,17 +156,19 @@ __global__ void __launch_bounds__(256) k_transform(DBodies B, DMeshes M, const i
         const int nchk = far ? T : M.nsusp[m];
         const int32_t* sl = M.susp + M.susp_off[m];
         int bad = 0;
-        for (int k = threadIdx.x; k < nchk; k += blockDim.x) {
-            const int t = far ? k : sl[k];
-            V3 a0 = ld3(wv + 3 * ti[3 * t]), a1 = ld3(wv + 3 * ti[3 * t + 1]), a2 = ld3(wv + 3 * ti[3 * t + 2]);
-            V3 n; double d;
-            bad += plane_of(a0, a1, a2, n, d) ? 0 : 1;
+        if (nchk > 0) {
+            // the world vertices just written by this wave's lanes: make them
+            // visible before other lanes read them back
+            __threadfence_block();
+            for (int k = ln; k < nchk; k += WAVE) {
+                const int t = far ? k : sl[k];
+                V3 a0 = ld3(wv + 3 * ti[3 * t]), a1 = ld3(wv + 3 * ti[3 * t + 1]), a2 = ld3(wv + 3 * ti[3 * t + 2]);
+                V3 n; double d;
+                bad += plane_of(a0, a1, a2, n, d) ? 0 : 1;
+            }
+            for (int d = 32; d >= 1; d >>= 1) bad += __shfl_xor(bad, d, WAVE);
         }
-        for (int d = 32; d >= 1; d >>= 1) bad += __shfl_xor(bad, d, WAVE);
-        if (nchk > 0 && lane_id() == 0) atomicAdd(&s_bad, bad);
-        __syncthreads();
-        if (threadIdx.x == 0) B.nondeg[b] = T - s_bad;
-        __syncthreads();
+        if (ln == 0) B.nondeg[b] = T - bad;
     }
     kt_end(kt);
 }
@@ -1367,7 +1362,7 @@ void launch_kt_reset(unsigned long long* kt, int n, hipStream_t s) {
 void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s, unsigned long long* kt, const int32_t* list,
                       const int32_t* n_list, int32_t groups) {
     if (B.n <= 0) return;
-    int g = list ? groups : B.n;
+    int g = list ? groups : (B.n + 3) / 4;
     if (g < 1) g = 1;
     hipLaunchKernelGGL(k_transform, dim3(g), dim3(256), 0, s, B, M, list, n_list, kt);
 }
