@@ -442,6 +442,10 @@ struct LaneHost {
         if (old == expected) *p = desired;
         return old;
     }
+    void atomic_min(int* p, int v) const { if (v < *p) *p = v; }
+    void atomic_add(int* p, int v) const { *p += v; }
+    int readlane(int v, int) const { return v; }
+    int scan_excl(int v, int& total) const { total = v; return 0; }   // exclusive prefix over the lanes
     void sync() const {}
     void mark(int) const {}   // phase boundary (timed by profiling builds of the device policy)
 };

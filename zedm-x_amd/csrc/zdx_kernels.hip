@@ -285,12 +285,16 @@ __global__ void __launch_bounds__(256) k_bp_bounds_part(const double* state, con
 __global__ void __launch_bounds__(64) k_bp_bounds(const double* part, int nparts, DGrid* g, int max_cells,
                                                   int32_t* status) {
     __shared__ double s[7][1];
-    if (threadIdx.x < 7) {
-        const int k = threadIdx.x;
-        double r = part[k];
-        for (int b = 1; b < nparts; ++b) r = k < 3 ? fmin(r, part[7 * b + k]) : fmax(r, part[7 * b + k]);
-        s[k][0] = r;
-    }
+    double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+        for (int k = 0; k < 7; ++k) v[k] = k < 3 ? fmin(v[k], part[7 * b + k]) : fmax(v[k], part[7 * b + k]);
+    for (int d = 32; d >= 1; d >>= 1)
+        for (int k = 0; k < 7; ++k) {
+            const double o = __shfl_xor(v[k], d, WAVE);
+            v[k] = k < 3 ? fmin(v[k], o) : fmax(v[k], o);
+        }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 7; ++k) s[k][0] = v[k];
     __syncthreads();
     if (threadIdx.x != 0) return;
     DGrid G;
@@ -785,6 +789,19 @@ struct LaneDev {
         return ((uint64_t)hi << 32) | lo;
     }
     __device__ int atomic_cas(int* p, int expected, int desired) const { return atomicCAS(p, expected, desired); }
+    __device__ void atomic_min(int* p, int v) const { atomicMin(p, v); }
+    __device__ void atomic_add(int* p, int v) const { atomicAdd(p, v); }
+    // value of lane l (l the same in every lane)
+    __device__ int readlane(int v, int l) const { return __builtin_amdgcn_readlane(v, l); }
+    __device__ int scan_excl(int v, int& total) const {
+        int x = v;
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const int y = __shfl_up(x, d, WAVE);
+            if (lane_id() >= d) x += y;
+        }
+        total = __shfl(x, WAVE - 1, WAVE);
+        return x - v;
+    }
     __device__ void sync() const { __syncthreads(); }
 #ifdef ZDX_WALK_PROFILE
     mutable unsigned long long t0 = 0;
@@ -846,18 +863,23 @@ __global__ void __launch_bounds__(256) k_tp_pass_a(DPairs P, DMeshes M, DBodies 
                 cntp[i] = m;
                 if (m > R) atomicMax(T.top + 3, (unsigned long long)m);
             }
+            // snap hits row by row (row a: pairs (a, b), b < a; a thread takes
+            // rows a and n-1-a), set in the zeroed words with atomic OR
             uint64_t* w = T.snapw + so;
-            // each wave covers 64 consecutive pair numbers: one word per ballot
-            for (int kb = 0; kb < nw * 64; kb += blockDim.x) {
-                const int t = kb + threadIdx.x;
-                bool hit = false;
-                if (t < npt) {
-                    int a, b;
-                    tp_pair_of(t, a, b);
-                    hit = tp_snap_hit(seg, c9, h, a, b);
+            for (int k = threadIdx.x; k < nw; k += blockDim.x) w[k] = 0ull;
+            __syncthreads();
+            const int np = 2 * S;
+            for (int r = threadIdx.x; r < (np + 1) / 2; r += blockDim.x) {
+                for (int side = 0; side < 2; ++side) {
+                    const int a = side ? np - 1 - r : r;
+                    if (side && a == r) break;
+                    const int t0 = a * (a - 1) / 2;
+                    for (int b = 0; b < a; ++b)
+                        if (tp_snap_hit(seg, c9, h, a, b)) {
+                            const int t = t0 + b;
+                            atomicOr((unsigned long long*)(w + (t >> 6)), 1ull << (t & 63));
+                        }
                 }
-                const uint64_t m = __ballot(hit);
-                if (lane_id() == 0 && (t >> 6) < nw) w[t >> 6] = m;
             }
         }
         __syncthreads();

@@ -171,6 +171,31 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
                 const int first = sp;
                 const int* row = adjl + (int64_t)i * R;
                 const int m = adjn[i];
+                if (m <= W) {
+                    // lanes take the row's entries (ascending j); the accepted,
+                    // unvisited ones go on the stack ordered by (scan key, id)
+                    const int e = ln < m ? row[ln] : 0;
+                    const int jj = e & 0xFFFFFF, kj = e >> 24;
+                    const bool acc = ln < m && !vis[jj];
+                    const uint64_t am = wp.ballot(acc);
+                    int rank = 0;
+                    for (uint64_t b = am; b; b &= b - 1) {
+                        const int l = ctz64(b);
+                        const int kl = wp.readlane(kj, l);
+                        rank += (kl < kj || (kl == kj && l < ln)) ? 1 : 0;
+                    }
+                    const int na = __builtin_popcountll(am);
+                    wp.sync();
+                    if (acc) {
+                        stk[first + rank] = jj;
+                        vis[jj] = 1;
+                        cord[nord + rank] = jj;
+                    }
+                    wp.sync();
+                    sp += na;
+                    nord += na;
+                    continue;
+                }
                 for (int q = 0; q < m; ++q) {
                     const int e = row[q];
                     const int jj = e & 0xFFFFFF, kj = e >> 24;
@@ -251,20 +276,46 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
     for (int q = 0; q < ncomp; ++q) {
         const int c = cidx[q];
         const int base = cbeg[c], ns = csz[c];
+        const int np = 2 * ns;
+        auto ptk = [&](int k) { return 2 * cord[base + (k >> 1)] + (k & 1); };
+        // first position of every cluster in the component's point order
+        int* firstp = rnk;
+        for (int k = ln; k < np; k += W) firstp[find(ptk(k))] = np;
+        wp.sync();
+        for (int k = ln; k < np; k += W) wp.atomic_min(&firstp[find(ptk(k))], k);
+        wp.sync();
+        // vertex id of a cluster = number of clusters that start before it
         int nv = 0;
-        for (int k = 0; k < 2 * ns; ++k) {
-            const int pt = 2 * cord[base + (k >> 1)] + (k & 1);
-            const int r = find(pt);
-            if (rootvid[r] < 0) { rootvid[r] = nv; cnt[nv] = 0; ++nv; }
-            const int v = rootvid[r];
-            cnt[v] += 1;
-            ptv[pt] = v;
+        for (int kb = 0; kb < np; kb += W) {
+            const int k = kb + ln;
+            const int r = k < np ? find(ptk(k)) : 0;
+            const bool f = k < np && firstp[r] == k;
+            const uint64_t fm = wp.ballot(f);
+            if (f) rootvid[r] = nv + (int)__builtin_popcountll(fm & ((1ull << ln) - 1ull));
+            nv += (int)__builtin_popcountll(fm);
         }
+        for (int v = ln; v < nv; v += W) cnt[v] = 0;
+        wp.sync();
+        for (int k = ln; k < np; k += W) {
+            const int pt = ptk(k);
+            const int v = rootvid[find(pt)];
+            ptv[pt] = v;
+            wp.atomic_add(&cnt[v], 1);
+        }
+        wp.sync();
         int acc = 0;
-        for (int v = 0; v < nv; ++v) { voff[v] = acc; vfill[v] = acc; acc += cnt[v]; }
-        voff[nv] = acc;
-        for (int k = 0; k < 2 * ns; ++k) {
-            const int pt = 2 * cord[base + (k >> 1)] + (k & 1);
+        for (int vb0 = 0; vb0 < nv; vb0 += W) {
+            const int v = vb0 + ln;
+            const int c = v < nv ? cnt[v] : 0;
+            int tot;
+            const int off = wp.scan_excl(c, tot);
+            if (v < nv) { voff[v] = acc + off; vfill[v] = acc + off; }
+            acc += tot;
+        }
+        if (ln == 0) voff[nv] = acc;
+        wp.sync();
+        for (int k = 0; k < np; ++k) {
+            const int pt = ptk(k);
             mem[vfill[ptv[pt]]++] = pt;
         }
         wp.sync();
