@@ -822,13 +822,18 @@ struct LaneDev {
 // (persistent); walk kernels: one wave per pair, scratch in LDS when it fits,
 // else the workgroup's slice of the global scratch.
 
+constexpr int TP_PASS_A_SEGS = 384;   // segments pass A stages in LDS (84 bytes each)
+
 // Pass A: representative-point cells, the adjacency rows (per segment the
 // count and up to R accepted candidates, one thread per row) and the snap hit
 // words of every pair; each pair's rows and snap words are bump-allocated (a
 // pair that does not fit gets offset -1 and the step is retried with larger
 // buffers: the counters keep the total needed, top[3] the longest row).
-__global__ void __launch_bounds__(256) k_tp_pass_a(DPairs P, DMeshes M, DBodies B, TopoBufs T,
+__global__ void __launch_bounds__(256) k_tp_pass_a(DPairs P, DMeshes M, DBodies B, TopoBufs T, int32_t lds_segs,
                                                   unsigned long long* kt) {
+    // the pair's segments (6 doubles) and cells (9 ints) are staged in LDS
+    // when it holds them (lds_segs segments), else read from the arena
+    extern __shared__ char s_pa[];
     __shared__ long long s_off[2];
     kt_start(kt);
     const int n = *P.topo_n;
@@ -838,8 +843,11 @@ __global__ void __launch_bounds__(256) k_tp_pass_a(DPairs P, DMeshes M, DBodies 
         const int S = P.nseg[p];
         const int64_t sb = P.seg_off[p];
         const double h = M.tol[B.mesh[P.pi[p]]];
-        const double* seg = P.arena + 6 * sb;
-        int* c9 = T.cell9 + 9 * sb;
+        const double* gseg = P.arena + 6 * sb;
+        int* gc9 = T.cell9 + 9 * sb;
+        const bool staged = S <= lds_segs;
+        double* seg = staged ? (double*)s_pa : (double*)gseg;
+        int* c9 = staged ? (int*)(s_pa + 48 * (int64_t)lds_segs) : gc9;
         const int nadj = S * (R + 1);
         const int npt = S * (2 * S - 1);   // endpoint pairs of 2S points
         const int nw = (npt + 63) >> 6;
@@ -852,6 +860,9 @@ __global__ void __launch_bounds__(256) k_tp_pass_a(DPairs P, DMeshes M, DBodies 
             T.adj_off[it] = ao;
             T.snap_off[it] = so;
         }
+        if (staged)
+            for (int k = threadIdx.x; k < 6 * S; k += blockDim.x) seg[k] = gseg[k];
+        __syncthreads();
         for (int k = threadIdx.x; k < S; k += blockDim.x) tp_cells(seg, k, h, c9 + 9 * k);
         __syncthreads();
         const long long ao = s_off[0], so = s_off[1];
@@ -1469,7 +1480,8 @@ void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, i
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
                      const TopoBufs& T, const SimParams& sp, hipStream_t s, unsigned long long* kt) {
     launch_fill_u64(T.top, 0ull, 4, s);
-    hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T, kt);
+    hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(256), 84 * TP_PASS_A_SEGS, s, P, M, B, T,
+                       (int32_t)TP_PASS_A_SEGS, kt);
     hipLaunchKernelGGL(k_tp_walk_b, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, T, sp.split,
                        T.lds_bytes);
     hipLaunchKernelGGL(k_tp_pass_c, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T);
@@ -1480,7 +1492,8 @@ void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const 
                      hipStream_t s) {
     launch_fill_u64(T.top, 0ull, 4, s);
     if (T.ej) hipLaunchKernelGGL(k_tp_endpoint_jac, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T);
-    hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T, nullptr);
+    hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(256), 84 * TP_PASS_A_SEGS, s, P, M, B, T,
+                       (int32_t)TP_PASS_A_SEGS, nullptr);
     hipLaunchKernelGGL(k_tp_walk_b, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, T, split,
                        T.lds_bytes);
     hipLaunchKernelGGL(k_tp_pass_c, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T);
