@@ -132,6 +132,115 @@ def _tpl_radius(wl, t):
     return _TPL_RAD[t]
 
 
+COPY_BYTES = 256 << 20   # plain copy used to scale the byte counters
+
+
+def copy_calibration():
+    """A device-to-device copy of COPY_BYTES by an elementwise copy kernel
+    (torch), for scaling FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+
+    n = COPY_BYTES // 8
+    src = torch.arange(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+
+
+def _pmc_by_kernel(d, counter):
+    """Per-dispatch values of `counter` (summed over its instances) from a
+    rocprofv3 --pmc csv directory: {kernel name: [values in dispatch order]}."""
+    import csv
+    import glob
+
+    out = {}
+    for f in glob.glob(os.path.join(d, "*counter_collection.csv")):
+        per = {}
+        for r in csv.DictReader(open(f)):
+            if r.get("Counter_Name") != counter:
+                continue
+            key = (int(r["Dispatch_Id"]), r["Kernel_Name"])
+            per[key] = per.get(key, 0.0) + float(r["Counter_Value"])
+        for (did, name), v in sorted(per.items()):
+            out.setdefault(name, []).append(v)
+    return out
+
+
+def _trace_durations(d, pred):
+    """Durations (us) of the dispatches of kernels matching pred, in dispatch
+    order, from a rocprofv3 --kernel-trace csv directory."""
+    import csv
+    import glob
+
+    rows = []
+    for f in glob.glob(os.path.join(d, "*kernel_trace.csv")):
+        for r in csv.DictReader(open(f)):
+            if pred(r["Kernel_Name"]):
+                rows.append((int(r["Dispatch_Id"]), (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000.0))
+    return [t for _, t in sorted(rows)]
+
+
+def summarize_profiles(stats_dir, fetch_dir, write_dir, command="", timed_stats=20, timed_pmc=5):
+    """profiles/ summaries: per-kernel durations of the bench
+    (rocprofv3 --kernel-trace --stats) and the HBM traffic of the narrow-phase
+    kernel from FETCH_SIZE / WRITE_SIZE (KiB per dispatch), each scaled by what
+    the same counter reads for a plain copy of COPY_BYTES."""
+    import csv
+    import glob
+
+    prof = os.path.join(ROOT, "profiles")
+    os.makedirs(prof, exist_ok=True)
+    stats = []
+    for f in glob.glob(os.path.join(stats_dir, "*kernel_stats.csv")):
+        stats += list(csv.DictReader(open(f)))
+    stats.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    lines = ["# Kernel durations -- granular bed, 100k polyhedra, batched step (graph capture off)", "",
+             f"Command: `{command}`" if command else "", "",
+             "| kernel | calls | average us | total % |", "|---|---|---|---|"]
+    for r in stats[:30]:
+        name = r["Name"].replace("zdx::(anonymous namespace)::", "").split("(")[0]
+        lines.append(f"| `{name}` | {r['Calls']} | {float(r['AverageNs']) / 1000:.1f} | {float(r['Percentage']):.2f} |")
+    tn = _trace_durations(stats_dir, lambda k: "k_narrow" in k)[-timed_stats:]
+    if tn:
+        lines += ["", f"k_narrow over the last {len(tn)} dispatches (the timed steps): average "
+                      f"{sum(tn) / len(tn):.1f} us (min {min(tn):.1f}, max {max(tn):.1f})"]
+    open(os.path.join(prof, "kernels_granular_bed_100k.md"), "w").write("\n".join(lines) + "\n")
+    fetch = _pmc_by_kernel(fetch_dir, "FETCH_SIZE")
+    write = _pmc_by_kernel(write_dir, "WRITE_SIZE")
+
+    def pick(d, pred):
+        vals = [v for k, vs in d.items() if pred(k) for v in vs]
+        return vals
+
+    is_copy = lambda k: ("copy" in k.lower() or "elementwise" in k.lower()) and "zdx" not in k
+    is_narrow = lambda k: "k_narrow" in k
+    cf, cw = pick(fetch, is_copy), pick(write, is_copy)
+    nf, nwr = pick(fetch, is_narrow)[-timed_pmc:], pick(write, is_narrow)[-timed_pmc:]
+    res = {}
+    if cf and cw and nf and nwr:
+        kib = COPY_BYTES / 1024.0
+        f_scale = kib / max(cf)     # bytes the copy moved / what the counter says
+        w_scale = kib / max(cw)
+        fetch_b = sum(nf) / len(nf) * f_scale * 1024.0
+        write_b = sum(nwr) / len(nwr) * w_scale * 1024.0
+        res = dict(kernel="k_narrow", dispatches=len(nf), fetch_bytes_per_launch=fetch_b,
+                   write_bytes_per_launch=write_b, bytes_per_launch=fetch_b + write_b,
+                   fetch_scale=f_scale, write_scale=w_scale, copy_bytes=COPY_BYTES,
+                   copy_fetch_kib=max(cf), copy_write_kib=max(cw))
+        json.dump(res, open(os.path.join(prof, "narrow_traffic.json"), "w"), indent=1)
+        md = ["# HBM traffic of k_narrow -- granular bed, 100k polyhedra", "",
+              f"Commands: `{command}` with `--pmc FETCH_SIZE` and `--pmc WRITE_SIZE` (separate runs)", "",
+              f"- dispatches: the last {len(nf)} (the timed steps)",
+              f"- FETCH_SIZE per dispatch: {sum(nf) / len(nf):.0f} KiB x scale {f_scale:.3f} = {fetch_b / 1e6:.1f} MB",
+              f"- WRITE_SIZE per dispatch: {sum(nwr) / len(nwr):.0f} KiB x scale {w_scale:.3f} = {write_b / 1e6:.1f} MB",
+              f"- scale = bytes of a plain {COPY_BYTES >> 20} MiB device copy / what the counter reads for it "
+              f"(fetch {max(cf):.0f} KiB, write {max(cw):.0f} KiB)",
+              f"- traffic per launch: {(fetch_b + write_b) / 1e6:.1f} MB"]
+        open(os.path.join(prof, "narrow_traffic_granular_bed_100k.md"), "w").write("\n".join(md) + "\n")
+    print(json.dumps(res))
+
+
 def run_steps(sim, k, mode):
     import zdx
 
@@ -152,10 +261,20 @@ def main():
     ap.add_argument("--particles", type=int, default=100_000)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tiles", default="", help="narrow-phase tiles 'block*100+items,groups_per_cu' (no autotune)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=4000)
     ap.add_argument("--cpu-timeout", type=float, default=120.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain device copy of known size (for scaling hardware byte counters)")
+    ap.add_argument("--summarize-profiles", nargs=3, metavar=("STATS_DIR", "FETCH_DIR", "WRITE_DIR"),
+                    help="write profiles/ summaries from rocprofv3 --stats / --pmc FETCH_SIZE / WRITE_SIZE "
+                         "output directories of this bench, then exit")
+    ap.add_argument("--profile-command", default="", help="command line recorded in the summaries")
     args = ap.parse_args()
+    if args.summarize_profiles:
+        summarize_profiles(*args.summarize_profiles, command=args.profile_command)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the step runs on a single MI355X and no "
                                    "collective or multi-GPU path exists; run with --gpus 1"}))
@@ -165,6 +284,8 @@ def main():
     from zdx import workloads as W
 
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
+    if args.copy_calibration:
+        copy_calibration()
     t0 = time.time()
     wl = W.granular_bed(n=args.particles)
     ctx = zdx.Context(0)
@@ -172,10 +293,13 @@ def main():
     log(f"setup {time.time() - t0:.1f}s: {wl.n} particles, {n_meshes} meshes")
 
     # warmup (untimed): half, autotune the narrow-phase tiles, the rest
+    tiles = None
+    if args.tiles:
+        tiles = [int(x) for x in args.tiles.split(",")]
+        sim.set_tiles(tiles[0], tiles[1])
     half = args.warmup // 2
     sim.step(half, zdx.MODE_BATCHED)
-    tiles = None
-    if not args.no_autotune:
+    if not args.tiles and not args.no_autotune:
         tiles = sim.autotune()
         log(f"autotune: block*100+items={tiles[0]} groups/CU={tiles[1]}")
     sim.step(args.warmup - half, zdx.MODE_BATCHED)
