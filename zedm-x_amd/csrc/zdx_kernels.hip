@@ -529,7 +529,10 @@ __device__ inline void load_tri(const double* wv, const int32_t* ti, int t, V3* 
 template <int BS, int K, bool PROV>
 __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, int32_t* list_i, float* list_f,
                                                double* stage, int32_t stage_cap, int32_t tmax, int32_t use_queue,
-                                               unsigned int* queue, unsigned long long* kt) {
+                                               unsigned int* queue, int32_t lds_b, unsigned long long* kt) {
+    // lds_b: body B's culled boxes and triangle ids (28 bytes per triangle,
+    // read for every item of the sweep) live in dynamic LDS
+    extern __shared__ char s_nb[];
     kt_start(kt);
     __shared__ int s_wave[BS / WAVE];
     __shared__ int s_item;
@@ -538,9 +541,9 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
     __shared__ unsigned long long s_base;
     const int tid = threadIdx.x;
     int32_t* LA = list_i + (int64_t)blockIdx.x * 2 * tmax;
-    int32_t* LB = LA + tmax;
+    int32_t* LB = lds_b ? (int32_t*)(s_nb + 24 * (int64_t)tmax) : LA + tmax;
     float* FA = list_f + (int64_t)blockIdx.x * 12 * tmax;
-    float* FB = FA + 6 * tmax;
+    float* FB = lds_b ? (float*)s_nb : FA + 6 * tmax;
     double* ST = stage + (int64_t)blockIdx.x * (PROV ? 8 : 6) * stage_cap;
     int32_t* STP = (int32_t*)(ST + 6 * (int64_t)stage_cap);   // PROV: 4 ints per staged segment
     const int npairs = *P.n;
@@ -722,24 +725,47 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
 
 // LPT order: pairs by descending cost bucket floor(log2(TA * TB)), so the
 // persistent queue hands out the expensive pairs first
-__global__ void k_lpt_count(DPairs P, DMeshes M, const int32_t* body_mesh, int32_t* bucket_cnt) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= *P.n) return;
-    int64_t c = (int64_t)M.ntris[body_mesh[P.pi[p]]] * M.ntris[body_mesh[P.pj[p]]];
-    int b = 63 - __clzll((unsigned long long)(c | 1));
-    atomicAdd(&bucket_cnt[63 - b], 1);
+__device__ inline int lpt_bucket(const DPairs& P, const DMeshes& M, const int32_t* body_mesh, int p) {
+    const int64_t c = (int64_t)M.ntris[body_mesh[P.pi[p]]] * M.ntris[body_mesh[P.pj[p]]];
+    return __clzll((unsigned long long)(c | 1));   // bucket = leading zeros: descending cost
+}
+// per-workgroup histograms first, one global atomic per (workgroup, bucket)
+__global__ void __launch_bounds__(256) k_lpt_count(DPairs P, DMeshes M, const int32_t* body_mesh,
+                                                   int32_t* bucket_cnt) {
+    __shared__ int h[64];
+    if (threadIdx.x < 64) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int n = *P.n;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
+        atomicAdd(&h[lpt_bucket(P, M, body_mesh, p)], 1);
+    __syncthreads();
+    if (threadIdx.x < 64 && h[threadIdx.x]) atomicAdd(&bucket_cnt[threadIdx.x], h[threadIdx.x]);
 }
 __global__ void k_lpt_scan(int32_t* bucket_cnt, int32_t* bucket_cur) {
     if (threadIdx.x != 0) return;
     int acc = 0;
     for (int b = 0; b < 64; ++b) { int c = bucket_cnt[b]; bucket_cur[b] = acc; acc += c; }
 }
-__global__ void k_lpt_scatter(DPairs P, DMeshes M, const int32_t* body_mesh, int32_t* bucket_cur, int32_t* order) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= *P.n) return;
-    int64_t c = (int64_t)M.ntris[body_mesh[P.pi[p]]] * M.ntris[body_mesh[P.pj[p]]];
-    int b = 63 - __clzll((unsigned long long)(c | 1));
-    order[atomicAdd(&bucket_cur[63 - b], 1)] = p;
+// order within a bucket is free (any order gives the same per-pair results)
+__global__ void __launch_bounds__(256) k_lpt_scatter(DPairs P, DMeshes M, const int32_t* body_mesh,
+                                                     int32_t* bucket_cur, int32_t* order) {
+    __shared__ int h[64], base[64];
+    const int n = *P.n;
+    for (int p0 = blockIdx.x * blockDim.x; p0 < n; p0 += gridDim.x * blockDim.x) {
+        if (threadIdx.x < 64) h[threadIdx.x] = 0;
+        __syncthreads();
+        const int p = p0 + threadIdx.x;
+        int b = 0, r = 0;
+        if (p < n) {
+            b = lpt_bucket(P, M, body_mesh, p);
+            r = atomicAdd(&h[b], 1);
+        }
+        __syncthreads();
+        if (threadIdx.x < 64 && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&bucket_cur[threadIdx.x], h[threadIdx.x]);
+        __syncthreads();
+        if (p < n) order[base[b] + r] = p;
+        __syncthreads();
+    }
 }
 
 // ------------------------------------------------------------ topology
@@ -1434,12 +1460,15 @@ void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s, unsigned long lo
 template <int BS, int K>
 static void narrow_t(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
                      double* st, int32_t sc, int32_t tmax, unsigned int* q, hipStream_t s, unsigned long long* kt) {
+    const size_t lds = 28 * (size_t)tmax;
+    const int use = lds <= 48 * 1024 ? 1 : 0;
+    const size_t dyn = use ? lds : 0;
     if (P.prov)
-        hipLaunchKernelGGL((k_narrow<BS, K, true>), dim3(cfg.groups), dim3(BS), 0, s, B, M, P, li, lf, st, sc, tmax,
-                           cfg.use_queue, q, kt);
+        hipLaunchKernelGGL((k_narrow<BS, K, true>), dim3(cfg.groups), dim3(BS), dyn, s, B, M, P, li, lf, st, sc, tmax,
+                           cfg.use_queue, q, use, kt);
     else
-        hipLaunchKernelGGL((k_narrow<BS, K, false>), dim3(cfg.groups), dim3(BS), 0, s, B, M, P, li, lf, st, sc, tmax,
-                           cfg.use_queue, q, kt);
+        hipLaunchKernelGGL((k_narrow<BS, K, false>), dim3(cfg.groups), dim3(BS), dyn, s, B, M, P, li, lf, st, sc, tmax,
+                           cfg.use_queue, q, use, kt);
 }
 
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
@@ -1465,9 +1494,10 @@ void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const Na
 void launch_lpt_order(const DPairs& P, const DMeshes& M, const int32_t* body_mesh, int32_t* order, int32_t* bcnt,
                       int32_t* bcur, int32_t pair_cap, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, bcnt, 0, nullptr, (int64_t)64);
-    hipLaunchKernelGGL(k_lpt_count, dim3(nblk(pair_cap, 256)), dim3(256), 0, s, P, M, body_mesh, bcnt);
+    const int g = std::max(1, std::min(nblk(pair_cap, 256), 512));
+    hipLaunchKernelGGL(k_lpt_count, dim3(g), dim3(256), 0, s, P, M, body_mesh, bcnt);
     hipLaunchKernelGGL(k_lpt_scan, dim3(1), dim3(64), 0, s, bcnt, bcur);
-    hipLaunchKernelGGL(k_lpt_scatter, dim3(nblk(pair_cap, 256)), dim3(256), 0, s, P, M, body_mesh, bcur, order);
+    hipLaunchKernelGGL(k_lpt_scatter, dim3(g), dim3(256), 0, s, P, M, body_mesh, bcur, order);
 }
 
 void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, int32_t pair_cap, hipStream_t s) {
