@@ -296,7 +296,7 @@ struct zdx_sim {
     DBuf<int32_t> d_counts;
     // broad phase
     DBuf<DGrid> d_grid;
-    DBuf<double> d_bp_part;
+    DBuf<double> d_bp_part, d_spos;
     int32_t max_cells = 1 << 24;
     DBuf<int32_t> d_cell_of, d_cell_count, d_cell_start, d_cell_cursor, d_sorted, d_cnt_up, d_cnt_all;
     DBuf<int32_t> d_pair_off, d_adj_off, d_adj_j, d_adj_pair, d_n_adj, d_scan_tmp;
@@ -388,7 +388,7 @@ struct zdx_sim {
     BroadPhaseBufs bpbufs(int par) {
         BroadPhaseBufs b;
         b.n = N; b.state = d_state[par].p; b.radius = d_radius.p; b.grid = d_grid.p; b.max_cells = max_cells;
-        b.part = d_bp_part.p;
+        b.part = d_bp_part.p; b.spos = d_spos.p;
         b.cell_of = d_cell_of.p; b.cell_count = d_cell_count.p; b.cell_start = d_cell_start.p;
         b.cell_cursor = d_cell_cursor.p; b.sorted = d_sorted.p; b.cnt_up = d_cnt_up.p; b.cnt_all = d_cnt_all.p;
         b.pair_off = d_pair_off.p; b.adj_off = d_adj_off.p; b.pi = ps[par].pi.p; b.pj = ps[par].pj.p;
@@ -535,6 +535,7 @@ void sim_setup_device(zdx_sim* s) {
     // broad phase
     s->d_grid.ensure(1);
     s->d_bp_part.ensure(7 * (size_t)BP_PARTS);
+    s->d_spos.ensure(4 * (size_t)N);
     int64_t cells = std::min<int64_t>((int64_t)1 << 24, std::max<int64_t>(64, 8 * (int64_t)N));
     s->max_cells = (int32_t)cells;
     s->d_cell_of.ensure(N); s->d_cell_count.ensure(cells); s->d_cell_start.ensure(cells + 1);

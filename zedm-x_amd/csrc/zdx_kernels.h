@@ -27,6 +27,7 @@ struct BroadPhaseBufs {
     int32_t* cell_start;       // max_cells + 1
     int32_t* cell_cursor;      // max_cells
     int32_t* sorted;           // per particle
+    double* spos;              // per cell-order slot: x, y, z, radius
     int32_t* cnt_up;           // per particle
     int32_t* cnt_all;          // per particle
     int32_t* pair_off;         // n + 1

@@ -353,23 +353,21 @@ __global__ void __launch_bounds__(256) k_bp_scatter(BroadPhaseBufs bp) {
     int c = bp.cell_of[i];
     int slot = bp.cell_start[c] + atomicAdd(&bp.cell_cursor[c], 1);
     bp.sorted[slot] = i;
+    // position and radius in cell order, for the neighbour scans
+    const double* p = bp.state + 16 * (int64_t)i;
+    double* o = bp.spos + 4 * (int64_t)slot;
+    o[0] = p[0]; o[1] = p[1]; o[2] = p[2]; o[3] = bp.radius[i];
 }
 
-// sphere test of the reference (main.cpp:2054-2059) for the pair (min, max)
-__device__ inline bool sphere_pass(const BroadPhaseBufs& bp, int i, int j) {
-    int a = i < j ? i : j, b = i < j ? j : i;
-    const double* pa = bp.state + 16 * (int64_t)a;
-    const double* pb = bp.state + 16 * (int64_t)b;
-    V3 dp = sub(ld3(pb), ld3(pa));
-    double d2 = dot(dp, dp);
-    double rs = bp.radius[a] + bp.radius[b];
-    return !(d2 > rs * rs);
-}
-
+// Neighbours j of particle i (at cell-order slot si) that pass the
+// reference's sphere test (main.cpp:2054-2059), evaluated for the pair
+// (min(i,j), max(i,j)) on the same doubles: f(j) in cell-scan order.
 template <class F>
-__device__ inline void for_neighbours(const BroadPhaseBufs& bp, const DGrid& G, int i, F&& f) {
+__device__ inline void for_neighbours(const BroadPhaseBufs& bp, const DGrid& G, int i, int si, F&& f) {
+    const double* qi = bp.spos + 4 * (int64_t)si;
+    const double xi = qi[0], yi = qi[1], zi = qi[2], ri = qi[3];
     int cx, cy, cz;
-    cell_coords(G, bp.state + 16 * (int64_t)i, cx, cy, cz);
+    cell_coords(G, qi, cx, cy, cz);
     for (int dz = -1; dz <= 1; ++dz) {
         int z = cz + dz;
         if (z < 0 || z >= G.nz) continue;
@@ -382,49 +380,53 @@ __device__ inline void for_neighbours(const BroadPhaseBufs& bp, const DGrid& G, 
                 int c = (z * G.ny + y) * G.nx + x;
                 int s0 = bp.cell_start[c], s1 = bp.cell_start[c + 1];
                 for (int k = s0; k < s1; ++k) {
-                    int j = bp.sorted[k];
-                    if (j != i && sphere_pass(bp, i, j)) f(j);
+                    const int j = bp.sorted[k];
+                    if (j == i) continue;
+                    const double* qj = bp.spos + 4 * (int64_t)k;
+                    const bool iflow = i < j;
+                    const V3 pa = iflow ? v3(xi, yi, zi) : v3(qj[0], qj[1], qj[2]);
+                    const V3 pb = iflow ? v3(qj[0], qj[1], qj[2]) : v3(xi, yi, zi);
+                    const V3 dp = sub(pb, pa);
+                    const double d2 = dot(dp, dp);
+                    const double rs = iflow ? ri + qj[3] : qj[3] + ri;
+                    if (!(d2 > rs * rs)) f(j);
                 }
             }
         }
     }
 }
 
+// threads take particles in cell order (neighbouring threads scan the same cells)
 __global__ void __launch_bounds__(256) k_bp_count(BroadPhaseBufs bp) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= bp.n) return;
+    const int si = blockIdx.x * blockDim.x + threadIdx.x;
+    if (si >= bp.n) return;
+    const int i = bp.sorted[si];
     const DGrid G = *bp.grid;
     int up = 0, all = 0;
-    for_neighbours(bp, G, i, [&](int j) { all++; if (j > i) up++; });
+    for_neighbours(bp, G, i, si, [&](int j) { all++; if (j > i) up++; });
     if (all > NEIGH_CAP) { atomicOr(bp.status, (int)ST_NEIGH); all = NEIGH_CAP; up = up > NEIGH_CAP ? NEIGH_CAP : up; }
     bp.cnt_up[i] = up;
     bp.cnt_all[i] = all;
 }
 
+constexpr int NEIGH_REG = 16;   // neighbour lists up to this length are sorted in registers
+
 __global__ void __launch_bounds__(128) k_bp_fill(BroadPhaseBufs bp) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= bp.n) return;
+    const int si = blockIdx.x * blockDim.x + threadIdx.x;
+    if (si >= bp.n) return;
+    const int i = bp.sorted[si];
     const DGrid G = *bp.grid;
-    int js[NEIGH_CAP];
-    int n = 0;
-    for_neighbours(bp, G, i, [&](int j) { if (n < NEIGH_CAP) js[n++] = j; });
-    for (int a = 1; a < n; ++a) {
-        int v = js[a], b = a - 1;
-        while (b >= 0 && js[b] > v) { js[b + 1] = js[b]; --b; }
-        js[b + 1] = v;
-    }
-    const int ao = bp.adj_off[i];
-    const int po = bp.pair_off[i];
     if (bp.adj_off[bp.n] > bp.adj_cap || bp.pair_off[bp.n] > bp.pair_cap) {
         atomicOr(bp.status, (int)ST_PAIRS);
         return;
     }
-    int u = 0;
-    for (int k = 0; k < n; ++k) {
-        int j = js[k];
+    const int ao = bp.adj_off[i];
+    const int po = bp.pair_off[i];
+    const int cnt = bp.cnt_all[i];
+    auto emit = [&](int k, int j, int& u) {
         bp.adj_j[ao + k] = j;
         if (j > i) {
-            int p = po + u;
+            const int p = po + u;
             bp.pi[p] = i;
             bp.pj[p] = j;
             bp.pkey[p] = ((int64_t)i << 32) | (int64_t)(uint32_t)j;
@@ -433,7 +435,38 @@ __global__ void __launch_bounds__(128) k_bp_fill(BroadPhaseBufs bp) {
         } else {
             bp.adj_pair[ao + k] = -1;
         }
+    };
+    int u = 0;
+    if (cnt <= NEIGH_REG) {
+        // ascending partner order: insertion into a sorted register array
+        int r[NEIGH_REG];
+        int n = 0;
+        for_neighbours(bp, G, i, si, [&](int j) {
+            int x = j;
+#pragma unroll
+            for (int q = 0; q < NEIGH_REG; ++q) {
+                if (q < n) {
+                    if (x < r[q]) { const int t = r[q]; r[q] = x; x = t; }
+                } else if (q == n) {
+                    r[q] = x;
+                }
+            }
+            ++n;
+        });
+#pragma unroll
+        for (int q = 0; q < NEIGH_REG; ++q)
+            if (q < n) emit(q, r[q], u);
+        return;
     }
+    int js[NEIGH_CAP];
+    int n = 0;
+    for_neighbours(bp, G, i, si, [&](int j) { if (n < NEIGH_CAP) js[n++] = j; });
+    for (int a = 1; a < n; ++a) {
+        int v = js[a], b = a - 1;
+        while (b >= 0 && js[b] > v) { js[b + 1] = js[b]; --b; }
+        js[b + 1] = v;
+    }
+    for (int k = 0; k < n; ++k) emit(k, js[k], u);
 }
 
 __global__ void __launch_bounds__(256) k_bp_adjfix(BroadPhaseBufs bp) {
