@@ -836,6 +836,17 @@ __device__ inline Mat load_mat(const DMat& m, int i) {
 // zdx_walk_profile().
 __device__ unsigned long long g_walk_prof[16];
 #endif
+// Wave-cooperative copy of n elements with four loads in flight per lane.
+template <class T>
+__device__ inline void wave_copy(T* dst, const T* src, int n) {
+    int k = lane_id();
+    for (; k + 3 * WAVE < n; k += 4 * WAVE) {
+        const T a0 = src[k], a1 = src[k + WAVE], a2 = src[k + 2 * WAVE], a3 = src[k + 3 * WAVE];
+        dst[k] = a0; dst[k + WAVE] = a1; dst[k + 2 * WAVE] = a2; dst[k + 3 * WAVE] = a3;
+    }
+    for (; k < n; k += WAVE) dst[k] = src[k];
+}
+
 // Lane policy of the walks: one wave, all lanes on the same serial path.
 struct LaneDev {
     static constexpr int W = WAVE;
@@ -881,22 +892,28 @@ struct LaneDev {
 // (persistent); walk kernels: one wave per pair, scratch in LDS when it fits,
 // else the workgroup's slice of the global scratch.
 
-constexpr int TP_PASS_A_SEGS = 384;   // segments pass A stages in LDS (84 bytes each)
+constexpr int TP_PASS_A_SEGS = 384;     // segments pass A stages in LDS (84 bytes each)
+constexpr int TP_PASS_A_KEYS = 32768;   // LDS bytes for the S x S key matrix (S <= 181)
+constexpr int TP_PASS_A_THREADS = 1024;
 
 // Pass A: representative-point cells, the adjacency rows (per segment the
-// count and up to R accepted candidates, one thread per row) and the snap hit
-// words of every pair; each pair's rows and snap words are bump-allocated (a
-// pair that does not fit gets offset -1 and the step is retried with larger
-// buffers: the counters keep the total needed, top[3] the longest row).
-__global__ void __launch_bounds__(256) k_tp_pass_a(DPairs P, DMeshes M, DBodies B, TopoBufs T, int32_t lds_segs,
-                                                  unsigned long long* kt) {
-    // the pair's segments (6 doubles) and cells (9 ints) are staged in LDS
-    // when it holds them (lds_segs segments), else read from the arena
+// count and up to R accepted candidates in ascending j) and the snap hit words
+// of every pair, one 1024-thread workgroup per pair.  Each pair's rows and
+// snap words are bump-allocated (a pair that does not fit gets offset -1 and
+// the step is retried with larger buffers: the counters keep the total
+// needed, top[3] the longest row).  The segments and cells are staged in LDS;
+// when S x S fits, all candidate tests run flat into an LDS key matrix that
+// one thread per row then compacts; the snap tests run flat, each thread over
+// a contiguous range of endpoint pairs.
+__global__ void __launch_bounds__(TP_PASS_A_THREADS) k_tp_pass_a(DPairs P, DMeshes M, DBodies B, TopoBufs T,
+                                                                int32_t lds_segs, unsigned long long* kt) {
     extern __shared__ char s_pa[];
     __shared__ long long s_off[2];
     kt_start(kt);
     const int n = *P.topo_n;
     const int R = T.adj_row;
+    const int NT = blockDim.x;
+    uint8_t* keys = (uint8_t*)(s_pa + 84 * (int64_t)lds_segs);
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
         const int p = P.topo_list[it];
         const int S = P.nseg[p];
@@ -920,35 +937,55 @@ __global__ void __launch_bounds__(256) k_tp_pass_a(DPairs P, DMeshes M, DBodies 
             T.snap_off[it] = so;
         }
         if (staged)
-            for (int k = threadIdx.x; k < 6 * S; k += blockDim.x) seg[k] = gseg[k];
+            for (int k = threadIdx.x; k < 6 * S; k += NT) seg[k] = gseg[k];
         __syncthreads();
-        for (int k = threadIdx.x; k < S; k += blockDim.x) tp_cells(seg, k, h, c9 + 9 * k);
-        __syncthreads();
+        for (int k = threadIdx.x; k < S; k += NT) tp_cells(seg, k, h, c9 + 9 * k);
         const long long ao = s_off[0], so = s_off[1];
+        uint64_t* w = T.snapw + so;
+        if (ao >= 0)
+            for (int k = threadIdx.x; k < nw; k += NT) w[k] = 0ull;
+        __syncthreads();
         if (ao >= 0) {
             int* cntp = T.adj + ao;
             int* rows = cntp + S;
-            for (int i = threadIdx.x; i < S; i += blockDim.x) {
-                const int m = tp_adj_row(seg, c9, h, S, i, rows + (int64_t)i * R, R);
-                cntp[i] = m;
-                if (m > R) atomicMax(T.top + 3, (unsigned long long)m);
+            if (S * S <= TP_PASS_A_KEYS) {
+                for (int k = threadIdx.x; k < S * S; k += NT) {
+                    const int i = k / S, j = k - (k / S) * S;
+                    keys[k] = (uint8_t)tp_adj_key(seg, c9, h, i, j);
+                }
+                __syncthreads();
+                for (int i = threadIdx.x; i < S; i += NT) {
+                    const uint8_t* kr = keys + i * S;
+                    int* out = rows + (int64_t)i * R;
+                    int m = 0;
+                    for (int j = 0; j < S; ++j) {
+                        const int key = kr[j];
+                        if (key == TP_NOADJ) continue;
+                        if (m < R) out[m] = j | (key << 24);
+                        ++m;
+                    }
+                    cntp[i] = m;
+                    if (m > R) atomicMax(T.top + 3, (unsigned long long)m);
+                }
+            } else {
+                for (int i = threadIdx.x; i < S; i += NT) {
+                    const int m = tp_adj_row(seg, c9, h, S, i, rows + (int64_t)i * R, R);
+                    cntp[i] = m;
+                    if (m > R) atomicMax(T.top + 3, (unsigned long long)m);
+                }
             }
-            // snap hits row by row (row a: pairs (a, b), b < a; a thread takes
-            // rows a and n-1-a), set in the zeroed words with atomic OR
-            uint64_t* w = T.snapw + so;
-            for (int k = threadIdx.x; k < nw; k += blockDim.x) w[k] = 0ull;
-            __syncthreads();
-            const int np = 2 * S;
-            for (int r = threadIdx.x; r < (np + 1) / 2; r += blockDim.x) {
-                for (int side = 0; side < 2; ++side) {
-                    const int a = side ? np - 1 - r : r;
-                    if (side && a == r) break;
-                    const int t0 = a * (a - 1) / 2;
-                    for (int b = 0; b < a; ++b)
-                        if (tp_snap_hit(seg, c9, h, a, b)) {
-                            const int t = t0 + b;
-                            atomicOr((unsigned long long*)(w + (t >> 6)), 1ull << (t & 63));
-                        }
+            // snap hits: each thread a contiguous range of pair numbers t
+            // (a(a-1)/2 + b), set in the zeroed words with atomic OR
+            const int per = (npt + NT - 1) / NT;
+            int t = threadIdx.x * per;
+            const int t1 = t + per < npt ? t + per : npt;
+            if (t < t1) {
+                int a, b;
+                tp_pair_of(t, a, b);
+                for (; t < t1; ++t) {
+                    if (tp_snap_hit(seg, c9, h, a, b))
+                        atomicOr((unsigned long long*)(w + (t >> 6)), 1ull << (t & 63));
+                    if (++b == a) { ++a; b = 0; }
                 }
             }
         }
@@ -1020,7 +1057,7 @@ __global__ void __launch_bounds__(64) k_tp_walk_b(DPairs P, DMeshes M, DBodies B
                                       [&](int* I, double*) {
                 int* adj = I + tp_b_ints(S);   // counts (S) then rows (S x R)
                 const int* src = T.adj + ao;
-                for (int k = lane_id(); k < S * (R + 1); k += WAVE) adj[k] = src[k];
+                wave_copy(adj, src, S * (R + 1));
                 __syncthreads();
                 wp.mark(0);
                 int ok = 1;
@@ -1088,8 +1125,8 @@ __device__ inline TopoOut tp_walk_d(const LaneDev& wp, const TopoBufs& T, int S,
         const int* comp = T.comp + TPC_N * sb;
         const int ne_all = comp[TPC_N * (ncomp - 1) + TPC_EB] + comp[TPC_N * (ncomp - 1) + TPC_NE];
         const int nv_all = comp[TPC_N * (ncomp - 1) + TPC_VB] + comp[TPC_N * (ncomp - 1) + TPC_NV];
-        for (int k = lane_id(); k < 2 * ne_all; k += WAVE) El[k] = T.E[2 * sb + k];
-        for (int k = lane_id(); k < 3 * nv_all; k += WAVE) vcl[k] = T.vcell[6 * sb + k];
+        wave_copy(El, T.E + 2 * sb, 2 * ne_all);
+        wave_copy(vcl, T.vcell + 6 * sb, 3 * nv_all);
         __syncthreads();
         return tp_loops(wp, S, ncomp, h, area_eps, dir, comp, T.V + 6 * sb, vcl, El, T.tjw + to, I, D, sink,
                         T.vj ? T.vj + 72 * sb : nullptr, D + tp_d_dbls(S));
@@ -1543,8 +1580,8 @@ void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, i
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
                      const TopoBufs& T, const SimParams& sp, hipStream_t s, unsigned long long* kt) {
     launch_fill_u64(T.top, 0ull, 4, s);
-    hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(256), 84 * TP_PASS_A_SEGS, s, P, M, B, T,
-                       (int32_t)TP_PASS_A_SEGS, kt);
+    hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(TP_PASS_A_THREADS),
+                       84 * TP_PASS_A_SEGS + TP_PASS_A_KEYS, s, P, M, B, T, (int32_t)TP_PASS_A_SEGS, kt);
     hipLaunchKernelGGL(k_tp_walk_b, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, T, sp.split,
                        T.lds_bytes);
     hipLaunchKernelGGL(k_tp_pass_c, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T);
@@ -1555,8 +1592,8 @@ void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const 
                      hipStream_t s) {
     launch_fill_u64(T.top, 0ull, 4, s);
     if (T.ej) hipLaunchKernelGGL(k_tp_endpoint_jac, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T);
-    hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(256), 84 * TP_PASS_A_SEGS, s, P, M, B, T,
-                       (int32_t)TP_PASS_A_SEGS, nullptr);
+    hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(TP_PASS_A_THREADS),
+                       84 * TP_PASS_A_SEGS + TP_PASS_A_KEYS, s, P, M, B, T, (int32_t)TP_PASS_A_SEGS, nullptr);
     hipLaunchKernelGGL(k_tp_walk_b, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, T, split,
                        T.lds_bytes);
     hipLaunchKernelGGL(k_tp_pass_c, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T);
