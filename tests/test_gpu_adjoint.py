@@ -170,3 +170,27 @@ def test_minimal_model_gradients_match_finite_differences(ctx):
         for name, g, fn, hstep in checks:
             fd = (float(fn(1)) - float(fn(-1))) / (2 * hstep)
             assert abs(g - fd) <= 1e-4 * max(abs(fd), abs(g)) + 1e-12, (name, g, fd)
+
+
+def test_calibration_reduces_loss(ctx):
+    """Drop-on-plane calibration (16 hulls of 128 points, 300 steps): the
+    positions of a run with (k*, e*, v0*) are the target; Adam on (log k,
+    logit e, v0) with adjoint gradients from a perturbed start lowers the loss."""
+    from zdx import workloads as W
+    from zdx.autograd import calibrate, scene_from_workload, simulate
+
+    wl = W.drop_on_plane(16)
+    scene, pos0, quat0, vel0 = scene_from_workload(ctx, wl)
+    n = pos0.shape[0]
+    k_true, e_true = torch.tensor(4e6, dtype=torch.float64), torch.tensor(0.6, dtype=torch.float64)
+    v_true = vel0.clone()
+    v_true[:, 2] = -0.3
+    with torch.no_grad():
+        target, _, _, _ = simulate(scene, k_true, e_true, pos0, quat0, v_true, torch.zeros(n, 3, dtype=torch.float64),
+                                   300, 50)
+    v_start = v_true.clone()
+    v_start[:, 2] = -0.25
+    hist, k_fit, e_fit, v_fit = calibrate(scene, pos0, quat0, target, 2e6, 0.4, v_start, 300, iters=5, lr=0.05)
+    losses = [h[0] for h in hist]
+    assert losses[0] > 0.0
+    assert min(losses[1:]) < 0.5 * losses[0], losses

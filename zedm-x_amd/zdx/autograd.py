@@ -218,3 +218,66 @@ def simulate(scene: Scene, k, e, pos, quat, vel, L, steps: int, ckpt_every: int 
             pos, quat, vel, L = seg(k, e, pos, quat, vel, L)
         s += m
     return pos, quat, vel, L
+
+
+def scene_from_workload(zctx, wl, floor_thickness: float = 0.02):
+    """Scene of a drop-on-plane workload (zdx.workloads.drop_on_plane): the
+    particles' meshes (one per template and scale), their mass properties, and
+    a floor slab whose top face is the workload's plane wall (z = 0).
+    Returns (scene, pos0, quat0, vel0) as float64 tensors."""
+    from . import synth
+
+    dens = float(wl.material[0, 0])
+    mesh = np.zeros(wl.n, dtype=np.int64)
+    rad = np.zeros(wl.n)
+    mass = np.zeros(wl.n)
+    Ibinv = np.zeros((wl.n, 3, 3))
+    req = np.zeros(wl.n)
+    cache = {}
+    for i in range(wl.n):
+        key = (int(wl.tpl[i]), float(wl.scale[i]))
+        if key not in cache:
+            tris = wl.templates[key[0]] * key[1]
+            mid = zctx.register_mesh(tris)
+            cache[key] = (mid, zctx.mesh_props[mid])   # volume, centroid(3), radius, mean edge, bbox, inertia(9)
+        mid, props = cache[key]
+        V, I = float(props[0]), props[7:16].reshape(3, 3) * dens
+        mesh[i] = mid
+        rad[i] = float(props[4])
+        mass[i] = dens * V
+        Ibinv[i] = np.linalg.inv(I)
+        req[i] = (3.0 * V / (4.0 * math.pi)) ** (1.0 / 3.0)
+    tris, wpos, _, _, _, _ = wl.walls[0]
+    ext = np.ptp(tris.reshape(-1, 3), axis=0)
+    floor = synth.box_wall(float(ext[0]), float(ext[1]), floor_thickness)
+    fid = zctx.register_mesh(floor)
+    t = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64))
+    scene = Scene(zctx=zctx, mesh=mesh, radius=rad, mass=t(mass), Ibinv=t(Ibinv), req=t(req), floor_mesh=fid,
+                  floor_pos=np.array([wpos[0], wpos[1], wpos[2] - 0.5 * floor_thickness]), floor_top=float(wpos[2]),
+                  gravity=t(wl.gravity), dt=float(wl.dt), split=bool(wl.split))
+    return scene, t(wl.init[:, 0:3]), t(wl.init[:, 3:7]), t(wl.init[:, 7:10])
+
+
+def calibrate(scene, pos0, quat0, target, k0, e0, v0, steps, iters=10, lr=0.05, ckpt_every=50, log=None):
+    """Fits the stiffness k, the restitution e and the initial velocities v0 so
+    that the positions after `steps` steps match `target` (loss = sum of
+    squared distances), by Adam on (log k, logit e, v0) with gradients from
+    the adjoint.  Returns the history [(loss, k, e)] and the fitted values."""
+    lk = torch.tensor(math.log(float(k0)), dtype=torch.float64, requires_grad=True)
+    le = torch.tensor(math.log(float(e0) / (1.0 - float(e0))), dtype=torch.float64, requires_grad=True)
+    v = v0.clone().detach().requires_grad_(True)
+    opt = torch.optim.Adam([lk, le, v], lr=lr)
+    n = pos0.shape[0]
+    hist = []
+    for _ in range(iters):
+        opt.zero_grad()
+        k = lk.exp()
+        e = torch.sigmoid(le)
+        p, _, _, _ = simulate(scene, k, e, pos0, quat0, v, torch.zeros(n, 3, dtype=torch.float64), steps, ckpt_every)
+        loss = ((p - target) ** 2).sum()
+        loss.backward()
+        hist.append((float(loss), float(k), float(e)))
+        if log:
+            log(hist[-1])
+        opt.step()
+    return hist, float(lk.exp()), float(torch.sigmoid(le)), v.detach()
