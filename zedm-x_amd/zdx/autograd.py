@@ -10,13 +10,16 @@ on the device contact pipeline, for calibration.
   endpoint an edge of one body crossing a triangle plane of the other), and
   the chain to Sc, xc0 and to positions/quaternions is done here.  The
   loop topology is held fixed, as a derivative must.
-* ``MinimalModel``: forces from loops -- elastic ``F_B = k/2 |Sn| n`` (the
+* ``step``: forces from loops -- elastic ``F_B = k/2 |Sn| n`` (the
   energy identity f = -dPsi/dlambda n with Psi = k/2 Omega, dOmega/dlambda =
-  |Sn|) plus normal damping ``-c (v_rel . n) n`` with ``c = 2 zeta(e)
-  sqrt(m_eff k_c)``, ``zeta(e) = -ln e / sqrt(pi^2 + ln^2 e)`` and the contact
-  stiffness of a sphere of the equivalent radius R, ``k_c = pi k R`` -- and
-  the rigid-body update of ``integrate_particle`` (src/main.cpp:1494-1510,
-  old rotation for omega) in torch.
+  |Sn|, polyline.py:655-701) plus normal damping ``-c (v_rel . n) n`` with
+  ``c = 2 zeta(e) sqrt(m_eff k_c)``, ``zeta(e) = -ln e / sqrt(pi^2 + ln^2 e)``
+  and the contact stiffness of a sphere of the equivalent radius R,
+  ``k_c = pi k R``, applied at the loop centroid (``-3/2 xc0`` up to a
+  component along n; polyline.py's minimal model has no torques, and the
+  world-frame ``xc0`` itself lies far from the loop for bodies away from the
+  origin) -- and the rigid-body update of ``integrate_particle``
+  (src/main.cpp:1494-1510, old rotation for omega) in torch.
 * ``simulate``: K-step segments under ``torch.utils.checkpoint`` so memory is
   O(N steps / K) states, the contact calls of a segment being re-run in
   backward (they are deterministic).
@@ -162,7 +165,11 @@ def step(scene: Scene, k, e, pos, quat, vel, L):
             fb = isfloor[lp]
             area = Sc.norm(dim=-1, keepdim=True)
             nrm = -Sc / area
-            # velocities of the material points at xc
+            # xc0 = (nA x Gn)/|Sn| = -2/3 of the loop centroid's part normal to n
+            # (Stokes on Gn); -3/2 xc0 is the centroid up to a multiple of n,
+            # which neither the torque of a normal force nor v_rel . n sees.
+            xc = -1.5 * xc
+            # velocities of the material points at the contact point
             R_a = qmat(quat[a])
             Iinv_a = R_a @ scene.Ibinv[a] @ R_a.transpose(-1, -2)
             wa = (Iinv_a @ L[a].unsqueeze(-1)).squeeze(-1)
@@ -258,15 +265,21 @@ def scene_from_workload(zctx, wl, floor_thickness: float = 0.02):
     return scene, t(wl.init[:, 0:3]), t(wl.init[:, 3:7]), t(wl.init[:, 7:10])
 
 
-def calibrate(scene, pos0, quat0, target, k0, e0, v0, steps, iters=10, lr=0.05, ckpt_every=50, log=None):
+def calibrate(scene, pos0, quat0, target, k0, e0, v0, steps, iters=10, lr=0.05, lr_v=None, ckpt_every=50,
+              log=None):
     """Fits the stiffness k, the restitution e and the initial velocities v0 so
     that the positions after `steps` steps match `target` (loss = sum of
     squared distances), by Adam on (log k, logit e, v0) with gradients from
-    the adjoint.  Returns the history [(loss, k, e)] and the fitted values."""
+    the adjoint.  Adam's first steps move every coordinate by about its
+    learning rate whatever the gradient's size, so v0 gets its own rate
+    `lr_v` in m/s (default: `lr` times the mean speed of v0, at least 1e-3 m/s).
+    Returns the history [(loss, k, e)] and the fitted k, e and v0."""
     lk = torch.tensor(math.log(float(k0)), dtype=torch.float64, requires_grad=True)
     le = torch.tensor(math.log(float(e0) / (1.0 - float(e0))), dtype=torch.float64, requires_grad=True)
     v = v0.clone().detach().requires_grad_(True)
-    opt = torch.optim.Adam([lk, le, v], lr=lr)
+    if lr_v is None:
+        lr_v = lr * max(float(v0.norm(dim=-1).mean()), 1e-3 / lr)
+    opt = torch.optim.Adam([{"params": [lk, le], "lr": lr}, {"params": [v], "lr": lr_v}])
     n = pos0.shape[0]
     hist = []
     for _ in range(iters):
@@ -276,7 +289,7 @@ def calibrate(scene, pos0, quat0, target, k0, e0, v0, steps, iters=10, lr=0.05, 
         p, _, _, _ = simulate(scene, k, e, pos0, quat0, v, torch.zeros(n, 3, dtype=torch.float64), steps, ckpt_every)
         loss = ((p - target) ** 2).sum()
         loss.backward()
-        hist.append((float(loss), float(k), float(e)))
+        hist.append((float(loss.detach()), float(k.detach()), float(e.detach())))
         if log:
             log(hist[-1])
         opt.step()
