@@ -39,6 +39,7 @@ typedef struct zdx_sim zdx_sim;
 /* ---------------------------------------------------------------- context */
 /* Opens device `device` (HIP ordinal) and creates the context's stream. */
 zdx_status zdx_create(zdx_ctx** out, int32_t device);
+/* Destroy every simulation created on the context first. */
 void zdx_destroy(zdx_ctx* ctx);
 /* Message of the last failing call on this context (static storage per ctx). */
 const char* zdx_last_error(const zdx_ctx* ctx);
@@ -191,6 +192,15 @@ zdx_status zdx_sim_restore(zdx_sim* sim, const char* path);
  * walls, fold+integrate+counters; out[6]: steps; out[7..7+ZDX_N_COUNTERS):
  * counter sums (ZDX_CTR_SEG_MAX: the maximum).  Writes min(n, 7 + ZDX_N_COUNTERS) values. */
 zdx_status zdx_sim_stage_times(zdx_sim* sim, double* out, int32_t n);
+/* Particle-particle force model of the time step.  ZDX_FORCE_ZDEM (default):
+ * the reference's force, src/main.cpp:2220-2287.  ZDX_FORCE_MINIMAL: the
+ * minimal-energy force of polyline.py:655-701, Fn = k/2 |Sn| along the A->B
+ * normal, plus normal damping from the restitution (c = 2 zeta(e)
+ * sqrt(m_eff pi k R), R = R1 R2/(R1+R2)), applied at the loop centroid; no
+ * tangential force.  Walls keep the reference's wall force either way.  In a
+ * config file: "force_model minimal" and "minimal_k <k>". */
+enum { ZDX_FORCE_ZDEM = 0, ZDX_FORCE_MINIMAL = 1 };
+zdx_status zdx_sim_set_force_model(zdx_sim* sim, int32_t model, double k);
 /* Narrow-phase launch shape; 0 keeps the current value.  The autotuner
  * (zdx_sim_autotune) sets them from timed trial steps. */
 zdx_status zdx_sim_set_tiles(zdx_sim* sim, int32_t np_block, int32_t np_groups_per_cu);

@@ -60,6 +60,7 @@ def lib():
         L.orc_sim_step.argtypes = [C.c_void_p, C.c_int, _lp]
         L.orc_sim_dt.argtypes = [C.c_void_p]
         L.orc_sim_dt.restype = C.c_double
+        L.orc_sim_set_force_model.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.orc_tri_tri.argtypes = [_dp, _dp, _dp]
         L.orc_sn_gn.argtypes = [_dp, C.c_int, _dp, _dp]
         L.orc_xc0.argtypes = [_dp, _dp, _dp, _dp, _dp]
@@ -167,6 +168,10 @@ class Sim:
 
     def set_state(self, s: np.ndarray) -> None:
         self._L.orc_sim_set_state(self.h, _f64(s).reshape(-1))
+
+    def set_force_model(self, model: int, k: float = 0.0) -> None:
+        """0: the reference's particle-particle force; 1: minimal-energy force with stiffness k."""
+        self._L.orc_sim_set_force_model(self.h, int(model), float(k))
 
     def forces(self):
         F = np.zeros((self.n, 3))

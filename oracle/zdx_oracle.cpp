@@ -869,6 +869,8 @@ struct Config {
     std::string stl;
     int output_interval = 1;
     V3 v0, v1;
+    int force_model = 0;       // 1: minimal-energy force (build extension key, ignored by the reference)
+    double k_minimal = 0.0;
     std::vector<PInit> parts;
     std::vector<WInit> walls;
 };
@@ -929,6 +931,8 @@ bool parse_config(const std::string& path, Config& c) {
         else if (k == "split_contacts") { int v = 1; is >> v; c.split = (v != 0); }
         else if (k == "gravity") is >> c.g.x >> c.g.y >> c.g.z;
         else if (k == "center_mesh") { int v = 1; is >> v; c.center = (v != 0); }
+        else if (k == "force_model") { std::string v; is >> v; c.force_model = (v == "minimal") ? 1 : 0; }
+        else if (k == "minimal_k") is >> c.k_minimal;
         else if (k == "output_interval") is >> c.output_interval;
         else if (k == "v0") is >> c.v0.x >> c.v0.y >> c.v0.z;
         else if (k == "v1") is >> c.v1.x >> c.v1.y >> c.v1.z;
@@ -1022,6 +1026,32 @@ void sim_step_once(orc_sim& s, int64_t* ctr) {
             ncomp += po.comps;
             nloop += po.loops_raw;
             for (const auto& lo : po.loops) {
+                if (cfg.force_model == 1) {
+                    // Minimal-energy force (polyline.py:676-679 elastic part) with
+                    // restitution damping, at the loop centroid (-3/2 xc0 up to a
+                    // multiple of nA).  Independent of the reference's history.
+                    V3 xc = lo.xc * -1.5;
+                    V3 rA = xc - a.tf.pos, rB = xc - b.tf.pos;
+                    V3 vrel = (b.vel + cross(b.omega, rB)) - (a.vel + cross(a.omega, rA));
+                    double vn = dot(vrel, lo.nA);
+                    double R1 = std::max(a.eqr, 1e-12), R2 = std::max(b.eqr, 1e-12);
+                    double Reff = R1 * R2 / (R1 + R2);
+                    double meff = 1.0 / (a.inv_mass + b.inv_mass);
+                    double e = std::min(0.9999, std::max(1e-6, std::min(a.rest, b.rest)));
+                    double le = std::log(e);
+                    double zeta = -le / std::sqrt(3.141592653589793 * 3.141592653589793 + le * le);
+                    double c = 2.0 * zeta * std::sqrt(meff * (3.141592653589793 * cfg.k_minimal * Reff));
+                    V3 f = lo.nA * (0.5 * cfg.k_minimal * lo.area - c * vn);
+                    s.F[i] -= f;
+                    s.F[j] += f;
+                    s.T[i] += cross(rA, f * -1.0);
+                    s.T[j] += cross(rB, f);
+                    active.insert(key);
+                    s.counts[i] += 1;
+                    s.counts[j] += 1;
+                    contacts++;
+                    continue;
+                }
                 // main.cpp:2220-2287
                 V3 nA = lo.nA;
                 double area = lo.area;
@@ -1328,6 +1358,10 @@ orc_sim* orc_sim_from_config(const char* path, const char* base_dir) {
 void orc_sim_destroy(orc_sim* s) { delete s; }
 int orc_sim_num_particles(const orc_sim* s) { return static_cast<int>(s->ps.size()); }
 double orc_sim_dt(const orc_sim* s) { return s->cfg.dt; }
+void orc_sim_set_force_model(orc_sim* s, int model, double k) {
+    s->cfg.force_model = model;
+    s->cfg.k_minimal = k;
+}
 
 void orc_sim_get_state(const orc_sim* s, double* o) {
     for (std::size_t i = 0; i < s->ps.size(); ++i) {

@@ -81,6 +81,8 @@ void orc_sim_get_forces(const orc_sim* s, double* F, double* T, int32_t* counts)
  * comps, loops, candidate_pairs, wall_contacts} */
 int orc_sim_step(orc_sim* s, int n, int64_t* counters7);
 double orc_sim_dt(const orc_sim* s);
+/* 1: minimal-energy particle-particle force with stiffness k; 0: the reference force */
+void orc_sim_set_force_model(orc_sim* s, int model, double k);
 
 /* ---- primitives exposed for the golden checks ---- */
 int orc_tri_tri(const double* triA, const double* triB, double* p0p1);

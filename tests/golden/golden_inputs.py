@@ -172,3 +172,21 @@ def sim_cases(inputs_dir: str):
              _particle("star2a.stl", (0.017, 0.003, 0.001), (-0.5, 0, 0), (0.8, 0.6, 0, 0), (0, 0, 5.0), scale=0.01)]
     cases.append(("star_nosplit", synth.format_config(g2, parts), {"star2a.stl": m["star2a"]}))
     return cases
+
+
+def box_pair_case(depth, off, pa, k):
+    """Two boxes at rest, no gravity, minimal-energy force with stiffness k:
+    A (1 x 1.2 x 0.8) centred at (0, pa), B (2 x 2 x 2) whose -x face sits at
+    x = 0.5 - depth inside A, shifted by (off) in y, z.  The contact loop is
+    A's 1.2 x 0.8 cross-section at x = 0.5 - depth.  Returns (config text,
+    {stl name: tris}, pA, pB, rectangle centre)."""
+    pA = np.array([0.0, pa[0], pa[1]])
+    pB = pA + np.array([1.5 - depth, off[0], off[1]])
+    g = dict(steps=1, dt=1e-6, split_contacts=1, gravity=[0.0, 0.0, 0.0], center_mesh=1, vtk_prefix="bx",
+             output_dir="out", output_interval=1, force_model="minimal", minimal_k=float(k))
+    parts = [_particle("boxA.stl", pA, (0, 0, 0), (1, 0, 0, 0), (0, 0, 0)),
+             _particle("boxB.stl", pB, (0, 0, 0), (1, 0, 0, 0), (0, 0, 0))]
+    files = {"boxA.stl": synth.box_wall(1.0, 1.2, 0.8), "boxB.stl": synth.box_wall(2.0, 2.0, 2.0)}
+    centre = np.array([0.5 - depth, pa[0], pa[1]])
+    return synth.format_config(g, parts), files, pA, pB, centre
+

@@ -155,3 +155,28 @@ def test_torque_free_rotation_keeps_angular_momentum():
         L.orc_integrate(s, np.ascontiguousarray(Ibinv.reshape(-1)), 1.0, np.zeros(3), np.zeros(3), np.zeros(3), 1e-3)
     assert np.array_equal(s[13:16], Lw)                  # L += T dt with T = 0: exactly unchanged
     assert abs(np.linalg.norm(s[3:7]) - 1.0) < 1e-14     # q renormalised every step
+
+
+@pytest.mark.parametrize("depth,off,pa", [(0.2, (0.0, 0.0), (0.0, 0.0)), (0.3, (0.2, 0.2), (0.3, -0.7))])
+def test_minimal_force_model_in_the_step_box_on_box(depth, off, pa, tmp_path):
+    """The oracle's time step with "force_model minimal": two boxes at rest
+    (no damping since v_rel = 0) get F_B = k/2 A n and -F_B, acting at the
+    centre of the contact rectangle (the torque about each body centre is
+    (c - x) x F), whatever the world position of the pair."""
+    import golden_inputs as GI
+
+    k = 5.0
+    text, files, pA, pB, c = GI.box_pair_case(depth, off, pa, k)
+    for fn, tris in files.items():
+        synth.write_stl_ascii(str(tmp_path / fn), tris)
+    cfg = tmp_path / "boxes.txt"
+    cfg.write_text(text)
+    s = O.Sim(str(cfg), str(tmp_path))
+    ctr = s.step(1)
+    assert ctr[0] == 1                                      # one contact loop
+    F, T, cnt = s.forces()
+    Fb = 0.5 * k * (1.2 * 0.8) * np.array([1.0, 0.0, 0.0])
+    assert np.allclose(F[1], Fb, rtol=1e-12, atol=1e-12) and np.allclose(F[0], -Fb, rtol=1e-12, atol=1e-12)
+    assert np.allclose(T[0], np.cross(c - pA, -Fb), atol=1e-12)
+    assert np.allclose(T[1], np.cross(c - pB, Fb), atol=1e-12)
+    assert list(cnt) == [1, 1]

@@ -110,7 +110,7 @@ def test_pair_contact_backward_matches_finite_differences(ctx):
 
 def _drop_scene(ctx, n_side=2, seed=4):
     """Hulls of 128 sphere points (r = 5 mm) just above a floor slab."""
-    from zdx.autograd import Scene
+    from zdx.autograd import Scene, body_inverse_inertia
 
     rng = np.random.default_rng(seed)
     hull = synth.sphere_hull(128, 7) * 0.005
@@ -129,7 +129,7 @@ def _drop_scene(ctx, n_side=2, seed=4):
     quat /= np.linalg.norm(quat, axis=1, keepdims=True)
     rad = np.full(n, props[4])
     scene = Scene(zctx=ctx, mesh=np.full(n, mid), radius=rad, mass=torch.full((n,), dens * V, dtype=torch.float64),
-                  Ibinv=torch.from_numpy(np.linalg.inv(I)).expand(n, 3, 3).clone(),
+                  Ibinv=torch.from_numpy(body_inverse_inertia(I)).expand(n, 3, 3).clone(),
                   req=torch.full((n,), (3 * V / (4 * np.pi)) ** (1 / 3), dtype=torch.float64), floor_mesh=fid,
                   floor_pos=np.array([0.0, 0.0, -0.01]), floor_top=0.0,
                   gravity=torch.tensor([0.0, 0.0, -9.81], dtype=torch.float64), dt=1e-5)
@@ -150,7 +150,8 @@ def test_minimal_model_gradients_match_finite_differences(ctx):
     e0 = torch.tensor(0.5, dtype=torch.float64)
 
     def loss(k, e, v):
-        p, q, vv, L = simulate(scene, k, e, pos0, quat0, v, torch.zeros(n, 3, dtype=torch.float64), 300, 50)
+        z = torch.zeros(n, 3, dtype=torch.float64)
+        p = simulate(scene, k, e, pos0, quat0, v, z, z, 300, 50)[0]
         return ((p - target) ** 2).sum()
 
     k, e, v = k0.clone().requires_grad_(True), e0.clone().requires_grad_(True), v0.clone().requires_grad_(True)
@@ -186,8 +187,8 @@ def test_calibration_reduces_loss(ctx):
     v_true = vel0.clone()
     v_true[:, 2] = -0.3
     with torch.no_grad():
-        target, _, _, _ = simulate(scene, k_true, e_true, pos0, quat0, v_true, torch.zeros(n, 3, dtype=torch.float64),
-                                   300, 50)
+        z = torch.zeros(n, 3, dtype=torch.float64)
+        target = simulate(scene, k_true, e_true, pos0, quat0, v_true, z, z, 300, 50)[0]
     v_start = v_true.clone()
     v_start[:, 2] = -0.25
     hist, k_fit, e_fit, v_fit = calibrate(scene, pos0, quat0, target, 2e6, 0.4, v_start, 300, iters=6, lr=0.05)

@@ -1197,6 +1197,11 @@ zdx_status zdx_sim_load_config(zdx_ctx* c, const char* path, const char* base_di
     if (st == ZDX_OK) {
         (*out)->cfg = cfg;
         (*out)->inits = inits;
+        if (cfg.force_model != 0) st = zdx_sim_set_force_model(*out, cfg.force_model, cfg.k_minimal);
+        if (st != ZDX_OK) {
+            zdx_sim_destroy(*out);
+            *out = nullptr;
+        }
     }
     return st;
 }
@@ -1377,6 +1382,18 @@ zdx_status zdx_sim_stage_times(zdx_sim* s, double* out, int32_t n) {
     for (int k = 0; k < 6; ++k) s->acc_ms[k] = 0.0;
     for (int k = 0; k < ZDX_N_COUNTERS; ++k) s->acc_ctr[k] = 0.0;
     s->acc_steps = 0;
+    return ZDX_OK;
+}
+
+zdx_status zdx_sim_set_force_model(zdx_sim* s, int32_t model, double k) {
+    if (!s || (model != ZDX_FORCE_ZDEM && model != ZDX_FORCE_MINIMAL)) return ZDX_BAD_ARG;
+    if (model == ZDX_FORCE_MINIMAL && !(k > 0.0 && k < 1e300)) {
+        s->ctx->err = "the minimal-energy force needs a finite stiffness k > 0";
+        return ZDX_BAD_ARG;
+    }
+    s->sp.force_model = model;
+    s->sp.k_minimal = model == ZDX_FORCE_MINIMAL ? k : 0.0;
+    s->drop_graphs();   // the graphs hold the step parameters by value
     return ZDX_OK;
 }
 

@@ -642,6 +642,36 @@ ZDX_HD inline Contact pp_force(const V3& nA, double area, const V3& xc, const Bo
     return c;
 }
 
+// Minimal-energy particle-particle force of the time step (force model
+// "minimal"): the elastic part of polyline.py:655-701, Fn = k/2 |Sn| along the
+// A->B normal nA, plus normal damping set by the restitution, c = 2 zeta
+// sqrt(m_eff k_c) with zeta = -ln e / sqrt(pi^2 + ln^2 e) and the contact
+// stiffness of a sphere of the effective radius R = R1 R2 / (R1 + R2),
+// k_c = pi k R.  The lever arm is the loop centroid: xc0 = (nA x Gn)/|Sn| is
+// -2/3 of the centroid's part normal to nA (Stokes), so -3/2 xc0 is the
+// centroid up to a multiple of nA, which neither the torque of a force along
+// nA nor v_rel . nA sees.  No tangential force, no history.
+ZDX_HD inline Contact pp_force_minimal(const V3& nA, double area, const V3& xc0, const Body& a, const Body& b,
+                                      const Mat& ma, const Mat& mb, double k) {
+    V3 xc = mul(xc0, -1.5);
+    V3 rA = sub(xc, a.pos), rB = sub(xc, b.pos);
+    V3 vA = add(a.vel, cross(a.omega, rA)), vB = add(b.vel, cross(b.omega, rB));
+    double vn = dot(sub(vB, vA), nA);
+    double R1 = dmax(ma.eqr, 1e-12), R2 = dmax(mb.eqr, 1e-12);
+    double R = R1 * R2 / (R1 + R2);
+    double meff = 1.0 / (ma.inv_mass + mb.inv_mass);
+    double e = dmin(0.9999, dmax(1e-6, dmin(ma.rest, mb.rest)));
+    double le = log(e);
+    double zeta = -le / sqrt(3.141592653589793 * 3.141592653589793 + le * le);
+    double c = 2.0 * zeta * sqrt(meff * (3.141592653589793 * k * R));
+    double fn = 0.5 * k * area - c * vn;
+    Contact r;
+    r.f = mul(nA, fn);
+    r.r1 = rA;
+    r.r2 = rB;
+    return r;
+}
+
 // particle-wall force, main.cpp:2363-2431 (normal damping on)
 ZDX_HD inline Contact pw_force(const V3& n_in, double area, const V3& xc, const Body& p, const Mat& m, double wall_mu,
                               double dt, V3& ds) {

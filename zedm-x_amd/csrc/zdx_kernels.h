@@ -121,6 +121,8 @@ struct SimParams {
     double g[3];
     double dt;
     int32_t split;
+    int32_t force_model;   // 0: the reference's particle-particle force, 1: minimal-energy force
+    double k_minimal;      // stiffness k of the minimal-energy force
 };
 
 struct NarrowCfg {

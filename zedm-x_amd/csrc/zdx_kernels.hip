@@ -811,9 +811,12 @@ struct SimSink {
     const Mat* mb;
     double dt;
     V3* ds;
+    int model;
+    double k_minimal;
     __device__ int operator()(int li, int, int, const V3&, const V3&, const V3& nA, const V3& xc, double area, int,
                               SArr<int>, SArr<double>, const double*) {
-        Contact c = pp_force(nA, area, xc, *a, *b, *ma, *mb, dt, *ds);
+        Contact c = model == 1 ? pp_force_minimal(nA, area, xc, *a, *b, *ma, *mb, k_minimal)
+                               : pp_force(nA, area, xc, *a, *b, *ma, *mb, dt, *ds);
         if (li >= lmax) return TOPO_LOOP_OVERFLOW;
         double* o = contrib + 9 * (int64_t)li;
         o[0] = c.f.x; o[1] = c.f.y; o[2] = c.f.z;
@@ -1164,7 +1167,8 @@ __global__ void __launch_bounds__(64) k_tp_walk_d_sim(DPairs P, DMeshes M, DBodi
                 if (k < key) lo = mid + 1; else hi = mid - 1;
             }
         }
-        SimSink sink{T.contrib + 9 * (int64_t)p * T.lmax, T.lmax, &a, &b, &ma, &mb, sp.dt, &ds};
+        SimSink sink{T.contrib + 9 * (int64_t)p * T.lmax, T.lmax, &a, &b, &ma, &mb, sp.dt, &ds, sp.force_model,
+                     sp.k_minimal};
         const int ncomp = T.ncomp[it];
         const long long to = T.tj_off[it];
         TopoOut o;

@@ -8,6 +8,7 @@ otherwise.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 import os
 
 import numpy as np
@@ -19,13 +20,14 @@ CLI_PATH = os.path.join(PKG, "bin", "zdx")
 
 OK, BAD_ARG, IO, CAPACITY, HIP_ERROR, INTERNAL = range(6)
 MODE_PLAIN, MODE_BATCHED = 0, 1
+FORCE_ZDEM, FORCE_MINIMAL = 0, 1
 COUNTERS = ["contacts", "tri_checks", "segments", "comps", "loops", "pairs", "wall_contacts", "tri_tests", "tri_hits",
             "tri_scanned", "topo_pairs", "seg_max"]
 EXPORTS = ["zdx_create", "zdx_destroy", "zdx_last_error", "zdx_load_stl", "zdx_mesh_register", "zdx_contact_pairs",
            "zdx_sim_load_config", "zdx_sim_create", "zdx_sim_destroy", "zdx_sim_num_particles", "zdx_sim_step",
            "zdx_sim_get_state", "zdx_sim_set_state", "zdx_sim_get_forces", "zdx_sim_write_vtk",
            "zdx_sim_write_state_txt", "zdx_sim_checkpoint", "zdx_sim_restore",
-           "zdx_sim_stage_times", "zdx_sim_set_tiles", "zdx_sim_autotune"]
+           "zdx_sim_stage_times", "zdx_sim_set_force_model", "zdx_sim_set_tiles", "zdx_sim_autotune"]
 LOOP_REC = 28
 
 
@@ -94,6 +96,7 @@ def lib():
         L.zdx_sim_checkpoint.argtypes = [vp, C.c_char_p]
         L.zdx_sim_restore.argtypes = [vp, C.c_char_p]
         L.zdx_sim_stage_times.argtypes = [vp, vp, i32]
+        L.zdx_sim_set_force_model.argtypes = [vp, i32, C.c_double]
         L.zdx_sim_set_tiles.argtypes = [vp, i32, i32]
         L.zdx_sim_autotune.argtypes = [vp, C.POINTER(i32), C.POINTER(i32)]
         for f in EXPORTS:
@@ -133,9 +136,14 @@ class Context:
             raise ZdxError(st, "zdx_create failed (no HIP device?)")
         self.h = h
         self.mesh_props: dict[int, np.ndarray] = {}
+        self._sims = weakref.WeakSet()
 
     def close(self):
+        """Destroys the context, after every simulation still open on it (a
+        simulation must not outlive its context)."""
         if getattr(self, "h", None):
+            for sim in list(self._sims):
+                sim.close()
             self._L.zdx_destroy(self.h)
             self.h = None
 
@@ -213,6 +221,7 @@ class Sim:
         self.ctx = ctx
         self.h = handle
         self.n = ctx._L.zdx_sim_num_particles(handle)
+        ctx._sims.add(self)
 
     @classmethod
     def from_config(cls, ctx: Context, path: str, base_dir: str | None = None) -> "Sim":
@@ -241,7 +250,8 @@ class Sim:
 
     def close(self):
         if getattr(self, "h", None):
-            self.ctx._L.zdx_sim_destroy(self.h)
+            if getattr(self.ctx, "h", None):
+                self.ctx._L.zdx_sim_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -280,6 +290,11 @@ class Sim:
 
     def restore(self, path: str) -> None:
         self.ctx._check(self.ctx._L.zdx_sim_restore(self.h, path.encode()))
+
+    def set_force_model(self, model: int, k: float = 0.0) -> None:
+        """FORCE_ZDEM (the reference's particle-particle force) or FORCE_MINIMAL
+        (Fn = k/2 |Sn| plus restitution damping, at the loop centroid)."""
+        self.ctx._check(self.ctx._L.zdx_sim_set_force_model(self.h, int(model), float(k)))
 
     STAGES = ["transform", "broadphase", "narrow", "topology", "walls", "fold_integrate"]
 

@@ -14,12 +14,14 @@ on the device contact pipeline, for calibration.
   energy identity f = -dPsi/dlambda n with Psi = k/2 Omega, dOmega/dlambda =
   |Sn|, polyline.py:655-701) plus normal damping ``-c (v_rel . n) n`` with
   ``c = 2 zeta(e) sqrt(m_eff k_c)``, ``zeta(e) = -ln e / sqrt(pi^2 + ln^2 e)``
-  and the contact stiffness of a sphere of the equivalent radius R,
-  ``k_c = pi k R``, applied at the loop centroid (``-3/2 xc0`` up to a
-  component along n; polyline.py's minimal model has no torques, and the
-  world-frame ``xc0`` itself lies far from the loop for bodies away from the
-  origin) -- and the rigid-body update of ``integrate_particle``
-  (src/main.cpp:1494-1510, old rotation for omega) in torch.
+  and the contact stiffness of a sphere of the effective radius
+  R = R1 R2 / (R1 + R2), ``k_c = pi k R``, applied at the loop centroid
+  (``-3/2 xc0`` up to a component along n; polyline.py's minimal model has no
+  torques, and the world-frame ``xc0`` itself lies far from the loop for
+  bodies away from the origin) -- and the rigid-body update of
+  ``integrate_particle`` (src/main.cpp:1494-1510, omega from the old rotation,
+  carried as state) in torch.  Without a floor this is the device time step
+  with ``zdx_sim_set_force_model(ZDX_FORCE_MINIMAL, k)``, term for term.
 * ``simulate``: K-step segments under ``torch.utils.checkpoint`` so memory is
   O(N steps / K) states, the contact calls of a segment being re-run in
   backward (they are deterministic).
@@ -110,17 +112,28 @@ def qmat(q):
                         torch.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], -2)
 
 
+def body_inverse_inertia(I: np.ndarray) -> np.ndarray:
+    """Inverse body inertia as the time step uses it: the reference's
+    mat3_inverse (src/main.cpp:203-207) returns the identity when
+    |det I| < 1e-18, which holds for every particle below about a centimetre
+    at mineral densities (I ~ 1e-8 kg m^2), so their omega equals L."""
+    I = np.asarray(I, dtype=np.float64)
+    det = np.linalg.det(I)
+    out = np.linalg.inv(np.where((np.abs(det) < 1e-18)[..., None, None], np.eye(3), I))
+    return np.where((np.abs(det) < 1e-18)[..., None, None], np.eye(3), out)
+
+
 @dataclass
 class Scene:
-    """Particles (registered meshes) above a static floor mesh."""
+    """Particles (registered meshes), optionally above a static floor mesh."""
     zctx: object
     mesh: np.ndarray            # per particle mesh id
     radius: np.ndarray          # bounding radius per particle (candidate test)
     mass: torch.Tensor          # (N,)
     Ibinv: torch.Tensor         # (N, 3, 3) body-frame inverse inertia
     req: torch.Tensor           # (N,) equivalent-sphere radius (cbrt(3V/4pi))
-    floor_mesh: int
-    floor_pos: np.ndarray
+    floor_mesh: int | None
+    floor_pos: np.ndarray | None
     floor_top: float            # z of the floor's top face
     gravity: torch.Tensor
     dt: float
@@ -129,19 +142,22 @@ class Scene:
 
 def _candidates(scene: Scene, pos: np.ndarray):
     """Particle pairs whose bounding spheres overlap (the reference's broad
-    phase test) and particles whose sphere reaches the floor."""
-    n = len(pos)
-    d = pos[:, None, :] - pos[None, :, :]
+    phase test, src/main.cpp:2049-2059) and particles whose sphere reaches the
+    floor."""
+    d = pos[None, :, :] - pos[:, None, :]
     rs = scene.radius[:, None] + scene.radius[None, :]
     ii, jj = np.nonzero(np.triu((d * d).sum(-1) <= rs * rs, 1))
+    if scene.floor_mesh is None:
+        return ii, jj, np.zeros(0, dtype=np.int64)
     fl = np.nonzero(pos[:, 2] - scene.radius <= scene.floor_top)[0]
     return ii, jj, fl
 
 
-def step(scene: Scene, k, e, pos, quat, vel, L):
-    """One step of the minimal model; all tensors float64, differentiable in
-    k, e and the state."""
-    n = pos.shape[0]
+def step(scene: Scene, k, e, pos, quat, vel, omega, L):
+    """One step of the minimal model (the device step with the minimal-energy
+    force, src/main.cpp:2034-2468 with polyline.py:655-701's force); all
+    tensors float64, differentiable in k, e and the state (pos, quat, vel,
+    omega, L).  Returns the new state."""
     ii, jj, fl = _candidates(scene, pos.detach().cpu().numpy())
     ia = np.concatenate([ii, fl]).astype(np.int64)
     ib = np.concatenate([jj, np.full(len(fl), -1)]).astype(np.int64)
@@ -149,14 +165,16 @@ def step(scene: Scene, k, e, pos, quat, vel, L):
     T = torch.zeros_like(pos)
     if len(ia):
         ident = torch.tensor([1.0, 0.0, 0.0, 0.0], dtype=pos.dtype, device=pos.device)
-        fpos = torch.tensor(scene.floor_pos, dtype=pos.dtype, device=pos.device)
+        fpos = torch.tensor(scene.floor_pos if scene.floor_mesh is not None else np.zeros(3), dtype=pos.dtype,
+                            device=pos.device)
         tia = torch.from_numpy(ia).to(pos.device)
         tib = torch.from_numpy(np.where(ib >= 0, ib, 0)).to(pos.device)
         isfloor = torch.from_numpy(ib < 0).to(pos.device)
         pA, qA = pos[tia], quat[tia]
         pB = torch.where(isfloor[:, None], fpos.expand(len(ia), 3), pos[tib])
         qB = torch.where(isfloor[:, None], ident.expand(len(ia), 4), quat[tib])
-        mb = np.where(ib >= 0, scene.mesh[np.where(ib >= 0, ib, 0)], scene.floor_mesh)
+        mb = np.where(ib >= 0, scene.mesh[np.where(ib >= 0, ib, 0)], -1 if scene.floor_mesh is None else
+                      scene.floor_mesh)
         Sc, xc, lp = PairContact.apply(pA, qA, pB, qB, scene.zctx, scene.mesh[ia], mb, scene.split)
         if Sc.shape[0]:
             lp = lp.long()
@@ -169,62 +187,58 @@ def step(scene: Scene, k, e, pos, quat, vel, L):
             # (Stokes on Gn); -3/2 xc0 is the centroid up to a multiple of n,
             # which neither the torque of a normal force nor v_rel . n sees.
             xc = -1.5 * xc
-            # velocities of the material points at the contact point
-            R_a = qmat(quat[a])
-            Iinv_a = R_a @ scene.Ibinv[a] @ R_a.transpose(-1, -2)
-            wa = (Iinv_a @ L[a].unsqueeze(-1)).squeeze(-1)
             rA = xc - pos[a]
-            va = vel[a] + torch.cross(wa, rA, dim=-1)
-            R_b = qmat(quat[b])
-            Iinv_b = R_b @ scene.Ibinv[b] @ R_b.transpose(-1, -2)
-            wb = (Iinv_b @ L[b].unsqueeze(-1)).squeeze(-1)
+            va = vel[a] + torch.cross(omega[a], rA, dim=-1)
             rB = xc - torch.where(fb[:, None], fpos.expand_as(xc), pos[b])
-            vb = torch.where(fb[:, None], torch.zeros_like(xc), vel[b] + torch.cross(wb, rB, dim=-1))
+            vb = torch.where(fb[:, None], torch.zeros_like(xc), vel[b] + torch.cross(omega[b], rB, dim=-1))
             vn = ((vb - va) * nrm).sum(-1, keepdim=True)
             ma = scene.mass[a][:, None]
-            mbm = torch.where(fb[:, None], torch.full_like(ma, math.inf), scene.mass[b][:, None])
-            meff = torch.where(fb[:, None], ma, ma * mbm / (ma + torch.where(fb[:, None], ma, mbm)))
-            req = torch.where(fb, scene.req[a], 0.5 * (scene.req[a] + scene.req[b]))[:, None]
+            mbm = torch.where(fb[:, None], ma, scene.mass[b][:, None])
+            meff = torch.where(fb[:, None], ma, ma * mbm / (ma + mbm))
+            ra, rb = scene.req[a], scene.req[b]
+            req = torch.where(fb, ra, ra * rb / (ra + rb))[:, None]
             kc = math.pi * k * req
             le = torch.log(e)
             zeta = -le / torch.sqrt(math.pi ** 2 + le * le)
             c = 2.0 * zeta * torch.sqrt(meff * kc)
-            fB = 0.5 * k * area * nrm - c * vn * nrm          # force on B
+            fB = (0.5 * k * area - c * vn) * nrm          # force on B
             F = F.index_add(0, a, -fB)
             T = T.index_add(0, a, torch.cross(rA, -fB, dim=-1))
             keep = ~fb
             F = F.index_add(0, b[keep], fB[keep])
             T = T.index_add(0, b[keep], torch.cross(rB[keep], fB[keep], dim=-1))
-    # integrate_particle (old rotation for omega)
+    # integrate_particle, src/main.cpp:1494-1510 (omega from the old rotation)
     vel = vel + (F / scene.mass[:, None] + scene.gravity) * scene.dt
     pos = pos + vel * scene.dt
     R = qmat(quat)
     Iinv = R @ scene.Ibinv @ R.transpose(-1, -2)
     L = L + T * scene.dt
-    om = (Iinv @ L.unsqueeze(-1)).squeeze(-1)
-    dq = qmul(torch.cat([torch.zeros_like(om[:, :1]), om], -1), quat)
+    omega = (Iinv @ L.unsqueeze(-1)).squeeze(-1)
+    dq = qmul(torch.cat([torch.zeros_like(omega[:, :1]), omega], -1), quat)
     quat = quat + 0.5 * dq * scene.dt
     quat = quat / quat.norm(dim=-1, keepdim=True)
-    return pos, quat, vel, L
+    return pos, quat, vel, omega, L
 
 
-def simulate(scene: Scene, k, e, pos, quat, vel, L, steps: int, ckpt_every: int = 50):
-    """`steps` steps; every `ckpt_every` steps form one checkpointed segment."""
+def simulate(scene: Scene, k, e, pos, quat, vel, omega, L, steps: int, ckpt_every: int = 50):
+    """`steps` steps; every `ckpt_every` steps form one checkpointed segment.
+    Returns the final (pos, quat, vel, omega, L)."""
     s = 0
+    state = (pos, quat, vel, omega, L)
     while s < steps:
         m = min(ckpt_every, steps - s)
 
-        def seg(k_, e_, p_, q_, v_, l_, m=m):
+        def seg(k_, e_, *st, m=m):
             for _ in range(m):
-                p_, q_, v_, l_ = step(scene, k_, e_, p_, q_, v_, l_)
-            return p_, q_, v_, l_
+                st = step(scene, k_, e_, *st)
+            return st
 
         if torch.is_grad_enabled():
-            pos, quat, vel, L = checkpoint(seg, k, e, pos, quat, vel, L, use_reentrant=False)
+            state = checkpoint(seg, k, e, *state, use_reentrant=False)
         else:
-            pos, quat, vel, L = seg(k, e, pos, quat, vel, L)
+            state = seg(k, e, *state)
         s += m
-    return pos, quat, vel, L
+    return state
 
 
 def scene_from_workload(zctx, wl, floor_thickness: float = 0.02):
@@ -252,7 +266,7 @@ def scene_from_workload(zctx, wl, floor_thickness: float = 0.02):
         mesh[i] = mid
         rad[i] = float(props[4])
         mass[i] = dens * V
-        Ibinv[i] = np.linalg.inv(I)
+        Ibinv[i] = body_inverse_inertia(I)
         req[i] = (3.0 * V / (4.0 * math.pi)) ** (1.0 / 3.0)
     tris, wpos, _, _, _, _ = wl.walls[0]
     ext = np.ptp(tris.reshape(-1, 3), axis=0)
@@ -286,11 +300,12 @@ def calibrate(scene, pos0, quat0, target, k0, e0, v0, steps, iters=10, lr=0.05, 
         opt.zero_grad()
         k = lk.exp()
         e = torch.sigmoid(le)
-        p, _, _, _ = simulate(scene, k, e, pos0, quat0, v, torch.zeros(n, 3, dtype=torch.float64), steps, ckpt_every)
+        z = torch.zeros(n, 3, dtype=torch.float64)
+        p = simulate(scene, k, e, pos0, quat0, v, z, z, steps, ckpt_every)[0]
         loss = ((p - target) ** 2).sum()
         loss.backward()
         hist.append((float(loss.detach()), float(k.detach()), float(e.detach())))
         if log:
             log(hist[-1])
         opt.step()
-    return hist, float(lk.exp()), float(torch.sigmoid(le)), v.detach()
+    return hist, float(lk.detach().exp()), float(torch.sigmoid(le.detach())), v.detach()
