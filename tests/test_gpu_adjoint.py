@@ -195,3 +195,59 @@ def test_calibration_reduces_loss(ctx):
     losses = [h[0] for h in hist]
     assert losses[0] > 0.0
     assert min(losses[1:]) < 0.5 * losses[0], losses
+
+
+def test_drop_on_plane_64_gradients_and_calibration(ctx):
+    """The drop-on-plane workload at full size (64 hulls of 128 points, r = 5 mm,
+    500 steps, dt 1e-5): adjoint d loss / d (k, e, v0) against central
+    differences with relative step 1e-6, then a calibration from a perturbed
+    start that lowers the loss."""
+    import time
+
+    from zdx import workloads as W
+    from zdx.autograd import calibrate, scene_from_workload, simulate
+
+    wl = W.drop_on_plane(64)
+    scene, pos0, quat0, vel0 = scene_from_workload(ctx, wl)
+    n, steps = pos0.shape[0], 500
+    assert n == 64
+    z = torch.zeros(n, 3, dtype=torch.float64)
+    k_true, e_true = torch.tensor(4e6, dtype=torch.float64), torch.tensor(0.6, dtype=torch.float64)
+    v_true = vel0.clone()
+    v_true[:, 2] = -0.3
+    t0 = time.time()
+    with torch.no_grad():
+        target = simulate(scene, k_true, e_true, pos0, quat0, v_true, z, z, steps, 50)[0]
+    print(f"forward {steps} steps: {time.time() - t0:.2f} s", flush=True)
+
+    k0, e0 = torch.tensor(3e6, dtype=torch.float64), torch.tensor(0.5, dtype=torch.float64)
+    v0 = v_true.clone()
+    v0[:, 2] = -0.28
+
+    def loss(k, e, v):
+        return ((simulate(scene, k, e, pos0, quat0, v, z, z, steps, 50)[0] - target) ** 2).sum()
+
+    k, e, v = k0.clone().requires_grad_(True), e0.clone().requires_grad_(True), v0.clone().requires_grad_(True)
+    t0 = time.time()
+    loss(k, e, v).backward()
+    print(f"forward + adjoint: {time.time() - t0:.2f} s", flush=True)
+    rel = 1e-6
+    with torch.no_grad():
+        checks = [("k", k.grad.item(), lambda s: loss(k0 * (1 + s * rel), e0, v0), float(k0) * rel),
+                  ("e", e.grad.item(), lambda s: loss(k0, e0 * (1 + s * rel), v0), float(e0) * rel)]
+        for i in (0, 27, 63):
+            def fv(s, i=i):
+                vv = v0.clone()
+                vv[i, 2] *= (1 + s * rel)
+                return loss(k0, e0, vv)
+            checks.append((f"v0[{i}].z", v.grad[i, 2].item(), fv, float(v0[i, 2]) * rel))
+        for name, g, fn, h in checks:
+            fd = (float(fn(1)) - float(fn(-1))) / (2 * h)
+            print(name, g, fd, flush=True)
+            assert abs(g - fd) <= 1e-4 * max(abs(fd), abs(g)) + 1e-14, (name, g, fd)
+
+    hist, k_fit, e_fit, v_fit = calibrate(scene, pos0, quat0, target, float(k0), float(e0), v0, steps, iters=6,
+                                          lr=0.05)
+    losses = [h[0] for h in hist]
+    print("calibration losses", losses, "k", k_fit, "e", e_fit, flush=True)
+    assert min(losses[1:]) < 0.5 * losses[0], losses
