@@ -538,6 +538,26 @@ __global__ void __launch_bounds__(256) k_need_list(int32_t n, const int32_t* fla
     if (i == 0) *n_list = off[n];
 }
 
+#ifdef ZDX_WALK_PROFILE
+// Profiling build: device clock ticks per phase, summed over all pairs, read
+// back with zdx_walk_profile().  Slots 0-7: the topology walks (slot 0: from
+// the start of a pair to its first mark); slots 8-14: the narrow phase (pair
+// set-up, cull A, cull B, sweep tests, exact tests, publish, pairs done).
+__device__ unsigned long long g_walk_prof[16];
+#define NP_MARK(k)                                                      \
+    do {                                                                \
+        if (threadIdx.x == 0) {                                         \
+            const unsigned long long t_ = wall_clock64();               \
+            atomicAdd(&g_walk_prof[k], t_ - np_t0);                     \
+            np_t0 = t_;                                                 \
+        }                                                               \
+    } while (0)
+#define NP_START() unsigned long long np_t0 = wall_clock64()
+#else
+#define NP_MARK(k) do {} while (0)
+#define NP_START() do {} while (0)
+#endif
+
 // ------------------------------------------------------------ narrow phase
 // outward-rounded float AABB of a double box
 __device__ inline void box_f(const double* lo, const double* hi, float* f) {
@@ -557,6 +577,23 @@ __device__ inline void load_tri(const double* wv, const int32_t* ti, int t, V3* 
     out[2] = ld3(wv + 3 * ti[3 * t + 2]);
 }
 
+// Vertex outcodes against a float box: bit 2k set when the vertex lies below
+// the box on axis k with its coordinate rounded up, bit 2k+1 when it lies
+// above with its coordinate rounded down.  A triangle's outward-rounded float
+// AABB misses the box exactly when the three outcodes share a set bit
+// (rounding is monotone, so rounding the vertex coordinates commutes with the
+// min/max of the box), which makes the cull below the same test as
+// ovl_f(box_f(tri_box(...)), box) with one load per vertex instead of three
+// per triangle.
+__device__ inline unsigned vertex_outcode(const double* w, const float* bx) {
+    const double x = w[0], y = w[1], z = w[2];
+    return (unsigned)(__double2float_ru(x) < bx[0]) | ((unsigned)(__double2float_rd(x) > bx[3]) << 1) |
+           ((unsigned)(__double2float_ru(y) < bx[1]) << 2) | ((unsigned)(__double2float_rd(y) > bx[4]) << 3) |
+           ((unsigned)(__double2float_ru(z) < bx[2]) << 4) | ((unsigned)(__double2float_rd(z) > bx[5]) << 5);
+}
+constexpr int OC_CAP = 2048;   // vertices per body whose outcodes fit in LDS
+constexpr int CULL_PER_THREAD = 8;
+
 // PROV: also record each segment's provenance (ta, tb, endpoint codes of
 // tri_tri_prov) in P.prov, staged per workgroup after its segments.
 template <int BS, int K, bool PROV>
@@ -571,6 +608,7 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
     __shared__ int s_item;
     __shared__ int s_ha[BS * K];
     __shared__ int s_hb[BS * K];
+    __shared__ unsigned char s_oc[2 * OC_CAP];
     __shared__ unsigned long long s_base;
     const int tid = threadIdx.x;
     int32_t* LA = list_i + (int64_t)blockIdx.x * 2 * tmax;
@@ -581,6 +619,7 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
     int32_t* STP = (int32_t*)(ST + 6 * (int64_t)stage_cap);   // PROV: 4 ints per staged segment
     const int npairs = *P.n;
     int item = blockIdx.x;
+    NP_START();
     while (true) {
         if (use_queue) {
             if (tid == 0) s_item = (int)atomicAdd(queue, 1u);
@@ -600,8 +639,69 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
         float boxA[6], boxB[6];
         box_f(B.aabb + 6 * (int64_t)i, B.aabb + 6 * (int64_t)i + 3, boxA);
         box_f(B.aabb + 6 * (int64_t)j, B.aabb + 6 * (int64_t)j + 3, boxB);
+        NP_MARK(8);
         // ---- cull each side against the other body's AABB (conservative)
         int nA = 0, nB = 0;
+        const int VA = M.nverts[mA], VB = M.nverts[mB];
+        if (VA <= OC_CAP && VB <= OC_CAP) {
+            // outcodes of both bodies' vertices, then per thread CULL_PER_THREAD
+            // consecutive triangles of each side, one ordered scan per round
+            for (int v0 = 0; v0 < VA + VB; v0 += 4 * BS) {
+                double w[4][3];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {   // all loads first, then the codes
+                    const int v = v0 + u * BS + tid;
+                    const double* src = v < VA ? wvA + 3 * v : wvB + 3 * (v - VA);
+                    if (v < VA + VB) { w[u][0] = src[0]; w[u][1] = src[1]; w[u][2] = src[2]; }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int v = v0 + u * BS + tid;
+                    if (v < VA + VB) {
+                        const bool a = v < VA;
+                        s_oc[a ? v : OC_CAP + v - VA] = (unsigned char)vertex_outcode(w[u], a ? boxB : boxA);
+                    }
+                }
+            }
+            __syncthreads();
+            constexpr int CPT = CULL_PER_THREAD;
+            for (int base = 0; base < TA || base < TB; base += BS * CPT) {
+                const int ta0 = base + tid * CPT;
+                unsigned ka = 0u, kb = 0u;
+#pragma unroll
+                for (int c = 0; c < CPT; ++c) {
+                    const int t = ta0 + c;
+                    if (t < TA) {
+                        const int32_t* ix = tiA + 3 * t;
+                        if ((s_oc[ix[0]] & s_oc[ix[1]] & s_oc[ix[2]]) == 0) ka |= 1u << c;
+                    }
+                    if (t < TB) {
+                        const int32_t* ix = tiB + 3 * t;
+                        if ((s_oc[OC_CAP + ix[0]] & s_oc[OC_CAP + ix[1]] & s_oc[OC_CAP + ix[2]]) == 0) kb |= 1u << c;
+                    }
+                }
+                int tot;
+                const int off = block_excl<BS>(__popc(ka) | (__popc(kb) << 16), s_wave, tot);
+                int oa = nA + (off & 0xffff), ob = nB + (off >> 16);
+                for (unsigned m = ka; m; m &= m - 1) LA[oa++] = ta0 + __ffs(m) - 1;
+                for (unsigned m = kb; m; m &= m - 1) LB[ob++] = ta0 + __ffs(m) - 1;
+                nA += tot & 0xffff;
+                nB += tot >> 16;
+            }
+            __syncthreads();
+            // float boxes of the kept triangles, all lanes in parallel
+            for (int k = tid; k < nA + nB; k += BS) {
+                const bool a = k < nA;
+                const int q = a ? k : k - nA;
+                const int t = a ? LA[q] : LB[q];
+                V3 tr[3]; double lo[3], hi[3];
+                load_tri(a ? wvA : wvB, a ? tiA : tiB, t, tr);
+                tri_box(tr, lo, hi);
+                float* f = a ? FA + 6 * q : FB + 6 * q;
+                box_f(lo, hi, f);
+            }
+            NP_MARK(9);
+        } else {
         for (int base = 0; base < TA; base += BS) {
             int t = base + tid;
             bool keep = false;
@@ -621,6 +721,7 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
             }
             nA += tot;
         }
+        NP_MARK(9);
         for (int base = 0; base < TB; base += BS) {
             int t = base + tid;
             bool keep = false;
@@ -640,7 +741,9 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
             }
             nB += tot;
         }
+        }
         __syncthreads();
+        NP_MARK(10);
         // ---- ordered sweep of the culled product
         const int64_t total = (int64_t)nA * nB;
         int nseg = 0, hits = 0;
@@ -650,7 +753,15 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
             unsigned hm = 0u;
             const int64_t g0 = base + (int64_t)tid * K;
             int ca0 = 0, cb0 = 0;
-            if (g0 < total) { ca0 = (int)(g0 / nB); cb0 = (int)(g0 - (int64_t)ca0 * nB); }
+            if (g0 < total) {
+                if (total <= 0x7fffffff) {   // 32-bit division when the product fits
+                    ca0 = (int)g0 / nB;
+                    cb0 = (int)g0 - ca0 * nB;
+                } else {
+                    ca0 = (int)(g0 / nB);
+                    cb0 = (int)(g0 - (int64_t)ca0 * nB);
+                }
+            }
             {
                 int ca = ca0, cb = cb0;
                 float fa[6];
@@ -678,6 +789,7 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
                 }
             }
             __syncthreads();
+            NP_MARK(11);
             hits += nh;
             for (int hb = 0; hb < nh; hb += BS) {
                 const int h = hb + tid;
@@ -718,6 +830,7 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
                 nseg += ns;
             }
             __syncthreads();
+            NP_MARK(12);
         }
         // ---- publish: allocate arena space, copy the ordered segments
         int nstore = nseg;
@@ -751,6 +864,10 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
             P.tri_scanned[p] = TA + TB;
         }
         __syncthreads();
+        NP_MARK(13);
+#ifdef ZDX_WALK_PROFILE
+        if (tid == 0) atomicAdd(&g_walk_prof[14], 1ull);
+#endif
         if (!use_queue) item += gridDim.x;
     }
     kt_end(kt);
@@ -833,12 +950,6 @@ __device__ inline Mat load_mat(const DMat& m, int i) {
     return r;
 }
 
-#ifdef ZDX_WALK_PROFILE
-// Profiling build: device clock ticks per phase of the walks, summed over all
-// pairs (slot 0: from the start of a pair to its first mark), read back with
-// zdx_walk_profile().
-__device__ unsigned long long g_walk_prof[16];
-#endif
 // Wave-cooperative copy of n elements with four loads in flight per lane.
 template <class T>
 __device__ inline void wave_copy(T* dst, const T* src, int n) {
