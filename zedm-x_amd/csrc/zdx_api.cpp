@@ -177,10 +177,11 @@ struct zdx_ctx {
         DBuf<int64_t> seg_off, tri_checks, tri_tests;
         DBuf<int32_t> nseg, tri_hits, tri_scanned;
         DBuf<double> arena;
-        DBuf<unsigned long long> arena_top;
         DBuf<int32_t> list_i;
         DBuf<float> list_f;
-        DBuf<double> stage;
+        DBuf<int32_t> hits, hit_tri, hit_flag;
+        DBuf<unsigned long long> hit_top;
+        DBuf<int64_t> hit_off;
         DBuf<unsigned int> queue;
         DBuf<int32_t> bcnt, bcur;
         DBuf<int32_t> ncontrib, comps, loops_raw, tstatus, topo_list, topo_n;
@@ -188,8 +189,8 @@ struct zdx_ctx {
         DBuf<double> rec, vtx;
         DBuf<int32_t> prov;            // sensitivity runs
         DBuf<double> ej, vj, ljac;
-        int32_t stage_cap = 1024;
-        int64_t arena_cap = 1 << 20;
+        int32_t stage_cap = 1024;      // most segments one pair may store
+        int64_t hit_cap = 1 << 16;     // hit slots = segment slots
         int32_t lmax = 32;
     } api;
 
@@ -313,13 +314,14 @@ struct zdx_sim {
     DBuf<int32_t> d_order, d_bcnt, d_bcur, d_topo_list, d_topo_n;
     DBuf<int64_t> d_seg_off, d_tri_checks, d_tri_tests;
     DBuf<int32_t> d_nseg, d_tri_hits, d_tri_scanned;
-    DBuf<double> d_arena;
-    DBuf<unsigned long long> d_arena_top;
-    int64_t arena_cap = 0;
+    DBuf<double> d_arena;        // segments, one slot per hit slot
     DBuf<int32_t> d_list_i;
     DBuf<float> d_list_f;
-    DBuf<double> d_stage;
-    int32_t stage_cap = 512;
+    DBuf<int32_t> d_hits, d_hit_flag;
+    DBuf<unsigned long long> d_hit_top;
+    DBuf<int64_t> d_hit_off;
+    int64_t hit_cap = 0;
+    int32_t stage_cap = 512;   // most segments one pair may store
     int32_t tmax = 1;            // largest triangle count of the meshes in use
     uint64_t mesh_version = 0;   // context mesh upload the graphs were captured with
     DBuf<unsigned int> d_queue;
@@ -402,9 +404,11 @@ struct zdx_sim {
         P.cap = pair_cap; P.n = ps[par].n.p; P.pi = ps[par].pi.p; P.pj = ps[par].pj.p;
         P.order = lpt ? d_order.p : nullptr;
         P.seg_off = d_seg_off.p; P.nseg = d_nseg.p; P.tri_checks = d_tri_checks.p; P.tri_tests = d_tri_tests.p;
-        P.tri_hits = d_tri_hits.p; P.tri_scanned = d_tri_scanned.p; P.arena = d_arena.p; P.arena_cap = arena_cap; P.arena_top = d_arena_top.p;
+        P.tri_hits = d_tri_hits.p; P.tri_scanned = d_tri_scanned.p; P.arena = d_arena.p; P.arena_cap = hit_cap;
         P.status = d_status.p;
         P.topo_list = d_topo_list.p; P.topo_n = d_topo_n.p;
+        P.hits = d_hits.p; P.hit_cap = hit_cap; P.hit_top = d_hit_top.p; P.hit_off = d_hit_off.p;
+        P.hit_flag = d_hit_flag.p;
         return P;
     }
     HistBufs hist(int par) {
@@ -464,6 +468,7 @@ struct zdx_sim {
         }
         d_adj_j.ensure(adj_cap); d_adj_pair.ensure(adj_cap);
         d_order.ensure(cap); d_topo_list.ensure(cap); d_topo_n.ensure(1);
+        d_hit_off.ensure(cap); d_hit_top.ensure(1);
         d_seg_off.ensure(cap); d_tri_checks.ensure(cap); d_tri_tests.ensure(cap);
         d_nseg.ensure(cap); d_tri_hits.ensure(cap); d_tri_scanned.ensure(cap);
         d_ncontrib.ensure(cap); d_comps.ensure(cap); d_loops_raw.ensure(cap); d_tstatus.ensure(cap);
@@ -476,21 +481,18 @@ struct zdx_sim {
         d_contrib.release();
         d_contrib.ensure((size_t)pair_cap * lmax * 9);
     }
-    void alloc_arena(int64_t cap) {
-        drop_graphs();
-        arena_cap = cap;
-        d_arena.release();
-        d_arena.ensure((size_t)cap * 6);
-        tw.ensure_arena(cap);
-    }
     void alloc_stage(int32_t cap) {
         drop_graphs();
         stage_cap = cap;
         seg_max = std::max(seg_max, cap);
-        size_t groups = (size_t)persistent_groups(ctx, 16);
-        d_stage.release();
-        d_stage.ensure(groups * 6 * (size_t)stage_cap);
         alloc_topo_scratch();
+    }
+    void alloc_hits(int64_t cap) {
+        drop_graphs();
+        hit_cap = cap;
+        d_hits.release(); d_hit_flag.release(); d_arena.release();
+        d_hits.ensure(6 * (size_t)cap); d_hit_flag.ensure((size_t)cap); d_arena.ensure(6 * (size_t)cap);
+        tw.ensure_arena(cap);
     }
     void alloc_topo_scratch() {
         drop_graphs();
@@ -519,6 +521,8 @@ void sim_setup_device(zdx_sim* s) {
         at += (int64_t)(c->meshes[p.mesh].h.tri_idx.size() / 3);
     }
     wvo[N] = av; wto[N] = at;
+    if (av >= ((int64_t)1 << 31))   // the narrow phase's hit records index world vertices with int32
+        throw ApiError{ZDX_CAPACITY, "more than 2^31 world vertices in one simulation"};
     hipStream_t st = c->stream;
     s->d_mesh.upload(s->mesh_of.data(), N, st);
     s->d_mass.upload(mass.data(), N, st); s->d_inv_mass.upload(im.data(), N, st);
@@ -542,9 +546,8 @@ void sim_setup_device(zdx_sim* s) {
     s->d_cell_cursor.ensure(cells); s->d_sorted.ensure(N); s->d_cnt_up.ensure(N); s->d_cnt_all.ensure(N);
     s->d_pair_off.ensure(N + 1); s->d_adj_off.ensure(N + 1); s->d_n_adj.ensure(1);
     s->d_scan_tmp.ensure((size_t)(std::max<int64_t>(cells, N) / 1024 + 4));
-    s->d_status.ensure(1); s->d_ctr.ensure(16); s->d_queue.ensure(1);
+    s->d_status.ensure(1); s->d_ctr.ensure(16); s->d_queue.ensure(2);
     s->d_bcnt.ensure(64); s->d_bcur.ensure(64);
-    s->d_arena_top.ensure(1);
     s->d_kt.ensure(12);
     {
         int khz = 0;
@@ -552,13 +555,13 @@ void sim_setup_device(zdx_sim* s) {
             s->clock_khz = (double)khz;
     }
     s->alloc_pairs(std::max(1024, 8 * N));
-    s->alloc_arena(std::max<int64_t>(1 << 16, 64 * (int64_t)N));
     size_t groups = (size_t)persistent_groups(c, 16);
     s->tmax = c->max_tris;
-    s->d_list_i.ensure(groups * 2 * (size_t)s->tmax);
+    s->d_list_i.ensure(groups * 8 * (size_t)s->tmax);
     s->d_list_f.ensure(groups * 12 * (size_t)s->tmax);
     s->topo_threads = std::min<int64_t>(2048, std::max<int64_t>(64, (int64_t)N));
     s->alloc_stage(s->stage_cap);
+    s->alloc_hits(std::max<int64_t>(1 << 16, 16 * (int64_t)N));
     s->tw.ensure_bump();   // before any graph capture
     // particle/wall histories start empty
     for (int k = 0; k < 2; ++k) {
@@ -643,10 +646,14 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     DBodies B = s->bodies(par);
     DMeshes M = s->ctx->dmeshes();
     DMat mat = s->mat();
-    unsigned long long* kt = s->d_kt.p;   // stage slots: 0 transform 1 broad 2 narrow 3 topo 4 walls 5 fold
+    // stage stamps: 0 start, 1 after the broad phase, 2 after the transform,
+    // 3 after the narrow phase, 4 after the topology, 5 after the walls, 6 after
+    // fold + integrate
+    unsigned long long* kt = s->d_kt.p;
     const int gmax = persistent_groups(s->ctx, s->np_groups_per_cu);
-    launch_kt_reset(kt, 6, st);
-    launch_broadphase(s->bpbufs(par), st, kt + 2);
+    launch_stamp(kt + 0, st);
+    launch_broadphase(s->bpbufs(par), st);
+    launch_stamp(kt + 1, st);
     launch_need_list(s->N, s->d_state[par].p, s->d_radius.p, s->d_cnt_all.p, s->wallbufs(), s->d_need_flag.p,
                      s->d_need_off.p, s->d_scan_tmp.p, s->d_need_list.p, s->d_n_need.p, s->d_wall_flag.p,
                      s->d_wall_off.p, s->d_wall_list.p, s->d_n_wall.p, st);
@@ -662,17 +669,16 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
         g_transform = std::max(1, std::min(c[1], g_transform));
         g_walls = std::max(1, std::min(c[2], g_walls));
     }
-    launch_transform(B, M, st, kt + 0, s->d_need_list.p, s->d_n_need.p, g_transform);
+    launch_transform(B, M, st, s->d_need_list.p, s->d_n_need.p, g_transform);
+    launch_stamp(kt + 2, st);
     DPairs P = s->dpairs(par, batched);
     if (batched) launch_lpt_order(P, M, s->d_mesh.p, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st);
-    launch_fill_u64(s->d_arena_top.p, 0ull, 1, st);
-    launch_fill_i32((int32_t*)s->d_queue.p, 0, 1, st);
     launch_fill_i32(s->d_topo_n.p, 0, 1, st);
     NarrowCfg nc = s->ncfg;
     nc.use_queue = batched ? 1 : 0;
     nc.groups = g_narrow;
-    launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->d_stage.p, s->stage_cap, s->tmax,
-                  s->d_queue.p, st, kt + 4);
+    launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->stage_cap, s->tmax, s->d_queue.p, st);
+    launch_stamp(kt + 3, st);
     TopoBufs T = s->topobufs();
     if (!batched) {
         int32_t nt = 0;
@@ -683,13 +689,16 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     }
     HistBufs H = s->hist(par);
     launch_topo_zero(P, T, H.cur_active, s->pair_cap, st);
-    launch_topo_sim(P, M, B, mat, H, T, s->sp, st, kt + 6);
+    launch_topo_sim(P, M, B, mat, H, T, s->sp, st);
+    launch_stamp(kt + 4, st);
     if (s->W > 0)
         launch_walls(B, M, mat, s->wallbufs(), H, s->d_wcontrib.p, s->d_wnc.p, s->d_wall_list.p, s->d_n_wall.p,
-                     s->d_wscr.p, g_walls, s->wall_pts_cap, s->sp, s->d_status.p, st, kt + 8);
+                     s->d_wscr.p, g_walls, s->wall_pts_cap, s->sp, s->d_status.p, st);
+    launch_stamp(kt + 5, st);
     launch_fold_integrate(s->N, s->d_state[par].p, s->d_state[nxt].p, mat, s->d_adj_off.p, s->d_adj_j.p,
                           s->d_adj_pair.p, s->adj_cap, T, s->W, s->d_wcontrib.p, s->d_wnc.p, s->max_wall_tris,
-                          s->d_F.p, s->d_T.p, s->d_counts.p, s->sp, st, kt + 10);
+                          s->d_F.p, s->d_T.p, s->d_counts.p, s->sp, st);
+    launch_stamp(kt + 6, st);
     launch_counters(P, T, s->N * s->W, s->d_wnc.p, s->d_ctr.p, st);
 }
 
@@ -712,7 +721,15 @@ bool grow_for(zdx_sim* s, int32_t status, int par) {
         s->alloc_pairs(std::max<int32_t>(2 * s->pair_cap, need + need / 4 + 16));
         grown = true;
     }
-    if (status & ST_ARENA) { s->alloc_arena(s->arena_cap * 2); grown = true; }
+    if (status & ST_HITS) {
+        // the hit counter holds what every pair asked for
+        unsigned long long want = 0;
+        HIPC(hipMemcpy(&want, s->d_hit_top.p, sizeof(want), hipMemcpyDeviceToHost));
+        int64_t cap = 2 * s->hit_cap;
+        while (cap < (int64_t)(want + want / 4)) cap *= 2;
+        s->alloc_hits(cap);
+        grown = true;
+    }
     if (status & ST_STAGE) {
         if (s->stage_cap >= (1 << 16)) return false;
         s->alloc_stage(s->stage_cap * 4);
@@ -799,11 +816,13 @@ void step_once(zdx_sim* s, bool batched) {
         int32_t status = read_status(s);
         if (status == 0) {
             read_counters(s);
-            unsigned long long kt[12];
+            unsigned long long kt[7];
             HIPC(hipMemcpy(kt, s->d_kt.p, sizeof(kt), hipMemcpyDeviceToHost));
+            // acc_ms: transform, broad phase, narrow, topology, walls, fold
+            const int from[6] = {1, 0, 2, 3, 4, 5};
             for (int k = 0; k < 6; ++k)
-                if (kt[2 * k + 1] > kt[2 * k] && kt[2 * k] != ~0ull)
-                    s->acc_ms[k] += (double)(kt[2 * k + 1] - kt[2 * k]) / s->clock_khz;
+                if (kt[from[k] + 1] >= kt[from[k]])
+                    s->acc_ms[k] += (double)(kt[from[k] + 1] - kt[from[k]]) / s->clock_khz;
             for (int k = 0; k < ZDX_N_COUNTERS; ++k)
                 s->acc_ctr[k] = k == ZDX_CTR_SEG_MAX ? std::max(s->acc_ctr[k], (double)s->last_ctr[k])
                                                      : s->acc_ctr[k] + (double)s->last_ctr[k];
@@ -919,6 +938,8 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
             }
         }
         wvo[nb] = av; wto[nb] = at;
+        if (av >= ((int64_t)1 << 31))   // the narrow phase's hit records index world vertices with int32
+            throw ApiError{ZDX_CAPACITY, "more than 2^31 world vertices in one call"};
         if (n == 0) {
             out->loop_offset[0] = 0;
             if (out->seg_offset) out->seg_offset[0] = 0;
@@ -939,40 +960,52 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
         launch_iota_pairs(A.pi.p, A.pj.p, n, st);
         HIPC(hipMemcpyAsync(A.npairs.p, &n, sizeof(int32_t), hipMemcpyHostToDevice, st));
         A.seg_off.ensure(n); A.tri_checks.ensure(n); A.tri_tests.ensure(n); A.nseg.ensure(n); A.tri_hits.ensure(n); A.tri_scanned.ensure(n);
-        A.arena_top.ensure(1); A.queue.ensure(1); A.bcnt.ensure(64); A.bcur.ensure(64);
+        A.queue.ensure(2); A.bcnt.ensure(64); A.bcur.ensure(64);
+        A.hit_off.ensure(n); A.hit_top.ensure(1);
         A.ncontrib.ensure(n); A.comps.ensure(n); A.loops_raw.ensure(n); A.tstatus.ensure(n);
         A.topo_list.ensure(n); A.topo_n.ensure(1);
         const bool batched = opts->mode == ZDX_MODE_BATCHED;
         const int groups = std::min<int>(persistent_groups(c, 8), std::max(1, n));
-        A.list_i.ensure((size_t)groups * 2 * c->max_tris);
+        A.list_i.ensure((size_t)groups * 8 * c->max_tris);
         A.list_f.ensure((size_t)groups * 12 * c->max_tris);
         const bool want_vtx = out->loop_vtx != nullptr;
         for (int attempt = 0; attempt < 10; ++attempt) {
-            A.arena.ensure((size_t)A.arena_cap * 6);
+            A.arena.ensure((size_t)A.hit_cap * 6);
+            A.hits.ensure(6 * (size_t)A.hit_cap);
+            A.hit_flag.ensure((size_t)A.hit_cap);
             const bool sens = opts->sensitivities != 0;
-            A.stage.ensure((size_t)groups * (sens ? 8 : 6) * A.stage_cap);
             DPairs P{};
             P.cap = n; P.n = A.npairs.p; P.pi = A.pi.p; P.pj = A.pj.p; P.order = batched ? A.order.p : nullptr;
             P.seg_off = A.seg_off.p; P.nseg = A.nseg.p; P.tri_checks = A.tri_checks.p; P.tri_tests = A.tri_tests.p;
-            P.tri_hits = A.tri_hits.p; P.tri_scanned = A.tri_scanned.p; P.arena = A.arena.p; P.arena_cap = A.arena_cap; P.arena_top = A.arena_top.p;
+            P.tri_hits = A.tri_hits.p; P.tri_scanned = A.tri_scanned.p; P.arena = A.arena.p; P.arena_cap = A.hit_cap;
             P.status = A.status.p;
             P.topo_list = A.topo_list.p; P.topo_n = A.topo_n.p;
+            P.hits = A.hits.p; P.hit_cap = A.hit_cap; P.hit_top = A.hit_top.p; P.hit_off = A.hit_off.p;
+            P.hit_flag = A.hit_flag.p;
             if (sens) {
-                A.prov.ensure((size_t)A.arena_cap * 4);
+                A.hit_tri.ensure(2 * (size_t)A.hit_cap);
+                P.hit_tri = A.hit_tri.p;
+                A.prov.ensure((size_t)A.hit_cap * 4);
                 P.prov = A.prov.p;
             }
             launch_fill_i32(A.status.p, 0, 1, st);
             launch_fill_i32(A.topo_n.p, 0, 1, st);
-            launch_fill_u64(A.arena_top.p, 0ull, 1, st);
-            launch_fill_i32((int32_t*)A.queue.p, 0, 1, st);
             if (batched) launch_lpt_order(P, M, A.mesh.p, A.order.p, A.bcnt.p, A.bcur.p, n, st);
             NarrowCfg nc{256, 4, groups, batched ? 1 : 0};
-            launch_narrow(B, M, P, nc, A.list_i.p, A.list_f.p, A.stage.p, A.stage_cap, c->max_tris, A.queue.p, st);
+            launch_narrow(B, M, P, nc, A.list_i.p, A.list_f.p, A.stage_cap, c->max_tris, A.queue.p, st);
             int32_t status = 0;
             HIPC(hipMemcpyAsync(&status, A.status.p, 4, hipMemcpyDeviceToHost, st));
             HIPC(hipStreamSynchronize(st));
-            if (status & ST_STAGE) { A.stage_cap *= 4; continue; }
-            if (status & ST_ARENA) { A.arena_cap *= 2; continue; }
+            if (status & (ST_STAGE | ST_HITS)) {
+                if (status & ST_STAGE) A.stage_cap *= 4;
+                if (status & ST_HITS) {   // the hit counter holds what every pair asked for
+                    unsigned long long want = 0;
+                    HIPC(hipMemcpy(&want, A.hit_top.p, sizeof(want), hipMemcpyDeviceToHost));
+                    A.hit_cap *= 2;
+                    while (A.hit_cap < (int64_t)(want + want / 4)) A.hit_cap *= 2;
+                }
+                continue;
+            }
             // topology with the minimal-energy force
             TopoBufs T{};
             T.lmax = A.lmax; T.ncontrib = A.ncontrib.p; T.comps = A.comps.p; T.loops_raw = A.loops_raw.p;
@@ -983,15 +1016,15 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
             T.lds_bytes = 32 * 1024;
             T.vmax = 4 * A.stage_cap + 16;
             T.k_stiff = opts->k;
-            A.tw.ensure_arena(A.arena_cap);
+            A.tw.ensure_arena(A.hit_cap);
             A.tw.ensure_items(n);
             A.tw.ensure_slices(T.nthreads, T.seg_max);
             A.tw.fill(T);
             A.rec.ensure((size_t)n * T.lmax * 28);
             if (want_vtx) A.vtx.ensure((size_t)n * T.vmax * 3);
             if (sens) {
-                A.ej.ensure((size_t)A.arena_cap * 72);
-                A.vj.ensure((size_t)A.arena_cap * 72);
+                A.ej.ensure((size_t)A.hit_cap * 72);
+                A.vj.ensure((size_t)A.hit_cap * 72);
                 A.ljac.ensure((size_t)n * T.lmax * 72);
                 T.ej = A.ej.p; T.vj = A.vj.p; T.ljac = A.ljac.p;
             }
@@ -1047,8 +1080,8 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
             std::vector<double> arena;
             if (out->segments && nseg > 0) {
                 unsigned long long top = 0;
-                HIPC(hipMemcpy(&top, A.arena_top.p, 8, hipMemcpyDeviceToHost));
-                arena.resize((size_t)top * 6);
+                HIPC(hipMemcpy(&top, A.hit_top.p, 8, hipMemcpyDeviceToHost));
+                arena.resize((size_t)std::min<unsigned long long>(top, (unsigned long long)A.hit_cap) * 6);
                 HIPC(hipMemcpy(arena.data(), A.arena.p, arena.size() * 8, hipMemcpyDeviceToHost));
             }
             int64_t lo = 0, vo = 0, sg = 0;
@@ -1433,7 +1466,7 @@ zdx_status zdx_sim_autotune(zdx_sim* s, int32_t* out_block, int32_t* out_groups)
         launch_need_list(s->N, s->d_state[par].p, s->d_radius.p, s->d_cnt_all.p, s->wallbufs(), s->d_need_flag.p,
                          s->d_need_off.p, s->d_scan_tmp.p, s->d_need_list.p, s->d_n_need.p, s->d_wall_flag.p,
                      s->d_wall_off.p, s->d_wall_list.p, s->d_n_wall.p, st);
-        launch_transform(B, M, st, nullptr, s->d_need_list.p, s->d_n_need.p, s->transform_groups());
+        launch_transform(B, M, st, s->d_need_list.p, s->d_n_need.p, s->transform_groups());
         DPairs P = s->dpairs(par, true);
         launch_lpt_order(P, M, s->d_mesh.p, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st);
         const int cand[][2] = {{128, 4}, {128, 8}, {256, 1}, {256, 2}, {256, 4}, {256, 8}, {512, 2}, {512, 4}};
@@ -1449,12 +1482,10 @@ zdx_status zdx_sim_autotune(zdx_sim* s, int32_t* out_block, int32_t* out_groups)
                 NarrowCfg nc{cb[0], cb[1], persistent_groups(s->ctx, g), 1};
                 float tot = 0;
                 for (int rep = 0; rep < 2; ++rep) {
-                    launch_fill_u64(s->d_arena_top.p, 0ull, 1, st);
-                    launch_fill_i32((int32_t*)s->d_queue.p, 0, 1, st);
                     launch_fill_i32(s->d_topo_n.p, 0, 1, st);
                     HIPC(hipEventRecord(e0, st));
-                    launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->d_stage.p, s->stage_cap,
-                                  s->tmax, s->d_queue.p, st);
+                    launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->stage_cap, s->tmax, s->d_queue.p,
+                                  st);
                     HIPC(hipEventRecord(e1, st));
                     HIPC(hipEventSynchronize(e1));
                     float ms = 0;

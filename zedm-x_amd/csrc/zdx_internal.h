@@ -138,23 +138,30 @@ struct DPairs {
     int64_t* tri_tests;      // per pair (executed AABB tests)
     int32_t* tri_hits;       // per pair
     int32_t* tri_scanned;    // per pair: TA + TB
-    double* arena;           // segments, 6 doubles each
-    int64_t arena_cap;       // in segments
+    double* arena;           // segments, 6 doubles each, one slot per hit slot
+    int64_t arena_cap;       // in segments (= hit_cap)
     int32_t* topo_list;      // pairs with at least one segment (appended by the narrow phase)
     int32_t* topo_n;         // device count of topo_list
-    unsigned long long* arena_top;
     int32_t* status;         // global error bits
     int32_t* prov;           // optional (sensitivity runs): 4 ints per arena segment: ta, tb, endpoint codes
+    int32_t* hits;           // hit arena: per triangle pair that passed the float AABB sweep, the six
+                             // world-vertex indices of (ta, tb); its slots are also the segment slots
+    int32_t* hit_tri;        // optional (sensitivity runs): (ta, tb) per hit
+    int64_t hit_cap;         // in hits
+    unsigned long long* hit_top;
+    int64_t* hit_off;        // per pair: first hit (count in tri_hits); the pair's segments start there
+    int32_t* hit_flag;       // per hit: 1 when the exact test produced a segment
 };
 
 enum : int32_t {
-    ST_ARENA = 1,        // segment arena full
+    ST_ARENA = 1,        // (no longer raised: segment slots are the hit slots)
     ST_STAGE = 2,        // a pair produced more segments than the staging capacity
     ST_TOPO = 4,         // topology scratch or loop capacity exceeded
     ST_PAIRS = 8,        // broad-phase pair capacity exceeded
     ST_NEIGH = 16,       // more neighbours than the per-particle candidate capacity
     ST_WALL = 32,        // wall section capacity exceeded
-    ST_GRID = 64         // broad-phase grid could not be built (non-finite positions)
+    ST_GRID = 64,        // broad-phase grid could not be built (non-finite positions)
+    ST_HITS = 128        // hit arena full
 };
 
 }  // namespace zdx

@@ -132,41 +132,41 @@ struct NarrowCfg {
     int32_t use_queue;         // 1: atomic work queue (batched)
 };
 
-// kt: optional kernel-span stamps (2 x unsigned long long per stage: earliest
-// start, latest end on the device clock); NULL disables them.
-void launch_kt_reset(unsigned long long* kt, int n_stages, hipStream_t s);
+// Writes the device clock into *slot when the stream reaches it (NULL: no-op).
+void launch_stamp(unsigned long long* slot, hipStream_t s);
 // list/n_list: bodies to transform (device list and count; NULL = all bodies)
-void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s, unsigned long long* kt = nullptr,
-                      const int32_t* list = nullptr, const int32_t* n_list = nullptr, int32_t groups = 0);
+void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s, const int32_t* list = nullptr, const int32_t* n_list = nullptr, int32_t groups = 0);
 // bodies that are in a candidate pair or near a wall triangle -> list
 // (and the particles near a wall triangle -> wlist)
 void launch_need_list(int32_t n, const double* state, const double* radius, const int32_t* cnt_all, const WallBufs& W,
                       int32_t* flag, int32_t* off, int32_t* scan_tmp, int32_t* list, int32_t* n_list, int32_t* wflag,
                       int32_t* woff, int32_t* wlist, int32_t* n_wlist, hipStream_t s);
-void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s, unsigned long long* kt = nullptr);
+void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s);
+// Narrow phase: cull + sweep into the hit arena, the exact tests per hit, and
+// the per-pair compaction into the segment arena.  seg_cap: most segments one
+// pair may store (ST_STAGE beyond); queue: the cull's work counter; resets
+// the queue, P.hit_top and P.exact_n itself.
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* list_i,
-                   float* list_f, double* stage, int32_t stage_cap, int32_t tmax, unsigned int* queue,
-                   hipStream_t s, unsigned long long* kt = nullptr);
+                   float* list_f, int32_t seg_cap, int32_t tmax, unsigned int* queue, hipStream_t s);
 void launch_lpt_order(const DPairs& P, const DMeshes& M, const int32_t* body_mesh, int32_t* order, int32_t* bucket_cnt,
                       int32_t* bucket_cur, int32_t pair_cap, hipStream_t s);
 // zero outputs of every pair (the topology kernels then visit only the
 // pairs of P.topo_list); cur_active may be NULL
 void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, int32_t pair_cap, hipStream_t s);
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
-                     const TopoBufs& T, const SimParams& sp, hipStream_t s, unsigned long long* kt = nullptr);
+                     const TopoBufs& T, const SimParams& sp, hipStream_t s);
 void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int32_t split,
                      hipStream_t s);
 // one wave per particle of `list` (particles near a wall triangle); scr holds
 // `groups` slices of 3 pts_cap + 4 (pts_cap + 8) doubles
 void launch_walls(const DBodies& B, const DMeshes& M, const DMat& mat, const WallBufs& W, const HistBufs& H,
                   double* wcontrib, int32_t* wncontrib, const int32_t* list, const int32_t* n_list, double* scr,
-                  int64_t groups, int32_t pts_cap, const SimParams& sp, int32_t* status, hipStream_t s,
-                  unsigned long long* kt = nullptr);
+                  int64_t groups, int32_t pts_cap, const SimParams& sp, int32_t* status, hipStream_t s);
 void launch_fold_integrate(int32_t n, const double* state_in, double* state_out, const DMat& mat,
                            const int32_t* adj_off, const int32_t* adj_j, const int32_t* adj_pair,
                            int32_t adj_cap, const TopoBufs& T, int32_t n_walls, const double* wcontrib, const int32_t* wncontrib,
                            int32_t max_wall_tris, double* F, double* Tq, int32_t* counts, const SimParams& sp,
-                           hipStream_t s, unsigned long long* kt = nullptr);
+                           hipStream_t s);
 // counters[ZDX_N_COUNTERS] (int64) reduced from per-pair / per-wall arrays
 void launch_counters(const DPairs& P, const TopoBufs& T, int32_t n_pw, const int32_t* wncontrib,
                      unsigned long long* counters, hipStream_t s);

@@ -88,19 +88,12 @@ __device__ inline int block_excl(int v, int* s_wave, int& total) {
 
 __device__ inline V3 ld3(const double* p) { return v3(p[0], p[1], p[2]); }
 
-// Kernel span stamps: kt[0] = earliest start (first workgroup), kt[1] =
-// latest end (last wave), read from the constant-rate device clock; vector
-// atomics on global memory, so they work the same inside a captured graph.
-__device__ inline void kt_start(unsigned long long* kt) {
-    if (kt && threadIdx.x == 0) atomicMin(kt, (unsigned long long)wall_clock64());
-}
-__device__ inline void kt_end(unsigned long long* kt) {
-    if (kt && lane_id() == 0) atomicMax(kt + 1, (unsigned long long)wall_clock64());
-}
-__global__ void k_kt_reset(unsigned long long* kt, int n) {
-    int k = threadIdx.x;
-    if (k < n) { kt[2 * k] = ~0ull; kt[2 * k + 1] = 0ull; }
-}
+// Stage stamps: one thread writes the device's constant-rate clock into a
+// slot; launched on the stream between stages, so consecutive stamps bracket
+// a stage (kernels of a stream run in order, also inside a captured graph).
+// A per-kernel scheme with an atomic from every wave would serialise tens of
+// thousands of same-address atomics per launch.
+__global__ void k_stamp(unsigned long long* slot) { *slot = (unsigned long long)wall_clock64(); }
 
 // ------------------------------------------------------------ transform
 // World vertices of the listed bodies.  quat_rotate (main.cpp:93-98)
@@ -114,9 +107,7 @@ __global__ void k_kt_reset(unsigned long long* kt, int n) {
 // One wave per body of the list (4 bodies per 256-thread workgroup): world
 // vertices, AABB, and the count of non-degenerate triangles (only the mesh's
 // suspect triangles can degenerate, unless the body is far from the origin).
-__global__ void __launch_bounds__(256) k_transform(DBodies B, DMeshes M, const int32_t* list, const int32_t* n_list,
-                                                   unsigned long long* kt) {
-    kt_start(kt);
+__global__ void __launch_bounds__(256) k_transform(DBodies B, DMeshes M, const int32_t* list, const int32_t* n_list) {
     const int nb = list ? *n_list : B.n;
     const int wpb = blockDim.x / WAVE;
     const int ln = lane_id();
@@ -170,7 +161,6 @@ __global__ void __launch_bounds__(256) k_transform(DBodies B, DMeshes M, const i
         }
         if (ln == 0) B.nondeg[b] = T - bad;
     }
-    kt_end(kt);
 }
 
 // ------------------------------------------------------------ scan
@@ -257,8 +247,7 @@ constexpr int NEIGH_CAP = 128;
 // Bounds of the particle positions and the largest radius: BP_PARTS
 // workgroups reduce a strided share each, then one workgroup combines them.
 __global__ void __launch_bounds__(256) k_bp_bounds_part(const double* state, const double* radius, int n,
-                                                        double* part, unsigned long long* kt) {
-    kt_start(kt);
+                                                        double* part) {
     double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double* p = state + 16 * (int64_t)i;
@@ -488,8 +477,7 @@ __global__ void __launch_bounds__(256) k_bp_adjfix(BroadPhaseBufs bp) {
     }
 }
 
-__global__ void k_bp_totals(BroadPhaseBufs bp, unsigned long long* kt) {
-    kt_end(kt);
+__global__ void k_bp_totals(BroadPhaseBufs bp) {
     int np = bp.pair_off[bp.n], na = bp.adj_off[bp.n];
     // on overflow the pair and adjacency lists were not written: later stages
     // see no pairs (the fold checks adj_off[n] against the capacity) and the
@@ -594,29 +582,32 @@ __device__ inline unsigned vertex_outcode(const double* w, const float* bx) {
 constexpr int OC_CAP = 2048;   // vertices per body whose outcodes fit in LDS
 constexpr int CULL_PER_THREAD = 8;
 
-// PROV: also record each segment's provenance (ta, tb, endpoint codes of
-// tri_tri_prov) in P.prov, staged per workgroup after its segments.
-template <int BS, int K, bool PROV>
+// Narrow phase, part 1 (cull and sweep): persistent workgroups, one pair at a
+// time.  Each side's triangles are culled against the other body's AABB, then
+// the flattened (ta, tb) product of the survivors is swept with the float
+// AABB test; the triangle pairs that pass ("hits") are written in (ta, tb)
+// order to the pair's slice of the hit arena, which is reserved with the
+// upper bound nA * nB so that every pair's hits are contiguous.  Pairs with
+// hits are listed for part 2 (k_narrow_exact).
+template <int BS, int K>
 __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, int32_t* list_i, float* list_f,
-                                               double* stage, int32_t stage_cap, int32_t tmax, int32_t use_queue,
-                                               unsigned int* queue, int32_t lds_b, unsigned long long* kt) {
-    // lds_b: body B's culled boxes and triangle ids (28 bytes per triangle,
-    // read for every item of the sweep) live in dynamic LDS
+                                               int32_t tmax, int32_t use_queue, unsigned int* queue, int32_t lds_b) {
+    // lds_b: body B's culled float boxes (24 bytes per triangle, read for
+    // every item of the sweep) live in dynamic LDS.  Per workgroup in global
+    // memory: the kept triangle ids of both sides (LA, LB), their world-vertex
+    // indices (VA3, VB3, 3 each) and A's float boxes (FA).
     extern __shared__ char s_nb[];
-    kt_start(kt);
     __shared__ int s_wave[BS / WAVE];
     __shared__ int s_item;
-    __shared__ int s_ha[BS * K];
-    __shared__ int s_hb[BS * K];
     __shared__ unsigned char s_oc[2 * OC_CAP];
     __shared__ unsigned long long s_base;
     const int tid = threadIdx.x;
-    int32_t* LA = list_i + (int64_t)blockIdx.x * 2 * tmax;
-    int32_t* LB = lds_b ? (int32_t*)(s_nb + 24 * (int64_t)tmax) : LA + tmax;
+    int32_t* LA = list_i + (int64_t)blockIdx.x * 8 * tmax;
+    int32_t* LB = LA + tmax;
+    int32_t* VA3 = LA + 2 * (int64_t)tmax;
+    int32_t* VB3 = LA + 5 * (int64_t)tmax;
     float* FA = list_f + (int64_t)blockIdx.x * 12 * tmax;
     float* FB = lds_b ? (float*)s_nb : FA + 6 * tmax;
-    double* ST = stage + (int64_t)blockIdx.x * (PROV ? 8 : 6) * stage_cap;
-    int32_t* STP = (int32_t*)(ST + 6 * (int64_t)stage_cap);   // PROV: 4 ints per staged segment
     const int npairs = *P.n;
     int item = blockIdx.x;
     NP_START();
@@ -688,18 +679,6 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
                 nA += tot & 0xffff;
                 nB += tot >> 16;
             }
-            __syncthreads();
-            // float boxes of the kept triangles, all lanes in parallel
-            for (int k = tid; k < nA + nB; k += BS) {
-                const bool a = k < nA;
-                const int q = a ? k : k - nA;
-                const int t = a ? LA[q] : LB[q];
-                V3 tr[3]; double lo[3], hi[3];
-                load_tri(a ? wvA : wvB, a ? tiA : tiB, t, tr);
-                tri_box(tr, lo, hi);
-                float* f = a ? FA + 6 * q : FB + 6 * q;
-                box_f(lo, hi, f);
-            }
             NP_MARK(9);
         } else {
         for (int base = 0; base < TA; base += BS) {
@@ -715,10 +694,7 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
             }
             int tot;
             int r = block_rank<BS>(keep, s_wave, tot);
-            if (keep) {
-                LA[nA + r] = t;
-                for (int k = 0; k < 6; ++k) FA[6 * (nA + r) + k] = f[k];
-            }
+            if (keep) LA[nA + r] = t;
             nA += tot;
         }
         NP_MARK(9);
@@ -735,133 +711,132 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
             }
             int tot;
             int r = block_rank<BS>(keep, s_wave, tot);
-            if (keep) {
-                LB[nB + r] = t;
-                for (int k = 0; k < 6; ++k) FB[6 * (nB + r) + k] = f[k];
-            }
+            if (keep) LB[nB + r] = t;
             nB += tot;
         }
         }
         __syncthreads();
-        NP_MARK(10);
-        // ---- ordered sweep of the culled product
-        const int64_t total = (int64_t)nA * nB;
-        int nseg = 0, hits = 0;
-        for (int64_t base = 0; base < total; base += (int64_t)BS * K) {
-            // K consecutive items of the flattened product per thread; hits
-            // as a bit mask (no per-thread arrays), written in item order
-            unsigned hm = 0u;
-            const int64_t g0 = base + (int64_t)tid * K;
-            int ca0 = 0, cb0 = 0;
-            if (g0 < total) {
-                if (total <= 0x7fffffff) {   // 32-bit division when the product fits
-                    ca0 = (int)g0 / nB;
-                    cb0 = (int)g0 - ca0 * nB;
-                } else {
-                    ca0 = (int)(g0 / nB);
-                    cb0 = (int)(g0 - (int64_t)ca0 * nB);
-                }
+        // float boxes and world-vertex indices of the kept triangles, all
+        // lanes in parallel
+        {
+            const int64_t oA = B.wv_off[i], oB = B.wv_off[j];
+            for (int k = tid; k < nA + nB; k += BS) {
+                const bool a = k < nA;
+                const int q = a ? k : k - nA;
+                const int t = a ? LA[q] : LB[q];
+                const int32_t* ix = (a ? tiA : tiB) + 3 * t;
+                const int i0 = ix[0], i1 = ix[1], i2 = ix[2];
+                const double* wv = a ? wvA : wvB;
+                V3 tr[3] = {ld3(wv + 3 * i0), ld3(wv + 3 * i1), ld3(wv + 3 * i2)};
+                double lo[3], hi[3];
+                tri_box(tr, lo, hi);
+                box_f(lo, hi, a ? FA + 6 * q : FB + 6 * q);
+                const int64_t o = a ? oA : oB;
+                int32_t* v3i = (a ? VA3 : VB3) + 3 * q;
+                v3i[0] = (int32_t)(o + i0); v3i[1] = (int32_t)(o + i1); v3i[2] = (int32_t)(o + i2);
             }
-            {
-                int ca = ca0, cb = cb0;
-                float fa[6];
-                for (int k = 0; k < 6; ++k) fa[k] = FA[6 * ca + k];
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    if (g0 + k < total) {
-                        if (ovl_f(fa, FB + 6 * cb)) hm |= 1u << k;
-                        if (++cb == nB) {
-                            cb = 0; ++ca;
-                            if (ca < nA) for (int q = 0; q < 6; ++q) fa[q] = FA[6 * ca + q];
-                        }
-                    }
-                }
-            }
-            const int myh = __popc(hm);
-            int nh;
-            int off = block_excl<BS>(myh, s_wave, nh);
-            {
-                int ca = ca0, cb = cb0;
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    if (hm & (1u << k)) { s_ha[off] = ca; s_hb[off] = cb; ++off; }
-                    if (++cb == nB) { cb = 0; ++ca; }
-                }
-            }
-            __syncthreads();
-            NP_MARK(11);
-            hits += nh;
-            for (int hb = 0; hb < nh; hb += BS) {
-                const int h = hb + tid;
-                bool sg = false;
-                V3 p0, p1;
-                int pv[4] = {0, 0, 0, 0};
-                if (h < nh) {
-                    const int ta = LA[s_ha[h]], tb = LB[s_hb[h]];
-                    V3 A[3], Bt[3];
-                    load_tri(wvA, tiA, ta, A);
-                    load_tri(wvB, tiB, tb, Bt);
-                    double alo[3], ahi[3], blo[3], bhi[3];
-                    tri_box(A, alo, ahi);
-                    tri_box(Bt, blo, bhi);
-                    // exact reference reject (main.cpp:2130-2132)
-                    bool rej = ahi[0] < blo[0] || alo[0] > bhi[0] || ahi[1] < blo[1] || alo[1] > bhi[1] ||
-                               ahi[2] < blo[2] || alo[2] > bhi[2];
-                    if (!rej) {
-                        V3 nA3, nB3; double dA, dB;
-                        plane_of(A[0], A[1], A[2], nA3, dA);
-                        plane_of(Bt[0], Bt[1], Bt[2], nB3, dB);
-                        if (PROV) {
-                            pv[0] = ta; pv[1] = tb;
-                            sg = tri_tri_prov(A, nA3, dA, Bt, nB3, dB, p0, p1, pv[2], pv[3]);
-                        } else {
-                            sg = tri_tri(A, nA3, dA, Bt, nB3, dB, p0, p1);
-                        }
-                    }
-                }
-                int ns;
-                int r = block_rank<BS>(sg, s_wave, ns);
-                if (sg && nseg + r < stage_cap) {
-                    double* o = ST + 6 * (int64_t)(nseg + r);
-                    o[0] = p0.x; o[1] = p0.y; o[2] = p0.z; o[3] = p1.x; o[4] = p1.y; o[5] = p1.z;
-                    if (PROV)
-                        for (int q = 0; q < 4; ++q) STP[4 * (nseg + r) + q] = pv[q];
-                }
-                nseg += ns;
-            }
-            __syncthreads();
-            NP_MARK(12);
-        }
-        // ---- publish: allocate arena space, copy the ordered segments
-        int nstore = nseg;
-        if (nseg > stage_cap) {
-            if (tid == 0) atomicOr(P.status, (int)ST_STAGE);
-            nstore = 0;
-        }
-        if (tid == 0) {
-            unsigned long long bse = nstore > 0 ? atomicAdd(P.arena_top, (unsigned long long)nstore) : 0ull;
-            if (bse + (unsigned long long)nstore > (unsigned long long)P.arena_cap) {
-                atomicOr(P.status, (int)ST_ARENA);
-                bse = ~0ull;
-            }
-            s_base = bse;
         }
         __syncthreads();
-        const unsigned long long bse = s_base;
-        if (bse != ~0ull) {
-            double* dst = P.arena + 6 * (int64_t)bse;
-            for (int k = tid; k < 6 * nstore; k += BS) dst[k] = ST[k];
-            if (PROV)
-                for (int k = tid; k < 4 * nstore; k += BS) P.prov[4 * (int64_t)bse + k] = STP[k];
+        NP_MARK(10);
+        // ---- ordered sweep of the culled product into the hit arena.  K
+        // consecutive items of the flattened product per thread, hits as a bit
+        // mask.  A pair of one round keeps its mask from the counting scan;
+        // longer pairs count first, then sweep again to write, so the arena
+        // holds exactly the hits.
+        const int64_t total = (int64_t)nA * nB;
+        auto sweep = [&](int64_t base, int& ca0, int& cb0) -> unsigned {
+            unsigned hm = 0u;
+            const int64_t g0 = base + (int64_t)tid * K;
+            ca0 = 0; cb0 = 0;
+            if (g0 >= total) return 0u;
+            if (total <= 0x7fffffff) {   // 32-bit division when the product fits
+                ca0 = (int)g0 / nB;
+                cb0 = (int)g0 - ca0 * nB;
+            } else {
+                ca0 = (int)(g0 / nB);
+                cb0 = (int)(g0 - (int64_t)ca0 * nB);
+            }
+            int ca = ca0, cb = cb0;
+            float fa[6];
+            for (int k = 0; k < 6; ++k) fa[k] = FA[6 * ca + k];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (g0 + k < total) {
+                    if (ovl_f(fa, FB + 6 * cb)) hm |= 1u << k;
+                    if (++cb == nB) {
+                        cb = 0; ++ca;
+                        if (ca < nA) for (int q = 0; q < 6; ++q) fa[q] = FA[6 * ca + q];
+                    }
+                }
+            }
+            return hm;
+        };
+        // a hit record: the six world-vertex indices of (ta, tb); with
+        // provenance also (ta, tb) itself
+        auto emit = [&](unsigned hm, int ca, int cb, int64_t at) {
+            if (!hm) return;
+            int2* dst = (int2*)(P.hits + 6 * at);
+            int2* dt = P.hit_tri ? (int2*)P.hit_tri + at : nullptr;
+            int a0 = VA3[3 * ca], a1 = VA3[3 * ca + 1], a2 = VA3[3 * ca + 2];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (hm & (1u << k)) {
+                    const int32_t* vb = VB3 + 3 * cb;
+                    dst[0] = make_int2(a0, a1);
+                    dst[1] = make_int2(a2, vb[0]);
+                    dst[2] = make_int2(vb[1], vb[2]);
+                    dst += 3;
+                    if (dt) *dt++ = make_int2(LA[ca], LB[cb]);
+                }
+                if (++cb == nB) {
+                    cb = 0; ++ca;
+                    if (ca < nA) { a0 = VA3[3 * ca]; a1 = VA3[3 * ca + 1]; a2 = VA3[3 * ca + 2]; }
+                }
+            }
+        };
+        const bool one_round = total <= (int64_t)BS * K;
+        int ca1 = 0, cb1 = 0, cnt = 0;
+        unsigned hm1 = 0u;
+        for (int64_t base = 0; base < total; base += (int64_t)BS * K) {
+            hm1 = sweep(base, ca1, cb1);
+            cnt += __popc(hm1);
         }
+        int hits;
+        const int off1 = block_excl<BS>(cnt, s_wave, hits);
         if (tid == 0) {
-            P.seg_off[p] = bse != ~0ull ? (int64_t)bse : 0;
-            P.nseg[p] = bse != ~0ull ? nstore : 0;
-            if (bse != ~0ull && nstore > 0) P.topo_list[atomicAdd(P.topo_n, 1)] = p;
+            unsigned long long hb = hits > 0 ? atomicAdd(P.hit_top, (unsigned long long)hits) : 0ull;
+            if (hb + (unsigned long long)hits > (unsigned long long)P.hit_cap) {
+                atomicOr(P.status, (int)ST_HITS);
+                hb = ~0ull;
+            }
+            s_base = hb;
+        }
+        __syncthreads();
+        const unsigned long long hbase = s_base;
+        if (hbase != ~0ull && hits > 0) {
+            if (one_round) {
+                emit(hm1, ca1, cb1, (int64_t)hbase + off1);
+            } else {
+                int done = 0;
+                for (int64_t base = 0; base < total; base += (int64_t)BS * K) {
+                    int ca0, cb0, nh;
+                    const unsigned hm = sweep(base, ca0, cb0);
+                    const int off = block_excl<BS>(__popc(hm), s_wave, nh);
+                    emit(hm, ca0, cb0, (int64_t)hbase + done + off);
+                    done += nh;
+                }
+            }
+        }
+        NP_MARK(11);
+        if (tid == 0) {
+            const bool ok = hbase != ~0ull;
+            P.hit_off[p] = ok ? (int64_t)hbase : 0;
+            P.tri_hits[p] = ok ? hits : 0;
             P.tri_checks[p] = (int64_t)B.nondeg[i] * TB;
             P.tri_tests[p] = total;
-            P.tri_hits[p] = hits;
             P.tri_scanned[p] = TA + TB;
+            P.seg_off[p] = 0;   // set by the compaction for pairs with hits
+            P.nseg[p] = 0;
         }
         __syncthreads();
         NP_MARK(13);
@@ -870,7 +845,99 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
 #endif
         if (!use_queue) item += gridDim.x;
     }
-    kt_end(kt);
+}
+
+// Narrow phase, part 2 (exact tests): one thread per hit of the hit arena.
+// Each hit is re-tested with the reference's exact AABB reject
+// (main.cpp:2130-2132) and, when it passes, with the triangle-triangle
+// intersection (main.cpp:579-631); the result goes to the hit's own slot: the
+// flag, the segment in the segment arena (whose slots are the hit slots), and
+// with provenance (ta, tb, endpoint codes of tri_tri_prov).
+template <bool PROV>
+__global__ void __launch_bounds__(256) k_narrow_exact(DBodies B, DPairs P) {
+    // after a hit-arena overflow part of the arena was never written: test
+    // nothing (the step is run again with a larger arena)
+    const int64_t nh = (*P.status & (int)ST_HITS) ? 0 : (int64_t)*P.hit_top;
+    const int64_t lim = nh < P.hit_cap ? nh : P.hit_cap;
+    for (int64_t h = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; h < lim; h += (int64_t)gridDim.x * blockDim.x) {
+        const int2* rec = (const int2*)(P.hits + 6 * h);
+        const int2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+        V3 A[3] = {ld3(B.wv + 3 * (int64_t)r0.x), ld3(B.wv + 3 * (int64_t)r0.y), ld3(B.wv + 3 * (int64_t)r1.x)};
+        V3 Bt[3] = {ld3(B.wv + 3 * (int64_t)r1.y), ld3(B.wv + 3 * (int64_t)r2.x), ld3(B.wv + 3 * (int64_t)r2.y)};
+        double alo[3], ahi[3], blo[3], bhi[3];
+        tri_box(A, alo, ahi);
+        tri_box(Bt, blo, bhi);
+        // exact reference reject (main.cpp:2130-2132)
+        const bool rej = ahi[0] < blo[0] || alo[0] > bhi[0] || ahi[1] < blo[1] || alo[1] > bhi[1] ||
+                         ahi[2] < blo[2] || alo[2] > bhi[2];
+        bool sg = false;
+        V3 p0, p1;
+        int c0 = 0, c1 = 0;
+        if (!rej) {
+            V3 nA3, nB3; double dA, dB;
+            plane_of(A[0], A[1], A[2], nA3, dA);
+            plane_of(Bt[0], Bt[1], Bt[2], nB3, dB);
+            if (PROV) sg = tri_tri_prov(A, nA3, dA, Bt, nB3, dB, p0, p1, c0, c1);
+            else sg = tri_tri(A, nA3, dA, Bt, nB3, dB, p0, p1);
+        }
+        P.hit_flag[h] = sg ? 1 : 0;
+        if (sg) {
+            double* o = P.arena + 6 * h;
+            o[0] = p0.x; o[1] = p0.y; o[2] = p0.z; o[3] = p1.x; o[4] = p1.y; o[5] = p1.z;
+            if (PROV) {
+                const int2 t = ((const int2*)P.hit_tri)[h];
+                ((int4*)P.prov)[h] = make_int4(t.x, t.y, c0, c1);
+            }
+        }
+    }
+}
+
+// Narrow phase, part 3: one wave per pair compacts its segments in hit
+// order, i.e. in the reference's (ta, tb) order, to the front of its hit
+// range (in place: a segment only moves to a lower slot, and each round of 64
+// reads its slots before it writes).  A pair with more than seg_cap segments
+// stores none and raises ST_STAGE.
+template <bool PROV>
+__global__ void __launch_bounds__(256) k_narrow_compact(DPairs P, int32_t seg_cap) {
+    const int lane = lane_id();
+    const int npairs = (*P.status & (int)ST_HITS) ? 0 : *P.n;
+    const int wpb = blockDim.x / WAVE;
+    for (int p = blockIdx.x * wpb + (int)threadIdx.x / WAVE; p < npairs; p += gridDim.x * wpb) {
+        const int nh = P.tri_hits[p];
+        if (nh == 0) continue;
+        const int64_t h0 = P.hit_off[p];
+        int nseg = 0;
+        for (int b = 0; b < nh; b += WAVE) {
+            const int64_t h = h0 + b + lane;
+            const bool sg = b + lane < nh && P.hit_flag[h] != 0;
+            double sv[6];
+            int4 pv = make_int4(0, 0, 0, 0);
+            if (sg) {
+                for (int q = 0; q < 6; ++q) sv[q] = P.arena[6 * h + q];
+                if (PROV) pv = ((const int4*)P.prov)[h];
+            }
+            const unsigned long long m = __ballot(sg);
+            if (sg) {
+                const int r = nseg + __popcll(m & ((1ull << lane) - 1ull));
+                if (r < seg_cap) {
+                    double* dst = P.arena + 6 * (h0 + r);
+                    for (int q = 0; q < 6; ++q) dst[q] = sv[q];
+                    if (PROV) ((int4*)P.prov)[h0 + r] = pv;
+                }
+            }
+            nseg += __popcll(m);
+        }
+        if (lane == 0) {
+            int nstore = nseg;
+            if (nstore > seg_cap) {
+                atomicOr(P.status, (int)ST_STAGE);
+                nstore = 0;
+            }
+            P.seg_off[p] = h0;
+            P.nseg[p] = nstore;
+            if (nstore > 0) P.topo_list[atomicAdd(P.topo_n, 1)] = p;
+        }
+    }
 }
 
 // LPT order: pairs by descending cost bucket floor(log2(TA * TB)), so the
@@ -1020,10 +1087,9 @@ constexpr int TP_PASS_A_THREADS = 1024;
 // one thread per row then compacts; the snap tests run flat, each thread over
 // a contiguous range of endpoint pairs.
 __global__ void __launch_bounds__(TP_PASS_A_THREADS) k_tp_pass_a(DPairs P, DMeshes M, DBodies B, TopoBufs T,
-                                                                int32_t lds_segs, unsigned long long* kt) {
+                                                                int32_t lds_segs) {
     extern __shared__ char s_pa[];
     __shared__ long long s_off[2];
-    kt_start(kt);
     const int n = *P.topo_n;
     const int R = T.adj_row;
     const int NT = blockDim.x;
@@ -1248,8 +1314,7 @@ __device__ inline TopoOut tp_walk_d(const LaneDev& wp, const TopoBufs& T, int S,
 }
 
 __global__ void __launch_bounds__(64) k_tp_walk_d_sim(DPairs P, DMeshes M, DBodies B, DMat mat, HistBufs H,
-                                                      TopoBufs T, SimParams sp, int32_t lds_bytes,
-                                                      unsigned long long* kt) {
+                                                      TopoBufs T, SimParams sp, int32_t lds_bytes) {
     extern __shared__ char s_walk[];
     const int n = *P.topo_n;
     const int nprev = *H.prev_n;
@@ -1299,7 +1364,6 @@ __global__ void __launch_bounds__(64) k_tp_walk_d_sim(DPairs P, DMeshes M, DBodi
         }
         __syncthreads();
     }
-    kt_end(kt);
 }
 
 // Outputs of pairs the topology kernels do not visit (no segments): no
@@ -1397,8 +1461,7 @@ __device__ inline bool wall_tri_candidate(const double* wt9, const V3& pmn, cons
 // Entries of particles not in the list were zeroed before this kernel.
 __global__ void __launch_bounds__(64) k_walls(DBodies B, DMeshes M, DMat mat, WallBufs W, HistBufs H, double* wcontrib,
                                               int32_t* wnc, const int32_t* list, const int32_t* n_list, double* scr,
-                                              int32_t pts_cap, SimParams sp, int32_t* status, unsigned long long* kt) {
-    kt_start(kt);
+                                              int32_t pts_cap, SimParams sp, int32_t* status) {
     LaneDev wp;
     const int nl = *n_list;
     const int64_t per = 3 * (int64_t)pts_cap + 4 * (int64_t)(pts_cap + 8);
@@ -1455,7 +1518,6 @@ __global__ void __launch_bounds__(64) k_walls(DBodies B, DMeshes M, DMat mat, Wa
         }
         __syncthreads();
     }
-    kt_end(kt);
 }
 
 __global__ void k_walls_clear(int64_t n, int32_t* wnc, uint8_t* wact) {
@@ -1471,11 +1533,9 @@ __global__ void __launch_bounds__(256) k_fold_integrate(int32_t n, const double*
                                                         const int32_t* adj_pair, int32_t adj_cap, TopoBufs T,
                                                         int32_t n_walls,
                                                         const double* wcontrib, const int32_t* wnc, int32_t mwt,
-                                                        double* Fo, double* To, int32_t* counts, SimParams sp,
-                                                        unsigned long long* kt) {
-    kt_start(kt);
+                                                        double* Fo, double* To, int32_t* counts, SimParams sp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) { kt_end(kt); return; }
+    if (i >= n) return;
     // forces[i] and torques[i] start at zero and receive, in the reference's
     // order (main.cpp:2049-2050 pair loop, then walls 2310-2447): pairs with
     // partners in ascending index, loops in order, then walls and their tris
@@ -1521,7 +1581,6 @@ __global__ void __launch_bounds__(256) k_fold_integrate(int32_t n, const double*
     for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) Ib.m[r][c] = mat.Ibinv[9 * (int64_t)i + 3 * r + c];
     integrate(b, Ib, mat.inv_mass[i], F, Tq, v3(sp.g[0], sp.g[1], sp.g[2]), sp.dt);
     store_body(sout + 16 * (int64_t)i, b);
-    kt_end(kt);
 }
 
 // ------------------------------------------------------------ counters
@@ -1599,16 +1658,16 @@ extern "C" int zdx_walk_profile(unsigned long long* out) {
 
 
 // ------------------------------------------------------------ launchers
-void launch_kt_reset(unsigned long long* kt, int n, hipStream_t s) {
-    if (kt) hipLaunchKernelGGL(k_kt_reset, dim3(1), dim3(64), 0, s, kt, n);
+void launch_stamp(unsigned long long* slot, hipStream_t s) {
+    if (slot) hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, s, slot);
 }
 
-void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s, unsigned long long* kt, const int32_t* list,
+void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s, const int32_t* list,
                       const int32_t* n_list, int32_t groups) {
     if (B.n <= 0) return;
     int g = list ? groups : (B.n + 3) / 4;
     if (g < 1) g = 1;
-    hipLaunchKernelGGL(k_transform, dim3(g), dim3(256), 0, s, B, M, list, n_list, kt);
+    hipLaunchKernelGGL(k_transform, dim3(g), dim3(256), 0, s, B, M, list, n_list);
 }
 
 void launch_need_list(int32_t n, const double* state, const double* radius, const int32_t* cnt_all, const WallBufs& W,
@@ -1621,9 +1680,9 @@ void launch_need_list(int32_t n, const double* state, const double* radius, cons
     hipLaunchKernelGGL(k_need_list, dim3(nblk(n, 256)), dim3(256), 0, s, n, wflag, woff, wlist, n_wlist);
 }
 
-void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s, unsigned long long* kt) {
+void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s) {
     const int n = bp.n;
-    hipLaunchKernelGGL(k_bp_bounds_part, dim3(BP_PARTS), dim3(256), 0, s, bp.state, bp.radius, n, bp.part, kt);
+    hipLaunchKernelGGL(k_bp_bounds_part, dim3(BP_PARTS), dim3(256), 0, s, bp.state, bp.radius, n, bp.part);
     hipLaunchKernelGGL(k_bp_bounds, dim3(1), dim3(64), 0, s, bp.part, (int)BP_PARTS, bp.grid, bp.max_cells,
                        bp.status);
     // cell counters for the grid actually in use
@@ -1639,40 +1698,43 @@ void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s, unsigned long lo
     scan_i32(bp.cnt_all, nullptr, n, bp.scan_tmp, bp.adj_off, nullptr, s);
     hipLaunchKernelGGL(k_bp_fill, dim3(nblk(n, 128)), dim3(128), 0, s, bp);
     hipLaunchKernelGGL(k_bp_adjfix, dim3(nblk(n, 256)), dim3(256), 0, s, bp);
-    hipLaunchKernelGGL(k_bp_totals, dim3(1), dim3(64), 0, s, bp, kt);
+    hipLaunchKernelGGL(k_bp_totals, dim3(1), dim3(64), 0, s, bp);
 }
 
 template <int BS, int K>
 static void narrow_t(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
-                     double* st, int32_t sc, int32_t tmax, unsigned int* q, hipStream_t s, unsigned long long* kt) {
-    const size_t lds = 28 * (size_t)tmax;
+                     int32_t tmax, unsigned int* q, hipStream_t s) {
+    const size_t lds = 24 * (size_t)tmax;
     const int use = lds <= 48 * 1024 ? 1 : 0;
     const size_t dyn = use ? lds : 0;
-    if (P.prov)
-        hipLaunchKernelGGL((k_narrow<BS, K, true>), dim3(cfg.groups), dim3(BS), dyn, s, B, M, P, li, lf, st, sc, tmax,
-                           cfg.use_queue, q, use, kt);
-    else
-        hipLaunchKernelGGL((k_narrow<BS, K, false>), dim3(cfg.groups), dim3(BS), dyn, s, B, M, P, li, lf, st, sc, tmax,
-                           cfg.use_queue, q, use, kt);
+    hipLaunchKernelGGL((k_narrow<BS, K>), dim3(cfg.groups), dim3(BS), dyn, s, B, M, P, li, lf, tmax, cfg.use_queue, q,
+                       use);
 }
 
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
-                   double* st, int32_t sc, int32_t tmax, unsigned int* q, hipStream_t s, unsigned long long* kt) {
-    if (P.prov) {   // sensitivity runs: one shape
-        narrow_t<256, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt);
-        return;
-    }
-    const int key = cfg.block * 100 + cfg.items;
+                   int32_t seg_cap, int32_t tmax, unsigned int* q, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int32_t*)q, 0, nullptr, (int64_t)1);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, P.hit_top, 0ull, (int64_t)1);
+    const int key = P.prov ? 25604 : cfg.block * 100 + cfg.items;
     switch (key) {
-        case 12804: narrow_t<128, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
-        case 12808: narrow_t<128, 8>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
-        case 25601: narrow_t<256, 1>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
-        case 25602: narrow_t<256, 2>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
-        case 25604: narrow_t<256, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
-        case 25608: narrow_t<256, 8>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
-        case 51202: narrow_t<512, 2>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
-        case 51204: narrow_t<512, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
-        default: narrow_t<256, 4>(B, M, P, cfg, li, lf, st, sc, tmax, q, s, kt); break;
+        case 12804: narrow_t<128, 4>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        case 12808: narrow_t<128, 8>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        case 25601: narrow_t<256, 1>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        case 25602: narrow_t<256, 2>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        case 25604: narrow_t<256, 4>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        case 25608: narrow_t<256, 8>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        case 51202: narrow_t<512, 2>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        case 51204: narrow_t<512, 4>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        default: narrow_t<256, 4>(B, M, P, cfg, li, lf, tmax, q, s); break;
+    }
+    const int ge = std::max(1, cfg.groups) * 4;   // grid-stride over the hits
+    const int gc = std::max(1, cfg.groups);        // 4 waves per workgroup over the listed pairs
+    if (P.prov) {
+        hipLaunchKernelGGL(k_narrow_exact<true>, dim3(ge), dim3(256), 0, s, B, P);
+        hipLaunchKernelGGL(k_narrow_compact<true>, dim3(gc), dim3(256), 0, s, P, seg_cap);
+    } else {
+        hipLaunchKernelGGL(k_narrow_exact<false>, dim3(ge), dim3(256), 0, s, B, P);
+        hipLaunchKernelGGL(k_narrow_compact<false>, dim3(gc), dim3(256), 0, s, P, seg_cap);
     }
 }
 
@@ -1693,22 +1755,22 @@ void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, i
 }
 
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
-                     const TopoBufs& T, const SimParams& sp, hipStream_t s, unsigned long long* kt) {
+                     const TopoBufs& T, const SimParams& sp, hipStream_t s) {
     launch_fill_u64(T.top, 0ull, 4, s);
     hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(TP_PASS_A_THREADS),
-                       84 * TP_PASS_A_SEGS + TP_PASS_A_KEYS, s, P, M, B, T, (int32_t)TP_PASS_A_SEGS, kt);
+                       84 * TP_PASS_A_SEGS + TP_PASS_A_KEYS, s, P, M, B, T, (int32_t)TP_PASS_A_SEGS);
     hipLaunchKernelGGL(k_tp_walk_b, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, T, sp.split,
                        T.lds_bytes);
     hipLaunchKernelGGL(k_tp_pass_c, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T);
     hipLaunchKernelGGL(k_tp_walk_d_sim, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, mat, H, T, sp,
-                       T.lds_bytes, kt);
+                       T.lds_bytes);
 }
 void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int32_t split,
                      hipStream_t s) {
     launch_fill_u64(T.top, 0ull, 4, s);
     if (T.ej) hipLaunchKernelGGL(k_tp_endpoint_jac, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T);
     hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(TP_PASS_A_THREADS),
-                       84 * TP_PASS_A_SEGS + TP_PASS_A_KEYS, s, P, M, B, T, (int32_t)TP_PASS_A_SEGS, nullptr);
+                       84 * TP_PASS_A_SEGS + TP_PASS_A_KEYS, s, P, M, B, T, (int32_t)TP_PASS_A_SEGS);
     hipLaunchKernelGGL(k_tp_walk_b, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, T, split,
                        T.lds_bytes);
     hipLaunchKernelGGL(k_tp_pass_c, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T);
@@ -1717,22 +1779,21 @@ void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const 
 }
 void launch_walls(const DBodies& B, const DMeshes& M, const DMat& mat, const WallBufs& W, const HistBufs& H,
                   double* wcontrib, int32_t* wnc, const int32_t* list, const int32_t* n_list, double* scr,
-                  int64_t groups, int32_t pts_cap, const SimParams& sp, int32_t* status, hipStream_t s,
-                  unsigned long long* kt) {
+                  int64_t groups, int32_t pts_cap, const SimParams& sp, int32_t* status, hipStream_t s) {
     if (W.n_walls <= 0) return;
     const int64_t n = (int64_t)B.n * W.n_walls;
     hipLaunchKernelGGL(k_walls_clear, dim3(nblk(n, 256) < 4096 ? nblk(n, 256) : 4096), dim3(256), 0, s, n, wnc,
                        H.wcur_active);
     hipLaunchKernelGGL(k_walls, dim3((unsigned)groups), dim3(64), 0, s, B, M, mat, W, H, wcontrib, wnc, list, n_list,
-                       scr, pts_cap, sp, status, kt);
+                       scr, pts_cap, sp, status);
 }
 void launch_fold_integrate(int32_t n, const double* sin, double* sout, const DMat& mat, const int32_t* adj_off,
                            const int32_t* adj_j, const int32_t* adj_pair, int32_t adj_cap, const TopoBufs& T,
                            int32_t n_walls,
                            const double* wcontrib, const int32_t* wnc, int32_t mwt, double* F, double* Tq,
-                           int32_t* counts, const SimParams& sp, hipStream_t s, unsigned long long* kt) {
+                           int32_t* counts, const SimParams& sp, hipStream_t s) {
     hipLaunchKernelGGL(k_fold_integrate, dim3(nblk(n, 256)), dim3(256), 0, s, n, sin, sout, mat, adj_off, adj_j,
-                       adj_pair, adj_cap, T, n_walls, wcontrib, wnc, mwt, F, Tq, counts, sp, kt);
+                       adj_pair, adj_cap, T, n_walls, wcontrib, wnc, mwt, F, Tq, counts, sp);
 }
 void launch_counters(const DPairs& P, const TopoBufs& T, int32_t n_pw, const int32_t* wnc, unsigned long long* ctr,
                      hipStream_t s) {
