@@ -532,6 +532,7 @@ __global__ void __launch_bounds__(256) k_need_list(int32_t n, const int32_t* fla
 // the start of a pair to its first mark); slots 8-14: the narrow phase (pair
 // set-up, cull A, cull B, sweep tests, exact tests, publish, pairs done).
 __device__ unsigned long long g_walk_prof[16];
+__device__ long long g_walk_pairs[12 * 8192];   // walk B per pair: S, R, scratch in LDS, ticks, phases 0-7
 #define NP_MARK(k)                                                      \
     do {                                                                \
         if (threadIdx.x == 0) {                                         \
@@ -1055,11 +1056,17 @@ struct LaneDev {
     }
     __device__ void sync() const { __syncthreads(); }
 #ifdef ZDX_WALK_PROFILE
-    mutable unsigned long long t0 = 0;
-    __device__ void start() const { t0 = wall_clock64(); }
+    // phase ticks accumulate in registers; the kernel stores them per pair
+    mutable unsigned long long t0 = 0, t_begin = 0;
+    mutable unsigned ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    __device__ void start() const {
+        t0 = wall_clock64(); t_begin = t0;
+        for (int k = 0; k < 8; ++k) ph[k] = 0;
+    }
     __device__ void mark(int k) const {
+        asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");   // charge outstanding loads to this phase
         const unsigned long long t = wall_clock64();
-        if (lane_id() == 0) atomicAdd(&g_walk_prof[k], t - t0);
+        if (k < 8) ph[k] += (unsigned)(t - t0);
         t0 = t;
     }
 #else
@@ -1249,6 +1256,16 @@ __global__ void __launch_bounds__(64) k_tp_walk_b(DPairs P, DMeshes M, DBodies B
                                                   T.vj ? T.vj + 72 * sb : nullptr);
             });
         }
+#ifdef ZDX_WALK_PROFILE
+        if (lane_id() == 0 && it < 8192) {
+            const int64_t ib = ((int64_t)tp_walk_b_ints(S, R) * 4 + 7) & ~(int64_t)7;
+            g_walk_pairs[12 * it] = S;
+            g_walk_pairs[12 * it + 1] = R;
+            g_walk_pairs[12 * it + 2] = ib + tp_walk_b_dbls(S) * 8 <= lds_bytes ? 1 : 0;
+            g_walk_pairs[12 * it + 3] = (long long)(wall_clock64() - wp.t_begin);
+            for (int k = 0; k < 8; ++k) g_walk_pairs[12 * it + 4 + k] = wp.ph[k];
+        }
+#endif
         if (lane_id() == 0) {
             long long to = -1;
             if (ncomp >= 0) {
@@ -1648,6 +1665,9 @@ inline int nblk(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 }  // namespace
 
 #ifdef ZDX_WALK_PROFILE
+extern "C" int zdx_walk_pairs(long long* out, int n) {
+    return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_walk_pairs), 12 * (size_t)n * sizeof(long long)) == hipSuccess ? 0 : 1;
+}
 extern "C" int zdx_walk_profile(unsigned long long* out) {
     if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_walk_prof), 16 * sizeof(unsigned long long)) != hipSuccess) return 1;
     unsigned long long z[16] = {0};

@@ -166,15 +166,18 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
             stk[sp++] = seed;
             vis[seed] = 1;
             cord[nord++] = seed;
+            int top = seed;   // stk[sp - 1], held in a register after a push
             while (sp > 0) {
-                const int i = stk[--sp];
+                const int i = top;
+                --sp;
                 const int first = sp;
                 const int* row = adjl + (int64_t)i * R;
                 const int m = adjn[i];
+                const int e0 = ln < R ? row[ln] : 0;   // loaded with the count
                 if (m <= W) {
                     // lanes take the row's entries (ascending j); the accepted,
                     // unvisited ones go on the stack ordered by (scan key, id)
-                    const int e = ln < m ? row[ln] : 0;
+                    const int e = ln < m ? e0 : 0;
                     const int jj = e & 0xFFFFFF, kj = e >> 24;
                     const bool acc = ln < m && !vis[jj];
                     const uint64_t am = wp.ballot(acc);
@@ -191,9 +194,16 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
                         vis[jj] = 1;
                         cord[nord + rank] = jj;
                     }
+                    // the new top: the accepted lane of the highest rank
+                    int tl = 0;
+                    if (na > 0) {
+                        const uint64_t hi = wp.ballot(acc && rank == na - 1);
+                        tl = wp.readlane(jj, ctz64(hi));
+                    }
                     wp.sync();
                     sp += na;
                     nord += na;
+                    top = na > 0 ? tl : (sp > 0 ? stk[sp - 1] : 0);
                     continue;
                 }
                 for (int q = 0; q < m; ++q) {
@@ -215,6 +225,7 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
                     vis[stk[k]] = 1;
                     cord[nord++] = stk[k];
                 }
+                top = sp > 0 ? stk[sp - 1] : 0;
             }
             csz[ncomp] = nord - cbeg[ncomp];
             ++ncomp;
@@ -223,9 +234,9 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
         CompBySizeDesc cmp; cmp.size = SArr<int>{csz, 1};
         ref_sort(SArr<int>{cidx, 1}, ncomp, cmp);
         for (int c = 0; c < ncomp; ++c)
-            for (int k = 0; k < csz[c]; ++k) compof[cord[cbeg[c] + k]] = c;
+            for (int k = ln; k < csz[c]; k += W) compof[cord[cbeg[c] + k]] = c;
     } else {
-        for (int s = 0; s < S; ++s) { cord[s] = s; compof[s] = 0; }
+        for (int s = ln; s < S; s += W) { cord[s] = s; compof[s] = 0; }
         cbeg[0] = 0; csz[0] = S; cidx[0] = 0;
         ncomp = 1;
     }
@@ -238,6 +249,7 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
     int* lab = par;            // 2S
     for (int k = ln; k < n; k += W) lab[k] = k;
     wp.sync();
+    wp.mark(5);
     auto root = [&](int x) {
         int y = lab[x];
         while (y != x) { x = y; y = lab[x]; }
@@ -245,26 +257,54 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
     };
     const int64_t npairs = (int64_t)n * (n - 1) / 2;
     const int64_t nwords = (npairs + 63) >> 6;
+    // the set bits of each round of W words are spread over the lanes through
+    // a list of up to W hit indices (in rnk.., free until the numbering below:
+    // min(W, hits) <= n (n - 1) / 2 and <= 7 n ints are there), so the unions
+    // run on all lanes at once instead of one lane per word
+    int* hl = rnk;
+    auto unite = [&](int64_t t) {
+        int a, b;
+        tp_pair_of(t, a, b);
+        if (compof[a >> 1] != compof[b >> 1]) return;
+        int ra = root(a), rb = root(b);
+        while (ra != rb) {
+            if (ra < rb) { const int x = ra; ra = rb; rb = x; }   // link the larger root under the smaller
+            const int old = wp.atomic_cas(&lab[ra], ra, rb);
+            if (old == ra) break;
+            ra = root(old);
+            rb = root(rb);
+        }
+    };
+    uint64_t nxt = ln < nwords ? snapw[ln] : 0ull;
     for (int64_t wb = 0; wb < nwords; wb += W) {
         const int64_t w = wb + ln;
-        uint64_t word = w < nwords ? snapw[w] : 0ull;
-        while (word) {
-            const int64_t t = (w << 6) + ctz64(word);
-            word &= word - 1;
-            int a, b;
-            tp_pair_of(t, a, b);
-            if (compof[a >> 1] != compof[b >> 1]) continue;
-            int ra = root(a), rb = root(b);
-            while (ra != rb) {
-                if (ra < rb) { const int x = ra; ra = rb; rb = x; }   // link the larger root under the smaller
-                const int old = wp.atomic_cas(&lab[ra], ra, rb);
-                if (old == ra) break;
-                ra = root(old);
-                rb = root(rb);
+        uint64_t word = nxt;
+        nxt = w + W < nwords ? snapw[w + W] : 0ull;   // next round's word in flight meanwhile
+        wp.mark(4);
+        if (W == 1) {   // one lane: take the bits in order
+            while (word) {
+                const int64_t t = (w << 6) + ctz64(word);
+                word &= word - 1;
+                unite(t);
             }
+            continue;
+        }
+        int nbits;
+        int g = wp.scan_excl(__builtin_popcountll(word), nbits);
+        for (int j0 = 0; j0 < nbits; j0 += W) {
+            while (word && g < j0 + W) {
+                hl[g - j0] = (int)(((w - wb) << 6) + ctz64(word));   // index within the round
+                word &= word - 1;
+                ++g;
+            }
+            wp.sync();
+            if (j0 + ln < nbits) unite((wb << 6) + hl[ln]);
+            wp.sync();
         }
     }
+    wp.mark(6);
     wp.sync();
+    wp.mark(7);
     for (int k = ln; k < n; k += W) lab[k] = root(k);
     for (int k = ln; k < n; k += W) rootvid[k] = -1;
     wp.sync();
@@ -314,11 +354,27 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
         }
         if (ln == 0) voff[nv] = acc;
         wp.sync();
-        for (int k = 0; k < np; ++k) {
-            const int pt = ptk(k);
-            mem[vfill[ptv[pt]]++] = pt;
+        // points grouped by vertex in point order: per chunk of W points, each
+        // point's rank among the earlier points of its vertex in the chunk
+        // (lanes with the same vertex found by ballots over the bits of the
+        // vertex id), then the lowest such lane advances the vertex's cursor
+        for (int kb = 0; kb < np; kb += W) {
+            const int k = kb + ln;
+            const bool act = k < np;
+            const int pt = act ? ptk(k) : 0;
+            const int v = act ? ptv[pt] : 0;
+            uint64_t same = wp.ballot(act);
+            for (int bit = 0; (1 << bit) < nv; ++bit) {
+                const bool on = ((v >> bit) & 1) != 0;
+                const uint64_t bm = wp.ballot(act && on);
+                same &= on ? bm : ~bm;
+            }
+            const uint64_t below = same & ((1ull << ln) - 1ull);
+            if (act) mem[vfill[v] + (int)__builtin_popcountll(below)] = pt;
+            wp.sync();
+            if (act && below == 0ull) vfill[v] += (int)__builtin_popcountll(same);
+            wp.sync();
         }
-        wp.sync();
         for (int v = ln; v < nv; v += W) {
             const int m0 = voff[v], m1 = voff[v + 1];
             const double* p = seg + 3 * mem[m0];     // point pt at seg + 6 (pt >> 1) + 3 (pt & 1) = seg + 3 pt
@@ -343,13 +399,22 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
                     VJ[36 * (int64_t)(vb + v) + cj] = a / (double)(voff[v + 1] - voff[v]);
                 }
         int ne = 0;
-        for (int k = 0; k < ns; ++k) {
-            const int s = cord[base + k];
-            const int a = ptv[2 * s], b = ptv[2 * s + 1];
-            if (a != b) {
-                if (ln == 0) { E[2 * (int64_t)(eb + ne)] = a; E[2 * (int64_t)(eb + ne) + 1] = b; }
-                ++ne;
+        for (int kb = 0; kb < ns; kb += W) {   // edges in segment order (ballot prefix)
+            const int k = kb + ln;
+            int a = 0, b = 0;
+            if (k < ns) {
+                const int s = cord[base + k];
+                a = ptv[2 * s];
+                b = ptv[2 * s + 1];
             }
+            const bool e = k < ns && a != b;
+            const uint64_t m = wp.ballot(e);
+            if (e) {
+                const int r = eb + ne + (int)__builtin_popcountll(m & ((1ull << ln) - 1ull));
+                E[2 * (int64_t)r] = a;
+                E[2 * (int64_t)r + 1] = b;
+            }
+            ne += (int)__builtin_popcountll(m);
         }
         if (ln == 0) {
             int* cr = comp + TPC_N * q;
