@@ -155,6 +155,10 @@ struct zdx_ctx {
     int device = 0;
     int n_cu = 256;
     hipStream_t stream = nullptr;
+    // a second stream (default priority) on which the wall contacts run
+    // alongside the narrow phase and the topology; fork/join events
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string err;
     std::vector<MeshRec> meshes;
     bool mesh_dirty = true;
@@ -647,8 +651,9 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     DMeshes M = s->ctx->dmeshes();
     DMat mat = s->mat();
     // stage stamps: 0 start, 1 after the broad phase, 2 after the transform,
-    // 3 after the narrow phase, 4 after the topology, 5 after the walls, 6 after
-    // fold + integrate
+    // 3 after the narrow phase, 4 after the topology, 5 when the wall stream has
+    // joined, 6 after fold + integrate, 7 after the walls (on the wall stream,
+    // which starts at 2 and runs alongside 2..4)
     unsigned long long* kt = s->d_kt.p;
     const int gmax = persistent_groups(s->ctx, s->np_groups_per_cu);
     launch_stamp(kt + 0, st);
@@ -671,6 +676,18 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     }
     launch_transform(B, M, st, s->d_need_list.p, s->d_n_need.p, g_transform);
     launch_stamp(kt + 2, st);
+    const bool walls = s->W > 0;
+    if (walls) {
+        // the wall contacts need only the world vertices and the wall list:
+        // fork them onto the second stream, join before the fold
+        hipStream_t st2 = s->ctx->stream2;
+        HIPC(hipEventRecord(s->ctx->ev_fork, st));
+        HIPC(hipStreamWaitEvent(st2, s->ctx->ev_fork, 0));
+        launch_walls(B, M, mat, s->wallbufs(), s->hist(par), s->d_wcontrib.p, s->d_wnc.p, s->d_wall_list.p,
+                     s->d_n_wall.p, s->d_wscr.p, g_walls, s->wall_pts_cap, s->sp, s->d_status.p, st2);
+        launch_stamp(kt + 7, st2);
+        HIPC(hipEventRecord(s->ctx->ev_join, st2));
+    }
     DPairs P = s->dpairs(par, batched);
     if (batched) launch_lpt_order(P, M, s->d_mesh.p, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st);
     launch_fill_i32(s->d_topo_n.p, 0, 1, st);
@@ -691,9 +708,7 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     launch_topo_zero(P, T, H.cur_active, s->pair_cap, st);
     launch_topo_sim(P, M, B, mat, H, T, s->sp, st);
     launch_stamp(kt + 4, st);
-    if (s->W > 0)
-        launch_walls(B, M, mat, s->wallbufs(), H, s->d_wcontrib.p, s->d_wnc.p, s->d_wall_list.p, s->d_n_wall.p,
-                     s->d_wscr.p, g_walls, s->wall_pts_cap, s->sp, s->d_status.p, st);
+    if (walls) HIPC(hipStreamWaitEvent(st, s->ctx->ev_join, 0));
     launch_stamp(kt + 5, st);
     launch_fold_integrate(s->N, s->d_state[par].p, s->d_state[nxt].p, mat, s->d_adj_off.p, s->d_adj_j.p,
                           s->d_adj_pair.p, s->adj_cap, T, s->W, s->d_wcontrib.p, s->d_wnc.p, s->max_wall_tris,
@@ -816,13 +831,15 @@ void step_once(zdx_sim* s, bool batched) {
         int32_t status = read_status(s);
         if (status == 0) {
             read_counters(s);
-            unsigned long long kt[7];
+            unsigned long long kt[8];
             HIPC(hipMemcpy(kt, s->d_kt.p, sizeof(kt), hipMemcpyDeviceToHost));
-            // acc_ms: transform, broad phase, narrow, topology, walls, fold
-            const int from[6] = {1, 0, 2, 3, 4, 5};
-            for (int k = 0; k < 6; ++k)
-                if (kt[from[k] + 1] >= kt[from[k]])
-                    s->acc_ms[k] += (double)(kt[from[k] + 1] - kt[from[k]]) / s->clock_khz;
+            // acc_ms: transform, broad phase, narrow, topology, walls (on its
+            // own stream, overlapping narrow and topology), fold
+            const int a[6] = {1, 0, 2, 3, 2, 5}, b[6] = {2, 1, 3, 4, 7, 6};
+            for (int k = 0; k < 6; ++k) {
+                if (k == 4 && s->W == 0) continue;
+                if (kt[b[k]] >= kt[a[k]]) s->acc_ms[k] += (double)(kt[b[k]] - kt[a[k]]) / s->clock_khz;
+            }
             for (int k = 0; k < ZDX_N_COUNTERS; ++k)
                 s->acc_ctr[k] = k == ZDX_CTR_SEG_MAX ? std::max(s->acc_ctr[k], (double)s->last_ctr[k])
                                                      : s->acc_ctr[k] + (double)s->last_ctr[k];
@@ -858,6 +875,9 @@ zdx_status zdx_create(zdx_ctx** out, int32_t device) {
         HIPC(hipGetDeviceProperties(&prop, device));
         c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
         HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIPC(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+        HIPC(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        HIPC(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     });
     if (st != ZDX_OK) {
         delete c;
@@ -871,6 +891,10 @@ void zdx_destroy(zdx_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
