@@ -167,18 +167,31 @@ def _pmc_by_kernel(d, counter):
     return out
 
 
-def _trace_durations(d, pred):
-    """Durations (us) of the dispatches of kernels matching pred, in dispatch
-    order, from a rocprofv3 --kernel-trace csv directory."""
+def _trace_by_kernel(d, pred):
+    """Durations (us) of the dispatches of each kernel matching pred, in
+    dispatch order, from a rocprofv3 --kernel-trace csv directory."""
     import csv
     import glob
 
-    rows = []
+    rows = {}
     for f in glob.glob(os.path.join(d, "*kernel_trace.csv")):
         for r in csv.DictReader(open(f)):
             if pred(r["Kernel_Name"]):
-                rows.append((int(r["Dispatch_Id"]), (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000.0))
-    return [t for _, t in sorted(rows)]
+                rows.setdefault(r["Kernel_Name"], []).append(
+                    (int(r["Dispatch_Id"]), (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000.0))
+    return {k: [t for _, t in sorted(v)] for k, v in rows.items()}
+
+
+def _short(name):
+    return name.replace("zdx::(anonymous namespace)::", "").split("(")[0].replace("void ", "")
+
+
+# the narrow phase: cull + sweep, exact tests, compaction (one launch each per step)
+NARROW_KERNELS = ("k_narrow<", "k_narrow_exact", "k_narrow_compact")
+
+
+def _is_narrow(name):
+    return any(k in name for k in NARROW_KERNELS)
 
 
 def summarize_profiles(stats_dir, fetch_dir, write_dir, command="", timed_stats=20, timed_pmc=5):
@@ -199,12 +212,14 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir, command="", timed_stats=
              f"Command: `{command}`" if command else "", "",
              "| kernel | calls | average us | total % |", "|---|---|---|---|"]
     for r in stats[:30]:
-        name = r["Name"].replace("zdx::(anonymous namespace)::", "").split("(")[0]
-        lines.append(f"| `{name}` | {r['Calls']} | {float(r['AverageNs']) / 1000:.1f} | {float(r['Percentage']):.2f} |")
-    tn = _trace_durations(stats_dir, lambda k: "k_narrow" in k)[-timed_stats:]
+        lines.append(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1000:.1f} | "
+                     f"{float(r['Percentage']):.2f} |")
+    tn = {k: v[-timed_stats:] for k, v in _trace_by_kernel(stats_dir, _is_narrow).items()}
     if tn:
-        lines += ["", f"k_narrow over the last {len(tn)} dispatches (the timed steps): average "
-                      f"{sum(tn) / len(tn):.1f} us (min {min(tn):.1f}, max {max(tn):.1f})"]
+        lines += ["", f"Narrow phase over the last {timed_stats} steps (the timed steps), average per launch:"]
+        for k, v in sorted(tn.items()):
+            lines.append(f"- `{_short(k)}`: {sum(v) / len(v):.1f} us (min {min(v):.1f}, max {max(v):.1f})")
+        lines.append(f"- sum of the three: {sum(sum(v) / len(v) for v in tn.values()):.1f} us per step")
     open(os.path.join(prof, "kernels_granular_bed_100k.md"), "w").write("\n".join(lines) + "\n")
     fetch = _pmc_by_kernel(fetch_dir, "FETCH_SIZE")
     write = _pmc_by_kernel(write_dir, "WRITE_SIZE")
@@ -214,26 +229,35 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir, command="", timed_stats=
         return vals
 
     is_copy = lambda k: ("copy" in k.lower() or "elementwise" in k.lower()) and "zdx" not in k
-    is_narrow = lambda k: "k_narrow" in k
     cf, cw = pick(fetch, is_copy), pick(write, is_copy)
-    nf, nwr = pick(fetch, is_narrow)[-timed_pmc:], pick(write, is_narrow)[-timed_pmc:]
+
+    def per_step(d):   # mean per launch of each narrow kernel over the timed steps, summed
+        vals = {k: v[-timed_pmc:] for k, v in d.items() if _is_narrow(k)}
+        return sum(sum(v) / len(v) for v in vals.values()) if vals else 0.0, vals
+
+    nf_kib, nf = per_step(fetch)
+    nw_kib, nwr = per_step(write)
     res = {}
     if cf and cw and nf and nwr:
         kib = COPY_BYTES / 1024.0
         f_scale = kib / max(cf)     # bytes the copy moved / what the counter says
         w_scale = kib / max(cw)
-        fetch_b = sum(nf) / len(nf) * f_scale * 1024.0
-        write_b = sum(nwr) / len(nwr) * w_scale * 1024.0
-        res = dict(kernel="k_narrow", dispatches=len(nf), fetch_bytes_per_launch=fetch_b,
+        fetch_b = nf_kib * f_scale * 1024.0
+        write_b = nw_kib * w_scale * 1024.0
+        res = dict(kernel="narrow phase: " + " + ".join(NARROW_KERNELS), dispatches=timed_pmc,
+                   fetch_bytes_per_launch=fetch_b,
                    write_bytes_per_launch=write_b, bytes_per_launch=fetch_b + write_b,
                    fetch_scale=f_scale, write_scale=w_scale, copy_bytes=COPY_BYTES,
                    copy_fetch_kib=max(cf), copy_write_kib=max(cw))
         json.dump(res, open(os.path.join(prof, "narrow_traffic.json"), "w"), indent=1)
-        md = ["# HBM traffic of k_narrow -- granular bed, 100k polyhedra", "",
+        md = ["# HBM traffic of the narrow phase -- granular bed, 100k polyhedra", "",
               f"Commands: `{command}` with `--pmc FETCH_SIZE` and `--pmc WRITE_SIZE` (separate runs)", "",
-              f"- dispatches: the last {len(nf)} (the timed steps)",
-              f"- FETCH_SIZE per dispatch: {sum(nf) / len(nf):.0f} KiB x scale {f_scale:.3f} = {fetch_b / 1e6:.1f} MB",
-              f"- WRITE_SIZE per dispatch: {sum(nwr) / len(nwr):.0f} KiB x scale {w_scale:.3f} = {write_b / 1e6:.1f} MB",
+              f"- kernels: {', '.join(_short(k) for k in sorted(nf))}, one launch each per step; the last "
+              f"{timed_pmc} steps (the timed steps)",
+              *[f"  - `{_short(k)}`: FETCH {sum(v) / len(v):.0f} KiB, WRITE {sum(nwr.get(k, [0])) / max(1, len(nwr.get(k, [0]))):.0f} KiB"
+                for k, v in sorted(nf.items())],
+              f"- FETCH_SIZE per step: {nf_kib:.0f} KiB x scale {f_scale:.3f} = {fetch_b / 1e6:.1f} MB",
+              f"- WRITE_SIZE per step: {nw_kib:.0f} KiB x scale {w_scale:.3f} = {write_b / 1e6:.1f} MB",
               f"- scale = bytes of a plain {COPY_BYTES >> 20} MiB device copy / what the counter reads for it "
               f"(fetch {max(cf):.0f} KiB, write {max(cw):.0f} KiB)",
               f"- traffic per launch: {(fetch_b + write_b) / 1e6:.1f} MB"]
@@ -331,7 +355,7 @@ def main():
     dom = max(stage_ms, key=stage_ms.get)
     per_step = {k: (v if k == "seg_max" else v / args.steps) for k, v in st_b["counters"].items()}
     ops, byts = narrow_model(per_step)
-    t_np = stage_ms["narrow"] * 1e-3
+    t_np = st_b["narrow_kernels"] * 1e-3   # the narrow-phase kernels alone (device clock stamps)
     ach_tflops = ops / t_np / 1e12 if t_np > 0 else 0.0
     ach_gbs = byts / t_np / 1e9 if t_np > 0 else 0.0
     f_flops, f_hbm = ach_tflops / FP64_PEAK_TFLOPS, ach_gbs / HBM_PEAK_GBS
@@ -340,8 +364,9 @@ def main():
     else:
         roof = dict(bound="mfma", achieved=ach_tflops, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=f_flops,
                     traffic=None)
-    roof.update(kernel="k_narrow (narrow phase)", compute="FP64 VALU (no matrix-core work in this path)",
-                avg_launch_ms=stage_ms["narrow"], fp64_ops_per_launch=ops, bytes_per_launch=byts,
+    roof.update(kernel="narrow phase: k_narrow (cull + sweep) + k_narrow_exact + k_narrow_compact",
+                compute="FP64 VALU (no matrix-core work in this path)",
+                avg_launch_ms=st_b["narrow_kernels"], fp64_ops_per_launch=ops, bytes_per_launch=byts,
                 frac_fp64=f_flops, frac_hbm=f_hbm, dominant_stage=dom, stage_ms=stage_ms)
     traffic_file = os.path.join(ROOT, "profiles", "narrow_traffic.json")
     if os.path.exists(traffic_file):

@@ -190,7 +190,11 @@ zdx_status zdx_sim_restore(zdx_sim* sim, const char* path);
  * both modes and inside the captured graph.  out[0..5]:
  * mean ms per step of transform, broad phase, narrow phase, topology+forces,
  * walls, fold+integrate+counters; out[6]: steps; out[7..7+ZDX_N_COUNTERS):
- * counter sums (ZDX_CTR_SEG_MAX: the maximum).  Writes min(n, 7 + ZDX_N_COUNTERS) values. */
+ * counter sums (ZDX_CTR_SEG_MAX: the maximum); out[7+ZDX_N_COUNTERS]: mean ms
+ * of the narrow-phase kernels alone (without the work ordering).  The walls
+ * run on a second stream alongside the narrow phase and the topology, so the
+ * stages can add up to more than the step.  Writes min(n, 8 + ZDX_N_COUNTERS)
+ * values. */
 zdx_status zdx_sim_stage_times(zdx_sim* sim, double* out, int32_t n);
 /* Particle-particle force model of the time step.  ZDX_FORCE_ZDEM (default):
  * the reference's force, src/main.cpp:2220-2287.  ZDX_FORCE_MINIMAL: the

@@ -358,7 +358,7 @@ struct zdx_sim {
     // (also inside the captured graphs): 6 stages x (start, end)
     DBuf<unsigned long long> d_kt;
     double clock_khz = 100000.0;
-    double acc_ms[6] = {0, 0, 0, 0, 0, 0};
+    double acc_ms[7] = {0, 0, 0, 0, 0, 0, 0};   // 6 stages, then the narrow-phase kernels alone
     double acc_ctr[ZDX_N_COUNTERS] = {0};
     int64_t acc_steps = 0;
     // graphs (one per parity)
@@ -653,7 +653,8 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     // stage stamps: 0 start, 1 after the broad phase, 2 after the transform,
     // 3 after the narrow phase, 4 after the topology, 5 when the wall stream has
     // joined, 6 after fold + integrate, 7 after the walls (on the wall stream,
-    // which starts at 2 and runs alongside 2..4)
+    // which starts at 2 and runs alongside 2..4), 8 before the narrow-phase
+    // kernels (after the work ordering)
     unsigned long long* kt = s->d_kt.p;
     const int gmax = persistent_groups(s->ctx, s->np_groups_per_cu);
     launch_stamp(kt + 0, st);
@@ -694,6 +695,7 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     NarrowCfg nc = s->ncfg;
     nc.use_queue = batched ? 1 : 0;
     nc.groups = g_narrow;
+    launch_stamp(kt + 8, st);
     launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->stage_cap, s->tmax, s->d_queue.p, st);
     launch_stamp(kt + 3, st);
     TopoBufs T = s->topobufs();
@@ -831,12 +833,12 @@ void step_once(zdx_sim* s, bool batched) {
         int32_t status = read_status(s);
         if (status == 0) {
             read_counters(s);
-            unsigned long long kt[8];
+            unsigned long long kt[9];
             HIPC(hipMemcpy(kt, s->d_kt.p, sizeof(kt), hipMemcpyDeviceToHost));
             // acc_ms: transform, broad phase, narrow, topology, walls (on its
-            // own stream, overlapping narrow and topology), fold
-            const int a[6] = {1, 0, 2, 3, 2, 5}, b[6] = {2, 1, 3, 4, 7, 6};
-            for (int k = 0; k < 6; ++k) {
+            // own stream, overlapping narrow and topology), fold, narrow kernels
+            const int a[7] = {1, 0, 2, 3, 2, 5, 8}, b[7] = {2, 1, 3, 4, 7, 6, 3};
+            for (int k = 0; k < 7; ++k) {
                 if (k == 4 && s->W == 0) continue;
                 if (kt[b[k]] >= kt[a[k]]) s->acc_ms[k] += (double)(kt[b[k]] - kt[a[k]]) / s->clock_khz;
             }
@@ -1430,13 +1432,14 @@ zdx_status zdx_sim_restore(zdx_sim* s, const char* path) {
 
 zdx_status zdx_sim_stage_times(zdx_sim* s, double* out, int32_t n) {
     if (!s || !out || n < 0) return ZDX_BAD_ARG;
-    double v[7 + ZDX_N_COUNTERS];
+    double v[8 + ZDX_N_COUNTERS];
     const double steps = (double)s->acc_steps;
     for (int k = 0; k < 6; ++k) v[k] = steps > 0 ? s->acc_ms[k] / steps : 0.0;
     v[6] = steps;
     for (int k = 0; k < ZDX_N_COUNTERS; ++k) v[7 + k] = s->acc_ctr[k];
-    for (int k = 0; k < n && k < 7 + ZDX_N_COUNTERS; ++k) out[k] = v[k];
-    for (int k = 0; k < 6; ++k) s->acc_ms[k] = 0.0;
+    v[7 + ZDX_N_COUNTERS] = steps > 0 ? s->acc_ms[6] / steps : 0.0;
+    for (int k = 0; k < n && k < 8 + ZDX_N_COUNTERS; ++k) out[k] = v[k];
+    for (int k = 0; k < 7; ++k) s->acc_ms[k] = 0.0;
     for (int k = 0; k < ZDX_N_COUNTERS; ++k) s->acc_ctr[k] = 0.0;
     s->acc_steps = 0;
     return ZDX_OK;
