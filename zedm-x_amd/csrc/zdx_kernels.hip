@@ -533,15 +533,14 @@ __global__ void __launch_bounds__(256) k_need_list(int32_t n, const int32_t* fla
 // set-up, cull A, cull B, sweep tests, exact tests, publish, pairs done).
 __device__ unsigned long long g_walk_prof[16];
 __device__ long long g_walk_pairs[12 * 8192];   // walk B per pair: S, R, scratch in LDS, ticks, phases 0-7
+__device__ long long g_np_pairs[8 * 16384];   // narrow per pair (queue item): nA, nB, hits, 4 phase ticks, p
+#define NP_START() unsigned long long np_t0 = wall_clock64(); unsigned np_ph[4] = {0, 0, 0, 0}
 #define NP_MARK(k)                                                      \
     do {                                                                \
-        if (threadIdx.x == 0) {                                         \
-            const unsigned long long t_ = wall_clock64();               \
-            atomicAdd(&g_walk_prof[k], t_ - np_t0);                     \
-            np_t0 = t_;                                                 \
-        }                                                               \
+        const unsigned long long t_ = wall_clock64();                   \
+        np_ph[(k) & 3] += (unsigned)(t_ - np_t0);                       \
+        np_t0 = t_;                                                     \
     } while (0)
-#define NP_START() unsigned long long np_t0 = wall_clock64()
 #else
 #define NP_MARK(k) do {} while (0)
 #define NP_START() do {} while (0)
@@ -631,7 +630,7 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
         float boxA[6], boxB[6];
         box_f(B.aabb + 6 * (int64_t)i, B.aabb + 6 * (int64_t)i + 3, boxA);
         box_f(B.aabb + 6 * (int64_t)j, B.aabb + 6 * (int64_t)j + 3, boxB);
-        NP_MARK(8);
+        NP_MARK(0);
         // ---- cull each side against the other body's AABB (conservative)
         int nA = 0, nB = 0;
         const int VA = M.nverts[mA], VB = M.nverts[mB];
@@ -680,7 +679,7 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
                 nA += tot & 0xffff;
                 nB += tot >> 16;
             }
-            NP_MARK(9);
+            NP_MARK(1);
         } else {
         for (int base = 0; base < TA; base += BS) {
             int t = base + tid;
@@ -698,7 +697,7 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
             if (keep) LA[nA + r] = t;
             nA += tot;
         }
-        NP_MARK(9);
+        NP_MARK(1);
         for (int base = 0; base < TB; base += BS) {
             int t = base + tid;
             bool keep = false;
@@ -738,72 +737,62 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
             }
         }
         __syncthreads();
-        NP_MARK(10);
-        // ---- ordered sweep of the culled product into the hit arena.  K
-        // consecutive items of the flattened product per thread, hits as a bit
-        // mask.  A pair of one round keeps its mask from the counting scan;
-        // longer pairs count first, then sweep again to write, so the arena
-        // holds exactly the hits.
+        NP_MARK(1);
+        // ---- ordered sweep of the culled product into the hit arena.  Each
+        // thread takes one contiguous range of the flattened (ta, tb) product,
+        // so the product's order is the threads' order: a counting pass, one
+        // block scan, an exactly sized range of the arena, then the write pass
+        // (from the hit bits of the counting pass when the range has at most 64
+        // items, else by testing again).  No barrier inside the sweep.
         const int64_t total = (int64_t)nA * nB;
-        auto sweep = [&](int64_t base, int& ca0, int& cb0) -> unsigned {
-            unsigned hm = 0u;
-            const int64_t g0 = base + (int64_t)tid * K;
-            ca0 = 0; cb0 = 0;
-            if (g0 >= total) return 0u;
-            if (total <= 0x7fffffff) {   // 32-bit division when the product fits
-                ca0 = (int)g0 / nB;
-                cb0 = (int)g0 - ca0 * nB;
-            } else {
-                ca0 = (int)(g0 / nB);
-                cb0 = (int)(g0 - (int64_t)ca0 * nB);
-            }
-            int ca = ca0, cb = cb0;
+        const int64_t per = (total + BS - 1) / BS;
+        const int64_t g_begin = total < (int64_t)tid * per ? total : (int64_t)tid * per;
+        const int64_t g_end = total < g_begin + per ? total : g_begin + per;
+        auto split = [&](int64_t g, int& ca, int& cb) {
+            if (total <= 0x7fffffff) { ca = (int)g / nB; cb = (int)g - ca * nB; }
+            else { ca = (int)(g / nB); cb = (int)(g - (int64_t)ca * nB); }
+        };
+        // f(k, ca, cb) for each hit of the range, k = item index within the range
+        auto for_hits = [&](auto&& f) {
+            if (g_begin >= g_end) return;
+            int ca, cb;
+            split(g_begin, ca, cb);
             float fa[6];
-            for (int k = 0; k < 6; ++k) fa[k] = FA[6 * ca + k];
+            for (int q = 0; q < 6; ++q) fa[q] = FA[6 * ca + q];
+            const int n = (int)(g_end - g_begin);
+            for (int k0 = 0; k0 < n; k0 += K) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                if (g0 + k < total) {
-                    if (ovl_f(fa, FB + 6 * cb)) hm |= 1u << k;
-                    if (++cb == nB) {
-                        cb = 0; ++ca;
-                        if (ca < nA) for (int q = 0; q < 6; ++q) fa[q] = FA[6 * ca + q];
+                for (int u = 0; u < K; ++u) {
+                    const int k = k0 + u;
+                    if (k < n) {
+                        if (ovl_f(fa, FB + 6 * cb)) f(k, ca, cb);
+                        if (++cb == nB) {
+                            cb = 0; ++ca;
+                            if (ca < nA) for (int q = 0; q < 6; ++q) fa[q] = FA[6 * ca + q];
+                        }
                     }
                 }
             }
-            return hm;
         };
         // a hit record: the six world-vertex indices of (ta, tb); with
         // provenance also (ta, tb) itself
-        auto emit = [&](unsigned hm, int ca, int cb, int64_t at) {
-            if (!hm) return;
+        auto emit = [&](int64_t at, int ca, int cb) {
             int2* dst = (int2*)(P.hits + 6 * at);
-            int2* dt = P.hit_tri ? (int2*)P.hit_tri + at : nullptr;
-            int a0 = VA3[3 * ca], a1 = VA3[3 * ca + 1], a2 = VA3[3 * ca + 2];
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                if (hm & (1u << k)) {
-                    const int32_t* vb = VB3 + 3 * cb;
-                    dst[0] = make_int2(a0, a1);
-                    dst[1] = make_int2(a2, vb[0]);
-                    dst[2] = make_int2(vb[1], vb[2]);
-                    dst += 3;
-                    if (dt) *dt++ = make_int2(LA[ca], LB[cb]);
-                }
-                if (++cb == nB) {
-                    cb = 0; ++ca;
-                    if (ca < nA) { a0 = VA3[3 * ca]; a1 = VA3[3 * ca + 1]; a2 = VA3[3 * ca + 2]; }
-                }
-            }
+            const int32_t* va = VA3 + 3 * ca;
+            const int32_t* vb = VB3 + 3 * cb;
+            dst[0] = make_int2(va[0], va[1]);
+            dst[1] = make_int2(va[2], vb[0]);
+            dst[2] = make_int2(vb[1], vb[2]);
+            if (P.hit_tri) ((int2*)P.hit_tri)[at] = make_int2(LA[ca], LB[cb]);
         };
-        const bool one_round = total <= (int64_t)BS * K;
-        int ca1 = 0, cb1 = 0, cnt = 0;
-        unsigned hm1 = 0u;
-        for (int64_t base = 0; base < total; base += (int64_t)BS * K) {
-            hm1 = sweep(base, ca1, cb1);
-            cnt += __popc(hm1);
-        }
+        int cnt = 0;
+        uint64_t bits = 0ull;   // hit bits of the range when per <= 64
+        for_hits([&](int k, int, int) {
+            ++cnt;
+            if (k < 64) bits |= 1ull << k;
+        });
         int hits;
-        const int off1 = block_excl<BS>(cnt, s_wave, hits);
+        const int off = block_excl<BS>(cnt, s_wave, hits);
         if (tid == 0) {
             unsigned long long hb = hits > 0 ? atomicAdd(P.hit_top, (unsigned long long)hits) : 0ull;
             if (hb + (unsigned long long)hits > (unsigned long long)P.hit_cap) {
@@ -814,21 +803,19 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
         }
         __syncthreads();
         const unsigned long long hbase = s_base;
-        if (hbase != ~0ull && hits > 0) {
-            if (one_round) {
-                emit(hm1, ca1, cb1, (int64_t)hbase + off1);
-            } else {
-                int done = 0;
-                for (int64_t base = 0; base < total; base += (int64_t)BS * K) {
-                    int ca0, cb0, nh;
-                    const unsigned hm = sweep(base, ca0, cb0);
-                    const int off = block_excl<BS>(__popc(hm), s_wave, nh);
-                    emit(hm, ca0, cb0, (int64_t)hbase + done + off);
-                    done += nh;
+        if (hbase != ~0ull && cnt > 0) {
+            int64_t at = (int64_t)hbase + off;
+            if (per <= 64) {
+                for (uint64_t m = bits; m; m &= m - 1) {
+                    int ca, cb;
+                    split(g_begin + ctz64(m), ca, cb);
+                    emit(at++, ca, cb);
                 }
+            } else {
+                for_hits([&](int, int ca, int cb) { emit(at++, ca, cb); });
             }
         }
-        NP_MARK(11);
+        NP_MARK(2);
         if (tid == 0) {
             const bool ok = hbase != ~0ull;
             P.hit_off[p] = ok ? (int64_t)hbase : 0;
@@ -840,9 +827,15 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
             P.nseg[p] = 0;
         }
         __syncthreads();
-        NP_MARK(13);
+        NP_MARK(3);
 #ifdef ZDX_WALK_PROFILE
-        if (tid == 0) atomicAdd(&g_walk_prof[14], 1ull);
+        if (tid == 0 && item < 16384) {
+            long long* r = g_np_pairs + 8 * (int64_t)item;
+            r[0] = nA; r[1] = nB; r[2] = hits;
+            for (int k = 0; k < 4; ++k) r[3 + k] = np_ph[k];
+            r[7] = p;
+            np_ph[0] = np_ph[1] = np_ph[2] = np_ph[3] = 0;
+        }
 #endif
         if (!use_queue) item += gridDim.x;
     }
@@ -1665,6 +1658,9 @@ inline int nblk(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 }  // namespace
 
 #ifdef ZDX_WALK_PROFILE
+extern "C" int zdx_narrow_pairs(long long* out, int n) {
+    return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_np_pairs), 8 * (size_t)n * sizeof(long long)) == hipSuccess ? 0 : 1;
+}
 extern "C" int zdx_walk_pairs(long long* out, int n) {
     return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_walk_pairs), 12 * (size_t)n * sizeof(long long)) == hipSuccess ? 0 : 1;
 }
