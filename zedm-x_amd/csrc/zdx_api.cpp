@@ -84,10 +84,11 @@ struct TopoWork {
     DBuf<int32_t> adj;
     DBuf<uint64_t> snapw, tjw;
     DBuf<unsigned long long> top;
+    DBuf<int32_t> units;
     DBuf<int64_t> adj_off, snap_off, tj_off;
     DBuf<int32_t> scr_i;
     DBuf<double> scr_d;
-    int64_t adj_cap = 1 << 20, snap_cap = 1 << 17, tj_cap = 1 << 17;
+    int64_t adj_cap = 1 << 20, snap_cap = 1 << 17, tj_cap = 1 << 17, units_cap = 1 << 12;
     int32_t adj_row = 16;
     int64_t scr_ni = 0, scr_nd = 0;
     int64_t slices = 0;
@@ -109,12 +110,13 @@ struct TopoWork {
         scr_d.ensure((size_t)(scr_nd * slices));
     }
     void ensure_bump() {
-        adj.ensure((size_t)adj_cap); snapw.ensure((size_t)snap_cap); tjw.ensure((size_t)tj_cap); top.ensure(4);
+        adj.ensure((size_t)adj_cap); snapw.ensure((size_t)snap_cap); tjw.ensure((size_t)tj_cap); top.ensure(5);
+        units.ensure(2 * (size_t)units_cap);
     }
     // After a step whose pass A or walk B ran out of room: the bump counters
     // hold what every pair asked for.  Returns true if a buffer was grown.
     bool grow_from_counters(hipStream_t st) {
-        unsigned long long t[4] = {0, 0, 0, 0};
+        unsigned long long t[5] = {0, 0, 0, 0, 0};
         HIPC(hipMemcpyAsync(t, top.p, sizeof(t), hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
         bool g = false;
@@ -133,8 +135,9 @@ struct TopoWork {
         grow(adj_cap, t[0]);
         grow(snap_cap, t[1]);
         grow(tj_cap, t[2]);
+        grow(units_cap, t[4]);
         if (g) {
-            adj.release(); snapw.release(); tjw.release();
+            adj.release(); snapw.release(); tjw.release(); units.release();
             ensure_bump();
         }
         return g;
@@ -144,6 +147,7 @@ struct TopoWork {
         T.cell9 = cell9.p; T.adj = adj.p; T.adj_cap = adj_cap; T.adj_row = adj_row; T.snapw = snapw.p;
         T.snap_cap = snap_cap;
         T.tjw = tjw.p; T.tj_cap = tj_cap; T.top = top.p; T.adj_off = adj_off.p; T.snap_off = snap_off.p;
+        T.units = units.p; T.units_cap = units_cap;
         T.tj_off = tj_off.p; T.ncomp = ncomp.p; T.comp = comp.p; T.V = V.p; T.vcell = vcell.p; T.E = E.p;
         T.scr_i = scr_i.p; T.scr_d = scr_d.p; T.scr_ni = scr_ni; T.scr_nd = scr_nd;
     }
@@ -1496,7 +1500,8 @@ zdx_status zdx_sim_autotune(zdx_sim* s, int32_t* out_block, int32_t* out_groups)
         launch_transform(B, M, st, s->d_need_list.p, s->d_n_need.p, s->transform_groups());
         DPairs P = s->dpairs(par, true);
         launch_lpt_order(P, M, s->d_mesh.p, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st);
-        const int cand[][2] = {{128, 4}, {128, 8}, {256, 1}, {256, 2}, {256, 4}, {256, 8}, {512, 2}, {512, 4}};
+        const int cand[][2] = {{64, 4}, {64, 8}, {128, 4}, {128, 8}, {256, 1}, {256, 2}, {256, 4}, {256, 8},
+                               {512, 2}, {512, 4}};
         const int gpc[] = {4, 8, 16};
         hipEvent_t e0, e1;
         HIPC(hipEventCreate(&e0));

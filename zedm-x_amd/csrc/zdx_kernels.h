@@ -89,7 +89,9 @@ struct TopoBufs {
     int64_t snap_cap;
     uint64_t* tjw;             // T-junction hit words (bump-allocated)
     int64_t tj_cap;
-    unsigned long long* top;   // bump counters: adjacency ints, snap words, T-junction words; longest row
+    unsigned long long* top;   // counters: adjacency ints, snap words, T-junction words; longest row; pass-A units
+    int32_t* units;            // pass A: extra units (item, unit) of the pairs split into several
+    int64_t units_cap;
     int64_t* adj_off;          // per item (-1: did not fit)
     int64_t* snap_off;
     int64_t* tj_off;
@@ -126,7 +128,7 @@ struct SimParams {
 };
 
 struct NarrowCfg {
-    int32_t block;             // 128, 256 or 512 threads
+    int32_t block;             // 64, 128, 256 or 512 threads
     int32_t items;             // AABB tests per thread per round
     int32_t groups;            // persistent workgroups
     int32_t use_queue;         // 1: atomic work queue (batched)

@@ -532,6 +532,7 @@ __global__ void __launch_bounds__(256) k_need_list(int32_t n, const int32_t* fla
 // the start of a pair to its first mark); slots 8-14: the narrow phase (pair
 // set-up, cull A, cull B, sweep tests, exact tests, publish, pairs done).
 __device__ unsigned long long g_walk_prof[16];
+__device__ long long g_topo_pairs[4 * 8192];    // per topology item: S, pass A ticks, walk D ticks
 __device__ long long g_walk_pairs[12 * 8192];   // walk B per pair: S, R, scratch in LDS, ticks, phases 0-7
 __device__ long long g_np_pairs[8 * 16384];   // narrow per pair (queue item): nA, nB, hits, 4 phase ticks, p
 #define NP_START() unsigned long long np_t0 = wall_clock64(); unsigned np_ph[4] = {0, 0, 0, 0}
@@ -591,9 +592,10 @@ constexpr int CULL_PER_THREAD = 8;
 // hits are listed for part 2 (k_narrow_exact).
 template <int BS, int K>
 __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, int32_t* list_i, float* list_f,
-                                               int32_t tmax, int32_t use_queue, unsigned int* queue, int32_t lds_b) {
-    // lds_b: body B's culled float boxes (24 bytes per triangle, read for
-    // every item of the sweep) live in dynamic LDS.  Per workgroup in global
+                                               int32_t tmax, int32_t use_queue, unsigned int* queue, int32_t lds_cap) {
+    // Body B's culled float boxes (24 bytes per triangle, read for every item
+    // of the sweep) live in dynamic LDS when there are at most lds_cap of
+    // them, else in the workgroup's global list.  Per workgroup in global
     // memory: the kept triangle ids of both sides (LA, LB), their world-vertex
     // indices (VA3, VB3, 3 each) and A's float boxes (FA).
     extern __shared__ char s_nb[];
@@ -607,7 +609,6 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
     int32_t* VA3 = LA + 2 * (int64_t)tmax;
     int32_t* VB3 = LA + 5 * (int64_t)tmax;
     float* FA = list_f + (int64_t)blockIdx.x * 12 * tmax;
-    float* FB = lds_b ? (float*)s_nb : FA + 6 * tmax;
     const int npairs = *P.n;
     int item = blockIdx.x;
     NP_START();
@@ -637,16 +638,17 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
         if (VA <= OC_CAP && VB <= OC_CAP) {
             // outcodes of both bodies' vertices, then per thread CULL_PER_THREAD
             // consecutive triangles of each side, one ordered scan per round
-            for (int v0 = 0; v0 < VA + VB; v0 += 4 * BS) {
-                double w[4][3];
+            constexpr int UV = BS <= 64 ? 8 : 4;   // vertices per thread per round, loads in flight
+            for (int v0 = 0; v0 < VA + VB; v0 += UV * BS) {
+                double w[UV][3];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {   // all loads first, then the codes
+                for (int u = 0; u < UV; ++u) {   // all loads first, then the codes
                     const int v = v0 + u * BS + tid;
                     const double* src = v < VA ? wvA + 3 * v : wvB + 3 * (v - VA);
                     if (v < VA + VB) { w[u][0] = src[0]; w[u][1] = src[1]; w[u][2] = src[2]; }
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < UV; ++u) {
                     const int v = v0 + u * BS + tid;
                     if (v < VA + VB) {
                         const bool a = v < VA;
@@ -716,6 +718,7 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
         }
         }
         __syncthreads();
+        float* FB = nB <= lds_cap ? (float*)s_nb : FA + 6 * tmax;
         // float boxes and world-vertex indices of the kept triangles, all
         // lanes in parallel
         {
@@ -1074,27 +1077,82 @@ struct LaneDev {
 // else the workgroup's slice of the global scratch.
 
 constexpr int TP_PASS_A_SEGS = 384;     // segments pass A stages in LDS (84 bytes each)
-constexpr int TP_PASS_A_KEYS = 32768;   // LDS bytes for the S x S key matrix (S <= 181)
+constexpr int TP_PASS_A_KEYS = 8192;    // LDS bytes for a unit's rows x S key matrix
+constexpr int TP_PASS_A_WORDS = 256;    // snap words a unit builds in LDS at a time
 constexpr int TP_PASS_A_THREADS = 1024;
+constexpr int TP_PASS_A_TESTS = 4096;   // segment-pair tests per unit (sets the units of a big pair)
+
+// Units of pass A for a pair of S segments: enough that each unit has at most
+// TP_PASS_A_TESTS candidate tests (rows x S) and TP_PASS_A_WORDS snap words,
+// and at least one row.
+__device__ inline int tp_pa_units(int S) {
+    const int64_t npt = (int64_t)S * (2 * S - 1);
+    const int64_t nw = (npt + 63) >> 6;
+    int64_t m = ((int64_t)S * S + TP_PASS_A_TESTS - 1) / TP_PASS_A_TESTS;
+    const int64_t mw = (nw + TP_PASS_A_WORDS - 1) / TP_PASS_A_WORDS;
+    if (mw > m) m = mw;
+    if (m > S) m = S;
+    return m < 1 ? 1 : (int)m;
+}
+
+// Pass A allocation, one thread per work-list item: the item's adjacency rows
+// and snap words (bump counters top[0], top[1]) and, for a pair split into
+// several units, its units 1.. in the unit list (counter top[4]).  An item that
+// does not fit gets offsets -1 (the step is retried with larger buffers: the
+// counters keep the totals asked for).
+__global__ void __launch_bounds__(256) k_tp_alloc(DPairs P, TopoBufs T) {
+    const int n = *P.topo_n;
+    const int R = T.adj_row;
+    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < n; it += gridDim.x * blockDim.x) {
+        const int S = P.nseg[P.topo_list[it]];
+        const long long nadj = (long long)S * (R + 1);
+        const long long nw = ((long long)S * (2 * S - 1) + 63) >> 6;
+        const int nu = tp_pa_units(S);
+        long long ao = (long long)atomicAdd(T.top + 0, (unsigned long long)nadj);
+        long long so = (long long)atomicAdd(T.top + 1, (unsigned long long)nw);
+        bool ok = ao + nadj <= T.adj_cap && so + nw <= T.snap_cap;
+        if (nu > 1) {
+            const long long uo = (long long)atomicAdd(T.top + 4, (unsigned long long)(nu - 1));
+            if (uo + nu - 1 > T.units_cap) ok = false;
+            else
+                for (int b = 1; b < nu; ++b) ((int2*)T.units)[uo + b - 1] = make_int2(it, b);
+        }
+        T.adj_off[it] = ok ? ao : -1;
+        T.snap_off[it] = ok ? so : -1;
+    }
+}
 
 // Pass A: representative-point cells, the adjacency rows (per segment the
 // count and up to R accepted candidates in ascending j) and the snap hit words
-// of every pair, one 1024-thread workgroup per pair.  Each pair's rows and
-// snap words are bump-allocated (a pair that does not fit gets offset -1 and
-// the step is retried with larger buffers: the counters keep the total
-// needed, top[3] the longest row).  The segments and cells are staged in LDS;
-// when S x S fits, all candidate tests run flat into an LDS key matrix that
-// one thread per row then compacts; the snap tests run flat, each thread over
-// a contiguous range of endpoint pairs.
+// of every pair.  One 1024-thread workgroup per unit: unit 0 of every item,
+// then the extra units of the big pairs.  A unit of a pair split into nu
+// units takes a band of rows and a range of snap words of its own.  The
+// segments and cells are staged in LDS; the candidate tests of the band run
+// flat into an LDS key matrix that one thread per row then compacts; the snap
+// words are built in LDS (each thread a contiguous range of endpoint pairs)
+// and stored whole.
 __global__ void __launch_bounds__(TP_PASS_A_THREADS) k_tp_pass_a(DPairs P, DMeshes M, DBodies B, TopoBufs T,
                                                                 int32_t lds_segs) {
     extern __shared__ char s_pa[];
-    __shared__ long long s_off[2];
     const int n = *P.topo_n;
+    const int nunits = n + (int)*(T.top + 4);
     const int R = T.adj_row;
     const int NT = blockDim.x;
     uint8_t* keys = (uint8_t*)(s_pa + 84 * (int64_t)lds_segs);
-    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+    unsigned long long* sw = (unsigned long long*)(s_pa + 84 * (int64_t)lds_segs + TP_PASS_A_KEYS);
+    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+#ifdef ZDX_WALK_PROFILE
+        const unsigned long long pa_t0 = wall_clock64();
+#endif
+        int it = u, b = 0;
+        if (u >= n) {
+            if (u - n >= T.units_cap) continue;
+            const int2 e = ((const int2*)T.units)[u - n];
+            it = e.x;
+            b = e.y;
+        }
+        const long long ao = T.adj_off[it], so = T.snap_off[it];
+        if (ao < 0) continue;
         const int p = P.topo_list[it];
         const int S = P.nseg[p];
         const int64_t sb = P.seg_off[p];
@@ -1104,72 +1162,75 @@ __global__ void __launch_bounds__(TP_PASS_A_THREADS) k_tp_pass_a(DPairs P, DMesh
         const bool staged = S <= lds_segs;
         double* seg = staged ? (double*)s_pa : (double*)gseg;
         int* c9 = staged ? (int*)(s_pa + 48 * (int64_t)lds_segs) : gc9;
-        const int nadj = S * (R + 1);
+        const int nu = tp_pa_units(S);
+        const int r0 = (int)((int64_t)b * S / nu), r1 = (int)((int64_t)(b + 1) * S / nu);
         const int npt = S * (2 * S - 1);   // endpoint pairs of 2S points
         const int nw = (npt + 63) >> 6;
-        if (threadIdx.x == 0) {
-            long long ao = (long long)atomicAdd(T.top + 0, (unsigned long long)nadj);
-            long long so = (long long)atomicAdd(T.top + 1, (unsigned long long)nw);
-            if (ao + nadj > T.adj_cap || so + nw > T.snap_cap) { ao = -1; so = -1; }
-            s_off[0] = ao;
-            s_off[1] = so;
-            T.adj_off[it] = ao;
-            T.snap_off[it] = so;
-        }
+        const int w0 = (int)((int64_t)b * nw / nu), w1 = (int)((int64_t)(b + 1) * nw / nu);
         if (staged)
             for (int k = threadIdx.x; k < 6 * S; k += NT) seg[k] = gseg[k];
         __syncthreads();
         for (int k = threadIdx.x; k < S; k += NT) tp_cells(seg, k, h, c9 + 9 * k);
-        const long long ao = s_off[0], so = s_off[1];
-        uint64_t* w = T.snapw + so;
-        if (ao >= 0)
-            for (int k = threadIdx.x; k < nw; k += NT) w[k] = 0ull;
         __syncthreads();
-        if (ao >= 0) {
-            int* cntp = T.adj + ao;
-            int* rows = cntp + S;
-            if (S * S <= TP_PASS_A_KEYS) {
-                for (int k = threadIdx.x; k < S * S; k += NT) {
-                    const int i = k / S, j = k - (k / S) * S;
-                    keys[k] = (uint8_t)tp_adj_key(seg, c9, h, i, j);
-                }
-                __syncthreads();
-                for (int i = threadIdx.x; i < S; i += NT) {
-                    const uint8_t* kr = keys + i * S;
-                    int* out = rows + (int64_t)i * R;
-                    int m = 0;
-                    for (int j = 0; j < S; ++j) {
-                        const int key = kr[j];
-                        if (key == TP_NOADJ) continue;
-                        if (m < R) out[m] = j | (key << 24);
-                        ++m;
-                    }
-                    cntp[i] = m;
-                    if (m > R) atomicMax(T.top + 3, (unsigned long long)m);
-                }
-            } else {
-                for (int i = threadIdx.x; i < S; i += NT) {
-                    const int m = tp_adj_row(seg, c9, h, S, i, rows + (int64_t)i * R, R);
-                    cntp[i] = m;
-                    if (m > R) atomicMax(T.top + 3, (unsigned long long)m);
-                }
+        int* cntp = T.adj + ao;
+        int* rows = cntp + S;
+        const int nr = r1 - r0;
+        if (nr * S <= TP_PASS_A_KEYS) {
+            for (int k = threadIdx.x; k < nr * S; k += NT) {
+                const int i = r0 + k / S, j = k - (k / S) * S;
+                keys[k] = (uint8_t)tp_adj_key(seg, c9, h, i, j);
             }
-            // snap hits: each thread a contiguous range of pair numbers t
-            // (a(a-1)/2 + b), set in the zeroed words with atomic OR
-            const int per = (npt + NT - 1) / NT;
-            int t = threadIdx.x * per;
-            const int t1 = t + per < npt ? t + per : npt;
-            if (t < t1) {
-                int a, b;
-                tp_pair_of(t, a, b);
-                for (; t < t1; ++t) {
-                    if (tp_snap_hit(seg, c9, h, a, b))
-                        atomicOr((unsigned long long*)(w + (t >> 6)), 1ull << (t & 63));
-                    if (++b == a) { ++a; b = 0; }
+            __syncthreads();
+            for (int i = r0 + (int)threadIdx.x; i < r1; i += NT) {
+                const uint8_t* kr = keys + (i - r0) * S;
+                int* out = rows + (int64_t)i * R;
+                int m = 0;
+                for (int j = 0; j < S; ++j) {
+                    const int key = kr[j];
+                    if (key == TP_NOADJ) continue;
+                    if (m < R) out[m] = j | (key << 24);
+                    ++m;
                 }
+                cntp[i] = m;
+                if (m > R) atomicMax(T.top + 3, (unsigned long long)m);
+            }
+        } else {
+            for (int i = r0 + (int)threadIdx.x; i < r1; i += NT) {
+                const int m = tp_adj_row(seg, c9, h, S, i, rows + (int64_t)i * R, R);
+                cntp[i] = m;
+                if (m > R) atomicMax(T.top + 3, (unsigned long long)m);
             }
         }
-        __syncthreads();
+        // snap hits of the unit's words, TP_PASS_A_WORDS at a time: each
+        // thread a contiguous range of pair numbers t (a(a-1)/2 + b) sets bits
+        // in LDS, then the words are stored whole
+        uint64_t* w = T.snapw + so;
+        for (int c0 = w0; c0 < w1; c0 += TP_PASS_A_WORDS) {
+            const int c1 = c0 + TP_PASS_A_WORDS < w1 ? c0 + TP_PASS_A_WORDS : w1;
+            for (int k = threadIdx.x; k < c1 - c0; k += NT) sw[k] = 0ull;
+            __syncthreads();
+            const int tb = 64 * c0, te = 64 * c1 < npt ? 64 * c1 : npt;
+            const int per = (te - tb + NT - 1) / NT;
+            int t = tb + (int)threadIdx.x * per;
+            const int t1 = t + per < te ? t + per : te;
+            if (t < t1) {
+                int a, bb;
+                tp_pair_of(t, a, bb);
+                for (; t < t1; ++t) {
+                    if (tp_snap_hit(seg, c9, h, a, bb)) atomicOr(&sw[(t >> 6) - c0], 1ull << (t & 63));
+                    if (++bb == a) { ++a; bb = 0; }
+                }
+            }
+            __syncthreads();
+            for (int k = threadIdx.x; k < c1 - c0; k += NT) w[c0 + k] = sw[k];
+            __syncthreads();
+        }
+#ifdef ZDX_WALK_PROFILE
+        if (threadIdx.x == 0 && it < 8192 && b == 0) {
+            g_topo_pairs[4 * it] = S;
+            g_topo_pairs[4 * it + 1] = (long long)(wall_clock64() - pa_t0);
+        }
+#endif
     }
 }
 
@@ -1355,6 +1416,7 @@ __global__ void __launch_bounds__(64) k_tp_walk_d_sim(DPairs P, DMeshes M, DBodi
         }
         SimSink sink{T.contrib + 9 * (int64_t)p * T.lmax, T.lmax, &a, &b, &ma, &mb, sp.dt, &ds, sp.force_model,
                      sp.k_minimal};
+        wp.start();
         const int ncomp = T.ncomp[it];
         const long long to = T.tj_off[it];
         TopoOut o;
@@ -1363,6 +1425,9 @@ __global__ void __launch_bounds__(64) k_tp_walk_d_sim(DPairs P, DMeshes M, DBodi
         else if (ncomp > 0)
             o = tp_walk_d(wp, T, S, ncomp, P.seg_off[p], to, M.tol[mA], M.area_eps[mA], sub(b.pos, a.pos), s_walk,
                           lds_bytes, sink);
+#ifdef ZDX_WALK_PROFILE
+        if (lane_id() == 0 && it < 8192) g_topo_pairs[4 * it + 2] = (long long)(wall_clock64() - wp.t_begin);
+#endif
         if (lane_id() == 0) {
             T.ncontrib[p] = o.loops < T.lmax ? o.loops : T.lmax;
             T.comps[p] = o.comps;
@@ -1658,6 +1723,9 @@ inline int nblk(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 }  // namespace
 
 #ifdef ZDX_WALK_PROFILE
+extern "C" int zdx_topo_pairs(long long* out, int n) {
+    return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_topo_pairs), 4 * (size_t)n * sizeof(long long)) == hipSuccess ? 0 : 1;
+}
 extern "C" int zdx_narrow_pairs(long long* out, int n) {
     return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_np_pairs), 8 * (size_t)n * sizeof(long long)) == hipSuccess ? 0 : 1;
 }
@@ -1720,11 +1788,11 @@ void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s) {
 template <int BS, int K>
 static void narrow_t(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
                      int32_t tmax, unsigned int* q, hipStream_t s) {
-    const size_t lds = 24 * (size_t)tmax;
-    const int use = lds <= 48 * 1024 ? 1 : 0;
-    const size_t dyn = use ? lds : 0;
-    hipLaunchKernelGGL((k_narrow<BS, K>), dim3(cfg.groups), dim3(BS), dyn, s, B, M, P, li, lf, tmax, cfg.use_queue, q,
-                       use);
+    // LDS for body B's culled boxes: 4 per thread (at most 2048), so that small
+    // workgroups leave room for many pairs in flight per CU
+    const int cap = tmax < 4 * BS ? tmax : (4 * BS < 2048 ? 4 * BS : 2048);
+    hipLaunchKernelGGL((k_narrow<BS, K>), dim3(cfg.groups), dim3(BS), 24 * (size_t)cap, s, B, M, P, li, lf, tmax,
+                       cfg.use_queue, q, cap);
 }
 
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
@@ -1733,6 +1801,8 @@ void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const Na
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, P.hit_top, 0ull, (int64_t)1);
     const int key = P.prov ? 25604 : cfg.block * 100 + cfg.items;
     switch (key) {
+        case 6404: narrow_t<64, 4>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        case 6408: narrow_t<64, 8>(B, M, P, cfg, li, lf, tmax, q, s); break;
         case 12804: narrow_t<128, 4>(B, M, P, cfg, li, lf, tmax, q, s); break;
         case 12808: narrow_t<128, 8>(B, M, P, cfg, li, lf, tmax, q, s); break;
         case 25601: narrow_t<256, 1>(B, M, P, cfg, li, lf, tmax, q, s); break;
@@ -1770,11 +1840,17 @@ void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, i
     hipLaunchKernelGGL(k_topo_zero, dim3(g), dim3(256), 0, s, P, T, cur_active);
 }
 
+static void launch_pass_a(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, hipStream_t s) {
+    launch_fill_u64(T.top, 0ull, 5, s);
+    hipLaunchKernelGGL(k_tp_alloc, dim3((unsigned)((T.pass_groups + 255) / 256 * 4)), dim3(256), 0, s, P, T);
+    hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(TP_PASS_A_THREADS),
+                       84 * TP_PASS_A_SEGS + TP_PASS_A_KEYS + 8 * TP_PASS_A_WORDS, s, P, M, B, T,
+                       (int32_t)TP_PASS_A_SEGS);
+}
+
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
                      const TopoBufs& T, const SimParams& sp, hipStream_t s) {
-    launch_fill_u64(T.top, 0ull, 4, s);
-    hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(TP_PASS_A_THREADS),
-                       84 * TP_PASS_A_SEGS + TP_PASS_A_KEYS, s, P, M, B, T, (int32_t)TP_PASS_A_SEGS);
+    launch_pass_a(P, M, B, T, s);
     hipLaunchKernelGGL(k_tp_walk_b, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, T, sp.split,
                        T.lds_bytes);
     hipLaunchKernelGGL(k_tp_pass_c, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T);
@@ -1783,10 +1859,8 @@ void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const 
 }
 void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int32_t split,
                      hipStream_t s) {
-    launch_fill_u64(T.top, 0ull, 4, s);
     if (T.ej) hipLaunchKernelGGL(k_tp_endpoint_jac, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T);
-    hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(TP_PASS_A_THREADS),
-                       84 * TP_PASS_A_SEGS + TP_PASS_A_KEYS, s, P, M, B, T, (int32_t)TP_PASS_A_SEGS);
+    launch_pass_a(P, M, B, T, s);
     hipLaunchKernelGGL(k_tp_walk_b, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, T, split,
                        T.lds_bytes);
     hipLaunchKernelGGL(k_tp_pass_c, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T);
