@@ -1512,8 +1512,9 @@ zdx_status zdx_sim_autotune(zdx_sim* s, int32_t* out_block, int32_t* out_groups)
         for (auto& cb : cand) {
             for (int g : gpc) {
                 NarrowCfg nc{cb[0], cb[1], persistent_groups(s->ctx, g), 1};
-                float tot = 0;
-                for (int rep = 0; rep < 2; ++rep) {
+                // one untimed launch, then the median of four timed ones
+                float t4[4] = {0, 0, 0, 0};
+                for (int rep = 0; rep < 5; ++rep) {
                     launch_fill_i32(s->d_topo_n.p, 0, 1, st);
                     HIPC(hipEventRecord(e0, st));
                     launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->stage_cap, s->tmax, s->d_queue.p,
@@ -1522,8 +1523,10 @@ zdx_status zdx_sim_autotune(zdx_sim* s, int32_t* out_block, int32_t* out_groups)
                     HIPC(hipEventSynchronize(e1));
                     float ms = 0;
                     HIPC(hipEventElapsedTime(&ms, e0, e1));
-                    if (rep > 0) tot += ms;
+                    if (rep > 0) t4[rep - 1] = ms;
                 }
+                std::sort(t4, t4 + 4);
+                const float tot = 0.5f * (t4[1] + t4[2]);
                 if (tot < best) { best = tot; bestc = nc; bestg = g; }
             }
         }

@@ -1276,94 +1276,103 @@ __device__ inline auto with_walk_scratch(const TopoBufs& T, int64_t ni, int64_t 
     return body(T.scr_i + (int64_t)blockIdx.x * T.scr_ni, T.scr_d + (int64_t)blockIdx.x * T.scr_nd);
 }
 
+struct WalkB {
+    int ncomp;       // -1: not evaluated (the step is retried with larger buffers)
+    long long to;    // first T-junction word of the item, -1: did not fit
+};
+__device__ inline WalkB tp_walk_b_item(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T,
+                                       int32_t split, int32_t lds_bytes, char* s_walk, const LaneDev& wp, int it) {
+    const int R = T.adj_row;
+    const int p = P.topo_list[it];
+    const int S = P.nseg[p];
+    const int64_t sb = P.seg_off[p];
+    const double h = M.tol[B.mesh[P.pi[p]]];
+    const long long ao = T.adj_off[it], so = T.snap_off[it];
+    int ncomp = -1, tjn = 0;
+    wp.start();
+    if (ao >= 0 && S <= T.seg_max) {
+        const double* seg = P.arena + 6 * sb;
+        ncomp = with_walk_scratch(T, tp_walk_b_ints(S, R), tp_walk_b_dbls(S), s_walk, lds_bytes,
+                                  [&](int* I, double*) {
+            int* adj = I + tp_b_ints(S);   // counts (S) then rows (S x R)
+            const int* src = T.adj + ao;
+            wave_copy(adj, src, S * (R + 1));
+            __syncthreads();
+            wp.mark(0);
+            int ok = 1;
+            for (int k = lane_id(); k < S; k += WAVE) ok &= adj[k] <= R;
+            if (!__all(ok)) return -1;
+            return tp_components_and_vertices(wp, seg, S, h, split != 0, adj, adj + S, R, T.snapw + so, I,
+                                              T.comp + TPC_N * sb, T.V + 6 * sb, T.vcell + 6 * sb,
+                                              T.E + 2 * sb, &tjn, T.ej ? T.ej + 72 * sb : nullptr,
+                                              T.vj ? T.vj + 72 * sb : nullptr);
+        });
+    }
+#ifdef ZDX_WALK_PROFILE
+    if (lane_id() == 0 && it < 8192) {
+        const int64_t ib = ((int64_t)tp_walk_b_ints(S, R) * 4 + 7) & ~(int64_t)7;
+        g_walk_pairs[12 * it] = S;
+        g_walk_pairs[12 * it + 1] = R;
+        g_walk_pairs[12 * it + 2] = ib + tp_walk_b_dbls(S) * 8 <= lds_bytes ? 1 : 0;
+        g_walk_pairs[12 * it + 3] = (long long)(wall_clock64() - wp.t_begin);
+        for (int k = 0; k < 8; ++k) g_walk_pairs[12 * it + 4 + k] = wp.ph[k];
+    }
+#endif
+    long long to = -1;
+    if (lane_id() == 0) {
+        if (ncomp >= 0) {
+            to = (long long)atomicAdd(T.top + 2, (unsigned long long)tjn);
+            if (to + tjn > T.tj_cap) to = -1;
+        }
+        T.ncomp[it] = ncomp;
+        T.tj_off[it] = ncomp >= 0 ? to : -1;
+    }
+    to = __shfl(to, 0, WAVE);
+    __syncthreads();
+    wp.mark(4);
+    return WalkB{ncomp, ncomp >= 0 ? to : -1};
+}
+
 // Walk B: components, snapped vertices and edges of each pair; allocates the
 // pair's T-junction words.  The adjacency rows are copied into the scratch.
 __global__ void __launch_bounds__(64) k_tp_walk_b(DPairs P, DMeshes M, DBodies B, TopoBufs T, int32_t split,
                                                   int32_t lds_bytes) {
     extern __shared__ char s_walk[];
     const int n = *P.topo_n;
-    const int R = T.adj_row;
     LaneDev wp;
-    for (int it = blockIdx.x; it < n; it += gridDim.x) {
-        const int p = P.topo_list[it];
-        const int S = P.nseg[p];
-        const int64_t sb = P.seg_off[p];
-        const double h = M.tol[B.mesh[P.pi[p]]];
-        const long long ao = T.adj_off[it], so = T.snap_off[it];
-        int ncomp = -1, tjn = 0;
-        wp.start();
-        if (ao >= 0 && S <= T.seg_max) {
-            const double* seg = P.arena + 6 * sb;
-            ncomp = with_walk_scratch(T, tp_walk_b_ints(S, R), tp_walk_b_dbls(S), s_walk, lds_bytes,
-                                      [&](int* I, double*) {
-                int* adj = I + tp_b_ints(S);   // counts (S) then rows (S x R)
-                const int* src = T.adj + ao;
-                wave_copy(adj, src, S * (R + 1));
-                __syncthreads();
-                wp.mark(0);
-                int ok = 1;
-                for (int k = lane_id(); k < S; k += WAVE) ok &= adj[k] <= R;
-                if (!__all(ok)) return -1;
-                return tp_components_and_vertices(wp, seg, S, h, split != 0, adj, adj + S, R, T.snapw + so, I,
-                                                  T.comp + TPC_N * sb, T.V + 6 * sb, T.vcell + 6 * sb,
-                                                  T.E + 2 * sb, &tjn, T.ej ? T.ej + 72 * sb : nullptr,
-                                                  T.vj ? T.vj + 72 * sb : nullptr);
-            });
-        }
-#ifdef ZDX_WALK_PROFILE
-        if (lane_id() == 0 && it < 8192) {
-            const int64_t ib = ((int64_t)tp_walk_b_ints(S, R) * 4 + 7) & ~(int64_t)7;
-            g_walk_pairs[12 * it] = S;
-            g_walk_pairs[12 * it + 1] = R;
-            g_walk_pairs[12 * it + 2] = ib + tp_walk_b_dbls(S) * 8 <= lds_bytes ? 1 : 0;
-            g_walk_pairs[12 * it + 3] = (long long)(wall_clock64() - wp.t_begin);
-            for (int k = 0; k < 8; ++k) g_walk_pairs[12 * it + 4 + k] = wp.ph[k];
-        }
-#endif
-        if (lane_id() == 0) {
-            long long to = -1;
-            if (ncomp >= 0) {
-                to = (long long)atomicAdd(T.top + 2, (unsigned long long)tjn);
-                if (to + tjn > T.tj_cap) to = -1;
-            }
-            T.ncomp[it] = ncomp;
-            T.tj_off[it] = ncomp >= 0 ? to : -1;
-        }
-        __syncthreads();
-        wp.mark(4);
-    }
+    for (int it = blockIdx.x; it < n; it += gridDim.x) tp_walk_b_item(P, M, B, T, split, lds_bytes, s_walk, wp, it);
 }
 
-// Pass C: T-junction hit words of every component; each wave covers 64
-// vertices of one edge row per ballot.
-__global__ void __launch_bounds__(256) k_tp_pass_c(DPairs P, DMeshes M, DBodies B, TopoBufs T) {
-    const int n = *P.topo_n;
-    for (int it = blockIdx.x; it < n; it += gridDim.x) {
-        const int ncomp = T.ncomp[it];
-        const long long to = T.tj_off[it];
-        if (ncomp <= 0 || to < 0) continue;
-        const int p = P.topo_list[it];
-        const int64_t sb = P.seg_off[p];
-        const double h = M.tol[B.mesh[P.pi[p]]];
-        const int* comp = T.comp + TPC_N * sb;
-        for (int q = 0; q < ncomp; ++q) {
-            const int* cr = comp + TPC_N * q;
-            const int nv = cr[TPC_NV], ne = cr[TPC_NE];
-            const double* V = T.V + 6 * sb + 3 * (int64_t)cr[TPC_VB];
-            const int* vc = T.vcell + 6 * sb + 3 * (int64_t)cr[TPC_VB];
-            const int* E = T.E + 2 * sb + 2 * (int64_t)cr[TPC_EB];
-            uint64_t* w = T.tjw + to + cr[TPC_TW];
-            const int rl = 64 * tp_row_words(nv);
-            const int items = ne * rl;
-            for (int kb = 0; kb < items; kb += blockDim.x) {
-                const int t = kb + threadIdx.x;
-                const int e = t / rl, v = t - (t / rl) * rl;
-                const bool hit = t < items && v < nv && tp_tj_hit(V, vc, E, h, e, v);
-                const uint64_t m = __ballot(hit);
-                if (lane_id() == 0 && t < items) w[t >> 6] = m;
-            }
+// Pass C: T-junction hit words of every component of an item; each wave of
+// the workgroup covers 64 vertices of one edge row per ballot.
+__device__ inline void tp_pass_c_item(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int it,
+                                      int ncomp, long long to) {
+    if (ncomp <= 0 || to < 0) return;
+    const int p = P.topo_list[it];
+    const int64_t sb = P.seg_off[p];
+    const double h = M.tol[B.mesh[P.pi[p]]];
+    const int* comp = T.comp + TPC_N * sb;
+    for (int q = 0; q < ncomp; ++q) {
+        const int* cr = comp + TPC_N * q;
+        const int nv = cr[TPC_NV], ne = cr[TPC_NE];
+        const double* V = T.V + 6 * sb + 3 * (int64_t)cr[TPC_VB];
+        const int* vc = T.vcell + 6 * sb + 3 * (int64_t)cr[TPC_VB];
+        const int* E = T.E + 2 * sb + 2 * (int64_t)cr[TPC_EB];
+        uint64_t* w = T.tjw + to + cr[TPC_TW];
+        const int rl = 64 * tp_row_words(nv);
+        const int items = ne * rl;
+        for (int kb = 0; kb < items; kb += blockDim.x) {
+            const int t = kb + threadIdx.x;
+            const int e = t / rl, v = t - (t / rl) * rl;
+            const bool hit = t < items && v < nv && tp_tj_hit(V, vc, E, h, e, v);
+            const uint64_t m = __ballot(hit);
+            if (lane_id() == 0 && t < items) w[t >> 6] = m;
         }
     }
+}
+__global__ void __launch_bounds__(256) k_tp_pass_c(DPairs P, DMeshes M, DBodies B, TopoBufs T) {
+    const int n = *P.topo_n;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) tp_pass_c_item(P, M, B, T, it, T.ncomp[it], T.tj_off[it]);
 }
 
 // Walk D with its inputs copied to the scratch: edges, vertex cells.
@@ -1384,60 +1393,80 @@ __device__ inline TopoOut tp_walk_d(const LaneDev& wp, const TopoBufs& T, int S,
     });
 }
 
+__device__ inline void tp_walk_d_sim_item(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat,
+                                          const HistBufs& H, const TopoBufs& T, const SimParams& sp, int nprev,
+                                          int32_t lds_bytes, char* s_walk, const LaneDev& wp, int it, int ncomp,
+                                          long long to) {
+    const int p = P.topo_list[it];
+    const int i = P.pi[p], j = P.pj[p];
+    const int S = P.nseg[p];
+    const int mA = B.mesh[i];
+    const Body a = load_body(B.state + 16 * (int64_t)i);
+    const Body b = load_body(B.state + 16 * (int64_t)j);
+    const Mat ma = load_mat(mat, i), mb = load_mat(mat, j);
+    // previous tangential spring of this pair (the reference's map entry,
+    // erased when the pair was not in contact on the previous step)
+    const int64_t key = ((int64_t)i << 32) | (int64_t)(uint32_t)j;
+    V3 ds = v3(0.0, 0.0, 0.0);
+    {
+        int lo = 0, hi = nprev - 1;
+        while (lo <= hi) {
+            const int mid = (lo + hi) >> 1;
+            const int64_t k = H.prev_key[mid];
+            if (k == key) {
+                if (H.prev_active[mid]) ds = ld3(H.prev_ds + 3 * (int64_t)mid);
+                break;
+            }
+            if (k < key) lo = mid + 1; else hi = mid - 1;
+        }
+    }
+    SimSink sink{T.contrib + 9 * (int64_t)p * T.lmax, T.lmax, &a, &b, &ma, &mb, sp.dt, &ds, sp.force_model,
+                 sp.k_minimal};
+    wp.start();
+    TopoOut o;
+    o.comps = 0; o.loops_raw = 0; o.loops = 0; o.status = 0;
+    if (ncomp < 0 || to < 0) o.status = TOPO_E2_OVERFLOW;
+    else if (ncomp > 0)
+        o = tp_walk_d(wp, T, S, ncomp, P.seg_off[p], to, M.tol[mA], M.area_eps[mA], sub(b.pos, a.pos), s_walk,
+                      lds_bytes, sink);
+#ifdef ZDX_WALK_PROFILE
+    if (lane_id() == 0 && it < 8192) g_topo_pairs[4 * it + 2] = (long long)(wall_clock64() - wp.t_begin);
+#endif
+    if (lane_id() == 0) {
+        T.ncontrib[p] = o.loops < T.lmax ? o.loops : T.lmax;
+        T.comps[p] = o.comps;
+        T.loops_raw[p] = o.loops_raw;
+        T.tstatus[p] = o.status;
+        H.cur_ds[3 * (int64_t)p] = ds.x; H.cur_ds[3 * (int64_t)p + 1] = ds.y; H.cur_ds[3 * (int64_t)p + 2] = ds.z;
+        H.cur_active[p] = o.loops > 0 ? 1 : 0;
+        if (o.status) atomicOr(P.status, (int)ST_TOPO);
+    }
+    __syncthreads();
+}
 __global__ void __launch_bounds__(64) k_tp_walk_d_sim(DPairs P, DMeshes M, DBodies B, DMat mat, HistBufs H,
                                                       TopoBufs T, SimParams sp, int32_t lds_bytes) {
     extern __shared__ char s_walk[];
     const int n = *P.topo_n;
     const int nprev = *H.prev_n;
     LaneDev wp;
+    for (int it = blockIdx.x; it < n; it += gridDim.x)
+        tp_walk_d_sim_item(P, M, B, mat, H, T, sp, nprev, lds_bytes, s_walk, wp, it, T.ncomp[it], T.tj_off[it]);
+}
+
+// Walk B, pass C and walk D of one item in a row on one wave (the time step):
+// each pair goes through its whole chain without waiting for the slowest
+// pair of each phase.  The walks' LDS scratch is reused from one to the next.
+__global__ void __launch_bounds__(64) k_tp_walk_bcd_sim(DPairs P, DMeshes M, DBodies B, DMat mat, HistBufs H,
+                                                        TopoBufs T, SimParams sp, int32_t lds_bytes) {
+    extern __shared__ char s_walk[];
+    const int n = *P.topo_n;
+    const int nprev = *H.prev_n;
+    LaneDev wp;
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
-        const int p = P.topo_list[it];
-        const int i = P.pi[p], j = P.pj[p];
-        const int S = P.nseg[p];
-        const int mA = B.mesh[i];
-        const Body a = load_body(B.state + 16 * (int64_t)i);
-        const Body b = load_body(B.state + 16 * (int64_t)j);
-        const Mat ma = load_mat(mat, i), mb = load_mat(mat, j);
-        // previous tangential spring of this pair (the reference's map entry,
-        // erased when the pair was not in contact on the previous step)
-        const int64_t key = ((int64_t)i << 32) | (int64_t)(uint32_t)j;
-        V3 ds = v3(0.0, 0.0, 0.0);
-        {
-            int lo = 0, hi = nprev - 1;
-            while (lo <= hi) {
-                const int mid = (lo + hi) >> 1;
-                const int64_t k = H.prev_key[mid];
-                if (k == key) {
-                    if (H.prev_active[mid]) ds = ld3(H.prev_ds + 3 * (int64_t)mid);
-                    break;
-                }
-                if (k < key) lo = mid + 1; else hi = mid - 1;
-            }
-        }
-        SimSink sink{T.contrib + 9 * (int64_t)p * T.lmax, T.lmax, &a, &b, &ma, &mb, sp.dt, &ds, sp.force_model,
-                     sp.k_minimal};
-        wp.start();
-        const int ncomp = T.ncomp[it];
-        const long long to = T.tj_off[it];
-        TopoOut o;
-        o.comps = 0; o.loops_raw = 0; o.loops = 0; o.status = 0;
-        if (ncomp < 0 || to < 0) o.status = TOPO_E2_OVERFLOW;
-        else if (ncomp > 0)
-            o = tp_walk_d(wp, T, S, ncomp, P.seg_off[p], to, M.tol[mA], M.area_eps[mA], sub(b.pos, a.pos), s_walk,
-                          lds_bytes, sink);
-#ifdef ZDX_WALK_PROFILE
-        if (lane_id() == 0 && it < 8192) g_topo_pairs[4 * it + 2] = (long long)(wall_clock64() - wp.t_begin);
-#endif
-        if (lane_id() == 0) {
-            T.ncontrib[p] = o.loops < T.lmax ? o.loops : T.lmax;
-            T.comps[p] = o.comps;
-            T.loops_raw[p] = o.loops_raw;
-            T.tstatus[p] = o.status;
-            H.cur_ds[3 * (int64_t)p] = ds.x; H.cur_ds[3 * (int64_t)p + 1] = ds.y; H.cur_ds[3 * (int64_t)p + 2] = ds.z;
-            H.cur_active[p] = o.loops > 0 ? 1 : 0;
-            if (o.status) atomicOr(P.status, (int)ST_TOPO);
-        }
+        const WalkB r = tp_walk_b_item(P, M, B, T, sp.split, lds_bytes, s_walk, wp, it);
+        tp_pass_c_item(P, M, B, T, it, r.ncomp, r.to);
         __syncthreads();
+        tp_walk_d_sim_item(P, M, B, mat, H, T, sp, nprev, lds_bytes, s_walk, wp, it, r.ncomp, r.to);
     }
 }
 
@@ -1851,11 +1880,8 @@ static void launch_pass_a(const DPairs& P, const DMeshes& M, const DBodies& B, c
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
                      const TopoBufs& T, const SimParams& sp, hipStream_t s) {
     launch_pass_a(P, M, B, T, s);
-    hipLaunchKernelGGL(k_tp_walk_b, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, T, sp.split,
-                       T.lds_bytes);
-    hipLaunchKernelGGL(k_tp_pass_c, dim3((unsigned)T.pass_groups), dim3(256), 0, s, P, M, B, T);
-    hipLaunchKernelGGL(k_tp_walk_d_sim, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, mat, H, T, sp,
-                       T.lds_bytes);
+    hipLaunchKernelGGL(k_tp_walk_bcd_sim, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, mat, H, T,
+                       sp, T.lds_bytes);
 }
 void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int32_t split,
                      hipStream_t s) {
