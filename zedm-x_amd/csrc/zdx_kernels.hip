@@ -591,7 +591,7 @@ constexpr int CULL_PER_THREAD = 8;
 // upper bound nA * nB so that every pair's hits are contiguous.  Pairs with
 // hits are listed for part 2 (k_narrow_exact).
 template <int BS, int K>
-__global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, int32_t* list_i, float* list_f,
+__global__ void __launch_bounds__(BS, BS >= 256 ? 5 : 1) k_narrow(DBodies B, DMeshes M, DPairs P, int32_t* list_i, float* list_f,
                                                int32_t tmax, int32_t use_queue, unsigned int* queue, int32_t lds_cap) {
     // Body B's culled float boxes (24 bytes per triangle, read for every item
     // of the sweep) live in dynamic LDS when there are at most lds_cap of
@@ -602,6 +602,7 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
     __shared__ int s_wave[BS / WAVE];
     __shared__ int s_item;
     __shared__ unsigned char s_oc[2 * OC_CAP];
+    __shared__ unsigned long long s_kmask[2 * WAVE];   // cull: keep ballots per (layer, wave), A then B
     __shared__ unsigned long long s_base;
     const int tid = threadIdx.x;
     int32_t* LA = list_i + (int64_t)blockIdx.x * 8 * tmax;
@@ -657,13 +658,19 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
                 }
             }
             __syncthreads();
+            // CPT layers of BS consecutive triangles per round (lane-consecutive
+            // index loads); the keep ballots of every (layer, wave) go to LDS
+            // and each wave turns their popcounts into ranks in triangle order
             constexpr int CPT = CULL_PER_THREAD;
+            constexpr int NW = BS / WAVE;
+            static_assert(CPT * NW <= WAVE, "keep masks of one round fit one wave");
+            const int wid = tid / WAVE, lane = tid % WAVE;
+            const uint64_t below = (1ull << lane) - 1ull;
             for (int base = 0; base < TA || base < TB; base += BS * CPT) {
-                const int ta0 = base + tid * CPT;
                 unsigned ka = 0u, kb = 0u;
 #pragma unroll
                 for (int c = 0; c < CPT; ++c) {
-                    const int t = ta0 + c;
+                    const int t = base + c * BS + tid;
                     if (t < TA) {
                         const int32_t* ix = tiA + 3 * t;
                         if ((s_oc[ix[0]] & s_oc[ix[1]] & s_oc[ix[2]]) == 0) ka |= 1u << c;
@@ -673,13 +680,34 @@ __global__ void __launch_bounds__(BS) k_narrow(DBodies B, DMeshes M, DPairs P, i
                         if ((s_oc[OC_CAP + ix[0]] & s_oc[OC_CAP + ix[1]] & s_oc[OC_CAP + ix[2]]) == 0) kb |= 1u << c;
                     }
                 }
-                int tot;
-                const int off = block_excl<BS>(__popc(ka) | (__popc(kb) << 16), s_wave, tot);
-                int oa = nA + (off & 0xffff), ob = nB + (off >> 16);
-                for (unsigned m = ka; m; m &= m - 1) LA[oa++] = ta0 + __ffs(m) - 1;
-                for (unsigned m = kb; m; m &= m - 1) LB[ob++] = ta0 + __ffs(m) - 1;
-                nA += tot & 0xffff;
-                nB += tot >> 16;
+#pragma unroll
+                for (int c = 0; c < CPT; ++c) {
+                    const uint64_t ma = __ballot((ka >> c) & 1u), mb = __ballot((kb >> c) & 1u);
+                    if (lane == 0) { s_kmask[c * NW + wid] = ma; s_kmask[WAVE + c * NW + wid] = mb; }
+                }
+                __syncthreads();
+                // inclusive prefix of the popcounts in (layer, wave) order, per side
+                const int q = lane;
+                int pa = q < CPT * NW ? __popcll(s_kmask[q]) : 0;
+                int pb = q < CPT * NW ? __popcll(s_kmask[WAVE + q]) : 0;
+#pragma unroll
+                for (int d = 1; d < WAVE; d <<= 1) {
+                    const int ya = __shfl_up(pa, d, WAVE), yb = __shfl_up(pb, d, WAVE);
+                    if (lane >= d) { pa += ya; pb += yb; }
+                }
+                const int tot_a = __shfl(pa, CPT * NW - 1, WAVE), tot_b = __shfl(pb, CPT * NW - 1, WAVE);
+#pragma unroll
+                for (int c = 0; c < CPT; ++c) {
+                    const int slot = c * NW + wid;
+                    const int ea = __shfl(pa, slot, WAVE) - __popcll(s_kmask[slot]);
+                    const int eb = __shfl(pb, slot, WAVE) - __popcll(s_kmask[WAVE + slot]);
+                    const int t = base + c * BS + tid;
+                    if ((ka >> c) & 1u) LA[nA + ea + __popcll(s_kmask[slot] & below)] = t;
+                    if ((kb >> c) & 1u) LB[nB + eb + __popcll(s_kmask[WAVE + slot] & below)] = t;
+                }
+                __syncthreads();   // the masks are rewritten by the next round
+                nA += tot_a;
+                nB += tot_b;
             }
             NP_MARK(1);
         } else {
