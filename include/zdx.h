@@ -72,6 +72,8 @@ typedef struct {
     double k;               /* stiffness of Fn = 0.5 k |Sn| */
     int32_t sensitivities;  /* 1: fill zdx_pair_out.loop_jac (no reference counterpart) */
     int32_t reserved;
+    double tol;             /* > 0: snap / adjacency tolerance of every pair (polyline.py --tol);
+                               0: mesh A's mean edge * 0.1 (main.cpp:2118-2121) */
 } zdx_contact_opts;
 
 enum { ZDX_MODE_PLAIN = 0, ZDX_MODE_BATCHED = 1 };

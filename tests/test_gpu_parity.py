@@ -70,6 +70,41 @@ def test_pairs_match_oracle_golden_cases(ctx, ref_inputs, mode):
             _check_against_oracle(res, p, ref, x[0])
 
 
+def _oracle_tol(t):
+    """t nudged up to the nearest double that the oracle's rule (mean edge *
+    0.1) reproduces exactly from the mean edge t / 0.1."""
+    for _ in range(64):
+        if (t / 0.1) * 0.1 == t:
+            return t
+        t = float(np.nextafter(t, np.inf))
+    raise AssertionError(t)
+
+
+@pytest.mark.parametrize("factor", [0.5, 2.0, 4.0])
+def test_pairs_tol_override_matches_oracle(ctx, ref_inputs, factor):
+    """opts.tol (polyline.py --tol) replaces mesh A's mean_edge * 0.1 for every
+    pair; the oracle gets a mean edge whose tenth is exactly that tol.  The
+    tolerances are kept near the default one: the reference's T-junction
+    search visits (edge length / tol)^3 grid cells per edge, which the oracle
+    repeats (the device does not scan cells)."""
+    cases = GI.pair_cases(ref_inputs)
+    for split in (True, False):
+        ids, refs, tols = [], [], []
+        for name, mA, mB, pose, sp in cases:
+            a, b = ctx.register_mesh(mA), ctx.register_mesh(mB)
+            ta, pa, _ = O.build_mesh(mA)
+            tb, _, _ = O.build_mesh(mB)
+            tol = _oracle_tol(factor * (pa[5] * 0.1))
+            ids.append((name, a, b, pose))
+            tols.append(tol)
+            refs.append(O.pair_contact(ta, tb, *pose, tol / 0.1, pa[6], split=split, k=1.5))
+        # one call per pair: the tolerance is per call
+        for x, ref, tol in zip(ids, refs, tols):
+            res = ctx.contact_pairs([x[1]], [x[2]], [x[3][0]], [x[3][1]], [x[3][2]], [x[3][3]], split=split,
+                                    mode=1, k=1.5, segments=True, tol=tol)
+            _check_against_oracle(res, 0, ref, (x[0], factor, split))
+
+
 def test_pairs_random_batch_plain_equals_batched_and_oracle(ctx):
     """A batch of uneven pairs (icospheres of 80-1280 triangles and concave
     stars) in one call; plain and batched must be bitwise identical, and each

@@ -1046,6 +1046,7 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
             T.lds_bytes = 32 * 1024;
             T.vmax = 4 * A.stage_cap + 16;
             T.k_stiff = opts->k;
+            T.tol_override = opts->tol > 0.0 ? opts->tol : 0.0;
             A.tw.ensure_arena(A.hit_cap);
             A.tw.ensure_items(n);
             A.tw.ensure_slices(T.nthreads, T.seg_max);

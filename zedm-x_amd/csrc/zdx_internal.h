@@ -75,6 +75,39 @@ bool write_vtk_particles(const std::string& path, int n, const HostMesh* const* 
                          const int32_t* counts);
 bool write_particle_state_txt(const std::string& path, int n, const double* state, const ParticleInit* init);
 
+// polyline.py's outputs (polyline.py:544-649): Python's repr of a float (the
+// text of f"{x}": shortest round-trip digits, fixed notation for decimal
+// exponents -4 <= e < 16 with ".0" on integral values, else d.ddde+XX), the
+// transformed-mesh POLYDATA (points merged within 1e-9 in order of first
+// use) and the contact-loop POLYDATA (one closed LINES cell per loop, one
+// VERTICES cell per contact point, CELL_DATA contact_id).
+std::string py_float_repr(double x);
+bool write_vtk_polydata_tris(const std::string& path, const double* tris, int64_t T, const std::string& comment);
+struct V3;
+// contact_points[k] with has_point[k] == 0 stands for polyline.py's None (counted, not written)
+bool write_vtk_polydata_polylines(const std::string& path, const std::vector<std::vector<V3>>& loops,
+                                  const std::vector<V3>& contact_points, const std::vector<char>& has_point,
+                                  const std::vector<int>& loop_contact_ids);
+
+// polyline.py's transform_points (polyline.py:36-52) of n points (3 doubles
+// each): q divided by its norm (identity below 1e-30), v + w t + qv x t with
+// t = 2 qv x v, plus the translation.
+void polyline_transform_points(const double* in, int64_t n, const double* t, const double* q, double* out);
+
+// polyline.py's contact of one closed loop of m points (polyline.py:193-218,
+// 676-679): Sn, Gn summed edge by edge from the first point, area = |Sn|,
+// nA = -Sn / area, xc0 = (nA x Gn) / area (none when area < 1e-14, then nA = 0),
+// Fn = 0.5 k area, F_A = Fn nA.  Dots and norms are numpy's (a running fused
+// multiply-add sum), products and differences are rounded one by one.
+struct PolylineContact {
+    double Sn[3], Gn[3], area, nA[3], xc0[3], Fn, FA[3];
+    bool has_xc0;
+};
+PolylineContact polyline_contact(const double* pts, int64_t m, double k);
+// polyline.py's loop filter (polyline.py:798-806): kept when it has at least
+// 3 points and the diagonal of its bounding box (numpy's norm) is >= 2 tol.
+bool polyline_keeps_loop(const double* pts, int64_t m, double tol);
+
 // per-particle constants derived at set-up (src/main.cpp:1955-1985)
 struct ParticleConst {
     double mass, inv_mass, Ibody[9], Ibinv[9], radius, eqr, young, poisson, mu, rest;

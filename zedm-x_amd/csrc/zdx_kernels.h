@@ -117,6 +117,7 @@ struct TopoBufs {
     double* vtx;               // pair * vmax * 3
     int32_t vmax;
     double k_stiff;
+    double tol_override;       // > 0: snap / adjacency tolerance for every pair (per-pair API)
 };
 
 struct SimParams {

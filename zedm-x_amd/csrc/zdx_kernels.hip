@@ -1104,6 +1104,12 @@ struct LaneDev {
 // (persistent); walk kernels: one wave per pair, scratch in LDS when it fits,
 // else the workgroup's slice of the global scratch.
 
+// Snap / adjacency tolerance of a pair: mesh A's (main.cpp:2118-2121) unless
+// the caller of the per-pair API gave one.
+__device__ inline double topo_tol(const TopoBufs& T, const DMeshes& M, int mesh_a) {
+    return T.tol_override > 0.0 ? T.tol_override : M.tol[mesh_a];
+}
+
 constexpr int TP_PASS_A_SEGS = 384;     // segments pass A stages in LDS (84 bytes each)
 constexpr int TP_PASS_A_KEYS = 8192;    // LDS bytes for a unit's rows x S key matrix
 constexpr int TP_PASS_A_WORDS = 256;    // snap words a unit builds in LDS at a time
@@ -1184,7 +1190,7 @@ __global__ void __launch_bounds__(TP_PASS_A_THREADS) k_tp_pass_a(DPairs P, DMesh
         const int p = P.topo_list[it];
         const int S = P.nseg[p];
         const int64_t sb = P.seg_off[p];
-        const double h = M.tol[B.mesh[P.pi[p]]];
+        const double h = topo_tol(T, M, B.mesh[P.pi[p]]);
         const double* gseg = P.arena + 6 * sb;
         int* gc9 = T.cell9 + 9 * sb;
         const bool staged = S <= lds_segs;
@@ -1314,7 +1320,7 @@ __device__ inline WalkB tp_walk_b_item(const DPairs& P, const DMeshes& M, const 
     const int p = P.topo_list[it];
     const int S = P.nseg[p];
     const int64_t sb = P.seg_off[p];
-    const double h = M.tol[B.mesh[P.pi[p]]];
+    const double h = topo_tol(T, M, B.mesh[P.pi[p]]);
     const long long ao = T.adj_off[it], so = T.snap_off[it];
     int ncomp = -1, tjn = 0;
     wp.start();
@@ -1378,7 +1384,7 @@ __device__ inline void tp_pass_c_item(const DPairs& P, const DMeshes& M, const D
     if (ncomp <= 0 || to < 0) return;
     const int p = P.topo_list[it];
     const int64_t sb = P.seg_off[p];
-    const double h = M.tol[B.mesh[P.pi[p]]];
+    const double h = topo_tol(T, M, B.mesh[P.pi[p]]);
     const int* comp = T.comp + TPC_N * sb;
     for (int q = 0; q < ncomp; ++q) {
         const int* cr = comp + TPC_N * q;
@@ -1455,7 +1461,7 @@ __device__ inline void tp_walk_d_sim_item(const DPairs& P, const DMeshes& M, con
     o.comps = 0; o.loops_raw = 0; o.loops = 0; o.status = 0;
     if (ncomp < 0 || to < 0) o.status = TOPO_E2_OVERFLOW;
     else if (ncomp > 0)
-        o = tp_walk_d(wp, T, S, ncomp, P.seg_off[p], to, M.tol[mA], M.area_eps[mA], sub(b.pos, a.pos), s_walk,
+        o = tp_walk_d(wp, T, S, ncomp, P.seg_off[p], to, topo_tol(T, M, mA), M.area_eps[mA], sub(b.pos, a.pos), s_walk,
                       lds_bytes, sink);
 #ifdef ZDX_WALK_PROFILE
     if (lane_id() == 0 && it < 8192) g_topo_pairs[4 * it + 2] = (long long)(wall_clock64() - wp.t_begin);
@@ -1568,7 +1574,7 @@ __global__ void __launch_bounds__(64) k_tp_walk_d_api(DPairs P, DMeshes M, DBodi
         o.comps = 0; o.loops_raw = 0; o.loops = 0; o.status = 0;
         if (ncomp < 0 || to < 0) o.status = TOPO_E2_OVERFLOW;
         else if (ncomp > 0)
-            o = tp_walk_d(wp, T, S, ncomp, P.seg_off[p], to, M.tol[mA], M.area_eps[mA], sub(pb, pa), s_walk,
+            o = tp_walk_d(wp, T, S, ncomp, P.seg_off[p], to, topo_tol(T, M, mA), M.area_eps[mA], sub(pb, pa), s_walk,
                           lds_bytes, sink);
         if (lane_id() == 0) {
             T.ncontrib[p] = o.loops;

@@ -44,7 +44,7 @@ class PairIn(C.Structure):
 
 class ContactOpts(C.Structure):
     _fields_ = [("split_contacts", C.c_int32), ("mode", C.c_int32), ("k", C.c_double), ("sensitivities", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("reserved", C.c_int32), ("tol", C.c_double)]
 
 
 class PairOut(C.Structure):
@@ -164,10 +164,11 @@ class Context:
         return mid.value
 
     def contact_pairs(self, mesh_a, mesh_b, pos_a, quat_a, pos_b, quat_b, split=True, mode=MODE_BATCHED, k=1.0,
-                      segments=False, vertices=True, sensitivities=False):
+                      segments=False, vertices=True, sensitivities=False, tol=0.0):
         """Batched per-pair contact evaluation.  Arrays are per pair; returns a
         dict with per-pair counters, per-loop records (ZDX_LOOP_REC layout) and,
-        optionally, loop vertices and narrow-phase segments."""
+        optionally, loop vertices and narrow-phase segments.  tol > 0 replaces
+        mesh A's snap / adjacency tolerance for every pair."""
         n = len(mesh_a)
         arr = (PairIn * max(n, 1))()
         pa, qa = np.asarray(pos_a, dtype=np.float64), np.asarray(quat_a, dtype=np.float64)
@@ -176,7 +177,7 @@ class Context:
             arr[p].mesh_a = int(mesh_a[p]); arr[p].mesh_b = int(mesh_b[p])
             arr[p].pos_a[:] = pa[p].tolist(); arr[p].quat_a[:] = qa[p].tolist()
             arr[p].pos_b[:] = pb[p].tolist(); arr[p].quat_b[:] = qb[p].tolist()
-        opts = ContactOpts(int(split), int(mode), float(k), int(bool(sensitivities)), 0)
+        opts = ContactOpts(int(split), int(mode), float(k), int(bool(sensitivities)), 0, float(tol))
         loop_cap, vtx_cap, seg_cap = max(64, 4 * n), max(1024, 64 * n), max(1024, 64 * n)
         for _ in range(4):
             tc = np.zeros(n, dtype=np.int64)
