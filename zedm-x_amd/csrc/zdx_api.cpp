@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -370,7 +371,13 @@ struct zdx_sim {
     hipGraph_t graph[2] = {nullptr, nullptr};
     bool graphs_valid = false;
 
-    ~zdx_sim() { drop_graphs(); }
+    // end-of-step report (pinned, coherent host memory written by k_report)
+    StepReport* rep = nullptr;
+
+    ~zdx_sim() {
+        drop_graphs();
+        if (rep) (void)hipHostFree(rep);
+    }
     void drop_graphs() {
         for (int k = 0; k < 2; ++k) {
             if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
@@ -557,6 +564,10 @@ void sim_setup_device(zdx_sim* s) {
     s->d_status.ensure(1); s->d_ctr.ensure(16); s->d_queue.ensure(2);
     s->d_bcnt.ensure(64); s->d_bcur.ensure(64);
     s->d_kt.ensure(12);
+    HIPC(hipHostMalloc((void**)&s->rep, sizeof(StepReport), hipHostMallocCoherent));
+    std::memset((void*)s->rep, 0, sizeof(StepReport));
+    launch_fill_u64(s->d_ctr.p, 0ull, 16, c->stream);
+    launch_fill_i32(s->d_status.p, 0, 1, c->stream);
     {
         int khz = 0;
         if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) == hipSuccess && khz > 0)
@@ -661,7 +672,7 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     // kernels (after the work ordering)
     unsigned long long* kt = s->d_kt.p;
     const int gmax = persistent_groups(s->ctx, s->np_groups_per_cu);
-    launch_stamp(kt + 0, st);
+    launch_step_begin(kt, s->d_status.p, st);
     launch_broadphase(s->bpbufs(par), st);
     launch_stamp(kt + 1, st);
     launch_need_list(s->N, s->d_state[par].p, s->d_radius.p, s->d_cnt_all.p, s->wallbufs(), s->d_need_flag.p,
@@ -721,6 +732,7 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
                           s->d_F.p, s->d_T.p, s->d_counts.p, s->sp, st);
     launch_stamp(kt + 6, st);
     launch_counters(P, T, s->N * s->W, s->d_wnc.p, s->d_ctr.p, st);
+    launch_report(s->d_status.p, s->d_ctr.p, kt, s->rep, st);
 }
 
 int32_t read_status(zdx_sim* s) {
@@ -795,11 +807,23 @@ bool grow_for(zdx_sim* s, int32_t status, int par) {
     return grown;
 }
 
-void read_counters(zdx_sim* s) {
-    unsigned long long c[16];
-    HIPC(hipMemcpyAsync(c, s->d_ctr.p, sizeof(c), hipMemcpyDeviceToHost, s->st()));
-    HIPC(hipStreamSynchronize(s->st()));
-    for (int k = 0; k < ZDX_N_COUNTERS; ++k) s->last_ctr[k] = (int64_t)c[k];
+// Waits for the step's report: spins on rep->seq (set by the step's last
+// kernel), asking the stream now and then whether it failed or drained.
+void wait_report(zdx_sim* s) {
+    volatile unsigned int* seq = &s->rep->seq;
+    for (uint64_t spin = 1;; ++spin) {
+        if (*seq) break;
+        if ((spin & 255) == 0) {
+            const hipError_t e = hipStreamQuery(s->st());
+            if (e == hipSuccess) {
+                if (*seq) break;
+                throw ApiError{ZDX_HIP_ERROR, "step finished without its report"};
+            }
+            if (e != hipErrorNotReady) HIPC(e);
+        }
+        __builtin_ia32_pause();
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
 }
 
 void step_once(zdx_sim* s, bool batched) {
@@ -811,7 +835,8 @@ void step_once(zdx_sim* s, bool batched) {
     }
     for (int attempt = 0; attempt < 12; ++attempt) {
         const int par = s->parity;
-        launch_fill_i32(s->d_status.p, 0, 1, st);
+        *(volatile unsigned int*)&s->rep->seq = 0u;
+        std::atomic_thread_fence(std::memory_order_seq_cst);
         // ZDX_GRAPHS=0 runs the batched sequence without graph capture (same
         // kernels and launch shapes), for tools that cannot trace graphs
         static const bool use_graphs = [] {
@@ -834,11 +859,12 @@ void step_once(zdx_sim* s, bool batched) {
         } else {
             enqueue_step(s, par, false);
         }
-        int32_t status = read_status(s);
+        wait_report(s);
+        const StepReport& r = *s->rep;
+        const int32_t status = r.status;
         if (status == 0) {
-            read_counters(s);
-            unsigned long long kt[9];
-            HIPC(hipMemcpy(kt, s->d_kt.p, sizeof(kt), hipMemcpyDeviceToHost));
+            for (int k = 0; k < ZDX_N_COUNTERS; ++k) s->last_ctr[k] = (int64_t)r.ctr[k];
+            const unsigned long long* kt = r.kt;
             // acc_ms: transform, broad phase, narrow, topology, walls (on its
             // own stream, overlapping narrow and topology), fold, narrow kernels
             const int a[7] = {1, 0, 2, 3, 2, 5, 8}, b[7] = {2, 1, 3, 4, 7, 6, 3};

@@ -137,6 +137,25 @@ struct NarrowCfg {
 
 // Writes the device clock into *slot when the stream reaches it (NULL: no-op).
 void launch_stamp(unsigned long long* slot, hipStream_t s);
+// First stamp of a step: the clock into kt[0] and *status = 0.
+void launch_step_begin(unsigned long long* kt, int32_t* status, hipStream_t s);
+
+// What the host reads after every step, in one block of pinned, coherent host
+// memory that the last kernel of the step writes: status, counters, stage
+// clocks, then seq = 1 (the host clears seq before it launches the step and
+// waits for it).
+struct StepReport {
+    int32_t status;
+    int32_t pad;
+    unsigned long long ctr[16];
+    unsigned long long kt[12];
+    unsigned int seq;
+    unsigned int pad2;
+};
+// Copies status, ctr and kt into rep, zeroes ctr for the next step's sums and
+// publishes rep->seq.
+void launch_report(const int32_t* status, unsigned long long* ctr, const unsigned long long* kt, StepReport* rep,
+                   hipStream_t s);
 // list/n_list: bodies to transform (device list and count; NULL = all bodies)
 void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s, const int32_t* list = nullptr, const int32_t* n_list = nullptr, int32_t groups = 0);
 // bodies that are in a candidate pair or near a wall triangle -> list
@@ -170,7 +189,8 @@ void launch_fold_integrate(int32_t n, const double* state_in, double* state_out,
                            int32_t adj_cap, const TopoBufs& T, int32_t n_walls, const double* wcontrib, const int32_t* wncontrib,
                            int32_t max_wall_tris, double* F, double* Tq, int32_t* counts, const SimParams& sp,
                            hipStream_t s);
-// counters[ZDX_N_COUNTERS] (int64) reduced from per-pair / per-wall arrays
+// counters[ZDX_N_COUNTERS] (int64) summed from per-pair / per-wall arrays into
+// `counters`, which must hold zeros (launch_report leaves them so)
 void launch_counters(const DPairs& P, const TopoBufs& T, int32_t n_pw, const int32_t* wncontrib,
                      unsigned long long* counters, hipStream_t s);
 void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);

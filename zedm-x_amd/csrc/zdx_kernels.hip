@@ -94,6 +94,31 @@ __device__ inline V3 ld3(const double* p) { return v3(p[0], p[1], p[2]); }
 // A per-kernel scheme with an atomic from every wave would serialise tens of
 // thousands of same-address atomics per launch.
 __global__ void k_stamp(unsigned long long* slot) { *slot = (unsigned long long)wall_clock64(); }
+__global__ void k_step_begin(unsigned long long* kt, int32_t* status) {
+    kt[0] = (unsigned long long)wall_clock64();
+    *status = 0;
+}
+
+// The end-of-step report into host memory (fine-grained, so the stores reach
+// it in order once fenced); seq goes last.
+__global__ void __launch_bounds__(64) k_report(const int32_t* status, unsigned long long* ctr,
+                                               const unsigned long long* kt, StepReport* rep) {
+    const int t = threadIdx.x;
+    if (t < 16) {
+        rep->ctr[t] = ctr[t];
+        ctr[t] = 0ull;
+    } else if (t < 28) {
+        rep->kt[t - 16] = kt[t - 16];
+    } else if (t == 28) {
+        rep->status = *status;
+    }
+    __threadfence_system();
+    __syncthreads();
+    if (t == 0) {
+        __threadfence_system();
+        *(volatile unsigned int*)&rep->seq = 1u;
+    }
+}
 
 // ------------------------------------------------------------ transform
 // World vertices of the listed bodies.  quat_rotate (main.cpp:93-98)
@@ -1808,6 +1833,13 @@ extern "C" int zdx_walk_profile(unsigned long long* out) {
 void launch_stamp(unsigned long long* slot, hipStream_t s) {
     if (slot) hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, s, slot);
 }
+void launch_step_begin(unsigned long long* kt, int32_t* status, hipStream_t s) {
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, s, kt, status);
+}
+void launch_report(const int32_t* status, unsigned long long* ctr, const unsigned long long* kt, StepReport* rep,
+                   hipStream_t s) {
+    hipLaunchKernelGGL(k_report, dim3(1), dim3(64), 0, s, status, ctr, kt, rep);
+}
 
 void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s, const int32_t* list,
                       const int32_t* n_list, int32_t groups) {
@@ -1947,7 +1979,6 @@ void launch_fold_integrate(int32_t n, const double* sin, double* sout, const DMa
 }
 void launch_counters(const DPairs& P, const TopoBufs& T, int32_t n_pw, const int32_t* wnc, unsigned long long* ctr,
                      hipStream_t s) {
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctr, 0ull, (int64_t)16);
     hipLaunchKernelGGL(k_counters, dim3(32), dim3(256), 0, s, P, T, n_pw, wnc, ctr);
 }
 void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s) {
