@@ -299,8 +299,8 @@ struct zdx_sim {
     DBuf<int64_t> d_wv_off, d_wt_off;
     DBuf<double> d_wv, d_aabb;
     DBuf<int32_t> d_nondeg;
-    DBuf<int32_t> d_need_flag, d_need_off, d_need_list, d_n_need;
-    DBuf<int32_t> d_wall_flag, d_wall_off, d_wall_list, d_n_wall;
+    DBuf<int32_t> d_need_list, d_n_need;
+    DBuf<int32_t> d_wall_list, d_n_wall;
     DBuf<double> d_state[2];
     DBuf<double> d_F, d_T;
     DBuf<int32_t> d_counts;
@@ -308,7 +308,7 @@ struct zdx_sim {
     DBuf<DGrid> d_grid;
     DBuf<double> d_bp_part, d_spos;
     int32_t max_cells = 1 << 24;
-    DBuf<int32_t> d_cell_of, d_cell_count, d_cell_start, d_cell_cursor, d_sorted, d_cnt_up, d_cnt_all;
+    DBuf<int32_t> d_cell_of, d_cell_count, d_cell_start, d_sorted, d_cnt_up, d_cnt_all;
     DBuf<int32_t> d_pair_off, d_adj_off, d_adj_j, d_adj_pair, d_n_adj, d_scan_tmp;
     int32_t pair_cap = 0, adj_cap = 0;
     // pair sets (double-buffered: this step / previous step for the histories)
@@ -407,7 +407,7 @@ struct zdx_sim {
         b.n = N; b.state = d_state[par].p; b.radius = d_radius.p; b.grid = d_grid.p; b.max_cells = max_cells;
         b.part = d_bp_part.p; b.spos = d_spos.p;
         b.cell_of = d_cell_of.p; b.cell_count = d_cell_count.p; b.cell_start = d_cell_start.p;
-        b.cell_cursor = d_cell_cursor.p; b.sorted = d_sorted.p; b.cnt_up = d_cnt_up.p; b.cnt_all = d_cnt_all.p;
+        b.sorted = d_sorted.p; b.cnt_up = d_cnt_up.p; b.cnt_all = d_cnt_all.p;
         b.pair_off = d_pair_off.p; b.adj_off = d_adj_off.p; b.pi = ps[par].pi.p; b.pj = ps[par].pj.p;
         b.pkey = ps[par].key.p; b.pair_cap = pair_cap; b.adj_j = d_adj_j.p; b.adj_pair = d_adj_pair.p;
         b.adj_cap = adj_cap; b.n_pairs = ps[par].n.p; b.n_adj = d_n_adj.p; b.scan_tmp = d_scan_tmp.p;
@@ -547,8 +547,8 @@ void sim_setup_device(zdx_sim* s) {
     s->d_wv_off.upload(wvo.data(), N + 1, st); s->d_wt_off.upload(wto.data(), N + 1, st);
     s->d_wv.ensure(3 * (size_t)av); s->d_aabb.ensure(6 * (size_t)N);
     s->d_nondeg.ensure(N);
-    s->d_need_flag.ensure(N); s->d_need_off.ensure(N + 1); s->d_need_list.ensure(N); s->d_n_need.ensure(1);
-    s->d_wall_flag.ensure(N); s->d_wall_off.ensure(N + 1); s->d_wall_list.ensure(N); s->d_n_wall.ensure(1);
+    s->d_need_list.ensure(N); s->d_n_need.ensure(1);
+    s->d_wall_list.ensure(N); s->d_n_wall.ensure(1);
     s->d_state[0].ensure(16 * (size_t)N); s->d_state[1].ensure(16 * (size_t)N);
     s->d_F.ensure(3 * (size_t)N); s->d_T.ensure(3 * (size_t)N); s->d_counts.ensure(N);
     // broad phase
@@ -558,12 +558,14 @@ void sim_setup_device(zdx_sim* s) {
     int64_t cells = std::min<int64_t>((int64_t)1 << 24, std::max<int64_t>(64, 8 * (int64_t)N));
     s->max_cells = (int32_t)cells;
     s->d_cell_of.ensure(N); s->d_cell_count.ensure(cells); s->d_cell_start.ensure(cells + 1);
-    s->d_cell_cursor.ensure(cells); s->d_sorted.ensure(N); s->d_cnt_up.ensure(N); s->d_cnt_all.ensure(N);
+    s->d_sorted.ensure(N); s->d_cnt_up.ensure(N); s->d_cnt_all.ensure(N);
     s->d_pair_off.ensure(N + 1); s->d_adj_off.ensure(N + 1); s->d_n_adj.ensure(1);
-    s->d_scan_tmp.ensure((size_t)(std::max<int64_t>(cells, N) / 1024 + 4));
+    s->d_scan_tmp.ensure((size_t)(2 * (std::max<int64_t>(cells, N) / 1024 + 4)));
     s->d_status.ensure(1); s->d_ctr.ensure(16); s->d_queue.ensure(2);
     s->d_bcnt.ensure(64); s->d_bcur.ensure(64);
     s->d_kt.ensure(12);
+    // the cell counters return to zero within every broad phase (k_bp_scatter)
+    launch_fill_i32(s->d_cell_count.p, 0, cells, c->stream);
     HIPC(hipHostMalloc((void**)&s->rep, sizeof(StepReport), hipHostMallocCoherent));
     std::memset((void*)s->rep, 0, sizeof(StepReport));
     launch_fill_u64(s->d_ctr.p, 0ull, 16, c->stream);
@@ -672,12 +674,22 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     // kernels (after the work ordering)
     unsigned long long* kt = s->d_kt.p;
     const int gmax = persistent_groups(s->ctx, s->np_groups_per_cu);
-    launch_step_begin(kt, s->d_status.p, st);
+    {
+        StepZero z{};
+        z.i32[0] = s->d_status.p; z.n32[0] = 1;
+        z.i32[1] = s->d_n_need.p; z.n32[1] = 1;
+        z.i32[2] = s->d_n_wall.p; z.n32[2] = 1;
+        z.i32[3] = s->d_topo_n.p; z.n32[3] = 1;
+        z.i32[4] = (int32_t*)s->d_queue.p; z.n32[4] = 1;
+        z.i32[5] = batched ? s->d_bcnt.p : nullptr; z.n32[5] = 64;
+        z.u64[0] = s->d_hit_top.p; z.n64[0] = 1;
+        z.u64[1] = s->tw.top.p; z.n64[1] = 5;
+        launch_step_begin(kt, z, st);
+    }
     launch_broadphase(s->bpbufs(par), st);
     launch_stamp(kt + 1, st);
-    launch_need_list(s->N, s->d_state[par].p, s->d_radius.p, s->d_cnt_all.p, s->wallbufs(), s->d_need_flag.p,
-                     s->d_need_off.p, s->d_scan_tmp.p, s->d_need_list.p, s->d_n_need.p, s->d_wall_flag.p,
-                     s->d_wall_off.p, s->d_wall_list.p, s->d_n_wall.p, st);
+    launch_need_list(s->N, s->d_state[par].p, s->d_radius.p, s->d_cnt_all.p, s->wallbufs(), s->d_need_list.p,
+                     s->d_n_need.p, s->d_wall_list.p, s->d_n_wall.p, true, st);
     int g_transform = s->transform_groups(), g_narrow = gmax, g_topo = (int)s->topo_threads,
         g_walls = (int)s->wall_threads;
     if (!batched) {
@@ -705,13 +717,12 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
         HIPC(hipEventRecord(s->ctx->ev_join, st2));
     }
     DPairs P = s->dpairs(par, batched);
-    if (batched) launch_lpt_order(P, M, s->d_mesh.p, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st);
-    launch_fill_i32(s->d_topo_n.p, 0, 1, st);
+    if (batched) launch_lpt_order(P, M, s->d_mesh.p, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st, true);
     NarrowCfg nc = s->ncfg;
     nc.use_queue = batched ? 1 : 0;
     nc.groups = g_narrow;
     launch_stamp(kt + 8, st);
-    launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->stage_cap, s->tmax, s->d_queue.p, st);
+    launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->stage_cap, s->tmax, s->d_queue.p, st, true);
     launch_stamp(kt + 3, st);
     TopoBufs T = s->topobufs();
     if (!batched) {
@@ -723,7 +734,7 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     }
     HistBufs H = s->hist(par);
     launch_topo_zero(P, T, H.cur_active, s->pair_cap, st);
-    launch_topo_sim(P, M, B, mat, H, T, s->sp, st);
+    launch_topo_sim(P, M, B, mat, H, T, s->sp, st, true);
     launch_stamp(kt + 4, st);
     if (walls) HIPC(hipStreamWaitEvent(st, s->ctx->ev_join, 0));
     launch_stamp(kt + 5, st);
@@ -1521,9 +1532,8 @@ zdx_status zdx_sim_autotune(zdx_sim* s, int32_t* out_block, int32_t* out_groups)
             launch_fill_i32(s->d_status.p, 0, 1, st);
             launch_broadphase(s->bpbufs(par), st);
         }
-        launch_need_list(s->N, s->d_state[par].p, s->d_radius.p, s->d_cnt_all.p, s->wallbufs(), s->d_need_flag.p,
-                         s->d_need_off.p, s->d_scan_tmp.p, s->d_need_list.p, s->d_n_need.p, s->d_wall_flag.p,
-                     s->d_wall_off.p, s->d_wall_list.p, s->d_n_wall.p, st);
+        launch_need_list(s->N, s->d_state[par].p, s->d_radius.p, s->d_cnt_all.p, s->wallbufs(), s->d_need_list.p,
+                         s->d_n_need.p, s->d_wall_list.p, s->d_n_wall.p, false, st);
         launch_transform(B, M, st, s->d_need_list.p, s->d_n_need.p, s->transform_groups());
         DPairs P = s->dpairs(par, true);
         launch_lpt_order(P, M, s->d_mesh.p, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st);

@@ -485,16 +485,34 @@ ZDX_HD inline void basis(const V3& n, V3& u, V3& v) {
 // wv: world vertex array (3 doubles each); tri: vertex indices (3 per tri).
 // Scratch (shared by the lanes): pts (3 cap_pts doubles), poly buffers A/B
 // (2 (cap_pts + 8) doubles each).
+// V: the number of world vertices (every triangle corner is one of them).
 template <class WP>
-ZDX_HD inline bool wall_contact(const WP& wp, const double* wv, const int* tri, int T, const V3* wt, const V3& wn,
-                                double tol, SArr<double> pts, SArr<double> PA, SArr<double> PB, int cap_pts,
-                                double& area_out, V3& xc_out, V3& n_out, int& status) {
+ZDX_HD inline bool wall_contact(const WP& wp, const double* wv, const int* tri, int T, int V, const V3* wt,
+                                const V3& wn, double tol, SArr<double> pts, SArr<double> PA, SArr<double> PB,
+                                int cap_pts, double& area_out, V3& xc_out, V3& n_out, int& status) {
     const int W = WP::W;
     const int ln = wp.lane();
     V3 pn = wn;
     double pd = -dot(pn, wt[0]);
     const double t2 = tol * tol;
     int np = 0;
+    // An edge crosses the plane only when its endpoints' signed distances
+    // (computed below exactly as here) have a negative product, so only when
+    // some vertex lies strictly below and some strictly above: without both,
+    // the section is empty and the contact is rejected (np < 3) unchanged.
+    {
+        bool below = false, above = false;
+        for (int vb = 0; vb < V; vb += W) {
+            const int v = vb + ln;
+            if (v < V) {
+                const double d = dot(pn, v3(wv[3 * v], wv[3 * v + 1], wv[3 * v + 2])) + pd;
+                below = below || d < 0.0;
+                above = above || d > 0.0;
+            }
+            if (wp.ballot(below) != 0ull && wp.ballot(above) != 0ull) break;
+        }
+        if (!(wp.ballot(below) != 0ull && wp.ballot(above) != 0ull)) return false;
+    }
     const int nslots = 3 * T;
     for (int base = 0; base < nslots; base += W) {
         const int f = base + ln;

@@ -25,7 +25,6 @@ struct BroadPhaseBufs {
     int32_t* cell_of;          // per particle
     int32_t* cell_count;       // max_cells
     int32_t* cell_start;       // max_cells + 1
-    int32_t* cell_cursor;      // max_cells
     int32_t* sorted;           // per particle
     double* spos;              // per cell-order slot: x, y, z, radius
     int32_t* cnt_up;           // per particle
@@ -137,8 +136,15 @@ struct NarrowCfg {
 
 // Writes the device clock into *slot when the stream reaches it (NULL: no-op).
 void launch_stamp(unsigned long long* slot, hipStream_t s);
-// First stamp of a step: the clock into kt[0] and *status = 0.
-void launch_step_begin(unsigned long long* kt, int32_t* status, hipStream_t s);
+// Small arrays a step clears before its stages run (NULL entries skipped).
+struct StepZero {
+    int32_t* i32[8];
+    int32_t n32[8];
+    unsigned long long* u64[2];
+    int32_t n64[2];
+};
+// First kernel of a step: the clock into kt[0], then the StepZero arrays.
+void launch_step_begin(unsigned long long* kt, const StepZero& z, hipStream_t s);
 
 // What the host reads after every step, in one block of pinned, coherent host
 // memory that the last kernel of the step writes: status, counters, stage
@@ -159,24 +165,24 @@ void launch_report(const int32_t* status, unsigned long long* ctr, const unsigne
 // list/n_list: bodies to transform (device list and count; NULL = all bodies)
 void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s, const int32_t* list = nullptr, const int32_t* n_list = nullptr, int32_t groups = 0);
 // bodies that are in a candidate pair or near a wall triangle -> list
-// (and the particles near a wall triangle -> wlist)
+// (and the particles near a wall triangle -> wlist), in no fixed order;
+// zeroed: *n_list and *n_wlist already hold 0
 void launch_need_list(int32_t n, const double* state, const double* radius, const int32_t* cnt_all, const WallBufs& W,
-                      int32_t* flag, int32_t* off, int32_t* scan_tmp, int32_t* list, int32_t* n_list, int32_t* wflag,
-                      int32_t* woff, int32_t* wlist, int32_t* n_wlist, hipStream_t s);
+                      int32_t* list, int32_t* n_list, int32_t* wlist, int32_t* n_wlist, bool zeroed, hipStream_t s);
 void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s);
 // Narrow phase: cull + sweep into the hit arena, the exact tests per hit, and
 // the per-pair compaction into the segment arena.  seg_cap: most segments one
 // pair may store (ST_STAGE beyond); queue: the cull's work counter; resets
-// the queue, P.hit_top and P.exact_n itself.
+// the queue and P.hit_top itself unless the caller did (zeroed).
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* list_i,
-                   float* list_f, int32_t seg_cap, int32_t tmax, unsigned int* queue, hipStream_t s);
+                   float* list_f, int32_t seg_cap, int32_t tmax, unsigned int* queue, hipStream_t s, bool zeroed = false);
 void launch_lpt_order(const DPairs& P, const DMeshes& M, const int32_t* body_mesh, int32_t* order, int32_t* bucket_cnt,
-                      int32_t* bucket_cur, int32_t pair_cap, hipStream_t s);
+                      int32_t* bucket_cur, int32_t pair_cap, hipStream_t s, bool zeroed = false);
 // zero outputs of every pair (the topology kernels then visit only the
 // pairs of P.topo_list); cur_active may be NULL
 void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, int32_t pair_cap, hipStream_t s);
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
-                     const TopoBufs& T, const SimParams& sp, hipStream_t s);
+                     const TopoBufs& T, const SimParams& sp, hipStream_t s, bool zeroed = false);
 void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int32_t split,
                      hipStream_t s);
 // one wave per particle of `list` (particles near a wall triangle); scr holds

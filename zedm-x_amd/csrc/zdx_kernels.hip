@@ -94,9 +94,14 @@ __device__ inline V3 ld3(const double* p) { return v3(p[0], p[1], p[2]); }
 // A per-kernel scheme with an atomic from every wave would serialise tens of
 // thousands of same-address atomics per launch.
 __global__ void k_stamp(unsigned long long* slot) { *slot = (unsigned long long)wall_clock64(); }
-__global__ void k_step_begin(unsigned long long* kt, int32_t* status) {
-    kt[0] = (unsigned long long)wall_clock64();
-    *status = 0;
+__global__ void __launch_bounds__(64) k_step_begin(unsigned long long* kt, StepZero z) {
+    if (threadIdx.x == 0) kt[0] = (unsigned long long)wall_clock64();
+    for (int a = 0; a < 8; ++a)
+        if (z.i32[a])
+            for (int k = threadIdx.x; k < z.n32[a]; k += blockDim.x) z.i32[a][k] = 0;
+    for (int a = 0; a < 2; ++a)
+        if (z.u64[a])
+            for (int k = threadIdx.x; k < z.n64[a]; k += blockDim.x) z.u64[a][k] = 0ull;
 }
 
 // The end-of-step report into host memory (fine-grained, so the stores reach
@@ -189,81 +194,112 @@ __global__ void __launch_bounds__(256) k_transform(DBodies B, DMeshes M, const i
 }
 
 // ------------------------------------------------------------ scan
-// exclusive scan of int32 counts (length *n_ptr or n) into out[0..n], out[n] = total
-__global__ void __launch_bounds__(256) k_scan_reduce(const int32_t* in, const int32_t* n_ptr, int32_t n_fixed,
-                                                    int32_t* bsum) {
+// Exclusive scans of one or two int32 count arrays of the same length (*n_ptr
+// or n) into out[0..n], out[n] = total.  Block sums of array a live at
+// bsum + a * bstride (bstride >= blocks + 1).
+__global__ void __launch_bounds__(256) k_scan_reduce(const int32_t* in0, const int32_t* in1, const int32_t* n_ptr,
+                                                    int32_t n_fixed, int32_t* bsum, int32_t bstride) {
     const int n = n_ptr ? *n_ptr : n_fixed;
     const int64_t base = (int64_t)blockIdx.x * 1024;
-    int s = 0;
-    for (int k = 0; k < 4; ++k) {
-        int64_t i = base + threadIdx.x * 4 + k;
-        if (i < n) s += in[i];
+    __shared__ int sw[2][4];
+    const int na = in1 ? 2 : 1;
+    for (int a = 0; a < na; ++a) {
+        const int32_t* in = a ? in1 : in0;
+        int s = 0;
+        for (int k = 0; k < 4; ++k) {
+            int64_t i = base + threadIdx.x * 4 + k;
+            if (i < n) s += in[i];
+        }
+        for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, WAVE);
+        if (lane_id() == 0) sw[a][threadIdx.x / WAVE] = s;
     }
-    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, WAVE);
-    __shared__ int sw[4];
-    if (lane_id() == 0) sw[threadIdx.x / WAVE] = s;
     __syncthreads();
-    if (threadIdx.x == 0) bsum[blockIdx.x] = sw[0] + sw[1] + sw[2] + sw[3];
+    if (threadIdx.x < (unsigned)na)
+        bsum[threadIdx.x * (int64_t)bstride + blockIdx.x] =
+            sw[threadIdx.x][0] + sw[threadIdx.x][1] + sw[threadIdx.x][2] + sw[threadIdx.x][3];
 }
-__global__ void __launch_bounds__(1024) k_scan_blocks(int32_t* bsum, int32_t nb_fixed, const int32_t* n_ptr) {
+__global__ void __launch_bounds__(1024) k_scan_blocks(int32_t* bsum0, int32_t nb_fixed, const int32_t* n_ptr, int32_t na,
+                                                      int32_t bstride) {
     // single workgroup: exclusive scan of the block sums in place; bsum[nb] = total
     __shared__ int sw[16];
     const int n = n_ptr ? *n_ptr : 0;
     const int nb = n_ptr ? (n + 1023) / 1024 : nb_fixed;
-    int carry = 0;
-    for (int base = 0; base < nb; base += 1024) {
-        int i = base + threadIdx.x;
-        int v = (i < nb) ? bsum[i] : 0;
-        int x = v;
-        for (int d = 1; d < WAVE; d <<= 1) {
-            int y = __shfl_up(x, d, WAVE);
-            if (lane_id() >= d) x += y;
+    for (int a = 0; a < na; ++a) {
+        int32_t* bsum = bsum0 + a * (int64_t)bstride;
+        int carry = 0;
+        for (int base = 0; base < nb; base += 1024) {
+            int i = base + threadIdx.x;
+            int v = (i < nb) ? bsum[i] : 0;
+            int x = v;
+            for (int d = 1; d < WAVE; d <<= 1) {
+                int y = __shfl_up(x, d, WAVE);
+                if (lane_id() >= d) x += y;
+            }
+            if (lane_id() == WAVE - 1) sw[threadIdx.x / WAVE] = x;
+            __syncthreads();
+            int off = 0, tot = 0;
+            for (int w = 0; w < 16; ++w) { off += (w < (int)threadIdx.x / WAVE) ? sw[w] : 0; tot += sw[w]; }
+            if (i < nb) bsum[i] = carry + off + x - v;
+            __syncthreads();
+            carry += tot;
         }
-        if (lane_id() == WAVE - 1) sw[threadIdx.x / WAVE] = x;
-        __syncthreads();
-        int off = 0, tot = 0;
-        for (int w = 0; w < 16; ++w) { off += (w < (int)threadIdx.x / WAVE) ? sw[w] : 0; tot += sw[w]; }
-        if (i < nb) bsum[i] = carry + off + x - v;
-        __syncthreads();
-        carry += tot;
+        if (threadIdx.x == 0) bsum[nb] = carry;
     }
-    if (threadIdx.x == 0) bsum[nb] = carry;
 }
-__global__ void __launch_bounds__(256) k_scan_apply(const int32_t* in, const int32_t* n_ptr, int32_t n_fixed,
-                                                   const int32_t* bsum, int32_t* out, int32_t* total_out) {
+__global__ void __launch_bounds__(256) k_scan_apply(const int32_t* in0, const int32_t* in1, const int32_t* n_ptr,
+                                                   int32_t n_fixed, const int32_t* bsum0, int32_t bstride,
+                                                   int32_t* out0, int32_t* out1, int32_t* total_out) {
     __shared__ int sw[4];
     const int n = n_ptr ? *n_ptr : n_fixed;
     const int64_t base = (int64_t)blockIdx.x * 1024;
     if (base > n) return;
-    int v[4], s = 0;
-    for (int k = 0; k < 4; ++k) {
-        int64_t i = base + threadIdx.x * 4 + k;
-        v[k] = (i < n) ? in[i] : 0;
-        s += v[k];
-    }
-    int total;
-    int off = block_excl<256>(s, sw, total);
-    int run = bsum[blockIdx.x] + off;
-    for (int k = 0; k < 4; ++k) {
-        int64_t i = base + threadIdx.x * 4 + k;
-        if (i < n) out[i] = run;
-        run += v[k];
-    }
     const int nb = (n + 1023) / 1024;
-    if (blockIdx.x == (unsigned)(nb > 0 ? nb - 1 : 0) && threadIdx.x == 0) {
-        out[n] = bsum[nb];
-        if (total_out) *total_out = bsum[nb];
+    const int na = in1 ? 2 : 1;
+    for (int a = 0; a < na; ++a) {
+        const int32_t* in = a ? in1 : in0;
+        int32_t* out = a ? out1 : out0;
+        const int32_t* bsum = bsum0 + a * (int64_t)bstride;
+        int v[4], s = 0;
+        for (int k = 0; k < 4; ++k) {
+            int64_t i = base + threadIdx.x * 4 + k;
+            v[k] = (i < n) ? in[i] : 0;
+            s += v[k];
+        }
+        int total;
+        int off = block_excl<256>(s, sw, total);
+        int run = bsum[blockIdx.x] + off;
+        for (int k = 0; k < 4; ++k) {
+            int64_t i = base + threadIdx.x * 4 + k;
+            if (i < n) out[i] = run;
+            run += v[k];
+        }
+        if (blockIdx.x == (unsigned)(nb > 0 ? nb - 1 : 0) && threadIdx.x == 0) {
+            out[n] = bsum[nb];
+            if (total_out && a == 0) *total_out = bsum[nb];
+        }
+        (void)total;
+        __syncthreads();   // sw is reused by the second array
     }
-    (void)total;
 }
 
 void scan_i32(const int32_t* in, const int32_t* n_ptr, int32_t n_max, int32_t* tmp, int32_t* out, int32_t* total,
               hipStream_t s) {
     int nb = (n_max + 1023) / 1024;
     if (nb < 1) nb = 1;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, in, n_ptr, n_max, tmp);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, tmp, nb, n_ptr);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, in, n_ptr, n_max, tmp, out, total);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, in, (const int32_t*)nullptr, n_ptr, n_max, tmp, nb + 1);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, tmp, nb, n_ptr, 1, nb + 1);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, in, (const int32_t*)nullptr, n_ptr, n_max, tmp, nb + 1,
+                       out, (int32_t*)nullptr, total);
+}
+// two arrays of the same length n in one pass; tmp holds 2 (blocks + 1)
+void scan_i32x2(const int32_t* in0, const int32_t* in1, int32_t n, int32_t* tmp, int32_t* out0, int32_t* out1,
+                hipStream_t s) {
+    int nb = (n + 1023) / 1024;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, in0, in1, (const int32_t*)nullptr, n, tmp, nb + 1);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, tmp, nb, (const int32_t*)nullptr, 2, nb + 1);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, in0, in1, (const int32_t*)nullptr, n, tmp, nb + 1,
+                       out0, out1, (int32_t*)nullptr);
 }
 
 // ------------------------------------------------------------ broad phase
@@ -297,7 +333,7 @@ __global__ void __launch_bounds__(256) k_bp_bounds_part(const double* state, con
 }
 
 __global__ void __launch_bounds__(64) k_bp_bounds(const double* part, int nparts, DGrid* g, int max_cells,
-                                                  int32_t* status) {
+                                                  int32_t* status, int32_t* ncells) {
     __shared__ double s[7][1];
     double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
     for (int b = threadIdx.x; b < nparts; b += blockDim.x)
@@ -331,6 +367,7 @@ __global__ void __launch_bounds__(64) k_bp_bounds(const double* part, int nparts
     G.h = h;
     G.ncells = G.nx * G.ny * G.nz;
     *g = G;
+    *ncells = G.ncells;
 }
 
 __device__ inline void cell_coords(const DGrid& G, const double* p, int& cx, int& cy, int& cz) {
@@ -361,11 +398,15 @@ __global__ void __launch_bounds__(256) k_bp_cells(BroadPhaseBufs bp) {
     atomicAdd(&bp.cell_count[c], 1);
 }
 
+// The slot within the cell counts the cell's counter back down, which leaves
+// every counter at zero for the next step's k_bp_cells (the counters are
+// cleared once when allocated).  The order within a cell is free: each
+// particle's partners are sorted by index in k_bp_fill.
 __global__ void __launch_bounds__(256) k_bp_scatter(BroadPhaseBufs bp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= bp.n) return;
     int c = bp.cell_of[i];
-    int slot = bp.cell_start[c] + atomicAdd(&bp.cell_cursor[c], 1);
+    int slot = bp.cell_start[c] + atomicSub(&bp.cell_count[c], 1) - 1;
     bp.sorted[slot] = i;
     // position and radius in cell order, for the neighbour scans
     const double* p = bp.state + 16 * (int64_t)i;
@@ -485,6 +526,16 @@ __global__ void __launch_bounds__(128) k_bp_fill(BroadPhaseBufs bp) {
 
 __global__ void __launch_bounds__(256) k_bp_adjfix(BroadPhaseBufs bp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        // totals: on overflow the pair and adjacency lists were not written;
+        // later stages see no pairs (the fold checks adj_off[n] against the
+        // capacity) and the host grows the buffers and runs the step again
+        const int np = bp.pair_off[bp.n], na = bp.adj_off[bp.n];
+        const bool over = np > bp.pair_cap || na > bp.adj_cap || (*bp.status & (int)ST_NEIGH);
+        if (over) atomicOr(bp.status, (int)ST_PAIRS);
+        *bp.n_pairs = over ? 0 : np;
+        *bp.n_adj = over ? 0 : na;
+    }
     if (i >= bp.n) return;
     if (bp.adj_off[bp.n] > bp.adj_cap || bp.pair_off[bp.n] > bp.pair_cap) return;
     const int a0 = bp.adj_off[i], a1 = bp.adj_off[i + 1];
@@ -502,18 +553,6 @@ __global__ void __launch_bounds__(256) k_bp_adjfix(BroadPhaseBufs bp) {
     }
 }
 
-__global__ void k_bp_totals(BroadPhaseBufs bp) {
-    int np = bp.pair_off[bp.n], na = bp.adj_off[bp.n];
-    // on overflow the pair and adjacency lists were not written: later stages
-    // see no pairs (the fold checks adj_off[n] against the capacity) and the
-    // host grows the buffers and runs the step again
-    bool over = np > bp.pair_cap || na > bp.adj_cap || (*bp.status & (int)ST_NEIGH);
-    if (over) atomicOr(bp.status, (int)ST_PAIRS);
-    *bp.n_pairs = over ? 0 : np;
-    *bp.n_adj = over ? 0 : na;
-}
-
-__global__ void k_grid_cells(const DGrid* g, int32_t* ncells) { *ncells = g->ncells; }
 
 // Sphere test and triangle AABB test of the particle-wall path
 // (main.cpp:2318-2323, 2338-2353): true if any wall triangle of wall w needs
@@ -531,24 +570,46 @@ __device__ inline bool near_wall(const WallBufs& W, int w, const V3& pos, double
     return false;
 }
 
-__global__ void __launch_bounds__(256) k_need(int32_t n, const double* state, const double* radius,
-                                              const int32_t* cnt_all, WallBufs W, int32_t* flag, int32_t* wflag) {
+// Particles in a candidate pair or near a wall triangle (list) and those near
+// a wall triangle (wlist), appended with one atomic per workgroup and list.
+// The order of the lists is free: the transform and the wall kernel write
+// per-body results.
+__global__ void __launch_bounds__(256) k_need_lists(int32_t n, const double* state, const double* radius,
+                                                    const int32_t* cnt_all, WallBufs W, int32_t* list,
+                                                    int32_t* n_list, int32_t* wlist, int32_t* n_wlist) {
+    __shared__ int s_cnt[2][4];
+    __shared__ int s_base[2];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    bool wall = false;
-    if (W.n_walls > 0) {
-        const double* p = state + 16 * (int64_t)i;
-        V3 pos = v3(p[0], p[1], p[2]);
-        for (int w = 0; w < W.n_walls && !wall; ++w) wall = near_wall(W, w, pos, radius[i]);
+    bool wall = false, need = false;
+    if (i < n) {
+        if (W.n_walls > 0) {
+            const double* p = state + 16 * (int64_t)i;
+            V3 pos = v3(p[0], p[1], p[2]);
+            for (int w = 0; w < W.n_walls && !wall; ++w) wall = near_wall(W, w, pos, radius[i]);
+        }
+        need = cnt_all[i] > 0 || wall;
     }
-    flag[i] = (cnt_all[i] > 0 || wall) ? 1 : 0;
-    wflag[i] = wall ? 1 : 0;
-}
-__global__ void __launch_bounds__(256) k_need_list(int32_t n, const int32_t* flag, const int32_t* off, int32_t* list,
-                                                   int32_t* n_list) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && flag[i]) list[off[i]] = i;
-    if (i == 0) *n_list = off[n];
+    const uint64_t bn = __ballot(need), bw = __ballot(wall);
+    const int wid = threadIdx.x / WAVE, lane = lane_id();
+    if (lane == 0) {
+        s_cnt[0][wid] = __popcll(bn);
+        s_cnt[1][wid] = __popcll(bw);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int a = threadIdx.x;
+        const int tot = s_cnt[a][0] + s_cnt[a][1] + s_cnt[a][2] + s_cnt[a][3];
+        s_base[a] = tot > 0 ? atomicAdd(a ? n_wlist : n_list, tot) : 0;
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+    int on = s_base[0], ow = s_base[1];
+    for (int w = 0; w < wid; ++w) {
+        on += s_cnt[0][w];
+        ow += s_cnt[1][w];
+    }
+    if (need) list[on + __popcll(bn & below)] = i;
+    if (wall) wlist[ow + __popcll(bw & below)] = i;
 }
 
 #ifdef ZDX_WALK_PROFILE
@@ -1659,7 +1720,7 @@ __global__ void __launch_bounds__(64) k_walls(DBodies B, DMeshes M, DMat mat, Wa
                     double area;
                     V3 xc, cn;
                     int st = 0;
-                    bool ok = wall_contact(wp, wv, ti, T, wt, ld3(W.normals + 3 * (int64_t)(t0 + t)), tol, PTS, PA,
+                    bool ok = wall_contact(wp, wv, ti, T, M.nverts[m], wt, ld3(W.normals + 3 * (int64_t)(t0 + t)), tol, PTS, PA,
                                            PB, pts_cap, area, xc, cn, st);
                     if (!ok) {
                         if (st && lane_id() == 0) atomicOr(status, (int)ST_WALL);
@@ -1833,8 +1894,8 @@ extern "C" int zdx_walk_profile(unsigned long long* out) {
 void launch_stamp(unsigned long long* slot, hipStream_t s) {
     if (slot) hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, s, slot);
 }
-void launch_step_begin(unsigned long long* kt, int32_t* status, hipStream_t s) {
-    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, s, kt, status);
+void launch_step_begin(unsigned long long* kt, const StepZero& z, hipStream_t s) {
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, kt, z);
 }
 void launch_report(const int32_t* status, unsigned long long* ctr, const unsigned long long* kt, StepReport* rep,
                    hipStream_t s) {
@@ -1850,34 +1911,30 @@ void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s, const i
 }
 
 void launch_need_list(int32_t n, const double* state, const double* radius, const int32_t* cnt_all, const WallBufs& W,
-                      int32_t* flag, int32_t* off, int32_t* scan_tmp, int32_t* list, int32_t* n_list, int32_t* wflag,
-                      int32_t* woff, int32_t* wlist, int32_t* n_wlist, hipStream_t s) {
-    hipLaunchKernelGGL(k_need, dim3(nblk(n, 256)), dim3(256), 0, s, n, state, radius, cnt_all, W, flag, wflag);
-    scan_i32(flag, nullptr, n, scan_tmp, off, nullptr, s);
-    hipLaunchKernelGGL(k_need_list, dim3(nblk(n, 256)), dim3(256), 0, s, n, flag, off, list, n_list);
-    scan_i32(wflag, nullptr, n, scan_tmp, woff, nullptr, s);
-    hipLaunchKernelGGL(k_need_list, dim3(nblk(n, 256)), dim3(256), 0, s, n, wflag, woff, wlist, n_wlist);
+                      int32_t* list, int32_t* n_list, int32_t* wlist, int32_t* n_wlist, bool zeroed, hipStream_t s) {
+    if (!zeroed) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, n_list, 0, nullptr, (int64_t)1);
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, n_wlist, 0, nullptr, (int64_t)1);
+    }
+    hipLaunchKernelGGL(k_need_lists, dim3(nblk(n, 256)), dim3(256), 0, s, n, state, radius, cnt_all, W, list, n_list,
+                       wlist, n_wlist);
 }
 
 void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s) {
     const int n = bp.n;
     hipLaunchKernelGGL(k_bp_bounds_part, dim3(BP_PARTS), dim3(256), 0, s, bp.state, bp.radius, n, bp.part);
+    // the cell count of the grid in use goes to a scalar (n_adj, written
+    // again at the end) for the cell scan
+    int32_t* ncells = bp.n_adj;
     hipLaunchKernelGGL(k_bp_bounds, dim3(1), dim3(64), 0, s, bp.part, (int)BP_PARTS, bp.grid, bp.max_cells,
-                       bp.status);
-    // cell counters for the grid actually in use
-    int32_t* ncells = bp.n_adj;  // reuse as a scalar before it is written below
-    hipLaunchKernelGGL(k_grid_cells, dim3(1), dim3(1), 0, s, bp.grid, ncells);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, s, bp.cell_count, 0, ncells, (int64_t)bp.max_cells);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, s, bp.cell_cursor, 0, ncells, (int64_t)bp.max_cells);
+                       bp.status, ncells);
     hipLaunchKernelGGL(k_bp_cells, dim3(nblk(n, 256)), dim3(256), 0, s, bp);
     scan_i32(bp.cell_count, ncells, bp.max_cells, bp.scan_tmp, bp.cell_start, nullptr, s);
     hipLaunchKernelGGL(k_bp_scatter, dim3(nblk(n, 256)), dim3(256), 0, s, bp);
     hipLaunchKernelGGL(k_bp_count, dim3(nblk(n, 256)), dim3(256), 0, s, bp);
-    scan_i32(bp.cnt_up, nullptr, n, bp.scan_tmp, bp.pair_off, nullptr, s);
-    scan_i32(bp.cnt_all, nullptr, n, bp.scan_tmp, bp.adj_off, nullptr, s);
+    scan_i32x2(bp.cnt_up, bp.cnt_all, n, bp.scan_tmp, bp.pair_off, bp.adj_off, s);
     hipLaunchKernelGGL(k_bp_fill, dim3(nblk(n, 128)), dim3(128), 0, s, bp);
-    hipLaunchKernelGGL(k_bp_adjfix, dim3(nblk(n, 256)), dim3(256), 0, s, bp);
-    hipLaunchKernelGGL(k_bp_totals, dim3(1), dim3(64), 0, s, bp);
+    hipLaunchKernelGGL(k_bp_adjfix, dim3(nblk(n, 256) > 0 ? nblk(n, 256) : 1), dim3(256), 0, s, bp);
 }
 
 template <int BS, int K>
@@ -1891,9 +1948,11 @@ static void narrow_t(const DBodies& B, const DMeshes& M, const DPairs& P, const 
 }
 
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
-                   int32_t seg_cap, int32_t tmax, unsigned int* q, hipStream_t s) {
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int32_t*)q, 0, nullptr, (int64_t)1);
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, P.hit_top, 0ull, (int64_t)1);
+                   int32_t seg_cap, int32_t tmax, unsigned int* q, hipStream_t s, bool zeroed) {
+    if (!zeroed) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int32_t*)q, 0, nullptr, (int64_t)1);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, P.hit_top, 0ull, (int64_t)1);
+    }
     const int key = P.prov ? 25604 : cfg.block * 100 + cfg.items;
     switch (key) {
         case 6404: narrow_t<64, 4>(B, M, P, cfg, li, lf, tmax, q, s); break;
@@ -1920,8 +1979,8 @@ void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const Na
 }
 
 void launch_lpt_order(const DPairs& P, const DMeshes& M, const int32_t* body_mesh, int32_t* order, int32_t* bcnt,
-                      int32_t* bcur, int32_t pair_cap, hipStream_t s) {
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, bcnt, 0, nullptr, (int64_t)64);
+                      int32_t* bcur, int32_t pair_cap, hipStream_t s, bool zeroed) {
+    if (!zeroed) hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, bcnt, 0, nullptr, (int64_t)64);
     const int g = std::max(1, std::min(nblk(pair_cap, 256), 512));
     hipLaunchKernelGGL(k_lpt_count, dim3(g), dim3(256), 0, s, P, M, body_mesh, bcnt);
     hipLaunchKernelGGL(k_lpt_scan, dim3(1), dim3(64), 0, s, bcnt, bcur);
@@ -1935,8 +1994,9 @@ void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, i
     hipLaunchKernelGGL(k_topo_zero, dim3(g), dim3(256), 0, s, P, T, cur_active);
 }
 
-static void launch_pass_a(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, hipStream_t s) {
-    launch_fill_u64(T.top, 0ull, 5, s);
+static void launch_pass_a(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, hipStream_t s,
+                          bool zeroed = false) {
+    if (!zeroed) launch_fill_u64(T.top, 0ull, 5, s);
     hipLaunchKernelGGL(k_tp_alloc, dim3((unsigned)((T.pass_groups + 255) / 256 * 4)), dim3(256), 0, s, P, T);
     hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(TP_PASS_A_THREADS),
                        84 * TP_PASS_A_SEGS + TP_PASS_A_KEYS + 8 * TP_PASS_A_WORDS, s, P, M, B, T,
@@ -1944,8 +2004,8 @@ static void launch_pass_a(const DPairs& P, const DMeshes& M, const DBodies& B, c
 }
 
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
-                     const TopoBufs& T, const SimParams& sp, hipStream_t s) {
-    launch_pass_a(P, M, B, T, s);
+                     const TopoBufs& T, const SimParams& sp, hipStream_t s, bool zeroed) {
+    launch_pass_a(P, M, B, T, s, zeroed);
     hipLaunchKernelGGL(k_tp_walk_bcd_sim, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, mat, H, T,
                        sp, T.lds_bytes);
 }
