@@ -183,6 +183,7 @@ struct zdx_ctx {
         DBuf<double> wv, aabb;
         DBuf<int32_t> nondeg;
         DBuf<int32_t> pi, pj, order, npairs, status;
+        DBuf<NarrowItem> items;
         DBuf<int64_t> seg_off, tri_checks, tri_tests;
         DBuf<int32_t> nseg, tri_hits, tri_scanned;
         DBuf<double> arena;
@@ -321,6 +322,7 @@ struct zdx_sim {
         DBuf<uint8_t> wactive;
     } ps[2];
     DBuf<int32_t> d_order, d_bcnt, d_bcur, d_topo_list, d_topo_n;
+    DBuf<NarrowItem> d_items;
     DBuf<int64_t> d_seg_off, d_tri_checks, d_tri_tests;
     DBuf<int32_t> d_nseg, d_tri_hits, d_tri_scanned;
     DBuf<double> d_arena;        // segments, one slot per hit slot
@@ -418,6 +420,7 @@ struct zdx_sim {
         DPairs P{};
         P.cap = pair_cap; P.n = ps[par].n.p; P.pi = ps[par].pi.p; P.pj = ps[par].pj.p;
         P.order = lpt ? d_order.p : nullptr;
+        P.items = lpt ? d_items.p : nullptr;
         P.seg_off = d_seg_off.p; P.nseg = d_nseg.p; P.tri_checks = d_tri_checks.p; P.tri_tests = d_tri_tests.p;
         P.tri_hits = d_tri_hits.p; P.tri_scanned = d_tri_scanned.p; P.arena = d_arena.p; P.arena_cap = hit_cap;
         P.status = d_status.p;
@@ -482,7 +485,7 @@ struct zdx_sim {
             HIPC(hipMemcpy(ps[k].n.p, &n, sizeof(int32_t), hipMemcpyHostToDevice));
         }
         d_adj_j.ensure(adj_cap); d_adj_pair.ensure(adj_cap);
-        d_order.ensure(cap); d_topo_list.ensure(cap); d_topo_n.ensure(1);
+        d_order.ensure(cap); d_items.ensure(cap); d_topo_list.ensure(cap); d_topo_n.ensure(1);
         d_hit_off.ensure(cap); d_hit_top.ensure(1);
         d_seg_off.ensure(cap); d_tri_checks.ensure(cap); d_tri_tests.ensure(cap);
         d_nseg.ensure(cap); d_tri_hits.ensure(cap); d_tri_scanned.ensure(cap);
@@ -717,7 +720,7 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
         HIPC(hipEventRecord(s->ctx->ev_join, st2));
     }
     DPairs P = s->dpairs(par, batched);
-    if (batched) launch_lpt_order(P, M, s->d_mesh.p, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st, true);
+    if (batched) launch_lpt_order(P, M, B, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st, true);
     NarrowCfg nc = s->ncfg;
     nc.use_queue = batched ? 1 : 0;
     nc.groups = g_narrow;
@@ -1023,7 +1026,7 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
         B.wv = A.wv.p; B.nondeg = A.nondeg.p; B.aabb = A.aabb.p;
         DMeshes M = c->dmeshes();
         launch_transform(B, M, st);
-        A.pi.ensure(n); A.pj.ensure(n); A.npairs.ensure(1); A.status.ensure(1); A.order.ensure(n);
+        A.pi.ensure(n); A.pj.ensure(n); A.npairs.ensure(1); A.status.ensure(1); A.order.ensure(n); A.items.ensure(n);
         launch_iota_pairs(A.pi.p, A.pj.p, n, st);
         HIPC(hipMemcpyAsync(A.npairs.p, &n, sizeof(int32_t), hipMemcpyHostToDevice, st));
         A.seg_off.ensure(n); A.tri_checks.ensure(n); A.tri_tests.ensure(n); A.nseg.ensure(n); A.tri_hits.ensure(n); A.tri_scanned.ensure(n);
@@ -1043,6 +1046,7 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
             const bool sens = opts->sensitivities != 0;
             DPairs P{};
             P.cap = n; P.n = A.npairs.p; P.pi = A.pi.p; P.pj = A.pj.p; P.order = batched ? A.order.p : nullptr;
+            P.items = batched ? A.items.p : nullptr;
             P.seg_off = A.seg_off.p; P.nseg = A.nseg.p; P.tri_checks = A.tri_checks.p; P.tri_tests = A.tri_tests.p;
             P.tri_hits = A.tri_hits.p; P.tri_scanned = A.tri_scanned.p; P.arena = A.arena.p; P.arena_cap = A.hit_cap;
             P.status = A.status.p;
@@ -1057,7 +1061,7 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
             }
             launch_fill_i32(A.status.p, 0, 1, st);
             launch_fill_i32(A.topo_n.p, 0, 1, st);
-            if (batched) launch_lpt_order(P, M, A.mesh.p, A.order.p, A.bcnt.p, A.bcur.p, n, st);
+            if (batched) launch_lpt_order(P, M, B, A.order.p, A.bcnt.p, A.bcur.p, n, st);
             NarrowCfg nc{256, 4, groups, batched ? 1 : 0};
             launch_narrow(B, M, P, nc, A.list_i.p, A.list_f.p, A.stage_cap, c->max_tris, A.queue.p, st);
             int32_t status = 0;
@@ -1536,7 +1540,7 @@ zdx_status zdx_sim_autotune(zdx_sim* s, int32_t* out_block, int32_t* out_groups)
                          s->d_n_need.p, s->d_wall_list.p, s->d_n_wall.p, false, st);
         launch_transform(B, M, st, s->d_need_list.p, s->d_n_need.p, s->transform_groups());
         DPairs P = s->dpairs(par, true);
-        launch_lpt_order(P, M, s->d_mesh.p, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st);
+        launch_lpt_order(P, M, B, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st);
         const int cand[][2] = {{64, 4}, {64, 8}, {128, 4}, {128, 8}, {256, 1}, {256, 2}, {256, 4}, {256, 8},
                                {512, 2}, {512, 4}};
         const int gpc[] = {4, 8, 16};

@@ -159,12 +159,23 @@ struct DMat {                // per particle material / inertia (sim only)
 };
 
 // Narrow-phase output arena and per-pair results.
+// One pair of the narrow phase in work order, gathered by the work-ordering
+// kernel so that the cull starts from a single 128-byte load: pair, bodies,
+// triangle / vertex counts and offsets of both meshes, A's non-degenerate
+// triangle count and both bodies' outward-rounded float AABBs.
+struct NarrowItem {
+    int32_t p, i, j, TA, TB, VA, VB, toA, toB, woA, woB, nondegA;
+    float boxA[6], boxB[6];
+    int32_t pad[8];
+};
+
 struct DPairs {
     int32_t cap;             // capacity of the pair arrays
     const int32_t* n;        // device count of pairs
     const int32_t* pi;       // body A (lower index in the sim)
     const int32_t* pj;       // body B
     const int32_t* order;    // processing order (LPT in batched mode)
+    NarrowItem* items;       // with order: the pairs' narrow-phase items in that order
     int64_t* seg_off;        // per pair: first segment in the arena
     int32_t* nseg;           // per pair
     int64_t* tri_checks;     // per pair (reference counter)

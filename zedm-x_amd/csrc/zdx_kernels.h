@@ -176,7 +176,9 @@ void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s);
 // the queue and P.hit_top itself unless the caller did (zeroed).
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* list_i,
                    float* list_f, int32_t seg_cap, int32_t tmax, unsigned int* queue, hipStream_t s, bool zeroed = false);
-void launch_lpt_order(const DPairs& P, const DMeshes& M, const int32_t* body_mesh, int32_t* order, int32_t* bucket_cnt,
+// LPT work order of the pairs (largest triangle product first) into order,
+// and the narrow-phase items (P.items, when set) in that order
+void launch_lpt_order(const DPairs& P, const DMeshes& M, const DBodies& B, int32_t* order, int32_t* bucket_cnt,
                       int32_t* bucket_cur, int32_t pair_cap, hipStream_t s, bool zeroed = false);
 // zero outputs of every pair (the topology kernels then visit only the
 // pairs of P.topo_list); cur_active may be NULL

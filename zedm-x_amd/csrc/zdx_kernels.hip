@@ -697,31 +697,50 @@ __global__ void __launch_bounds__(BS, BS >= 256 ? 5 : 1) k_narrow(DBodies B, DMe
     int32_t* VB3 = LA + 5 * (int64_t)tmax;
     float* FA = list_f + (int64_t)blockIdx.x * 12 * tmax;
     const int npairs = *P.n;
+    // with the work queue, each workgroup asks for its next item while it
+    // works on the current one
     int item = blockIdx.x;
+    if (use_queue) {
+        if (tid == 0) s_item = (int)atomicAdd(queue, 1u);
+        __syncthreads();
+        item = s_item;
+        __syncthreads();
+    }
     NP_START();
-    while (true) {
-        if (use_queue) {
-            if (tid == 0) s_item = (int)atomicAdd(queue, 1u);
-            __syncthreads();
-            item = s_item;
-            __syncthreads();
-        }
-        if (item >= npairs) break;
-        const int p = P.order ? P.order[item] : item;
-        const int i = P.pi[p], j = P.pj[p];
-        const int mA = B.mesh[i], mB = B.mesh[j];
-        const int TA = M.ntris[mA], TB = M.ntris[mB];
-        const int32_t* tiA = M.tri_idx + 3 * (int64_t)M.tri_off[mA];
-        const int32_t* tiB = M.tri_idx + 3 * (int64_t)M.tri_off[mB];
-        const double* wvA = B.wv + 3 * B.wv_off[i];
-        const double* wvB = B.wv + 3 * B.wv_off[j];
+    while (item < npairs) {
+        int next = 0;
+        if (use_queue && tid == 0) next = (int)atomicAdd(queue, 1u);
+        int p, i, j, TA, TB, VA, VB, ndA;
+        int64_t oA, oB;
+        const int32_t* tiA;
+        const int32_t* tiB;
         float boxA[6], boxB[6];
-        box_f(B.aabb + 6 * (int64_t)i, B.aabb + 6 * (int64_t)i + 3, boxA);
-        box_f(B.aabb + 6 * (int64_t)j, B.aabb + 6 * (int64_t)j + 3, boxB);
+        if (P.items) {
+            const NarrowItem& it = P.items[item];
+            p = it.p; i = it.i; j = it.j;
+            TA = it.TA; TB = it.TB; VA = it.VA; VB = it.VB; ndA = it.nondegA;
+            tiA = M.tri_idx + 3 * (int64_t)it.toA;
+            tiB = M.tri_idx + 3 * (int64_t)it.toB;
+            oA = it.woA; oB = it.woB;
+            for (int k = 0; k < 6; ++k) { boxA[k] = it.boxA[k]; boxB[k] = it.boxB[k]; }
+        } else {
+            p = P.order ? P.order[item] : item;
+            i = P.pi[p]; j = P.pj[p];
+            const int mA = B.mesh[i], mB = B.mesh[j];
+            TA = M.ntris[mA]; TB = M.ntris[mB];
+            VA = M.nverts[mA]; VB = M.nverts[mB];
+            ndA = B.nondeg[i];
+            tiA = M.tri_idx + 3 * (int64_t)M.tri_off[mA];
+            tiB = M.tri_idx + 3 * (int64_t)M.tri_off[mB];
+            oA = B.wv_off[i]; oB = B.wv_off[j];
+            box_f(B.aabb + 6 * (int64_t)i, B.aabb + 6 * (int64_t)i + 3, boxA);
+            box_f(B.aabb + 6 * (int64_t)j, B.aabb + 6 * (int64_t)j + 3, boxB);
+        }
+        const double* wvA = B.wv + 3 * oA;
+        const double* wvB = B.wv + 3 * oB;
         NP_MARK(0);
         // ---- cull each side against the other body's AABB (conservative)
         int nA = 0, nB = 0;
-        const int VA = M.nverts[mA], VB = M.nverts[mB];
         if (VA <= OC_CAP && VB <= OC_CAP) {
             // outcodes of both bodies' vertices, then per thread CULL_PER_THREAD
             // consecutive triangles of each side, one ordered scan per round
@@ -836,7 +855,6 @@ __global__ void __launch_bounds__(BS, BS >= 256 ? 5 : 1) k_narrow(DBodies B, DMe
         // float boxes and world-vertex indices of the kept triangles, all
         // lanes in parallel
         {
-            const int64_t oA = B.wv_off[i], oB = B.wv_off[j];
             for (int k = tid; k < nA + nB; k += BS) {
                 const bool a = k < nA;
                 const int q = a ? k : k - nA;
@@ -937,7 +955,7 @@ __global__ void __launch_bounds__(BS, BS >= 256 ? 5 : 1) k_narrow(DBodies B, DMe
             const bool ok = hbase != ~0ull;
             P.hit_off[p] = ok ? (int64_t)hbase : 0;
             P.tri_hits[p] = ok ? hits : 0;
-            P.tri_checks[p] = (int64_t)B.nondeg[i] * TB;
+            P.tri_checks[p] = (int64_t)ndA * TB;
             P.tri_tests[p] = total;
             P.tri_scanned[p] = TA + TB;
             P.seg_off[p] = 0;   // set by the compaction for pairs with hits
@@ -954,7 +972,13 @@ __global__ void __launch_bounds__(BS, BS >= 256 ? 5 : 1) k_narrow(DBodies B, DMe
             np_ph[0] = np_ph[1] = np_ph[2] = np_ph[3] = 0;
         }
 #endif
-        if (!use_queue) item += gridDim.x;
+        if (use_queue) {
+            if (tid == 0) s_item = next;
+            __syncthreads();
+            item = s_item;
+        } else {
+            item += gridDim.x;
+        }
     }
 }
 
@@ -1075,8 +1099,10 @@ __global__ void k_lpt_scan(int32_t* bucket_cnt, int32_t* bucket_cur) {
     for (int b = 0; b < 64; ++b) { int c = bucket_cnt[b]; bucket_cur[b] = acc; acc += c; }
 }
 // order within a bucket is free (any order gives the same per-pair results)
-__global__ void __launch_bounds__(256) k_lpt_scatter(DPairs P, DMeshes M, const int32_t* body_mesh,
-                                                     int32_t* bucket_cur, int32_t* order) {
+__device__ inline void box_f(const double* lo, const double* hi, float* f);
+__global__ void __launch_bounds__(256) k_lpt_scatter(DPairs P, DMeshes M, DBodies B, int32_t* bucket_cur,
+                                                     int32_t* order) {
+    const int32_t* body_mesh = B.mesh;
     __shared__ int h[64], base[64];
     const int n = *P.n;
     for (int p0 = blockIdx.x * blockDim.x; p0 < n; p0 += gridDim.x * blockDim.x) {
@@ -1091,7 +1117,25 @@ __global__ void __launch_bounds__(256) k_lpt_scatter(DPairs P, DMeshes M, const 
         __syncthreads();
         if (threadIdx.x < 64 && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&bucket_cur[threadIdx.x], h[threadIdx.x]);
         __syncthreads();
-        if (p < n) order[base[b] + r] = p;
+        if (p < n) {
+            const int slot = base[b] + r;
+            order[slot] = p;
+            if (P.items) {
+                NarrowItem it;
+                const int i = P.pi[p], j = P.pj[p];
+                const int mA = B.mesh[i], mB = B.mesh[j];
+                it.p = p; it.i = i; it.j = j;
+                it.TA = M.ntris[mA]; it.TB = M.ntris[mB];
+                it.VA = M.nverts[mA]; it.VB = M.nverts[mB];
+                it.toA = M.tri_off[mA]; it.toB = M.tri_off[mB];
+                it.woA = (int32_t)B.wv_off[i]; it.woB = (int32_t)B.wv_off[j];
+                it.nondegA = B.nondeg[i];
+                box_f(B.aabb + 6 * (int64_t)i, B.aabb + 6 * (int64_t)i + 3, it.boxA);
+                box_f(B.aabb + 6 * (int64_t)j, B.aabb + 6 * (int64_t)j + 3, it.boxB);
+                for (int k = 0; k < 8; ++k) it.pad[k] = 0;
+                P.items[slot] = it;
+            }
+        }
         __syncthreads();
     }
 }
@@ -1978,13 +2022,14 @@ void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const Na
     }
 }
 
-void launch_lpt_order(const DPairs& P, const DMeshes& M, const int32_t* body_mesh, int32_t* order, int32_t* bcnt,
+void launch_lpt_order(const DPairs& P, const DMeshes& M, const DBodies& B, int32_t* order, int32_t* bcnt,
                       int32_t* bcur, int32_t pair_cap, hipStream_t s, bool zeroed) {
+    const int32_t* body_mesh = B.mesh;
     if (!zeroed) hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, bcnt, 0, nullptr, (int64_t)64);
     const int g = std::max(1, std::min(nblk(pair_cap, 256), 512));
     hipLaunchKernelGGL(k_lpt_count, dim3(g), dim3(256), 0, s, P, M, body_mesh, bcnt);
     hipLaunchKernelGGL(k_lpt_scan, dim3(1), dim3(64), 0, s, bcnt, bcur);
-    hipLaunchKernelGGL(k_lpt_scatter, dim3(g), dim3(256), 0, s, P, M, body_mesh, bcur, order);
+    hipLaunchKernelGGL(k_lpt_scatter, dim3(g), dim3(256), 0, s, P, M, B, bcur, order);
 }
 
 void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, int32_t pair_cap, hipStream_t s) {
