@@ -475,9 +475,10 @@ __global__ void __launch_bounds__(128) k_bp_fill(BroadPhaseBufs bp) {
         atomicOr(bp.status, (int)ST_PAIRS);
         return;
     }
+    const int cnt = bp.cnt_all[i];
+    if (cnt == 0) return;   // no partners: nothing to write
     const int ao = bp.adj_off[i];
     const int po = bp.pair_off[i];
-    const int cnt = bp.cnt_all[i];
     auto emit = [&](int k, int j, int& u) {
         bp.adj_j[ao + k] = j;
         if (j > i) {
@@ -675,9 +676,11 @@ constexpr int CULL_PER_THREAD = 8;
 // AABB test; the triangle pairs that pass ("hits") are written in (ta, tb)
 // order to the pair's slice of the hit arena, which is reserved with the
 // upper bound nA * nB so that every pair's hits are contiguous.  Pairs with
-// hits are listed for part 2 (k_narrow_exact).
+// hits are listed for part 2 (k_narrow_exact).  Register budget: 5 waves per
+// SIMD for 256 threads (5 workgroups per CU), 4 for 512 threads (the same 2
+// workgroups per CU as 5 would give, with room to keep everything in registers).
 template <int BS, int K>
-__global__ void __launch_bounds__(BS, BS >= 256 ? 5 : 1) k_narrow(DBodies B, DMeshes M, DPairs P, int32_t* list_i, float* list_f,
+__global__ void __launch_bounds__(BS, BS >= 512 ? 4 : (BS >= 256 ? 5 : 1)) k_narrow(DBodies B, DMeshes M, DPairs P, int32_t* list_i, float* list_f,
                                                int32_t tmax, int32_t use_queue, unsigned int* queue, int32_t lds_cap) {
     // Body B's culled float boxes (24 bytes per triangle, read for every item
     // of the sweep) live in dynamic LDS when there are at most lds_cap of
@@ -758,7 +761,12 @@ __global__ void __launch_bounds__(BS, BS >= 256 ? 5 : 1) k_narrow(DBodies B, DMe
                     const int v = v0 + u * BS + tid;
                     if (v < VA + VB) {
                         const bool a = v < VA;
-                        s_oc[a ? v : OC_CAP + v - VA] = (unsigned char)vertex_outcode(w[u], a ? boxB : boxA);
+                        // the other body's box, selected per component (a pointer
+                        // select would force both boxes out of registers)
+                        float bx[6];
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) bx[k] = a ? boxB[k] : boxA[k];
+                        s_oc[a ? v : OC_CAP + v - VA] = (unsigned char)vertex_outcode(w[u], bx);
                     }
                 }
             }
@@ -1041,26 +1049,43 @@ __global__ void __launch_bounds__(256) k_narrow_compact(DPairs P, int32_t seg_ca
         const int nh = P.tri_hits[p];
         if (nh == 0) continue;
         const int64_t h0 = P.hit_off[p];
+        // CC chunks of WAVE hits per round: all their flags, then all their
+        // segments in flight at once.  A segment only moves down (to its rank
+        // within the pair), below every source of the later chunks, so the
+        // in-place moves never overwrite a segment still to be read.
+        constexpr int CC = 4;
         int nseg = 0;
-        for (int b = 0; b < nh; b += WAVE) {
-            const int64_t h = h0 + b + lane;
-            const bool sg = b + lane < nh && P.hit_flag[h] != 0;
-            double sv[6];
-            int4 pv = make_int4(0, 0, 0, 0);
-            if (sg) {
-                for (int q = 0; q < 6; ++q) sv[q] = P.arena[6 * h + q];
-                if (PROV) pv = ((const int4*)P.prov)[h];
+        for (int b0 = 0; b0 < nh; b0 += CC * WAVE) {
+            bool sg[CC];
+#pragma unroll
+            for (int u = 0; u < CC; ++u) {
+                const int k = b0 + u * WAVE + lane;
+                sg[u] = k < nh && P.hit_flag[h0 + k] != 0;
             }
-            const unsigned long long m = __ballot(sg);
-            if (sg) {
-                const int r = nseg + __popcll(m & ((1ull << lane) - 1ull));
-                if (r < seg_cap) {
-                    double* dst = P.arena + 6 * (h0 + r);
-                    for (int q = 0; q < 6; ++q) dst[q] = sv[q];
-                    if (PROV) ((int4*)P.prov)[h0 + r] = pv;
+            double sv[CC][6];
+            int4 pv[CC];
+#pragma unroll
+            for (int u = 0; u < CC; ++u) {
+                const int64_t h = h0 + b0 + u * WAVE + lane;
+                pv[u] = make_int4(0, 0, 0, 0);
+                if (sg[u]) {
+                    for (int q = 0; q < 6; ++q) sv[u][q] = P.arena[6 * h + q];
+                    if (PROV) pv[u] = ((const int4*)P.prov)[h];
                 }
             }
-            nseg += __popcll(m);
+#pragma unroll
+            for (int u = 0; u < CC; ++u) {
+                const unsigned long long m = __ballot(sg[u]);
+                if (sg[u]) {
+                    const int r = nseg + __popcll(m & ((1ull << lane) - 1ull));
+                    if (r < seg_cap) {
+                        double* dst = P.arena + 6 * (h0 + r);
+                        for (int q = 0; q < 6; ++q) dst[q] = sv[u][q];
+                        if (PROV) ((int4*)P.prov)[h0 + r] = pv[u];
+                    }
+                }
+                nseg += __popcll(m);
+            }
         }
         if (lane == 0) {
             int nstore = nseg;

@@ -22,8 +22,36 @@
 
 namespace zdx {
 
+// Stack of the introsort's pending right-hand ranges (first, last, depth).
+#if defined(__HIP_DEVICE_COMPILE__)
+// On the GPU every lane of the wave runs the sort with identical values, so
+// entry k is kept in lane k's registers (64 entries per wave; the depth limit
+// keeps the stack below that) and read back with a lane read, instead of a
+// per-lane array in scratch memory.
+struct IntroStack {
+    int f, l, d;   // this lane's entry
+    __device__ static int lane() { return (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+    __device__ void put(int k, int vf, int vl, int vd) {
+        if (lane() == k) { f = vf; l = vl; d = vd; }
+    }
+    __device__ void get(int k, int& vf, int& vl, int& vd) const {
+        const int ks = __builtin_amdgcn_readfirstlane(k);
+        vf = __builtin_amdgcn_readlane(f, ks);
+        vl = __builtin_amdgcn_readlane(l, ks);
+        vd = __builtin_amdgcn_readlane(d, ks);
+    }
+};
+#else
+struct IntroStack {
+    int f[64], l[64], d[64];
+    void put(int k, int vf, int vl, int vd) { f[k] = vf; l[k] = vl; d[k] = vd; }
+    void get(int k, int& vf, int& vl, int& vd) const { vf = f[k]; vl = l[k]; vd = d[k]; }
+};
+#endif
+
 template <class Arr, class Less>
 struct IntroSorter {
+    using Stack = IntroStack;
     Arr a;        // a[i] read/write, element type Arr::value_type
     Less less;    // strict weak ordering on values
     using T = typename Arr::value_type;
@@ -113,20 +141,21 @@ struct IntroSorter {
 
     ZDX_HD void introsort_loop(int first, int last, int depth) {
         if (last - first <= 16) return;   // left to the final insertion sort
-        // explicit stack for the right-hand recursion (depth <= 2*log2(n))
-        int st_first[64], st_last[64], st_depth[64];
+        // explicit stack for the right-hand recursion (depth <= 2*log2(n) < 64)
+        Stack st;
         int sp = 0;
-        st_first[sp] = first; st_last[sp] = last; st_depth[sp] = depth; ++sp;
+        st.put(sp, first, last, depth); ++sp;
         while (sp > 0) {
             --sp;
-            int f = st_first[sp], l = st_last[sp], d = st_depth[sp];
+            int f, l, d;
+            st.get(sp, f, l, d);
             while (l - f > 16) {
                 if (d == 0) { heap_sort(f, l); break; }
                 --d;
                 int cut = partition_pivot(f, l);
                 // the reference recurses on [cut, l) before continuing with [f, cut)
                 // order of processing is irrelevant to the result: disjoint ranges
-                st_first[sp] = cut; st_last[sp] = l; st_depth[sp] = d; ++sp;
+                st.put(sp, cut, l, d); ++sp;
                 l = cut;
             }
         }

@@ -275,31 +275,47 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
             rb = root(rb);
         }
     };
-    uint64_t nxt = ln < nwords ? snapw[ln] : 0ull;
-    for (int64_t wb = 0; wb < nwords; wb += W) {
-        const int64_t w = wb + ln;
-        uint64_t word = nxt;
-        nxt = w + W < nwords ? snapw[w + W] : 0ull;   // next round's word in flight meanwhile
-        wp.mark(4);
-        if (W == 1) {   // one lane: take the bits in order
-            while (word) {
-                const int64_t t = (w << 6) + ctz64(word);
-                word &= word - 1;
-                unite(t);
-            }
-            continue;
+    // the words of SNAP_BATCH rounds are loaded together, the next batch's
+    // loads in flight while this batch's bits are united
+    constexpr int SNAP_BATCH = 4;
+    uint64_t nxt[SNAP_BATCH];
+    for (int r = 0; r < SNAP_BATCH; ++r) {
+        const int64_t w = (int64_t)r * W + ln;
+        nxt[r] = w < nwords ? snapw[w] : 0ull;
+    }
+    for (int64_t wb0 = 0; wb0 < nwords; wb0 += (int64_t)SNAP_BATCH * W) {
+        uint64_t cur[SNAP_BATCH];
+        for (int r = 0; r < SNAP_BATCH; ++r) {
+            cur[r] = nxt[r];
+            const int64_t w = wb0 + (int64_t)(SNAP_BATCH + r) * W + ln;
+            nxt[r] = w < nwords ? snapw[w] : 0ull;
         }
-        int nbits;
-        int g = wp.scan_excl(__builtin_popcountll(word), nbits);
-        for (int j0 = 0; j0 < nbits; j0 += W) {
-            while (word && g < j0 + W) {
-                hl[g - j0] = (int)(((w - wb) << 6) + ctz64(word));   // index within the round
-                word &= word - 1;
-                ++g;
+        wp.mark(4);
+        for (int r = 0; r < SNAP_BATCH; ++r) {
+            const int64_t wb = wb0 + (int64_t)r * W;
+            if (wb >= nwords) break;
+            const int64_t w = wb + ln;
+            uint64_t word = cur[r];
+            if (W == 1) {   // one lane: take the bits in order
+                while (word) {
+                    const int64_t t = (w << 6) + ctz64(word);
+                    word &= word - 1;
+                    unite(t);
+                }
+                continue;
             }
-            wp.sync();
-            if (j0 + ln < nbits) unite((wb << 6) + hl[ln]);
-            wp.sync();
+            int nbits;
+            int g = wp.scan_excl(__builtin_popcountll(word), nbits);
+            for (int j0 = 0; j0 < nbits; j0 += W) {
+                while (word && g < j0 + W) {
+                    hl[g - j0] = (int)(((w - wb) << 6) + ctz64(word));   // index within the round
+                    word &= word - 1;
+                    ++g;
+                }
+                wp.sync();
+                if (j0 + ln < nbits) unite((wb << 6) + hl[ln]);
+                wp.sync();
+            }
         }
     }
     wp.mark(6);
