@@ -309,7 +309,7 @@ struct zdx_sim {
     DBuf<DGrid> d_grid;
     DBuf<double> d_bp_part, d_spos;
     int32_t max_cells = 1 << 24;
-    DBuf<int32_t> d_cell_of, d_cell_count, d_cell_start, d_sorted, d_cnt_up, d_cnt_all;
+    DBuf<int32_t> d_cell_of, d_cell_count, d_cell_start, d_sorted, d_cnt_up, d_cnt_all, d_nbr;
     DBuf<int32_t> d_pair_off, d_adj_off, d_adj_j, d_adj_pair, d_n_adj, d_scan_tmp;
     int32_t pair_cap = 0, adj_cap = 0;
     // pair sets (double-buffered: this step / previous step for the histories)
@@ -409,7 +409,7 @@ struct zdx_sim {
         b.n = N; b.state = d_state[par].p; b.radius = d_radius.p; b.grid = d_grid.p; b.max_cells = max_cells;
         b.part = d_bp_part.p; b.spos = d_spos.p;
         b.cell_of = d_cell_of.p; b.cell_count = d_cell_count.p; b.cell_start = d_cell_start.p;
-        b.sorted = d_sorted.p; b.cnt_up = d_cnt_up.p; b.cnt_all = d_cnt_all.p;
+        b.sorted = d_sorted.p; b.cnt_up = d_cnt_up.p; b.cnt_all = d_cnt_all.p; b.nbr = d_nbr.p;
         b.pair_off = d_pair_off.p; b.adj_off = d_adj_off.p; b.pi = ps[par].pi.p; b.pj = ps[par].pj.p;
         b.pkey = ps[par].key.p; b.pair_cap = pair_cap; b.adj_j = d_adj_j.p; b.adj_pair = d_adj_pair.p;
         b.adj_cap = adj_cap; b.n_pairs = ps[par].n.p; b.n_adj = d_n_adj.p; b.scan_tmp = d_scan_tmp.p;
@@ -561,7 +561,7 @@ void sim_setup_device(zdx_sim* s) {
     int64_t cells = std::min<int64_t>((int64_t)1 << 24, std::max<int64_t>(64, 8 * (int64_t)N));
     s->max_cells = (int32_t)cells;
     s->d_cell_of.ensure(N); s->d_cell_count.ensure(cells); s->d_cell_start.ensure(cells + 1);
-    s->d_sorted.ensure(N); s->d_cnt_up.ensure(N); s->d_cnt_all.ensure(N);
+    s->d_sorted.ensure(N); s->d_nbr.ensure(16 * (size_t)N); s->d_cnt_up.ensure(N); s->d_cnt_all.ensure(N);
     s->d_pair_off.ensure(N + 1); s->d_adj_off.ensure(N + 1); s->d_n_adj.ensure(1);
     s->d_scan_tmp.ensure((size_t)(2 * (std::max<int64_t>(cells, N) / 1024 + 4)));
     s->d_status.ensure(1); s->d_ctr.ensure(16); s->d_queue.ensure(2);

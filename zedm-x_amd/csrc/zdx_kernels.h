@@ -29,6 +29,7 @@ struct BroadPhaseBufs {
     double* spos;              // per cell-order slot: x, y, z, radius
     int32_t* cnt_up;           // per particle
     int32_t* cnt_all;          // per particle
+    int32_t* nbr;              // per cell-order slot: the first 16 partners found by the count pass
     int32_t* pair_off;         // n + 1
     int32_t* adj_off;          // n + 1
     int32_t* pi;               // pair capacity

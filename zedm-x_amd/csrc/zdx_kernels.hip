@@ -451,20 +451,26 @@ __device__ inline void for_neighbours(const BroadPhaseBufs& bp, const DGrid& G, 
     }
 }
 
-// threads take particles in cell order (neighbouring threads scan the same cells)
+constexpr int NEIGH_REG = 16;   // partners kept from the count pass and sorted in registers
+
+// threads take particles in cell order (neighbouring threads scan the same
+// cells); the first NEIGH_REG partners found are kept for k_bp_fill
 __global__ void __launch_bounds__(256) k_bp_count(BroadPhaseBufs bp) {
     const int si = blockIdx.x * blockDim.x + threadIdx.x;
     if (si >= bp.n) return;
     const int i = bp.sorted[si];
     const DGrid G = *bp.grid;
     int up = 0, all = 0;
-    for_neighbours(bp, G, i, si, [&](int j) { all++; if (j > i) up++; });
+    int32_t* keep = bp.nbr + (int64_t)NEIGH_REG * si;
+    for_neighbours(bp, G, i, si, [&](int j) {
+        if (all < NEIGH_REG) keep[all] = j;
+        all++;
+        if (j > i) up++;
+    });
     if (all > NEIGH_CAP) { atomicOr(bp.status, (int)ST_NEIGH); all = NEIGH_CAP; up = up > NEIGH_CAP ? NEIGH_CAP : up; }
     bp.cnt_up[i] = up;
     bp.cnt_all[i] = all;
 }
-
-constexpr int NEIGH_REG = 16;   // neighbour lists up to this length are sorted in registers
 
 __global__ void __launch_bounds__(128) k_bp_fill(BroadPhaseBufs bp) {
     const int si = blockIdx.x * blockDim.x + threadIdx.x;
@@ -494,11 +500,13 @@ __global__ void __launch_bounds__(128) k_bp_fill(BroadPhaseBufs bp) {
     };
     int u = 0;
     if (cnt <= NEIGH_REG) {
-        // ascending partner order: insertion into a sorted register array
+        // the partners kept by the count pass, in ascending order: insertion
+        // into a sorted register array
+        const int32_t* kept = bp.nbr + (int64_t)NEIGH_REG * si;
         int r[NEIGH_REG];
         int n = 0;
-        for_neighbours(bp, G, i, si, [&](int j) {
-            int x = j;
+        for (int k = 0; k < cnt; ++k) {
+            int x = kept[k];
 #pragma unroll
             for (int q = 0; q < NEIGH_REG; ++q) {
                 if (q < n) {
@@ -508,7 +516,7 @@ __global__ void __launch_bounds__(128) k_bp_fill(BroadPhaseBufs bp) {
                 }
             }
             ++n;
-        });
+        }
 #pragma unroll
         for (int q = 0; q < NEIGH_REG; ++q)
             if (q < n) emit(q, r[q], u);
