@@ -321,7 +321,7 @@ struct zdx_sim {
         DBuf<double> wds;
         DBuf<uint8_t> wactive;
     } ps[2];
-    DBuf<int32_t> d_order, d_bcnt, d_bcur, d_topo_list, d_topo_n;
+    DBuf<int32_t> d_order, d_bcnt, d_bcur, d_topo_list, d_topo_n, d_topo_big, d_topo_small, d_topo_nbs;
     DBuf<NarrowItem> d_items;
     DBuf<int64_t> d_seg_off, d_tri_checks, d_tri_tests;
     DBuf<int32_t> d_nseg, d_tri_hits, d_tri_scanned;
@@ -425,6 +425,7 @@ struct zdx_sim {
         P.tri_hits = d_tri_hits.p; P.tri_scanned = d_tri_scanned.p; P.arena = d_arena.p; P.arena_cap = hit_cap;
         P.status = d_status.p;
         P.topo_list = d_topo_list.p; P.topo_n = d_topo_n.p;
+        P.topo_big = d_topo_big.p; P.topo_small = d_topo_small.p; P.n_big = d_topo_nbs.p; P.n_small = d_topo_nbs.p + 1;
         P.hits = d_hits.p; P.hit_cap = hit_cap; P.hit_top = d_hit_top.p; P.hit_off = d_hit_off.p;
         P.hit_flag = d_hit_flag.p;
         return P;
@@ -486,6 +487,7 @@ struct zdx_sim {
         }
         d_adj_j.ensure(adj_cap); d_adj_pair.ensure(adj_cap);
         d_order.ensure(cap); d_items.ensure(cap); d_topo_list.ensure(cap); d_topo_n.ensure(1);
+        d_topo_big.ensure(cap); d_topo_small.ensure(cap); d_topo_nbs.ensure(2);
         d_hit_off.ensure(cap); d_hit_top.ensure(1);
         d_seg_off.ensure(cap); d_tri_checks.ensure(cap); d_tri_tests.ensure(cap);
         d_nseg.ensure(cap); d_tri_hits.ensure(cap); d_tri_scanned.ensure(cap);
@@ -685,6 +687,7 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
         z.i32[3] = s->d_topo_n.p; z.n32[3] = 1;
         z.i32[4] = (int32_t*)s->d_queue.p; z.n32[4] = 1;
         z.i32[5] = batched ? s->d_bcnt.p : nullptr; z.n32[5] = 64;
+        z.i32[6] = s->d_topo_nbs.p; z.n32[6] = 2;
         z.u64[0] = s->d_hit_top.p; z.n64[0] = 1;
         z.u64[1] = s->tw.top.p; z.n64[1] = 5;
         launch_step_begin(kt, z, st);
@@ -1540,6 +1543,7 @@ zdx_status zdx_sim_autotune(zdx_sim* s, int32_t* out_block, int32_t* out_groups)
                          s->d_n_need.p, s->d_wall_list.p, s->d_n_wall.p, false, st);
         launch_transform(B, M, st, s->d_need_list.p, s->d_n_need.p, s->transform_groups());
         DPairs P = s->dpairs(par, true);
+        P.topo_big = P.topo_small = P.n_big = P.n_small = nullptr;   // timing the narrow phase only
         launch_lpt_order(P, M, B, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st);
         const int cand[][2] = {{64, 4}, {64, 8}, {128, 4}, {128, 8}, {256, 1}, {256, 2}, {256, 4}, {256, 8},
                                {512, 2}, {512, 4}};

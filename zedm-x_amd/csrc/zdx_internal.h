@@ -186,6 +186,10 @@ struct DPairs {
     int64_t arena_cap;       // in segments (= hit_cap)
     int32_t* topo_list;      // pairs with at least one segment (appended by the narrow phase)
     int32_t* topo_n;         // device count of topo_list
+    int32_t* topo_big;       // optional: topo_list indices of pairs with many segments (walked first) ...
+    int32_t* topo_small;     // ... and of the others; with their device counts
+    int32_t* n_big;
+    int32_t* n_small;
     int32_t* status;         // global error bits
     int32_t* prov;           // optional (sensitivity runs): 4 ints per arena segment: ta, tb, endpoint codes
     int32_t* hits;           // hit arena: per triangle pair that passed the float AABB sweep, the six

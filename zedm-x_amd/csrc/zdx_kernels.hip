@@ -1043,11 +1043,16 @@ __global__ void __launch_bounds__(256) k_narrow_exact(DBodies B, DPairs P) {
     }
 }
 
+// pairs with at least this many segments are walked first (their walks are
+// the longest, so they should not wait for a free workgroup)
+constexpr int TOPO_BIG_SEGS = 40;
+
 // Narrow phase, part 3: one wave per pair compacts its segments in hit
 // order, i.e. in the reference's (ta, tb) order, to the front of its hit
 // range (in place: a segment only moves to a lower slot, and each round of 64
 // reads its slots before it writes).  A pair with more than seg_cap segments
-// stores none and raises ST_STAGE.
+// stores none and raises ST_STAGE.  With P.topo_big, the pairs with segments
+// are also listed as many-segment or other.
 template <bool PROV>
 __global__ void __launch_bounds__(256) k_narrow_compact(DPairs P, int32_t seg_cap) {
     const int lane = lane_id();
@@ -1103,7 +1108,14 @@ __global__ void __launch_bounds__(256) k_narrow_compact(DPairs P, int32_t seg_ca
             }
             P.seg_off[p] = h0;
             P.nseg[p] = nstore;
-            if (nstore > 0) P.topo_list[atomicAdd(P.topo_n, 1)] = p;
+            if (nstore > 0) {
+                const int idx = atomicAdd(P.topo_n, 1);
+                P.topo_list[idx] = p;
+                if (P.topo_big) {
+                    if (nstore >= TOPO_BIG_SEGS) P.topo_big[atomicAdd(P.n_big, 1)] = idx;
+                    else P.topo_small[atomicAdd(P.n_small, 1)] = idx;
+                }
+            }
         }
     }
 }
@@ -1658,8 +1670,11 @@ __global__ void __launch_bounds__(64) k_tp_walk_bcd_sim(DPairs P, DMeshes M, DBo
     extern __shared__ char s_walk[];
     const int n = *P.topo_n;
     const int nprev = *H.prev_n;
+    const int nbig = P.topo_big ? *P.n_big : 0;
     LaneDev wp;
-    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+        // the pairs with many segments first, in workgroup order
+        const int it = !P.topo_big ? k : (k < nbig ? P.topo_big[k] : P.topo_small[k - nbig]);
         const WalkB r = tp_walk_b_item(P, M, B, T, sp.split, lds_bytes, s_walk, wp, it);
         tp_pass_c_item(P, M, B, T, it, r.ncomp, r.to);
         __syncthreads();
