@@ -1340,20 +1340,25 @@ __global__ void __launch_bounds__(256) k_tp_alloc(DPairs P, TopoBufs T) {
 // flat into an LDS key matrix that one thread per row then compacts; the snap
 // words are built in LDS (each thread a contiguous range of endpoint pairs)
 // and stored whole.
-__global__ void __launch_bounds__(TP_PASS_A_THREADS) k_tp_pass_a(DPairs P, DMeshes M, DBodies B, TopoBufs T,
-                                                                int32_t lds_segs) {
+// MODE 0: every item and every extra unit; MODE 1: the items of
+// P.topo_small only (fewer than TOPO_BIG_SEGS segments, so one unit each);
+// MODE 2: the items of P.topo_big and every extra unit.  KEYS and WORDS size
+// the LDS key matrix and snap-word chunk.
+template <int NT_MAX, int KEYS, int WORDS, int MODE>
+__global__ void __launch_bounds__(NT_MAX) k_tp_pass_a(DPairs P, DMeshes M, DBodies B, TopoBufs T,
+                                                     int32_t lds_segs) {
     extern __shared__ char s_pa[];
-    const int n = *P.topo_n;
-    const int nunits = n + (int)*(T.top + 4);
+    const int n = MODE == 0 ? *P.topo_n : (MODE == 1 ? *P.n_small : *P.n_big);
+    const int nunits = n + (MODE == 1 ? 0 : (int)*(T.top + 4));
     const int R = T.adj_row;
     const int NT = blockDim.x;
     uint8_t* keys = (uint8_t*)(s_pa + 84 * (int64_t)lds_segs);
-    unsigned long long* sw = (unsigned long long*)(s_pa + 84 * (int64_t)lds_segs + TP_PASS_A_KEYS);
+    unsigned long long* sw = (unsigned long long*)(s_pa + 84 * (int64_t)lds_segs + KEYS);
     for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
 #ifdef ZDX_WALK_PROFILE
         const unsigned long long pa_t0 = wall_clock64();
 #endif
-        int it = u, b = 0;
+        int it = MODE == 0 ? u : (u < n ? (MODE == 1 ? P.topo_small[u] : P.topo_big[u]) : 0), b = 0;
         if (u >= n) {
             if (u - n >= T.units_cap) continue;
             const int2 e = ((const int2*)T.units)[u - n];
@@ -1384,7 +1389,7 @@ __global__ void __launch_bounds__(TP_PASS_A_THREADS) k_tp_pass_a(DPairs P, DMesh
         int* cntp = T.adj + ao;
         int* rows = cntp + S;
         const int nr = r1 - r0;
-        if (nr * S <= TP_PASS_A_KEYS) {
+        if (nr * S <= KEYS) {
             for (int k = threadIdx.x; k < nr * S; k += NT) {
                 const int i = r0 + k / S, j = k - (k / S) * S;
                 keys[k] = (uint8_t)tp_adj_key(seg, c9, h, i, j);
@@ -1410,12 +1415,12 @@ __global__ void __launch_bounds__(TP_PASS_A_THREADS) k_tp_pass_a(DPairs P, DMesh
                 if (m > R) atomicMax(T.top + 3, (unsigned long long)m);
             }
         }
-        // snap hits of the unit's words, TP_PASS_A_WORDS at a time: each
+        // snap hits of the unit's words, WORDS at a time: each
         // thread a contiguous range of pair numbers t (a(a-1)/2 + b) sets bits
         // in LDS, then the words are stored whole
         uint64_t* w = T.snapw + so;
-        for (int c0 = w0; c0 < w1; c0 += TP_PASS_A_WORDS) {
-            const int c1 = c0 + TP_PASS_A_WORDS < w1 ? c0 + TP_PASS_A_WORDS : w1;
+        for (int c0 = w0; c0 < w1; c0 += WORDS) {
+            const int c1 = c0 + WORDS < w1 ? c0 + WORDS : w1;
             for (int k = threadIdx.x; k < c1 - c0; k += NT) sw[k] = 0ull;
             __syncthreads();
             const int tb = 64 * c0, te = 64 * c1 < npt ? 64 * c1 : npt;
@@ -2091,9 +2096,22 @@ static void launch_pass_a(const DPairs& P, const DMeshes& M, const DBodies& B, c
                           bool zeroed = false) {
     if (!zeroed) launch_fill_u64(T.top, 0ull, 5, s);
     hipLaunchKernelGGL(k_tp_alloc, dim3((unsigned)((T.pass_groups + 255) / 256 * 4)), dim3(256), 0, s, P, T);
-    hipLaunchKernelGGL(k_tp_pass_a, dim3((unsigned)T.pass_groups), dim3(TP_PASS_A_THREADS),
-                       84 * TP_PASS_A_SEGS + TP_PASS_A_KEYS + 8 * TP_PASS_A_WORDS, s, P, M, B, T,
-                       (int32_t)TP_PASS_A_SEGS);
+    if (P.topo_big) {
+        // the pairs with few segments in small workgroups (many per CU), then
+        // the big ones and their extra units in 1024-thread workgroups
+        constexpr int SK = TOPO_BIG_SEGS * TOPO_BIG_SEGS, SW = 64;
+        hipLaunchKernelGGL((k_tp_pass_a<256, SK, SW, 1>), dim3((unsigned)T.pass_groups * 4), dim3(256),
+                           84 * TOPO_BIG_SEGS + SK + 8 * SW, s, P, M, B, T, (int32_t)TOPO_BIG_SEGS);
+        hipLaunchKernelGGL((k_tp_pass_a<TP_PASS_A_THREADS, TP_PASS_A_KEYS, TP_PASS_A_WORDS, 2>),
+                           dim3((unsigned)T.pass_groups), dim3(TP_PASS_A_THREADS),
+                           84 * TP_PASS_A_SEGS + TP_PASS_A_KEYS + 8 * TP_PASS_A_WORDS, s, P, M, B, T,
+                           (int32_t)TP_PASS_A_SEGS);
+    } else {
+        hipLaunchKernelGGL((k_tp_pass_a<TP_PASS_A_THREADS, TP_PASS_A_KEYS, TP_PASS_A_WORDS, 0>),
+                           dim3((unsigned)T.pass_groups), dim3(TP_PASS_A_THREADS),
+                           84 * TP_PASS_A_SEGS + TP_PASS_A_KEYS + 8 * TP_PASS_A_WORDS, s, P, M, B, T,
+                           (int32_t)TP_PASS_A_SEGS);
+    }
 }
 
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
