@@ -676,7 +676,9 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     // 3 after the narrow phase, 4 after the topology, 5 when the wall stream has
     // joined, 6 after fold + integrate, 7 after the walls (on the wall stream,
     // which starts at 2 and runs alongside 2..4), 8 before the narrow-phase
-    // kernels (after the work ordering)
+    // kernels (after the work ordering).  Most are written by the first
+    // workgroup of the kernel that starts the next stage; 7 (and 2, 3 in
+    // plain mode, where the host reads counts in between) by a stamp kernel.
     unsigned long long* kt = s->d_kt.p;
     const int gmax = persistent_groups(s->ctx, s->np_groups_per_cu);
     {
@@ -693,9 +695,8 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
         launch_step_begin(kt, z, st);
     }
     launch_broadphase(s->bpbufs(par), st);
-    launch_stamp(kt + 1, st);
     launch_need_list(s->N, s->d_state[par].p, s->d_radius.p, s->d_cnt_all.p, s->wallbufs(), s->d_need_list.p,
-                     s->d_n_need.p, s->d_wall_list.p, s->d_n_wall.p, true, st);
+                     s->d_n_need.p, s->d_wall_list.p, s->d_n_wall.p, true, st, kt + 1);
     int g_transform = s->transform_groups(), g_narrow = gmax, g_topo = (int)s->topo_threads,
         g_walls = (int)s->wall_threads;
     if (!batched) {
@@ -709,7 +710,7 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
         g_walls = std::max(1, std::min(c[2], g_walls));
     }
     launch_transform(B, M, st, s->d_need_list.p, s->d_n_need.p, g_transform);
-    launch_stamp(kt + 2, st);
+    if (!batched) launch_stamp(kt + 2, st);   // batched: stamped by the work ordering's first kernel
     const bool walls = s->W > 0;
     if (walls) {
         // the wall contacts need only the world vertices and the wall list:
@@ -723,13 +724,13 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
         HIPC(hipEventRecord(s->ctx->ev_join, st2));
     }
     DPairs P = s->dpairs(par, batched);
-    if (batched) launch_lpt_order(P, M, B, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st, true);
+    if (batched)
+        launch_lpt_order(P, M, B, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st, true, kt + 2);
     NarrowCfg nc = s->ncfg;
     nc.use_queue = batched ? 1 : 0;
     nc.groups = g_narrow;
-    launch_stamp(kt + 8, st);
-    launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->stage_cap, s->tmax, s->d_queue.p, st, true);
-    launch_stamp(kt + 3, st);
+    launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->stage_cap, s->tmax, s->d_queue.p, st, true, kt + 8);
+    if (!batched) launch_stamp(kt + 3, st);   // batched: stamped by k_topo_zero (no host read in between)
     TopoBufs T = s->topobufs();
     if (!batched) {
         int32_t nt = 0;
@@ -739,16 +740,15 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
         T.pass_groups = std::max<int64_t>(1, std::min<int64_t>(nt, T.pass_groups));
     }
     HistBufs H = s->hist(par);
-    launch_topo_zero(P, T, H.cur_active, s->pair_cap, st);
+    launch_topo_zero(P, T, H.cur_active, s->pair_cap, st, batched ? kt + 3 : nullptr);
     launch_topo_sim(P, M, B, mat, H, T, s->sp, st, true);
-    launch_stamp(kt + 4, st);
     if (walls) HIPC(hipStreamWaitEvent(st, s->ctx->ev_join, 0));
-    launch_stamp(kt + 5, st);
+    // stamps 4 (topology done, with any wait for the wall stream) and 5 (fold
+    // start) come from the fold's first workgroup, 6 from the counters'
     launch_fold_integrate(s->N, s->d_state[par].p, s->d_state[nxt].p, mat, s->d_adj_off.p, s->d_adj_j.p,
                           s->d_adj_pair.p, s->adj_cap, T, s->W, s->d_wcontrib.p, s->d_wnc.p, s->max_wall_tris,
-                          s->d_F.p, s->d_T.p, s->d_counts.p, s->sp, st);
-    launch_stamp(kt + 6, st);
-    launch_counters(P, T, s->N * s->W, s->d_wnc.p, s->d_ctr.p, st);
+                          s->d_F.p, s->d_T.p, s->d_counts.p, s->sp, st, kt + 4, kt + 5);
+    launch_counters(P, T, s->N * s->W, s->d_wnc.p, s->d_ctr.p, st, kt + 6);
     launch_report(s->d_status.p, s->d_ctr.p, kt, s->rep, st);
 }
 

@@ -169,21 +169,25 @@ void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s, const i
 // (and the particles near a wall triangle -> wlist), in no fixed order;
 // zeroed: *n_list and *n_wlist already hold 0
 void launch_need_list(int32_t n, const double* state, const double* radius, const int32_t* cnt_all, const WallBufs& W,
-                      int32_t* list, int32_t* n_list, int32_t* wlist, int32_t* n_wlist, bool zeroed, hipStream_t s);
+                      int32_t* list, int32_t* n_list, int32_t* wlist, int32_t* n_wlist, bool zeroed, hipStream_t s,
+                      unsigned long long* stamp = nullptr);
 void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s);
 // Narrow phase: cull + sweep into the hit arena, the exact tests per hit, and
 // the per-pair compaction into the segment arena.  seg_cap: most segments one
 // pair may store (ST_STAGE beyond); queue: the cull's work counter; resets
 // the queue and P.hit_top itself unless the caller did (zeroed).
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* list_i,
-                   float* list_f, int32_t seg_cap, int32_t tmax, unsigned int* queue, hipStream_t s, bool zeroed = false);
+                   float* list_f, int32_t seg_cap, int32_t tmax, unsigned int* queue, hipStream_t s, bool zeroed = false,
+                   unsigned long long* stamp = nullptr);
 // LPT work order of the pairs (largest triangle product first) into order,
 // and the narrow-phase items (P.items, when set) in that order
 void launch_lpt_order(const DPairs& P, const DMeshes& M, const DBodies& B, int32_t* order, int32_t* bucket_cnt,
-                      int32_t* bucket_cur, int32_t pair_cap, hipStream_t s, bool zeroed = false);
+                      int32_t* bucket_cur, int32_t pair_cap, hipStream_t s, bool zeroed = false,
+                      unsigned long long* stamp = nullptr);
 // zero outputs of every pair (the topology kernels then visit only the
 // pairs of P.topo_list); cur_active may be NULL
-void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, int32_t pair_cap, hipStream_t s);
+void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, int32_t pair_cap, hipStream_t s,
+                      unsigned long long* stamp = nullptr);
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
                      const TopoBufs& T, const SimParams& sp, hipStream_t s, bool zeroed = false);
 void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int32_t split,
@@ -197,11 +201,12 @@ void launch_fold_integrate(int32_t n, const double* state_in, double* state_out,
                            const int32_t* adj_off, const int32_t* adj_j, const int32_t* adj_pair,
                            int32_t adj_cap, const TopoBufs& T, int32_t n_walls, const double* wcontrib, const int32_t* wncontrib,
                            int32_t max_wall_tris, double* F, double* Tq, int32_t* counts, const SimParams& sp,
-                           hipStream_t s);
+                           hipStream_t s,
+                           unsigned long long* stamp_a = nullptr, unsigned long long* stamp_b = nullptr);
 // counters[ZDX_N_COUNTERS] (int64) summed from per-pair / per-wall arrays into
 // `counters`, which must hold zeros (launch_report leaves them so)
 void launch_counters(const DPairs& P, const TopoBufs& T, int32_t n_pw, const int32_t* wncontrib,
-                     unsigned long long* counters, hipStream_t s);
+                     unsigned long long* counters, hipStream_t s, unsigned long long* stamp = nullptr);
 void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, int64_t n, hipStream_t s);
 void launch_iota_pairs(int32_t* pi, int32_t* pj, int32_t n, hipStream_t s);

@@ -94,6 +94,15 @@ __device__ inline V3 ld3(const double* p) { return v3(p[0], p[1], p[2]); }
 // A per-kernel scheme with an atomic from every wave would serialise tens of
 // thousands of same-address atomics per launch.
 __global__ void k_stamp(unsigned long long* slot) { *slot = (unsigned long long)wall_clock64(); }
+// The same from inside the kernel that follows a stage (workgroup 0, thread
+// 0, at its start): saves a launch per stamp.
+__device__ inline void stamp_here(unsigned long long* a, unsigned long long* b = nullptr) {
+    if ((a || b) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const unsigned long long t = (unsigned long long)wall_clock64();
+        if (a) *a = t;
+        if (b) *b = t;
+    }
+}
 __global__ void __launch_bounds__(64) k_step_begin(unsigned long long* kt, StepZero z) {
     if (threadIdx.x == 0) kt[0] = (unsigned long long)wall_clock64();
     for (int a = 0; a < 8; ++a)
@@ -585,7 +594,9 @@ __device__ inline bool near_wall(const WallBufs& W, int w, const V3& pos, double
 // per-body results.
 __global__ void __launch_bounds__(256) k_need_lists(int32_t n, const double* state, const double* radius,
                                                     const int32_t* cnt_all, WallBufs W, int32_t* list,
-                                                    int32_t* n_list, int32_t* wlist, int32_t* n_wlist) {
+                                                    int32_t* n_list, int32_t* wlist, int32_t* n_wlist,
+                                                    unsigned long long* stamp) {
+    stamp_here(stamp);
     __shared__ int s_cnt[2][4];
     __shared__ int s_base[2];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -689,7 +700,9 @@ constexpr int CULL_PER_THREAD = 8;
 // workgroups per CU as 5 would give, with room to keep everything in registers).
 template <int BS, int K>
 __global__ void __launch_bounds__(BS, BS >= 512 ? 4 : (BS >= 256 ? 5 : 1)) k_narrow(DBodies B, DMeshes M, DPairs P, int32_t* list_i, float* list_f,
-                                               int32_t tmax, int32_t use_queue, unsigned int* queue, int32_t lds_cap) {
+                                               int32_t tmax, int32_t use_queue, unsigned int* queue, int32_t lds_cap,
+                                               unsigned long long* stamp) {
+    stamp_here(stamp);
     // Body B's culled float boxes (24 bytes per triangle, read for every item
     // of the sweep) live in dynamic LDS when there are at most lds_cap of
     // them, else in the workgroup's global list.  Per workgroup in global
@@ -1128,7 +1141,8 @@ __device__ inline int lpt_bucket(const DPairs& P, const DMeshes& M, const int32_
 }
 // per-workgroup histograms first, one global atomic per (workgroup, bucket)
 __global__ void __launch_bounds__(256) k_lpt_count(DPairs P, DMeshes M, const int32_t* body_mesh,
-                                                   int32_t* bucket_cnt) {
+                                                   int32_t* bucket_cnt, unsigned long long* stamp) {
+    stamp_here(stamp);
     __shared__ int h[64];
     if (threadIdx.x < 64) h[threadIdx.x] = 0;
     __syncthreads();
@@ -1690,7 +1704,8 @@ __global__ void __launch_bounds__(64) k_tp_walk_bcd_sim(DPairs P, DMeshes M, DBo
 // Outputs of pairs the topology kernels do not visit (no segments): no
 // loops, no history entry (the reference's map entry is only created for an
 // accepted loop and erased otherwise).
-__global__ void k_topo_zero(DPairs P, TopoBufs T, uint8_t* cur_active) {
+__global__ void k_topo_zero(DPairs P, TopoBufs T, uint8_t* cur_active, unsigned long long* stamp) {
+    stamp_here(stamp);
     const int n = *P.n;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
         T.ncontrib[p] = 0;
@@ -1854,7 +1869,9 @@ __global__ void __launch_bounds__(256) k_fold_integrate(int32_t n, const double*
                                                         const int32_t* adj_pair, int32_t adj_cap, TopoBufs T,
                                                         int32_t n_walls,
                                                         const double* wcontrib, const int32_t* wnc, int32_t mwt,
-                                                        double* Fo, double* To, int32_t* counts, SimParams sp) {
+                                                        double* Fo, double* To, int32_t* counts, SimParams sp,
+                                                        unsigned long long* stamp_a, unsigned long long* stamp_b) {
+    stamp_here(stamp_a, stamp_b);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     // forces[i] and torques[i] start at zero and receive, in the reference's
@@ -1906,7 +1923,8 @@ __global__ void __launch_bounds__(256) k_fold_integrate(int32_t n, const double*
 
 // ------------------------------------------------------------ counters
 __global__ void __launch_bounds__(256) k_counters(DPairs P, TopoBufs T, int32_t n_pw, const int32_t* wnc,
-                                                  unsigned long long* ctr) {
+                                                  unsigned long long* ctr, unsigned long long* stamp) {
+    stamp_here(stamp);
     const int n = *P.n;
     unsigned long long c_loops = 0, c_checks = 0, c_seg = 0, c_comps = 0, c_raw = 0, c_tests = 0, c_hits = 0,
                        c_wall = 0, c_scan = 0, c_tp = 0, c_smax = 0;
@@ -2008,13 +2026,14 @@ void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s, const i
 }
 
 void launch_need_list(int32_t n, const double* state, const double* radius, const int32_t* cnt_all, const WallBufs& W,
-                      int32_t* list, int32_t* n_list, int32_t* wlist, int32_t* n_wlist, bool zeroed, hipStream_t s) {
+                      int32_t* list, int32_t* n_list, int32_t* wlist, int32_t* n_wlist, bool zeroed, hipStream_t s,
+                      unsigned long long* stamp) {
     if (!zeroed) {
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, n_list, 0, nullptr, (int64_t)1);
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, n_wlist, 0, nullptr, (int64_t)1);
     }
     hipLaunchKernelGGL(k_need_lists, dim3(nblk(n, 256)), dim3(256), 0, s, n, state, radius, cnt_all, W, list, n_list,
-                       wlist, n_wlist);
+                       wlist, n_wlist, stamp);
 }
 
 void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s) {
@@ -2036,33 +2055,34 @@ void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s) {
 
 template <int BS, int K>
 static void narrow_t(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
-                     int32_t tmax, unsigned int* q, hipStream_t s) {
+                     int32_t tmax, unsigned int* q, hipStream_t s, unsigned long long* stamp) {
     // LDS for body B's culled boxes: 4 per thread (at most 2048), so that small
     // workgroups leave room for many pairs in flight per CU
     const int cap = tmax < 4 * BS ? tmax : (4 * BS < 2048 ? 4 * BS : 2048);
     hipLaunchKernelGGL((k_narrow<BS, K>), dim3(cfg.groups), dim3(BS), 24 * (size_t)cap, s, B, M, P, li, lf, tmax,
-                       cfg.use_queue, q, cap);
+                       cfg.use_queue, q, cap, stamp);
 }
 
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
-                   int32_t seg_cap, int32_t tmax, unsigned int* q, hipStream_t s, bool zeroed) {
+                   int32_t seg_cap, int32_t tmax, unsigned int* q, hipStream_t s, bool zeroed,
+                   unsigned long long* stamp) {
     if (!zeroed) {
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int32_t*)q, 0, nullptr, (int64_t)1);
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, P.hit_top, 0ull, (int64_t)1);
     }
     const int key = P.prov ? 25604 : cfg.block * 100 + cfg.items;
     switch (key) {
-        case 6404: narrow_t<64, 4>(B, M, P, cfg, li, lf, tmax, q, s); break;
-        case 6408: narrow_t<64, 8>(B, M, P, cfg, li, lf, tmax, q, s); break;
-        case 12804: narrow_t<128, 4>(B, M, P, cfg, li, lf, tmax, q, s); break;
-        case 12808: narrow_t<128, 8>(B, M, P, cfg, li, lf, tmax, q, s); break;
-        case 25601: narrow_t<256, 1>(B, M, P, cfg, li, lf, tmax, q, s); break;
-        case 25602: narrow_t<256, 2>(B, M, P, cfg, li, lf, tmax, q, s); break;
-        case 25604: narrow_t<256, 4>(B, M, P, cfg, li, lf, tmax, q, s); break;
-        case 25608: narrow_t<256, 8>(B, M, P, cfg, li, lf, tmax, q, s); break;
-        case 51202: narrow_t<512, 2>(B, M, P, cfg, li, lf, tmax, q, s); break;
-        case 51204: narrow_t<512, 4>(B, M, P, cfg, li, lf, tmax, q, s); break;
-        default: narrow_t<256, 4>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        case 6404: narrow_t<64, 4>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
+        case 6408: narrow_t<64, 8>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
+        case 12804: narrow_t<128, 4>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
+        case 12808: narrow_t<128, 8>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
+        case 25601: narrow_t<256, 1>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
+        case 25602: narrow_t<256, 2>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
+        case 25604: narrow_t<256, 4>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
+        case 25608: narrow_t<256, 8>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
+        case 51202: narrow_t<512, 2>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
+        case 51204: narrow_t<512, 4>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
+        default: narrow_t<256, 4>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
     }
     const int ge = std::max(1, cfg.groups) * 4;   // grid-stride over the hits
     const int gc = std::max(1, cfg.groups);        // 4 waves per workgroup over the listed pairs
@@ -2076,20 +2096,21 @@ void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const Na
 }
 
 void launch_lpt_order(const DPairs& P, const DMeshes& M, const DBodies& B, int32_t* order, int32_t* bcnt,
-                      int32_t* bcur, int32_t pair_cap, hipStream_t s, bool zeroed) {
+                      int32_t* bcur, int32_t pair_cap, hipStream_t s, bool zeroed, unsigned long long* stamp) {
     const int32_t* body_mesh = B.mesh;
     if (!zeroed) hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, bcnt, 0, nullptr, (int64_t)64);
     const int g = std::max(1, std::min(nblk(pair_cap, 256), 512));
-    hipLaunchKernelGGL(k_lpt_count, dim3(g), dim3(256), 0, s, P, M, body_mesh, bcnt);
+    hipLaunchKernelGGL(k_lpt_count, dim3(g), dim3(256), 0, s, P, M, body_mesh, bcnt, stamp);
     hipLaunchKernelGGL(k_lpt_scan, dim3(1), dim3(64), 0, s, bcnt, bcur);
     hipLaunchKernelGGL(k_lpt_scatter, dim3(g), dim3(256), 0, s, P, M, B, bcur, order);
 }
 
-void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, int32_t pair_cap, hipStream_t s) {
+void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, int32_t pair_cap, hipStream_t s,
+                      unsigned long long* stamp) {
     int g = nblk(pair_cap, 256);
     if (g > 2048) g = 2048;
     if (g < 1) g = 1;
-    hipLaunchKernelGGL(k_topo_zero, dim3(g), dim3(256), 0, s, P, T, cur_active);
+    hipLaunchKernelGGL(k_topo_zero, dim3(g), dim3(256), 0, s, P, T, cur_active, stamp);
 }
 
 static void launch_pass_a(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, hipStream_t s,
@@ -2144,13 +2165,14 @@ void launch_fold_integrate(int32_t n, const double* sin, double* sout, const DMa
                            const int32_t* adj_j, const int32_t* adj_pair, int32_t adj_cap, const TopoBufs& T,
                            int32_t n_walls,
                            const double* wcontrib, const int32_t* wnc, int32_t mwt, double* F, double* Tq,
-                           int32_t* counts, const SimParams& sp, hipStream_t s) {
+                           int32_t* counts, const SimParams& sp, hipStream_t s, unsigned long long* stamp_a,
+                           unsigned long long* stamp_b) {
     hipLaunchKernelGGL(k_fold_integrate, dim3(nblk(n, 256)), dim3(256), 0, s, n, sin, sout, mat, adj_off, adj_j,
-                       adj_pair, adj_cap, T, n_walls, wcontrib, wnc, mwt, F, Tq, counts, sp);
+                       adj_pair, adj_cap, T, n_walls, wcontrib, wnc, mwt, F, Tq, counts, sp, stamp_a, stamp_b);
 }
 void launch_counters(const DPairs& P, const TopoBufs& T, int32_t n_pw, const int32_t* wnc, unsigned long long* ctr,
-                     hipStream_t s) {
-    hipLaunchKernelGGL(k_counters, dim3(32), dim3(256), 0, s, P, T, n_pw, wnc, ctr);
+                     hipStream_t s, unsigned long long* stamp) {
+    hipLaunchKernelGGL(k_counters, dim3(32), dim3(256), 0, s, P, T, n_pw, wnc, ctr, stamp);
 }
 void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s) {
     if (n <= 0) return;
