@@ -244,7 +244,8 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir, command="", timed_stats=
         w_scale = kib / max(cw)
         fetch_b = nf_kib * f_scale * 1024.0
         write_b = nw_kib * w_scale * 1024.0
-        res = dict(kernel="narrow phase: " + " + ".join(NARROW_KERNELS), dispatches=timed_pmc,
+        res = dict(kernel="narrow phase: k_narrow (cull + sweep) + k_narrow_exact + k_narrow_compact",
+                   dispatches=timed_pmc,
                    fetch_bytes_per_launch=fetch_b,
                    write_bytes_per_launch=write_b, bytes_per_launch=fetch_b + write_b,
                    fetch_scale=f_scale, write_scale=w_scale, copy_bytes=COPY_BYTES,
