@@ -432,17 +432,19 @@ __device__ inline void for_neighbours(const BroadPhaseBufs& bp, const DGrid& G, 
     const double xi = qi[0], yi = qi[1], zi = qi[2], ri = qi[3];
     int cx, cy, cz;
     cell_coords(G, qi, cx, cy, cz);
+    // the (up to) three cells along x of a row are consecutive cell indices,
+    // so their particles are one contiguous range of the cell order: one range
+    // per (dy, dz), visited in the same order as cell by cell
+    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < G.nx ? cx + 1 : G.nx - 1;
     for (int dz = -1; dz <= 1; ++dz) {
         int z = cz + dz;
         if (z < 0 || z >= G.nz) continue;
         for (int dy = -1; dy <= 1; ++dy) {
             int y = cy + dy;
             if (y < 0 || y >= G.ny) continue;
-            for (int dx = -1; dx <= 1; ++dx) {
-                int x = cx + dx;
-                if (x < 0 || x >= G.nx) continue;
-                int c = (z * G.ny + y) * G.nx + x;
-                int s0 = bp.cell_start[c], s1 = bp.cell_start[c + 1];
+            {
+                const int crow = (z * G.ny + y) * G.nx;
+                int s0 = bp.cell_start[crow + x0], s1 = bp.cell_start[crow + x1 + 1];
                 for (int k = s0; k < s1; ++k) {
                     const int j = bp.sorted[k];
                     if (j == i) continue;
