@@ -162,13 +162,27 @@ __global__ void __launch_bounds__(256) k_transform(DBodies B, DMeshes M, const i
         const double* bv = M.verts + 3 * (int64_t)M.vert_off[m];
         double* wv = B.wv + 3 * B.wv_off[b];
         double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (int v = ln; v < nv; v += WAVE) {
-            Qt pv; pv.w = 0.0; pv.x = bv[3 * v]; pv.y = bv[3 * v + 1]; pv.z = bv[3 * v + 2];
-            Qt r = qmul(qmul(qn, pv), qc);
-            V3 w = add(v3(r.x, r.y, r.z), pos);
-            wv[3 * v] = w.x; wv[3 * v + 1] = w.y; wv[3 * v + 2] = w.z;
-            mn[0] = fmin(mn[0], w.x); mn[1] = fmin(mn[1], w.y); mn[2] = fmin(mn[2], w.z);
-            mx[0] = fmax(mx[0], w.x); mx[1] = fmax(mx[1], w.y); mx[2] = fmax(mx[2], w.z);
+        // TV rounds of the wave's vertices per pass: all their loads first
+        constexpr int TV = 4;
+        for (int v0 = ln; v0 < nv; v0 += TV * WAVE) {
+            double bx[TV][3];
+#pragma unroll
+            for (int u = 0; u < TV; ++u) {
+                const int v = v0 + u * WAVE;
+                if (v < nv) { bx[u][0] = bv[3 * v]; bx[u][1] = bv[3 * v + 1]; bx[u][2] = bv[3 * v + 2]; }
+            }
+#pragma unroll
+            for (int u = 0; u < TV; ++u) {
+                const int v = v0 + u * WAVE;
+                if (v < nv) {
+                    Qt pv; pv.w = 0.0; pv.x = bx[u][0]; pv.y = bx[u][1]; pv.z = bx[u][2];
+                    Qt r = qmul(qmul(qn, pv), qc);
+                    V3 w = add(v3(r.x, r.y, r.z), pos);
+                    wv[3 * v] = w.x; wv[3 * v + 1] = w.y; wv[3 * v + 2] = w.z;
+                    mn[0] = fmin(mn[0], w.x); mn[1] = fmin(mn[1], w.y); mn[2] = fmin(mn[2], w.z);
+                    mx[0] = fmax(mx[0], w.x); mx[1] = fmax(mx[1], w.y); mx[2] = fmax(mx[2], w.z);
+                }
+            }
         }
         for (int k = 0; k < 3; ++k) {
             for (int d = 32; d >= 1; d >>= 1) {
