@@ -164,6 +164,9 @@ struct zdx_ctx {
     // alongside the narrow phase and the topology; fork/join events
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // third stream: pass A of the many-segment pairs beside that of the others
+    hipStream_t stream3 = nullptr;
+    hipEvent_t ev_fork3 = nullptr, ev_join3 = nullptr;
     std::string err;
     std::vector<MeshRec> meshes;
     bool mesh_dirty = true;
@@ -741,7 +744,7 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     }
     HistBufs H = s->hist(par);
     launch_topo_zero(P, T, H.cur_active, s->pair_cap, st, batched ? kt + 3 : nullptr);
-    launch_topo_sim(P, M, B, mat, H, T, s->sp, st, true);
+    launch_topo_sim(P, M, B, mat, H, T, s->sp, st, true, s->ctx->stream3, s->ctx->ev_fork3, s->ctx->ev_join3);
     if (walls) HIPC(hipStreamWaitEvent(st, s->ctx->ev_join, 0));
     // stamps 4 (topology done, with any wait for the wall stream) and 5 (fold
     // start) come from the fold's first workgroup, 6 from the counters'
@@ -927,6 +930,9 @@ zdx_status zdx_create(zdx_ctx** out, int32_t device) {
         HIPC(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
         HIPC(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
         HIPC(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+        HIPC(hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking));
+        HIPC(hipEventCreateWithFlags(&c->ev_fork3, hipEventDisableTiming));
+        HIPC(hipEventCreateWithFlags(&c->ev_join3, hipEventDisableTiming));
     });
     if (st != ZDX_OK) {
         delete c;
@@ -941,6 +947,10 @@ void zdx_destroy(zdx_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    if (c->stream3) (void)hipStreamSynchronize(c->stream3);
+    if (c->ev_fork3) (void)hipEventDestroy(c->ev_fork3);
+    if (c->ev_join3) (void)hipEventDestroy(c->ev_join3);
+    if (c->stream3) (void)hipStreamDestroy(c->stream3);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);

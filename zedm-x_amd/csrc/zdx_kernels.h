@@ -188,8 +188,11 @@ void launch_lpt_order(const DPairs& P, const DMeshes& M, const DBodies& B, int32
 // pairs of P.topo_list); cur_active may be NULL
 void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, int32_t pair_cap, hipStream_t s,
                       unsigned long long* stamp = nullptr);
+// side/ev_fork/ev_join (optional, with P.topo_big): pass A of the many-segment
+// pairs runs on `side`, forked from and joined back into s
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
-                     const TopoBufs& T, const SimParams& sp, hipStream_t s, bool zeroed = false);
+                     const TopoBufs& T, const SimParams& sp, hipStream_t s, bool zeroed = false,
+                     hipStream_t side = nullptr, hipEvent_t ev_fork = nullptr, hipEvent_t ev_join = nullptr);
 void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int32_t split,
                      hipStream_t s);
 // one wave per particle of `list` (particles near a wall triangle); scr holds

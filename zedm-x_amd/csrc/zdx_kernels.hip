@@ -2130,19 +2130,29 @@ void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, i
 }
 
 static void launch_pass_a(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, hipStream_t s,
-                          bool zeroed = false) {
+                          bool zeroed = false, hipStream_t side = nullptr, hipEvent_t ev_fork = nullptr,
+                          hipEvent_t ev_join = nullptr) {
     if (!zeroed) launch_fill_u64(T.top, 0ull, 5, s);
     hipLaunchKernelGGL(k_tp_alloc, dim3((unsigned)((T.pass_groups + 255) / 256 * 4)), dim3(256), 0, s, P, T);
     if (P.topo_big) {
-        // the pairs with few segments in small workgroups (many per CU), then
-        // the big ones and their extra units in 1024-thread workgroups
+        // the pairs with few segments in small workgroups (many per CU) and
+        // the big ones with their extra units in 1024-thread workgroups; with
+        // a side stream the two launches run side by side
+        const bool fork = side && ev_fork && ev_join;
+        hipStream_t sb = fork ? side : s;
+        if (fork) {
+            (void)hipEventRecord(ev_fork, s);
+            (void)hipStreamWaitEvent(side, ev_fork, 0);
+        }
+        hipLaunchKernelGGL((k_tp_pass_a<TP_PASS_A_THREADS, TP_PASS_A_KEYS, TP_PASS_A_WORDS, 2>),
+                           dim3((unsigned)T.pass_groups), dim3(TP_PASS_A_THREADS),
+                           84 * TP_PASS_A_SEGS + TP_PASS_A_KEYS + 8 * TP_PASS_A_WORDS, sb, P, M, B, T,
+                           (int32_t)TP_PASS_A_SEGS);
+        if (fork) (void)hipEventRecord(ev_join, side);
         constexpr int SK = TOPO_BIG_SEGS * TOPO_BIG_SEGS, SW = 64;
         hipLaunchKernelGGL((k_tp_pass_a<256, SK, SW, 1>), dim3((unsigned)T.pass_groups * 4), dim3(256),
                            84 * TOPO_BIG_SEGS + SK + 8 * SW, s, P, M, B, T, (int32_t)TOPO_BIG_SEGS);
-        hipLaunchKernelGGL((k_tp_pass_a<TP_PASS_A_THREADS, TP_PASS_A_KEYS, TP_PASS_A_WORDS, 2>),
-                           dim3((unsigned)T.pass_groups), dim3(TP_PASS_A_THREADS),
-                           84 * TP_PASS_A_SEGS + TP_PASS_A_KEYS + 8 * TP_PASS_A_WORDS, s, P, M, B, T,
-                           (int32_t)TP_PASS_A_SEGS);
+        if (fork) (void)hipStreamWaitEvent(s, ev_join, 0);
     } else {
         hipLaunchKernelGGL((k_tp_pass_a<TP_PASS_A_THREADS, TP_PASS_A_KEYS, TP_PASS_A_WORDS, 0>),
                            dim3((unsigned)T.pass_groups), dim3(TP_PASS_A_THREADS),
@@ -2152,8 +2162,9 @@ static void launch_pass_a(const DPairs& P, const DMeshes& M, const DBodies& B, c
 }
 
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
-                     const TopoBufs& T, const SimParams& sp, hipStream_t s, bool zeroed) {
-    launch_pass_a(P, M, B, T, s, zeroed);
+                     const TopoBufs& T, const SimParams& sp, hipStream_t s, bool zeroed, hipStream_t side,
+                     hipEvent_t ev_fork, hipEvent_t ev_join) {
+    launch_pass_a(P, M, B, T, s, zeroed, side, ev_fork, ev_join);
     hipLaunchKernelGGL(k_tp_walk_bcd_sim, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, mat, H, T,
                        sp, T.lds_bytes);
 }
