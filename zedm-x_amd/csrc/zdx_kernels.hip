@@ -1710,10 +1710,16 @@ __global__ void __launch_bounds__(64) k_tp_walk_bcd_sim(DPairs P, DMeshes M, DBo
     for (int k = blockIdx.x; k < n; k += gridDim.x) {
         // the pairs with many segments first, in workgroup order
         const int it = !P.topo_big ? k : (k < nbig ? P.topo_big[k] : P.topo_small[k - nbig]);
+#ifdef ZDX_WALK_PROFILE
+        const unsigned long long chain_t0 = wall_clock64();
+#endif
         const WalkB r = tp_walk_b_item(P, M, B, T, sp.split, lds_bytes, s_walk, wp, it);
         tp_pass_c_item(P, M, B, T, it, r.ncomp, r.to);
         __syncthreads();
         tp_walk_d_sim_item(P, M, B, mat, H, T, sp, nprev, lds_bytes, s_walk, wp, it, r.ncomp, r.to);
+#ifdef ZDX_WALK_PROFILE
+        if (lane_id() == 0 && it < 8192) g_topo_pairs[4 * it + 3] = (long long)(wall_clock64() - chain_t0);
+#endif
     }
 }
 

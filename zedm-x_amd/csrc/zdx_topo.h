@@ -550,13 +550,26 @@ ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double 
         // ---- split_t_junctions: per edge, its hits in vertex order, put in
         // the reference's cell-scan order, std::sort by t, chained
         int ne2 = 0;
+        // with at most W words per row, lane w's word of the rows of the next
+        // four edges is already in flight while an edge is split
+        const bool ring = rw <= W;
+        auto row_word = [&](int e) { return (ring && e < ne && ln < rw) ? Tq[(int64_t)e * rw + ln] : 0ull; };
+        struct Ends { int x, y; };
+        auto ends = [&](int e) { Ends r{0, 0}; if (e < ne) { r.x = Eq[2 * e]; r.y = Eq[2 * e + 1]; } return r; };
+        uint64_t q0 = row_word(0), q1 = row_word(1), q2 = row_word(2), q3 = row_word(3);
+        Ends g0 = ends(0), g1 = ends(1), g2 = ends(2), g3 = ends(3);
         for (int e = 0; e < ne; ++e) {
-            const int a = Eq[2 * e], b = Eq[2 * e + 1];
+            const int a = g0.x, b = g0.y;
             const uint64_t* row = Tq + (int64_t)e * rw;
+            const uint64_t cur = q0;
+            q0 = q1; q1 = q2; q2 = q3;
+            q3 = row_word(e + 4);
+            g0 = g1; g1 = g2; g2 = g3;
+            g3 = ends(e + 4);
             int nsp = 0;
             for (int wb = 0; wb < rw; wb += W) {
                 const int w = wb + ln;
-                const uint64_t word = w < rw ? row[w] : 0ull;
+                const uint64_t word = ring ? cur : (w < rw ? row[w] : 0ull);
                 uint64_t m = wp.ballot(word != 0ull);
                 while (m) {
                     const int src = ctz64(m);
