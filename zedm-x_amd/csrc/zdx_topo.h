@@ -627,18 +627,57 @@ ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double 
                 E2[2 * ne2] = prevv; E2[2 * ne2 + 1] = b; ++ne2;
             }
         }
-        // ---- extract_loops
-        for (int v = 0; v < nv; ++v) deg[v] = 0;
-        for (int e = 0; e < ne2; ++e) { deg[E2[2 * e]] += 1; deg[E2[2 * e + 1]] += 1; }
-        int acc = 0;
-        for (int v = 0; v < nv; ++v) { aoff[v] = acc; afill[v] = acc; acc += deg[v]; }
-        for (int e = 0; e < ne2; ++e) {
-            const int a = E2[2 * e], b = E2[2 * e + 1];
-            const int pa = afill[a]++; adjn[pa] = b; adje[pa] = e;
-            const int pb = afill[b]++; adjn[pb] = a; adje[pb] = e;
+        // ---- extract_loops: degrees, adjacency offsets and the adjacency of
+        // each vertex in edge order (the reference's append order), lane-parallel
+        wp.sync();
+        for (int v = ln; v < nv; v += W) deg[v] = 0;
+        wp.sync();
+        for (int e = ln; e < ne2; e += W) {
+            wp.atomic_add(&deg[E2[2 * e]], 1);
+            wp.atomic_add(&deg[E2[2 * e + 1]], 1);
         }
-        for (int e = 0; e < ne2; ++e) used[e] = 0;
-        for (int v = 0; v < nv; ++v) sord[v] = v;
+        wp.sync();
+        {
+            int acc = 0;
+            for (int vb = 0; vb < nv; vb += W) {
+                const int v = vb + ln;
+                const int c = v < nv ? deg[v] : 0;
+                int tot;
+                const int off = wp.scan_excl(c, tot);
+                if (v < nv) { aoff[v] = acc + off; afill[v] = acc + off; }
+                acc += tot;
+            }
+        }
+        wp.sync();
+        // slots 2e (end a) and 2e + 1 (end b) in order: per chunk of W slots,
+        // each slot's rank among the chunk's earlier slots of the same vertex
+        // (ballots over the bits of the vertex id), then the lowest such lane
+        // advances the vertex's cursor
+        for (int kb = 0; kb < 2 * ne2; kb += W) {
+            const int k = kb + ln;
+            const bool act = k < 2 * ne2;
+            const int e = k >> 1;
+            const int v = act ? E2[k] : 0;
+            const int o = act ? E2[k ^ 1] : 0;
+            uint64_t same = wp.ballot(act);
+            for (int bit = 0; (1 << bit) < nv; ++bit) {
+                const bool on = ((v >> bit) & 1) != 0;
+                const uint64_t bm = wp.ballot(act && on);
+                same &= on ? bm : ~bm;
+            }
+            const uint64_t below = same & ((1ull << ln) - 1ull);
+            if (act) {
+                const int pos = afill[v] + (int)__builtin_popcountll(below);
+                adjn[pos] = o;
+                adje[pos] = e;
+            }
+            wp.sync();
+            if (act && below == 0ull) afill[v] += (int)__builtin_popcountll(same);
+            wp.sync();
+        }
+        for (int e = ln; e < ne2; e += W) used[e] = 0;
+        for (int v = ln; v < nv; v += W) sord[v] = v;
+        wp.sync();
         StartOrder so; so.deg = SArr<int>{deg, 1};
         ref_sort(SArr<int>{sord, 1}, nv, so);
         int nloops_here = 0;
