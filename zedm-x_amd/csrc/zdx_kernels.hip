@@ -1589,8 +1589,11 @@ __global__ void __launch_bounds__(64) k_tp_walk_b(DPairs P, DMeshes M, DBodies B
 
 // Pass C: T-junction hit words of every component of an item; each wave of
 // the workgroup covers 64 vertices of one edge row per ballot.
+// lds / lds_bytes (optional): a component's vertices, cells and edges are
+// staged there when they fit (the walks' scratch is free between walk B and
+// walk D), so the ne x nv tests read LDS instead of global memory.
 __device__ inline void tp_pass_c_item(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int it,
-                                      int ncomp, long long to) {
+                                      int ncomp, long long to, char* lds = nullptr, int lds_bytes = 0) {
     if (ncomp <= 0 || to < 0) return;
     const int p = P.topo_list[it];
     const int64_t sb = P.seg_off[p];
@@ -1602,6 +1605,15 @@ __device__ inline void tp_pass_c_item(const DPairs& P, const DMeshes& M, const D
         const double* V = T.V + 6 * sb + 3 * (int64_t)cr[TPC_VB];
         const int* vc = T.vcell + 6 * sb + 3 * (int64_t)cr[TPC_VB];
         const int* E = T.E + 2 * sb + 2 * (int64_t)cr[TPC_EB];
+        if (lds && 24 * nv + 12 * nv + 8 * ne <= lds_bytes) {
+            double* lv = (double*)lds;
+            int* lc = (int*)(lds + 24 * nv);
+            int* le = lc + 3 * nv;
+            for (int k = threadIdx.x; k < 3 * nv; k += blockDim.x) { lv[k] = V[k]; lc[k] = vc[k]; }
+            for (int k = threadIdx.x; k < 2 * ne; k += blockDim.x) le[k] = E[k];
+            __syncthreads();
+            V = lv; vc = lc; E = le;
+        }
         uint64_t* w = T.tjw + to + cr[TPC_TW];
         const int rl = 64 * tp_row_words(nv);
         const int items = ne * rl;
@@ -1612,6 +1624,7 @@ __device__ inline void tp_pass_c_item(const DPairs& P, const DMeshes& M, const D
             const uint64_t m = __ballot(hit);
             if (lane_id() == 0 && t < items) w[t >> 6] = m;
         }
+        __syncthreads();   // the staged component is replaced by the next one
     }
 }
 __global__ void __launch_bounds__(256) k_tp_pass_c(DPairs P, DMeshes M, DBodies B, TopoBufs T) {
@@ -1714,7 +1727,8 @@ __global__ void __launch_bounds__(64) k_tp_walk_bcd_sim(DPairs P, DMeshes M, DBo
         const unsigned long long chain_t0 = wall_clock64();
 #endif
         const WalkB r = tp_walk_b_item(P, M, B, T, sp.split, lds_bytes, s_walk, wp, it);
-        tp_pass_c_item(P, M, B, T, it, r.ncomp, r.to);
+        __syncthreads();
+        tp_pass_c_item(P, M, B, T, it, r.ncomp, r.to, s_walk, lds_bytes);
         __syncthreads();
         tp_walk_d_sim_item(P, M, B, mat, H, T, sp, nprev, lds_bytes, s_walk, wp, it, r.ncomp, r.to);
 #ifdef ZDX_WALK_PROFILE
