@@ -656,6 +656,7 @@ __global__ void __launch_bounds__(256) k_need_lists(int32_t n, const double* sta
 __device__ unsigned long long g_walk_prof[16];
 __device__ long long g_topo_pairs[4 * 8192];    // per topology item: S, pass A ticks, walk D ticks
 __device__ long long g_walk_pairs[12 * 8192];   // walk B per pair: S, R, scratch in LDS, ticks, phases 0-7
+__device__ long long g_walkd_pairs[4 * 8192];   // walk D per pair: T-junctions, adjacency, start sort, loops
 __device__ long long g_np_pairs[8 * 16384];   // narrow per pair (queue item): nA, nB, hits, 4 phase ticks, p
 #define NP_START() unsigned long long np_t0 = wall_clock64(); unsigned np_ph[4] = {0, 0, 0, 0}
 #define NP_MARK(k)                                                      \
@@ -1687,7 +1688,10 @@ __device__ inline void tp_walk_d_sim_item(const DPairs& P, const DMeshes& M, con
         o = tp_walk_d(wp, T, S, ncomp, P.seg_off[p], to, topo_tol(T, M, mA), M.area_eps[mA], sub(b.pos, a.pos), s_walk,
                       lds_bytes, sink);
 #ifdef ZDX_WALK_PROFILE
-    if (lane_id() == 0 && it < 8192) g_topo_pairs[4 * it + 2] = (long long)(wall_clock64() - wp.t_begin);
+    if (lane_id() == 0 && it < 8192) {
+        g_topo_pairs[4 * it + 2] = (long long)(wall_clock64() - wp.t_begin);
+        for (int k = 0; k < 4; ++k) g_walkd_pairs[4 * it + k] = wp.ph[k];
+    }
 #endif
     if (lane_id() == 0) {
         T.ncontrib[p] = o.loops < T.lmax ? o.loops : T.lmax;
@@ -2031,6 +2035,9 @@ extern "C" int zdx_narrow_pairs(long long* out, int n) {
 }
 extern "C" int zdx_walk_pairs(long long* out, int n) {
     return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_walk_pairs), 12 * (size_t)n * sizeof(long long)) == hipSuccess ? 0 : 1;
+}
+extern "C" int zdx_walkd_pairs(long long* out, int n) {
+    return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_walkd_pairs), 4 * (size_t)n * sizeof(long long)) == hipSuccess ? 0 : 1;
 }
 extern "C" int zdx_walk_profile(unsigned long long* out) {
     if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_walk_prof), 16 * sizeof(unsigned long long)) != hipSuccess) return 1;

@@ -627,6 +627,7 @@ ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double 
                 E2[2 * ne2] = prevv; E2[2 * ne2 + 1] = b; ++ne2;
             }
         }
+        wp.mark(0);
         // ---- extract_loops: degrees, adjacency offsets and the adjacency of
         // each vertex in edge order (the reference's append order), lane-parallel
         wp.sync();
@@ -678,8 +679,10 @@ ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double 
         for (int e = ln; e < ne2; e += W) used[e] = 0;
         for (int v = ln; v < nv; v += W) sord[v] = v;
         wp.sync();
+        wp.mark(1);
         StartOrder so; so.deg = SArr<int>{deg, 1};
         ref_sort(SArr<int>{sord, 1}, nv, so);
+        wp.mark(2);
         int nloops_here = 0;
         for (int si = 0; si < nv; ++si) {
             const int sv = sord[si];
@@ -743,6 +746,7 @@ ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double 
         }
         out.loops_raw += nloops_here;
         wp.sync();
+        wp.mark(3);
     }
     return out;
 }
