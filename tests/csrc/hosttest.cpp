@@ -21,6 +21,17 @@ void ht_start_order(const int* deg, int n, int* order) {
     for (int i = 0; i < n; ++i) order[i] = o[i];
 }
 
+// the same through (key, vertex) elements, as the device walk sorts them
+void ht_start_order_keyed(const int* deg, int n, int* order) {
+    std::vector<int> k(n), o(n);
+    for (int i = 0; i < n; ++i) { o[i] = i; k[i] = start_key(deg[i]); }
+    KeyIdArr ka;
+    ka.k = k.data();
+    ka.v = o.data();
+    ref_sort(ka, n, ByKey{});
+    for (int i = 0; i < n; ++i) order[i] = o[i];
+}
+
 // generic: sort int keys with a plain < comparator, returning the permutation
 // of (key, original index) records compared by key only
 struct KI { int k; int i; };

@@ -27,6 +27,7 @@ def H(tmp_path_factory):
     ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
     dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
     L.ht_start_order.argtypes = [ip, C.c_int, ip]
+    L.ht_start_order_keyed.argtypes = [ip, C.c_int, ip]
     L.ht_sort_keys.argtypes = [ip, C.c_int, ip]
     L.ht_tri_tri.argtypes = [dp, dp, dp]
     L.ht_pair.argtypes = [dp, C.c_int, dp, C.c_int, dp, dp, dp, dp, C.c_double, C.c_double, C.c_int,
@@ -59,6 +60,9 @@ def test_introsort_matches_std_sort(H):
         L.orc_loop_start_order(deg, n, a)
         H.ht_start_order(deg, n, b)
         assert np.array_equal(a, b), (n, deg[:20])
+        c = np.zeros(n, dtype=np.int32)
+        H.ht_start_order_keyed(deg, n, c)
+        assert np.array_equal(a, c), (n, deg[:20])
         stable = np.array(sorted(range(n), key=lambda i: (0 if deg[i] == 2 else 1, -deg[i])), dtype=np.int32)
         n_unstable += int(not np.array_equal(a, stable))
     assert n_unstable > 5     # the cases do exercise the unstable permutation

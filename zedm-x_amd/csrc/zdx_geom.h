@@ -404,6 +404,20 @@ struct CompBySizeDesc {
     SArr<int> size;
     ZDX_HD bool operator()(int a, int b) const { return size.get(a) > size.get(b); }
 };
+// The same order on (key, vertex) elements: key = INT_MIN for degree 2, else
+// -degree, so key a < key b exactly when StartOrder puts a before b; the sort
+// then moves the key with its vertex (the same permutation, one load per
+// element instead of a vertex and then its degree).
+struct KeyId { int k, v; };
+struct KeyIdArr {
+    using value_type = KeyId;
+    int* k;
+    int* v;
+    ZDX_HD KeyId get(int i) const { KeyId r; r.k = k[i]; r.v = v[i]; return r; }
+    ZDX_HD void set(int i, const KeyId& x) { k[i] = x.k; v[i] = x.v; }
+};
+struct ByKey { ZDX_HD bool operator()(const KeyId& a, const KeyId& b) const { return a.k < b.k; } };
+ZDX_HD inline int start_key(int deg) { return deg == 2 ? (-2147483647 - 1) : -deg; }
 // extract_loops start-vertex comparator main.cpp:977-982
 struct StartOrder {
     SArr<int> deg;

@@ -676,12 +676,17 @@ ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double 
             if (act && below == 0ull) afill[v] += (int)__builtin_popcountll(same);
             wp.sync();
         }
+        // start order: the sort runs on (key, vertex) with the keys in afill
+        // (its cursors are spent); same permutation as sorting by StartOrder
         for (int e = ln; e < ne2; e += W) used[e] = 0;
-        for (int v = ln; v < nv; v += W) sord[v] = v;
+        for (int v = ln; v < nv; v += W) {
+            sord[v] = v;
+            afill[v] = start_key(deg[v]);
+        }
         wp.sync();
         wp.mark(1);
-        StartOrder so; so.deg = SArr<int>{deg, 1};
-        ref_sort(SArr<int>{sord, 1}, nv, so);
+        KeyIdArr ka; ka.k = afill; ka.v = sord;
+        ref_sort(ka, nv, ByKey{});
         wp.mark(2);
         int nloops_here = 0;
         for (int si = 0; si < nv; ++si) {
