@@ -16,7 +16,7 @@ checkpoint of the post-warmup state; the two trajectories must be bitwise
 equal.  value = contact pairs/s of the batched step, where a contact pair is a
 broad-phase candidate pair evaluated by the narrow phase.
 
-Prints one JSON line.  --gpus N with N > 1 exits: this build measures one GPU.
+Prints one JSON line.  --gpus N with N > 1 exits: the step runs on one GPU.
 """
 from __future__ import annotations
 
@@ -301,8 +301,8 @@ def main():
         summarize_profiles(*args.summarize_profiles, command=args.profile_command)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU: the step runs on a single MI355X and no "
-                                   "collective or multi-GPU path exists; run with --gpus 1"}))
+        print(json.dumps({"error": "the step runs on a single MI355X: there is no collective or multi-GPU "
+                                   "path; run with --gpus 1"}))
         sys.exit(2)
 
     import zdx
