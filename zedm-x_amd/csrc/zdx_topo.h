@@ -689,7 +689,24 @@ ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double 
         ref_sort(ka, nv, ByKey{});
         wp.mark(2);
         int nloops_here = 0;
+        // start vertices in order, skipping those whose edges are all used;
+        // the next one with an unused edge is found W vertices at a time
         for (int si = 0; si < nv; ++si) {
+            {
+                int found = nv;
+                for (int b0 = si; b0 < nv; b0 += W) {
+                    const int s2 = b0 + ln;
+                    bool has = false;
+                    if (s2 < nv) {
+                        const int v2 = sord[s2];
+                        for (int m = aoff[v2]; m < aoff[v2] + deg[v2] && !has; ++m) has = used[adje[m]] == 0;
+                    }
+                    const uint64_t bm = wp.ballot(has);
+                    if (bm) { found = b0 + ctz64(bm); break; }
+                }
+                if (found >= nv) break;
+                si = found;
+            }
             const int sv = sord[si];
             for (int k = aoff[sv]; k < aoff[sv] + deg[sv]; ++k) {
                 const int ei = adje[k];
