@@ -8,8 +8,8 @@
  * polyline.py (polyline.py:707-834).  Each entry point below names the
  * reference code it replaces.  Conventions:
  *   - plain C types only; every function returns a zdx_status;
- *   - the caller owns every host buffer; a context owns device memory and one
- *     HIP stream; calls on one context must not overlap (distinct contexts may
+ *   - the caller owns every host buffer; a context owns device memory and its
+ *     HIP streams; calls on one context must not overlap (distinct contexts may
  *     be used from different threads); calls return when results are on the host;
  *   - quaternions are (w, x, y, z); vectors are 3 doubles; all values FP64;
  *   - a ZDX_CAPACITY return fills the required sizes so the caller can retry.
@@ -37,7 +37,7 @@ typedef struct zdx_ctx zdx_ctx;
 typedef struct zdx_sim zdx_sim;
 
 /* ---------------------------------------------------------------- context */
-/* Opens device `device` (HIP ordinal) and creates the context's stream. */
+/* Opens device `device` (HIP ordinal) and creates the context's streams. */
 zdx_status zdx_create(zdx_ctx** out, int32_t device);
 /* Destroy every simulation created on the context first. */
 void zdx_destroy(zdx_ctx* ctx);
