@@ -185,6 +185,20 @@ def test_loops_writer_bytes(W, FX, tmp_path):
         assert p.read_bytes() == open(os.path.join(GOLD, c["file"]), "rb").read(), c["file"]
 
 
+def test_zdx_pair_arguments(tmp_path):
+    """argparse's behaviour for the command line: usage and exit status 2
+    on missing or unknown arguments, exit status 1 on unreadable STL files
+    (no GPU is touched before the meshes are read)."""
+    if not os.path.exists(PAIR):
+        pytest.skip("zdx-pair is not built (make -C zedm-x_amd)")
+    for args in ([], ["a.stl"], ["a.stl", "b.stl", "--bogus"], ["a.stl", "b.stl", "--tA", "1", "2"]):
+        r = subprocess.run([PAIR] + args, cwd=str(tmp_path), capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2 and r.stderr.startswith("usage: zdx-pair stlA stlB"), args
+    r = subprocess.run([PAIR, "missing_a.stl", "missing_b.stl"], cwd=str(tmp_path), capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "failed to read" in r.stderr
+
+
 # ------------------------------------------------------------------ GPU
 def _run_pair(cwd, args):
     r = subprocess.run([PAIR] + args, cwd=str(cwd), capture_output=True, text=True, timeout=120)
