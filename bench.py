@@ -8,13 +8,18 @@ log-uniform 0.5-2 mm, density 2500, young 1e7) in a 0.2 m box bounded by a
 
 A step is one full time step (transform, cell-binned broad phase, narrow phase,
 topology, forces with tangential history, walls, ordered force fold,
-integrator).  W warmup steps run first (untimed: the agitated bed starts
-colliding), then K timed steps in batched mode (persistent work queue,
-hipGraph-captured step, autotuned narrow-phase tiles), then the same K steps
-again in plain mode (one kernel per stage, host-sized launches) from a
-checkpoint of the post-warmup state; the two trajectories must be bitwise
-equal.  value = contact pairs/s of the batched step, where a contact pair is a
-broad-phase candidate pair evaluated by the narrow phase.
+integrator).  300 untimed steps settle the bed first, whatever --warmup
+says (the lattice-seeded bed reaches its colliding state of about 12k
+candidate pairs per step within that), the autotuner picks the narrow-phase
+shape on the settled state, W more untimed warmup steps follow, then K timed
+steps in batched mode (persistent work queue, hipGraph-captured step), then
+the same K steps again in plain mode (one kernel per stage, host-sized
+launches) from a checkpoint of the post-warmup state; the two trajectories
+must be bitwise equal.  value = contact pairs/s of the batched step, where a
+contact pair is a broad-phase candidate pair evaluated by the narrow phase.
+The line also carries steps/s, the per-step counters, the steps run again
+after a buffer growth and the graph captures inside the timed window, the
+narrow phase's roofline and the single-thread reference CPU baseline.
 
 Prints one JSON line.  --gpus N with N > 1 exits: the step runs on one GPU.
 """
@@ -67,37 +72,74 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(wl, state, sample_target, timeout_s):
-    """Times the reference program (zdem_cpu, compiled from the reference's own
-    source into oracle/_ref/) for one step on a sub-box of the post-warmup bed,
-    single-threaded.  Returns None when that binary is not present."""
+def _candidate_pairs(P, rad):
+    """Candidate pairs (a < b) of positions P with radii rad under the
+    reference's exact sphere test (src/main.cpp:2054-2059, dpos = P[b] - P[a],
+    rejected when dist2 > (ra + rb)^2), from a k-d tree superset."""
+    from scipy.spatial import cKDTree
+
+    pr = np.array(sorted(cKDTree(P).query_pairs(2.0 * float(rad.max()) * (1 + 1e-9) + 1e-12)), dtype=np.int64)
+    if len(pr) == 0:
+        return pr.reshape(0, 2)
+    dp = P[pr[:, 1]] - P[pr[:, 0]]
+    d2 = dp[:, 0] * dp[:, 0] + dp[:, 1] * dp[:, 1] + dp[:, 2] * dp[:, 2]
+    rs = rad[pr[:, 0]] + rad[pr[:, 1]]
+    return pr[~(d2 > rs * rs)]
+
+
+def pick_cpu_sample(wl, state, min_pairs, max_particles):
+    """Particles of the densest part of the bed: the k nearest (Chebyshev) to
+    the coarse-grid cell holding the most candidate-pair midpoints, k doubled
+    from 2000 until the sample holds min_pairs candidate pairs or reaches
+    max_particles.  Returns (sorted indices, candidate pairs of the sample)."""
+    pos = state[:, :3]
+    rad = np.array([wl.scale[i] * _tpl_radius(wl, wl.tpl[i]) for i in range(wl.n)])
+    allp = _candidate_pairs(pos, rad)
+    if len(allp) == 0:
+        raise RuntimeError("the bed has no candidate pairs: nothing for the reference to evaluate")
+    mid = 0.5 * (pos[allp[:, 0]] + pos[allp[:, 1]])
+    lo, hi = pos.min(axis=0), pos.max(axis=0)
+    g = 10
+    cell = np.clip(((mid - lo) / np.maximum(hi - lo, 1e-12) * g).astype(np.int64), 0, g - 1)
+    key = (cell[:, 0] * g + cell[:, 1]) * g + cell[:, 2]
+    best = np.bincount(key, minlength=g ** 3).argmax()
+    centre = mid[key == best].mean(axis=0)
+    d = np.max(np.abs(pos - centre), axis=1)
+    order = np.argsort(d, kind="stable")
+    k = 2000
+    while True:
+        idx = np.sort(order[:min(k, wl.n)])
+        pairs = _candidate_pairs(pos[idx], rad[idx])
+        if len(pairs) >= min_pairs or k >= max_particles or k >= wl.n:
+            return idx, pairs
+        k *= 2
+
+
+def cpu_baseline(wl, state, min_pairs, max_particles, timeout_s):
+    """Times the reference program (zdem_cpu, compiled from the reference's
+    own source into oracle/_ref/) for one step on the densest part of the
+    settled bed, single-threaded.  value = the sample's candidate pairs
+    divided by the step's contact time as the program reports it (tri_ms +
+    rebuild_ms: narrow phase, topology and forces; its O(N^2) pair loop, the
+    walls and its output are not counted).  The program's VTK / state
+    output is pointed at a directory that does not exist, so it costs
+    nothing (the reference reports the failed open and goes on).  Returns
+    None when the program is not built; raises when the sample has no
+    candidate pair or the program prints no step line."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from zdx import workloads as W
 
     if not os.path.exists(O.REF_BIN):
         return None
-    pos = state[:, :3]
-    c = np.median(pos, axis=0)
-    d = np.max(np.abs(pos - c), axis=1)
-    idx = np.argsort(d, kind="stable")[:sample_target]
-    idx = np.sort(idx)
+    idx, pairs = pick_cpu_sample(wl, state, min_pairs, max_particles)
+    n_pairs = len(pairs)
+    if n_pairs == 0:
+        raise RuntimeError(f"cpu baseline sample of {len(idx)} particles has no candidate pair")
     tmp = tempfile.mkdtemp(prefix="zdx_cpu_")
     try:
-        wl_sample = wl
-        cfg = W.write_reference_config(wl_sample, idx, tmp, steps=1, output_interval=1, state=state,
-                                       prefix="sample")
-        # broad-phase candidate pairs of the sample (the same sphere test)
-        rad = np.array([wl.scale[i] * _tpl_radius(wl, wl.tpl[i]) for i in idx])
-        P = pos[idx]
-        n_pairs = 0
-        from scipy.spatial import cKDTree
-
-        tree = cKDTree(P)
-        for a, b in tree.query_pairs(2.0 * float(rad.max()) + 1e-9):
-            dp = P[b] - P[a]
-            if not (dp @ dp > (rad[a] + rad[b]) ** 2):
-                n_pairs += 1
+        cfg = W.write_reference_config(wl, idx, tmp, steps=1, output_interval=1, state=state,
+                                       prefix="sample", vtk_prefix=os.path.join("no_such_dir", "sample"))
         env = dict(os.environ)
         env["OMP_NUM_THREADS"] = "1"
         cmd = [O.REF_BIN, "--config", cfg]
@@ -109,14 +151,17 @@ def cpu_baseline(wl, state, sample_target, timeout_s):
         m = re.search(r"step=0 contacts=(\d+) step_ms=(\d+) tri_ms=(\d+) rebuild_ms=(\d+) output_ms=(\d+) "
                       r"tri_checks=(\d+)", out.stdout)
         if not m:
-            return dict(error="reference run produced no step line", stderr=out.stderr[-400:])
+            raise RuntimeError("reference run produced no step line: " + out.stderr[-400:])
         contact_ms = int(m.group(3)) + int(m.group(4))
-        return dict(value=n_pairs / max(contact_ms, 1) * 1000.0, unit="contact pairs/s", cores=1, kind="reference",
-                    sample=f"{len(idx)} particles around the bed centre after warmup, {n_pairs} candidate pairs, "
-                           f"1 step of zdem_cpu (reference src/main.cpp, g++ -O2), narrow phase + topology + "
-                           f"forces {contact_ms} ms (tri_ms + rebuild_ms), tri_checks {m.group(6)}, "
-                           f"wall {wall:.1f} s incl. VTK output",
-                    cpu=cpu_model(), ms_per_pair=contact_ms / max(n_pairs, 1))
+        if contact_ms <= 0:
+            raise RuntimeError(f"reference contact time of {n_pairs} pairs below its 1 ms resolution")
+        return dict(value=n_pairs / contact_ms * 1000.0, unit="contact pairs/s", cores=1, kind="reference",
+                    sample=f"{len(idx)} particles of the densest part of the settled bed, {n_pairs} candidate "
+                           f"pairs, 1 step of zdem_cpu (reference src/main.cpp, g++ -O2, one thread pinned to "
+                           f"core 0): narrow phase + topology + forces {contact_ms} ms (its tri_ms + rebuild_ms), "
+                           f"tri_checks {m.group(6)}, active contacts {m.group(1)}, step {m.group(2)} ms incl. "
+                           f"its O(N^2) pair loop, process wall {wall:.1f} s incl. set-up",
+                    cpu=cpu_model(), ms_per_pair=contact_ms / n_pairs)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
 
@@ -278,18 +323,28 @@ def run_steps(sim, k, mode):
     return time.perf_counter() - t0, tot
 
 
+# untimed steps that settle the bed before anything is timed or tuned, whatever
+# --warmup asks: the lattice-seeded bed starts with no overlapping bounding
+# spheres and reaches its colliding state (about 12k candidate pairs and 1.5k
+# active contacts per step) within about 250 steps
+SETTLE_STEPS = 300
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20,
+                    help="untimed steps after the settle and the autotune (at least 3 run)")
+    ap.add_argument("--settle", type=int, default=SETTLE_STEPS, help=argparse.SUPPRESS)
     ap.add_argument("--particles", type=int, default=100_000)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--tiles", default="", help="narrow-phase tiles 'block*100+items,groups_per_cu' (no autotune)")
+    ap.add_argument("--tiles", default="", help="narrow-phase shape 'threads*100+tests,groups_per_cu' (no autotune)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=4000)
-    ap.add_argument("--cpu-timeout", type=float, default=120.0)
+    ap.add_argument("--cpu-pairs", type=int, default=4000, help="candidate pairs wanted in the CPU sample")
+    ap.add_argument("--cpu-max-particles", type=int, default=32000)
+    ap.add_argument("--cpu-timeout", type=float, default=240.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device copy of known size (for scaling hardware byte counters)")
     ap.add_argument("--summarize-profiles", nargs=3, metavar=("STATS_DIR", "FETCH_DIR", "WRITE_DIR"),
@@ -317,18 +372,24 @@ def main():
     sim, n_meshes = W.build_sim(ctx, wl)
     log(f"setup {time.time() - t0:.1f}s: {wl.n} particles, {n_meshes} meshes")
 
-    # warmup (untimed): half, autotune the narrow-phase tiles, the rest
+    # settle (untimed, fixed length), then the autotune on the settled
+    # state, then the warmup (the first step after the autotune captures the
+    # graphs again)
     tiles = None
     if args.tiles:
         tiles = [int(x) for x in args.tiles.split(",")]
         sim.set_tiles(tiles[0], tiles[1])
-    half = args.warmup // 2
-    sim.step(half, zdx.MODE_BATCHED)
+    sim.step(args.settle, zdx.MODE_BATCHED)
+    st_settle = sim.stage_times()
+    log(f"settle {args.settle} steps done {time.time() - t0:.1f}s, retries {st_settle['retries']}, "
+        f"graph captures {st_settle['graph_captures']}")
     if not args.tiles and not args.no_autotune:
-        tiles = sim.autotune()
-        log(f"autotune: block*100+items={tiles[0]} groups/CU={tiles[1]}")
-    sim.step(args.warmup - half, zdx.MODE_BATCHED)
-    log(f"warmup done {time.time() - t0:.1f}s")
+        tiles = list(sim.autotune())
+        log(f"autotune: threads*100+tests={tiles[0]} groups/CU={tiles[1]}")
+    warm = max(args.warmup, 3)
+    sim.step(warm, zdx.MODE_BATCHED)
+    sim.stage_times()
+    log(f"warmup {warm} steps done {time.time() - t0:.1f}s")
     tmpdir = tempfile.mkdtemp(prefix="zdx_bench_")
     ck = os.path.join(tmpdir, "warm.ckpt")
     sim.checkpoint(ck)
@@ -344,12 +405,14 @@ def main():
     plain = None
     if not args.no_plain:
         sim.restore(ck)
+        sim.step(1, zdx.MODE_PLAIN)     # untimed: the plain sequence's first launches
+        sim.restore(ck)
         sim.stage_times()
         tp, cp = run_steps(sim, args.steps, zdx.MODE_PLAIN)
         st_p = sim.stage_times()
         state_p = sim.state()
         plain = dict(seconds=tp, counters=cp, stages=st_p, bitwise_equal=bool(np.array_equal(state_p, state_b)))
-        log(f"plain {tp:.3f}s equal={plain['bitwise_equal']}")
+        log(f"plain {tp:.3f}s equal={plain['bitwise_equal']} retries {st_p['retries']}")
 
     # roofline of the dominant kernel of the batched step, from the live events
     stage_ms = {k: st_b[k] for k in zdx.Sim.STAGES}
@@ -372,16 +435,26 @@ def main():
     traffic_file = os.path.join(ROOT, "profiles", "narrow_traffic.json")
     if os.path.exists(traffic_file):
         try:
-            roof["traffic"] = json.load(open(traffic_file)).get("bytes_per_launch")
+            tr = json.load(open(traffic_file))
+            roof["traffic"] = tr.get("bytes_per_launch")
+            roof["traffic_source"] = (f"profiles/narrow_traffic.json: rocprofv3 FETCH_SIZE + WRITE_SIZE of the "
+                                      f"narrow-phase kernels on this workload after the same settle; there "
+                                      f"{tr.get('algorithmic_bytes_per_launch', 0) / 1e6:.0f} MB algorithmic")
         except (OSError, ValueError):
             pass
 
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        cpu = cpu_baseline(wl, state_warm, args.cpu_sample, args.cpu_timeout)
+        try:
+            cpu = cpu_baseline(wl, state_warm, args.cpu_pairs, args.cpu_max_particles, args.cpu_timeout)
+        except (RuntimeError, subprocess.TimeoutExpired) as e:
+            log(f"cpu baseline FAILED: {e}")
+            cpu = dict(value=None, unit="contact pairs/s", cores=1, kind="reference", error=str(e)[:300],
+                       sample="failed")
     if cpu is None:
-        cpu = dict(value=None, unit="contact pairs/s", cores=1, kind="reference", sample="not run")
+        cpu = dict(value=None, unit="contact pairs/s", cores=1, kind="reference",
+                   sample="not run (the reference program is not built)")
 
     pairs_b = cb["pairs"]
     res = {
@@ -397,14 +470,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "granular bed, 100k polyhedra (8 templates, 60-1280 triangles), 0.2 m box, "
-                               "12-triangle box wall, gravity -9.81, dt 1e-5, agitated non-overlapping seeding",
+        "config": {"workload": f"granular bed, 100k polyhedra (8 templates, 60-1280 triangles), 0.2 m box, "
+                               f"12-triangle box wall, gravity -9.81, dt 1e-5, agitated non-overlapping seeding; "
+                               f"timed after {args.settle} untimed settle steps, the autotune on the settled "
+                               f"state and {warm} warmup steps",
                    "particles": wl.n, "meshes": n_meshes, "mode": "batched (persistent queue + hipGraph)",
-                   "tiles": tiles},
+                   "tiles": tiles, "settle_steps": args.settle},
         "steps_per_s": args.steps / tb,
         "contact_pairs_per_step": pairs_b / args.steps,
         "active_contacts_per_step": cb["contacts"] / args.steps,
         "counters_per_step": per_step,
+        "timed_retries": st_b["retries"],
+        "timed_graph_captures": st_b["graph_captures"],
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -413,6 +490,7 @@ def main():
                         "steps_per_s": args.steps / plain["seconds"],
                         "ms_per_step": plain["seconds"] / args.steps * 1e3,
                         "stage_ms": {k: plain["stages"][k] for k in zdx.Sim.STAGES},
+                        "timed_retries": plain["stages"]["retries"],
                         "bitwise_equal_to_batched": plain["bitwise_equal"]}
         res["batched_over_plain"] = (pairs_b / tb) / (plain["counters"]["pairs"] / plain["seconds"])
     shutil.rmtree(tmpdir, ignore_errors=True)

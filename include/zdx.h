@@ -193,10 +193,12 @@ zdx_status zdx_sim_restore(zdx_sim* sim, const char* path);
  * mean ms per step of transform, broad phase, narrow phase, topology+forces,
  * walls, fold+integrate+counters; out[6]: steps; out[7..7+ZDX_N_COUNTERS):
  * counter sums (ZDX_CTR_SEG_MAX: the maximum); out[7+ZDX_N_COUNTERS]: mean ms
- * of the narrow-phase kernels alone (without the work ordering).  The walls
- * run on a second stream alongside the narrow phase and the topology, so the
- * stages can add up to more than the step.  Writes min(n, 8 + ZDX_N_COUNTERS)
- * values. */
+ * of the narrow-phase kernels alone (without the work ordering);
+ * out[8+ZDX_N_COUNTERS]: steps run again after a buffer growth;
+ * out[9+ZDX_N_COUNTERS]: captures of the batched step's graphs (a growth or a
+ * parameter change invalidates them).  The walls run on a second stream
+ * alongside the narrow phase and the topology, so the stages can add up to
+ * more than the step.  Writes min(n, 10 + ZDX_N_COUNTERS) values. */
 zdx_status zdx_sim_stage_times(zdx_sim* sim, double* out, int32_t n);
 /* Particle-particle force model of the time step.  ZDX_FORCE_ZDEM (default):
  * the reference's force, src/main.cpp:2220-2287.  ZDX_FORCE_MINIMAL: the
@@ -207,10 +209,16 @@ zdx_status zdx_sim_stage_times(zdx_sim* sim, double* out, int32_t n);
  * config file: "force_model minimal" and "minimal_k <k>". */
 enum { ZDX_FORCE_ZDEM = 0, ZDX_FORCE_MINIMAL = 1 };
 zdx_status zdx_sim_set_force_model(zdx_sim* sim, int32_t model, double k);
-/* Narrow-phase launch shape; 0 keeps the current value.  The autotuner
- * (zdx_sim_autotune) sets them from timed trial steps. */
-zdx_status zdx_sim_set_tiles(zdx_sim* sim, int32_t np_block, int32_t np_groups_per_cu);
-zdx_status zdx_sim_autotune(zdx_sim* sim, int32_t* np_block, int32_t* np_groups_per_cu);
+/* Narrow-phase launch shape (no reference counterpart).  np_shape encodes
+ * the workgroup size and the triangle-pair tests per thread of one sweep
+ * round as threads * 100 + tests: one of 6404, 6408, 12804, 12808, 25601,
+ * 25602, 25604, 25608, 51202, 51204 (the compiled kernels); any other value
+ * returns ZDX_BAD_ARG.  np_groups_per_cu (1..16) sets the persistent grid to
+ * that many workgroups per CU.  0 keeps the current value of either.  The
+ * autotuner (zdx_sim_autotune) times every compiled shape on the current
+ * step's pair set and keeps the fastest; it reports the same encoding. */
+zdx_status zdx_sim_set_tiles(zdx_sim* sim, int32_t np_shape, int32_t np_groups_per_cu);
+zdx_status zdx_sim_autotune(zdx_sim* sim, int32_t* np_shape, int32_t* np_groups_per_cu);
 
 #ifdef __cplusplus
 }

@@ -371,6 +371,9 @@ struct zdx_sim {
     double acc_ms[7] = {0, 0, 0, 0, 0, 0, 0};   // 6 stages, then the narrow-phase kernels alone
     double acc_ctr[ZDX_N_COUNTERS] = {0};
     int64_t acc_steps = 0;
+    // steps run again after a capacity growth, and graph captures, since the
+    // last zdx_sim_stage_times call
+    int64_t acc_retries = 0, acc_captures = 0;
     // graphs (one per parity)
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
@@ -867,6 +870,7 @@ void step_once(zdx_sim* s, bool batched) {
             enqueue_step(s, par, true);
         } else if (batched) {
             if (!s->graphs_valid) {
+                s->acc_captures += 1;
                 for (int k = 0; k < 2; ++k) {
                     HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
                     enqueue_step(s, k, true);
@@ -902,6 +906,7 @@ void step_once(zdx_sim* s, bool batched) {
         }
         // the step wrote only the other state buffer and this step's pair set:
         // grow and run it again from the same state
+        s->acc_retries += 1;
         if (!grow_for(s, status, par))
             throw ApiError{ZDX_CAPACITY, "simulation step capacity exceeded (status " + std::to_string(status) + ")"};
     }
@@ -1491,16 +1496,20 @@ zdx_status zdx_sim_restore(zdx_sim* s, const char* path) {
 
 zdx_status zdx_sim_stage_times(zdx_sim* s, double* out, int32_t n) {
     if (!s || !out || n < 0) return ZDX_BAD_ARG;
-    double v[8 + ZDX_N_COUNTERS];
+    double v[10 + ZDX_N_COUNTERS];
     const double steps = (double)s->acc_steps;
     for (int k = 0; k < 6; ++k) v[k] = steps > 0 ? s->acc_ms[k] / steps : 0.0;
     v[6] = steps;
     for (int k = 0; k < ZDX_N_COUNTERS; ++k) v[7 + k] = s->acc_ctr[k];
     v[7 + ZDX_N_COUNTERS] = steps > 0 ? s->acc_ms[6] / steps : 0.0;
-    for (int k = 0; k < n && k < 8 + ZDX_N_COUNTERS; ++k) out[k] = v[k];
+    v[8 + ZDX_N_COUNTERS] = (double)s->acc_retries;
+    v[9 + ZDX_N_COUNTERS] = (double)s->acc_captures;
+    for (int k = 0; k < n && k < 10 + ZDX_N_COUNTERS; ++k) out[k] = v[k];
     for (int k = 0; k < 7; ++k) s->acc_ms[k] = 0.0;
     for (int k = 0; k < ZDX_N_COUNTERS; ++k) s->acc_ctr[k] = 0.0;
     s->acc_steps = 0;
+    s->acc_retries = 0;
+    s->acc_captures = 0;
     return ZDX_OK;
 }
 
@@ -1516,17 +1525,25 @@ zdx_status zdx_sim_set_force_model(zdx_sim* s, int32_t model, double k) {
     return ZDX_OK;
 }
 
-zdx_status zdx_sim_set_tiles(zdx_sim* s, int32_t block, int32_t groups_per_cu) {
+zdx_status zdx_sim_set_tiles(zdx_sim* s, int32_t shape, int32_t groups_per_cu) {
     if (!s) return ZDX_BAD_ARG;
-    if (block > 0) {
-        s->ncfg.block = block / 100 > 0 ? block / 100 : block;
-        s->ncfg.items = block % 100 > 0 ? block % 100 : 4;
-        if (block < 100) { s->ncfg.block = block; s->ncfg.items = 4; }
+    // the narrow-phase kernel shapes that are compiled (launch_narrow)
+    static const int32_t shapes[] = {6404, 6408, 12804, 12808, 25601, 25602, 25604, 25608, 51202, 51204};
+    if (shape > 0 && std::find(std::begin(shapes), std::end(shapes), shape) == std::end(shapes)) {
+        s->ctx->err = "narrow-phase shape " + std::to_string(shape) +
+                      " is not threads*100+tests per thread of a compiled kernel (6404 6408 12804 12808 25601 "
+                      "25602 25604 25608 51202 51204)";
+        return ZDX_BAD_ARG;
     }
-    if (groups_per_cu > 0) {
-        if (groups_per_cu > 16) return ZDX_BAD_ARG;
-        s->np_groups_per_cu = groups_per_cu;
+    if (groups_per_cu < 0 || groups_per_cu > 16) {
+        s->ctx->err = "workgroups per CU must be 1..16 (0 keeps the current value)";
+        return ZDX_BAD_ARG;
     }
+    if (shape > 0) {
+        s->ncfg.block = shape / 100;
+        s->ncfg.items = shape % 100;
+    }
+    if (groups_per_cu > 0) s->np_groups_per_cu = groups_per_cu;
     s->drop_graphs();
     return ZDX_OK;
 }

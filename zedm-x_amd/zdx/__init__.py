@@ -302,12 +302,14 @@ class Sim:
     def stage_times(self) -> dict:
         """Mean per-stage device ms and counter sums over the steps since the
         previous call (then reset)."""
-        v = np.zeros(8 + len(COUNTERS))
+        v = np.zeros(10 + len(COUNTERS))
         self.ctx._check(self.ctx._L.zdx_sim_stage_times(self.h, v.ctypes.data, len(v)))
         out = {k: float(v[i]) for i, k in enumerate(self.STAGES)}
         out["steps"] = int(v[6])
         out["counters"] = {k: int(v[7 + i]) for i, k in enumerate(COUNTERS)}
         out["narrow_kernels"] = float(v[7 + len(COUNTERS)])
+        out["retries"] = int(v[8 + len(COUNTERS)])
+        out["graph_captures"] = int(v[9 + len(COUNTERS)])
         return out
 
     def set_tiles(self, block: int = 0, groups_per_cu: int = 0) -> None:

@@ -177,9 +177,10 @@ def build_sim(ctx, wl: Workload, center: bool = True):
 
 
 def write_reference_config(wl: Workload, idx, out_dir: str, steps: int, output_interval: int, state=None,
-                           prefix: str = "zdx") -> str:
+                           prefix: str = "zdx", vtk_prefix: str | None = None) -> str:
     """Writes the particles `idx` (with `state` rows pos quat vel omega if
-    given) as a zdem_cpu config with the template STLs next to it."""
+    given) as a zdem_cpu config `<prefix>.txt` with the template STLs next to
+    it; the program's output files are named by vtk_prefix (default: prefix)."""
     os.makedirs(out_dir, exist_ok=True)
     for t, name in zip(wl.templates, wl.template_names):
         p = os.path.join(out_dir, name + ".stl")
@@ -202,7 +203,7 @@ def write_reference_config(wl: Workload, idx, out_dir: str, steps: int, output_i
                           poisson=float(wl.material[i, 2]), mu=float(wl.material[i, 3]),
                           restitution=float(wl.material[i, 4])))
     g = dict(steps=steps, dt=wl.dt, split_contacts=int(wl.split), contact_debug=0, gravity=list(wl.gravity),
-             center_mesh=1, vtk_prefix=prefix, output_dir=os.path.join(out_dir, "out"), output_interval=output_interval)
+             center_mesh=1, vtk_prefix=prefix if vtk_prefix is None else vtk_prefix, output_dir=os.path.join(out_dir, "out"), output_interval=output_interval)
     path = os.path.join(out_dir, prefix + ".txt")
     with open(path, "w") as f:
         f.write(synth.format_config(g, parts, walls))
