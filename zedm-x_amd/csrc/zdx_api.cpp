@@ -825,7 +825,6 @@ bool grow_for(zdx_sim* s, int32_t status, int par) {
         s->drop_graphs();
         grown = true;
     }
-    if (status & ST_NEIGH) return false;
     if (status & ST_GRID) return false;
     return grown;
 }

@@ -206,7 +206,7 @@ enum : int32_t {
     ST_STAGE = 2,        // a pair produced more segments than the staging capacity
     ST_TOPO = 4,         // topology scratch or loop capacity exceeded
     ST_PAIRS = 8,        // broad-phase pair capacity exceeded
-    ST_NEIGH = 16,       // more neighbours than the per-particle candidate capacity
+    ST_NEIGH = 16,       // (no longer raised: the broad phase keeps every partner of a particle)
     ST_WALL = 32,        // wall section capacity exceeded
     ST_GRID = 64,        // broad-phase grid could not be built (non-finite positions)
     ST_HITS = 128        // hit arena full

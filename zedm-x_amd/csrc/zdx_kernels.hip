@@ -10,15 +10,21 @@
 //                     sort of particles by cell, per-particle neighbour
 //                     enumeration, scans, ordered pair list (i < j) and the
 //                     per-particle adjacency used by the ordered force fold
-//   k_narrow          persistent workgroups, one pair at a time: cull each
-//                     mesh's triangles against the other mesh's AABB, then a
-//                     flattened, order-preserving sweep of the culled product
-//                     with float (outward-rounded) AABB prefilter, exact FP64
-//                     AABB re-test and tri-tri on the hits; ballot/prefix-sum
-//                     compaction keeps the reference's (ta, tb) segment order
-//   k_topo_*          one thread per pair over strided scratch: components,
-//                     snap, T-split, loops, Sn/Gn, force (zdx_geom.h)
-//   k_walls           one thread per (particle, wall)
+//   narrow phase      k_narrow: persistent workgroups, one pair at a time:
+//                     cull each mesh's triangles against the other mesh's
+//                     box, then a flattened, order-preserving sweep of the
+//                     culled product with the float (outward-rounded) AABB
+//                     test, hits written in (ta, tb) order; k_narrow_exact:
+//                     one thread per hit, the exact FP64 AABB re-test and
+//                     tri-tri; k_narrow_compact: one wave per pair, ballot /
+//                     prefix-sum compaction in the reference's (ta, tb) order
+//   topology          k_tp_alloc (one thread per pair) and k_tp_pass_a / pass
+//                     C (workgroups over flat S^2 test ranges) for the
+//                     quadratic tests; walks B and D (components, snap
+//                     union-find, T-junction chains, loops, Sn/Gn/xc0, force)
+//                     one wave per pair with scratch in LDS (zdx_topo.h)
+//   k_walls           one wave per particle near a wall: plane sections of
+//                     every candidate wall triangle, clip, area, force
 //   k_fold_integrate  one thread per particle: ordered, atomic-free fold of
 //                     the contact records in the reference's summation order,
 //                     then the rigid-body integrator
@@ -326,7 +332,6 @@ void scan_i32x2(const int32_t* in0, const int32_t* in1, int32_t n, int32_t* tmp,
 }
 
 // ------------------------------------------------------------ broad phase
-constexpr int NEIGH_CAP = 128;
 
 // Bounds of the particle positions and the largest radius: BP_PARTS
 // workgroups reduce a strided share each, then one workgroup combines them.
@@ -492,7 +497,6 @@ __global__ void __launch_bounds__(256) k_bp_count(BroadPhaseBufs bp) {
         all++;
         if (j > i) up++;
     });
-    if (all > NEIGH_CAP) { atomicOr(bp.status, (int)ST_NEIGH); all = NEIGH_CAP; up = up > NEIGH_CAP ? NEIGH_CAP : up; }
     bp.cnt_up[i] = up;
     bp.cnt_all[i] = all;
 }
@@ -547,15 +551,23 @@ __global__ void __launch_bounds__(128) k_bp_fill(BroadPhaseBufs bp) {
             if (q < n) emit(q, r[q], u);
         return;
     }
-    int js[NEIGH_CAP];
+    // more partners than the count pass kept: scan again straight into the
+    // particle's adjacency slice (exactly cnt entries: the same test on the
+    // same doubles), sort it in place (Shell sort, Ciura's gaps), then emit
+    int32_t* js = bp.adj_j + ao;
     int n = 0;
-    for_neighbours(bp, G, i, si, [&](int j) { if (n < NEIGH_CAP) js[n++] = j; });
-    for (int a = 1; a < n; ++a) {
-        int v = js[a], b = a - 1;
-        while (b >= 0 && js[b] > v) { js[b + 1] = js[b]; --b; }
-        js[b + 1] = v;
+    for_neighbours(bp, G, i, si, [&](int j) { if (n < cnt) js[n] = j; ++n; });
+    constexpr int gaps[8] = {701, 301, 132, 57, 23, 10, 4, 1};
+    for (int gi = 0; gi < 8; ++gi) {
+        const int gp = gaps[gi];
+        for (int a = gp; a < cnt; ++a) {
+            const int v = js[a];
+            int b = a;
+            while (b >= gp && js[b - gp] > v) { js[b] = js[b - gp]; b -= gp; }
+            js[b] = v;
+        }
     }
-    for (int k = 0; k < n; ++k) emit(k, js[k], u);
+    for (int k = 0; k < cnt; ++k) emit(k, js[k], u);
 }
 
 __global__ void __launch_bounds__(256) k_bp_adjfix(BroadPhaseBufs bp) {
@@ -565,7 +577,7 @@ __global__ void __launch_bounds__(256) k_bp_adjfix(BroadPhaseBufs bp) {
         // later stages see no pairs (the fold checks adj_off[n] against the
         // capacity) and the host grows the buffers and runs the step again
         const int np = bp.pair_off[bp.n], na = bp.adj_off[bp.n];
-        const bool over = np > bp.pair_cap || na > bp.adj_cap || (*bp.status & (int)ST_NEIGH);
+        const bool over = np > bp.pair_cap || na > bp.adj_cap;
         if (over) atomicOr(bp.status, (int)ST_PAIRS);
         *bp.n_pairs = over ? 0 : np;
         *bp.n_adj = over ? 0 : na;
@@ -1352,10 +1364,12 @@ __global__ void __launch_bounds__(256) k_tp_alloc(DPairs P, TopoBufs T) {
         long long so = (long long)atomicAdd(T.top + 1, (unsigned long long)nw);
         bool ok = ao + nadj <= T.adj_cap && so + nw <= T.snap_cap;
         if (nu > 1) {
+            // every slot of the reservation that lies below units_cap names this
+            // item, also when the reservation straddles units_cap: pass A reads
+            // every slot below units_cap, and this item's offsets are -1 then
             const long long uo = (long long)atomicAdd(T.top + 4, (unsigned long long)(nu - 1));
             if (uo + nu - 1 > T.units_cap) ok = false;
-            else
-                for (int b = 1; b < nu; ++b) ((int2*)T.units)[uo + b - 1] = make_int2(it, b);
+            for (int b = 1; b < nu && uo + b - 1 < T.units_cap; ++b) ((int2*)T.units)[uo + b - 1] = make_int2(it, b);
         }
         T.adj_off[it] = ok ? ao : -1;
         T.snap_off[it] = ok ? so : -1;
@@ -1395,11 +1409,15 @@ __global__ void __launch_bounds__(NT_MAX) k_tp_pass_a(DPairs P, DMeshes M, DBodi
             const int2 e = ((const int2*)T.units)[u - n];
             it = e.x;
             b = e.y;
+            // k_tp_alloc wrote every slot below units_cap this step; this only
+            // guards against a slot it did not write
+            if (it < 0 || it >= *P.topo_n || b < 1) continue;
         }
         const long long ao = T.adj_off[it], so = T.snap_off[it];
         if (ao < 0) continue;
         const int p = P.topo_list[it];
         const int S = P.nseg[p];
+        if (b >= tp_pa_units(S)) continue;
         const int64_t sb = P.seg_off[p];
         const double h = topo_tol(T, M, B.mesh[P.pi[p]]);
         const double* gseg = P.arena + 6 * sb;
