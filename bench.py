@@ -166,6 +166,76 @@ def cpu_baseline(wl, state, min_pairs, max_particles, timeout_s):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def uneven_block(ctx, reps: int, cpu_sample: int, log):
+    """The uneven-pair batch (zdx.workloads.uneven_pairs: 4096 independent
+    pairs, icospheres of 80-5120 triangles and concave stars of 1280) in one
+    zdx_contact_pairs call per repetition, batched and plain.  Reports pairs/s
+    on the device time of the call (transform + narrow phase + topology, HIP
+    events) and end to end (host packing and PCIe transfers included), and a
+    single-thread CPU baseline: the oracle (the FP64 restatement of the
+    reference's per-pair body, src/main.cpp:2061-2219) on a seeded sample of
+    pairs, extrapolated to the batch by the reference's tri_checks counter."""
+    import zdx
+    from zdx import workloads as W
+
+    meshes, names, pairs = W.uneven_pairs(4096)
+    ids = [ctx.register_mesh(m) for m in meshes]
+    P = dict(mesh_a=[ids[p[0]] for p in pairs], mesh_b=[ids[p[1]] for p in pairs],
+             pos_a=np.array([p[2] for p in pairs]), quat_a=np.array([p[3] for p in pairs]),
+             pos_b=np.array([p[4] for p in pairs]), quat_b=np.array([p[5] for p in pairs]))
+    out = {"workload": "4096 independent pairs: 2048 icosphere pairs (levels 1-4 per body, 80-5120 triangles), "
+                       "2048 concave star pairs (1280 triangles, radius x (1 + 0.3 noise)), random quaternions, "
+                       "overlap 1-10 % of the smaller radius along the line of centres; split contacts, FP64 "
+                       "per-loop Sn, Gn, n, xc0, area, Fn, force and torques"}
+    ref = None
+    for mode, name in ((zdx.MODE_BATCHED, "batched"), (zdx.MODE_PLAIN, "plain")):
+        ctx.contact_pairs(**P, split=True, mode=mode, k=1.0)      # untimed: buffers grow here
+        dev, wall = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            r = ctx.contact_pairs(**P, split=True, mode=mode, k=1.0)
+            wall.append(time.perf_counter() - t0)
+            t = ctx.contact_times()
+            assert t["attempts"] == 1
+            dev.append((t["transform"] + t["narrow"] + t["topology"], t["narrow"], t["topology"]))
+        d = np.median(np.array(dev), axis=0)
+        if ref is None:
+            ref = r
+        else:
+            out["plain_equals_batched"] = bool(all(np.array_equal(ref[k], r[k]) for k in
+                                                   ("tri_checks", "n_segments", "loop_offset", "loops")))
+        out[name] = dict(pairs_per_s=4096 / (d[0] * 1e-3), device_ms=d[0], narrow_ms=d[1], topology_ms=d[2],
+                         pairs_per_s_end_to_end=4096 / float(np.median(wall)),
+                         end_to_end_ms=float(np.median(wall)) * 1e3)
+        log(f"uneven {name}: device {d[0]:.2f} ms (narrow {d[1]:.2f}, topology {d[2]:.2f}), "
+            f"end to end {np.median(wall) * 1e3:.1f} ms")
+    out["batched_over_plain"] = out["plain"]["device_ms"] / out["batched"]["device_ms"]
+    tc = ref["tri_checks"].astype(np.float64)
+    out["tri_checks"] = float(tc.sum())
+    out["segments"] = int(ref["n_segments"].sum())
+    out["loops"] = int(len(ref["loops"]))
+    if cpu_sample > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        built = [O.build_mesh(m) for m in meshes]
+        sel = np.random.default_rng(11).choice(4096, cpu_sample, replace=False)
+        t0 = time.perf_counter()
+        for p in sel:
+            a, b, pa, qa, pb, qb = pairs[p]
+            O.pair_contact(built[a][0], built[b][0], pa, qa, pb, qb, built[a][1][5], built[a][1][6], split=True,
+                           k=1.0)
+        ts = time.perf_counter() - t0
+        est = ts * tc.sum() / tc[sel].sum()
+        out["cpu_baseline"] = dict(value=4096 / est, unit="contact pairs/s", cores=1, kind="port",
+                                   sample=f"{cpu_sample} seeded pairs of the batch through the oracle (FP64 "
+                                          f"restatement of src/main.cpp:2061-2219, g++ -O2, one thread) in "
+                                          f"{ts:.2f} s, extrapolated to the 4096 pairs by tri_checks "
+                                          f"({tc[sel].sum():.3g} of {tc.sum():.3g}): {est:.1f} s per batch")
+        log(f"uneven cpu: {cpu_sample} pairs {ts:.2f} s, batch estimate {est:.1f} s")
+    return out
+
+
 _TPL_RAD = {}
 
 
@@ -239,11 +309,13 @@ def _is_narrow(name):
     return any(k in name for k in NARROW_KERNELS)
 
 
-def summarize_profiles(stats_dir, fetch_dir, write_dir, command="", timed_stats=20, timed_pmc=5):
+def summarize_profiles(stats_dir, fetch_dir, write_dir, command="", timed_stats=20, timed_pmc=5, bench_json=None):
     """profiles/ summaries: per-kernel durations of the bench
     (rocprofv3 --kernel-trace --stats) and the HBM traffic of the narrow-phase
     kernel from FETCH_SIZE / WRITE_SIZE (KiB per dispatch), each scaled by what
-    the same counter reads for a plain copy of COPY_BYTES."""
+    the same counter reads for a plain copy of COPY_BYTES.  bench_json: the
+    JSON line the profiled bench printed, whose algorithmic bytes per launch
+    are recorded beside the counted ones."""
     import csv
     import glob
 
@@ -295,6 +367,10 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir, command="", timed_stats=
                    write_bytes_per_launch=write_b, bytes_per_launch=fetch_b + write_b,
                    fetch_scale=f_scale, write_scale=w_scale, copy_bytes=COPY_BYTES,
                    copy_fetch_kib=max(cf), copy_write_kib=max(cw))
+        if bench_json:
+            line = [x for x in open(bench_json).read().splitlines() if x.startswith("{")][-1]
+            rf = json.loads(line)["roofline"]
+            res.update(algorithmic_bytes_per_launch=rf["bytes_per_launch"], profiled_avg_launch_ms=rf["avg_launch_ms"])
         json.dump(res, open(os.path.join(prof, "narrow_traffic.json"), "w"), indent=1)
         md = ["# HBM traffic of the narrow phase -- granular bed, 100k polyhedra", "",
               f"Commands: `{command}` with `--pmc FETCH_SIZE` and `--pmc WRITE_SIZE` (separate runs)", "",
@@ -306,7 +382,10 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir, command="", timed_stats=
               f"- WRITE_SIZE per step: {nw_kib:.0f} KiB x scale {w_scale:.3f} = {write_b / 1e6:.1f} MB",
               f"- scale = bytes of a plain {COPY_BYTES >> 20} MiB device copy / what the counter reads for it "
               f"(fetch {max(cf):.0f} KiB, write {max(cw):.0f} KiB)",
-              f"- traffic per launch: {(fetch_b + write_b) / 1e6:.1f} MB"]
+              f"- traffic per launch: {(fetch_b + write_b) / 1e6:.1f} MB",
+              *([f"- algorithmic bytes per launch of the profiled run (its counters): "
+                 f"{res['algorithmic_bytes_per_launch'] / 1e6:.1f} MB"] if "algorithmic_bytes_per_launch" in res
+                else [])]
         open(os.path.join(prof, "narrow_traffic_granular_bed_100k.md"), "w").write("\n".join(md) + "\n")
     print(json.dumps(res))
 
@@ -342,6 +421,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tiles", default="", help="narrow-phase shape 'threads*100+tests,groups_per_cu' (no autotune)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the 4096-pair uneven batch block")
+    ap.add_argument("--uneven-reps", type=int, default=5)
+    ap.add_argument("--uneven-cpu-pairs", type=int, default=256)
     ap.add_argument("--cpu-pairs", type=int, default=4000, help="candidate pairs wanted in the CPU sample")
     ap.add_argument("--cpu-max-particles", type=int, default=32000)
     ap.add_argument("--cpu-timeout", type=float, default=240.0)
@@ -351,9 +433,10 @@ def main():
                     help="write profiles/ summaries from rocprofv3 --stats / --pmc FETCH_SIZE / WRITE_SIZE "
                          "output directories of this bench, then exit")
     ap.add_argument("--profile-command", default="", help="command line recorded in the summaries")
+    ap.add_argument("--profile-bench-json", default=None, help="JSON line printed by the profiled bench run")
     args = ap.parse_args()
     if args.summarize_profiles:
-        summarize_profiles(*args.summarize_profiles, command=args.profile_command)
+        summarize_profiles(*args.summarize_profiles, command=args.profile_command, bench_json=args.profile_bench_json)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "the step runs on a single MI355X: there is no collective or multi-GPU "
@@ -456,6 +539,10 @@ def main():
         cpu = dict(value=None, unit="contact pairs/s", cores=1, kind="reference",
                    sample="not run (the reference program is not built)")
 
+    uneven = None
+    if not args.no_uneven:
+        uneven = uneven_block(ctx, args.uneven_reps, 0 if args.no_cpu_baseline else args.uneven_cpu_pairs, log)
+
     pairs_b = cb["pairs"]
     res = {
         "metric": "contact pairs/sec + steps/sec @100k polyhedra, batched vs plain GPU",
@@ -493,6 +580,8 @@ def main():
                         "timed_retries": plain["stages"]["retries"],
                         "bitwise_equal_to_batched": plain["bitwise_equal"]}
         res["batched_over_plain"] = (pairs_b / tb) / (plain["counters"]["pairs"] / plain["seconds"])
+    if uneven:
+        res["uneven_pairs"] = uneven
     shutil.rmtree(tmpdir, ignore_errors=True)
     print(json.dumps(res))
 

@@ -113,6 +113,15 @@ typedef struct {
 zdx_status zdx_contact_pairs(zdx_ctx* ctx, const zdx_pair_in* pairs, int32_t n_pairs, const zdx_contact_opts* opts,
                              zdx_pair_out* out);
 
+/* Device time of the last zdx_contact_pairs call on this context (no
+ * reference counterpart), from HIP events on the context's stream: out[0]
+ * transform of the bodies, out[1] narrow phase (work ordering, cull, sweep,
+ * exact tests, compaction) of the attempt that fitted the buffers, out[2]
+ * topology and loop records, all in ms; out[3] narrow-phase attempts (1 when
+ * no buffer had to grow).  The host packing and the transfers are not
+ * included.  Writes min(n, 4) values. */
+zdx_status zdx_contact_times(zdx_ctx* ctx, double* out, int32_t n);
+
 /* ------------------------------------------------------------- simulation */
 /* The zdem_cpu time step (src/main.cpp:2034-2468): broad phase, narrow phase,
  * topology, particle-particle and particle-wall forces with tangential

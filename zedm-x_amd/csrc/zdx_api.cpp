@@ -167,6 +167,11 @@ struct zdx_ctx {
     // third stream: pass A of the many-segment pairs beside that of the others
     hipStream_t stream3 = nullptr;
     hipEvent_t ev_fork3 = nullptr, ev_join3 = nullptr;
+    // timing events of the per-pair call: transform, narrow phase (the
+    // attempt that fitted), topology -- start and end of each
+    hipEvent_t ev_api[6] = {};
+    double api_ms[3] = {0.0, 0.0, 0.0};
+    int32_t api_attempts = 0;
     std::string err;
     std::vector<MeshRec> meshes;
     bool mesh_dirty = true;
@@ -937,6 +942,7 @@ zdx_status zdx_create(zdx_ctx** out, int32_t device) {
         HIPC(hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking));
         HIPC(hipEventCreateWithFlags(&c->ev_fork3, hipEventDisableTiming));
         HIPC(hipEventCreateWithFlags(&c->ev_join3, hipEventDisableTiming));
+        for (auto& e : c->ev_api) HIPC(hipEventCreate(&e));
     });
     if (st != ZDX_OK) {
         delete c;
@@ -952,6 +958,8 @@ void zdx_destroy(zdx_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     if (c->stream3) (void)hipStreamSynchronize(c->stream3);
+    for (auto& e : c->ev_api)
+        if (e) (void)hipEventDestroy(e);
     if (c->ev_fork3) (void)hipEventDestroy(c->ev_fork3);
     if (c->ev_join3) (void)hipEventDestroy(c->ev_join3);
     if (c->stream3) (void)hipStreamDestroy(c->stream3);
@@ -1042,7 +1050,9 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
         B.n = nb; B.mesh = A.mesh.p; B.state = A.state.p; B.wv_off = A.wv_off.p; B.wt_off = A.wt_off.p;
         B.wv = A.wv.p; B.nondeg = A.nondeg.p; B.aabb = A.aabb.p;
         DMeshes M = c->dmeshes();
+        HIPC(hipEventRecord(c->ev_api[0], st));
         launch_transform(B, M, st);
+        HIPC(hipEventRecord(c->ev_api[1], st));
         A.pi.ensure(n); A.pj.ensure(n); A.npairs.ensure(1); A.status.ensure(1); A.order.ensure(n); A.items.ensure(n);
         launch_iota_pairs(A.pi.p, A.pj.p, n, st);
         HIPC(hipMemcpyAsync(A.npairs.p, &n, sizeof(int32_t), hipMemcpyHostToDevice, st));
@@ -1057,6 +1067,7 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
         A.list_f.ensure((size_t)groups * 12 * c->max_tris);
         const bool want_vtx = out->loop_vtx != nullptr;
         for (int attempt = 0; attempt < 10; ++attempt) {
+            c->api_attempts = attempt + 1;
             A.arena.ensure((size_t)A.hit_cap * 6);
             A.hits.ensure(6 * (size_t)A.hit_cap);
             A.hit_flag.ensure((size_t)A.hit_cap);
@@ -1078,9 +1089,11 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
             }
             launch_fill_i32(A.status.p, 0, 1, st);
             launch_fill_i32(A.topo_n.p, 0, 1, st);
+            HIPC(hipEventRecord(c->ev_api[2], st));
             if (batched) launch_lpt_order(P, M, B, A.order.p, A.bcnt.p, A.bcur.p, n, st);
             NarrowCfg nc{256, 4, groups, batched ? 1 : 0};
             launch_narrow(B, M, P, nc, A.list_i.p, A.list_f.p, A.stage_cap, c->max_tris, A.queue.p, st);
+            HIPC(hipEventRecord(c->ev_api[3], st));
             int32_t status = 0;
             HIPC(hipMemcpyAsync(&status, A.status.p, 4, hipMemcpyDeviceToHost, st));
             HIPC(hipStreamSynchronize(st));
@@ -1118,10 +1131,17 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
                 T.ej = A.ej.p; T.vj = A.vj.p; T.ljac = A.ljac.p;
             }
             T.rec = A.rec.p; T.vtx = want_vtx ? A.vtx.p : nullptr;
+            HIPC(hipEventRecord(c->ev_api[4], st));
             launch_topo_zero(P, T, nullptr, n, st);
             launch_topo_api(P, M, B, T, opts->split_contacts, st);
+            HIPC(hipEventRecord(c->ev_api[5], st));
             HIPC(hipMemcpyAsync(&status, A.status.p, 4, hipMemcpyDeviceToHost, st));
             HIPC(hipStreamSynchronize(st));
+            for (int k = 0; k < 3; ++k) {
+                float ms = 0.0f;
+                HIPC(hipEventElapsedTime(&ms, c->ev_api[2 * k], c->ev_api[2 * k + 1]));
+                c->api_ms[k] = ms;
+            }
             if (status & ST_TOPO) {
                 std::vector<int32_t> ts(n);
                 HIPC(hipMemcpy(ts.data(), A.tstatus.p, n * 4, hipMemcpyDeviceToHost));
@@ -1208,6 +1228,13 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
         }
         throw ApiError{ZDX_CAPACITY, "per-pair buffers did not fit after repeated growth"};
     });
+}
+
+zdx_status zdx_contact_times(zdx_ctx* c, double* out, int32_t n) {
+    if (!c || (!out && n > 0) || n < 0) return ZDX_BAD_ARG;
+    const double v[4] = {c->api_ms[0], c->api_ms[1], c->api_ms[2], (double)c->api_attempts};
+    for (int k = 0; k < n && k < 4; ++k) out[k] = v[k];
+    return ZDX_OK;
 }
 
 zdx_status zdx_sim_create(zdx_ctx* c, const zdx_sim_desc* d, zdx_sim** out) {

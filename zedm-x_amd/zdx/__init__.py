@@ -24,6 +24,7 @@ FORCE_ZDEM, FORCE_MINIMAL = 0, 1
 COUNTERS = ["contacts", "tri_checks", "segments", "comps", "loops", "pairs", "wall_contacts", "tri_tests", "tri_hits",
             "tri_scanned", "topo_pairs", "seg_max"]
 EXPORTS = ["zdx_create", "zdx_destroy", "zdx_last_error", "zdx_load_stl", "zdx_mesh_register", "zdx_contact_pairs",
+           "zdx_contact_times",
            "zdx_sim_load_config", "zdx_sim_create", "zdx_sim_destroy", "zdx_sim_num_particles", "zdx_sim_step",
            "zdx_sim_get_state", "zdx_sim_set_state", "zdx_sim_get_forces", "zdx_sim_write_vtk",
            "zdx_sim_write_state_txt", "zdx_sim_checkpoint", "zdx_sim_restore",
@@ -82,6 +83,7 @@ def lib():
         L.zdx_load_stl.argtypes = [C.c_char_p, vp, i32, C.POINTER(i32)]
         L.zdx_mesh_register.argtypes = [vp, vp, i32, i32, C.POINTER(i32), vp]
         L.zdx_contact_pairs.argtypes = [vp, vp, i32, C.POINTER(ContactOpts), C.POINTER(PairOut)]
+        L.zdx_contact_times.argtypes = [vp, vp, i32]
         L.zdx_sim_load_config.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(vp)]
         L.zdx_sim_create.argtypes = [vp, C.POINTER(SimDesc), C.POINTER(vp)]
         L.zdx_sim_destroy.argtypes = [vp]
@@ -213,6 +215,13 @@ class Context:
                 res["segments"] = seg[:int(out.n_seg_total)]
             return res
         raise ZdxError(CAPACITY, "contact_pairs output did not fit")
+
+    def contact_times(self) -> dict:
+        """Device ms of the last contact_pairs call (transform, narrow phase,
+        topology) and its narrow-phase attempts."""
+        v = np.zeros(4)
+        self._check(self._L.zdx_contact_times(self.h, v.ctypes.data, 4))
+        return dict(transform=float(v[0]), narrow=float(v[1]), topology=float(v[2]), attempts=int(v[3]))
 
 
 class Sim:

@@ -106,16 +106,48 @@ def granular_bed(n: int = 100_000, seed: int = 3, box: float = 0.2, r_min: float
                     dict(seed=seed, box=box, lattice=s, r_min=r_min, r_max=r_max, speed=speed))
 
 
+def _rotmat(q):
+    w, x, y, z = q / np.linalg.norm(q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def _ray_extent(tris, d):
+    """Distance from the origin to the farthest crossing of the ray t d (t > 0)
+    with the triangle soup `tris` (Moller-Trumbore, all triangles at once)."""
+    v0, e1, e2 = tris[:, 0], tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0]
+    h = np.cross(d, e2)
+    a = np.einsum("ij,ij->i", e1, h)
+    ok = np.abs(a) > 1e-300
+    f = np.where(ok, 1.0 / np.where(ok, a, 1.0), 0.0)
+    s = -v0
+    u = f * np.einsum("ij,ij->i", s, h)
+    q = np.cross(s, e1)
+    v = f * (q @ d)
+    t = f * np.einsum("ij,ij->i", e2, q)
+    hit = ok & (u >= 0.0) & (v >= 0.0) & (u + v <= 1.0) & (t > 0.0)
+    return float(t[hit].max())
+
+
 def uneven_pairs(n_pairs: int = 4096, seed: int = 2):
     """Independent pairs: half icosphere pairs (levels 1-4 each, 80-5120
     triangles), half concave star pairs (level-3 icosphere, 1280 triangles,
     radius x (1 + 0.3 noise)); random quaternions; overlap depth uniform 1-10 %
-    of the radius along a random direction.  Returns (meshes, names, pairs)
-    with pairs = list of (mesh a, mesh b, posA, quatA, posB, quatB)."""
+    of the smaller bounding radius along a random direction d.  The depth is
+    measured along the line of centres: A's surface is crossed by the ray
+    from its centroid along d, B's by the ray along -d, and the two crossings
+    overlap by the depth, so every pair interpenetrates, the stars included
+    (their surface lies well inside the bounding sphere in most directions,
+    and each is star-shaped about its centroid).  Positions refer to the
+    volume centroid, which is where mesh registration with centring puts the
+    body origin.  Returns (meshes, names, pairs) with pairs = list of
+    (mesh a, mesh b, posA, quatA, posB, quatB)."""
     names = ["ico1", "ico2", "ico3", "ico4", "starA", "starB"]
     meshes = [synth.icosphere(1), synth.icosphere(2), synth.icosphere(3), synth.icosphere(4),
               synth.star_mesh(seed * 100 + 1, 3), synth.star_mesh(seed * 100 + 2, 3)]
     rad = [_centred_radius(t) for t in meshes]
+    cen = [t - _centroid(t) for t in meshes]
     g = np.random.Generator(np.random.PCG64(seed))
     pairs = []
     for p in range(n_pairs):
@@ -129,7 +161,10 @@ def uneven_pairs(n_pairs: int = 4096, seed: int = 2):
         qa /= np.linalg.norm(qa)
         qb /= np.linalg.norm(qb)
         depth = g.uniform(0.01, 0.10) * min(rad[a], rad[b])
-        dist = rad[a] + rad[b] - depth
+        # the rays in the body frames: R^T d
+        ha = _ray_extent(cen[a], _rotmat(qa).T @ d)
+        hb = _ray_extent(cen[b], _rotmat(qb).T @ -d)
+        dist = ha + hb - depth
         pairs.append((a, b, np.zeros(3), qa, d * dist, qb))
     return meshes, names, pairs
 
