@@ -207,6 +207,10 @@ struct zdx_ctx {
         DBuf<double> rec, vtx;
         DBuf<int32_t> prov;            // sensitivity runs
         DBuf<double> ej, vj, ljac;
+        // packed output (device side of the copies to the caller)
+        DBuf<int32_t> pk_loop_off, pk_seg_off, pk_vtx_off;
+        DBuf<int64_t> pk_pvtx, pk_tot;
+        DBuf<double> pk_rec, pk_ljac, pk_vtx, pk_seg;
         int32_t stage_cap = 1024;      // most segments one pair may store
         int64_t hit_cap = 1 << 16;     // hit slots = segment slots
         int32_t lmax = 32;
@@ -1153,77 +1157,58 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
                 if (!lo && !vo && !grown) throw ApiError{ZDX_CAPACITY, "contact topology exceeded its edge capacity"};
                 continue;
             }
-            // gather to the host and pack
-            std::vector<int32_t> nl(n), ns(n), nc2(n), nr(n);
-            std::vector<int64_t> so(n), tc(n);
-            HIPC(hipMemcpy(nl.data(), A.ncontrib.p, n * 4, hipMemcpyDeviceToHost));
-            HIPC(hipMemcpy(ns.data(), A.nseg.p, n * 4, hipMemcpyDeviceToHost));
-            HIPC(hipMemcpy(nc2.data(), A.comps.p, n * 4, hipMemcpyDeviceToHost));
-            HIPC(hipMemcpy(nr.data(), A.loops_raw.p, n * 4, hipMemcpyDeviceToHost));
-            HIPC(hipMemcpy(so.data(), A.seg_off.p, n * 8, hipMemcpyDeviceToHost));
-            HIPC(hipMemcpy(tc.data(), A.tri_checks.p, n * 8, hipMemcpyDeviceToHost));
-            std::vector<double> rec((size_t)n * T.lmax * 28);
-            HIPC(hipMemcpy(rec.data(), A.rec.p, rec.size() * 8, hipMemcpyDeviceToHost));
-            std::vector<double> ljac;
-            if (sens && out->loop_jac) {
-                ljac.resize((size_t)n * T.lmax * 72);
-                HIPC(hipMemcpy(ljac.data(), A.ljac.p, ljac.size() * 8, hipMemcpyDeviceToHost));
-            }
-            int64_t nloops = 0, nseg = 0, nvtx = 0;
-            for (int p = 0; p < n; ++p) {
-                nloops += nl[p];
-                nseg += ns[p];
-                for (int l = 0; l < nl[p]; ++l) nvtx += (int64_t)rec[((size_t)p * T.lmax + l) * 28 + 24];
-            }
-            out->n_loops_total = nloops;
-            out->n_seg_total = nseg;
-            out->n_vtx_total = nvtx;
-            if (nloops > out->loop_cap || (out->segments && nseg > out->seg_cap) ||
-                (out->loop_vtx && nvtx > out->vtx_cap))
-                throw ApiError{ZDX_CAPACITY, "zdx_pair_out capacity too small (see n_*_total)"};
-            std::vector<double> vtx;
+            // pack on the device, then one copy per output array
+            const bool want_jac = sens && out->loop_jac;
+            const bool want_seg = out->segments != nullptr;
+            ApiPack K{};
+            K.n = n; K.lmax = T.lmax; K.vmax = T.vmax;
+            K.nloops = A.ncontrib.p; K.nseg = A.nseg.p; K.seg_off = A.seg_off.p;
+            K.rec = A.rec.p; K.ljac = want_jac ? A.ljac.p : nullptr; K.vtx = want_vtx ? A.vtx.p : nullptr;
+            K.arena = A.arena.p;
+            A.pk_loop_off.ensure((size_t)n + 1); A.pk_seg_off.ensure((size_t)n + 1);
+            A.pk_pvtx.ensure((size_t)n + 1); A.pk_tot.ensure(3);
+            K.loop_off = A.pk_loop_off.p; K.seg_out_off = A.pk_seg_off.p; K.pvtx_off = A.pk_pvtx.p;
+            K.totals = A.pk_tot.p;
+            K.loop_cap = std::max<int64_t>(0, std::min<int64_t>(out->loop_cap, (int64_t)n * T.lmax));
+            K.vtx_cap = want_vtx ? std::max<int64_t>(0, std::min<int64_t>(out->vtx_cap, (int64_t)n * T.vmax)) : 0;
+            K.seg_cap = want_seg ? std::max<int64_t>(0, std::min<int64_t>(out->seg_cap, A.hit_cap)) : 0;
+            if (out->loop_rec) { A.pk_rec.ensure((size_t)K.loop_cap * 28 + 1); K.rec_out = A.pk_rec.p; }
+            if (want_jac) { A.pk_ljac.ensure((size_t)K.loop_cap * 72 + 1); K.ljac_out = A.pk_ljac.p; }
             if (want_vtx) {
-                vtx.resize((size_t)n * T.vmax * 3);
-                HIPC(hipMemcpy(vtx.data(), A.vtx.p, vtx.size() * 8, hipMemcpyDeviceToHost));
+                A.pk_vtx.ensure((size_t)K.vtx_cap * 3 + 1); K.vtx_out = A.pk_vtx.p;
+                A.pk_vtx_off.ensure((size_t)K.loop_cap + 1); K.vtx_off_out = A.pk_vtx_off.p;
             }
-            std::vector<double> arena;
-            if (out->segments && nseg > 0) {
-                unsigned long long top = 0;
-                HIPC(hipMemcpy(&top, A.hit_top.p, 8, hipMemcpyDeviceToHost));
-                arena.resize((size_t)std::min<unsigned long long>(top, (unsigned long long)A.hit_cap) * 6);
-                HIPC(hipMemcpy(arena.data(), A.arena.p, arena.size() * 8, hipMemcpyDeviceToHost));
+            if (want_seg) { A.pk_seg.ensure((size_t)K.seg_cap * 6 + 1); K.seg_out = A.pk_seg.p; }
+            launch_api_pack(K, st);
+            int64_t tot[3] = {0, 0, 0};
+            HIPC(hipMemcpyAsync(tot, A.pk_tot.p, sizeof(tot), hipMemcpyDeviceToHost, st));
+            HIPC(hipStreamSynchronize(st));
+            out->n_loops_total = tot[0];
+            out->n_vtx_total = tot[1];
+            out->n_seg_total = tot[2];
+            if (tot[0] > out->loop_cap || (want_seg && tot[2] > out->seg_cap) || (want_vtx && tot[1] > out->vtx_cap))
+                throw ApiError{ZDX_CAPACITY, "zdx_pair_out capacity too small (see n_*_total)"};
+            if (tot[0] >= ((int64_t)1 << 31) || tot[1] >= ((int64_t)1 << 31) || tot[2] >= ((int64_t)1 << 31))
+                throw ApiError{ZDX_CAPACITY, "more than 2^31 loops, loop vertices or segments in one call"};
+            auto d2h = [&](void* dst, const void* src, size_t bytes) {
+                if (bytes) HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+            };
+            d2h(out->loop_offset, A.pk_loop_off.p, 4 * ((size_t)n + 1));
+            if (out->tri_checks) d2h(out->tri_checks, A.tri_checks.p, 8 * (size_t)n);
+            if (out->n_segments) d2h(out->n_segments, A.nseg.p, 4 * (size_t)n);
+            if (out->n_comps) d2h(out->n_comps, A.comps.p, 4 * (size_t)n);
+            if (out->n_loops_raw) d2h(out->n_loops_raw, A.loops_raw.p, 4 * (size_t)n);
+            if (out->loop_rec) d2h(out->loop_rec, A.pk_rec.p, 8 * 28 * (size_t)tot[0]);
+            if (want_jac) d2h(out->loop_jac, A.pk_ljac.p, 8 * 72 * (size_t)tot[0]);
+            if (want_vtx) {
+                d2h(out->vtx_offset, A.pk_vtx_off.p, 4 * ((size_t)tot[0] + 1));
+                d2h(out->loop_vtx, A.pk_vtx.p, 8 * 3 * (size_t)tot[1]);
             }
-            int64_t lo = 0, vo = 0, sg = 0;
-            for (int p = 0; p < n; ++p) {
-                out->loop_offset[p] = (int32_t)lo;
-                if (out->tri_checks) out->tri_checks[p] = tc[p];
-                if (out->n_segments) out->n_segments[p] = ns[p];
-                if (out->n_comps) out->n_comps[p] = nc2[p];
-                if (out->n_loops_raw) out->n_loops_raw[p] = nr[p];
-                int64_t pv = 0;
-                for (int l = 0; l < nl[p]; ++l) {
-                    const double* r = &rec[((size_t)p * T.lmax + l) * 28];
-                    if (out->loop_rec) std::memcpy(out->loop_rec + 28 * (lo + l), r, 28 * sizeof(double));
-                    if (!ljac.empty())
-                        std::memcpy(out->loop_jac + 72 * (lo + l), &ljac[((size_t)p * T.lmax + l) * 72], 72 * sizeof(double));
-                    int nv = (int)r[24];
-                    if (want_vtx) {
-                        out->vtx_offset[lo + l] = (int32_t)vo;
-                        std::memcpy(out->loop_vtx + 3 * vo, &vtx[((size_t)p * T.vmax + pv) * 3], 3 * nv * sizeof(double));
-                    }
-                    vo += nv;
-                    pv += nv;
-                }
-                lo += nl[p];
-                if (out->segments) {
-                    out->seg_offset[p] = (int32_t)sg;
-                    if (ns[p] > 0) std::memcpy(out->segments + 6 * sg, &arena[6 * (size_t)so[p]], 6 * ns[p] * sizeof(double));
-                    sg += ns[p];
-                }
+            if (want_seg) {
+                d2h(out->seg_offset, A.pk_seg_off.p, 4 * ((size_t)n + 1));
+                d2h(out->segments, A.pk_seg.p, 8 * 6 * (size_t)tot[2]);
             }
-            out->loop_offset[n] = (int32_t)lo;
-            if (want_vtx) out->vtx_offset[lo] = (int32_t)vo;
-            if (out->segments) out->seg_offset[n] = (int32_t)sg;
+            HIPC(hipStreamSynchronize(st));
             return;
         }
         throw ApiError{ZDX_CAPACITY, "per-pair buffers did not fit after repeated growth"};

@@ -214,4 +214,32 @@ void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, int64_t n, hipStream_t s);
 void launch_iota_pairs(int32_t* pi, int32_t* pj, int32_t n, hipStream_t s);
 
+// Output of the per-pair call packed on the device, so that each array
+// reaches the caller in one copy: per-pair loop, vertex and segment offsets
+// (exclusive scans), then the accepted loops' records (28 doubles), their
+// Jacobians (72, optional), their vertices (optional) and the segments
+// (optional) back to back in pair order.  Optional outputs are NULL.
+struct ApiPack {
+    int32_t n;                 // pairs
+    int32_t lmax, vmax;        // per-pair slots of rec / ljac and of vtx
+    const int32_t* nloops;     // accepted loops per pair
+    const int32_t* nseg;       // segments per pair
+    const int64_t* seg_off;    // the pair's first segment in the arena
+    const double* rec;         // n * lmax * 28
+    const double* ljac;        // n * lmax * 72
+    const double* vtx;         // n * vmax * 3
+    const double* arena;       // segments, 6 doubles each
+    int32_t* loop_off;         // n + 1
+    int32_t* seg_out_off;      // n + 1
+    int64_t* pvtx_off;         // n + 1: first vertex of each pair's loops
+    int64_t* totals;           // loops, vertices, segments
+    double* rec_out;           // loop_cap * 28
+    double* ljac_out;          // loop_cap * 72
+    double* vtx_out;           // vtx_cap * 3
+    int32_t* vtx_off_out;      // loop_cap + 1
+    double* seg_out;           // seg_cap * 6
+    int64_t loop_cap, vtx_cap, seg_cap;   // rows of the *_out arrays: nothing is written past them
+};
+void launch_api_pack(const ApiPack& k, hipStream_t s);
+
 }  // namespace zdx

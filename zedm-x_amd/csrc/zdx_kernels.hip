@@ -2040,6 +2040,92 @@ __global__ void k_iota_pairs(int32_t* pi, int32_t* pj, int32_t n) {
     if (p < n) { pi[p] = 2 * p; pj[p] = 2 * p + 1; }
 }
 
+
+// Offsets of the per-pair call's packed output: one workgroup scans the
+// pairs 1024 at a time (loops, loop vertices, segments of each pair) with a
+// carry between chunks.
+__global__ void __launch_bounds__(1024) k_api_scan(ApiPack k) {
+    __shared__ long long sh[3][1024];
+    __shared__ long long carry[3];
+    const int t = threadIdx.x;
+    if (t < 3) carry[t] = 0;
+    __syncthreads();
+    for (int base = 0; base < k.n; base += 1024) {
+        const int p = base + t;
+        long long v[3] = {0, 0, 0};
+        if (p < k.n) {
+            v[0] = k.nloops[p];
+            for (int l = 0; l < k.nloops[p]; ++l) v[1] += (long long)k.rec[((int64_t)p * k.lmax + l) * 28 + 24];
+            v[2] = k.nseg[p];
+        }
+        for (int c = 0; c < 3; ++c) sh[c][t] = v[c];
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            long long a[3];
+            for (int c = 0; c < 3; ++c) a[c] = t >= off ? sh[c][t - off] : 0;
+            __syncthreads();
+            for (int c = 0; c < 3; ++c) sh[c][t] += a[c];
+            __syncthreads();
+        }
+        if (p < k.n) {
+            k.loop_off[p] = (int32_t)(carry[0] + sh[0][t] - v[0]);
+            k.pvtx_off[p] = carry[1] + sh[1][t] - v[1];
+            k.seg_out_off[p] = (int32_t)(carry[2] + sh[2][t] - v[2]);
+        }
+        __syncthreads();
+        if (t < 3) carry[t] += sh[t][1023];
+        __syncthreads();
+    }
+    if (t == 0) {
+        k.loop_off[k.n] = (int32_t)carry[0];
+        k.pvtx_off[k.n] = carry[1];
+        k.seg_out_off[k.n] = (int32_t)carry[2];
+        k.totals[0] = carry[0];
+        k.totals[1] = carry[1];
+        k.totals[2] = carry[2];
+        if (k.vtx_off_out && carry[0] <= k.loop_cap) k.vtx_off_out[carry[0]] = (int32_t)carry[1];
+    }
+}
+
+// One wave per pair moves its rows to the packed positions; a pair whose
+// rows would pass an output's capacity writes nothing to that output (the
+// host then reports the totals and the call fails with ZDX_CAPACITY).
+__global__ void __launch_bounds__(256) k_api_pack(ApiPack k) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t p = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); p < k.n; p += nw) {
+        const int nl = k.nloops[p];
+        const int64_t lo = k.loop_off[p];
+        if (lo + nl <= k.loop_cap) {
+            if (k.rec_out)
+                for (int64_t e = lane; e < 28 * (int64_t)nl; e += 64)
+                    k.rec_out[28 * lo + e] = k.rec[(p * k.lmax) * 28 + e];
+            if (k.ljac_out)
+                for (int64_t e = lane; e < 72 * (int64_t)nl; e += 64)
+                    k.ljac_out[72 * lo + e] = k.ljac[(p * k.lmax) * 72 + e];
+        }
+        if (k.vtx_out) {
+            const int64_t vo = k.pvtx_off[p];
+            const int64_t nv = k.pvtx_off[p + 1] - vo;
+            if (lane == 0 && lo + nl <= k.loop_cap) {
+                int64_t acc = vo;
+                for (int l = 0; l < nl; ++l) {
+                    k.vtx_off_out[lo + l] = (int32_t)acc;
+                    acc += (int64_t)k.rec[(p * k.lmax + l) * 28 + 24];
+                }
+            }
+            if (vo + nv <= k.vtx_cap)
+                for (int64_t e = lane; e < 3 * nv; e += 64) k.vtx_out[3 * vo + e] = k.vtx[(p * k.vmax) * 3 + e];
+        }
+        if (k.seg_out) {
+            const int64_t so = k.seg_out_off[p];
+            const int ns = k.nseg[p];
+            if (so + ns <= k.seg_cap)
+                for (int64_t e = lane; e < 6 * (int64_t)ns; e += 64) k.seg_out[6 * so + e] = k.arena[6 * k.seg_off[p] + e];
+        }
+    }
+}
+
 inline int nblk(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
 }  // namespace
@@ -2261,6 +2347,12 @@ void launch_fill_u64(unsigned long long* p, unsigned long long v, int64_t n, hip
 void launch_iota_pairs(int32_t* pi, int32_t* pj, int32_t n, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_iota_pairs, dim3(nblk(n, 256)), dim3(256), 0, s, pi, pj, n);
+}
+
+void launch_api_pack(const ApiPack& k, hipStream_t s) {
+    if (k.n <= 0) return;
+    hipLaunchKernelGGL(k_api_scan, dim3(1), dim3(1024), 0, s, k);
+    hipLaunchKernelGGL(k_api_pack, dim3(std::min(nblk(k.n, 4), 4096)), dim3(256), 0, s, k);
 }
 
 }  // namespace zdx

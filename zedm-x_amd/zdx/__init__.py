@@ -43,6 +43,16 @@ class PairIn(C.Structure):
                 ("pos_b", C.c_double * 3), ("quat_b", C.c_double * 4)]
 
 
+# zdx_pair_in as a numpy record (same offsets and size as the C struct), so a
+# batch of pairs is filled with array assignments
+PAIR_IN_DTYPE = np.dtype({"names": ["mesh_a", "mesh_b", "pos_a", "quat_a", "pos_b", "quat_b"],
+                          "formats": [np.int32, np.int32, (np.float64, 3), (np.float64, 4), (np.float64, 3),
+                                      (np.float64, 4)],
+                          "offsets": [PairIn.mesh_a.offset, PairIn.mesh_b.offset, PairIn.pos_a.offset,
+                                      PairIn.quat_a.offset, PairIn.pos_b.offset, PairIn.quat_b.offset],
+                          "itemsize": C.sizeof(PairIn)})
+
+
 class ContactOpts(C.Structure):
     _fields_ = [("split_contacts", C.c_int32), ("mode", C.c_int32), ("k", C.c_double), ("sensitivities", C.c_int32),
                 ("reserved", C.c_int32), ("tol", C.c_double)]
@@ -172,13 +182,14 @@ class Context:
         optionally, loop vertices and narrow-phase segments.  tol > 0 replaces
         mesh A's snap / adjacency tolerance for every pair."""
         n = len(mesh_a)
-        arr = (PairIn * max(n, 1))()
-        pa, qa = np.asarray(pos_a, dtype=np.float64), np.asarray(quat_a, dtype=np.float64)
-        pb, qb = np.asarray(pos_b, dtype=np.float64), np.asarray(quat_b, dtype=np.float64)
-        for p in range(n):
-            arr[p].mesh_a = int(mesh_a[p]); arr[p].mesh_b = int(mesh_b[p])
-            arr[p].pos_a[:] = pa[p].tolist(); arr[p].quat_a[:] = qa[p].tolist()
-            arr[p].pos_b[:] = pb[p].tolist(); arr[p].quat_b[:] = qb[p].tolist()
+        arr = np.zeros(max(n, 1), dtype=PAIR_IN_DTYPE)
+        if n:
+            arr["mesh_a"][:n] = np.asarray(mesh_a, dtype=np.int32)
+            arr["mesh_b"][:n] = np.asarray(mesh_b, dtype=np.int32)
+            arr["pos_a"][:n] = np.asarray(pos_a, dtype=np.float64).reshape(n, 3)
+            arr["quat_a"][:n] = np.asarray(quat_a, dtype=np.float64).reshape(n, 4)
+            arr["pos_b"][:n] = np.asarray(pos_b, dtype=np.float64).reshape(n, 3)
+            arr["quat_b"][:n] = np.asarray(quat_b, dtype=np.float64).reshape(n, 4)
         opts = ContactOpts(int(split), int(mode), float(k), int(bool(sensitivities)), 0, float(tol))
         loop_cap, vtx_cap, seg_cap = max(64, 4 * n), max(1024, 64 * n), max(1024, 64 * n)
         for _ in range(4):
@@ -196,7 +207,7 @@ class Context:
             out = PairOut(tc.ctypes.data, ns.ctypes.data, nc.ctypes.data, nr.ctypes.data, lo.ctypes.data, _ptr(so),
                           rec.ctypes.data, loop_cap, _ptr(vo), _ptr(vtx), vtx_cap, _ptr(seg), seg_cap, 0, 0, 0,
                           _ptr(jac))
-            st = self._L.zdx_contact_pairs(self.h, C.cast(arr, C.c_void_p), n, C.byref(opts), C.byref(out))
+            st = self._L.zdx_contact_pairs(self.h, arr.ctypes.data, n, C.byref(opts), C.byref(out))
             if st == CAPACITY:
                 loop_cap = max(loop_cap, int(out.n_loops_total) + 1)
                 vtx_cap = max(vtx_cap, int(out.n_vtx_total) + 1)
