@@ -12,7 +12,11 @@
  *     HIP streams; calls on one context must not overlap (distinct contexts may
  *     be used from different threads); calls return when results are on the host;
  *   - quaternions are (w, x, y, z); vectors are 3 doubles; all values FP64;
- *   - a ZDX_CAPACITY return fills the required sizes so the caller can retry.
+ *   - a ZDX_CAPACITY return fills the required sizes so the caller can retry;
+ *   - device buffers grow on demand (a step that overflows one is run again
+ *     from the same state); with the environment variable
+ *     ZDX_START_CAPACITY=tiny, contexts and simulations created meanwhile
+ *     start every such buffer at a few entries (tests of those paths).
  */
 #ifndef ZDX_H
 #define ZDX_H
@@ -180,6 +184,12 @@ zdx_status zdx_sim_set_state(zdx_sim* sim, const double* state);
 /* Force, torque and contact count of each particle from the last step
  * (main.cpp:2027-2029); any pointer may be NULL. */
 zdx_status zdx_sim_get_forces(zdx_sim* sim, double* F, double* T, int32_t* contact_counts);
+/* Candidate pairs of the last completed step: the broad phase's pairs (i < j,
+ * ascending by i then j) of the state that step started from, under the
+ * reference's bounding-sphere test (src/main.cpp:2049-2059, rejected when
+ * dist2 > (ri + rj)^2).  *n receives the count (0 before the first step);
+ * ZDX_CAPACITY when it exceeds cap. */
+zdx_status zdx_sim_get_pairs(zdx_sim* sim, int32_t* pi, int32_t* pj, int64_t cap, int64_t* n);
 /* write_vtk_particles, src/main.cpp:1710-1840: legacy ASCII VTK polydata of
  * every particle's world triangles with id, mass, radius, contact_count,
  * velocity, omega, force, torque and orientation, byte-compatible with the

@@ -185,6 +185,7 @@ struct zdx_ctx {
     DBuf<int32_t> m_soff, m_ns, m_susp;
     // per-pair API buffers
     struct Api {
+        DBuf<zdx_pair_in> pairs_in;
         DBuf<int32_t> mesh;
         DBuf<double> state;
         DBuf<int64_t> wv_off, wt_off;
@@ -202,7 +203,7 @@ struct zdx_ctx {
         DBuf<int64_t> hit_off;
         DBuf<unsigned int> queue;
         DBuf<int32_t> bcnt, bcur;
-        DBuf<int32_t> ncontrib, comps, loops_raw, tstatus, topo_list, topo_n;
+        DBuf<int32_t> ncontrib, comps, loops_raw, tstatus, topo_list, topo_n, status2;
         TopoWork tw;
         DBuf<double> rec, vtx;
         DBuf<int32_t> prov;            // sensitivity runs
@@ -289,6 +290,20 @@ zdx_status guard(zdx_ctx* ctx, F&& f) {
         if (ctx) ctx->err = e.what();
         return ZDX_INTERNAL;
     }
+}
+
+struct CallLimits {
+    bool may_allocate = true;                       // false: meshes must already be on the device
+    int64_t list_bytes = (int64_t)1 << 60;          // cap of the cull's per-workgroup lists
+    int64_t walk_slices = 4096;                     // most topology walk workgroups
+};
+
+// ZDX_START_CAPACITY=tiny: every growable buffer of a context or simulation
+// created while it is set starts at a few entries, so the first calls and
+// steps run the grow-and-retry paths (results do not depend on it).
+bool tiny_start_capacity() {
+    const char* e = std::getenv("ZDX_START_CAPACITY");
+    return e && std::strcmp(e, "tiny") == 0;
 }
 
 int32_t persistent_groups(const zdx_ctx* c, int per_cu) { return std::max(1, c->n_cu * per_cu); }
@@ -595,14 +610,19 @@ void sim_setup_device(zdx_sim* s) {
         if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) == hipSuccess && khz > 0)
             s->clock_khz = (double)khz;
     }
-    s->alloc_pairs(std::max(1024, 8 * N));
+    const bool tiny = tiny_start_capacity();
+    if (tiny) {
+        s->stage_cap = 4; s->seg_max = 4; s->lmax = 1;
+        s->tw.adj_cap = 64; s->tw.snap_cap = 64; s->tw.tj_cap = 64; s->tw.units_cap = 4;
+    }
+    s->alloc_pairs(tiny ? 8 : std::max(1024, 8 * N));
     size_t groups = (size_t)persistent_groups(c, 16);
     s->tmax = c->max_tris;
     s->d_list_i.ensure(groups * 8 * (size_t)s->tmax);
     s->d_list_f.ensure(groups * 12 * (size_t)s->tmax);
     s->topo_threads = std::min<int64_t>(2048, std::max<int64_t>(64, (int64_t)N));
     s->alloc_stage(s->stage_cap);
-    s->alloc_hits(std::max<int64_t>(1 << 16, 16 * (int64_t)N));
+    s->alloc_hits(tiny ? 64 : std::max<int64_t>(1 << 16, 16 * (int64_t)N));
     s->tw.ensure_bump();   // before any graph capture
     // particle/wall histories start empty
     for (int k = 0; k < 2; ++k) {
@@ -648,7 +668,7 @@ void sim_setup_walls(zdx_sim* s, const std::vector<WallHost>& walls) {
         s->ps[k].wactive.ensure(npw);
         HIPC(hipMemsetAsync(s->ps[k].wactive.p, 0, npw, st));
     }
-    s->wall_pts_cap = std::max<int32_t>(64, s->ctx->max_tris);
+    s->wall_pts_cap = tiny_start_capacity() ? 4 : std::max<int32_t>(64, s->ctx->max_tris);
     s->wall_threads = std::min<int64_t>(2048, std::max<int64_t>(1, (int64_t)s->N));   // one-wave workgroups
     s->d_wscr.ensure((size_t)s->wall_threads * (size_t)(3 * s->wall_pts_cap + 4 * (s->wall_pts_cap + 8)));
     HIPC(hipStreamSynchronize(st));
@@ -923,120 +943,29 @@ void step_once(zdx_sim* s, bool batched) {
 
 }  // namespace
 
-// ======================================================================= C ABI
-extern "C" {
+namespace {
 
-zdx_status zdx_create(zdx_ctx** out, int32_t device) {
-    if (!out) return ZDX_BAD_ARG;
-    *out = nullptr;
-    auto* c = new zdx_ctx();
-    zdx_status st = guard(c, [&] {
-        int n = 0;
-        HIPC(hipGetDeviceCount(&n));
-        if (device < 0 || device >= n) throw ApiError{ZDX_BAD_ARG, "no such HIP device"};
-        HIPC(hipSetDevice(device));
-        c->device = device;
-        hipDeviceProp_t prop;
-        HIPC(hipGetDeviceProperties(&prop, device));
-        c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-        HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        HIPC(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
-        HIPC(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-        HIPC(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
-        HIPC(hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking));
-        HIPC(hipEventCreateWithFlags(&c->ev_fork3, hipEventDisableTiming));
-        HIPC(hipEventCreateWithFlags(&c->ev_join3, hipEventDisableTiming));
-        for (auto& e : c->ev_api) HIPC(hipEventCreate(&e));
-    });
-    if (st != ZDX_OK) {
-        delete c;
-        return st;
-    }
-    *out = c;
-    return ZDX_OK;
-}
-
-void zdx_destroy(zdx_ctx* c) {
-    if (!c) return;
-    (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
-    if (c->stream3) (void)hipStreamSynchronize(c->stream3);
-    for (auto& e : c->ev_api)
-        if (e) (void)hipEventDestroy(e);
-    if (c->ev_fork3) (void)hipEventDestroy(c->ev_fork3);
-    if (c->ev_join3) (void)hipEventDestroy(c->ev_join3);
-    if (c->stream3) (void)hipStreamDestroy(c->stream3);
-    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
-    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
-    if (c->stream2) (void)hipStreamDestroy(c->stream2);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
-    delete c;
-}
-
-const char* zdx_last_error(const zdx_ctx* c) { return c ? c->err.c_str() : "null context"; }
-
-zdx_status zdx_load_stl(const char* path, double* tris, int32_t cap, int32_t* T) {
-    if (!path || !T) return ZDX_BAD_ARG;
-    std::vector<double> t;
-    if (!load_stl(path, t)) return ZDX_IO;
-    *T = (int32_t)(t.size() / 9);
-    if (tris) {
-        if (*T > cap) return ZDX_CAPACITY;
-        std::memcpy(tris, t.data(), t.size() * sizeof(double));
-    }
-    return ZDX_OK;
-}
-
-zdx_status zdx_mesh_register(zdx_ctx* c, const double* tris, int32_t T, int32_t center, int32_t* mesh_id,
-                             double* props) {
-    if (!c || !tris || T <= 0 || !mesh_id) return ZDX_BAD_ARG;
-    return guard(c, [&] {
-        std::vector<double> t(tris, tris + 9 * (size_t)T);
-        HostMesh h = build_mesh(t, center != 0);
-        if (props) {
-            props[0] = h.volume;
-            std::memcpy(props + 1, h.center, 3 * sizeof(double));
-            props[4] = h.radius; props[5] = h.mean_edge; props[6] = h.bbox_diag;
-            std::memcpy(props + 7, h.inertia, 9 * sizeof(double));
-        }
-        *mesh_id = c->add_mesh(std::move(h));
-    });
-}
-
-zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, const zdx_contact_opts* opts,
-                             zdx_pair_out* out) {
-    if (!c || (!pairs && n > 0) || n < 0 || !opts || !out || !out->loop_offset) return ZDX_BAD_ARG;
-    return guard(c, [&] {
+// The per-pair call on the buffers of A, grown on demand; lim bounds the
+// per-workgroup lists and the topology walk slices (sized by workgroup
+// count) and says whether meshes may still be uploaded.
+void contact_pairs_impl(zdx_ctx* c, zdx_ctx::Api& A, const zdx_pair_in* pairs, int32_t n,
+                        const zdx_contact_opts* opts, zdx_pair_out* out, const CallLimits& lim) {
+    {
         HIPC(hipSetDevice(c->device));
         for (int p = 0; p < n; ++p)
             if (pairs[p].mesh_a < 0 || pairs[p].mesh_a >= (int)c->meshes.size() || pairs[p].mesh_b < 0 ||
                 pairs[p].mesh_b >= (int)c->meshes.size())
                 throw ApiError{ZDX_BAD_ARG, "pair references an unregistered mesh"};
-        c->sync_meshes();
-        auto& A = c->api;
+        if (lim.may_allocate) c->sync_meshes();
+        else if (c->mesh_dirty)
+            throw ApiError{ZDX_BAD_ARG, "meshes registered since the last allocating call are not on the device yet "
+                                        "(call zdx_contact_pairs once after registering)"};
         hipStream_t st = c->stream;
         const int nb = 2 * n;
-        std::vector<int32_t> mesh(nb);
-        std::vector<double> state(16 * (size_t)nb, 0.0);
-        std::vector<int64_t> wvo(nb + 1), wto(nb + 1);
-        int64_t av = 0, at = 0;
-        for (int p = 0; p < n; ++p) {
-            const zdx_pair_in& q = pairs[p];
-            for (int side = 0; side < 2; ++side) {
-                int b = 2 * p + side;
-                mesh[b] = side ? q.mesh_b : q.mesh_a;
-                const double* pos = side ? q.pos_b : q.pos_a;
-                const double* qu = side ? q.quat_b : q.quat_a;
-                double* s = &state[16 * (size_t)b];
-                s[0] = pos[0]; s[1] = pos[1]; s[2] = pos[2];
-                s[3] = qu[0]; s[4] = qu[1]; s[5] = qu[2]; s[6] = qu[3];
-                wvo[b] = av; wto[b] = at;
-                av += (int64_t)(c->meshes[mesh[b]].h.verts.size() / 3);
-                at += (int64_t)(c->meshes[mesh[b]].h.tri_idx.size() / 3);
-            }
-        }
-        wvo[nb] = av; wto[nb] = at;
+        int64_t av = 0;
+        for (int p = 0; p < n; ++p)
+            av += (int64_t)(c->meshes[pairs[p].mesh_a].h.verts.size() / 3) +
+                  (int64_t)(c->meshes[pairs[p].mesh_b].h.verts.size() / 3);
         if (av >= ((int64_t)1 << 31))   // the narrow phase's hit records index world vertices with int32
             throw ApiError{ZDX_CAPACITY, "more than 2^31 world vertices in one call"};
         if (n == 0) {
@@ -1045,28 +974,32 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
             out->n_loops_total = out->n_vtx_total = out->n_seg_total = 0;
             return;
         }
-        A.mesh.upload(mesh.data(), nb, st);
-        A.state.upload(state.data(), state.size(), st);
-        A.wv_off.upload(wvo.data(), nb + 1, st);
-        A.wt_off.upload(wto.data(), nb + 1, st);
+        // the pair records go to the device as they are; bodies, poses and
+        // world-vertex offsets are expanded there
+        A.pairs_in.ensure((size_t)n);
+        HIPC(hipMemcpyAsync(A.pairs_in.p, pairs, (size_t)n * sizeof(zdx_pair_in), hipMemcpyHostToDevice, st));
+        A.mesh.ensure(nb); A.state.ensure(16 * (size_t)nb); A.wv_off.ensure((size_t)nb + 1); A.wt_off.ensure((size_t)nb + 1);
+        DMeshes M = c->dmeshes();
+        launch_api_bodies(A.pairs_in.p, n, M, A.mesh.p, A.state.p, A.wv_off.p, A.wt_off.p, st);
         A.wv.ensure(3 * (size_t)av); A.aabb.ensure(6 * (size_t)nb); A.nondeg.ensure(nb);
         DBodies B;
         B.n = nb; B.mesh = A.mesh.p; B.state = A.state.p; B.wv_off = A.wv_off.p; B.wt_off = A.wt_off.p;
         B.wv = A.wv.p; B.nondeg = A.nondeg.p; B.aabb = A.aabb.p;
-        DMeshes M = c->dmeshes();
         HIPC(hipEventRecord(c->ev_api[0], st));
         launch_transform(B, M, st);
         HIPC(hipEventRecord(c->ev_api[1], st));
         A.pi.ensure(n); A.pj.ensure(n); A.npairs.ensure(1); A.status.ensure(1); A.order.ensure(n); A.items.ensure(n);
+        A.status2.ensure(1);
         launch_iota_pairs(A.pi.p, A.pj.p, n, st);
-        HIPC(hipMemcpyAsync(A.npairs.p, &n, sizeof(int32_t), hipMemcpyHostToDevice, st));
+        launch_fill_i32(A.npairs.p, n, 1, st);
         A.seg_off.ensure(n); A.tri_checks.ensure(n); A.tri_tests.ensure(n); A.nseg.ensure(n); A.tri_hits.ensure(n); A.tri_scanned.ensure(n);
         A.queue.ensure(2); A.bcnt.ensure(64); A.bcur.ensure(64);
         A.hit_off.ensure(n); A.hit_top.ensure(1);
         A.ncontrib.ensure(n); A.comps.ensure(n); A.loops_raw.ensure(n); A.tstatus.ensure(n);
         A.topo_list.ensure(n); A.topo_n.ensure(1);
         const bool batched = opts->mode == ZDX_MODE_BATCHED;
-        const int groups = std::min<int>(persistent_groups(c, 8), std::max(1, n));
+        int groups = std::min<int>(persistent_groups(c, 8), std::max(1, n));
+        groups = (int)std::max<int64_t>(1, std::min<int64_t>(groups, lim.list_bytes / (80 * (int64_t)c->max_tris)));
         A.list_i.ensure((size_t)groups * 8 * c->max_tris);
         A.list_f.ensure((size_t)groups * 12 * c->max_tris);
         const bool want_vtx = out->loop_vtx != nullptr;
@@ -1115,7 +1048,7 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
             TopoBufs T{};
             T.lmax = A.lmax; T.ncontrib = A.ncontrib.p; T.comps = A.comps.p; T.loops_raw = A.loops_raw.p;
             T.tstatus = A.tstatus.p;
-            T.nthreads = std::min<int64_t>(4096, (int64_t)n);
+            T.nthreads = std::min<int64_t>(lim.walk_slices, (int64_t)n);
             T.pass_groups = std::min<int64_t>(2048, (int64_t)n);
             T.seg_max = A.stage_cap;
             T.lds_bytes = 32 * 1024;
@@ -1147,10 +1080,11 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
                 c->api_ms[k] = ms;
             }
             if (status & ST_TOPO) {
-                std::vector<int32_t> ts(n);
-                HIPC(hipMemcpy(ts.data(), A.tstatus.p, n * 4, hipMemcpyDeviceToHost));
-                bool lo = false, vo = false;
-                for (int p = 0; p < n; ++p) { lo |= (ts[p] & TOPO_LOOP_OVERFLOW) != 0; vo |= (ts[p] & TOPO_VTX_OVERFLOW) != 0; }
+                int32_t ts = 0;
+                launch_or_i32(A.tstatus.p, n, A.status2.p, st);
+                HIPC(hipMemcpyAsync(&ts, A.status2.p, 4, hipMemcpyDeviceToHost, st));
+                HIPC(hipStreamSynchronize(st));
+                const bool lo = (ts & TOPO_LOOP_OVERFLOW) != 0, vo = (ts & TOPO_VTX_OVERFLOW) != 0;
                 const bool grown = A.tw.grow_from_counters(st);
                 if (lo) A.lmax *= 2;
                 if (vo) A.stage_cap *= 2;
@@ -1212,7 +1146,101 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
             return;
         }
         throw ApiError{ZDX_CAPACITY, "per-pair buffers did not fit after repeated growth"};
+    }
+}
+
+}  // namespace
+
+// ======================================================================= C ABI
+extern "C" {
+
+zdx_status zdx_create(zdx_ctx** out, int32_t device) {
+    if (!out) return ZDX_BAD_ARG;
+    *out = nullptr;
+    auto* c = new zdx_ctx();
+    zdx_status st = guard(c, [&] {
+        int n = 0;
+        HIPC(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) throw ApiError{ZDX_BAD_ARG, "no such HIP device"};
+        HIPC(hipSetDevice(device));
+        c->device = device;
+        hipDeviceProp_t prop;
+        HIPC(hipGetDeviceProperties(&prop, device));
+        c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIPC(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+        HIPC(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        HIPC(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+        HIPC(hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking));
+        HIPC(hipEventCreateWithFlags(&c->ev_fork3, hipEventDisableTiming));
+        HIPC(hipEventCreateWithFlags(&c->ev_join3, hipEventDisableTiming));
+        for (auto& e : c->ev_api) HIPC(hipEventCreate(&e));
+        if (tiny_start_capacity()) {
+            auto& A = c->api;
+            A.stage_cap = 4; A.hit_cap = 64; A.lmax = 1;
+            A.tw.adj_cap = 64; A.tw.snap_cap = 64; A.tw.tj_cap = 64; A.tw.units_cap = 4;
+        }
     });
+    if (st != ZDX_OK) {
+        delete c;
+        return st;
+    }
+    *out = c;
+    return ZDX_OK;
+}
+
+void zdx_destroy(zdx_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    if (c->stream3) (void)hipStreamSynchronize(c->stream3);
+    for (auto& e : c->ev_api)
+        if (e) (void)hipEventDestroy(e);
+    if (c->ev_fork3) (void)hipEventDestroy(c->ev_fork3);
+    if (c->ev_join3) (void)hipEventDestroy(c->ev_join3);
+    if (c->stream3) (void)hipStreamDestroy(c->stream3);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* zdx_last_error(const zdx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+zdx_status zdx_load_stl(const char* path, double* tris, int32_t cap, int32_t* T) {
+    if (!path || !T) return ZDX_BAD_ARG;
+    std::vector<double> t;
+    if (!load_stl(path, t)) return ZDX_IO;
+    *T = (int32_t)(t.size() / 9);
+    if (tris) {
+        if (*T > cap) return ZDX_CAPACITY;
+        std::memcpy(tris, t.data(), t.size() * sizeof(double));
+    }
+    return ZDX_OK;
+}
+
+zdx_status zdx_mesh_register(zdx_ctx* c, const double* tris, int32_t T, int32_t center, int32_t* mesh_id,
+                             double* props) {
+    if (!c || !tris || T <= 0 || !mesh_id) return ZDX_BAD_ARG;
+    return guard(c, [&] {
+        std::vector<double> t(tris, tris + 9 * (size_t)T);
+        HostMesh h = build_mesh(t, center != 0);
+        if (props) {
+            props[0] = h.volume;
+            std::memcpy(props + 1, h.center, 3 * sizeof(double));
+            props[4] = h.radius; props[5] = h.mean_edge; props[6] = h.bbox_diag;
+            std::memcpy(props + 7, h.inertia, 9 * sizeof(double));
+        }
+        *mesh_id = c->add_mesh(std::move(h));
+    });
+}
+
+zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, const zdx_contact_opts* opts,
+                             zdx_pair_out* out) {
+    if (!c || (!pairs && n > 0) || n < 0 || !opts || !out || !out->loop_offset) return ZDX_BAD_ARG;
+    return guard(c, [&] { contact_pairs_impl(c, c->api, pairs, n, opts, out, CallLimits{}); });
 }
 
 zdx_status zdx_contact_times(zdx_ctx* c, double* out, int32_t n) {
@@ -1381,6 +1409,24 @@ zdx_status zdx_sim_get_forces(zdx_sim* s, double* F, double* T, int32_t* cnt) {
         if (F) HIPC(hipMemcpy(F, s->d_F.p, 3 * (size_t)s->N * 8, hipMemcpyDeviceToHost));
         if (T) HIPC(hipMemcpy(T, s->d_T.p, 3 * (size_t)s->N * 8, hipMemcpyDeviceToHost));
         if (cnt) HIPC(hipMemcpy(cnt, s->d_counts.p, (size_t)s->N * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+zdx_status zdx_sim_get_pairs(zdx_sim* s, int32_t* pi, int32_t* pj, int64_t cap, int64_t* n) {
+    if (!s || !n || cap < 0 || (cap > 0 && (!pi || !pj))) return ZDX_BAD_ARG;
+    return guard(s->ctx, [&] {
+        HIPC(hipStreamSynchronize(s->st()));
+        *n = 0;
+        if (s->n_steps_done == 0) return;
+        const int last = 1 - s->parity;   // the pair set the last completed step wrote
+        int32_t np = 0;
+        HIPC(hipMemcpy(&np, s->ps[last].n.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+        *n = np;
+        if (np > cap) throw ApiError{ZDX_CAPACITY, "pair buffers too small (see *n)"};
+        if (np > 0) {
+            HIPC(hipMemcpy(pi, s->ps[last].pi.p, (size_t)np * 4, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(pj, s->ps[last].pj.p, (size_t)np * 4, hipMemcpyDeviceToHost));
+        }
     });
 }
 

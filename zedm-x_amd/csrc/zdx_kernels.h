@@ -241,5 +241,14 @@ struct ApiPack {
     int64_t loop_cap, vtx_cap, seg_cap;   // rows of the *_out arrays: nothing is written past them
 };
 void launch_api_pack(const ApiPack& k, hipStream_t s);
+// Bodies of the per-pair call from the caller's pair records (zdx_pair_in,
+// copied to the device as they are, `in`): body 2p is pair p's A, 2p + 1 its
+// B.  Writes mesh ids, state rows (pos, quat, the rest 0) and each body's
+// first world vertex and triangle (exclusive scans over the meshes' counts,
+// 2n + 1 entries: the last holds the totals).
+void launch_api_bodies(const void* in, int32_t n, const DMeshes& M, int32_t* mesh, double* state, int64_t* wv_off,
+                       int64_t* wt_off, hipStream_t s);
+// *out = OR of v[0..n) (out is cleared first)
+void launch_or_i32(const int32_t* v, int32_t n, int32_t* out, hipStream_t s);
 
 }  // namespace zdx

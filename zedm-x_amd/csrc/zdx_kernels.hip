@@ -30,6 +30,7 @@
 //                     then the rigid-body integrator
 #include <hip/hip_runtime.h>
 
+#include "../../include/zdx.h"
 #include "zdx_geom.h"
 #include "zdx_kernels.h"
 #include "zdx_topo.h"
@@ -2126,6 +2127,61 @@ __global__ void __launch_bounds__(256) k_api_pack(ApiPack k) {
     }
 }
 
+__global__ void __launch_bounds__(1024) k_api_bodies(const zdx_pair_in* in, int32_t n, DMeshes M, int32_t* mesh,
+                                                    double* state, int64_t* wvo, int64_t* wto) {
+    __shared__ long long sh[2][1024];
+    __shared__ long long carry[2];
+    const int t = threadIdx.x;
+    const int64_t nb = 2 * (int64_t)n;
+    if (t < 2) carry[t] = 0;
+    __syncthreads();
+    for (int64_t base = 0; base < nb; base += 1024) {
+        const int64_t b = base + t;
+        long long v[2] = {0, 0};
+        if (b < nb) {
+            const zdx_pair_in& q = in[b >> 1];
+            const int side = (int)(b & 1);
+            const int m = side ? q.mesh_b : q.mesh_a;
+            const double* pos = side ? q.pos_b : q.pos_a;
+            const double* qu = side ? q.quat_b : q.quat_a;
+            mesh[b] = m;
+            double* st = state + 16 * b;
+            st[0] = pos[0]; st[1] = pos[1]; st[2] = pos[2];
+            st[3] = qu[0]; st[4] = qu[1]; st[5] = qu[2]; st[6] = qu[3];
+            for (int k = 7; k < 16; ++k) st[k] = 0.0;
+            v[0] = M.nverts[m];
+            v[1] = M.ntris[m];
+        }
+        for (int c = 0; c < 2; ++c) sh[c][t] = v[c];
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            long long a[2];
+            for (int c = 0; c < 2; ++c) a[c] = t >= off ? sh[c][t - off] : 0;
+            __syncthreads();
+            for (int c = 0; c < 2; ++c) sh[c][t] += a[c];
+            __syncthreads();
+        }
+        if (b < nb) {
+            wvo[b] = carry[0] + sh[0][t] - v[0];
+            wto[b] = carry[1] + sh[1][t] - v[1];
+        }
+        __syncthreads();
+        if (t < 2) carry[t] += sh[t][1023];
+        __syncthreads();
+    }
+    if (t == 0) {
+        wvo[nb] = carry[0];
+        wto[nb] = carry[1];
+    }
+}
+
+__global__ void k_or_i32(const int32_t* v, int32_t n, int32_t* out) {
+    int32_t acc = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        acc |= v[i];
+    if (acc) atomicOr(out, acc);
+}
+
 inline int nblk(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
 }  // namespace
@@ -2353,6 +2409,18 @@ void launch_api_pack(const ApiPack& k, hipStream_t s) {
     if (k.n <= 0) return;
     hipLaunchKernelGGL(k_api_scan, dim3(1), dim3(1024), 0, s, k);
     hipLaunchKernelGGL(k_api_pack, dim3(std::min(nblk(k.n, 4), 4096)), dim3(256), 0, s, k);
+}
+
+void launch_api_bodies(const void* in, int32_t n, const DMeshes& M, int32_t* mesh, double* state, int64_t* wv_off,
+                       int64_t* wt_off, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_api_bodies, dim3(1), dim3(1024), 0, s, (const zdx_pair_in*)in, n, M, mesh, state, wv_off,
+                       wt_off);
+}
+
+void launch_or_i32(const int32_t* v, int32_t n, int32_t* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, out, 0, nullptr, (int64_t)1);
+    if (n > 0) hipLaunchKernelGGL(k_or_i32, dim3(std::min(nblk(n, 256), 1024)), dim3(256), 0, s, v, n, out);
 }
 
 }  // namespace zdx

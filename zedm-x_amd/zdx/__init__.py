@@ -26,7 +26,7 @@ COUNTERS = ["contacts", "tri_checks", "segments", "comps", "loops", "pairs", "wa
 EXPORTS = ["zdx_create", "zdx_destroy", "zdx_last_error", "zdx_load_stl", "zdx_mesh_register", "zdx_contact_pairs",
            "zdx_contact_times",
            "zdx_sim_load_config", "zdx_sim_create", "zdx_sim_destroy", "zdx_sim_num_particles", "zdx_sim_step",
-           "zdx_sim_get_state", "zdx_sim_set_state", "zdx_sim_get_forces", "zdx_sim_write_vtk",
+           "zdx_sim_get_state", "zdx_sim_set_state", "zdx_sim_get_forces", "zdx_sim_get_pairs", "zdx_sim_write_vtk",
            "zdx_sim_write_state_txt", "zdx_sim_checkpoint", "zdx_sim_restore",
            "zdx_sim_stage_times", "zdx_sim_set_force_model", "zdx_sim_set_tiles", "zdx_sim_autotune"]
 LOOP_REC = 28
@@ -103,6 +103,7 @@ def lib():
         L.zdx_sim_get_state.argtypes = [vp, vp]
         L.zdx_sim_set_state.argtypes = [vp, vp]
         L.zdx_sim_get_forces.argtypes = [vp, vp, vp, vp]
+        L.zdx_sim_get_pairs.argtypes = [vp, vp, vp, i64, C.POINTER(i64)]
         L.zdx_sim_write_vtk.argtypes = [vp, C.c_char_p]
         L.zdx_sim_write_state_txt.argtypes = [vp, C.c_char_p]
         L.zdx_sim_checkpoint.argtypes = [vp, C.c_char_p]
@@ -297,6 +298,17 @@ class Sim:
         c = np.zeros(self.n, dtype=np.int32)
         self.ctx._check(self.ctx._L.zdx_sim_get_forces(self.h, F.ctypes.data, T.ctypes.data, c.ctypes.data))
         return F, T, c
+
+    def pairs(self) -> np.ndarray:
+        """Candidate pairs (i < j, ascending) of the last completed step, (P, 2) int32."""
+        n = C.c_int64(0)
+        st = self.ctx._L.zdx_sim_get_pairs(self.h, None, None, 0, C.byref(n))
+        if st not in (OK, CAPACITY):
+            self.ctx._check(st)
+        pi = np.zeros(max(n.value, 1), dtype=np.int32)
+        pj = np.zeros(max(n.value, 1), dtype=np.int32)
+        self.ctx._check(self.ctx._L.zdx_sim_get_pairs(self.h, pi.ctypes.data, pj.ctypes.data, n.value, C.byref(n)))
+        return np.stack([pi[:n.value], pj[:n.value]], axis=1)
 
     def write_vtk(self, path: str) -> None:
         """The reference's per-step particle VTK (write_vtk_particles)."""

@@ -208,6 +208,7 @@ def build_sim(ctx, wl: Workload, center: bool = True):
     for (tris, pos, quat, mu, rest, _name) in wl.walls:
         walls.append((ctx.register_mesh(tris, center), pos, quat, mu, rest))
     sim = Sim.create(ctx, mesh_id, wl.init, wl.material, wl.gravity, wl.dt, wl.split, walls)
+    sim.mesh_id = mesh_id
     return sim, len(cache)
 
 
