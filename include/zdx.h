@@ -117,6 +117,21 @@ typedef struct {
 zdx_status zdx_contact_pairs(zdx_ctx* ctx, const zdx_pair_in* pairs, int32_t n_pairs, const zdx_contact_opts* opts,
                              zdx_pair_out* out);
 
+/* zdx_contact_pairs without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Same
+ * arguments, results and errors as zdx_contact_pairs (the same kernels run),
+ * but every device buffer of the call comes from one 256 MiB array per GPU
+ * that the library holds for the life of the process; a call that does not
+ * fit in it returns ZDX_CAPACITY (use zdx_contact_pairs), and meshes
+ * registered since the last allocating call must first be uploaded by one
+ * zdx_contact_pairs call (else ZDX_BAD_ARG).  The caller's host buffers are
+ * read and written as by zdx_contact_pairs.  All contexts on one GPU share
+ * that array: calls that overlap in time on one GPU race and must be
+ * serialised by the caller. */
+zdx_status zdx_contact_pairs_noalloc(zdx_ctx* ctx, const zdx_pair_in* pairs, int32_t n_pairs,
+                                     const zdx_contact_opts* opts, zdx_pair_out* out);
+
 /* Device time of the last zdx_contact_pairs call on this context (no
  * reference counterpart), from HIP events on the context's stream: out[0]
  * transform of the bodies, out[1] narrow phase (work ordering, cull, sweep,

@@ -42,17 +42,36 @@ struct ApiError {
     std::string msg;
 };
 
-// device buffer that grows on demand (contents are not preserved on growth)
+// A fixed block of device memory handed out front to back (256-byte
+// aligned pieces); nothing is returned before reset().
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, top = 0;
+    void* carve(size_t bytes) {
+        const size_t at = (top + 255) & ~(size_t)255;
+        if (at + bytes > cap)
+            throw ApiError{ZDX_CAPACITY, "the no-allocation scratch (" + std::to_string(cap >> 20) +
+                                             " MiB) is too small for this call: use zdx_contact_pairs"};
+        top = at + bytes;
+        return base + at;
+    }
+    void reset() { top = 0; }
+};
+
+// device buffer that grows on demand (contents are not preserved on growth);
+// bound to an Arena (ar != NULL) it takes its memory from there instead of
+// hipMalloc and gives nothing back (release() only forgets the piece)
 template <class T>
 struct DBuf {
     T* p = nullptr;
     size_t n = 0;
+    Arena* ar = nullptr;
     DBuf() = default;
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !ar) (void)hipFree(p);
         p = nullptr;
         n = 0;
     }
@@ -60,7 +79,8 @@ struct DBuf {
         if (m == 0) m = 1;
         if (m <= n) return p;
         release();
-        HIPC(hipMalloc((void**)&p, m * sizeof(T)));
+        if (ar) p = (T*)ar->carve(m * sizeof(T));
+        else HIPC(hipMalloc((void**)&p, m * sizeof(T)));
         n = m;
         return p;
     }
@@ -143,6 +163,11 @@ struct TopoWork {
         }
         return g;
     }
+    template <class F>
+    void each_buf(F&& f) {
+        f(cell9); f(comp); f(vcell); f(E); f(ncomp); f(V); f(adj); f(snapw); f(tjw); f(top); f(units);
+        f(adj_off); f(snap_off); f(tj_off); f(scr_i); f(scr_d);
+    }
     void fill(TopoBufs& T) {
         ensure_bump();
         T.cell9 = cell9.p; T.adj = adj.p; T.adj_cap = adj_cap; T.adj_row = adj_row; T.snapw = snapw.p;
@@ -183,7 +208,8 @@ struct zdx_ctx {
     DBuf<int32_t> m_voff, m_nv, m_toff, m_nt;
     DBuf<double> m_tol, m_aeps;
     DBuf<int32_t> m_soff, m_ns, m_susp;
-    // per-pair API buffers
+    // per-pair API buffers (api: grown with hipMalloc; api_na: carved from
+    // na_arena, this device's no-allocation scratch, at every call)
     struct Api {
         DBuf<zdx_pair_in> pairs_in;
         DBuf<int32_t> mesh;
@@ -215,7 +241,28 @@ struct zdx_ctx {
         int32_t stage_cap = 1024;      // most segments one pair may store
         int64_t hit_cap = 1 << 16;     // hit slots = segment slots
         int32_t lmax = 32;
-    } api;
+
+        template <class F>
+        void each_buf(F&& f) {
+            f(pairs_in); f(mesh); f(state); f(wv_off); f(wt_off); f(wv); f(aabb); f(nondeg); f(pi); f(pj); f(order);
+            f(npairs); f(status); f(items); f(seg_off); f(tri_checks); f(tri_tests); f(nseg); f(tri_hits);
+            f(tri_scanned); f(arena); f(list_i); f(list_f); f(hits); f(hit_tri); f(hit_flag); f(hit_top); f(hit_off);
+            f(queue); f(bcnt); f(bcur); f(ncontrib); f(comps); f(loops_raw); f(tstatus); f(topo_list); f(topo_n);
+            f(status2); f(rec); f(vtx); f(prov); f(ej); f(vj); f(ljac); f(pk_loop_off); f(pk_seg_off);
+            f(pk_vtx_off); f(pk_pvtx); f(pk_tot); f(pk_rec); f(pk_ljac); f(pk_vtx); f(pk_seg);
+            tw.each_buf(f);
+        }
+        // takes every buffer from `ar` from now on, starting from small
+        // capacities (a call grows them inside the arena as needed)
+        void bind(Arena* ar) {
+            each_buf([&](auto& b) { b.release(); b.ar = ar; });
+            stage_cap = 256; hit_cap = 1 << 15; lmax = 8;
+            tw.adj_cap = 1 << 16; tw.snap_cap = 1 << 14; tw.tj_cap = 1 << 14; tw.units_cap = 1 << 10;
+            tw.adj_row = 16; tw.slices = 0; tw.slice_seg_max = 0;
+        }
+    } api, api_na;
+    void* scratch_dev = nullptr;   // this device's `scratch` array
+    Arena na_arena;
 
     DMeshes dmeshes() {
         DMeshes M;
@@ -292,8 +339,20 @@ zdx_status guard(zdx_ctx* ctx, F&& f) {
     }
 }
 
+// Device memory of zdx_contact_pairs_noalloc: one array per GPU, made by the
+// code object's loader, so that entry point calls no allocator.  Host code
+// never uses the name `scratch` as an address: each context looks up this
+// device's copy with hipGetSymbolAddress when it is created.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+// bounds of the no-allocation call's workgroup-sized buffers: the cull's
+// per-workgroup triangle lists (80 bytes per triangle of the largest mesh)
+// and the topology walks' scratch slices
+constexpr int64_t NOALLOC_LIST_BYTES = (int64_t)16 << 20;
+constexpr int64_t NOALLOC_WALK_SLICES = 256;
+
 struct CallLimits {
-    bool may_allocate = true;                       // false: meshes must already be on the device
+    bool may_allocate = true;                       // false: device memory only from the arena
     int64_t list_bytes = (int64_t)1 << 60;          // cap of the cull's per-workgroup lists
     int64_t walk_slices = 4096;                     // most topology walk workgroups
 };
@@ -945,9 +1004,10 @@ void step_once(zdx_sim* s, bool batched) {
 
 namespace {
 
-// The per-pair call on the buffers of A, grown on demand; lim bounds the
-// per-workgroup lists and the topology walk slices (sized by workgroup
-// count) and says whether meshes may still be uploaded.
+// The per-pair call on the buffers of A: grown on demand (hipMalloc) for
+// zdx_contact_pairs, carved from the context's no-allocation arena for
+// zdx_contact_pairs_noalloc (lim then bounds the per-workgroup lists and the
+// topology walk slices, which are sized by workgroup count).
 void contact_pairs_impl(zdx_ctx* c, zdx_ctx::Api& A, const zdx_pair_in* pairs, int32_t n,
                         const zdx_contact_opts* opts, zdx_pair_out* out, const CallLimits& lim) {
     {
@@ -1175,6 +1235,12 @@ zdx_status zdx_create(zdx_ctx** out, int32_t device) {
         HIPC(hipEventCreateWithFlags(&c->ev_fork3, hipEventDisableTiming));
         HIPC(hipEventCreateWithFlags(&c->ev_join3, hipEventDisableTiming));
         for (auto& e : c->ev_api) HIPC(hipEventCreate(&e));
+        // this device's copy of the no-allocation scratch
+        void* sp = nullptr;
+        HIPC(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+        c->scratch_dev = sp;
+        c->na_arena.base = (char*)sp;
+        c->na_arena.cap = NOALLOC_SCRATCH_BYTES;
         if (tiny_start_capacity()) {
             auto& A = c->api;
             A.stage_cap = 4; A.hit_cap = 64; A.lmax = 1;
@@ -1241,6 +1307,22 @@ zdx_status zdx_contact_pairs(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, co
                              zdx_pair_out* out) {
     if (!c || (!pairs && n > 0) || n < 0 || !opts || !out || !out->loop_offset) return ZDX_BAD_ARG;
     return guard(c, [&] { contact_pairs_impl(c, c->api, pairs, n, opts, out, CallLimits{}); });
+}
+
+zdx_status zdx_contact_pairs_noalloc(zdx_ctx* c, const zdx_pair_in* pairs, int32_t n, const zdx_contact_opts* opts,
+                                     zdx_pair_out* out) {
+    if (!c || (!pairs && n > 0) || n < 0 || !opts || !out || !out->loop_offset) return ZDX_BAD_ARG;
+    return guard(c, [&] {
+        if (!c->scratch_dev) throw ApiError{ZDX_INTERNAL, "no-allocation scratch address not set up"};
+        // every call starts from the front of this device's scratch array
+        c->na_arena.reset();
+        c->api_na.bind(&c->na_arena);
+        CallLimits lim;
+        lim.may_allocate = false;
+        lim.list_bytes = NOALLOC_LIST_BYTES;
+        lim.walk_slices = NOALLOC_WALK_SLICES;
+        contact_pairs_impl(c, c->api_na, pairs, n, opts, out, lim);
+    });
 }
 
 zdx_status zdx_contact_times(zdx_ctx* c, double* out, int32_t n) {

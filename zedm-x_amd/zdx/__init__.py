@@ -24,7 +24,7 @@ FORCE_ZDEM, FORCE_MINIMAL = 0, 1
 COUNTERS = ["contacts", "tri_checks", "segments", "comps", "loops", "pairs", "wall_contacts", "tri_tests", "tri_hits",
             "tri_scanned", "topo_pairs", "seg_max"]
 EXPORTS = ["zdx_create", "zdx_destroy", "zdx_last_error", "zdx_load_stl", "zdx_mesh_register", "zdx_contact_pairs",
-           "zdx_contact_times",
+           "zdx_contact_times", "zdx_contact_pairs_noalloc",
            "zdx_sim_load_config", "zdx_sim_create", "zdx_sim_destroy", "zdx_sim_num_particles", "zdx_sim_step",
            "zdx_sim_get_state", "zdx_sim_set_state", "zdx_sim_get_forces", "zdx_sim_get_pairs", "zdx_sim_write_vtk",
            "zdx_sim_write_state_txt", "zdx_sim_checkpoint", "zdx_sim_restore",
@@ -94,6 +94,7 @@ def lib():
         L.zdx_mesh_register.argtypes = [vp, vp, i32, i32, C.POINTER(i32), vp]
         L.zdx_contact_pairs.argtypes = [vp, vp, i32, C.POINTER(ContactOpts), C.POINTER(PairOut)]
         L.zdx_contact_times.argtypes = [vp, vp, i32]
+        L.zdx_contact_pairs_noalloc.argtypes = [vp, vp, i32, C.POINTER(ContactOpts), C.POINTER(PairOut)]
         L.zdx_sim_load_config.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(vp)]
         L.zdx_sim_create.argtypes = [vp, C.POINTER(SimDesc), C.POINTER(vp)]
         L.zdx_sim_destroy.argtypes = [vp]
@@ -177,11 +178,13 @@ class Context:
         return mid.value
 
     def contact_pairs(self, mesh_a, mesh_b, pos_a, quat_a, pos_b, quat_b, split=True, mode=MODE_BATCHED, k=1.0,
-                      segments=False, vertices=True, sensitivities=False, tol=0.0):
+                      segments=False, vertices=True, sensitivities=False, tol=0.0, noalloc=False):
         """Batched per-pair contact evaluation.  Arrays are per pair; returns a
         dict with per-pair counters, per-loop records (ZDX_LOOP_REC layout) and,
         optionally, loop vertices and narrow-phase segments.  tol > 0 replaces
-        mesh A's snap / adjacency tolerance for every pair."""
+        mesh A's snap / adjacency tolerance for every pair.  noalloc: through
+        zdx_contact_pairs_noalloc (no device allocation; the library's
+        per-GPU scratch array bounds the batch)."""
         n = len(mesh_a)
         arr = np.zeros(max(n, 1), dtype=PAIR_IN_DTYPE)
         if n:
@@ -208,7 +211,8 @@ class Context:
             out = PairOut(tc.ctypes.data, ns.ctypes.data, nc.ctypes.data, nr.ctypes.data, lo.ctypes.data, _ptr(so),
                           rec.ctypes.data, loop_cap, _ptr(vo), _ptr(vtx), vtx_cap, _ptr(seg), seg_cap, 0, 0, 0,
                           _ptr(jac))
-            st = self._L.zdx_contact_pairs(self.h, arr.ctypes.data, n, C.byref(opts), C.byref(out))
+            fn = self._L.zdx_contact_pairs_noalloc if noalloc else self._L.zdx_contact_pairs
+            st = fn(self.h, arr.ctypes.data, n, C.byref(opts), C.byref(out))
             if st == CAPACITY:
                 loop_cap = max(loop_cap, int(out.n_loops_total) + 1)
                 vtx_cap = max(vtx_cap, int(out.n_vtx_total) + 1)
