@@ -128,6 +128,24 @@ def icosphere(level: int) -> np.ndarray:
     return np.ascontiguousarray(P[F])
 
 
+def _radial_noise(seed: int, u: np.ndarray) -> np.ndarray:
+    rng = Rng(seed)
+    d = rng.unit_vectors(6)
+    ph = rng.uniform(6) * 2.0 * math.pi
+    return np.mean(np.cos(4.0 * math.pi * (u @ d.T) + ph[None, :]), axis=1)
+
+
+def star_hull_mesh(seed: int, n_points: int, amp: float = 0.3) -> np.ndarray:
+    """Concave radial-noise star on the triangulation of the convex hull of
+    n_points seeded unit vectors (2 n_points - 4 triangles): each vertex u is
+    scaled by 1 + amp*noise(u), noise as in star_mesh (seeded by seed + 1)."""
+    tris = convex_hull_mesh(Rng(seed).unit_vectors(n_points))
+    flat = tris.reshape(-1, 3)
+    u = flat / np.linalg.norm(flat, axis=1, keepdims=True)
+    noise = _radial_noise(seed + 1, u)
+    return np.ascontiguousarray((u * (1.0 + amp * noise)[:, None]).reshape(-1, 3, 3))
+
+
 def star_mesh(seed: int, level: int = 3, amp: float = 0.3) -> np.ndarray:
     """Concave radial-noise star: icosphere vertices scaled by 1 + amp*noise(u).
 

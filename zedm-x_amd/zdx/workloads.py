@@ -5,6 +5,8 @@ granular_bed():   100k polyhedra from 8 templates in a 0.2 m box with a
 uneven_pairs():   4096 independent pairs, icospheres of 80-5120 triangles and
                   concave stars (the batched per-pair workload)
 drop_on_plane():  64 convex hulls above a plane wall (the differentiable drop)
+concave_packing(): 1M concave stars of 320-2560 triangles in a 0.5 m box (the
+                  large single-GPU configuration)
 
 Particles carry the zdem_cpu fields (template, scale, pos, quat, vel, omega,
 material); write_reference_config() writes any subset as a zdem_cpu config
@@ -190,6 +192,65 @@ def drop_on_plane(n: int = 64, seed: int = 4, radius: float = 5e-3, height: floa
             np.array([1.0, 0.0, 0.0, 0.0]), 0.5, 0.5, "plane")
     return Workload("drop_on_plane", [hull], ["hull128"], np.zeros(n, dtype=np.int64), np.full(n, scale), init, mat,
                     [wall], (0.0, 0.0, -9.81), 1e-5, 500, True, dict(seed=seed))
+
+
+def concave_packing(n: int = 1_000_000, seed: int = 5, box: float = 0.5, r_min: float = 1e-3, r_max: float = 4e-3,
+                    n_templates: int = 16, radius_levels: int = 256, speed: float = 0.2,
+                    overlap: float = 0.12) -> Workload:
+    """Large concave packing: n radial-noise stars from n_templates templates
+    of 320-2560 triangles (hulls of 162-1282 seeded sphere points, each vertex
+    scaled by 1 + 0.3 noise), radii log-uniform in [r_min, r_max] on
+    radius_levels log-spaced values (so at most n_templates x radius_levels
+    distinct meshes, the reference's path|scale cache), density 2500, young
+    1e7, in a box of edge `box` bounded by one 12-triangle box wall, gravity
+    -9.81 z, dt 5e-6.
+
+    Seeding: the particles sorted by radius (largest first) fill horizontal
+    layers from the bottom; a layer is a hexagonal lattice whose spacing is
+    (1 - overlap) x twice the largest radius in it, so the bounding spheres of
+    neighbours in a layer overlap by up to 12 % of their diameter: every
+    neighbour pair is a candidate pair from the first step and the stars'
+    lobes meet in about a third of them, often in several contact loops.  Gaussian velocities of
+    `speed` m/s per component, random orientations."""
+    names, tris = [], []
+    for k, m in enumerate(np.linspace(162, 1282, n_templates).round().astype(int)):
+        names.append(f"star{2 * m - 4}")
+        tris.append(synth.star_hull_mesh(seed * 1000 + 17 * k, int(m)))
+    rad_t = np.array([_centred_radius(t) for t in tris])
+    g = np.random.Generator(np.random.PCG64(seed))
+    tpl = g.integers(0, n_templates, n)
+    lv = np.exp(np.linspace(math.log(r_min), math.log(r_max), radius_levels))
+    r = lv[g.integers(0, radius_levels, n)]
+    scale = np.maximum(np.round(r / rad_t[tpl], 6), 1e-6)
+    rad = scale * rad_t[tpl]
+    order = np.argsort(-rad, kind="stable")
+    pos = np.zeros((n, 3))
+    i, z = 0, 0.0
+    while i < n:
+        s = 2.0 * rad[order[i]] * (1.0 - overlap)
+        dy = s * math.sqrt(3.0) / 2.0
+        nx = int(math.floor((box - s) / s)) + 1
+        ny = int(math.floor((box - s) / dy)) + 1
+        if z == 0.0:
+            z = s / 2.0
+        if z + s / 2.0 > box:
+            raise ValueError(f"{n} particles do not fit the box of {box} (stopped at {i})")
+        row, col = np.divmod(np.arange(nx * ny), nx)
+        x = s / 2.0 + col * s + np.where(row % 2 == 1, s / 2.0, 0.0)
+        keep = x <= box - s / 2.0           # odd rows hold one site fewer
+        x, y = x[keep], s / 2.0 + row[keep] * dy
+        cnt = min(n - i, len(x))
+        pos[order[i:i + cnt]] = np.stack([x[:cnt], y[:cnt], np.full(cnt, z)], axis=1)
+        i += cnt
+        z += s
+    q = g.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    vel = g.normal(size=(n, 3)) * speed
+    init = np.concatenate([pos, q, vel, np.zeros((n, 3))], axis=1)
+    mat = np.tile(np.array([2500.0, 1e7, 0.25, 0.5, 0.5]), (n, 1))
+    wall = (synth.box_wall(box, box, box), np.array([box / 2] * 3), np.array([1.0, 0.0, 0.0, 0.0]), 0.5, 0.5, "box")
+    return Workload("concave_packing", tris, names, tpl, scale, init, mat, [wall], (0.0, 0.0, -9.81), 5e-6, 200, True,
+                    dict(seed=seed, box=box, r_min=r_min, r_max=r_max, height=z, overlap=overlap))
 
 
 def build_sim(ctx, wl: Workload, center: bool = True):
