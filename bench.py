@@ -236,6 +236,34 @@ def uneven_block(ctx, reps: int, cpu_sample: int, log):
     return out
 
 
+def packing_block(ctx, steps: int, log):
+    """The 1M-particle concave packing (zdx.workloads.concave_packing: 16
+    star templates of 320-2560 triangles, radii 1-4 mm, 0.5 m box, dt 5e-6)
+    on the same GPU: 2 untimed batched steps (buffer growth, graph capture),
+    then `steps` timed batched steps."""
+    import zdx
+    from zdx import workloads as W
+
+    t0 = time.time()
+    wl = W.concave_packing()
+    sim, n_meshes = W.build_sim(ctx, wl)
+    setup = time.time() - t0
+    sim.step(2, zdx.MODE_BATCHED)
+    sim.stage_times()
+    t, c = run_steps(sim, steps, zdx.MODE_BATCHED)
+    st = sim.stage_times()
+    sim.close()
+    log(f"packing 1M: {steps} steps {t:.3f} s, pairs/step {c['pairs'] / steps:.0f}")
+    return {"workload": f"1M concave stars (16 templates of 320-2560 triangles, radii log-uniform 1-4 mm on 256 "
+                        f"levels), 0.5 m box, 12-triangle box wall, gravity -9.81, dt 5e-6; hexagonal layers "
+                        f"with bounding spheres overlapping up to 12 %; timed after 2 untimed steps",
+            "particles": wl.n, "meshes": n_meshes, "setup_s": setup, "steps": steps,
+            "contact_pairs_per_s": c["pairs"] / t, "steps_per_s": steps / t, "ms_per_step": t / steps * 1e3,
+            "contact_pairs_per_step": c["pairs"] / steps, "active_contacts_per_step": c["contacts"] / steps,
+            "segments_per_step": c["segments"] / steps, "timed_retries": st["retries"],
+            "stage_ms": {k: st[k] for k in zdx.Sim.STAGES}}
+
+
 _TPL_RAD = {}
 
 
@@ -390,6 +418,65 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir, command="", timed_stats=
     print(json.dumps(res))
 
 
+VALU_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_WAVE_CYCLES",
+                 "SQ_BUSY_CYCLES", "SQ_WAIT_INST_ANY", "SQ_INSTS_VALU_FLOPS_FP64", "GRBM_GUI_ACTIVE")
+
+
+def summarize_valu(pmc_dir, stats_dir, command="", timed=5, n_cu=256):
+    """profiles/ summary of the VALU and occupancy counters (one rocprofv3
+    --pmc run with VALU_COUNTERS) per kernel, averaged over the last `timed`
+    dispatches of each: the rocprofv3 derived metrics computed from the raw
+    counters -- VALUBusy = 100 sum(SQ_ACTIVE_INST_VALU) / CU_NUM /
+    max(GRBM_GUI_ACTIVE), VALUUtilization = 100 sum(SQ_THREAD_CYCLES_VALU) /
+    (sum(SQ_ACTIVE_INST_VALU) 64), OccupancyPercent = 400 sum(SQ_WAVE_CYCLES)
+    / max(GRBM_GUI_ACTIVE) / CU_NUM / 32 -- and the FP64 VALU rate from
+    SQ_INSTS_VALU_FLOPS_FP64 over the kernel's average duration in the
+    kernel trace of stats_dir."""
+    import csv
+    import glob
+
+    per = {}
+    for f in glob.glob(os.path.join(pmc_dir, "*counter_collection.csv")):
+        for r in csv.DictReader(open(f)):
+            key = (int(r["Dispatch_Id"]), r["Kernel_Name"])
+            d = per.setdefault(key, {})
+            c, v = r["Counter_Name"], float(r["Counter_Value"])
+            d[c] = max(d.get(c, 0.0), v) if c.startswith("GRBM") else d.get(c, 0.0) + v
+    by = {}
+    for (did, name), d in sorted(per.items()):
+        by.setdefault(name, []).append(d)
+    dur = {k: v[-timed:] for k, v in _trace_by_kernel(stats_dir, lambda n: True).items()}
+    rows = []
+    for name, ds in by.items():
+        ds = ds[-timed:]
+        m = {c: sum(d.get(c, 0.0) for d in ds) / len(ds) for c in VALU_COUNTERS}
+        g = max(m["GRBM_GUI_ACTIVE"], 1.0)
+        t_us = sum(dur.get(name, [0.0])) / max(1, len(dur.get(name, [0.0])))
+        rows.append(dict(name=name, t_us=t_us, waves=m["SQ_WAVES"],
+                         valu_busy=100.0 * m["SQ_ACTIVE_INST_VALU"] / n_cu / g,
+                         valu_util=100.0 * m["SQ_THREAD_CYCLES_VALU"] / max(m["SQ_ACTIVE_INST_VALU"] * 64.0, 1.0),
+                         occupancy=400.0 * m["SQ_WAVE_CYCLES"] / g / n_cu / 32.0,
+                         wait_frac=m["SQ_WAIT_INST_ANY"] / max(m["SQ_WAVE_CYCLES"], 1.0),
+                         fp64=m["SQ_INSTS_VALU_FLOPS_FP64"],
+                         fp64_tflops=m["SQ_INSTS_VALU_FLOPS_FP64"] / (t_us * 1e-6) / 1e12 if t_us > 0 else 0.0))
+    rows.sort(key=lambda r: -r["t_us"])
+    lines = ["# VALU and occupancy -- granular bed, 100k polyhedra, batched step (graph capture off)", "",
+             f"Command: `{command}`" if command else "", "",
+             f"Counters: {' '.join(VALU_COUNTERS)} (one --pmc pass), per kernel averaged over its last {timed} "
+             f"dispatches (the timed steps); durations from the kernel trace of the same bench.  VALUBusy, "
+             f"VALUUtilization and OccupancyPercent are rocprofv3's derived metrics computed from these raw counters "
+             f"(CU_NUM = {n_cu}); wait = SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES; FP64 = SQ_INSTS_VALU_FLOPS_FP64 per "
+             f"dispatch and its rate over the kernel's duration (peak 78.6 TFLOP/s).", "",
+             "| kernel | us | waves | VALUBusy % | VALUUtil % | Occupancy % | wait | FP64 GFLOP | FP64 TFLOP/s |",
+             "|---|---|---|---|---|---|---|---|---|"]
+    for r in rows[:16]:
+        lines.append(f"| `{_short(r['name'])}` | {r['t_us']:.1f} | {r['waves']:.0f} | {r['valu_busy']:.1f} | "
+                     f"{r['valu_util']:.1f} | {r['occupancy']:.1f} | {r['wait_frac']:.2f} | {r['fp64'] / 1e9:.3f} | "
+                     f"{r['fp64_tflops']:.2f} |")
+    open(os.path.join(ROOT, "profiles", "valu_occupancy_granular_bed_100k.md"), "w").write("\n".join(lines) + "\n")
+    print(json.dumps(rows[:16]))
+
+
 def run_steps(sim, k, mode):
     import zdx
 
@@ -424,6 +511,8 @@ def main():
     ap.add_argument("--no-uneven", action="store_true", help="skip the 4096-pair uneven batch block")
     ap.add_argument("--uneven-reps", type=int, default=5)
     ap.add_argument("--uneven-cpu-pairs", type=int, default=256)
+    ap.add_argument("--no-packing", action="store_true", help="skip the 1M-particle packing block")
+    ap.add_argument("--packing-steps", type=int, default=5)
     ap.add_argument("--cpu-pairs", type=int, default=4000, help="candidate pairs wanted in the CPU sample")
     ap.add_argument("--cpu-max-particles", type=int, default=32000)
     ap.add_argument("--cpu-timeout", type=float, default=240.0)
@@ -434,7 +523,13 @@ def main():
                          "output directories of this bench, then exit")
     ap.add_argument("--profile-command", default="", help="command line recorded in the summaries")
     ap.add_argument("--profile-bench-json", default=None, help="JSON line printed by the profiled bench run")
+    ap.add_argument("--summarize-valu", nargs=2, metavar=("PMC_DIR", "STATS_DIR"),
+                    help="write the profiles/ VALU and occupancy summary from a rocprofv3 --pmc run with "
+                         "VALU_COUNTERS and the --kernel-trace --stats run of the same bench, then exit")
     args = ap.parse_args()
+    if args.summarize_valu:
+        summarize_valu(*args.summarize_valu, command=args.profile_command)
+        return
     if args.summarize_profiles:
         summarize_profiles(*args.summarize_profiles, command=args.profile_command, bench_json=args.profile_bench_json)
         return
@@ -542,6 +637,9 @@ def main():
     uneven = None
     if not args.no_uneven:
         uneven = uneven_block(ctx, args.uneven_reps, 0 if args.no_cpu_baseline else args.uneven_cpu_pairs, log)
+    packing = None
+    if not args.no_packing:
+        packing = packing_block(ctx, args.packing_steps, log)
 
     pairs_b = cb["pairs"]
     res = {
@@ -582,6 +680,8 @@ def main():
         res["batched_over_plain"] = (pairs_b / tb) / (plain["counters"]["pairs"] / plain["seconds"])
     if uneven:
         res["uneven_pairs"] = uneven
+    if packing:
+        res["packing_1m"] = packing
     shutil.rmtree(tmpdir, ignore_errors=True)
     print(json.dumps(res))
 

@@ -141,6 +141,26 @@ zdx_status zdx_contact_pairs_noalloc(zdx_ctx* ctx, const zdx_pair_in* pairs, int
  * included.  Writes min(n, 4) values. */
 zdx_status zdx_contact_times(zdx_ctx* ctx, double* out, int32_t n);
 
+/* Particle-wall contact geometry (compute_wall_contact, src/main.cpp:1205-1258,
+ * for each wall triangle the reference's step tests, 2325-2369): for every
+ * (body b, wall triangle t) the plane section of the body's mesh clipped by
+ * the triangle, its area and contact point, and the wall normal turned
+ * towards the body (main.cpp:2363-2367).  The caller applies the reference's
+ * bounding-sphere and box pre-tests (2318-2343) when choosing the pairs.
+ * wall_tris: n_wall_tris x 9 world coordinates; wall_normals: x 3.  out:
+ * n_bodies x n_wall_tris records of ZDX_WALL_REC doubles:
+ *   [0] 1 if in contact  [1] area  [2:5] contact point  [5:8] normal
+ *   with jac != 0 (no reference counterpart): [8:14] d area / d xi,
+ *   [14:32] d point / d xi (3 x 6, row-major), xi = (d, w) a rigid motion of
+ *   the body (its points x move by d + w x (x - pos)). */
+typedef struct {
+    int32_t mesh, reserved;
+    double pos[3], quat[4];
+} zdx_body_pose;
+#define ZDX_WALL_REC 32
+zdx_status zdx_wall_contacts(zdx_ctx* ctx, const zdx_body_pose* bodies, int32_t n_bodies, const double* wall_tris,
+                             const double* wall_normals, int32_t n_wall_tris, int32_t jac, double* out);
+
 /* ------------------------------------------------------------- simulation */
 /* The zdem_cpu time step (src/main.cpp:2034-2468): broad phase, narrow phase,
  * topology, particle-particle and particle-wall forces with tangential
@@ -237,11 +257,13 @@ zdx_status zdx_sim_stage_times(zdx_sim* sim, double* out, int32_t n);
 /* Particle-particle force model of the time step.  ZDX_FORCE_ZDEM (default):
  * the reference's force, src/main.cpp:2220-2287.  ZDX_FORCE_MINIMAL: the
  * minimal-energy force of polyline.py:655-701, Fn = k/2 |Sn| along the A->B
- * normal, plus normal damping from the restitution (c = 2 zeta(e)
- * sqrt(m_eff pi k R), R = R1 R2/(R1+R2)), applied at the loop centroid; no
- * tangential force.  Walls keep the reference's wall force either way.  In a
- * config file: "force_model minimal" and "minimal_k <k>". */
-enum { ZDX_FORCE_ZDEM = 0, ZDX_FORCE_MINIMAL = 1 };
+ * normal, applied at the loop centroid, with no damping and no tangential
+ * force.  ZDX_FORCE_ZDEM_DAMPED: the reference's force plus the normal
+ * damping it computes and leaves commented out (main.cpp:2244-2248,
+ * 2258-2260): fn += -cn vn while the bodies approach.  Walls keep the
+ * reference's wall force in every model.  In a config file: "force_model
+ * minimal" with "minimal_k <k>", or "force_model zdem_damped". */
+enum { ZDX_FORCE_ZDEM = 0, ZDX_FORCE_MINIMAL = 1, ZDX_FORCE_ZDEM_DAMPED = 2 };
 zdx_status zdx_sim_set_force_model(zdx_sim* sim, int32_t model, double k);
 /* Narrow-phase launch shape (no reference counterpart).  np_shape encodes
  * the workgroup size and the triangle-pair tests per thread of one sweep

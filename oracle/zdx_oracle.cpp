@@ -869,7 +869,8 @@ struct Config {
     std::string stl;
     int output_interval = 1;
     V3 v0, v1;
-    int force_model = 0;       // 1: minimal-energy force (build extension key, ignored by the reference)
+    int force_model = 0;       // 1: minimal-energy force, 2: the reference's force with its dormant
+                               // normal damping (build extension key, ignored by the reference)
     double k_minimal = 0.0;
     std::vector<PInit> parts;
     std::vector<WInit> walls;
@@ -931,7 +932,11 @@ bool parse_config(const std::string& path, Config& c) {
         else if (k == "split_contacts") { int v = 1; is >> v; c.split = (v != 0); }
         else if (k == "gravity") is >> c.g.x >> c.g.y >> c.g.z;
         else if (k == "center_mesh") { int v = 1; is >> v; c.center = (v != 0); }
-        else if (k == "force_model") { std::string v; is >> v; c.force_model = (v == "minimal") ? 1 : 0; }
+        else if (k == "force_model") {
+            std::string v;
+            is >> v;
+            c.force_model = v == "minimal" ? 1 : v == "zdem_damped" ? 2 : 0;
+        }
         else if (k == "minimal_k") is >> c.k_minimal;
         else if (k == "output_interval") is >> c.output_interval;
         else if (k == "v0") is >> c.v0.x >> c.v0.y >> c.v0.z;
@@ -1027,21 +1032,12 @@ void sim_step_once(orc_sim& s, int64_t* ctr) {
             nloop += po.loops_raw;
             for (const auto& lo : po.loops) {
                 if (cfg.force_model == 1) {
-                    // Minimal-energy force (polyline.py:676-679 elastic part) with
-                    // restitution damping, at the loop centroid (-3/2 xc0 up to a
-                    // multiple of nA).  Independent of the reference's history.
+                    // Minimal-energy force, polyline.py:676-679: Fn = k/2 |Sn|
+                    // along nA, at the loop centroid (-3/2 xc0 up to a multiple
+                    // of nA).  No damping, no tangential force, no history.
                     V3 xc = lo.xc * -1.5;
                     V3 rA = xc - a.tf.pos, rB = xc - b.tf.pos;
-                    V3 vrel = (b.vel + cross(b.omega, rB)) - (a.vel + cross(a.omega, rA));
-                    double vn = dot(vrel, lo.nA);
-                    double R1 = std::max(a.eqr, 1e-12), R2 = std::max(b.eqr, 1e-12);
-                    double Reff = R1 * R2 / (R1 + R2);
-                    double meff = 1.0 / (a.inv_mass + b.inv_mass);
-                    double e = std::min(0.9999, std::max(1e-6, std::min(a.rest, b.rest)));
-                    double le = std::log(e);
-                    double zeta = -le / std::sqrt(3.141592653589793 * 3.141592653589793 + le * le);
-                    double c = 2.0 * zeta * std::sqrt(meff * (3.141592653589793 * cfg.k_minimal * Reff));
-                    V3 f = lo.nA * (0.5 * cfg.k_minimal * lo.area - c * vn);
+                    V3 f = lo.nA * (0.5 * cfg.k_minimal * lo.area);
                     s.F[i] -= f;
                     s.F[j] += f;
                     s.T[i] += cross(rA, f * -1.0);
@@ -1068,10 +1064,15 @@ void sim_step_once(orc_sim& s, int64_t* ctr) {
                 double e = std::min(a.rest, b.rest);
                 e = std::min(0.9999, std::max(1e-6, e));
                 double le = std::log(e);
+                double cn = 2.0 * le * std::sqrt(kn * meff) / std::sqrt(le * le + 3.141592653589793 * 3.141592653589793);
                 double ct = 2.0 * le * std::sqrt(kt * meff) / std::sqrt(le * le + 3.141592653589793 * 3.141592653589793);
+                cn = std::abs(cn);
                 ct = std::abs(ct);
                 double dV = std::pow(area, 1.5);
                 double fn = kn * std::cbrt(dV);
+                // force model 2: the normal damping the reference computes and
+                // leaves commented out (main.cpp:2244-2248, 2258-2260)
+                if (cfg.force_model == 2 && vn < 0.0) fn += -cn * vn;
                 if (fn < 0.0) fn = 0.0;
                 V3 fnv = nA * fn;
                 V3 vt = vrel - nA * vn;

@@ -157,16 +157,23 @@ def test_torque_free_rotation_keeps_angular_momentum():
     assert abs(np.linalg.norm(s[3:7]) - 1.0) < 1e-14     # q renormalised every step
 
 
-@pytest.mark.parametrize("depth,off,pa", [(0.2, (0.0, 0.0), (0.0, 0.0)), (0.3, (0.2, 0.2), (0.3, -0.7))])
-def test_minimal_force_model_in_the_step_box_on_box(depth, off, pa, tmp_path):
-    """The oracle's time step with "force_model minimal": two boxes at rest
-    (no damping since v_rel = 0) get F_B = k/2 A n and -F_B, acting at the
-    centre of the contact rectangle (the torque about each body centre is
-    (c - x) x F), whatever the world position of the pair."""
+@pytest.mark.parametrize("depth,off,pa,moving", [(0.2, (0.0, 0.0), (0.0, 0.0), False),
+                                                 (0.3, (0.2, 0.2), (0.3, -0.7), False),
+                                                 (0.3, (0.2, 0.2), (0.3, -0.7), True)])
+def test_minimal_force_model_in_the_step_box_on_box(depth, off, pa, moving, tmp_path):
+    """The oracle's time step with "force_model minimal": two boxes get
+    F_B = k/2 A n and -F_B, acting at the centre of the contact rectangle (the
+    torque about each body centre is (c - x) x F), whatever the world position
+    of the pair, at rest or approaching and spinning (polyline.py's force has
+    no velocity term)."""
     import golden_inputs as GI
 
     k = 5.0
     text, files, pA, pB, c = GI.box_pair_case(depth, off, pa, k)
+    if moving:
+        text = text.replace("vel = 0 0 0", "vel = 1.5 0 0", 1).replace("vel = 0 0 0", "vel = -1.5 0 0", 1)
+        text = text.replace("omega = 0 0 0", "omega = 0.3 -0.2 0.1", 1)
+        assert "vel = 1.5 0 0" in text and "vel = -1.5 0 0" in text
     for fn, tris in files.items():
         synth.write_stl_ascii(str(tmp_path / fn), tris)
     cfg = tmp_path / "boxes.txt"

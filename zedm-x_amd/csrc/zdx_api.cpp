@@ -261,6 +261,12 @@ struct zdx_ctx {
             tw.adj_row = 16; tw.slices = 0; tw.slice_seg_max = 0;
         }
     } api, api_na;
+    // wall-contact queries (zdx_wall_contacts)
+    struct WallQ {
+        DBuf<int32_t> mesh, nondeg, isrc, status;
+        DBuf<double> state, wv, aabb, tris, normals, scr, out;
+        DBuf<int64_t> wv_off, wt_off;
+    } wq;
     void* scratch_dev = nullptr;   // this device's `scratch` array
     Arena na_arena;
 
@@ -1325,6 +1331,66 @@ zdx_status zdx_contact_pairs_noalloc(zdx_ctx* c, const zdx_pair_in* pairs, int32
     });
 }
 
+zdx_status zdx_wall_contacts(zdx_ctx* c, const zdx_body_pose* bodies, int32_t n, const double* wall_tris,
+                             const double* wall_normals, int32_t n_tris, int32_t jac, double* out) {
+    if (!c || n < 0 || n_tris < 0 || ((n > 0 && n_tris > 0) && (!bodies || !wall_tris || !wall_normals || !out)))
+        return ZDX_BAD_ARG;
+    return guard(c, [&] {
+        HIPC(hipSetDevice(c->device));
+        if (n == 0 || n_tris == 0) return;
+        for (int b = 0; b < n; ++b)
+            if (bodies[b].mesh < 0 || bodies[b].mesh >= (int)c->meshes.size())
+                throw ApiError{ZDX_BAD_ARG, "body references an unregistered mesh"};
+        c->sync_meshes();
+        hipStream_t st = c->stream;
+        auto& Q = c->wq;
+        std::vector<int32_t> mesh(n);
+        std::vector<double> state(16 * (size_t)n, 0.0);
+        std::vector<int64_t> wvo(n + 1), wto(n + 1);
+        int64_t av = 0, at = 0;
+        int32_t tmax = 1;
+        for (int b = 0; b < n; ++b) {
+            mesh[b] = bodies[b].mesh;
+            for (int k = 0; k < 3; ++k) state[16 * (size_t)b + k] = bodies[b].pos[k];
+            for (int k = 0; k < 4; ++k) state[16 * (size_t)b + 3 + k] = bodies[b].quat[k];
+            wvo[b] = av; wto[b] = at;
+            av += (int64_t)(c->meshes[mesh[b]].h.verts.size() / 3);
+            const int32_t T = (int32_t)(c->meshes[mesh[b]].h.tri_idx.size() / 3);
+            at += T;
+            tmax = std::max(tmax, T);
+        }
+        wvo[n] = av; wto[n] = at;
+        Q.mesh.upload(mesh.data(), n, st);
+        Q.state.upload(state.data(), state.size(), st);
+        Q.wv_off.upload(wvo.data(), n + 1, st);
+        Q.wt_off.upload(wto.data(), n + 1, st);
+        Q.wv.ensure(3 * (size_t)av); Q.aabb.ensure(6 * (size_t)n); Q.nondeg.ensure(n);
+        Q.tris.upload(wall_tris, 9 * (size_t)n_tris, st);
+        Q.normals.upload(wall_normals, 3 * (size_t)n_tris, st);
+        DBodies B;
+        B.n = n; B.mesh = Q.mesh.p; B.state = Q.state.p; B.wv_off = Q.wv_off.p; B.wt_off = Q.wt_off.p;
+        B.wv = Q.wv.p; B.nondeg = Q.nondeg.p; B.aabb = Q.aabb.p;
+        DMeshes M = c->dmeshes();
+        launch_transform(B, M, st);
+        // section points: at most one per edge slot of the body mesh
+        const int32_t cap = 3 * tmax;
+        const int64_t nq = (int64_t)n * n_tris;
+        Q.scr.ensure((size_t)nq * (size_t)(3 * (int64_t)cap + 32 * (int64_t)(cap + 8)));
+        Q.isrc.ensure((size_t)nq * (size_t)(cap + 1));
+        Q.out.ensure(32 * (size_t)nq);
+        Q.status.ensure(1);
+        launch_fill_i32(Q.status.p, 0, 1, st);
+        launch_wall_query(B, M, n, Q.tris.p, Q.normals.p, n_tris, jac ? 1 : 0, cap, Q.scr.p, Q.isrc.p, Q.out.p,
+                          Q.status.p, st);
+        int32_t status = 0;
+        HIPC(hipMemcpyAsync(&status, Q.status.p, 4, hipMemcpyDeviceToHost, st));
+        HIPC(hipMemcpyAsync(out, Q.out.p, 32 * 8 * (size_t)nq, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        if (status & 2) throw ApiError{ZDX_INTERNAL, "wall contact derivative disagreed with the contact"};
+        if (status) throw ApiError{ZDX_INTERNAL, "wall section exceeded its point capacity"};
+    });
+}
+
 zdx_status zdx_contact_times(zdx_ctx* c, double* out, int32_t n) {
     if (!c || (!out && n > 0) || n < 0) return ZDX_BAD_ARG;
     const double v[4] = {c->api_ms[0], c->api_ms[1], c->api_ms[2], (double)c->api_attempts};
@@ -1653,7 +1719,8 @@ zdx_status zdx_sim_stage_times(zdx_sim* s, double* out, int32_t n) {
 }
 
 zdx_status zdx_sim_set_force_model(zdx_sim* s, int32_t model, double k) {
-    if (!s || (model != ZDX_FORCE_ZDEM && model != ZDX_FORCE_MINIMAL)) return ZDX_BAD_ARG;
+    if (!s || (model != ZDX_FORCE_ZDEM && model != ZDX_FORCE_MINIMAL && model != ZDX_FORCE_ZDEM_DAMPED))
+        return ZDX_BAD_ARG;
     if (model == ZDX_FORCE_MINIMAL && !(k > 0.0 && k < 1e300)) {
         s->ctx->err = "the minimal-energy force needs a finite stiffness k > 0";
         return ZDX_BAD_ARG;

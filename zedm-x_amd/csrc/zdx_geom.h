@@ -500,10 +500,13 @@ ZDX_HD inline void basis(const V3& n, V3& u, V3& v) {
 // Scratch (shared by the lanes): pts (3 cap_pts doubles), poly buffers A/B
 // (2 (cap_pts + 8) doubles each).
 // V: the number of world vertices (every triangle corner is one of them).
+// srcs (optional, cap_pts entries): the (triangle, edge) slot 3 t + e each
+// kept section point came from, for wall_contact_jac.
 template <class WP>
 ZDX_HD inline bool wall_contact(const WP& wp, const double* wv, const int* tri, int T, int V, const V3* wt,
                                 const V3& wn, double tol, SArr<double> pts, SArr<double> PA, SArr<double> PB,
-                                int cap_pts, double& area_out, V3& xc_out, V3& n_out, int& status) {
+                                int cap_pts, double& area_out, V3& xc_out, V3& n_out, int& status,
+                                int32_t* srcs = nullptr) {
     const int W = WP::W;
     const int ln = wp.lane();
     V3 pn = wn;
@@ -560,10 +563,12 @@ ZDX_HD inline bool wall_contact(const WP& wp, const double* wv, const int* tri, 
             if (!dup) {
                 if (np >= cap_pts) { status = 1; return false; }
                 pts[3 * np] = q.x; pts[3 * np + 1] = q.y; pts[3 * np + 2] = q.z;
+                if (srcs) srcs[np] = base + src;
                 ++np;
             }
         }
     }
+    if (srcs) srcs[cap_pts] = np;   // the count, after the slots
     if (np < 3) return false;
     V3 bu, bv;
     basis(pn, bu, bv);
@@ -624,6 +629,134 @@ ZDX_HD inline bool wall_contact(const WP& wp, const double* wv, const int* tri, 
     return true;
 }
 
+// Derivatives of a wall contact (area and contact point of wall_contact)
+// with respect to a rigid motion xi = (d, w) of the particle: every world
+// vertex p moves by d + w x (p - x).  One thread; it repeats wall_contact's
+// arithmetic on the kept section points (srcs: their edge slots and count,
+// as wall_contact recorded them) with six tangents carried beside each value,
+// so every branch (clip tests, area sign) is decided on the same doubles.
+// A kept point p = p0 + t (p1 - p0), t = d0 / (d0 - d1) on the edge (p0,
+// p1) with signed distances d0, d1 to the wall plane, has the tangent
+// dp0 + t (dp1 - dp0) + (p1 - p0) dt, dt = (d0 dd1 - d1 dd0) / (d0 - d1)^2.
+// Scratch T: 14 (cap_pts + 8) doubles per polygon buffer, two buffers.
+// dA: 6 doubles; dX: 3 x 6 (row-major).  Returns false when wall_contact
+// would (no contact).
+ZDX_HD inline bool wall_contact_jac(const double* wv, const int* tri, const V3& x, const V3* wt, const V3& wn,
+                                    double tol, const int32_t* srcs, int cap_pts, SArr<double> TA, SArr<double> TB,
+                                    double* dA, double* dX) {
+    const int np = srcs[cap_pts];
+    if (np < 3) return false;
+    V3 pn = wn;
+    double pd = -dot(pn, wt[0]);
+    V3 bu, bv;
+    basis(pn, bu, bv);
+    V3 o = wt[0];
+    // polygon vertex m: [0] u, [1] v, [2..8) du/dxi, [8..14) dv/dxi
+    for (int m = 0; m < np; ++m) {
+        const int f = srcs[m];
+        const int t = f / 3, i = f - 3 * t;
+        const int i0 = tri[3 * t + i], i1 = tri[3 * t + (i + 1) % 3];
+        V3 p0 = v3(wv[3 * i0], wv[3 * i0 + 1], wv[3 * i0 + 2]);
+        V3 p1 = v3(wv[3 * i1], wv[3 * i1 + 1], wv[3 * i1 + 2]);
+        double d0 = dot(pn, p0) + pd, d1 = dot(pn, p1) + pd;
+        double tt = d0 / (d0 - d1);
+        const bool clamped = tt < 0.0 || tt > 1.0;
+        tt = dmin(1.0, dmax(0.0, tt));
+        V3 p = add(p0, mul(sub(p1, p0), tt));
+        V3 e = sub(p1, p0);
+        V3 r0 = sub(p0, x), r1 = sub(p1, x);
+        V3 dd = sub(p, o);
+        TA[14 * m] = dot(dd, bu);
+        TA[14 * m + 1] = dot(dd, bv);
+        const double den = (d0 - d1) * (d0 - d1);
+        for (int j = 0; j < 6; ++j) {
+            V3 a = j < 3 ? v3(j == 0, j == 1, j == 2) : cross(v3(j == 3, j == 4, j == 5), r0);
+            V3 b = j < 3 ? a : cross(v3(j == 3, j == 4, j == 5), r1);
+            const double dtt = clamped ? 0.0 : (d0 * dot(pn, b) - d1 * dot(pn, a)) / den;
+            V3 dp = add(add(a, mul(sub(b, a), tt)), mul(e, dtt));
+            TA[14 * m + 2 + j] = dot(dp, bu);
+            TA[14 * m + 8 + j] = dot(dp, bv);
+        }
+    }
+    V2 clip[3];
+    for (int k = 0; k < 3; ++k) { V3 d = sub(wt[k], o); clip[k] = v2(dot(d, bu), dot(d, bv)); }
+    // sutherland_hodgman_clip 1093-1122 with tangents
+    int nin = np;
+    SArr<double> in = TA, outp = TB;
+    V2 c1 = clip[2];
+    auto emit = [&](int& nout, const double* src14) {
+        for (int q = 0; q < 14; ++q) outp[14 * nout + q] = src14[q];
+        ++nout;
+    };
+    for (int ck = 0; ck < 3; ++ck) {
+        V2 c2 = clip[ck];
+        if (nin == 0) break;
+        int nout = 0;
+        int sidx = nin - 1;
+        for (int m = 0; m < nin; ++m) {
+            V2 e = v2(in[14 * m], in[14 * m + 1]);
+            V2 sv = v2(in[14 * sidx], in[14 * sidx + 1]);
+            const bool ein = inside_clip(e, c1, c2), sin_ = inside_clip(sv, c1, c2);
+            if (ein != sin_) {
+                // the crossing of the edge (sv, e) with the clip line
+                double rec[14];
+                V2 dc = sub2(c1, c2), dp = sub2(sv, e);
+                double n1 = cr2(c1, c2), n2 = cr2(sv, e), n3 = cr2(dc, dp);
+                if (fabs(n3) < 1e-30) {
+                    for (int q = 0; q < 14; ++q) rec[q] = in[14 * sidx + q];
+                } else {
+                    const double X = (n1 * dp.x - n2 * dc.x) / n3, Y = (n1 * dp.y - n2 * dc.y) / n3;
+                    rec[0] = X;
+                    rec[1] = Y;
+                    for (int j = 0; j < 6; ++j) {
+                        V2 dsv = v2(in[14 * sidx + 2 + j], in[14 * sidx + 8 + j]);
+                        V2 de = v2(in[14 * m + 2 + j], in[14 * m + 8 + j]);
+                        V2 ddp = sub2(dsv, de);
+                        const double dn2 = cr2(dsv, e) + cr2(sv, de), dn3 = cr2(dc, ddp);
+                        rec[2 + j] = (n1 * ddp.x - dn2 * dc.x - X * dn3) / n3;
+                        rec[8 + j] = (n1 * ddp.y - dn2 * dc.y - Y * dn3) / n3;
+                    }
+                }
+                if (nout >= cap_pts + 6) return false;
+                emit(nout, rec);
+            }
+            if (ein) {
+                if (nout >= cap_pts + 6) return false;
+                double rec[14];
+                for (int q = 0; q < 14; ++q) rec[q] = in[14 * m + q];
+                emit(nout, rec);
+            }
+            sidx = m;
+        }
+        SArr<double> tmp = in; in = outp; outp = tmp;
+        nin = nout;
+        c1 = c2;
+    }
+    if (nin < 3) return false;
+    double a = 0.0;
+    double da[6] = {0, 0, 0, 0, 0, 0};
+    for (int m = 0; m < nin; ++m) {
+        int j = (m + 1) % nin;
+        V2 pm = v2(in[14 * m], in[14 * m + 1]), pj = v2(in[14 * j], in[14 * j + 1]);
+        a += cr2(pm, pj);
+        for (int q = 0; q < 6; ++q)
+            da[q] += cr2(v2(in[14 * m + 2 + q], in[14 * m + 8 + q]), pj) +
+                     cr2(pm, v2(in[14 * j + 2 + q], in[14 * j + 8 + q]));
+    }
+    const double sg = a < 0.0 ? -0.5 : 0.5;
+    a = fabs(a) * 0.5;
+    if (a < tol * tol * 0.01) return false;
+    const double nn = (double)nin;
+    for (int q = 0; q < 6; ++q) {
+        double su = 0.0, sv2 = 0.0;
+        for (int m = 0; m < nin; ++m) { su += in[14 * m + 2 + q]; sv2 += in[14 * m + 8 + q]; }
+        dA[q] = sg * da[q];
+        V3 dxq = add(mul(bu, su / nn), mul(bv, sv2 / nn));
+        dX[q] = dxq.x; dX[6 + q] = dxq.y; dX[12 + q] = dxq.z;
+    }
+    return true;
+}
+
 // ------------------------------------------------------------ force laws
 struct Mat {
     double eqr, young, poisson, mu, rest, inv_mass, mass;
@@ -635,9 +768,11 @@ struct Contact {
 };
 
 // particle-particle force, main.cpp:2220-2278; ds is the pair's tangential
-// spring, read-modify-written in loop order.
+// spring, read-modify-written in loop order.  damped: add the normal damping
+// the reference computes and leaves commented out (main.cpp:2244-2248,
+// 2258-2260), -cn vn while the bodies approach (vn < 0).
 ZDX_HD inline Contact pp_force(const V3& nA, double area, const V3& xc, const Body& a, const Body& b, const Mat& ma,
-                              const Mat& mb, double dt, V3& ds) {
+                              const Mat& mb, double dt, V3& ds, bool damped = false) {
     V3 rA = sub(xc, a.pos), rB = sub(xc, b.pos);
     V3 vA = add(a.vel, cross(a.omega, rA)), vB = add(b.vel, cross(b.omega, rB));
     V3 vrel = sub(vB, vA);
@@ -651,10 +786,13 @@ ZDX_HD inline Contact pp_force(const V3& nA, double area, const V3& xc, const Bo
     double e = dmin(ma.rest, mb.rest);
     e = dmin(0.9999, dmax(1e-6, e));
     double le = log(e);
+    double cn = 2.0 * le * sqrt(kn * meff) / sqrt(le * le + 3.141592653589793 * 3.141592653589793);
     double ct = 2.0 * le * sqrt(kt * meff) / sqrt(le * le + 3.141592653589793 * 3.141592653589793);
+    cn = fabs(cn);
     ct = fabs(ct);
     double dV = pow(area, 1.5);
     double fn = kn * cbrt(dV);
+    if (damped && vn < 0.0) fn += -cn * vn;
     if (fn < 0.0) fn = 0.0;
     V3 fnv = mul(nA, fn);
     V3 vt = sub(vrel, mul(nA, vn));
@@ -675,32 +813,18 @@ ZDX_HD inline Contact pp_force(const V3& nA, double area, const V3& xc, const Bo
 }
 
 // Minimal-energy particle-particle force of the time step (force model
-// "minimal"): the elastic part of polyline.py:655-701, Fn = k/2 |Sn| along the
-// A->B normal nA, plus normal damping set by the restitution, c = 2 zeta
-// sqrt(m_eff k_c) with zeta = -ln e / sqrt(pi^2 + ln^2 e) and the contact
-// stiffness of a sphere of the effective radius R = R1 R2 / (R1 + R2),
-// k_c = pi k R.  The lever arm is the loop centroid: xc0 = (nA x Gn)/|Sn| is
-// -2/3 of the centroid's part normal to nA (Stokes), so -3/2 xc0 is the
-// centroid up to a multiple of nA, which neither the torque of a force along
-// nA nor v_rel . nA sees.  No tangential force, no history.
+// "minimal"): polyline.py:655-701, Fn = k/2 |Sn| along the A->B normal nA,
+// with no damping and no tangential force.  The lever arm is the loop
+// centroid: xc0 = (nA x Gn)/|Sn| is -2/3 of the centroid's part normal to nA
+// (Stokes), so -3/2 xc0 is the centroid up to a multiple of nA, which the
+// torque of a force along nA does not see.
 ZDX_HD inline Contact pp_force_minimal(const V3& nA, double area, const V3& xc0, const Body& a, const Body& b,
-                                      const Mat& ma, const Mat& mb, double k) {
+                                      double k) {
     V3 xc = mul(xc0, -1.5);
-    V3 rA = sub(xc, a.pos), rB = sub(xc, b.pos);
-    V3 vA = add(a.vel, cross(a.omega, rA)), vB = add(b.vel, cross(b.omega, rB));
-    double vn = dot(sub(vB, vA), nA);
-    double R1 = dmax(ma.eqr, 1e-12), R2 = dmax(mb.eqr, 1e-12);
-    double R = R1 * R2 / (R1 + R2);
-    double meff = 1.0 / (ma.inv_mass + mb.inv_mass);
-    double e = dmin(0.9999, dmax(1e-6, dmin(ma.rest, mb.rest)));
-    double le = log(e);
-    double zeta = -le / sqrt(3.141592653589793 * 3.141592653589793 + le * le);
-    double c = 2.0 * zeta * sqrt(meff * (3.141592653589793 * k * R));
-    double fn = 0.5 * k * area - c * vn;
     Contact r;
-    r.f = mul(nA, fn);
-    r.r1 = rA;
-    r.r2 = rB;
+    r.f = mul(nA, 0.5 * k * area);
+    r.r1 = sub(xc, a.pos);
+    r.r2 = sub(xc, b.pos);
     return r;
 }
 

@@ -307,7 +307,11 @@ bool parse_config(const std::string& path, SimConfig& c) {
         else if (k == "contact_debug") { int v = 0; is >> v; c.contact_debug = (v != 0); }
         else if (k == "gravity") is >> c.gravity[0] >> c.gravity[1] >> c.gravity[2];
         else if (k == "center_mesh") { int v = 1; is >> v; c.center = (v != 0); }
-        else if (k == "force_model") { std::string v; is >> v; c.force_model = (v == "minimal") ? 1 : 0; }
+        else if (k == "force_model") {
+            std::string v;
+            is >> v;
+            c.force_model = v == "minimal" ? 1 : v == "zdem_damped" ? 2 : 0;
+        }
         else if (k == "minimal_k") is >> c.k_minimal;
         else if (k == "vtk_prefix") is >> c.vtk_prefix;
         else if (k == "output_interval") is >> c.output_interval;

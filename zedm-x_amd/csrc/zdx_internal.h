@@ -47,7 +47,8 @@ struct SimConfig {
     std::string stl, vtk_prefix = "particles", output_dir = "output";
     int output_interval = 1;
     double v0[3] = {0, 0, 0}, v1[3] = {0, 0, 0};
-    int force_model = 0;        // build extension: "force_model minimal" (the reference ignores the key)
+    int force_model = 0;        // build extension: "force_model minimal" / "zdem_damped" (the reference
+                                // ignores the key)
     double k_minimal = 0.0;     // "minimal_k <k>"
     std::vector<ParticleInit> parts;
     std::vector<WallInit> walls;

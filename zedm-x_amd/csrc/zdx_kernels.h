@@ -124,7 +124,8 @@ struct SimParams {
     double g[3];
     double dt;
     int32_t split;
-    int32_t force_model;   // 0: the reference's particle-particle force, 1: minimal-energy force
+    int32_t force_model;   // 0: the reference's particle-particle force, 1: minimal-energy force,
+                           // 2: the reference's force with its dormant normal damping
     double k_minimal;      // stiffness k of the minimal-energy force
 };
 
@@ -248,6 +249,17 @@ void launch_api_pack(const ApiPack& k, hipStream_t s);
 // 2n + 1 entries: the last holds the totals).
 void launch_api_bodies(const void* in, int32_t n, const DMeshes& M, int32_t* mesh, double* state, int64_t* wv_off,
                        int64_t* wt_off, hipStream_t s);
+// Wall contacts of bodies against wall triangles (the per-triangle part of
+// the reference's particle-wall contact, src/main.cpp:2325-2369, without its
+// bounding-sphere and box pre-tests): one wave per (body b, triangle t);
+// out[32 (b nt + t) ...]: flag, area, xc(3), normal(3, the wall normal turned
+// towards the body's centre, main.cpp:2363-2367), and with jac d area/dxi
+// (6) and d xc/dxi (3 x 6) for rigid motions xi = (d, w) of the body.
+// scr: per wave 3 cap + 4 (cap + 8) + 28 (cap + 8) doubles and cap + 1 ints
+// (in isrc).  status bit 1: a section held more than cap points.
+void launch_wall_query(const DBodies& B, const DMeshes& M, int32_t n_bodies, const double* tris, const double* normals,
+                       int32_t n_tris, int32_t jac, int32_t cap, double* scr, int32_t* isrc, double* out,
+                       int32_t* status, hipStream_t s);
 // *out = OR of v[0..n) (out is cleared first)
 void launch_or_i32(const int32_t* v, int32_t n, int32_t* out, hipStream_t s);
 

@@ -1243,8 +1243,8 @@ struct SimSink {
     double k_minimal;
     __device__ int operator()(int li, int, int, const V3&, const V3&, const V3& nA, const V3& xc, double area, int,
                               SArr<int>, SArr<double>, const double*) {
-        Contact c = model == 1 ? pp_force_minimal(nA, area, xc, *a, *b, *ma, *mb, k_minimal)
-                               : pp_force(nA, area, xc, *a, *b, *ma, *mb, dt, *ds);
+        Contact c = model == 1 ? pp_force_minimal(nA, area, xc, *a, *b, k_minimal)
+                               : pp_force(nA, area, xc, *a, *b, *ma, *mb, dt, *ds, model == 2);
         if (li >= lmax) return TOPO_LOOP_OVERFLOW;
         double* o = contrib + 9 * (int64_t)li;
         o[0] = c.f.x; o[1] = c.f.y; o[2] = c.f.z;
@@ -2175,6 +2175,50 @@ __global__ void __launch_bounds__(1024) k_api_bodies(const zdx_pair_in* in, int3
     }
 }
 
+__global__ void __launch_bounds__(64) k_wall_query(DBodies B, DMeshes M, int32_t n_bodies, const double* tris,
+                                                   const double* normals, int32_t n_tris, int32_t jac, int32_t cap,
+                                                   double* scr, int32_t* isrc, double* out, int32_t* status) {
+    LaneDev wp;
+    const int64_t q = blockIdx.x;
+    if (q >= (int64_t)n_bodies * n_tris) return;
+    const int b = (int)(q / n_tris), t = (int)(q - (int64_t)b * n_tris);
+    const int64_t per = 3 * (int64_t)cap + 4 * (int64_t)(cap + 8) + 28 * (int64_t)(cap + 8);
+    SArr<double> PTS; PTS.p = scr + q * per; PTS.s = 1;
+    SArr<double> PA = PTS.at(3 * (int64_t)cap);
+    SArr<double> PB = PA.at(2 * (int64_t)(cap + 8));
+    SArr<double> TA = PB.at(2 * (int64_t)(cap + 8));
+    SArr<double> TB = TA.at(14 * (int64_t)(cap + 8));
+    int32_t* src = isrc + q * (int64_t)(cap + 1);
+    const int m = B.mesh[b];
+    const double* wv = B.wv + 3 * B.wv_off[b];
+    const int32_t* ti = M.tri_idx + 3 * (int64_t)M.tri_off[m];
+    const double* wt9 = tris + 9 * (int64_t)t;
+    V3 wt[3] = {ld3(wt9), ld3(wt9 + 3), ld3(wt9 + 6)};
+    const V3 wn = ld3(normals + 3 * (int64_t)t);
+    double area = 0.0;
+    V3 xc, cn;
+    int st = 0;
+    const bool ok = wall_contact(wp, wv, ti, M.ntris[m], M.nverts[m], wt, wn, M.tol[m], PTS, PA, PB, cap, area, xc,
+                                 cn, st, src);
+    double* o = out + 32 * q;
+    if (lane_id() != 0) return;
+    for (int k = 0; k < 32; ++k) o[k] = 0.0;
+    if (!ok) {
+        if (st) atomicOr(status, 1);
+        return;
+    }
+    const Body p = load_body(B.state + 16 * (int64_t)b);
+    if (dot(cn, sub(p.pos, xc)) < 0.0) cn = mul(cn, -1.0);   // main.cpp:2363-2367
+    o[0] = 1.0; o[1] = area;
+    o[2] = xc.x; o[3] = xc.y; o[4] = xc.z;
+    o[5] = cn.x; o[6] = cn.y; o[7] = cn.z;
+    if (jac) {
+        if (!wall_contact_jac(wv, ti, p.pos, wt, wn, M.tol[m], src, cap, TA, TB, o + 8, o + 14)) {
+            atomicOr(status, 2);   // cannot differ from wall_contact: same doubles, same branches
+        }
+    }
+}
+
 __global__ void k_or_i32(const int32_t* v, int32_t n, int32_t* out) {
     int32_t acc = 0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -2421,6 +2465,15 @@ void launch_api_bodies(const void* in, int32_t n, const DMeshes& M, int32_t* mes
 void launch_or_i32(const int32_t* v, int32_t n, int32_t* out, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, out, 0, nullptr, (int64_t)1);
     if (n > 0) hipLaunchKernelGGL(k_or_i32, dim3(std::min(nblk(n, 256), 1024)), dim3(256), 0, s, v, n, out);
+}
+
+void launch_wall_query(const DBodies& B, const DMeshes& M, int32_t n_bodies, const double* tris, const double* normals,
+                       int32_t n_tris, int32_t jac, int32_t cap, double* scr, int32_t* isrc, double* out,
+                       int32_t* status, hipStream_t s) {
+    const int64_t nq = (int64_t)n_bodies * n_tris;
+    if (nq <= 0) return;
+    hipLaunchKernelGGL(k_wall_query, dim3((unsigned)nq), dim3(64), 0, s, B, M, n_bodies, tris, normals, n_tris, jac,
+                       cap, scr, isrc, out, status);
 }
 
 }  // namespace zdx

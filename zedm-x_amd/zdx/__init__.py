@@ -20,11 +20,11 @@ CLI_PATH = os.path.join(PKG, "bin", "zdx")
 
 OK, BAD_ARG, IO, CAPACITY, HIP_ERROR, INTERNAL = range(6)
 MODE_PLAIN, MODE_BATCHED = 0, 1
-FORCE_ZDEM, FORCE_MINIMAL = 0, 1
+FORCE_ZDEM, FORCE_MINIMAL, FORCE_ZDEM_DAMPED = 0, 1, 2
 COUNTERS = ["contacts", "tri_checks", "segments", "comps", "loops", "pairs", "wall_contacts", "tri_tests", "tri_hits",
             "tri_scanned", "topo_pairs", "seg_max"]
 EXPORTS = ["zdx_create", "zdx_destroy", "zdx_last_error", "zdx_load_stl", "zdx_mesh_register", "zdx_contact_pairs",
-           "zdx_contact_times", "zdx_contact_pairs_noalloc",
+           "zdx_contact_times", "zdx_contact_pairs_noalloc", "zdx_wall_contacts",
            "zdx_sim_load_config", "zdx_sim_create", "zdx_sim_destroy", "zdx_sim_num_particles", "zdx_sim_step",
            "zdx_sim_get_state", "zdx_sim_set_state", "zdx_sim_get_forces", "zdx_sim_get_pairs", "zdx_sim_write_vtk",
            "zdx_sim_write_state_txt", "zdx_sim_checkpoint", "zdx_sim_restore",
@@ -51,6 +51,11 @@ PAIR_IN_DTYPE = np.dtype({"names": ["mesh_a", "mesh_b", "pos_a", "quat_a", "pos_
                           "offsets": [PairIn.mesh_a.offset, PairIn.mesh_b.offset, PairIn.pos_a.offset,
                                       PairIn.quat_a.offset, PairIn.pos_b.offset, PairIn.quat_b.offset],
                           "itemsize": C.sizeof(PairIn)})
+
+
+# zdx_body_pose (mesh, reserved, pos[3], quat[4]: 64 bytes)
+BODY_POSE_DTYPE = np.dtype({"names": ["mesh", "pos", "quat"], "formats": [np.int32, (np.float64, 3), (np.float64, 4)],
+                            "offsets": [0, 8, 32], "itemsize": 64})
 
 
 class ContactOpts(C.Structure):
@@ -95,6 +100,7 @@ def lib():
         L.zdx_contact_pairs.argtypes = [vp, vp, i32, C.POINTER(ContactOpts), C.POINTER(PairOut)]
         L.zdx_contact_times.argtypes = [vp, vp, i32]
         L.zdx_contact_pairs_noalloc.argtypes = [vp, vp, i32, C.POINTER(ContactOpts), C.POINTER(PairOut)]
+        L.zdx_wall_contacts.argtypes = [vp, vp, i32, vp, vp, i32, i32, vp]
         L.zdx_sim_load_config.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(vp)]
         L.zdx_sim_create.argtypes = [vp, C.POINTER(SimDesc), C.POINTER(vp)]
         L.zdx_sim_destroy.argtypes = [vp]
@@ -232,6 +238,24 @@ class Context:
             return res
         raise ZdxError(CAPACITY, "contact_pairs output did not fit")
 
+    def wall_contacts(self, mesh, pos, quat, wall_tris, wall_normals, jac=False) -> np.ndarray:
+        """Contact of each body (mesh id, pos, quat per body) with each wall
+        triangle (world coordinates, (W, 3, 3), normals (W, 3)): records
+        (n, W, 32) -- flag, area, point, normal, then with jac d area/dxi and
+        d point/dxi for rigid motions xi = (d, w) of the body."""
+        n = len(mesh)
+        wt = np.ascontiguousarray(wall_tris, dtype=np.float64).reshape(-1, 9)
+        wn = np.ascontiguousarray(wall_normals, dtype=np.float64).reshape(-1, 3)
+        arr = np.zeros(max(n, 1), dtype=BODY_POSE_DTYPE)
+        if n:
+            arr["mesh"][:n] = np.asarray(mesh, dtype=np.int32)
+            arr["pos"][:n] = np.asarray(pos, dtype=np.float64).reshape(n, 3)
+            arr["quat"][:n] = np.asarray(quat, dtype=np.float64).reshape(n, 4)
+        out = np.zeros((max(n, 1), max(len(wt), 1), 32))
+        self._check(self._L.zdx_wall_contacts(self.h, arr.ctypes.data, n, wt.ctypes.data, wn.ctypes.data, len(wt),
+                                              int(bool(jac)), out.ctypes.data))
+        return out[:n, :len(wt)]
+
     def contact_times(self) -> dict:
         """Device ms of the last contact_pairs call (transform, narrow phase,
         topology) and its narrow-phase attempts."""
@@ -329,8 +353,9 @@ class Sim:
         self.ctx._check(self.ctx._L.zdx_sim_restore(self.h, path.encode()))
 
     def set_force_model(self, model: int, k: float = 0.0) -> None:
-        """FORCE_ZDEM (the reference's particle-particle force) or FORCE_MINIMAL
-        (Fn = k/2 |Sn| plus restitution damping, at the loop centroid)."""
+        """FORCE_ZDEM (the reference's particle-particle force), FORCE_MINIMAL
+        (Fn = k/2 |Sn| at the loop centroid) or FORCE_ZDEM_DAMPED (the
+        reference's force with its dormant normal damping)."""
         self.ctx._check(self.ctx._L.zdx_sim_set_force_model(self.h, int(model), float(k)))
 
     STAGES = ["transform", "broadphase", "narrow", "topology", "walls", "fold_integrate"]

@@ -4,7 +4,8 @@ granular_bed():   100k polyhedra from 8 templates in a 0.2 m box with a
                   12-triangle box wall (the step throughput workload)
 uneven_pairs():   4096 independent pairs, icospheres of 80-5120 triangles and
                   concave stars (the batched per-pair workload)
-drop_on_plane():  64 convex hulls above a plane wall (the differentiable drop)
+drop_on_plane():  64 convex hulls dropped onto a plane wall, colliding with it
+                  and with each other (the differentiable drop)
 concave_packing(): 1M concave stars of 320-2560 triangles in a 0.5 m box (the
                   large single-GPU configuration)
 
@@ -171,24 +172,31 @@ def uneven_pairs(n_pairs: int = 4096, seed: int = 2):
     return meshes, names, pairs
 
 
-def drop_on_plane(n: int = 64, seed: int = 4, radius: float = 5e-3, height: float = 0.2e-3):
-    """64 convex hulls of 128 points (radius 5 mm) on a 8x8 grid just above a
-    2-triangle plane wall at z = 0, gravity -9.81 z, dt 1e-5, 500 steps."""
+def drop_on_plane(n: int = 64, seed: int = 4, radius: float = 5e-3, height: float = 0.2e-3, gap: float = 0.2e-3,
+                  speed: float = 0.1):
+    """n convex hulls of 128 points (radius 5 mm) on a square grid just above
+    a 2-triangle plane wall at z = 0, gravity -9.81 z, dt 1e-5, 500 steps.
+    The grid spacing leaves `gap` between neighbouring bounding spheres and
+    the bodies start with Gaussian horizontal velocities of `speed` m/s per
+    component and -0.05 m/s downwards, so within the 500 steps they hit the
+    wall and each other."""
     hull = synth.sphere_hull(128, seed)
     rt = _centred_radius(hull)
     scale = round(radius / rt, 6)
     g = np.random.Generator(np.random.PCG64(seed))
     side = int(math.ceil(math.sqrt(n)))
     k = np.arange(n)
-    pos = np.stack([(k % side) * 2.4 * radius, (k // side) * 2.4 * radius, np.full(n, radius + height)], axis=1)
+    s = 2.0 * radius + gap
+    pos = np.stack([(k % side) * s, (k // side) * s, np.full(n, radius + height)], axis=1)
     pos[:, 2] += g.uniform(0, 0.5e-3, n)
     q = g.normal(size=(n, 4))
     q /= np.linalg.norm(q, axis=1, keepdims=True)
     vel = np.zeros((n, 3))
+    vel[:, 0:2] = g.normal(size=(n, 2)) * speed
     vel[:, 2] = -0.05
     init = np.concatenate([pos, q, vel, np.zeros((n, 3))], axis=1)
     mat = np.tile(np.array([2500.0, 1e7, 0.25, 0.5, 0.5]), (n, 1))
-    wall = (synth.plane_wall(1.0, 1.0), np.array([side * 1.2 * radius, side * 1.2 * radius, 0.0]),
+    wall = (synth.plane_wall(1.0, 1.0), np.array([(side - 1) * s / 2, (side - 1) * s / 2, 0.0]),
             np.array([1.0, 0.0, 0.0, 0.0]), 0.5, 0.5, "plane")
     return Workload("drop_on_plane", [hull], ["hull128"], np.zeros(n, dtype=np.int64), np.full(n, scale), init, mat,
                     [wall], (0.0, 0.0, -9.81), 1e-5, 500, True, dict(seed=seed))
