@@ -15,6 +15,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/zdx.h"
@@ -228,6 +229,7 @@ struct zdx_ctx {
         DBuf<unsigned long long> hit_top;
         DBuf<int64_t> hit_off;
         DBuf<unsigned int> queue;
+        DBuf<int32_t> heavy;
         DBuf<int32_t> bcnt, bcur;
         DBuf<int32_t> ncontrib, comps, loops_raw, tstatus, topo_list, topo_n, status2;
         TopoWork tw;
@@ -247,7 +249,7 @@ struct zdx_ctx {
             f(pairs_in); f(mesh); f(state); f(wv_off); f(wt_off); f(wv); f(aabb); f(nondeg); f(pi); f(pj); f(order);
             f(npairs); f(status); f(items); f(seg_off); f(tri_checks); f(tri_tests); f(nseg); f(tri_hits);
             f(tri_scanned); f(arena); f(list_i); f(list_f); f(hits); f(hit_tri); f(hit_flag); f(hit_top); f(hit_off);
-            f(queue); f(bcnt); f(bcur); f(ncontrib); f(comps); f(loops_raw); f(tstatus); f(topo_list); f(topo_n);
+            f(queue); f(heavy); f(bcnt); f(bcur); f(ncontrib); f(comps); f(loops_raw); f(tstatus); f(topo_list); f(topo_n);
             f(status2); f(rec); f(vtx); f(prov); f(ej); f(vj); f(ljac); f(pk_loop_off); f(pk_seg_off);
             f(pk_vtx_off); f(pk_pvtx); f(pk_tot); f(pk_rec); f(pk_ljac); f(pk_vtx); f(pk_seg);
             tw.each_buf(f);
@@ -283,16 +285,29 @@ struct zdx_ctx {
         std::vector<int32_t> t, voff, nv, toff, nt;
         std::vector<double> tol, aeps;
         std::vector<int32_t> soff, ns, susp;
+        // meshes with the same triangle indices (scaled copies of one shape)
+        // share one copy on the device, so that the narrow phase's index
+        // loads stay in cache
+        std::unordered_map<uint64_t, std::vector<int32_t>> seen;   // hash -> first triangles of index arrays
         for (auto& m : meshes) {
             soff.push_back((int32_t)susp.size());
             ns.push_back((int32_t)m.h.suspect.size());
             susp.insert(susp.end(), m.h.suspect.begin(), m.h.suspect.end());
             voff.push_back((int32_t)(v.size() / 3));
             nv.push_back((int32_t)(m.h.verts.size() / 3));
-            toff.push_back((int32_t)(t.size() / 3));
             nt.push_back((int32_t)(m.h.tri_idx.size() / 3));
             v.insert(v.end(), m.h.verts.begin(), m.h.verts.end());
-            t.insert(t.end(), m.h.tri_idx.begin(), m.h.tri_idx.end());
+            uint64_t hsh = 1469598103934665603ull ^ m.h.tri_idx.size();
+            for (int32_t x : m.h.tri_idx) hsh = (hsh ^ (uint32_t)x) * 1099511628211ull;
+            int32_t at = -1;
+            for (int32_t o : seen[hsh])
+                if (std::equal(m.h.tri_idx.begin(), m.h.tri_idx.end(), t.begin() + 3 * (size_t)o)) { at = o; break; }
+            if (at < 0) {
+                at = (int32_t)(t.size() / 3);
+                seen[hsh].push_back(at);
+                t.insert(t.end(), m.h.tri_idx.begin(), m.h.tri_idx.end());
+            }
+            toff.push_back(at);
             tol.push_back(m.tol);
             aeps.push_back(m.area_eps);
         }
@@ -373,6 +388,16 @@ bool tiny_start_capacity() {
 
 int32_t persistent_groups(const zdx_ctx* c, int per_cu) { return std::max(1, c->n_cu * per_cu); }
 
+// The hit arena after an overflow: each shard's counter holds what its pairs
+// asked for, so twice the largest one sizes every shard (the shards' loads
+// differ and change from step to step; the arena at least doubles).
+int64_t grown_hit_cap(const unsigned long long* d_top, int shards, int64_t cap) {
+    std::vector<unsigned long long> top(shards, 0);
+    HIPC(hipMemcpy(top.data(), d_top, shards * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    const unsigned long long want = *std::max_element(top.begin(), top.end());
+    return std::max<int64_t>(2 * cap, (int64_t)shards * (int64_t)(2 * want + 1));
+}
+
 }  // namespace
 
 // ======================================================================= sim
@@ -428,6 +453,7 @@ struct zdx_sim {
     int32_t tmax = 1;            // largest triangle count of the meshes in use
     uint64_t mesh_version = 0;   // context mesh upload the graphs were captured with
     DBuf<unsigned int> d_queue;
+    DBuf<int32_t> d_heavy;
     // topology
     DBuf<double> d_contrib;
     DBuf<int32_t> d_ncontrib, d_comps, d_loops_raw, d_tstatus;
@@ -522,7 +548,9 @@ struct zdx_sim {
         P.topo_list = d_topo_list.p; P.topo_n = d_topo_n.p;
         P.topo_big = d_topo_big.p; P.topo_small = d_topo_small.p; P.n_big = d_topo_nbs.p; P.n_small = d_topo_nbs.p + 1;
         P.hits = d_hits.p; P.hit_cap = hit_cap; P.hit_top = d_hit_top.p; P.hit_off = d_hit_off.p;
+        P.hit_shards = (int32_t)std::min<int64_t>(HIT_SHARDS, hit_cap);
         P.hit_flag = d_hit_flag.p;
+        P.heavy = d_heavy.p;
         return P;
     }
     HistBufs hist(int par) {
@@ -581,9 +609,9 @@ struct zdx_sim {
             HIPC(hipMemcpy(ps[k].n.p, &n, sizeof(int32_t), hipMemcpyHostToDevice));
         }
         d_adj_j.ensure(adj_cap); d_adj_pair.ensure(adj_cap);
-        d_order.ensure(cap); d_items.ensure(cap); d_topo_list.ensure(cap); d_topo_n.ensure(1);
+        d_order.ensure(cap); d_items.ensure(cap); d_heavy.ensure(cap); d_topo_list.ensure(cap); d_topo_n.ensure(1);
         d_topo_big.ensure(cap); d_topo_small.ensure(cap); d_topo_nbs.ensure(2);
-        d_hit_off.ensure(cap); d_hit_top.ensure(1);
+        d_hit_off.ensure(cap); d_hit_top.ensure(HIT_SHARDS);
         d_seg_off.ensure(cap); d_tri_checks.ensure(cap); d_tri_tests.ensure(cap);
         d_nseg.ensure(cap); d_tri_hits.ensure(cap); d_tri_scanned.ensure(cap);
         d_ncontrib.ensure(cap); d_comps.ensure(cap); d_loops_raw.ensure(cap); d_tstatus.ensure(cap);
@@ -661,7 +689,7 @@ void sim_setup_device(zdx_sim* s) {
     s->d_sorted.ensure(N); s->d_nbr.ensure(16 * (size_t)N); s->d_cnt_up.ensure(N); s->d_cnt_all.ensure(N);
     s->d_pair_off.ensure(N + 1); s->d_adj_off.ensure(N + 1); s->d_n_adj.ensure(1);
     s->d_scan_tmp.ensure((size_t)(2 * (std::max<int64_t>(cells, N) / 1024 + 4)));
-    s->d_status.ensure(1); s->d_ctr.ensure(16); s->d_queue.ensure(2);
+    s->d_status.ensure(1); s->d_ctr.ensure(16); s->d_queue.ensure(4);
     s->d_bcnt.ensure(64); s->d_bcur.ensure(64);
     s->d_kt.ensure(12);
     // the cell counters return to zero within every broad phase (k_bp_scatter)
@@ -687,7 +715,7 @@ void sim_setup_device(zdx_sim* s) {
     s->d_list_f.ensure(groups * 12 * (size_t)s->tmax);
     s->topo_threads = std::min<int64_t>(2048, std::max<int64_t>(64, (int64_t)N));
     s->alloc_stage(s->stage_cap);
-    s->alloc_hits(tiny ? 64 : std::max<int64_t>(1 << 16, 16 * (int64_t)N));
+    s->alloc_hits(tiny ? 64 : std::max<int64_t>(1 << 16, 32 * (int64_t)N));
     s->tw.ensure_bump();   // before any graph capture
     // particle/wall histories start empty
     for (int k = 0; k < 2; ++k) {
@@ -787,10 +815,10 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
         z.i32[1] = s->d_n_need.p; z.n32[1] = 1;
         z.i32[2] = s->d_n_wall.p; z.n32[2] = 1;
         z.i32[3] = s->d_topo_n.p; z.n32[3] = 1;
-        z.i32[4] = (int32_t*)s->d_queue.p; z.n32[4] = 1;
+        z.i32[4] = (int32_t*)s->d_queue.p; z.n32[4] = 3;
         z.i32[5] = batched ? s->d_bcnt.p : nullptr; z.n32[5] = 64;
         z.i32[6] = s->d_topo_nbs.p; z.n32[6] = 2;
-        z.u64[0] = s->d_hit_top.p; z.n64[0] = 1;
+        z.u64[0] = s->d_hit_top.p; z.n64[0] = HIT_SHARDS;
         z.u64[1] = s->tw.top.p; z.n64[1] = 5;
         launch_step_begin(kt, z, st);
     }
@@ -829,6 +857,7 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     NarrowCfg nc = s->ncfg;
     nc.use_queue = batched ? 1 : 0;
     nc.groups = g_narrow;
+    nc.wave_groups = persistent_groups(s->ctx, 4);
     launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->stage_cap, s->tmax, s->d_queue.p, st, true, kt + 8);
     if (!batched) launch_stamp(kt + 3, st);   // batched: stamped by k_topo_zero (no host read in between)
     TopoBufs T = s->topobufs();
@@ -872,12 +901,7 @@ bool grow_for(zdx_sim* s, int32_t status, int par) {
         grown = true;
     }
     if (status & ST_HITS) {
-        // the hit counter holds what every pair asked for
-        unsigned long long want = 0;
-        HIPC(hipMemcpy(&want, s->d_hit_top.p, sizeof(want), hipMemcpyDeviceToHost));
-        int64_t cap = 2 * s->hit_cap;
-        while (cap < (int64_t)(want + want / 4)) cap *= 2;
-        s->alloc_hits(cap);
+        s->alloc_hits(grown_hit_cap(s->d_hit_top.p, s->dpairs(par, false).hit_shards, s->hit_cap));
         grown = true;
     }
     if (status & ST_STAGE) {
@@ -1059,8 +1083,8 @@ void contact_pairs_impl(zdx_ctx* c, zdx_ctx::Api& A, const zdx_pair_in* pairs, i
         launch_iota_pairs(A.pi.p, A.pj.p, n, st);
         launch_fill_i32(A.npairs.p, n, 1, st);
         A.seg_off.ensure(n); A.tri_checks.ensure(n); A.tri_tests.ensure(n); A.nseg.ensure(n); A.tri_hits.ensure(n); A.tri_scanned.ensure(n);
-        A.queue.ensure(2); A.bcnt.ensure(64); A.bcur.ensure(64);
-        A.hit_off.ensure(n); A.hit_top.ensure(1);
+        A.queue.ensure(4); A.heavy.ensure(n); A.bcnt.ensure(64); A.bcur.ensure(64);
+        A.hit_off.ensure(n); A.hit_top.ensure(HIT_SHARDS);
         A.ncontrib.ensure(n); A.comps.ensure(n); A.loops_raw.ensure(n); A.tstatus.ensure(n);
         A.topo_list.ensure(n); A.topo_n.ensure(1);
         const bool batched = opts->mode == ZDX_MODE_BATCHED;
@@ -1083,7 +1107,9 @@ void contact_pairs_impl(zdx_ctx* c, zdx_ctx::Api& A, const zdx_pair_in* pairs, i
             P.status = A.status.p;
             P.topo_list = A.topo_list.p; P.topo_n = A.topo_n.p;
             P.hits = A.hits.p; P.hit_cap = A.hit_cap; P.hit_top = A.hit_top.p; P.hit_off = A.hit_off.p;
+            P.hit_shards = (int32_t)std::min<int64_t>(std::min<int64_t>(HIT_SHARDS, n), A.hit_cap);
             P.hit_flag = A.hit_flag.p;
+            P.heavy = A.heavy.p;
             if (sens) {
                 A.hit_tri.ensure(2 * (size_t)A.hit_cap);
                 P.hit_tri = A.hit_tri.p;
@@ -1094,7 +1120,7 @@ void contact_pairs_impl(zdx_ctx* c, zdx_ctx::Api& A, const zdx_pair_in* pairs, i
             launch_fill_i32(A.topo_n.p, 0, 1, st);
             HIPC(hipEventRecord(c->ev_api[2], st));
             if (batched) launch_lpt_order(P, M, B, A.order.p, A.bcnt.p, A.bcur.p, n, st);
-            NarrowCfg nc{256, 4, groups, batched ? 1 : 0};
+            NarrowCfg nc{256, 4, groups, batched ? 1 : 0, std::min(persistent_groups(c, 4), (n + 3) / 4)};
             launch_narrow(B, M, P, nc, A.list_i.p, A.list_f.p, A.stage_cap, c->max_tris, A.queue.p, st);
             HIPC(hipEventRecord(c->ev_api[3], st));
             int32_t status = 0;
@@ -1102,12 +1128,7 @@ void contact_pairs_impl(zdx_ctx* c, zdx_ctx::Api& A, const zdx_pair_in* pairs, i
             HIPC(hipStreamSynchronize(st));
             if (status & (ST_STAGE | ST_HITS)) {
                 if (status & ST_STAGE) A.stage_cap *= 4;
-                if (status & ST_HITS) {   // the hit counter holds what every pair asked for
-                    unsigned long long want = 0;
-                    HIPC(hipMemcpy(&want, A.hit_top.p, sizeof(want), hipMemcpyDeviceToHost));
-                    A.hit_cap *= 2;
-                    while (A.hit_cap < (int64_t)(want + want / 4)) A.hit_cap *= 2;
-                }
+                if (status & ST_HITS) A.hit_cap = grown_hit_cap(A.hit_top.p, P.hit_shards, A.hit_cap);
                 continue;
             }
             // topology with the minimal-energy force
@@ -1789,7 +1810,7 @@ zdx_status zdx_sim_autotune(zdx_sim* s, int32_t* out_block, int32_t* out_groups)
         int bestg = s->np_groups_per_cu;
         for (auto& cb : cand) {
             for (int g : gpc) {
-                NarrowCfg nc{cb[0], cb[1], persistent_groups(s->ctx, g), 1};
+                NarrowCfg nc{cb[0], cb[1], persistent_groups(s->ctx, g), 1, persistent_groups(s->ctx, 4)};
                 // one untimed launch, then the median of four timed ones
                 float t4[4] = {0, 0, 0, 0};
                 for (int rep = 0; rep < 5; ++rep) {

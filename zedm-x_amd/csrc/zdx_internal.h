@@ -170,6 +170,12 @@ struct NarrowItem {
     int32_t pad[8];
 };
 
+// The hit arena is cut into shards, each with its own counter: a pair's
+// hits go to shard (work item % hit_shards), so that the reservations are
+// spread over that many addresses (atomics on one address serialise, at about
+// 11.5 ns each on MI355X).
+constexpr int HIT_SHARDS = 64;
+
 struct DPairs {
     int32_t cap;             // capacity of the pair arrays
     const int32_t* n;        // device count of pairs
@@ -196,10 +202,12 @@ struct DPairs {
     int32_t* hits;           // hit arena: per triangle pair that passed the float AABB sweep, the six
                              // world-vertex indices of (ta, tb); its slots are also the segment slots
     int32_t* hit_tri;        // optional (sensitivity runs): (ta, tb) per hit
-    int64_t hit_cap;         // in hits
-    unsigned long long* hit_top;
+    int64_t hit_cap;         // in hits; shard k holds [k, k + 1) * (hit_cap / hit_shards)
+    unsigned long long* hit_top;   // hit_shards counters: hits each shard's pairs asked for
+    int32_t hit_shards;      // 1 .. HIT_SHARDS
     int64_t* hit_off;        // per pair: first hit (count in tri_hits); the pair's segments start there
     int32_t* hit_flag;       // per hit: 1 when the exact test produced a segment
+    int32_t* heavy;          // narrow phase: work items left by the one-wave kernel for the workgroup kernel
 };
 
 enum : int32_t {

@@ -41,59 +41,7 @@ namespace {
 
 constexpr int WAVE = 64;
 
-__device__ inline int lane_id() { return __lane_id(); }
-
-// Exclusive rank of `pred` among the workgroup's threads (thread order) and
-// the total count.  Uses wave ballots and one LDS array of per-wave counts.
-template <int BS>
-__device__ inline int block_rank(bool pred, int* s_wave, int& total) {
-    constexpr int NW = BS / WAVE;
-    const int lane = lane_id();
-    const int wid = threadIdx.x / WAVE;
-    unsigned long long m = __ballot(pred);
-    unsigned long long below = (lane == 0) ? 0ull : (m & ((~0ull) >> (64 - lane)));
-    int r = __popcll(below);
-    if (lane == 0) s_wave[wid] = __popcll(m);
-    __syncthreads();
-    int off = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-        int c = s_wave[w];
-        off += (w < wid) ? c : 0;
-        tot += c;
-    }
-    __syncthreads();
-    total = tot;
-    return off + r;
-}
-
-// Exclusive prefix of per-thread counts (thread order) and the total.
-template <int BS>
-__device__ inline int block_excl(int v, int* s_wave, int& total) {
-    constexpr int NW = BS / WAVE;
-    const int lane = lane_id();
-    const int wid = threadIdx.x / WAVE;
-    int x = v;
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-        int y = __shfl_up(x, d, WAVE);
-        if (lane >= d) x += y;
-    }
-    if (lane == WAVE - 1) s_wave[wid] = x;
-    __syncthreads();
-    int off = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-        int c = s_wave[w];
-        off += (w < wid) ? c : 0;
-        tot += c;
-    }
-    __syncthreads();
-    total = tot;
-    return off + x - v;
-}
-
-__device__ inline V3 ld3(const double* p) { return v3(p[0], p[1], p[2]); }
+#include "zdx_wave.h"
 
 // Stage stamps: one thread writes the device's constant-rate clock into a
 // slot; launched on the stream between stages, so consecutive stamps bracket
@@ -101,15 +49,6 @@ __device__ inline V3 ld3(const double* p) { return v3(p[0], p[1], p[2]); }
 // A per-kernel scheme with an atomic from every wave would serialise tens of
 // thousands of same-address atomics per launch.
 __global__ void k_stamp(unsigned long long* slot) { *slot = (unsigned long long)wall_clock64(); }
-// The same from inside the kernel that follows a stage (workgroup 0, thread
-// 0, at its start): saves a launch per stamp.
-__device__ inline void stamp_here(unsigned long long* a, unsigned long long* b = nullptr) {
-    if ((a || b) && blockIdx.x == 0 && threadIdx.x == 0) {
-        const unsigned long long t = (unsigned long long)wall_clock64();
-        if (a) *a = t;
-        if (b) *b = t;
-    }
-}
 __global__ void __launch_bounds__(64) k_step_begin(unsigned long long* kt, StepZero z) {
     if (threadIdx.x == 0) kt[0] = (unsigned long long)wall_clock64();
     for (int a = 0; a < 8; ++a)
@@ -670,376 +609,10 @@ __device__ unsigned long long g_walk_prof[16];
 __device__ long long g_topo_pairs[4 * 8192];    // per topology item: S, pass A ticks, walk D ticks
 __device__ long long g_walk_pairs[12 * 8192];   // walk B per pair: S, R, scratch in LDS, ticks, phases 0-7
 __device__ long long g_walkd_pairs[4 * 8192];   // walk D per pair: T-junctions, adjacency, start sort, loops
-__device__ long long g_np_pairs[8 * 16384];   // narrow per pair (queue item): nA, nB, hits, 4 phase ticks, p
-#define NP_START() unsigned long long np_t0 = wall_clock64(); unsigned np_ph[4] = {0, 0, 0, 0}
-#define NP_MARK(k)                                                      \
-    do {                                                                \
-        const unsigned long long t_ = wall_clock64();                   \
-        np_ph[(k) & 3] += (unsigned)(t_ - np_t0);                       \
-        np_t0 = t_;                                                     \
-    } while (0)
-#else
-#define NP_MARK(k) do {} while (0)
-#define NP_START() do {} while (0)
 #endif
 
 // ------------------------------------------------------------ narrow phase
-// outward-rounded float AABB of a double box
-__device__ inline void box_f(const double* lo, const double* hi, float* f) {
-    f[0] = __double2float_rd(lo[0]); f[1] = __double2float_rd(lo[1]); f[2] = __double2float_rd(lo[2]);
-    f[3] = __double2float_ru(hi[0]); f[4] = __double2float_ru(hi[1]); f[5] = __double2float_ru(hi[2]);
-}
-__device__ inline bool ovl_f(const float* a, const float* b) {
-    return !(a[3] < b[0] || a[0] > b[3] || a[4] < b[1] || a[1] > b[4] || a[5] < b[2] || a[2] > b[5]);
-}
-__device__ inline void tri_box(const V3* t, double* lo, double* hi) {
-    lo[0] = min3(t[0].x, t[1].x, t[2].x); lo[1] = min3(t[0].y, t[1].y, t[2].y); lo[2] = min3(t[0].z, t[1].z, t[2].z);
-    hi[0] = max3(t[0].x, t[1].x, t[2].x); hi[1] = max3(t[0].y, t[1].y, t[2].y); hi[2] = max3(t[0].z, t[1].z, t[2].z);
-}
-__device__ inline void load_tri(const double* wv, const int32_t* ti, int t, V3* out) {
-    out[0] = ld3(wv + 3 * ti[3 * t]);
-    out[1] = ld3(wv + 3 * ti[3 * t + 1]);
-    out[2] = ld3(wv + 3 * ti[3 * t + 2]);
-}
-
-// Vertex outcodes against a float box: bit 2k set when the vertex lies below
-// the box on axis k with its coordinate rounded up, bit 2k+1 when it lies
-// above with its coordinate rounded down.  A triangle's outward-rounded float
-// AABB misses the box exactly when the three outcodes share a set bit
-// (rounding is monotone, so rounding the vertex coordinates commutes with the
-// min/max of the box), which makes the cull below the same test as
-// ovl_f(box_f(tri_box(...)), box) with one load per vertex instead of three
-// per triangle.
-__device__ inline unsigned vertex_outcode(const double* w, const float* bx) {
-    const double x = w[0], y = w[1], z = w[2];
-    return (unsigned)(__double2float_ru(x) < bx[0]) | ((unsigned)(__double2float_rd(x) > bx[3]) << 1) |
-           ((unsigned)(__double2float_ru(y) < bx[1]) << 2) | ((unsigned)(__double2float_rd(y) > bx[4]) << 3) |
-           ((unsigned)(__double2float_ru(z) < bx[2]) << 4) | ((unsigned)(__double2float_rd(z) > bx[5]) << 5);
-}
-constexpr int OC_CAP = 2048;   // vertices per body whose outcodes fit in LDS
-constexpr int CULL_PER_THREAD = 8;
-
-// Narrow phase, part 1 (cull and sweep): persistent workgroups, one pair at a
-// time.  Each side's triangles are culled against the other body's AABB, then
-// the flattened (ta, tb) product of the survivors is swept with the float
-// AABB test; the triangle pairs that pass ("hits") are written in (ta, tb)
-// order to the pair's slice of the hit arena, which is reserved with the
-// upper bound nA * nB so that every pair's hits are contiguous.  Pairs with
-// hits are listed for part 2 (k_narrow_exact).  Register budget: 5 waves per
-// SIMD for 256 threads (5 workgroups per CU), 4 for 512 threads (the same 2
-// workgroups per CU as 5 would give, with room to keep everything in registers).
-template <int BS, int K>
-__global__ void __launch_bounds__(BS, BS >= 512 ? 4 : (BS >= 256 ? 5 : 1)) k_narrow(DBodies B, DMeshes M, DPairs P, int32_t* list_i, float* list_f,
-                                               int32_t tmax, int32_t use_queue, unsigned int* queue, int32_t lds_cap,
-                                               unsigned long long* stamp) {
-    stamp_here(stamp);
-    // Body B's culled float boxes (24 bytes per triangle, read for every item
-    // of the sweep) live in dynamic LDS when there are at most lds_cap of
-    // them, else in the workgroup's global list.  Per workgroup in global
-    // memory: the kept triangle ids of both sides (LA, LB), their world-vertex
-    // indices (VA3, VB3, 3 each) and A's float boxes (FA).
-    extern __shared__ char s_nb[];
-    __shared__ int s_wave[BS / WAVE];
-    __shared__ int s_item;
-    __shared__ unsigned char s_oc[2 * OC_CAP];
-    __shared__ unsigned long long s_kmask[2 * WAVE];   // cull: keep ballots per (layer, wave), A then B
-    __shared__ unsigned long long s_base;
-    const int tid = threadIdx.x;
-    int32_t* LA = list_i + (int64_t)blockIdx.x * 8 * tmax;
-    int32_t* LB = LA + tmax;
-    int32_t* VA3 = LA + 2 * (int64_t)tmax;
-    int32_t* VB3 = LA + 5 * (int64_t)tmax;
-    float* FA = list_f + (int64_t)blockIdx.x * 12 * tmax;
-    const int npairs = *P.n;
-    // with the work queue, each workgroup asks for its next item while it
-    // works on the current one
-    int item = blockIdx.x;
-    if (use_queue) {
-        if (tid == 0) s_item = (int)atomicAdd(queue, 1u);
-        __syncthreads();
-        item = s_item;
-        __syncthreads();
-    }
-    NP_START();
-    while (item < npairs) {
-        int next = 0;
-        if (use_queue && tid == 0) next = (int)atomicAdd(queue, 1u);
-        int p, i, j, TA, TB, VA, VB, ndA;
-        int64_t oA, oB;
-        const int32_t* tiA;
-        const int32_t* tiB;
-        float boxA[6], boxB[6];
-        if (P.items) {
-            const NarrowItem& it = P.items[item];
-            p = it.p; i = it.i; j = it.j;
-            TA = it.TA; TB = it.TB; VA = it.VA; VB = it.VB; ndA = it.nondegA;
-            tiA = M.tri_idx + 3 * (int64_t)it.toA;
-            tiB = M.tri_idx + 3 * (int64_t)it.toB;
-            oA = it.woA; oB = it.woB;
-            for (int k = 0; k < 6; ++k) { boxA[k] = it.boxA[k]; boxB[k] = it.boxB[k]; }
-        } else {
-            p = P.order ? P.order[item] : item;
-            i = P.pi[p]; j = P.pj[p];
-            const int mA = B.mesh[i], mB = B.mesh[j];
-            TA = M.ntris[mA]; TB = M.ntris[mB];
-            VA = M.nverts[mA]; VB = M.nverts[mB];
-            ndA = B.nondeg[i];
-            tiA = M.tri_idx + 3 * (int64_t)M.tri_off[mA];
-            tiB = M.tri_idx + 3 * (int64_t)M.tri_off[mB];
-            oA = B.wv_off[i]; oB = B.wv_off[j];
-            box_f(B.aabb + 6 * (int64_t)i, B.aabb + 6 * (int64_t)i + 3, boxA);
-            box_f(B.aabb + 6 * (int64_t)j, B.aabb + 6 * (int64_t)j + 3, boxB);
-        }
-        const double* wvA = B.wv + 3 * oA;
-        const double* wvB = B.wv + 3 * oB;
-        NP_MARK(0);
-        // ---- cull each side against the other body's AABB (conservative)
-        int nA = 0, nB = 0;
-        if (VA <= OC_CAP && VB <= OC_CAP) {
-            // outcodes of both bodies' vertices, then per thread CULL_PER_THREAD
-            // consecutive triangles of each side, one ordered scan per round
-            constexpr int UV = BS <= 64 ? 8 : 4;   // vertices per thread per round, loads in flight
-            for (int v0 = 0; v0 < VA + VB; v0 += UV * BS) {
-                double w[UV][3];
-#pragma unroll
-                for (int u = 0; u < UV; ++u) {   // all loads first, then the codes
-                    const int v = v0 + u * BS + tid;
-                    const double* src = v < VA ? wvA + 3 * v : wvB + 3 * (v - VA);
-                    if (v < VA + VB) { w[u][0] = src[0]; w[u][1] = src[1]; w[u][2] = src[2]; }
-                }
-#pragma unroll
-                for (int u = 0; u < UV; ++u) {
-                    const int v = v0 + u * BS + tid;
-                    if (v < VA + VB) {
-                        const bool a = v < VA;
-                        // the other body's box, selected per component (a pointer
-                        // select would force both boxes out of registers)
-                        float bx[6];
-#pragma unroll
-                        for (int k = 0; k < 6; ++k) bx[k] = a ? boxB[k] : boxA[k];
-                        s_oc[a ? v : OC_CAP + v - VA] = (unsigned char)vertex_outcode(w[u], bx);
-                    }
-                }
-            }
-            __syncthreads();
-            // CPT layers of BS consecutive triangles per round (lane-consecutive
-            // index loads); the keep ballots of every (layer, wave) go to LDS
-            // and each wave turns their popcounts into ranks in triangle order
-            constexpr int CPT = CULL_PER_THREAD;
-            constexpr int NW = BS / WAVE;
-            static_assert(CPT * NW <= WAVE, "keep masks of one round fit one wave");
-            const int wid = tid / WAVE, lane = tid % WAVE;
-            const uint64_t below = (1ull << lane) - 1ull;
-            for (int base = 0; base < TA || base < TB; base += BS * CPT) {
-                unsigned ka = 0u, kb = 0u;
-#pragma unroll
-                for (int c = 0; c < CPT; ++c) {
-                    const int t = base + c * BS + tid;
-                    if (t < TA) {
-                        const int32_t* ix = tiA + 3 * t;
-                        if ((s_oc[ix[0]] & s_oc[ix[1]] & s_oc[ix[2]]) == 0) ka |= 1u << c;
-                    }
-                    if (t < TB) {
-                        const int32_t* ix = tiB + 3 * t;
-                        if ((s_oc[OC_CAP + ix[0]] & s_oc[OC_CAP + ix[1]] & s_oc[OC_CAP + ix[2]]) == 0) kb |= 1u << c;
-                    }
-                }
-#pragma unroll
-                for (int c = 0; c < CPT; ++c) {
-                    const uint64_t ma = __ballot((ka >> c) & 1u), mb = __ballot((kb >> c) & 1u);
-                    if (lane == 0) { s_kmask[c * NW + wid] = ma; s_kmask[WAVE + c * NW + wid] = mb; }
-                }
-                __syncthreads();
-                // inclusive prefix of the popcounts in (layer, wave) order, per side
-                const int q = lane;
-                int pa = q < CPT * NW ? __popcll(s_kmask[q]) : 0;
-                int pb = q < CPT * NW ? __popcll(s_kmask[WAVE + q]) : 0;
-#pragma unroll
-                for (int d = 1; d < WAVE; d <<= 1) {
-                    const int ya = __shfl_up(pa, d, WAVE), yb = __shfl_up(pb, d, WAVE);
-                    if (lane >= d) { pa += ya; pb += yb; }
-                }
-                const int tot_a = __shfl(pa, CPT * NW - 1, WAVE), tot_b = __shfl(pb, CPT * NW - 1, WAVE);
-#pragma unroll
-                for (int c = 0; c < CPT; ++c) {
-                    const int slot = c * NW + wid;
-                    const int ea = __shfl(pa, slot, WAVE) - __popcll(s_kmask[slot]);
-                    const int eb = __shfl(pb, slot, WAVE) - __popcll(s_kmask[WAVE + slot]);
-                    const int t = base + c * BS + tid;
-                    if ((ka >> c) & 1u) LA[nA + ea + __popcll(s_kmask[slot] & below)] = t;
-                    if ((kb >> c) & 1u) LB[nB + eb + __popcll(s_kmask[WAVE + slot] & below)] = t;
-                }
-                __syncthreads();   // the masks are rewritten by the next round
-                nA += tot_a;
-                nB += tot_b;
-            }
-            NP_MARK(1);
-        } else {
-        for (int base = 0; base < TA; base += BS) {
-            int t = base + tid;
-            bool keep = false;
-            float f[6];
-            if (t < TA) {
-                V3 tr[3]; double lo[3], hi[3];
-                load_tri(wvA, tiA, t, tr);
-                tri_box(tr, lo, hi);
-                box_f(lo, hi, f);
-                keep = ovl_f(f, boxB);
-            }
-            int tot;
-            int r = block_rank<BS>(keep, s_wave, tot);
-            if (keep) LA[nA + r] = t;
-            nA += tot;
-        }
-        NP_MARK(1);
-        for (int base = 0; base < TB; base += BS) {
-            int t = base + tid;
-            bool keep = false;
-            float f[6];
-            if (t < TB) {
-                V3 tr[3]; double lo[3], hi[3];
-                load_tri(wvB, tiB, t, tr);
-                tri_box(tr, lo, hi);
-                box_f(lo, hi, f);
-                keep = ovl_f(f, boxA);
-            }
-            int tot;
-            int r = block_rank<BS>(keep, s_wave, tot);
-            if (keep) LB[nB + r] = t;
-            nB += tot;
-        }
-        }
-        __syncthreads();
-        float* FB = nB <= lds_cap ? (float*)s_nb : FA + 6 * tmax;
-        // float boxes and world-vertex indices of the kept triangles, all
-        // lanes in parallel
-        {
-            for (int k = tid; k < nA + nB; k += BS) {
-                const bool a = k < nA;
-                const int q = a ? k : k - nA;
-                const int t = a ? LA[q] : LB[q];
-                const int32_t* ix = (a ? tiA : tiB) + 3 * t;
-                const int i0 = ix[0], i1 = ix[1], i2 = ix[2];
-                const double* wv = a ? wvA : wvB;
-                V3 tr[3] = {ld3(wv + 3 * i0), ld3(wv + 3 * i1), ld3(wv + 3 * i2)};
-                double lo[3], hi[3];
-                tri_box(tr, lo, hi);
-                box_f(lo, hi, a ? FA + 6 * q : FB + 6 * q);
-                const int64_t o = a ? oA : oB;
-                int32_t* v3i = (a ? VA3 : VB3) + 3 * q;
-                v3i[0] = (int32_t)(o + i0); v3i[1] = (int32_t)(o + i1); v3i[2] = (int32_t)(o + i2);
-            }
-        }
-        __syncthreads();
-        NP_MARK(1);
-        // ---- ordered sweep of the culled product into the hit arena.  Each
-        // thread takes one contiguous range of the flattened (ta, tb) product,
-        // so the product's order is the threads' order: a counting pass, one
-        // block scan, an exactly sized range of the arena, then the write pass
-        // (from the hit bits of the counting pass when the range has at most 64
-        // items, else by testing again).  No barrier inside the sweep.
-        const int64_t total = (int64_t)nA * nB;
-        const int64_t per = (total + BS - 1) / BS;
-        const int64_t g_begin = total < (int64_t)tid * per ? total : (int64_t)tid * per;
-        const int64_t g_end = total < g_begin + per ? total : g_begin + per;
-        auto split = [&](int64_t g, int& ca, int& cb) {
-            if (total <= 0x7fffffff) { ca = (int)g / nB; cb = (int)g - ca * nB; }
-            else { ca = (int)(g / nB); cb = (int)(g - (int64_t)ca * nB); }
-        };
-        // f(k, ca, cb) for each hit of the range, k = item index within the range
-        auto for_hits = [&](auto&& f) {
-            if (g_begin >= g_end) return;
-            int ca, cb;
-            split(g_begin, ca, cb);
-            float fa[6];
-            for (int q = 0; q < 6; ++q) fa[q] = FA[6 * ca + q];
-            const int n = (int)(g_end - g_begin);
-            for (int k0 = 0; k0 < n; k0 += K) {
-#pragma unroll
-                for (int u = 0; u < K; ++u) {
-                    const int k = k0 + u;
-                    if (k < n) {
-                        if (ovl_f(fa, FB + 6 * cb)) f(k, ca, cb);
-                        if (++cb == nB) {
-                            cb = 0; ++ca;
-                            if (ca < nA) for (int q = 0; q < 6; ++q) fa[q] = FA[6 * ca + q];
-                        }
-                    }
-                }
-            }
-        };
-        // a hit record: the six world-vertex indices of (ta, tb); with
-        // provenance also (ta, tb) itself
-        auto emit = [&](int64_t at, int ca, int cb) {
-            int2* dst = (int2*)(P.hits + 6 * at);
-            const int32_t* va = VA3 + 3 * ca;
-            const int32_t* vb = VB3 + 3 * cb;
-            dst[0] = make_int2(va[0], va[1]);
-            dst[1] = make_int2(va[2], vb[0]);
-            dst[2] = make_int2(vb[1], vb[2]);
-            if (P.hit_tri) ((int2*)P.hit_tri)[at] = make_int2(LA[ca], LB[cb]);
-        };
-        int cnt = 0;
-        uint64_t bits = 0ull;   // hit bits of the range when per <= 64
-        for_hits([&](int k, int, int) {
-            ++cnt;
-            if (k < 64) bits |= 1ull << k;
-        });
-        int hits;
-        const int off = block_excl<BS>(cnt, s_wave, hits);
-        if (tid == 0) {
-            unsigned long long hb = hits > 0 ? atomicAdd(P.hit_top, (unsigned long long)hits) : 0ull;
-            if (hb + (unsigned long long)hits > (unsigned long long)P.hit_cap) {
-                atomicOr(P.status, (int)ST_HITS);
-                hb = ~0ull;
-            }
-            s_base = hb;
-        }
-        __syncthreads();
-        const unsigned long long hbase = s_base;
-        if (hbase != ~0ull && cnt > 0) {
-            int64_t at = (int64_t)hbase + off;
-            if (per <= 64) {
-                for (uint64_t m = bits; m; m &= m - 1) {
-                    int ca, cb;
-                    split(g_begin + ctz64(m), ca, cb);
-                    emit(at++, ca, cb);
-                }
-            } else {
-                for_hits([&](int, int ca, int cb) { emit(at++, ca, cb); });
-            }
-        }
-        NP_MARK(2);
-        if (tid == 0) {
-            const bool ok = hbase != ~0ull;
-            P.hit_off[p] = ok ? (int64_t)hbase : 0;
-            P.tri_hits[p] = ok ? hits : 0;
-            P.tri_checks[p] = (int64_t)ndA * TB;
-            P.tri_tests[p] = total;
-            P.tri_scanned[p] = TA + TB;
-            P.seg_off[p] = 0;   // set by the compaction for pairs with hits
-            P.nseg[p] = 0;
-        }
-        __syncthreads();
-        NP_MARK(3);
-#ifdef ZDX_WALK_PROFILE
-        if (tid == 0 && item < 16384) {
-            long long* r = g_np_pairs + 8 * (int64_t)item;
-            r[0] = nA; r[1] = nB; r[2] = hits;
-            for (int k = 0; k < 4; ++k) r[3 + k] = np_ph[k];
-            r[7] = p;
-            np_ph[0] = np_ph[1] = np_ph[2] = np_ph[3] = 0;
-        }
-#endif
-        if (use_queue) {
-            if (tid == 0) s_item = next;
-            __syncthreads();
-            item = s_item;
-        } else {
-            item += gridDim.x;
-        }
-    }
-}
+#include "zdx_narrow.h"
 
 // Narrow phase, part 2 (exact tests): one thread per hit of the hit arena.
 // Each hit is re-tested with the reference's exact AABB reject
@@ -1049,11 +622,38 @@ __global__ void __launch_bounds__(BS, BS >= 512 ? 4 : (BS >= 256 ? 5 : 1)) k_nar
 // with provenance (ta, tb, endpoint codes of tri_tri_prov).
 template <bool PROV>
 __global__ void __launch_bounds__(256) k_narrow_exact(DBodies B, DPairs P) {
-    // after a hit-arena overflow part of the arena was never written: test
-    // nothing (the step is run again with a larger arena)
-    const int64_t nh = (*P.status & (int)ST_HITS) ? 0 : (int64_t)*P.hit_top;
-    const int64_t lim = nh < P.hit_cap ? nh : P.hit_cap;
-    for (int64_t h = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; h < lim; h += (int64_t)gridDim.x * blockDim.x) {
+    // the used prefix of every shard of the arena, one thread per used slot:
+    // the slot of linear index i is found in the prefix sums of the shards'
+    // counts (after an overflow part of the arena was never written: test
+    // nothing, the step is run again with a larger arena)
+    static_assert(HIT_SHARDS == WAVE, "one lane per shard");
+    __shared__ long long s_pre[HIT_SHARDS + 1];
+    const int64_t size = P.hit_cap / P.hit_shards;
+    if (threadIdx.x < WAVE) {
+        const int l = threadIdx.x;
+        long long c = 0;
+        if (l < P.hit_shards) {
+            const long long t = (long long)P.hit_top[l];
+            c = t < size ? t : size;
+        }
+        long long x = c;
+#pragma unroll
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const long long y = __shfl_up(x, d, WAVE);
+            if (l >= d) x += y;
+        }
+        s_pre[l + 1] = x;
+        if (l == 0) s_pre[0] = 0;
+    }
+    __syncthreads();
+    const int64_t lim = (*P.status & (int)ST_HITS) ? 0 : s_pre[HIT_SHARDS];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < lim; i += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = HIT_SHARDS;   // s_pre[lo] <= i < s_pre[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) / 2;
+            if (s_pre[mid] <= i) lo = mid; else hi = mid;
+        }
+        const int64_t h = (int64_t)lo * size + (i - s_pre[lo]);
         const int2* rec = (const int2*)(P.hits + 6 * h);
         const int2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
         V3 A[3] = {ld3(B.wv + 3 * (int64_t)r0.x), ld3(B.wv + 3 * (int64_t)r0.y), ld3(B.wv + 3 * (int64_t)r1.x)};
@@ -1094,8 +694,8 @@ constexpr int TOPO_BIG_SEGS = 40;
 // order, i.e. in the reference's (ta, tb) order, to the front of its hit
 // range (in place: a segment only moves to a lower slot, and each round of 64
 // reads its slots before it writes).  A pair with more than seg_cap segments
-// stores none and raises ST_STAGE.  With P.topo_big, the pairs with segments
-// are also listed as many-segment or other.
+// stores none and raises ST_STAGE.  (k_topo_lists then lists the pairs with
+// segments.)
 template <bool PROV>
 __global__ void __launch_bounds__(256) k_narrow_compact(DPairs P, int32_t seg_cap) {
     const int lane = lane_id();
@@ -1151,14 +751,34 @@ __global__ void __launch_bounds__(256) k_narrow_compact(DPairs P, int32_t seg_ca
             }
             P.seg_off[p] = h0;
             P.nseg[p] = nstore;
-            if (nstore > 0) {
-                const int idx = atomicAdd(P.topo_n, 1);
-                P.topo_list[idx] = p;
-                if (P.topo_big) {
-                    if (nstore >= TOPO_BIG_SEGS) P.topo_big[atomicAdd(P.n_big, 1)] = idx;
-                    else P.topo_small[atomicAdd(P.n_small, 1)] = idx;
-                }
-            }
+        }
+    }
+}
+
+// The topology's work lists: every pair with stored segments in topo_list,
+// and (with P.topo_big) its index there in topo_big (TOPO_BIG_SEGS or more
+// segments) or topo_small.  One thread per pair, one atomic per list and
+// wave (one counter taking an atomic per pair would serialise them).
+__device__ inline int wave_append(bool put, int* counter) {
+    const uint64_t m = __ballot(put);
+    int base = 0;
+    if (lane_id() == 0 && m) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, 0, WAVE);
+    return base + __popcll(m & ((1ull << lane_id()) - 1ull));
+}
+__global__ void __launch_bounds__(256) k_topo_lists(DPairs P) {
+    const int npairs = (*P.status & (int)ST_HITS) ? 0 : *P.n;
+    const int stride = gridDim.x * blockDim.x;
+    for (int p0 = blockIdx.x * blockDim.x + (int)(threadIdx.x / WAVE) * WAVE; p0 < npairs; p0 += stride) {
+        const int p = p0 + lane_id();
+        const int ns = p < npairs ? P.nseg[p] : 0;
+        const int idx = wave_append(ns > 0, P.topo_n);
+        if (ns > 0) P.topo_list[idx] = p;
+        if (P.topo_big) {
+            const int ib = wave_append(ns >= TOPO_BIG_SEGS, P.n_big);
+            const int is = wave_append(ns > 0 && ns < TOPO_BIG_SEGS, P.n_small);
+            if (ns >= TOPO_BIG_SEGS) P.topo_big[ib] = idx;
+            else if (ns > 0) P.topo_small[is] = idx;
         }
     }
 }
@@ -2235,7 +1855,7 @@ extern "C" int zdx_topo_pairs(long long* out, int n) {
     return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_topo_pairs), 4 * (size_t)n * sizeof(long long)) == hipSuccess ? 0 : 1;
 }
 extern "C" int zdx_narrow_pairs(long long* out, int n) {
-    return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_np_pairs), 8 * (size_t)n * sizeof(long long)) == hipSuccess ? 0 : 1;
+    return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_np_pairs), NP_REC * (size_t)n * sizeof(long long)) == hipSuccess ? 0 : 1;
 }
 extern "C" int zdx_walk_pairs(long long* out, int n) {
     return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_walk_pairs), 12 * (size_t)n * sizeof(long long)) == hipSuccess ? 0 : 1;
@@ -2300,36 +1920,39 @@ void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s) {
     hipLaunchKernelGGL(k_bp_adjfix, dim3(nblk(n, 256) > 0 ? nblk(n, 256) : 1), dim3(256), 0, s, bp);
 }
 
+// the workgroup kernel over the heavy list left by the wave kernel
 template <int BS, int K>
 static void narrow_t(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
-                     int32_t tmax, unsigned int* q, hipStream_t s, unsigned long long* stamp) {
-    // LDS for body B's culled boxes: 4 per thread (at most 2048), so that small
-    // workgroups leave room for many pairs in flight per CU
-    const int cap = tmax < 4 * BS ? tmax : (4 * BS < 2048 ? 4 * BS : 2048);
-    hipLaunchKernelGGL((k_narrow<BS, K>), dim3(cfg.groups), dim3(BS), 24 * (size_t)cap, s, B, M, P, li, lf, tmax,
-                       cfg.use_queue, q, cap, stamp);
+                     int32_t tmax, unsigned int* q, hipStream_t s) {
+    hipLaunchKernelGGL((k_narrow<BS, K>), dim3(cfg.groups), dim3(BS), (size_t)narrow_lds_bytes(BS), s, B, M, P, li,
+                       lf, tmax, cfg.use_queue, q + 2, P.heavy, q + 1, nullptr);
 }
 
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
                    int32_t seg_cap, int32_t tmax, unsigned int* q, hipStream_t s, bool zeroed,
                    unsigned long long* stamp) {
     if (!zeroed) {
-        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int32_t*)q, 0, nullptr, (int64_t)1);
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, P.hit_top, 0ull, (int64_t)1);
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int32_t*)q, 0, nullptr, (int64_t)3);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, P.hit_top, 0ull, (int64_t)HIT_SHARDS);
     }
+    // every pair through the one-wave kernel; the pairs it leaves (heavy)
+    // through the workgroup kernel of the configured shape
+    const int wg = cfg.wave_groups > 0 ? cfg.wave_groups : std::max(1, cfg.groups);
+    hipLaunchKernelGGL(k_narrow_wave<4>, dim3(wg), dim3(NWV * WAVE), (size_t)narrow_wave_lds_bytes(), s, B, M, P,
+                       P.heavy, q, stamp);
     const int key = P.prov ? 25604 : cfg.block * 100 + cfg.items;
     switch (key) {
-        case 6404: narrow_t<64, 4>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
-        case 6408: narrow_t<64, 8>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
-        case 12804: narrow_t<128, 4>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
-        case 12808: narrow_t<128, 8>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
-        case 25601: narrow_t<256, 1>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
-        case 25602: narrow_t<256, 2>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
-        case 25604: narrow_t<256, 4>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
-        case 25608: narrow_t<256, 8>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
-        case 51202: narrow_t<512, 2>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
-        case 51204: narrow_t<512, 4>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
-        default: narrow_t<256, 4>(B, M, P, cfg, li, lf, tmax, q, s, stamp); break;
+        case 6404: narrow_t<64, 4>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        case 6408: narrow_t<64, 8>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        case 12804: narrow_t<128, 4>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        case 12808: narrow_t<128, 8>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        case 25601: narrow_t<256, 1>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        case 25602: narrow_t<256, 2>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        case 25604: narrow_t<256, 4>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        case 25608: narrow_t<256, 8>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        case 51202: narrow_t<512, 2>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        case 51204: narrow_t<512, 4>(B, M, P, cfg, li, lf, tmax, q, s); break;
+        default: narrow_t<256, 4>(B, M, P, cfg, li, lf, tmax, q, s); break;
     }
     const int ge = std::max(1, cfg.groups) * 4;   // grid-stride over the hits
     const int gc = std::max(1, cfg.groups);        // 4 waves per workgroup over the listed pairs
@@ -2340,6 +1963,7 @@ void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const Na
         hipLaunchKernelGGL(k_narrow_exact<false>, dim3(ge), dim3(256), 0, s, B, P);
         hipLaunchKernelGGL(k_narrow_compact<false>, dim3(gc), dim3(256), 0, s, P, seg_cap);
     }
+    hipLaunchKernelGGL(k_topo_lists, dim3(gc), dim3(256), 0, s, P);
 }
 
 void launch_lpt_order(const DPairs& P, const DMeshes& M, const DBodies& B, int32_t* order, int32_t* bcnt,
