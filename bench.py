@@ -329,8 +329,12 @@ def _short(name):
     return name.replace("zdx::(anonymous namespace)::", "").split("(")[0].replace("void ", "")
 
 
-# the narrow phase: cull + sweep, exact tests, compaction (one launch each per step)
-NARROW_KERNELS = ("k_narrow<", "k_narrow_exact", "k_narrow_compact")
+# the narrow phase (one launch each per step): cull + sweep with one wave per
+# pair, then a workgroup per pair for the pairs it leaves, exact tests,
+# compaction, the topology's work lists
+NARROW_KERNELS = ("k_narrow_wave", "k_narrow<", "k_narrow_exact", "k_narrow_compact", "k_topo_lists")
+NARROW_DESC = ("narrow phase: k_narrow_wave + k_narrow (cull + sweep) + k_narrow_exact + k_narrow_compact + "
+               "k_topo_lists")
 
 
 def _is_narrow(name):
@@ -364,7 +368,7 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir, command="", timed_stats=
         lines += ["", f"Narrow phase over the last {timed_stats} steps (the timed steps), average per launch:"]
         for k, v in sorted(tn.items()):
             lines.append(f"- `{_short(k)}`: {sum(v) / len(v):.1f} us (min {min(v):.1f}, max {max(v):.1f})")
-        lines.append(f"- sum of the three: {sum(sum(v) / len(v) for v in tn.values()):.1f} us per step")
+        lines.append(f"- sum of these kernels: {sum(sum(v) / len(v) for v in tn.values()):.1f} us per step")
     open(os.path.join(prof, "kernels_granular_bed_100k.md"), "w").write("\n".join(lines) + "\n")
     fetch = _pmc_by_kernel(fetch_dir, "FETCH_SIZE")
     write = _pmc_by_kernel(write_dir, "WRITE_SIZE")
@@ -389,7 +393,7 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir, command="", timed_stats=
         w_scale = kib / max(cw)
         fetch_b = nf_kib * f_scale * 1024.0
         write_b = nw_kib * w_scale * 1024.0
-        res = dict(kernel="narrow phase: k_narrow (cull + sweep) + k_narrow_exact + k_narrow_compact",
+        res = dict(kernel=NARROW_DESC,
                    dispatches=timed_pmc,
                    fetch_bytes_per_launch=fetch_b,
                    write_bytes_per_launch=write_b, bytes_per_launch=fetch_b + write_b,
@@ -426,15 +430,23 @@ def summarize_valu(pmc_dir, stats_dir, command="", timed=5, n_cu=256):
     """profiles/ summary of the VALU and occupancy counters (one rocprofv3
     --pmc run with VALU_COUNTERS) per kernel, averaged over the last `timed`
     dispatches of each: the rocprofv3 derived metrics computed from the raw
-    counters -- VALUBusy = 100 sum(SQ_ACTIVE_INST_VALU) / CU_NUM /
-    max(GRBM_GUI_ACTIVE), VALUUtilization = 100 sum(SQ_THREAD_CYCLES_VALU) /
-    (sum(SQ_ACTIVE_INST_VALU) 64), OccupancyPercent = 400 sum(SQ_WAVE_CYCLES)
-    / max(GRBM_GUI_ACTIVE) / CU_NUM / 32 -- and the FP64 VALU rate from
-    SQ_INSTS_VALU_FLOPS_FP64 over the kernel's average duration in the
-    kernel trace of stats_dir."""
+    counters -- VALUBusy = 100 sum(SQ_ACTIVE_INST_VALU) / CU_NUM / G,
+    VALUUtilization = 100 sum(SQ_THREAD_CYCLES_VALU) / (sum(SQ_ACTIVE_INST_VALU)
+    64), OccupancyPercent = 400 sum(SQ_WAVE_CYCLES) / G / CU_NUM / 32 -- with
+    G the kernel's GPU cycles: GRBM_GUI_ACTIVE as rocprofv3 reports it per
+    dispatch on MI355X is the sum over the 8 XCDs (Num_Xcc of the agent
+    information; it equals 8 x the kernel's duration x the 2.4 GHz clock), so G
+    = GRBM_GUI_ACTIVE / Num_Xcc -- and the FP64 VALU rate from
+    SQ_INSTS_VALU_FLOPS_FP64 over the kernel's average duration in the kernel
+    trace of stats_dir."""
     import csv
     import glob
 
+    n_xcc = 8
+    for f in glob.glob(os.path.join(pmc_dir, "*agent_info.csv")):
+        for r in csv.DictReader(open(f)):
+            if int(r.get("Num_Xcc", 0) or 0) > 0:
+                n_xcc = int(r["Num_Xcc"])
     per = {}
     for f in glob.glob(os.path.join(pmc_dir, "*counter_collection.csv")):
         for r in csv.DictReader(open(f)):
@@ -450,9 +462,9 @@ def summarize_valu(pmc_dir, stats_dir, command="", timed=5, n_cu=256):
     for name, ds in by.items():
         ds = ds[-timed:]
         m = {c: sum(d.get(c, 0.0) for d in ds) / len(ds) for c in VALU_COUNTERS}
-        g = max(m["GRBM_GUI_ACTIVE"], 1.0)
+        g = max(m["GRBM_GUI_ACTIVE"] / n_xcc, 1.0)
         t_us = sum(dur.get(name, [0.0])) / max(1, len(dur.get(name, [0.0])))
-        rows.append(dict(name=name, t_us=t_us, waves=m["SQ_WAVES"],
+        rows.append(dict(name=name, t_us=t_us, waves=m["SQ_WAVES"], gpu_mhz=g / max(t_us, 1e-9),
                          valu_busy=100.0 * m["SQ_ACTIVE_INST_VALU"] / n_cu / g,
                          valu_util=100.0 * m["SQ_THREAD_CYCLES_VALU"] / max(m["SQ_ACTIVE_INST_VALU"] * 64.0, 1.0),
                          occupancy=400.0 * m["SQ_WAVE_CYCLES"] / g / n_cu / 32.0,
@@ -465,14 +477,16 @@ def summarize_valu(pmc_dir, stats_dir, command="", timed=5, n_cu=256):
              f"Counters: {' '.join(VALU_COUNTERS)} (one --pmc pass), per kernel averaged over its last {timed} "
              f"dispatches (the timed steps); durations from the kernel trace of the same bench.  VALUBusy, "
              f"VALUUtilization and OccupancyPercent are rocprofv3's derived metrics computed from these raw counters "
-             f"(CU_NUM = {n_cu}); wait = SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES; FP64 = SQ_INSTS_VALU_FLOPS_FP64 per "
+             f"(CU_NUM = {n_cu}) with the kernel's GPU cycles G = GRBM_GUI_ACTIVE / {n_xcc} (rocprofv3 sums "
+             f"GRBM_GUI_ACTIVE over the {n_xcc} XCDs; the column MHz = G / duration checks it against the 2.4 GHz "
+             f"clock); wait = SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES; FP64 = SQ_INSTS_VALU_FLOPS_FP64 per "
              f"dispatch and its rate over the kernel's duration (peak 78.6 TFLOP/s).", "",
-             "| kernel | us | waves | VALUBusy % | VALUUtil % | Occupancy % | wait | FP64 GFLOP | FP64 TFLOP/s |",
-             "|---|---|---|---|---|---|---|---|---|"]
+             "| kernel | us | MHz | waves | VALUBusy % | VALUUtil % | Occupancy % | wait | FP64 GFLOP | FP64 TFLOP/s |",
+             "|---|---|---|---|---|---|---|---|---|---|"]
     for r in rows[:16]:
-        lines.append(f"| `{_short(r['name'])}` | {r['t_us']:.1f} | {r['waves']:.0f} | {r['valu_busy']:.1f} | "
-                     f"{r['valu_util']:.1f} | {r['occupancy']:.1f} | {r['wait_frac']:.2f} | {r['fp64'] / 1e9:.3f} | "
-                     f"{r['fp64_tflops']:.2f} |")
+        lines.append(f"| `{_short(r['name'])}` | {r['t_us']:.1f} | {r['gpu_mhz']:.0f} | {r['waves']:.0f} | "
+                     f"{r['valu_busy']:.1f} | {r['valu_util']:.1f} | {r['occupancy']:.1f} | {r['wait_frac']:.2f} | "
+                     f"{r['fp64'] / 1e9:.3f} | {r['fp64_tflops']:.2f} |")
     open(os.path.join(ROOT, "profiles", "valu_occupancy_granular_bed_100k.md"), "w").write("\n".join(lines) + "\n")
     print(json.dumps(rows[:16]))
 
@@ -606,7 +620,7 @@ def main():
     else:
         roof = dict(bound="mfma", achieved=ach_tflops, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=f_flops,
                     traffic=None)
-    roof.update(kernel="narrow phase: k_narrow (cull + sweep) + k_narrow_exact + k_narrow_compact",
+    roof.update(kernel=NARROW_DESC,
                 compute="FP64 VALU (no matrix-core work in this path)",
                 avg_launch_ms=st_b["narrow_kernels"], fp64_ops_per_launch=ops, bytes_per_launch=byts,
                 frac_fp64=f_flops, frac_hbm=f_hbm, dominant_stage=dom, stage_ms=stage_ms)

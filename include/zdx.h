@@ -265,14 +265,18 @@ zdx_status zdx_sim_stage_times(zdx_sim* sim, double* out, int32_t n);
  * minimal" with "minimal_k <k>", or "force_model zdem_damped". */
 enum { ZDX_FORCE_ZDEM = 0, ZDX_FORCE_MINIMAL = 1, ZDX_FORCE_ZDEM_DAMPED = 2 };
 zdx_status zdx_sim_set_force_model(zdx_sim* sim, int32_t model, double k);
-/* Narrow-phase launch shape (no reference counterpart).  np_shape encodes
- * the workgroup size and the triangle-pair tests per thread of one sweep
- * round as threads * 100 + tests: one of 6404, 6408, 12804, 12808, 25601,
- * 25602, 25604, 25608, 51202, 51204 (the compiled kernels); any other value
- * returns ZDX_BAD_ARG.  np_groups_per_cu (1..16) sets the persistent grid to
- * that many workgroups per CU.  0 keeps the current value of either.  The
- * autotuner (zdx_sim_autotune) times every compiled shape on the current
- * step's pair set and keeps the fastest; it reports the same encoding. */
+/* Narrow-phase launch shape (no reference counterpart).  The narrow phase
+ * runs every pair on one wave each and hands the pairs too large for one wave
+ * to a workgroup kernel; np_shape encodes that kernel's workgroup size and
+ * triangle-pair tests per thread of one sweep round as threads * 100 +
+ * tests: one of 6404, 6408, 12804, 12808, 25601, 25602, 25604, 25608, 51202,
+ * 51204 (the compiled kernels); any other value returns ZDX_BAD_ARG.
+ * np_groups_per_cu (1..16) sets its persistent grid to that many workgroups
+ * per CU.  0 keeps the current value of either.  The autotuner
+ * (zdx_sim_autotune) times every compiled shape, and the most float-box
+ * tests a pair may have on the one-wave kernel (4096 or 16384), on the
+ * current step's pair set and keeps the fastest; it reports the shape in the
+ * same encoding. */
 zdx_status zdx_sim_set_tiles(zdx_sim* sim, int32_t np_shape, int32_t np_groups_per_cu);
 zdx_status zdx_sim_autotune(zdx_sim* sim, int32_t* np_shape, int32_t* np_groups_per_cu);
 

@@ -164,11 +164,11 @@ static std::string launch_narrow_emu(const DBodies& B, const DMeshes& M, const D
                                      int32_t tmax, int groups, int use_queue, unsigned* q, bool wave) {
     if (wave) {
         const std::string e = simt::launch((unsigned)groups, NWV * WAVE, (size_t)narrow_wave_lds_bytes(),
-                                           [&] { k_narrow_wave<4>(B, M, P, P.heavy, q, nullptr); });
+                                           [&] { k_narrow_wave<4>(B, M, P, P.heavy, q, use_queue, (int32_t)W_HEAVY_TESTS, nullptr); });
         if (!e.empty()) return "wave kernel: " + e;
     }
     return simt::launch((unsigned)groups, BS, (size_t)narrow_lds_bytes(BS),
-                        [&] { k_narrow<BS, K>(B, M, P, li, lf, tmax, use_queue, q + 2, P.heavy, q + 1, nullptr); });
+                        [&] { k_narrow<BS, K>(B, M, P, li, lf, tmax, use_queue, q + NQ_WG, P.heavy, q + NQ_HEAVY, nullptr); });
 }
 
 static int check_run(const Input& in, const std::vector<Expect>& ex, const Run& r, long& hits_total) {
@@ -218,11 +218,11 @@ static int check_run(const Input& in, const std::vector<Expect>& ex, const Run& 
     std::fill(hit_tri.get(), hit_tri.get() + 2 * (size_t)hit_cap + 2, -1);
     int32_t n = np, status = 0;
     unsigned long long hit_top[HIT_SHARDS] = {};
-    unsigned queue[3] = {0, 0, 0};
+    unsigned queue[NQ_COUNTERS] = {};
     std::unique_ptr<int32_t[]> heavy(new int32_t[np]);
     if (!r.wave) {   // every item to the workgroup kernel
         for (int s = 0; s < np; ++s) heavy[s] = s;
-        queue[1] = (unsigned)np;
+        queue[NQ_HEAVY] = (unsigned)np;
     }
     DPairs P{};
     P.cap = np; P.n = &n; P.pi = in.pi.data(); P.pj = in.pj.data();
@@ -312,7 +312,7 @@ static int check_run(const Input& in, const std::vector<Expect>& ex, const Run& 
     hits_total += sum;
     if (bad == 0)
         std::printf("ok %s: %d pairs (%u to the workgroup kernel), %ld hits, %ld barriers, %ld wave operations\n", tag,
-                    np, queue[1], (long)sum, simt::blk().barriers, simt::blk().waveops);
+                    np, queue[NQ_HEAVY], (long)sum, simt::blk().barriers, simt::blk().waveops);
     simt::blk().barriers = simt::blk().waveops = 0;
     return bad ? 1 : 0;
 }

@@ -689,7 +689,7 @@ void sim_setup_device(zdx_sim* s) {
     s->d_sorted.ensure(N); s->d_nbr.ensure(16 * (size_t)N); s->d_cnt_up.ensure(N); s->d_cnt_all.ensure(N);
     s->d_pair_off.ensure(N + 1); s->d_adj_off.ensure(N + 1); s->d_n_adj.ensure(1);
     s->d_scan_tmp.ensure((size_t)(2 * (std::max<int64_t>(cells, N) / 1024 + 4)));
-    s->d_status.ensure(1); s->d_ctr.ensure(16); s->d_queue.ensure(4);
+    s->d_status.ensure(1); s->d_ctr.ensure(16); s->d_queue.ensure(NARROW_COUNTERS);
     s->d_bcnt.ensure(64); s->d_bcur.ensure(64);
     s->d_kt.ensure(12);
     // the cell counters return to zero within every broad phase (k_bp_scatter)
@@ -815,7 +815,7 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
         z.i32[1] = s->d_n_need.p; z.n32[1] = 1;
         z.i32[2] = s->d_n_wall.p; z.n32[2] = 1;
         z.i32[3] = s->d_topo_n.p; z.n32[3] = 1;
-        z.i32[4] = (int32_t*)s->d_queue.p; z.n32[4] = 3;
+        z.i32[4] = (int32_t*)s->d_queue.p; z.n32[4] = NARROW_COUNTERS;
         z.i32[5] = batched ? s->d_bcnt.p : nullptr; z.n32[5] = 64;
         z.i32[6] = s->d_topo_nbs.p; z.n32[6] = 2;
         z.u64[0] = s->d_hit_top.p; z.n64[0] = HIT_SHARDS;
@@ -1083,7 +1083,7 @@ void contact_pairs_impl(zdx_ctx* c, zdx_ctx::Api& A, const zdx_pair_in* pairs, i
         launch_iota_pairs(A.pi.p, A.pj.p, n, st);
         launch_fill_i32(A.npairs.p, n, 1, st);
         A.seg_off.ensure(n); A.tri_checks.ensure(n); A.tri_tests.ensure(n); A.nseg.ensure(n); A.tri_hits.ensure(n); A.tri_scanned.ensure(n);
-        A.queue.ensure(4); A.heavy.ensure(n); A.bcnt.ensure(64); A.bcur.ensure(64);
+        A.queue.ensure(NARROW_COUNTERS); A.heavy.ensure(n); A.bcnt.ensure(64); A.bcur.ensure(64);
         A.hit_off.ensure(n); A.hit_top.ensure(HIT_SHARDS);
         A.ncontrib.ensure(n); A.comps.ensure(n); A.loops_raw.ensure(n); A.tstatus.ensure(n);
         A.topo_list.ensure(n); A.topo_n.ensure(1);
@@ -1776,8 +1776,9 @@ zdx_status zdx_sim_set_tiles(zdx_sim* s, int32_t shape, int32_t groups_per_cu) {
 }
 
 // Autotuner: times the narrow phase alone on the current step's pair set for
-// each (threads per workgroup, tests per thread, workgroups per CU) and keeps
-// the fastest.  Every configuration produces the identical segment lists.
+// each (threads per workgroup, tests per thread, workgroups per CU of the
+// workgroup kernel; most tests of a pair on the one-wave kernel) and keeps the
+// fastest.  Every configuration produces the identical segment lists.
 zdx_status zdx_sim_autotune(zdx_sim* s, int32_t* out_block, int32_t* out_groups) {
     if (!s) return ZDX_BAD_ARG;
     return guard(s->ctx, [&] {
@@ -1808,9 +1809,11 @@ zdx_status zdx_sim_autotune(zdx_sim* s, int32_t* out_block, int32_t* out_groups)
         float best = 1e30f;
         NarrowCfg bestc = s->ncfg;
         int bestg = s->np_groups_per_cu;
+        // and the most float-box tests of a pair on the one-wave kernel
+        const int wtests[] = {4096, 16384};
         for (auto& cb : cand) {
-            for (int g : gpc) {
-                NarrowCfg nc{cb[0], cb[1], persistent_groups(s->ctx, g), 1, persistent_groups(s->ctx, 4)};
+            for (int g : gpc) for (int wt : wtests) {
+                NarrowCfg nc{cb[0], cb[1], persistent_groups(s->ctx, g), 1, persistent_groups(s->ctx, 4), wt};
                 // one untimed launch, then the median of four timed ones
                 float t4[4] = {0, 0, 0, 0};
                 for (int rep = 0; rep < 5; ++rep) {
@@ -1833,6 +1836,7 @@ zdx_status zdx_sim_autotune(zdx_sim* s, int32_t* out_block, int32_t* out_groups)
         (void)hipEventDestroy(e1);
         s->ncfg.block = bestc.block;
         s->ncfg.items = bestc.items;
+        s->ncfg.wave_tests = bestc.wave_tests;
         s->np_groups_per_cu = bestg;
         s->drop_graphs();
         if (out_block) *out_block = bestc.block * 100 + bestc.items;

@@ -135,6 +135,7 @@ struct NarrowCfg {
     int32_t groups;            // persistent workgroups (of the workgroup kernel, for the heavy pairs)
     int32_t use_queue;         // 1: atomic work queue (batched)
     int32_t wave_groups;       // workgroups of the one-wave-per-pair kernel (0: 4 per CU's worth of groups)
+    int32_t wave_tests;        // most float-box tests of a pair on the one-wave kernel (0: its default)
 };
 
 // Writes the device clock into *slot when the stream reaches it (NULL: no-op).
@@ -178,8 +179,10 @@ void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s);
 // the per-pair compaction into the segment arena.  seg_cap: most segments one
 // pair may store (ST_STAGE beyond); queue: the cull's work counter; resets
 // the queue and P.hit_top itself unless the caller did (zeroed).
-// queue: 3 counters (the wave kernel's work queue, the heavy list's count,
-// the workgroup kernel's queue over it), zeroed here unless `zeroed`.
+// queue: NARROW_COUNTERS counters (the wave kernel's work queues, the heavy
+// list's count, the workgroup kernel's queue over it), zeroed here unless
+// `zeroed`.
+constexpr int NARROW_COUNTERS = 66;
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* list_i,
                    float* list_f, int32_t seg_cap, int32_t tmax, unsigned int* queue, hipStream_t s, bool zeroed = false,
                    unsigned long long* stamp = nullptr);

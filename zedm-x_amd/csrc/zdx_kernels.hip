@@ -613,6 +613,7 @@ __device__ long long g_walkd_pairs[4 * 8192];   // walk D per pair: T-junctions,
 
 // ------------------------------------------------------------ narrow phase
 #include "zdx_narrow.h"
+static_assert(NQ_COUNTERS == NARROW_COUNTERS, "narrow-phase counters");
 
 // Narrow phase, part 2 (exact tests): one thread per hit of the hit arena.
 // Each hit is re-tested with the reference's exact AABB reject
@@ -1925,21 +1926,22 @@ template <int BS, int K>
 static void narrow_t(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
                      int32_t tmax, unsigned int* q, hipStream_t s) {
     hipLaunchKernelGGL((k_narrow<BS, K>), dim3(cfg.groups), dim3(BS), (size_t)narrow_lds_bytes(BS), s, B, M, P, li,
-                       lf, tmax, cfg.use_queue, q + 2, P.heavy, q + 1, nullptr);
+                       lf, tmax, cfg.use_queue, q + NQ_WG, P.heavy, q + NQ_HEAVY, nullptr);
 }
 
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
                    int32_t seg_cap, int32_t tmax, unsigned int* q, hipStream_t s, bool zeroed,
                    unsigned long long* stamp) {
     if (!zeroed) {
-        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int32_t*)q, 0, nullptr, (int64_t)3);
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int32_t*)q, 0, nullptr, (int64_t)NQ_COUNTERS);
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, P.hit_top, 0ull, (int64_t)HIT_SHARDS);
     }
     // every pair through the one-wave kernel; the pairs it leaves (heavy)
     // through the workgroup kernel of the configured shape
     const int wg = cfg.wave_groups > 0 ? cfg.wave_groups : std::max(1, cfg.groups);
+    const int32_t wt = cfg.wave_tests > 0 ? cfg.wave_tests : (int32_t)W_HEAVY_TESTS;
     hipLaunchKernelGGL(k_narrow_wave<4>, dim3(wg), dim3(NWV * WAVE), (size_t)narrow_wave_lds_bytes(), s, B, M, P,
-                       P.heavy, q, stamp);
+                       P.heavy, q, cfg.use_queue, wt, stamp);
     const int key = P.prov ? 25604 : cfg.block * 100 + cfg.items;
     switch (key) {
         case 6404: narrow_t<64, 4>(B, M, P, cfg, li, lf, tmax, q, s); break;

@@ -599,14 +599,20 @@ __global__ void __launch_bounds__(BS, BS >= 512 ? 4 : (BS >= 256 ? 5 : 1)) k_nar
 // Waves of the wave kernel per workgroup, LDS per wave, and the limits of a
 // pair that one wave takes: its two bodies' vertices (one outcode byte each)
 // and kept triangles (id 4 bytes, float box 24, world-vertex indices 12) fit
-// the wave's LDS, and its float-box tests number at most W_HEAVY_TESTS (64
-// per lane, so that the counting pass keeps every lane's hits as bits).
+// the wave's LDS, and its float-box tests number at most the launch's
+// heavy_tests (W_HEAVY_TESTS unless tuned: fewer leave more pairs to the
+// workgroup kernel, which shortens the wave kernel's tail when pairs are few;
+// more keep more pairs on the cheaper wave path).
 // Other pairs ("heavy") are listed for the workgroup kernel.  4 workgroups of
 // 4 waves share a CU's 160 KB.
 constexpr int NWV = 4;
 constexpr int WSLICE = 9728;
+// the narrow phase's counters (zeroed before it): WQ work queues of the wave
+// kernel, then the heavy list's count and the workgroup kernel's queue
+constexpr int WQ = 64;
+constexpr int NQ_HEAVY = WQ, NQ_WG = WQ + 1, NQ_COUNTERS = WQ + 2;
 constexpr int W_OC_MAX = 2048;
-constexpr int64_t W_HEAVY_TESTS = 64 * WAVE;
+constexpr int64_t W_HEAVY_TESTS = 256 * WAVE;
 __host__ __device__ constexpr int narrow_wave_lds_bytes() { return NWV * WSLICE; }
 static_assert(4 * (NWV * WSLICE + 64) <= 160 * 1024, "four wave-kernel workgroups per CU");
 
@@ -620,16 +626,20 @@ __device__ inline void wave_sync() {
 }
 
 // Narrow phase, part 1, one wave per pair: the same cull, sweep and hit
-// records as k_narrow, with no workgroup barrier.  Each wave takes its share
-// of the work items (in a fixed order), culls both bodies' triangles by
-// vertex outcodes into its LDS, makes the kept triangles' float boxes there,
-// sweeps their product with each lane on one contiguous range, reserves the
-// pair's exact share of the hit arena with one atomic and writes the hits in
-// (ta, tb) order.  A pair beyond the wave's limits is appended to hlist (its
-// count at queue[1]) for k_narrow, which writes all of that pair's results.
+// records as k_narrow, with no workgroup barrier.  Each wave takes work items
+// from one of (up to) WQ queues (queue q of n hands out items q, q + n, ... of
+// the largest-first order; one counter per queue, so that no address takes an
+// atomic per item) or, without queues, a fixed share; it culls both bodies'
+// triangles by vertex outcodes into its LDS, makes the kept triangles' float
+// boxes there, sweeps their product with each lane on one contiguous range,
+// reserves the pair's exact share of the hit arena with one atomic and writes
+// the hits in (ta, tb) order.  A pair beyond the wave's limits is appended to hlist (its
+// count at queue[NQ_HEAVY]) for k_narrow, which writes all of that pair's
+// results.
 template <int K>
 __global__ void __launch_bounds__(NWV * WAVE, 4) k_narrow_wave(DBodies B, DMeshes M, DPairs P, int32_t* hlist,
-                                                             unsigned int* queue, unsigned long long* stamp) {
+                                                             unsigned int* queue, int32_t use_queue,
+                                                             int32_t heavy_tests, unsigned long long* stamp) {
     stamp_here(stamp);
     ZDX_DYN_LDS(s_nb);
     const int lane = lane_id();
@@ -638,17 +648,27 @@ __global__ void __launch_bounds__(NWV * WAVE, 4) k_narrow_wave(DBodies B, DMeshe
     unsigned char* oc = (unsigned char*)sl;
     const uint64_t below = (1ull << lane) - 1ull;
     const int npairs = *P.n;
-    // the items in rounds of one per wave, each round in the opposite order
-    // of the last (the items come largest first, so every wave gets a like
-    // share of the work, with no shared counter to take them from)
+    // without queues: the items in rounds of one per wave, each round in the
+    // opposite order of the last
     const int nw = (int)gridDim.x * NWV;
     const int gw = (int)blockIdx.x * NWV + w;
+    const int nq = nw < WQ ? nw : WQ;   // every queue served by at least one wave
+    const int wq = gw % nq;
+    auto take = [&](int round) {
+        int it = 0;
+        if (use_queue) {
+            if (lane == 0) it = wq + nq * (int)atomicAdd(queue + wq, 1u);
+        } else {
+            it = round * nw + ((round & 1) ? nw - 1 - gw : gw);
+        }
+        return it;
+    };
     int round = 0;
-    int item = uni(gw);
+    int item = uni(__shfl(take(0), 0, WAVE));
     NP_START();
     while (item < npairs) {
         ++round;
-        const int next = round * nw + ((round & 1) ? nw - 1 - gw : gw);
+        const int next = take(round);   // lane 0's with queues: in flight during the item
         int p, i, j, TA, TB, VA, VB, ndA;
         int64_t oA, oB;
         const int32_t* tiA;
@@ -729,11 +749,11 @@ __global__ void __launch_bounds__(NWV * WAVE, 4) k_narrow_wave(DBodies B, DMeshe
             };
             cull(tiA, TA, oc, 0, nA);
             cull(tiB, TB, oc + VA, nA, nB);
-            heavy = nA + nB > cap || (int64_t)nA * nB > W_HEAVY_TESTS;
+            heavy = nA + nB > cap || (int64_t)nA * nB > heavy_tests;
             NP_MARK(2);
         }
         if (heavy) {
-            if (lane == 0) hlist[atomicAdd(queue + 1, 1u)] = item;
+            if (lane == 0) hlist[atomicAdd(queue + NQ_HEAVY, 1u)] = item;
         } else {
             wave_sync();
             // float boxes and world-vertex indices of the kept triangles
@@ -755,9 +775,10 @@ __global__ void __launch_bounds__(NWV * WAVE, 4) k_narrow_wave(DBodies B, DMeshe
             }
             wave_sync();
             NP_MARK(3);
-            // ordered sweep: each lane one contiguous range of at most 64 tests
-            // of the product, a counting pass that keeps the hits as bits, a
-            // wave scan, one reservation, the write pass from the bits
+            // ordered sweep: each lane one contiguous range of the product, a
+            // counting pass (which keeps the hits as bits when the range has
+            // at most 64 tests), a wave scan, one reservation, the write pass
+            // (from the bits, or by testing again)
             const float* FA = bx;
             const float* FB = bx + 6 * nA;
             const int total = nA * nB;
@@ -768,7 +789,7 @@ __global__ void __launch_bounds__(NWV * WAVE, 4) k_narrow_wave(DBodies B, DMeshe
             uint64_t bits = 0ull;
             sweep_range<K>(g0, g1 - g0, nB, FA, FB, [&](int k, int, int) {
                 ++cnt;
-                bits |= 1ull << k;
+                if (k < 64) bits |= 1ull << k;
             });
             int incl = cnt;
 #pragma unroll
@@ -785,9 +806,7 @@ __global__ void __launch_bounds__(NWV * WAVE, 4) k_narrow_wave(DBodies B, DMeshe
             if (ok && cnt > 0) {
                 int64_t at = (int64_t)hb + (incl - cnt);
                 // the hit records: the world-vertex indices of (ta, tb)
-                for (uint64_t m = bits; m; m &= m - 1) {
-                    const int g = g0 + ctz64(m);
-                    const int ca = g / nB, cb = g - ca * nB;
+                auto emit = [&](int ca, int cb) {
                     const int32_t* va = v3 + 3 * ca;
                     const int32_t* vb = v3 + 3 * (nA + cb);
                     int2* dst = (int2*)(P.hits + 6 * at);
@@ -796,6 +815,15 @@ __global__ void __launch_bounds__(NWV * WAVE, 4) k_narrow_wave(DBodies B, DMeshe
                     dst[2] = make_int2(vb[1], vb[2]);
                     if (P.hit_tri) ((int2*)P.hit_tri)[at] = make_int2(ids[ca], ids[nA + cb]);
                     ++at;
+                };
+                if (per <= 64) {
+                    for (uint64_t m = bits; m; m &= m - 1) {
+                        const int g = g0 + ctz64(m);
+                        const int ca = g / nB;
+                        emit(ca, g - ca * nB);
+                    }
+                } else {
+                    sweep_range<K>(g0, g1 - g0, nB, FA, FB, [&](int, int ca, int cb) { emit(ca, cb); });
                 }
             }
             NP_MARK(5);
@@ -822,6 +850,6 @@ __global__ void __launch_bounds__(NWV * WAVE, 4) k_narrow_wave(DBodies B, DMeshe
             r[16] = gw; r[17] = (long long)np_t0 - tot; r[18] = (long long)np_t0; r[19] = 1;
         }
 #endif
-        item = uni(next);
+        item = uni(__shfl(next, 0, WAVE));
     }
 }
