@@ -804,7 +804,8 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     // 3 after the narrow phase, 4 after the topology, 5 when the wall stream has
     // joined, 6 after fold + integrate, 7 after the walls (on the wall stream,
     // which starts at 2 and runs alongside 2..4), 8 before the narrow-phase
-    // kernels (after the work ordering).  Most are written by the first
+    // kernels (after the work ordering), 9 when the wall stream starts its
+    // kernels.  Most are written by the first
     // workgroup of the kernel that starts the next stage; 7 (and 2, 3 in
     // plain mode, where the host reads counts in between) by a stamp kernel.
     unsigned long long* kt = s->d_kt.p;
@@ -842,12 +843,14 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     const bool walls = s->W > 0;
     if (walls) {
         // the wall contacts need only the world vertices and the wall list:
-        // fork them onto the second stream, join before the fold
+        // fork them onto the second stream beside the narrow phase and the
+        // topology, join before the fold (forked after the narrow phase they
+        // slow the topology's pass A more than they cost the narrow phase)
         hipStream_t st2 = s->ctx->stream2;
         HIPC(hipEventRecord(s->ctx->ev_fork, st));
         HIPC(hipStreamWaitEvent(st2, s->ctx->ev_fork, 0));
         launch_walls(B, M, mat, s->wallbufs(), s->hist(par), s->d_wcontrib.p, s->d_wnc.p, s->d_wall_list.p,
-                     s->d_n_wall.p, s->d_wscr.p, g_walls, s->wall_pts_cap, s->sp, s->d_status.p, st2);
+                     s->d_n_wall.p, s->d_wscr.p, g_walls, s->wall_pts_cap, s->sp, s->d_status.p, st2, kt + 9);
         launch_stamp(kt + 7, st2);
         HIPC(hipEventRecord(s->ctx->ev_join, st2));
     }
@@ -1008,7 +1011,7 @@ void step_once(zdx_sim* s, bool batched) {
             const unsigned long long* kt = r.kt;
             // acc_ms: transform, broad phase, narrow, topology, walls (on its
             // own stream, overlapping narrow and topology), fold, narrow kernels
-            const int a[7] = {1, 0, 2, 3, 2, 5, 8}, b[7] = {2, 1, 3, 4, 7, 6, 3};
+            const int a[7] = {1, 0, 2, 3, 9, 5, 8}, b[7] = {2, 1, 3, 4, 7, 6, 3};
             for (int k = 0; k < 7; ++k) {
                 if (k == 4 && s->W == 0) continue;
                 if (kt[b[k]] >= kt[a[k]]) s->acc_ms[k] += (double)(kt[b[k]] - kt[a[k]]) / s->clock_khz;

@@ -206,7 +206,8 @@ void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const 
 // `groups` slices of 3 pts_cap + 4 (pts_cap + 8) doubles
 void launch_walls(const DBodies& B, const DMeshes& M, const DMat& mat, const WallBufs& W, const HistBufs& H,
                   double* wcontrib, int32_t* wncontrib, const int32_t* list, const int32_t* n_list, double* scr,
-                  int64_t groups, int32_t pts_cap, const SimParams& sp, int32_t* status, hipStream_t s);
+                  int64_t groups, int32_t pts_cap, const SimParams& sp, int32_t* status, hipStream_t s,
+                  unsigned long long* stamp = nullptr);
 void launch_fold_integrate(int32_t n, const double* state_in, double* state_out, const DMat& mat,
                            const int32_t* adj_off, const int32_t* adj_j, const int32_t* adj_pair,
                            int32_t adj_cap, const TopoBufs& T, int32_t n_walls, const double* wcontrib, const int32_t* wncontrib,

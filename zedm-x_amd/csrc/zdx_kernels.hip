@@ -1536,7 +1536,8 @@ __global__ void __launch_bounds__(64) k_walls(DBodies B, DMeshes M, DMat mat, Wa
     }
 }
 
-__global__ void k_walls_clear(int64_t n, int32_t* wnc, uint8_t* wact) {
+__global__ void k_walls_clear(int64_t n, int32_t* wnc, uint8_t* wact, unsigned long long* stamp) {
+    stamp_here(stamp);
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
         wnc[k] = 0;
         wact[k] = 0;
@@ -2037,11 +2038,12 @@ void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const 
 }
 void launch_walls(const DBodies& B, const DMeshes& M, const DMat& mat, const WallBufs& W, const HistBufs& H,
                   double* wcontrib, int32_t* wnc, const int32_t* list, const int32_t* n_list, double* scr,
-                  int64_t groups, int32_t pts_cap, const SimParams& sp, int32_t* status, hipStream_t s) {
+                  int64_t groups, int32_t pts_cap, const SimParams& sp, int32_t* status, hipStream_t s,
+                  unsigned long long* stamp) {
     if (W.n_walls <= 0) return;
     const int64_t n = (int64_t)B.n * W.n_walls;
     hipLaunchKernelGGL(k_walls_clear, dim3(nblk(n, 256) < 4096 ? nblk(n, 256) : 4096), dim3(256), 0, s, n, wnc,
-                       H.wcur_active);
+                       H.wcur_active, stamp);
     hipLaunchKernelGGL(k_walls, dim3((unsigned)groups), dim3(64), 0, s, B, M, mat, W, H, wcontrib, wnc, list, n_list,
                        scr, pts_cap, sp, status);
 }
