@@ -217,7 +217,7 @@ static int check_run(const Input& in, const std::vector<Expect>& ex, const Run& 
     std::fill(hits.get(), hits.get() + 6 * (size_t)hit_cap + 6, -1);
     std::fill(hit_tri.get(), hit_tri.get() + 2 * (size_t)hit_cap + 2, -1);
     int32_t n = np, status = 0;
-    unsigned long long hit_top[HIT_SHARDS] = {};
+    unsigned long long hit_top[HIT_SHARDS * HIT_TOP_STRIDE] = {};
     unsigned queue[NQ_COUNTERS] = {};
     std::unique_ptr<int32_t[]> heavy(new int32_t[np]);
     if (!r.wave) {   // every item to the workgroup kernel
@@ -299,12 +299,12 @@ static int check_run(const Input& in, const std::vector<Expect>& ex, const Run& 
     // every pair's range inside the used prefix of its shard, the ranges
     // disjoint, and the shards' counters add up to the hits
     unsigned long long top_sum = 0;
-    for (int k = 0; k < r.shards; ++k) top_sum += hit_top[k];
+    for (int k = 0; k < r.shards; ++k) top_sum += hit_top[k * HIT_TOP_STRIDE];
     if ((int64_t)top_sum != sum) fail(-1, "shard counters", (long)top_sum, (long)sum);
     const int64_t ssz = hit_cap / r.shards;
     for (const auto& rg : ranges) {
         const int64_t sh = rg.first / ssz;
-        if (sh >= r.shards || rg.second > sh * ssz + (int64_t)hit_top[sh]) fail(-1, "hit range past its shard", (long)rg.first, 0);
+        if (sh >= r.shards || rg.second > sh * ssz + (int64_t)hit_top[sh * HIT_TOP_STRIDE]) fail(-1, "hit range past its shard", (long)rg.first, 0);
     }
     std::sort(ranges.begin(), ranges.end());
     for (size_t k = 1; k < ranges.size(); ++k)

@@ -392,9 +392,10 @@ int32_t persistent_groups(const zdx_ctx* c, int per_cu) { return std::max(1, c->
 // asked for, so twice the largest one sizes every shard (the shards' loads
 // differ and change from step to step; the arena at least doubles).
 int64_t grown_hit_cap(const unsigned long long* d_top, int shards, int64_t cap) {
-    std::vector<unsigned long long> top(shards, 0);
-    HIPC(hipMemcpy(top.data(), d_top, shards * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    const unsigned long long want = *std::max_element(top.begin(), top.end());
+    std::vector<unsigned long long> top((size_t)shards * HIT_TOP_STRIDE, 0);
+    HIPC(hipMemcpy(top.data(), d_top, top.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    unsigned long long want = 0;
+    for (int k = 0; k < shards; ++k) want = std::max(want, top[(size_t)k * HIT_TOP_STRIDE]);
     return std::max<int64_t>(2 * cap, (int64_t)shards * (int64_t)(2 * want + 1));
 }
 
@@ -611,7 +612,7 @@ struct zdx_sim {
         d_adj_j.ensure(adj_cap); d_adj_pair.ensure(adj_cap);
         d_order.ensure(cap); d_items.ensure(cap); d_heavy.ensure(cap); d_topo_list.ensure(cap); d_topo_n.ensure(1);
         d_topo_big.ensure(cap); d_topo_small.ensure(cap); d_topo_nbs.ensure(2);
-        d_hit_off.ensure(cap); d_hit_top.ensure(HIT_SHARDS);
+        d_hit_off.ensure(cap); d_hit_top.ensure(HIT_SHARDS * HIT_TOP_STRIDE);
         d_seg_off.ensure(cap); d_tri_checks.ensure(cap); d_tri_tests.ensure(cap);
         d_nseg.ensure(cap); d_tri_hits.ensure(cap); d_tri_scanned.ensure(cap);
         d_ncontrib.ensure(cap); d_comps.ensure(cap); d_loops_raw.ensure(cap); d_tstatus.ensure(cap);
@@ -819,7 +820,7 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
         z.i32[4] = (int32_t*)s->d_queue.p; z.n32[4] = NARROW_COUNTERS;
         z.i32[5] = batched ? s->d_bcnt.p : nullptr; z.n32[5] = 64;
         z.i32[6] = s->d_topo_nbs.p; z.n32[6] = 2;
-        z.u64[0] = s->d_hit_top.p; z.n64[0] = HIT_SHARDS;
+        z.u64[0] = s->d_hit_top.p; z.n64[0] = HIT_SHARDS * HIT_TOP_STRIDE;
         z.u64[1] = s->tw.top.p; z.n64[1] = 5;
         launch_step_begin(kt, z, st);
     }
@@ -1087,7 +1088,7 @@ void contact_pairs_impl(zdx_ctx* c, zdx_ctx::Api& A, const zdx_pair_in* pairs, i
         launch_fill_i32(A.npairs.p, n, 1, st);
         A.seg_off.ensure(n); A.tri_checks.ensure(n); A.tri_tests.ensure(n); A.nseg.ensure(n); A.tri_hits.ensure(n); A.tri_scanned.ensure(n);
         A.queue.ensure(NARROW_COUNTERS); A.heavy.ensure(n); A.bcnt.ensure(64); A.bcur.ensure(64);
-        A.hit_off.ensure(n); A.hit_top.ensure(HIT_SHARDS);
+        A.hit_off.ensure(n); A.hit_top.ensure(HIT_SHARDS * HIT_TOP_STRIDE);
         A.ncontrib.ensure(n); A.comps.ensure(n); A.loops_raw.ensure(n); A.tstatus.ensure(n);
         A.topo_list.ensure(n); A.topo_n.ensure(1);
         const bool batched = opts->mode == ZDX_MODE_BATCHED;

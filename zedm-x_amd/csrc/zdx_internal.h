@@ -173,8 +173,10 @@ struct NarrowItem {
 // The hit arena is cut into shards, each with its own counter: a pair's
 // hits go to shard (work item % hit_shards), so that the reservations are
 // spread over that many addresses (atomics on one address serialise, at about
-// 11.5 ns each on MI355X).
+// 11.5 ns each on MI355X, and so do atomics on one 128-byte line: each counter
+// has a line of its own, HIT_TOP_STRIDE apart).
 constexpr int HIT_SHARDS = 64;
+constexpr int HIT_TOP_STRIDE = 16;   // in 8-byte counters
 
 struct DPairs {
     int32_t cap;             // capacity of the pair arrays
@@ -203,7 +205,7 @@ struct DPairs {
                              // world-vertex indices of (ta, tb); its slots are also the segment slots
     int32_t* hit_tri;        // optional (sensitivity runs): (ta, tb) per hit
     int64_t hit_cap;         // in hits; shard k holds [k, k + 1) * (hit_cap / hit_shards)
-    unsigned long long* hit_top;   // hit_shards counters: hits each shard's pairs asked for
+    unsigned long long* hit_top;   // hit_shards counters, HIT_TOP_STRIDE apart: hits each shard's pairs asked for
     int32_t hit_shards;      // 1 .. HIT_SHARDS
     int64_t* hit_off;        // per pair: first hit (count in tri_hits); the pair's segments start there
     int32_t* hit_flag;       // per hit: 1 when the exact test produced a segment

@@ -634,7 +634,7 @@ __global__ void __launch_bounds__(256) k_narrow_exact(DBodies B, DPairs P) {
         const int l = threadIdx.x;
         long long c = 0;
         if (l < P.hit_shards) {
-            const long long t = (long long)P.hit_top[l];
+            const long long t = (long long)P.hit_top[l * HIT_TOP_STRIDE];
             c = t < size ? t : size;
         }
         long long x = c;
@@ -1935,7 +1935,7 @@ void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const Na
                    unsigned long long* stamp) {
     if (!zeroed) {
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int32_t*)q, 0, nullptr, (int64_t)NQ_COUNTERS);
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, P.hit_top, 0ull, (int64_t)HIT_SHARDS);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, P.hit_top, 0ull, (int64_t)HIT_SHARDS * HIT_TOP_STRIDE);
     }
     // every pair through the one-wave kernel; the pairs it leaves (heavy)
     // through the workgroup kernel of the configured shape

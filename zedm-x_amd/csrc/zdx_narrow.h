@@ -88,7 +88,7 @@ __device__ inline unsigned long long reserve_hits(const DPairs& P, int item, int
     if (hits <= 0) return 0ull;
     const int sh = item % P.hit_shards;
     const unsigned long long size = (unsigned long long)(P.hit_cap / P.hit_shards);
-    const unsigned long long hb = atomicAdd(P.hit_top + sh, (unsigned long long)hits);
+    const unsigned long long hb = atomicAdd(P.hit_top + sh * HIT_TOP_STRIDE, (unsigned long long)hits);
     if (hb + (unsigned long long)hits > size) {
         atomicOr(P.status, (int)ST_HITS);
         return ~0ull;
@@ -607,10 +607,12 @@ __global__ void __launch_bounds__(BS, BS >= 512 ? 4 : (BS >= 256 ? 5 : 1)) k_nar
 // 4 waves share a CU's 160 KB.
 constexpr int NWV = 4;
 constexpr int WSLICE = 9728;
-// the narrow phase's counters (zeroed before it): WQ work queues of the wave
-// kernel, then the heavy list's count and the workgroup kernel's queue
+// the narrow phase's counters (zeroed before it), each on a 128-byte line of
+// its own (atomics on one line serialise): WQ work queues of the wave kernel,
+// then the heavy list's count and the workgroup kernel's queue
 constexpr int WQ = 64;
-constexpr int NQ_HEAVY = WQ, NQ_WG = WQ + 1, NQ_COUNTERS = WQ + 2;
+constexpr int QSTRIDE = 32;   // in 4-byte counters
+constexpr int NQ_HEAVY = WQ * QSTRIDE, NQ_WG = (WQ + 1) * QSTRIDE, NQ_COUNTERS = (WQ + 2) * QSTRIDE;
 constexpr int W_OC_MAX = 2048;
 constexpr int64_t W_HEAVY_TESTS = 256 * WAVE;
 __host__ __device__ constexpr int narrow_wave_lds_bytes() { return NWV * WSLICE; }
@@ -657,12 +659,15 @@ __global__ void __launch_bounds__(NWV * WAVE, 4) k_narrow_wave(DBodies B, DMeshe
     auto take = [&](int round) {
         int it = 0;
         if (use_queue) {
-            if (lane == 0) it = wq + nq * (int)atomicAdd(queue + wq, 1u);
+            if (lane == 0) it = wq + nq * (int)atomicAdd(queue + wq * QSTRIDE, 1u);
         } else {
             it = round * nw + ((round & 1) ? nw - 1 - gw : gw);
         }
         return it;
     };
+#ifdef ZDX_WALK_PROFILE
+    const unsigned long long np_entry = wall_clock64();
+#endif
     int round = 0;
     int item = uni(__shfl(take(0), 0, WAVE));
     NP_START();
@@ -847,7 +852,8 @@ __global__ void __launch_bounds__(NWV * WAVE, 4) k_narrow_wave(DBodies B, DMeshe
             r[4] = TA; r[5] = TB; r[6] = VA; r[7] = VB;
             long long tot = 0;
             for (int k = 0; k < 8; ++k) { r[8 + k] = np_ph[k]; tot += np_ph[k]; np_ph[k] = 0; }
-            r[16] = gw; r[17] = (long long)np_t0 - tot; r[18] = (long long)np_t0; r[19] = 1;
+            r[16] = gw; r[17] = (long long)np_t0 - tot; r[18] = (long long)np_t0; r[19] = 1 + 2 * (long long)__smid();
+            if (round == 1) r[2] = (long long)np_entry;   // the wave's first instruction (first item only)
         }
 #endif
         item = uni(__shfl(next, 0, WAVE));
