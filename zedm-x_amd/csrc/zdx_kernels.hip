@@ -609,6 +609,11 @@ __device__ unsigned long long g_walk_prof[16];
 __device__ long long g_topo_pairs[4 * 8192];    // per topology item: S, pass A ticks, walk D ticks
 __device__ long long g_walk_pairs[12 * 8192];   // walk B per pair: S, R, scratch in LDS, ticks, phases 0-7
 __device__ long long g_walkd_pairs[4 * 8192];   // walk D per pair: T-junctions, adjacency, start sort, loops
+// per topology item, absolute clocks: walk B's marks 0-15 (last call), chain
+// start, walk D start, walk D end, chain end, hardware unit, order, components,
+// snap hits
+__device__ long long g_walk_abs[32 * 8192];
+__device__ long long g_snap_hits[8192];
 #endif
 
 // ------------------------------------------------------------ narrow phase
@@ -923,6 +928,7 @@ struct LaneDev {
     // phase ticks accumulate in registers; the kernel stores them per pair
     mutable unsigned long long t0 = 0, t_begin = 0;
     mutable unsigned ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    mutable unsigned long long at[16] = {};
     __device__ void start() const {
         t0 = wall_clock64(); t_begin = t0;
         for (int k = 0; k < 8; ++k) ph[k] = 0;
@@ -931,6 +937,7 @@ struct LaneDev {
         asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");   // charge outstanding loads to this phase
         const unsigned long long t = wall_clock64();
         if (k < 8) ph[k] += (unsigned)(t - t0);
+        if (k < 16) at[k] = t;
         t0 = t;
     }
 #else
@@ -1201,6 +1208,14 @@ __device__ inline WalkB tp_walk_b_item(const DPairs& P, const DMeshes& M, const 
         g_walk_pairs[12 * it + 2] = ib + tp_walk_b_dbls(S) * 8 <= lds_bytes ? 1 : 0;
         g_walk_pairs[12 * it + 3] = (long long)(wall_clock64() - wp.t_begin);
         for (int k = 0; k < 8; ++k) g_walk_pairs[12 * it + 4 + k] = wp.ph[k];
+        for (int k = 0; k < 16; ++k) g_walk_abs[32 * it + k] = (long long)wp.at[k];
+    }
+    if (ao >= 0 && it < 8192) {
+        const int64_t np2 = (int64_t)(2 * S) * (2 * S - 1) / 2, nw = (np2 + 63) >> 6;
+        long long c = 0;
+        for (int64_t w = lane_id(); w < nw; w += WAVE) c += __popcll(T.snapw[so + w]);
+        for (int d = 32; d; d >>= 1) c += __shfl_xor(c, d);
+        if (lane_id() == 0) g_snap_hits[it] = c;
     }
 #endif
     long long to = -1;
@@ -1246,24 +1261,58 @@ __device__ inline void tp_pass_c_item(const DPairs& P, const DMeshes& M, const D
         const double* V = T.V + 6 * sb + 3 * (int64_t)cr[TPC_VB];
         const int* vc = T.vcell + 6 * sb + 3 * (int64_t)cr[TPC_VB];
         const int* E = T.E + 2 * sb + 2 * (int64_t)cr[TPC_EB];
+        uint64_t* w = T.tjw + to + cr[TPC_TW];
+        const int rl = 64 * tp_row_words(nv);
+        const int items = ne * rl;
         if (lds && 24 * nv + 12 * nv + 8 * ne <= lds_bytes) {
+            // the component in LDS; one thread per (edge, word of 64 vertices):
+            // the edge's cell range once, then its vertices in order, their
+            // cells read by every thread at once
             double* lv = (double*)lds;
             int* lc = (int*)(lds + 24 * nv);
             int* le = lc + 3 * nv;
             for (int k = threadIdx.x; k < 3 * nv; k += blockDim.x) { lv[k] = V[k]; lc[k] = vc[k]; }
             for (int k = threadIdx.x; k < 2 * ne; k += blockDim.x) le[k] = E[k];
             __syncthreads();
-            V = lv; vc = lc; E = le;
-        }
-        uint64_t* w = T.tjw + to + cr[TPC_TW];
-        const int rl = 64 * tp_row_words(nv);
-        const int items = ne * rl;
-        for (int kb = 0; kb < items; kb += blockDim.x) {
-            const int t = kb + threadIdx.x;
-            const int e = t / rl, v = t - (t / rl) * rl;
-            const bool hit = t < items && v < nv && tp_tj_hit(V, vc, E, h, e, v);
-            const uint64_t m = __ballot(hit);
-            if (lane_id() == 0 && t < items) w[t >> 6] = m;
+            const int rw = rl >> 6;
+            for (int u = threadIdx.x; u < ne * rw; u += blockDim.x) {
+                const int e = u / rw, wi = u - e * rw;
+                int cb[6];
+                tp_tj_box(lv, le, h, e, cb);
+                const int a = le[2 * e], b = le[2 * e + 1];
+                const int v0 = 64 * wi, v1 = nv < v0 + 64 ? nv : v0 + 64;
+                // cell test of four vertices at a time (their loads in flight
+                // together); the distance test for those that pass
+                uint64_t cand = 0;
+                for (int vb = v0; vb < v1; vb += 4) {
+                    int c[12];
+                    for (int r = 0; r < 4; ++r) {
+                        const int v = vb + r < v1 ? vb + r : vb;
+                        c[3 * r] = lc[3 * v]; c[3 * r + 1] = lc[3 * v + 1]; c[3 * r + 2] = lc[3 * v + 2];
+                    }
+                    for (int r = 0; r < 4; ++r) {
+                        const int v = vb + r;
+                        const bool in = v < v1 && v != a && v != b && c[3 * r] >= cb[0] && c[3 * r] <= cb[3] &&
+                                        c[3 * r + 1] >= cb[1] && c[3 * r + 1] <= cb[4] && c[3 * r + 2] >= cb[2] &&
+                                        c[3 * r + 2] <= cb[5];
+                        if (in) cand |= 1ull << (v - v0);
+                    }
+                }
+                uint64_t word = 0;
+                for (uint64_t m = cand; m; m &= m - 1) {
+                    const int v = v0 + (int)__builtin_ctzll(m);
+                    if (tp_tj_near(lv, h, a, b, v)) word |= 1ull << (v - v0);
+                }
+                w[u] = word;
+            }
+        } else {
+            for (int kb = 0; kb < items; kb += blockDim.x) {
+                const int t = kb + threadIdx.x;
+                const int e = t / rl, v = t - e * rl;
+                const bool hit = t < items && v < nv && tp_tj_hit(V, vc, E, h, e, v);
+                const uint64_t m = __ballot(hit);
+                if (lane_id() == 0 && t < items) w[t >> 6] = m;
+            }
         }
         __syncthreads();   // the staged component is replaced by the next one
     }
@@ -1307,15 +1356,25 @@ __device__ inline void tp_walk_d_sim_item(const DPairs& P, const DMeshes& M, con
     const int64_t key = ((int64_t)i << 32) | (int64_t)(uint32_t)j;
     V3 ds = v3(0.0, 0.0, 0.0);
     {
-        int lo = 0, hi = nprev - 1;
-        while (lo <= hi) {
-            const int mid = (lo + hi) >> 1;
-            const int64_t k = H.prev_key[mid];
-            if (k == key) {
-                if (H.prev_active[mid]) ds = ld3(H.prev_ds + 3 * (int64_t)mid);
-                break;
-            }
-            if (k < key) lo = mid + 1; else hi = mid - 1;
+        // 64-way search of the sorted keys: each round the lanes read 64
+        // evenly spaced keys of [lo, hi) and keep the slice that can hold key
+        int lo = 0, hi = nprev;
+        while (hi - lo > WAVE) {
+            const int64_t span = hi - lo;
+            const int pos = lo + (int)(span * lane_id() / WAVE);
+            const uint64_t le = __ballot(H.prev_key[pos] <= key);   // lanes in ascending key order
+            if (le == 0ull) { hi = lo; break; }
+            const int l = 63 - __builtin_clzll(le);                   // last lane with a key <= key
+            const int nlo = lo + (int)(span * l / WAVE);
+            hi = l == WAVE - 1 ? hi : lo + (int)(span * (l + 1) / WAVE);
+            lo = nlo;
+        }
+        const int pos = lo + lane_id();
+        const bool hit = pos < hi && H.prev_key[pos] == key;
+        const uint64_t hm = __ballot(hit);
+        if (hm) {
+            const int mid = lo + __builtin_ctzll(hm);
+            if (H.prev_active[mid]) ds = ld3(H.prev_ds + 3 * (int64_t)mid);
         }
     }
     SimSink sink{T.contrib + 9 * (int64_t)p * T.lmax, T.lmax, &a, &b, &ma, &mb, sp.dt, &ds, sp.force_model,
@@ -1330,6 +1389,8 @@ __device__ inline void tp_walk_d_sim_item(const DPairs& P, const DMeshes& M, con
 #ifdef ZDX_WALK_PROFILE
     if (lane_id() == 0 && it < 8192) {
         g_topo_pairs[4 * it + 2] = (long long)(wall_clock64() - wp.t_begin);
+        g_walk_abs[32 * it + 17] = (long long)wp.t_begin;
+        g_walk_abs[32 * it + 18] = (long long)wall_clock64();
         for (int k = 0; k < 4; ++k) g_walkd_pairs[4 * it + k] = wp.ph[k];
     }
 #endif
@@ -1376,7 +1437,14 @@ __global__ void __launch_bounds__(64) k_tp_walk_bcd_sim(DPairs P, DMeshes M, DBo
         __syncthreads();
         tp_walk_d_sim_item(P, M, B, mat, H, T, sp, nprev, lds_bytes, s_walk, wp, it, r.ncomp, r.to);
 #ifdef ZDX_WALK_PROFILE
-        if (lane_id() == 0 && it < 8192) g_topo_pairs[4 * it + 3] = (long long)(wall_clock64() - chain_t0);
+        if (lane_id() == 0 && it < 8192) {
+            g_topo_pairs[4 * it + 3] = (long long)(wall_clock64() - chain_t0);
+            g_walk_abs[32 * it + 16] = (long long)chain_t0;
+            g_walk_abs[32 * it + 19] = (long long)wall_clock64();
+            g_walk_abs[32 * it + 20] = (long long)__smid();
+            g_walk_abs[32 * it + 21] = k;
+            g_walk_abs[32 * it + 22] = r.ncomp;
+        }
 #endif
     }
 }
@@ -1861,6 +1929,12 @@ extern "C" int zdx_narrow_pairs(long long* out, int n) {
 }
 extern "C" int zdx_walk_pairs(long long* out, int n) {
     return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_walk_pairs), 12 * (size_t)n * sizeof(long long)) == hipSuccess ? 0 : 1;
+}
+extern "C" int zdx_walk_abs(long long* out, int n) {
+    return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_walk_abs), 32 * (size_t)n * sizeof(long long)) == hipSuccess ? 0 : 1;
+}
+extern "C" int zdx_snap_hits(long long* out, int n) {
+    return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_snap_hits), (size_t)n * sizeof(long long)) == hipSuccess ? 0 : 1;
 }
 extern "C" int zdx_walkd_pairs(long long* out, int n) {
     return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_walkd_pairs), 4 * (size_t)n * sizeof(long long)) == hipSuccess ? 0 : 1;

@@ -250,10 +250,20 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
     for (int k = ln; k < n; k += W) lab[k] = k;
     wp.sync();
     wp.mark(5);
+    // find with pointer jumping: every node passed is pointed at its
+    // grandparent.  A parent always has the smaller index and a node that is
+    // not a root never becomes one again, so these plain stores never touch a
+    // root (which only the compare and swap changes) and only ever point a
+    // node at one of its ancestors: the forest and its partition are kept.
     auto root = [&](int x) {
         int y = lab[x];
-        while (y != x) { x = y; y = lab[x]; }
-        return x;
+        if (y == x) return x;
+        int z = lab[y];
+        while (z != y) {
+            lab[x] = z;
+            x = y; y = z; z = lab[y];
+        }
+        return y;
     };
     const int64_t npairs = (int64_t)n * (n - 1) / 2;
     const int64_t nwords = (npairs + 63) >> 6;
@@ -340,6 +350,7 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
         wp.sync();
         for (int k = ln; k < np; k += W) wp.atomic_min(&firstp[find(ptk(k))], k);
         wp.sync();
+        wp.mark(8);
         // vertex id of a cluster = number of clusters that start before it
         int nv = 0;
         for (int kb = 0; kb < np; kb += W) {
@@ -350,6 +361,7 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
             if (f) rootvid[r] = nv + (int)__builtin_popcountll(fm & ((1ull << ln) - 1ull));
             nv += (int)__builtin_popcountll(fm);
         }
+        wp.mark(9);
         for (int v = ln; v < nv; v += W) cnt[v] = 0;
         wp.sync();
         for (int k = ln; k < np; k += W) {
@@ -359,6 +371,7 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
             wp.atomic_add(&cnt[v], 1);
         }
         wp.sync();
+        wp.mark(10);
         int acc = 0;
         for (int vb0 = 0; vb0 < nv; vb0 += W) {
             const int v = vb0 + ln;
@@ -370,6 +383,7 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
         }
         if (ln == 0) voff[nv] = acc;
         wp.sync();
+        wp.mark(11);
         // points grouped by vertex in point order: per chunk of W points, each
         // point's rank among the earlier points of its vertex in the chunk
         // (lanes with the same vertex found by ballots over the bits of the
@@ -391,11 +405,25 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
             if (act && below == 0ull) vfill[v] += (int)__builtin_popcountll(same);
             wp.sync();
         }
+        wp.mark(12);
         for (int v = ln; v < nv; v += W) {
             const int m0 = voff[v], m1 = voff[v + 1];
             const double* p = seg + 3 * mem[m0];     // point pt at seg + 6 (pt >> 1) + 3 (pt & 1) = seg + 3 pt
             double sx = p[0], sy = p[1], sz = p[2];
-            for (int m = m0 + 1; m < m1; ++m) {
+            int m = m0 + 1;
+            for (; m + 4 <= m1; m += 4) {   // four members' loads in flight, summed in member order
+                const double* q0 = seg + 3 * mem[m];
+                const double* q1 = seg + 3 * mem[m + 1];
+                const double* q2 = seg + 3 * mem[m + 2];
+                const double* q3 = seg + 3 * mem[m + 3];
+                const double x0 = q0[0], y0 = q0[1], z0 = q0[2], x1 = q1[0], y1 = q1[1], z1 = q1[2];
+                const double x2 = q2[0], y2 = q2[1], z2 = q2[2], x3 = q3[0], y3 = q3[1], z3 = q3[2];
+                sx += x0; sy += y0; sz += z0;
+                sx += x1; sy += y1; sz += z1;
+                sx += x2; sy += y2; sz += z2;
+                sx += x3; sy += y3; sz += z3;
+            }
+            for (; m < m1; ++m) {
                 const double* pm = seg + 3 * mem[m];
                 sx += pm[0]; sy += pm[1]; sz += pm[2];
             }
@@ -414,6 +442,7 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
                     for (int m = voff[v] + 1; m < voff[v + 1]; ++m) a += EJ[36 * (int64_t)mem[m] + cj];
                     VJ[36 * (int64_t)(vb + v) + cj] = a / (double)(voff[v + 1] - voff[v]);
                 }
+        wp.mark(13);
         int ne = 0;
         for (int kb = 0; kb < ns; kb += W) {   // edges in segment order (ballot prefix)
             const int k = kb + ln;
@@ -448,25 +477,41 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
 
 // ---------------------------------------------------------------- pass C
 // split_t_junctions candidate test (main.cpp:908-935) of vertex v against
-// edge e of one component (V, vcell: that component's vertices).
-ZDX_HD inline bool tp_tj_hit(const double* V, const int* vcell, const int* E, double h, int e, int v) {
+// edge e of one component (V, vcell: that component's vertices), in two
+// parts: the cell range of the edge's box grown by h (cb: 6 ints, lower then
+// upper cell), which depends on the edge only, and the test of one vertex.
+ZDX_HD inline void tp_tj_box(const double* V, const int* E, double h, int e, int* cb) {
     const int a = E[2 * e], b = E[2 * e + 1];
-    if (v == a || v == b) return false;
     const V3 pa = v3(V[3 * a], V[3 * a + 1], V[3 * a + 2]), pb = v3(V[3 * b], V[3 * b + 1], V[3 * b + 2]);
     const V3 mn = v3(dmin(pa.x, pb.x) - h, dmin(pa.y, pb.y) - h, dmin(pa.z, pb.z) - h);
     const V3 mx = v3(dmax(pa.x, pb.x) + h, dmax(pa.y, pb.y) + h, dmax(pa.z, pb.z) + h);
     const I3 c0 = cell_of(mn, h), c1 = cell_of(mx, h);
-    const int cx = vcell[3 * v], cy = vcell[3 * v + 1], cz = vcell[3 * v + 2];
-    if (cx < c0.x || cx > c1.x || cy < c0.y || cy > c1.y || cz < c0.z || cz > c1.z) return false;
+    cb[0] = c0.x; cb[1] = c0.y; cb[2] = c0.z; cb[3] = c1.x; cb[4] = c1.y; cb[5] = c1.z;
+}
+// the distance test of vertex v against the edge (a, b), after the cell test
+ZDX_HD inline bool tp_tj_near(const double* V, double h, int a, int b, int v) {
+    const V3 pa = v3(V[3 * a], V[3 * a + 1], V[3 * a + 2]), pb = v3(V[3 * b], V[3 * b + 1], V[3 * b + 2]);
     double d2, t;
     pt_seg(v3(V[3 * v], V[3 * v + 1], V[3 * v + 2]), pa, pb, d2, t);
     return d2 <= h * h && t > 1e-6 && t < 1.0 - 1e-6;
 }
+ZDX_HD inline bool tp_tj_test(const double* V, const int* vcell, const int* E, double h, int e, int v, const int* cb) {
+    const int a = E[2 * e], b = E[2 * e + 1];
+    if (v == a || v == b) return false;
+    const int cx = vcell[3 * v], cy = vcell[3 * v + 1], cz = vcell[3 * v + 2];
+    if (cx < cb[0] || cx > cb[3] || cy < cb[1] || cy > cb[4] || cz < cb[2] || cz > cb[5]) return false;
+    return tp_tj_near(V, h, a, b, v);
+}
+ZDX_HD inline bool tp_tj_hit(const double* V, const int* vcell, const int* E, double h, int e, int v) {
+    int cb[6];
+    tp_tj_box(V, E, h, e, cb);
+    return tp_tj_test(V, vcell, E, h, e, v, cb);
+}
 
 // ---------------------------------------------------------------- walk D
 // Scratch of walk D for a pair of S segments (vertices <= 2S, split edges <= 4S).
-ZDX_HD inline int64_t tp_d_ints(int S) { return 4 * (int64_t)(2 * S) + 2 * (4 * (int64_t)S) * 3 + 2 * (4 * (int64_t)S) + 2 * (2 * (int64_t)S) + 8; }
-ZDX_HD inline int64_t tp_d_dbls(int S) { return 3 * (2 * (int64_t)S) + 2 * (int64_t)S + 8; }
+ZDX_HD inline int64_t tp_d_ints(int S) { return 4 * (int64_t)(2 * S) + 2 * (4 * (int64_t)S) * 3 + 2 * (4 * (int64_t)S) + 8; }
+ZDX_HD inline int64_t tp_d_dbls(int S) { return 3 * (2 * (int64_t)S) + 4 * (int64_t)S + 8; }
 
 // Sensitivities of a loop's Sn and Gn (accumulate_Sn_Gn_from_polyline) to
 // the motion parameters, from the vertex Jacobians VJ (36 per vertex):
@@ -532,10 +577,8 @@ ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double 
     int* adje = adjn + 2 * E2c;  // 2 E2c
     int* used = adje + 2 * E2c;  // E2c
     int* path = used + E2c;      // E2c + 1
-    int* spv = path + E2c + 1;   // NV  hits of one edge: vertex
-    int* spk = spv + NV;         // NV  scan key
     double* Vs = D;              // 3 NV  this component's vertices
-    double* spt = Vs + 3 * NV;   // NV? (2S) hit parameter t
+    double* ht = Vs + 3 * NV;    // E2c  T-junction hits: parameter t
     for (int q = 0; q < ncomp; ++q) {
         const int* cr = comp + TPC_N * q;
         const int nv = cr[TPC_NV], ne = cr[TPC_NE];
@@ -543,90 +586,101 @@ ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double 
         const int* vcq = vcell + 3 * (int64_t)cr[TPC_VB];
         const int* Eq = E + 2 * (int64_t)cr[TPC_EB];
         const uint64_t* Tq = tjw + cr[TPC_TW];
-        const int rw = tp_row_words(nv);
         for (int k = ln; k < 3 * nv; k += W) Vs[k] = Vq[k];
         wp.sync();
         auto VV = [&](int v) { return v3(Vs[3 * v], Vs[3 * v + 1], Vs[3 * v + 2]); };
         // ---- split_t_junctions: per edge, its hits in vertex order, put in
-        // the reference's cell-scan order, std::sort by t, chained
-        int ne2 = 0;
-        // with at most W words per row, lane w's word of the rows of the next
-        // four edges is already in flight while an edge is split
-        const bool ring = rw <= W;
-        auto row_word = [&](int e) { return (ring && e < ne && ln < rw) ? Tq[(int64_t)e * rw + ln] : 0ull; };
-        struct Ends { int x, y; };
-        auto ends = [&](int e) { Ends r{0, 0}; if (e < ne) { r.x = Eq[2 * e]; r.y = Eq[2 * e + 1]; } return r; };
-        uint64_t q0 = row_word(0), q1 = row_word(1), q2 = row_word(2), q3 = row_word(3);
-        Ends g0 = ends(0), g1 = ends(1), g2 = ends(2), g3 = ends(3);
-        for (int e = 0; e < ne; ++e) {
-            const int a = g0.x, b = g0.y;
-            const uint64_t* row = Tq + (int64_t)e * rw;
-            const uint64_t cur = q0;
-            q0 = q1; q1 = q2; q2 = q3;
-            q3 = row_word(e + 4);
-            g0 = g1; g1 = g2; g2 = g3;
-            g3 = ends(e + 4);
+        // the reference's cell-scan order, std::sort by t, chained.  The hit
+        // vertices of an edge are distinct and never its ends, so edge e
+        // becomes exactly hits(e) + 1 edges: the output offsets are a scan of
+        // the row popcounts, and the lanes split their own edges (each lane
+        // sorts its edge's hits in its own slice of the free adjacency arrays).
+        const int rw = tp_row_words(nv);
+        int* hv = adjn;                       // E2c  hit vertex (adjn is free until the adjacency below)
+        int* hk = adjn + E2c;                 // E2c  scan key
+        int ne2 = 0, nh = 0;
+        for (int eb0 = 0; eb0 < ne; eb0 += W) {
+            const int e = eb0 + ln;
             int nsp = 0;
-            for (int wb = 0; wb < rw; wb += W) {
-                const int w = wb + ln;
-                const uint64_t word = ring ? cur : (w < rw ? row[w] : 0ull);
-                uint64_t m = wp.ballot(word != 0ull);
-                while (m) {
-                    const int src = ctz64(m);
-                    m &= m - 1;
-                    uint64_t bits = wp.shfl64(word, src);
-                    while (bits) {
-                        const int v = ((wb + src) << 6) + ctz64(bits);
-                        bits &= bits - 1;
-                        spv[nsp] = v;
-                        ++nsp;
+            if (e < ne)
+                for (int w = 0; w < rw; ++w) nsp += __builtin_popcountll(Tq[(int64_t)e * rw + w]);
+            int tot_h;
+            const int h0 = nh + wp.scan_excl(nsp, tot_h);   // first hit slot of edge e
+            const int o0 = ne2 + (h0 - nh) + (e - eb0);        // first output edge of edge e
+            const int tot_e = tot_h + (ne - eb0 < W ? ne - eb0 : W);
+            if (ne2 + tot_e > E2c) { out.status |= TOPO_E2_OVERFLOW; return out; }
+            // the hits of edge e get their parameter and scan key, then are
+            // sorted and chained by the edge's own lane when they are at most
+            // 16 (the sort is then an insertion sort); longer edges follow
+            // below, sorted by the whole wave
+            auto split_keys = [&](int a, int b, int hb, int j0, int j1, int dj) {
+                const V3 pa = VV(a), pb = VV(b);
+                const V3 mn = v3(dmin(pa.x, pb.x) - h, dmin(pa.y, pb.y) - h, dmin(pa.z, pb.z) - h);
+                const V3 mx = v3(dmax(pa.x, pb.x) + h, dmax(pa.y, pb.y) + h, dmax(pa.z, pb.z) + h);
+                const I3 c0 = cell_of(mn, h), c1 = cell_of(mx, h);
+                const int nyz = (c1.y - c0.y + 1) * (c1.z - c0.z + 1);
+                for (int j = j0; j < j1; j += dj) {
+                    const int v = hv[hb + j];
+                    double d2, t;
+                    pt_seg(VV(v), pa, pb, d2, t);
+                    ht[hb + j] = t;
+                    // the reference scans cells ix, iy, iz over [c0, c1]; key = scan index
+                    hk[hb + j] = (vcq[3 * v] - c0.x) * nyz + (vcq[3 * v + 1] - c0.y) * (c1.z - c0.z + 1) +
+                                 (vcq[3 * v + 2] - c0.z);
+                }
+            };
+            auto split_sort = [&](int hb, int n) {
+                SplitArr sa; sa.t = SArr<double>{ht + hb, 1}; sa.v = SArr<int>{hv + hb, 1}; sa.k = SArr<int>{hk + hb, 1};
+                SplitByScan byscan;
+                for (int i2 = 1; i2 < n; ++i2) {
+                    Split x = sa.get(i2);
+                    int j2 = i2 - 1;
+                    while (j2 >= 0 && byscan(x, sa.get(j2))) { sa.set(j2 + 1, sa.get(j2)); --j2; }
+                    sa.set(j2 + 1, x);
+                }
+                SplitByT byt;
+                ref_sort(sa, n, byt);
+            };
+            constexpr int LANE_SORT = 16;
+            if (e < ne) {
+                const int a = Eq[2 * e], b = Eq[2 * e + 1];
+                int k = 0;
+                for (int w = 0; w < rw; ++w)
+                    for (uint64_t bits = Tq[(int64_t)e * rw + w]; bits; bits &= bits - 1) hv[h0 + k++] = (w << 6) + ctz64(bits);
+                if (nsp <= LANE_SORT) {
+                    if (nsp > 0) {
+                        split_keys(a, b, h0, 0, nsp, 1);
+                        split_sort(h0, nsp);
                     }
+                    int prevv = a;
+                    for (int j = 0; j < nsp; ++j) {
+                        const int v = hv[h0 + j];
+                        E2[2 * (o0 + j)] = prevv; E2[2 * (o0 + j) + 1] = v;
+                        prevv = v;
+                    }
+                    E2[2 * (o0 + nsp)] = prevv; E2[2 * (o0 + nsp) + 1] = b;
                 }
             }
-            if (nsp == 0) {
-                if (ne2 + 1 > E2c) { out.status |= TOPO_E2_OVERFLOW; return out; }
-                E2[2 * ne2] = a; E2[2 * ne2 + 1] = b; ++ne2;
-                continue;
-            }
-            const V3 pa = VV(a), pb = VV(b);
-            const V3 mn = v3(dmin(pa.x, pb.x) - h, dmin(pa.y, pb.y) - h, dmin(pa.z, pb.z) - h);
-            const V3 mx = v3(dmax(pa.x, pb.x) + h, dmax(pa.y, pb.y) + h, dmax(pa.z, pb.z) + h);
-            const I3 c0 = cell_of(mn, h), c1 = cell_of(mx, h);
-            const int nyz = (c1.y - c0.y + 1) * (c1.z - c0.z + 1);
-            for (int k = 0; k < nsp; ++k) {
-                const int v = spv[k];
-                double d2, t;
-                pt_seg(VV(v), pa, pb, d2, t);
-                spt[k] = t;
-                // the reference scans cells ix, iy, iz over [c0, c1]; key = scan index
-                spk[k] = (vcq[3 * v] - c0.x) * nyz + (vcq[3 * v + 1] - c0.y) * (c1.z - c0.z + 1) + (vcq[3 * v + 2] - c0.z);
-            }
-            SplitArr sa; sa.t = SArr<double>{spt, 1}; sa.v = SArr<int>{spv, 1}; sa.k = SArr<int>{spk, 1};
-            SplitByScan byscan;
-            for (int i2 = 1; i2 < nsp; ++i2) {
-                Split x = sa.get(i2);
-                int j2 = i2 - 1;
-                while (j2 >= 0 && byscan(x, sa.get(j2))) { sa.set(j2 + 1, sa.get(j2)); --j2; }
-                sa.set(j2 + 1, x);
-            }
-            SplitByT byt;
-            ref_sort(sa, nsp, byt);
-            int prevv = a, last = -1;
-            for (int k = 0; k < nsp; ++k) {
-                const int v = spv[k];
-                if (v == last) continue;
-                last = v;
-                if (prevv != v) {
-                    if (ne2 + 1 > E2c) { out.status |= TOPO_E2_OVERFLOW; return out; }
-                    E2[2 * ne2] = prevv; E2[2 * ne2 + 1] = v; ++ne2;
+            wp.sync();
+            for (uint64_t big = wp.ballot(e < ne && nsp > LANE_SORT); big; big &= big - 1) {
+                const int l = ctz64(big);
+                const int ee = wp.readlane(e, l), hb = wp.readlane(h0, l), ob = wp.readlane(o0, l);
+                const int n = wp.readlane(nsp, l);
+                const int a = Eq[2 * ee], b = Eq[2 * ee + 1];
+                split_keys(a, b, hb, ln, n, W);
+                wp.sync();
+                split_sort(hb, n);
+                wp.sync();
+                for (int j = ln; j <= n; j += W) {
+                    E2[2 * (ob + j)] = j == 0 ? a : hv[hb + j - 1];
+                    E2[2 * (ob + j) + 1] = j < n ? hv[hb + j] : b;
                 }
-                prevv = v;
+                wp.sync();
             }
-            if (prevv != b) {
-                if (ne2 + 1 > E2c) { out.status |= TOPO_E2_OVERFLOW; return out; }
-                E2[2 * ne2] = prevv; E2[2 * ne2 + 1] = b; ++ne2;
-            }
+            ne2 += tot_e;
+            nh += tot_h;
         }
+        wp.sync();
         wp.mark(0);
         // ---- extract_loops: degrees, adjacency offsets and the adjacency of
         // each vertex in edge order (the reference's append order), lane-parallel
