@@ -101,7 +101,7 @@ struct MeshRec {
 // edges), per work-list item (offsets, component counts), the bump-allocated
 // adjacency / snap / T-junction buffers and the walks' global scratch slices.
 struct TopoWork {
-    DBuf<int32_t> cell9, comp, vcell, E, ncomp;
+    DBuf<int32_t> cell9, comp, vcell, E, ncomp, lab;
     DBuf<double> V;
     DBuf<int32_t> adj;
     DBuf<uint64_t> snapw, tjw;
@@ -118,7 +118,7 @@ struct TopoWork {
 
     void ensure_arena(int64_t segs) {
         cell9.ensure(9 * (size_t)segs); comp.ensure(TPC_N * (size_t)segs); V.ensure(6 * (size_t)segs);
-        vcell.ensure(6 * (size_t)segs); E.ensure(2 * (size_t)segs);
+        vcell.ensure(6 * (size_t)segs); E.ensure(2 * (size_t)segs); lab.ensure(2 * (size_t)segs);
     }
     void ensure_items(int64_t items) {
         adj_off.ensure(items); snap_off.ensure(items); tj_off.ensure(items); ncomp.ensure(items);
@@ -166,7 +166,7 @@ struct TopoWork {
     }
     template <class F>
     void each_buf(F&& f) {
-        f(cell9); f(comp); f(vcell); f(E); f(ncomp); f(V); f(adj); f(snapw); f(tjw); f(top); f(units);
+        f(cell9); f(comp); f(vcell); f(E); f(ncomp); f(lab); f(V); f(adj); f(snapw); f(tjw); f(top); f(units);
         f(adj_off); f(snap_off); f(tj_off); f(scr_i); f(scr_d);
     }
     void fill(TopoBufs& T) {
@@ -176,6 +176,7 @@ struct TopoWork {
         T.tjw = tjw.p; T.tj_cap = tj_cap; T.top = top.p; T.adj_off = adj_off.p; T.snap_off = snap_off.p;
         T.units = units.p; T.units_cap = units_cap;
         T.tj_off = tj_off.p; T.ncomp = ncomp.p; T.comp = comp.p; T.V = V.p; T.vcell = vcell.p; T.E = E.p;
+        T.lab = lab.p;
         T.scr_i = scr_i.p; T.scr_d = scr_d.p; T.scr_ni = scr_ni; T.scr_nd = scr_nd;
     }
 };

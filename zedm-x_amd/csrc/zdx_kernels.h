@@ -100,6 +100,8 @@ struct TopoBufs {
     double* V;                 // 2 vertices (6 doubles) per arena segment
     int32_t* vcell;            // 2 vertex cells (6 ints) per arena segment
     int32_t* E;                // 1 edge (2 ints) per arena segment
+    int32_t* lab;              // 2 per arena segment: snap cluster (its smallest endpoint) of each endpoint, for
+                               // the pairs of the big list (k_tp_snap_cc); first entry -1: not computed
     // sensitivities (API runs with provenance; nullptr otherwise)
     double* ej;                // 2 endpoint Jacobians (72 doubles) per arena segment
     double* vj;                // 2 vertex Jacobians (72 doubles) per arena segment

@@ -1126,6 +1126,73 @@ __global__ void __launch_bounds__(NT_MAX) k_tp_pass_a(DPairs P, DMeshes M, DBodi
     }
 }
 
+// Snap clusters of the big-list pairs, before their walks: one 1024-thread
+// workgroup per pair takes all of the pair's snap hits (pass A's words) and
+// links them lock-free in LDS (a root only ever points to a smaller index, so
+// every cluster ends at its smallest point), regardless of the pair's
+// components; walk B uses the result when no cluster spans two components.
+constexpr int CC_THREADS = 1024, CC_POINTS = 8192;
+__global__ void __launch_bounds__(CC_THREADS) k_tp_snap_cc(DPairs P, TopoBufs T) {
+    __shared__ int s_lab[CC_POINTS];
+    __shared__ int s_hl[CC_THREADS];
+    __shared__ int s_wave[CC_THREADS / WAVE];
+    const int nb = *P.n_big;
+    const int tid = threadIdx.x;
+    auto root = [&](int x) {   // with pointer jumping (see tp_components_and_vertices)
+        int y = s_lab[x];
+        if (y == x) return x;
+        int z = s_lab[y];
+        while (z != y) {
+            s_lab[x] = z;
+            x = y; y = z; z = s_lab[y];
+        }
+        return y;
+    };
+    for (int u = blockIdx.x; u < nb; u += gridDim.x) {
+        const int it = P.topo_big[u];
+        const int p = P.topo_list[it];
+        const int n = 2 * P.nseg[p];
+        int* glab = T.lab + 2 * P.seg_off[p];
+        const long long ao = T.adj_off[it], so = T.snap_off[it];
+        if (ao < 0 || n > CC_POINTS) {
+            if (tid == 0) glab[0] = -1;
+            continue;
+        }
+        for (int k = tid; k < n; k += CC_THREADS) s_lab[k] = k;
+        __syncthreads();
+        const int64_t nwords = ((int64_t)n * (n - 1) / 2 + 63) >> 6;
+        const uint64_t* w = T.snapw + so;
+        for (int64_t wb = 0; wb < nwords; wb += CC_THREADS) {
+            uint64_t word = wb + tid < nwords ? w[wb + tid] : 0ull;
+            int total;
+            int g = block_excl<CC_THREADS>(__popcll(word), s_wave, total);
+            for (int j0 = 0; j0 < total; j0 += CC_THREADS) {
+                while (word && g < j0 + CC_THREADS) {
+                    s_hl[g - j0] = (tid << 6) + __builtin_ctzll(word);
+                    word &= word - 1;
+                    ++g;
+                }
+                __syncthreads();
+                if (j0 + tid < total) {
+                    int a, b;
+                    tp_pair_of((wb << 6) + s_hl[tid], a, b);
+                    int ra = root(a), rb = root(b);
+                    while (ra != rb) {
+                        if (ra < rb) { const int x = ra; ra = rb; rb = x; }
+                        const int old = atomicCAS(&s_lab[ra], ra, rb);
+                        if (old == ra) break;
+                        ra = root(old);
+                        rb = root(rb);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int k = tid; k < n; k += CC_THREADS) glab[k] = root(k);
+        __syncthreads();
+    }
+}
+
 // Sensitivity runs: Jacobian of every segment endpoint (endpoint_jacobian)
 // from the narrow phase's provenance; body A of a pair is pi, B is pj.
 __global__ void __launch_bounds__(256) k_tp_endpoint_jac(DPairs P, DMeshes M, DBodies B, TopoBufs T) {
@@ -1172,8 +1239,10 @@ struct WalkB {
     int ncomp;       // -1: not evaluated (the step is retried with larger buffers)
     long long to;    // first T-junction word of the item, -1: did not fit
 };
+// pre: the pair's snap clusters were found by k_tp_snap_cc
 __device__ inline WalkB tp_walk_b_item(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T,
-                                       int32_t split, int32_t lds_bytes, char* s_walk, const LaneDev& wp, int it) {
+                                       int32_t split, int32_t lds_bytes, char* s_walk, const LaneDev& wp, int it,
+                                       bool pre = false) {
     const int R = T.adj_row;
     const int p = P.topo_list[it];
     const int S = P.nseg[p];
@@ -1197,7 +1266,7 @@ __device__ inline WalkB tp_walk_b_item(const DPairs& P, const DMeshes& M, const 
             return tp_components_and_vertices(wp, seg, S, h, split != 0, adj, adj + S, R, T.snapw + so, I,
                                               T.comp + TPC_N * sb, T.V + 6 * sb, T.vcell + 6 * sb,
                                               T.E + 2 * sb, &tjn, T.ej ? T.ej + 72 * sb : nullptr,
-                                              T.vj ? T.vj + 72 * sb : nullptr);
+                                              T.vj ? T.vj + 72 * sb : nullptr, pre ? T.lab + 2 * sb : nullptr);
         });
     }
 #ifdef ZDX_WALK_PROFILE
@@ -1431,7 +1500,7 @@ __global__ void __launch_bounds__(64) k_tp_walk_bcd_sim(DPairs P, DMeshes M, DBo
 #ifdef ZDX_WALK_PROFILE
         const unsigned long long chain_t0 = wall_clock64();
 #endif
-        const WalkB r = tp_walk_b_item(P, M, B, T, sp.split, lds_bytes, s_walk, wp, it);
+        const WalkB r = tp_walk_b_item(P, M, B, T, sp.split, lds_bytes, s_walk, wp, it, k < nbig);
         __syncthreads();
         tp_pass_c_item(P, M, B, T, it, r.ncomp, r.to, s_walk, lds_bytes);
         __syncthreads();
@@ -2080,6 +2149,7 @@ static void launch_pass_a(const DPairs& P, const DMeshes& M, const DBodies& B, c
                            dim3((unsigned)T.pass_groups), dim3(TP_PASS_A_THREADS),
                            84 * TP_PASS_A_SEGS + TP_PASS_A_KEYS + 8 * TP_PASS_A_WORDS, sb, P, M, B, T,
                            (int32_t)TP_PASS_A_SEGS);
+        hipLaunchKernelGGL(k_tp_snap_cc, dim3(256), dim3(CC_THREADS), 0, sb, P, T);
         if (fork) (void)hipEventRecord(ev_join, side);
         constexpr int SK = TOPO_BIG_SEGS * TOPO_BIG_SEGS, SW = 64;
         hipLaunchKernelGGL((k_tp_pass_a<256, SK, SW, 1>), dim3((unsigned)T.pass_groups * 4), dim3(256),

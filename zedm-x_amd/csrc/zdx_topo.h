@@ -133,7 +133,8 @@ template <class WP>
 ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, int S, double h, bool split,
                                              const int* adjn, const int* adjl, int R, const uint64_t* snapw, int* I,
                                              int* comp, double* V, int* vcell, int* E, int* tj_words,
-                                             const double* EJ = nullptr, double* VJ = nullptr) {
+                                             const double* EJ = nullptr, double* VJ = nullptr,
+                                             const int* glab = nullptr) {
     const int W = WP::W;
     const int ln = wp.lane();
     const int n = 2 * S;
@@ -247,9 +248,18 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
     // roots lock-free: a root only ever points to a smaller index (compare and
     // swap), so lab[] stays a forest; then every point is pointed at its root.
     int* lab = par;            // 2S
-    for (int k = ln; k < n; k += W) lab[k] = k;
-    wp.sync();
-    wp.mark(5);
+    // glab: the snap clusters of the whole pair found beforehand (each point's
+    // smallest point), used when every cluster lies in one component: then
+    // no snap hit joins two components and they are the same partition
+    bool pre = false;
+    if (glab != nullptr && glab[0] >= 0) {
+        for (int k = ln; k < n; k += W) lab[k] = glab[k];
+        wp.sync();
+        bool bad = false;
+        if (split && ncomp > 1)
+            for (int k = ln; k < n; k += W) bad = bad || compof[k >> 1] != compof[lab[k] >> 1];
+        pre = wp.ballot(bad) == 0ull;
+    }
     // find with pointer jumping: every node passed is pointed at its
     // grandparent.  A parent always has the smaller index and a node that is
     // not a root never becomes one again, so these plain stores never touch a
@@ -265,70 +275,75 @@ ZDX_HD inline int tp_components_and_vertices(const WP& wp, const double* seg, in
         }
         return y;
     };
-    const int64_t npairs = (int64_t)n * (n - 1) / 2;
-    const int64_t nwords = (npairs + 63) >> 6;
-    // the set bits of each round of W words are spread over the lanes through
-    // a list of up to W hit indices (in rnk.., free until the numbering below:
-    // min(W, hits) <= n (n - 1) / 2 and <= 7 n ints are there), so the unions
-    // run on all lanes at once instead of one lane per word
-    int* hl = rnk;
-    auto unite = [&](int64_t t) {
-        int a, b;
-        tp_pair_of(t, a, b);
-        if (compof[a >> 1] != compof[b >> 1]) return;
-        int ra = root(a), rb = root(b);
-        while (ra != rb) {
-            if (ra < rb) { const int x = ra; ra = rb; rb = x; }   // link the larger root under the smaller
-            const int old = wp.atomic_cas(&lab[ra], ra, rb);
-            if (old == ra) break;
-            ra = root(old);
-            rb = root(rb);
-        }
-    };
-    // the words of SNAP_BATCH rounds are loaded together, the next batch's
-    // loads in flight while this batch's bits are united
-    constexpr int SNAP_BATCH = 4;
-    uint64_t nxt[SNAP_BATCH];
-    for (int r = 0; r < SNAP_BATCH; ++r) {
-        const int64_t w = (int64_t)r * W + ln;
-        nxt[r] = w < nwords ? snapw[w] : 0ull;
-    }
-    for (int64_t wb0 = 0; wb0 < nwords; wb0 += (int64_t)SNAP_BATCH * W) {
-        uint64_t cur[SNAP_BATCH];
+    if (!pre) {
+        for (int k = ln; k < n; k += W) lab[k] = k;
+        wp.sync();
+        wp.mark(5);
+        const int64_t npairs = (int64_t)n * (n - 1) / 2;
+        const int64_t nwords = (npairs + 63) >> 6;
+        // the set bits of each round of W words are spread over the lanes through
+        // a list of up to W hit indices (in rnk.., free until the numbering below:
+        // min(W, hits) <= n (n - 1) / 2 and <= 7 n ints are there), so the unions
+        // run on all lanes at once instead of one lane per word
+        int* hl = rnk;
+        auto unite = [&](int64_t t) {
+            int a, b;
+            tp_pair_of(t, a, b);
+            if (compof[a >> 1] != compof[b >> 1]) return;
+            int ra = root(a), rb = root(b);
+            while (ra != rb) {
+                if (ra < rb) { const int x = ra; ra = rb; rb = x; }   // link the larger root under the smaller
+                const int old = wp.atomic_cas(&lab[ra], ra, rb);
+                if (old == ra) break;
+                ra = root(old);
+                rb = root(rb);
+            }
+        };
+        // the words of SNAP_BATCH rounds are loaded together, the next batch's
+        // loads in flight while this batch's bits are united
+        constexpr int SNAP_BATCH = 4;
+        uint64_t nxt[SNAP_BATCH];
         for (int r = 0; r < SNAP_BATCH; ++r) {
-            cur[r] = nxt[r];
-            const int64_t w = wb0 + (int64_t)(SNAP_BATCH + r) * W + ln;
+            const int64_t w = (int64_t)r * W + ln;
             nxt[r] = w < nwords ? snapw[w] : 0ull;
         }
-        wp.mark(4);
-        for (int r = 0; r < SNAP_BATCH; ++r) {
-            const int64_t wb = wb0 + (int64_t)r * W;
-            if (wb >= nwords) break;
-            const int64_t w = wb + ln;
-            uint64_t word = cur[r];
-            if (W == 1) {   // one lane: take the bits in order
-                while (word) {
-                    const int64_t t = (w << 6) + ctz64(word);
-                    word &= word - 1;
-                    unite(t);
-                }
-                continue;
+        for (int64_t wb0 = 0; wb0 < nwords; wb0 += (int64_t)SNAP_BATCH * W) {
+            uint64_t cur[SNAP_BATCH];
+            for (int r = 0; r < SNAP_BATCH; ++r) {
+                cur[r] = nxt[r];
+                const int64_t w = wb0 + (int64_t)(SNAP_BATCH + r) * W + ln;
+                nxt[r] = w < nwords ? snapw[w] : 0ull;
             }
-            int nbits;
-            int g = wp.scan_excl(__builtin_popcountll(word), nbits);
-            for (int j0 = 0; j0 < nbits; j0 += W) {
-                while (word && g < j0 + W) {
-                    hl[g - j0] = (int)(((w - wb) << 6) + ctz64(word));   // index within the round
-                    word &= word - 1;
-                    ++g;
+            wp.mark(4);
+            for (int r = 0; r < SNAP_BATCH; ++r) {
+                const int64_t wb = wb0 + (int64_t)r * W;
+                if (wb >= nwords) break;
+                const int64_t w = wb + ln;
+                uint64_t word = cur[r];
+                if (W == 1) {   // one lane: take the bits in order
+                    while (word) {
+                        const int64_t t = (w << 6) + ctz64(word);
+                        word &= word - 1;
+                        unite(t);
+                    }
+                    continue;
                 }
-                wp.sync();
-                if (j0 + ln < nbits) unite((wb << 6) + hl[ln]);
-                wp.sync();
+                int nbits;
+                int g = wp.scan_excl(__builtin_popcountll(word), nbits);
+                for (int j0 = 0; j0 < nbits; j0 += W) {
+                    while (word && g < j0 + W) {
+                        hl[g - j0] = (int)(((w - wb) << 6) + ctz64(word));   // index within the round
+                        word &= word - 1;
+                        ++g;
+                    }
+                    wp.sync();
+                    if (j0 + ln < nbits) unite((wb << 6) + hl[ln]);
+                    wp.sync();
+                }
             }
         }
+        wp.mark(6);
     }
-    wp.mark(6);
     wp.sync();
     wp.mark(7);
     for (int k = ln; k < n; k += W) lab[k] = root(k);
