@@ -158,7 +158,7 @@ struct Run {
     int shards;  // hit-arena shards
 };
 
-// q: the wave kernel's queue, the heavy count, the workgroup kernel's queue
+// q: the wave kernel's queues, the heavy list's counts, the workgroup kernel's queue
 template <int BS, int K>
 static std::string launch_narrow_emu(const DBodies& B, const DMeshes& M, const DPairs& P, int32_t* li, float* lf,
                                      int32_t tmax, int groups, int use_queue, unsigned* q, bool wave) {
@@ -168,7 +168,8 @@ static std::string launch_narrow_emu(const DBodies& B, const DMeshes& M, const D
         if (!e.empty()) return "wave kernel: " + e;
     }
     return simt::launch((unsigned)groups, BS, (size_t)narrow_lds_bytes(BS),
-                        [&] { k_narrow<BS, K>(B, M, P, li, lf, tmax, use_queue, q + NQ_WG, P.heavy, q + NQ_HEAVY, nullptr); });
+                        [&] { k_narrow<BS, K>(B, M, P, li, lf, tmax, use_queue, q + NQ_WG, P.heavy, q + NQ_HEAVY,
+                                              q + NQ_HEAVY_BACK, nullptr); });
 }
 
 static int check_run(const Input& in, const std::vector<Expect>& ex, const Run& r, long& hits_total) {
@@ -312,7 +313,7 @@ static int check_run(const Input& in, const std::vector<Expect>& ex, const Run& 
     hits_total += sum;
     if (bad == 0)
         std::printf("ok %s: %d pairs (%u to the workgroup kernel), %ld hits, %ld barriers, %ld wave operations\n", tag,
-                    np, queue[NQ_HEAVY], (long)sum, simt::blk().barriers, simt::blk().waveops);
+                    np, queue[NQ_HEAVY] + queue[NQ_HEAVY_BACK], (long)sum, simt::blk().barriers, simt::blk().waveops);
     simt::blk().barriers = simt::blk().waveops = 0;
     return bad ? 1 : 0;
 }

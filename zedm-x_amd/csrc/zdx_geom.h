@@ -459,9 +459,11 @@ struct LaneHost {
     void atomic_min(int* p, int v) const { if (v < *p) *p = v; }
     void atomic_add(int* p, int v) const { *p += v; }
     int readlane(int v, int) const { return v; }
+    uint64_t readlane64(uint64_t v, int) const { return v; }
     int scan_excl(int v, int& total) const { total = v; return 0; }   // exclusive prefix over the lanes
     void sync() const {}
     void mark(int) const {}   // phase boundary (timed by profiling builds of the device policy)
+    void count(int) const {}  // work counter of profiling builds
 };
 
 // index of the lowest set bit (m != 0)

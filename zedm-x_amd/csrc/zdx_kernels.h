@@ -182,9 +182,9 @@ void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s);
 // pair may store (ST_STAGE beyond); queue: the cull's work counter; resets
 // the queue and P.hit_top itself unless the caller did (zeroed).
 // queue: NARROW_COUNTERS counters (the wave kernel's work queues, the heavy
-// list's count, the workgroup kernel's queue over it, each on a 128-byte line
-// of its own), zeroed here unless `zeroed`.
-constexpr int NARROW_COUNTERS = 66 * 32;
+// list's front and back counts, the workgroup kernel's queue over it, each on a
+// 128-byte line of its own), zeroed here unless `zeroed`.
+constexpr int NARROW_COUNTERS = 67 * 32;
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* list_i,
                    float* list_f, int32_t seg_cap, int32_t tmax, unsigned int* queue, hipStream_t s, bool zeroed = false,
                    unsigned long long* stamp = nullptr);

@@ -608,7 +608,8 @@ __global__ void __launch_bounds__(256) k_need_lists(int32_t n, const double* sta
 __device__ unsigned long long g_walk_prof[16];
 __device__ long long g_topo_pairs[4 * 8192];    // per topology item: S, pass A ticks, walk D ticks
 __device__ long long g_walk_pairs[12 * 8192];   // walk B per pair: S, R, scratch in LDS, ticks, phases 0-7
-__device__ long long g_walkd_pairs[4 * 8192];   // walk D per pair: T-junctions, adjacency, start sort, loops
+__device__ long long g_walkd_pairs[8 * 8192];   // walk D per pair: T-junctions, adjacency, start sort, loops;
+                                                // loop phase: start search, walks, Sn/Gn, force and record
 // per topology item, absolute clocks: walk B's marks 0-15 (last call), chain
 // start, walk D start, walk D end, chain end, hardware unit, order, components,
 // snap hits
@@ -914,6 +915,11 @@ struct LaneDev {
     __device__ void atomic_add(int* p, int v) const { atomicAdd(p, v); }
     // value of lane l (l the same in every lane)
     __device__ int readlane(int v, int l) const { return __builtin_amdgcn_readlane(v, l); }
+    __device__ uint64_t readlane64(uint64_t v, int l) const {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+        return ((uint64_t)hi << 32) | lo;
+    }
     __device__ int scan_excl(int v, int& total) const {
         int x = v;
         for (int d = 1; d < WAVE; d <<= 1) {
@@ -932,6 +938,7 @@ struct LaneDev {
     __device__ void start() const {
         t0 = wall_clock64(); t_begin = t0;
         for (int k = 0; k < 8; ++k) ph[k] = 0;
+        cnt = 0;
     }
     __device__ void mark(int k) const {
         asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");   // charge outstanding loads to this phase
@@ -940,9 +947,12 @@ struct LaneDev {
         if (k < 16) at[k] = t;
         t0 = t;
     }
+    mutable long long cnt = 0;
+    __device__ void count(int v) const { cnt += v; }
 #else
     __device__ void start() const {}
     __device__ void mark(int) const {}
+    __device__ void count(int) const {}
 #endif
 };
 
@@ -1460,7 +1470,10 @@ __device__ inline void tp_walk_d_sim_item(const DPairs& P, const DMeshes& M, con
         g_topo_pairs[4 * it + 2] = (long long)(wall_clock64() - wp.t_begin);
         g_walk_abs[32 * it + 17] = (long long)wp.t_begin;
         g_walk_abs[32 * it + 18] = (long long)wall_clock64();
-        for (int k = 0; k < 4; ++k) g_walkd_pairs[4 * it + k] = wp.ph[k];
+        for (int k = 0; k < 8; ++k) g_walkd_pairs[8 * it + k] = wp.ph[k];
+        g_walk_abs[32 * it + 23] = o.loops;
+        g_walk_abs[32 * it + 24] = o.loops_raw;
+        g_walk_abs[32 * it + 25] = wp.cnt;
     }
 #endif
     if (lane_id() == 0) {
@@ -2006,7 +2019,7 @@ extern "C" int zdx_snap_hits(long long* out, int n) {
     return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_snap_hits), (size_t)n * sizeof(long long)) == hipSuccess ? 0 : 1;
 }
 extern "C" int zdx_walkd_pairs(long long* out, int n) {
-    return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_walkd_pairs), 4 * (size_t)n * sizeof(long long)) == hipSuccess ? 0 : 1;
+    return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_walkd_pairs), 8 * (size_t)n * sizeof(long long)) == hipSuccess ? 0 : 1;
 }
 extern "C" int zdx_walk_profile(unsigned long long* out) {
     if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_walk_prof), 16 * sizeof(unsigned long long)) != hipSuccess) return 1;
@@ -2070,7 +2083,7 @@ template <int BS, int K>
 static void narrow_t(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,
                      int32_t tmax, unsigned int* q, hipStream_t s) {
     hipLaunchKernelGGL((k_narrow<BS, K>), dim3(cfg.groups), dim3(BS), (size_t)narrow_lds_bytes(BS), s, B, M, P, li,
-                       lf, tmax, cfg.use_queue, q + NQ_WG, P.heavy, q + NQ_HEAVY, nullptr);
+                       lf, tmax, cfg.use_queue, q + NQ_WG, P.heavy, q + NQ_HEAVY, q + NQ_HEAVY_BACK, nullptr);
 }
 
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* li, float* lf,

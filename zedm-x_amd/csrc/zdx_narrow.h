@@ -165,7 +165,7 @@ static_assert(5 * (narrow_lds_bytes(256) + 5400) <= 160 * 1024, "five 256-thread
 template <int BS, int K>
 __global__ void __launch_bounds__(BS, BS >= 512 ? 4 : (BS >= 256 ? 5 : 1)) k_narrow(DBodies B, DMeshes M, DPairs P, int32_t* list_i, float* list_f,
                                                int32_t tmax, int32_t use_queue, unsigned int* queue,
-                                               const int32_t* hlist, const unsigned* hcount,
+                                               const int32_t* hlist, const unsigned* hcount, const unsigned* hback,
                                                unsigned long long* stamp) {
     stamp_here(stamp);
     constexpr int VF = narrow_vf_cap(BS), KC = narrow_kept_cap(BS);
@@ -188,14 +188,18 @@ __global__ void __launch_bounds__(BS, BS >= 512 ? 4 : (BS >= 256 ? 5 : 1)) k_nar
     int32_t* VB3 = LA + 5 * (int64_t)tmax;
     float* FA = list_f + (int64_t)blockIdx.x * 12 * tmax;
     float* FBg = FA + 6 * (int64_t)tmax;
-    // the work: every pair, or (with hlist) the *hcount pairs listed there
-    const int nq = hlist ? (int)*hcount : *P.n;
+    // the work: every pair, or (with hlist) the pairs listed there: *hcount
+    // from the front of hlist, then *hback from its end backwards (the item
+    // count is the list's length)
+    const int hfront = hlist ? (int)*hcount : 0, hend = *P.n - 1;
+    const int nq = hlist ? hfront + (int)*hback : *P.n;
+    auto listed = [&](int k) { return k < hfront ? hlist[k] : hlist[hend - (k - hfront)]; };
     // first queue entry; with the work queue each workgroup asks for its next
     // entry while it works on the current one
     if (tid == 0) s_next = use_queue ? (int)atomicAdd(queue, 1u) : (int)blockIdx.x;
     __syncthreads();
     int qi = uni(s_next);
-    int item = qi < nq ? (hlist ? uni(hlist[qi]) : qi) : 0;
+    int item = qi < nq ? (hlist ? uni(listed(qi)) : qi) : 0;
     if (P.items && tid < 32 && qi < nq) ((uint32_t*)&s_it)[tid] = ((const uint32_t*)(P.items + item))[tid];
     __syncthreads();
     NP_START();
@@ -502,7 +506,7 @@ __global__ void __launch_bounds__(BS, BS >= 512 ? 4 : (BS >= 256 ? 5 : 1)) k_nar
         // the next work item, read now and stored to LDS after the sweep
         uint32_t nxt_word = 0u;
         const int nxt = uni(s_next);
-        const int nxt_item = nxt < nq ? (hlist ? hlist[nxt] : nxt) : 0;
+        const int nxt_item = nxt < nq ? (hlist ? listed(nxt) : nxt) : 0;
         if (P.items && tid < 32 && nxt < nq) nxt_word = ((const uint32_t*)(P.items + nxt_item))[tid];
         // ---- ordered sweep of the culled product into the hit arena.  Each
         // thread takes one contiguous range of the flattened (ta, tb) product,
@@ -609,10 +613,15 @@ constexpr int NWV = 4;
 constexpr int WSLICE = 9728;
 // the narrow phase's counters (zeroed before it), each on a 128-byte line of
 // its own (atomics on one line serialise): WQ work queues of the wave kernel,
-// then the heavy list's count and the workgroup kernel's queue
+// then the heavy list's two counts (front, back) and the workgroup kernel's queue
 constexpr int WQ = 64;
 constexpr int QSTRIDE = 32;   // in 4-byte counters
-constexpr int NQ_HEAVY = WQ * QSTRIDE, NQ_WG = (WQ + 1) * QSTRIDE, NQ_COUNTERS = (WQ + 2) * QSTRIDE;
+constexpr int NQ_HEAVY = WQ * QSTRIDE, NQ_WG = (WQ + 1) * QSTRIDE, NQ_HEAVY_BACK = (WQ + 2) * QSTRIDE,
+              NQ_COUNTERS = (WQ + 3) * QSTRIDE;
+// heavy pairs of at least this many float-box tests (or too many vertices for
+// the wave) go to the front of the heavy list, the others to its back, so that
+// the workgroup kernel starts the costliest first
+constexpr int W_HEAVY_FIRST = 16384;
 constexpr int W_OC_MAX = 2048;
 constexpr int64_t W_HEAVY_TESTS = 256 * WAVE;
 __host__ __device__ constexpr int narrow_wave_lds_bytes() { return NWV * WSLICE; }
@@ -635,9 +644,10 @@ __device__ inline void wave_sync() {
 // triangles by vertex outcodes into its LDS, makes the kept triangles' float
 // boxes there, sweeps their product with each lane on one contiguous range,
 // reserves the pair's exact share of the hit arena with one atomic and writes
-// the hits in (ta, tb) order.  A pair beyond the wave's limits is appended to hlist (its
-// count at queue[NQ_HEAVY]) for k_narrow, which writes all of that pair's
-// results.
+// the hits in (ta, tb) order.  A pair beyond the wave's limits is put in hlist
+// for k_narrow, which writes all of that pair's results: at its front (count at
+// queue[NQ_HEAVY]) when it is costly, else at its back (queue[NQ_HEAVY_BACK]).
+// hlist holds *P.n entries.
 template <int K>
 __global__ void __launch_bounds__(NWV * WAVE, 4) k_narrow_wave(DBodies B, DMeshes M, DPairs P, int32_t* hlist,
                                                              unsigned int* queue, int32_t use_queue,
@@ -758,7 +768,10 @@ __global__ void __launch_bounds__(NWV * WAVE, 4) k_narrow_wave(DBodies B, DMeshe
             NP_MARK(2);
         }
         if (heavy) {
-            if (lane == 0) hlist[atomicAdd(queue + NQ_HEAVY, 1u)] = item;
+            if (lane == 0) {
+                if (VA + VB > W_OC_MAX || (int64_t)nA * nB >= W_HEAVY_FIRST) hlist[atomicAdd(queue + NQ_HEAVY, 1u)] = item;
+                else hlist[npairs - 1 - (int)atomicAdd(queue + NQ_HEAVY_BACK, 1u)] = item;
+            }
         } else {
             wave_sync();
             // float boxes and world-vertex indices of the kept triangles

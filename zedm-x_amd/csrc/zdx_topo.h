@@ -745,6 +745,25 @@ ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double 
             if (act && below == 0ull) afill[v] += (int)__builtin_popcountll(same);
             wp.sync();
         }
+        // degree-2 vertices (most of a loop's): their two (edge, neighbour)
+        // entries in adjacency order, in E2's place (E2 is spent); -1: another degree
+        int* pk = E2;
+        for (int v = ln; v < nv; v += W) {
+            if (deg[v] == 2) {
+                const int o = aoff[v];
+                pk[4 * v] = adje[o]; pk[4 * v + 1] = adjn[o]; pk[4 * v + 2] = adje[o + 1]; pk[4 * v + 3] = adjn[o + 1];
+            } else {
+                pk[4 * v] = -1;
+            }
+        }
+        // the used flags of the edges, also as bits in the lanes' registers
+        // (edge e: bit e & 63 of lane e >> 6) when they fit, for the walks
+        const bool regbits = W == 64 && ne2 <= 64 * W;
+        uint64_t usedw = 0;
+        auto set_used = [&](int e) {
+            used[e] = 1;
+            if (regbits && ln == (e >> 6)) usedw |= 1ull << (e & 63);
+        };
         // start order: the sort runs on (key, vertex) with the keys in afill
         // (its cursors are spent); same permutation as sorting by StartOrder
         for (int e = ln; e < ne2; e += W) used[e] = 0;
@@ -773,6 +792,7 @@ ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double 
                     const uint64_t bm = wp.ballot(has);
                     if (bm) { found = b0 + ctz64(bm); break; }
                 }
+                wp.mark(4);
                 if (found >= nv) break;
                 si = found;
             }
@@ -782,28 +802,60 @@ ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double 
                 if (used[ei]) continue;
                 int plen = 0;
                 path[plen++] = sv;
-                int prev = sv, cur = adjn[k];
-                used[ei] = 1;
+                int prev = sv, cur = adjn[k], ein = ei;   // ein: the edge cur was reached by
+                set_used(ei);
                 path[plen++] = cur;
                 int steps = 0;
                 bool closed = false;
                 while (steps < 100000) {
                     steps += 1;
                     if (cur == sv) { closed = plen > 1; break; }
-                    int pick = -1, first = -1;
-                    for (int m = aoff[cur]; m < aoff[cur] + deg[cur]; ++m) {
-                        if (used[adje[m]]) continue;
-                        if (first < 0) first = m;
-                        if (adjn[m] != prev) { pick = m; break; }
+                    // the next edge: the first unused one of cur (adjacency
+                    // order) that does not lead back to prev, else the first
+                    // unused one
+                    int pe = 0, pn = 0;
+                    const int x0 = pk[4 * cur];
+                    if (x0 >= 0) {
+                        // degree 2: ein is used, so the other edge, if unused
+                        const int n0 = pk[4 * cur + 1], x1 = pk[4 * cur + 2], n1 = pk[4 * cur + 3];
+                        pe = x0 == ein ? x1 : x0;
+                        pn = x0 == ein ? n1 : n0;
+                        const bool u = regbits ? ((wp.readlane64(usedw, pe >> 6) >> (pe & 63)) & 1ull) != 0ull
+                                               : used[pe] != 0;
+                        if (u) break;
+                    } else {
+                        // the lanes take W entries at once
+                        const int co = aoff[cur], cd = deg[cur];
+                        int pick = -1, first = -1, fe = 0, fn = 0;
+                        for (int c0 = 0; c0 < cd && pick < 0; c0 += W) {
+                            const bool in = c0 + ln < cd;
+                            const int m = co + c0 + ln;
+                            const int e = in ? adje[m] : 0, nb = in ? adjn[m] : 0;
+                            const bool un = in && used[e] == 0;
+                            const uint64_t um = wp.ballot(un), pm = wp.ballot(un && nb != prev);
+                            if (first < 0 && um) {
+                                const int l = ctz64(um);
+                                first = c0 + l;
+                                fe = wp.readlane(e, l); fn = wp.readlane(nb, l);
+                            }
+                            if (pm) {
+                                const int l = ctz64(pm);
+                                pick = c0 + l;
+                                pe = wp.readlane(e, l); pn = wp.readlane(nb, l);
+                            }
+                        }
+                        if (first < 0) break;
+                        if (pick < 0) { pe = fe; pn = fn; }
                     }
-                    if (first < 0) break;
-                    if (pick < 0) pick = first;
-                    used[adje[pick]] = 1;
+                    set_used(pe);
                     prev = cur;
-                    cur = adjn[pick];
+                    cur = pn;
+                    ein = pe;
                     if (plen > E2c) { out.status |= TOPO_E2_OVERFLOW; return out; }
                     path[plen++] = cur;
                 }
+                wp.mark(5);
+                wp.count(plen);
                 if (!closed) continue;
                 --plen;  // drop the repeated start vertex
                 const int raw = out.loops_raw + nloops_here;
@@ -811,14 +863,20 @@ ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double 
                 if (plen < 3) continue;
                 // accumulate_Sn_Gn_from_polyline
                 V3 Sn = v3(0.0, 0.0, 0.0), Gn = v3(0.0, 0.0, 0.0);
-                for (int m = 0; m < plen; ++m) {
-                    const V3 p0 = VV(path[m]);
-                    const V3 p1 = VV(path[(m + 1) % plen]);
+                auto term = [&](const V3& p0, const V3& p1) {
                     Sn = add(Sn, mul(cross(p0, p1), 0.5));
                     const V3 dx = sub(p1, p0);
                     const double s = dot(p0, p1) + (1.0 / 3.0) * dot(dx, dx);
                     Gn = add(Gn, mul(dx, -(1.0 / 3.0) * s));
+                };
+                int m = 0;
+                for (; m + 4 <= plen; m += 4) {   // five vertices' loads in flight, summed in path order
+                    const int i0 = path[m], i1 = path[m + 1], i2 = path[m + 2], i3 = path[m + 3];
+                    const int i4 = path[m + 4 < plen ? m + 4 : 0];
+                    const V3 q0 = VV(i0), q1 = VV(i1), q2 = VV(i2), q3 = VV(i3), q4 = VV(i4);
+                    term(q0, q1); term(q1, q2); term(q2, q3); term(q3, q4);
                 }
+                for (; m < plen; ++m) term(VV(path[m]), VV(path[(m + 1) % plen]));
                 // contact_point_xc0
                 const double area = norm(Sn);
                 if (area < 1e-14) continue;
@@ -826,6 +884,7 @@ ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double 
                 const V3 xc = dvd(cross(nA, Gn), area);
                 if (area < area_eps) continue;
                 if (dot(nA, dirAB) < 0.0) nA = mul(nA, -1.0);
+                wp.mark(6);
                 if (VJ) {
                     tp_loop_jacobian(wp, path, plen, Vs, VJ + 36 * (int64_t)cr[TPC_VB], jac);
                     wp.sync();
@@ -833,6 +892,7 @@ ZDX_HD inline TopoOut tp_loops(const WP& wp, int S, int ncomp, double h, double 
                 out.status |= sink(out.loops, q, raw, Sn, Gn, nA, xc, area, plen, SArr<int>{path, 1},
                                    SArr<double>{Vs, 1}, VJ ? jac : nullptr);
                 out.loops += 1;
+                wp.mark(7);
             }
         }
         out.loops_raw += nloops_here;
