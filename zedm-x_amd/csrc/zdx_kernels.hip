@@ -1500,20 +1500,22 @@ __global__ void __launch_bounds__(64) k_tp_walk_d_sim(DPairs P, DMeshes M, DBodi
 // Walk B, pass C and walk D of one item in a row on one wave (the time step):
 // each pair goes through its whole chain without waiting for the slowest
 // pair of each phase.  The walks' LDS scratch is reused from one to the next.
+// mode 0: every item, the big list's first; 1: the small list; 2: the big list.
 __global__ void __launch_bounds__(64) k_tp_walk_bcd_sim(DPairs P, DMeshes M, DBodies B, DMat mat, HistBufs H,
-                                                        TopoBufs T, SimParams sp, int32_t lds_bytes) {
+                                                        TopoBufs T, SimParams sp, int32_t lds_bytes, int32_t mode) {
     extern __shared__ char s_walk[];
-    const int n = *P.topo_n;
     const int nprev = *H.prev_n;
     const int nbig = P.topo_big ? *P.n_big : 0;
+    const int n = mode == 0 ? *P.topo_n : (mode == 1 ? *P.n_small : nbig);
     LaneDev wp;
     for (int k = blockIdx.x; k < n; k += gridDim.x) {
-        // the pairs with many segments first, in workgroup order
-        const int it = !P.topo_big ? k : (k < nbig ? P.topo_big[k] : P.topo_small[k - nbig]);
+        const bool big = mode == 2 || (mode == 0 && k < nbig);
+        const int it = mode == 1 ? P.topo_small[k]
+                                 : (!P.topo_big ? k : (k < nbig ? P.topo_big[k] : P.topo_small[k - nbig]));
 #ifdef ZDX_WALK_PROFILE
         const unsigned long long chain_t0 = wall_clock64();
 #endif
-        const WalkB r = tp_walk_b_item(P, M, B, T, sp.split, lds_bytes, s_walk, wp, it, k < nbig);
+        const WalkB r = tp_walk_b_item(P, M, B, T, sp.split, lds_bytes, s_walk, wp, it, big && P.topo_big);
         __syncthreads();
         tp_pass_c_item(P, M, B, T, it, r.ncomp, r.to, s_walk, lds_bytes);
         __syncthreads();
@@ -2179,9 +2181,45 @@ static void launch_pass_a(const DPairs& P, const DMeshes& M, const DBodies& B, c
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
                      const TopoBufs& T, const SimParams& sp, hipStream_t s, bool zeroed, hipStream_t side,
                      hipEvent_t ev_fork, hipEvent_t ev_join) {
-    launch_pass_a(P, M, B, T, s, zeroed, side, ev_fork, ev_join);
-    hipLaunchKernelGGL(k_tp_walk_bcd_sim, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, mat, H, T,
-                       sp, T.lds_bytes);
+    constexpr int WALK_BIG_GROUPS = 256;
+    if (!(side && ev_fork && ev_join && P.topo_big) || T.nthreads < 2 * WALK_BIG_GROUPS) {
+        launch_pass_a(P, M, B, T, s, zeroed, side, ev_fork, ev_join);
+        hipLaunchKernelGGL(k_tp_walk_bcd_sim, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, mat, H,
+                           T, sp, T.lds_bytes, 0);
+        return;
+    }
+    // the big pairs (the long chains) on the side stream: pass A, their snap
+    // clusters and their walks, which so start without waiting for pass A of
+    // the small pairs; those and their walks on the main stream.  The two walk
+    // launches use disjoint slices of the global walk scratch.
+    if (!zeroed) launch_fill_u64(T.top, 0ull, 5, s);
+    hipLaunchKernelGGL(k_tp_alloc, dim3((unsigned)((T.pass_groups + 255) / 256 * 4)), dim3(256), 0, s, P, T);
+    (void)hipEventRecord(ev_fork, s);
+    (void)hipStreamWaitEvent(side, ev_fork, 0);
+    hipLaunchKernelGGL((k_tp_pass_a<TP_PASS_A_THREADS, TP_PASS_A_KEYS, TP_PASS_A_WORDS, 2>),
+                       dim3((unsigned)T.pass_groups), dim3(TP_PASS_A_THREADS),
+                       84 * TP_PASS_A_SEGS + TP_PASS_A_KEYS + 8 * TP_PASS_A_WORDS, side, P, M, B, T,
+                       (int32_t)TP_PASS_A_SEGS);
+    hipLaunchKernelGGL(k_tp_snap_cc, dim3(256), dim3(CC_THREADS), 0, side, P, T);
+    const int64_t gs = T.nthreads - WALK_BIG_GROUPS;
+    TopoBufs Tb = T;
+    Tb.scr_i = T.scr_i + gs * T.scr_ni;
+    Tb.scr_d = T.scr_d + gs * T.scr_nd;
+    hipLaunchKernelGGL(k_tp_walk_bcd_sim, dim3((unsigned)WALK_BIG_GROUPS), dim3(64), T.lds_bytes, side, P, M, B, mat,
+                       H, Tb, sp, T.lds_bytes, 2);
+    (void)hipEventRecord(ev_join, side);
+    constexpr int SK = TOPO_BIG_SEGS * TOPO_BIG_SEGS, SW = 64;
+    hipLaunchKernelGGL((k_tp_pass_a<256, SK, SW, 1>), dim3((unsigned)T.pass_groups * 4), dim3(256),
+                       84 * TOPO_BIG_SEGS + SK + 8 * SW, s, P, M, B, T, (int32_t)TOPO_BIG_SEGS);
+    // the small pairs' walks with the LDS their largest pair needs, so that
+    // many more of them fit a CU at once
+    const int Ss = TOPO_BIG_SEGS - 1;
+    int64_t need = std::max(tp_scratch_bytes(tp_walk_b_ints(Ss, T.adj_row), tp_walk_b_dbls(Ss)),
+                            tp_scratch_bytes(tp_walk_d_ints(Ss), tp_walk_d_dbls(Ss)));
+    need = std::min<int64_t>((need + 1023) & ~(int64_t)1023, T.lds_bytes);
+    hipLaunchKernelGGL(k_tp_walk_bcd_sim, dim3((unsigned)gs), dim3(64), (size_t)need, s, P, M, B, mat, H, T, sp,
+                       (int32_t)need, 1);
+    (void)hipStreamWaitEvent(s, ev_join, 0);
 }
 void launch_topo_api(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T, int32_t split,
                      hipStream_t s) {
