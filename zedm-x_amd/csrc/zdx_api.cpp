@@ -193,7 +193,7 @@ struct zdx_ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // third stream: pass A of the many-segment pairs beside that of the others
     hipStream_t stream3 = nullptr;
-    hipEvent_t ev_fork3 = nullptr, ev_join3 = nullptr;
+    hipEvent_t ev_fork3 = nullptr, ev_join3 = nullptr, ev_cc3 = nullptr;
     // timing events of the per-pair call: transform, narrow phase (the
     // attempt that fitted), topology -- start and end of each
     hipEvent_t ev_api[6] = {};
@@ -875,7 +875,8 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
     }
     HistBufs H = s->hist(par);
     launch_topo_zero(P, T, H.cur_active, s->pair_cap, st, batched ? kt + 3 : nullptr);
-    launch_topo_sim(P, M, B, mat, H, T, s->sp, st, true, s->ctx->stream3, s->ctx->ev_fork3, s->ctx->ev_join3);
+    launch_topo_sim(P, M, B, mat, H, T, s->sp, st, true, s->ctx->stream3, s->ctx->ev_fork3, s->ctx->ev_join3,
+                    s->ctx->ev_cc3);
     if (walls) HIPC(hipStreamWaitEvent(st, s->ctx->ev_join, 0));
     // stamps 4 (topology done, with any wait for the wall stream) and 5 (fold
     // start) come from the fold's first workgroup, 6 from the counters'
@@ -1266,6 +1267,7 @@ zdx_status zdx_create(zdx_ctx** out, int32_t device) {
         HIPC(hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking));
         HIPC(hipEventCreateWithFlags(&c->ev_fork3, hipEventDisableTiming));
         HIPC(hipEventCreateWithFlags(&c->ev_join3, hipEventDisableTiming));
+        HIPC(hipEventCreateWithFlags(&c->ev_cc3, hipEventDisableTiming));
         for (auto& e : c->ev_api) HIPC(hipEventCreate(&e));
         // this device's copy of the no-allocation scratch
         void* sp = nullptr;
@@ -1297,6 +1299,7 @@ void zdx_destroy(zdx_ctx* c) {
         if (e) (void)hipEventDestroy(e);
     if (c->ev_fork3) (void)hipEventDestroy(c->ev_fork3);
     if (c->ev_join3) (void)hipEventDestroy(c->ev_join3);
+    if (c->ev_cc3) (void)hipEventDestroy(c->ev_cc3);
     if (c->stream3) (void)hipStreamDestroy(c->stream3);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);

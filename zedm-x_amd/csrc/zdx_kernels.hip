@@ -1141,7 +1141,7 @@ __global__ void __launch_bounds__(NT_MAX) k_tp_pass_a(DPairs P, DMeshes M, DBodi
 // links them lock-free in LDS (a root only ever points to a smaller index, so
 // every cluster ends at its smallest point), regardless of the pair's
 // components; walk B uses the result when no cluster spans two components.
-constexpr int CC_THREADS = 1024, CC_POINTS = 8192;
+constexpr int CC_THREADS = 512, CC_POINTS = 8192;
 __global__ void __launch_bounds__(CC_THREADS) k_tp_snap_cc(DPairs P, TopoBufs T) {
     __shared__ int s_lab[CC_POINTS];
     __shared__ int s_hl[CC_THREADS];
@@ -1501,8 +1501,12 @@ __global__ void __launch_bounds__(64) k_tp_walk_d_sim(DPairs P, DMeshes M, DBodi
 // each pair goes through its whole chain without waiting for the slowest
 // pair of each phase.  The walks' LDS scratch is reused from one to the next.
 // mode 0: every item, the big list's first; 1: the small list; 2: the big list.
-__global__ void __launch_bounds__(64) k_tp_walk_bcd_sim(DPairs P, DMeshes M, DBodies B, DMat mat, HistBufs H,
-                                                        TopoBufs T, SimParams sp, int32_t lds_bytes, int32_t mode) {
+// WPS: waves per SIMD the registers are sized for -- 1 lets a long chain use
+// all it needs (over 256 VGPRs); 2 fits twice as many of the short chains of
+// the small list at once, each somewhat slower.
+template <int WPS>
+__global__ void __launch_bounds__(64, WPS) k_tp_walk_bcd_sim(DPairs P, DMeshes M, DBodies B, DMat mat, HistBufs H,
+                                                             TopoBufs T, SimParams sp, int32_t lds_bytes, int32_t mode) {
     extern __shared__ char s_walk[];
     const int nprev = *H.prev_n;
     const int nbig = P.topo_big ? *P.n_big : 0;
@@ -2180,18 +2184,22 @@ static void launch_pass_a(const DPairs& P, const DMeshes& M, const DBodies& B, c
 
 void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const DMat& mat, const HistBufs& H,
                      const TopoBufs& T, const SimParams& sp, hipStream_t s, bool zeroed, hipStream_t side,
-                     hipEvent_t ev_fork, hipEvent_t ev_join) {
-    constexpr int WALK_BIG_GROUPS = 256;
-    if (!(side && ev_fork && ev_join && P.topo_big) || T.nthreads < 2 * WALK_BIG_GROUPS) {
+                     hipEvent_t ev_fork, hipEvent_t ev_join, hipEvent_t ev_cc) {
+    constexpr int WALK_BIG_GROUPS = 128;   // the big list is short (pairs of 40 or more segments)
+    if (!(side && ev_fork && ev_join && ev_cc && P.topo_big) || T.nthreads < 2 * WALK_BIG_GROUPS) {
         launch_pass_a(P, M, B, T, s, zeroed, side, ev_fork, ev_join);
-        hipLaunchKernelGGL(k_tp_walk_bcd_sim, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, mat, H,
-                           T, sp, T.lds_bytes, 0);
+        hipLaunchKernelGGL(k_tp_walk_bcd_sim<1>, dim3((unsigned)T.nthreads), dim3(64), T.lds_bytes, s, P, M, B, mat,
+                           H, T, sp, T.lds_bytes, 0);
         return;
     }
     // the big pairs (the long chains) on the side stream: pass A, their snap
     // clusters and their walks, which so start without waiting for pass A of
-    // the small pairs; those and their walks on the main stream.  The two walk
-    // launches use disjoint slices of the global walk scratch.
+    // the small pairs; those and their walks on the main stream.  The small
+    // pairs' walks also wait for the snap clusters: a walk wave (over 256
+    // VGPRs) leaves too few registers on its SIMD for the multi-wave
+    // workgroups of pass A and k_tp_snap_cc, which would otherwise wait for
+    // the walks to end.  The two walk launches use disjoint slices of the
+    // global walk scratch.
     if (!zeroed) launch_fill_u64(T.top, 0ull, 5, s);
     hipLaunchKernelGGL(k_tp_alloc, dim3((unsigned)((T.pass_groups + 255) / 256 * 4)), dim3(256), 0, s, P, T);
     (void)hipEventRecord(ev_fork, s);
@@ -2201,12 +2209,13 @@ void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const 
                        84 * TP_PASS_A_SEGS + TP_PASS_A_KEYS + 8 * TP_PASS_A_WORDS, side, P, M, B, T,
                        (int32_t)TP_PASS_A_SEGS);
     hipLaunchKernelGGL(k_tp_snap_cc, dim3(256), dim3(CC_THREADS), 0, side, P, T);
+    (void)hipEventRecord(ev_cc, side);
     const int64_t gs = T.nthreads - WALK_BIG_GROUPS;
     TopoBufs Tb = T;
     Tb.scr_i = T.scr_i + gs * T.scr_ni;
     Tb.scr_d = T.scr_d + gs * T.scr_nd;
-    hipLaunchKernelGGL(k_tp_walk_bcd_sim, dim3((unsigned)WALK_BIG_GROUPS), dim3(64), T.lds_bytes, side, P, M, B, mat,
-                       H, Tb, sp, T.lds_bytes, 2);
+    hipLaunchKernelGGL(k_tp_walk_bcd_sim<1>, dim3((unsigned)WALK_BIG_GROUPS), dim3(64), T.lds_bytes, side, P, M, B,
+                       mat, H, Tb, sp, T.lds_bytes, 2);
     (void)hipEventRecord(ev_join, side);
     constexpr int SK = TOPO_BIG_SEGS * TOPO_BIG_SEGS, SW = 64;
     hipLaunchKernelGGL((k_tp_pass_a<256, SK, SW, 1>), dim3((unsigned)T.pass_groups * 4), dim3(256),
@@ -2217,7 +2226,8 @@ void launch_topo_sim(const DPairs& P, const DMeshes& M, const DBodies& B, const 
     int64_t need = std::max(tp_scratch_bytes(tp_walk_b_ints(Ss, T.adj_row), tp_walk_b_dbls(Ss)),
                             tp_scratch_bytes(tp_walk_d_ints(Ss), tp_walk_d_dbls(Ss)));
     need = std::min<int64_t>((need + 1023) & ~(int64_t)1023, T.lds_bytes);
-    hipLaunchKernelGGL(k_tp_walk_bcd_sim, dim3((unsigned)gs), dim3(64), (size_t)need, s, P, M, B, mat, H, T, sp,
+    (void)hipStreamWaitEvent(s, ev_cc, 0);
+    hipLaunchKernelGGL(k_tp_walk_bcd_sim<2>, dim3((unsigned)gs), dim3(64), (size_t)need, s, P, M, B, mat, H, T, sp,
                        (int32_t)need, 1);
     (void)hipStreamWaitEvent(s, ev_join, 0);
 }
