@@ -3,14 +3,15 @@
 // FP64 throughout; built with -ffp-contract=off so that every expression
 // rounds as the reference's CPU build does (see zdx_geom.h).
 //
-//   k_transform       one workgroup per body: world vertices (quat_rotate of
+//   k_transform       one wave per body: world vertices (quat_rotate of
 //                     each unique body vertex), body AABB, degenerate-triangle
 //                     flags and counts
 //   broad phase       device-sized uniform grid (cell >= 2 r_max), counting
 //                     sort of particles by cell, per-particle neighbour
 //                     enumeration, scans, ordered pair list (i < j) and the
 //                     per-particle adjacency used by the ordered force fold
-//   narrow phase      k_narrow: persistent workgroups, one pair at a time:
+//   narrow phase      k_narrow_wave (one wave per pair) and, for the pairs
+//                     too large for a wave, k_narrow (persistent workgroups):
 //                     cull each mesh's triangles against the other mesh's
 //                     box, then a flattened, order-preserving sweep of the
 //                     culled product with the float (outward-rounded) AABB
@@ -18,11 +19,13 @@
 //                     one thread per hit, the exact FP64 AABB re-test and
 //                     tri-tri; k_narrow_compact: one wave per pair, ballot /
 //                     prefix-sum compaction in the reference's (ta, tb) order
-//   topology          k_tp_alloc (one thread per pair) and k_tp_pass_a / pass
-//                     C (workgroups over flat S^2 test ranges) for the
-//                     quadratic tests; walks B and D (components, snap
-//                     union-find, T-junction chains, loops, Sn/Gn/xc0, force)
-//                     one wave per pair with scratch in LDS (zdx_topo.h)
+//   topology          k_tp_alloc (one thread per pair) and k_tp_pass_a
+//                     (workgroups over flat S^2 test ranges) for the
+//                     quadratic tests; k_tp_snap_cc (one workgroup per pair
+//                     of 40 or more segments: its snap clusters); walks B and
+//                     D with pass C between (components, snap union-find,
+//                     T-junction chains, loops, Sn/Gn/xc0, force) one wave
+//                     per pair with scratch in LDS (zdx_topo.h)
 //   k_walls           one wave per particle near a wall: plane sections of
 //                     every candidate wall triangle, clip, area, force
 //   k_fold_integrate  one thread per particle: ordered, atomic-free fold of
