@@ -793,22 +793,41 @@ __global__ void __launch_bounds__(256) k_topo_lists(DPairs P) {
     }
 }
 
-// LPT order: pairs by descending cost bucket floor(log2(TA * TB)), so the
-// persistent queue hands out the expensive pairs first
-__device__ inline int lpt_bucket(const DPairs& P, const DMeshes& M, const int32_t* body_mesh, int p) {
-    const int64_t c = (int64_t)M.ntris[body_mesh[P.pi[p]]] * M.ntris[body_mesh[P.pj[p]]];
-    return __clzll((unsigned long long)(c | 1));   // bucket = leading zeros: descending cost
+// LPT order: pairs by descending estimated cost, so the persistent queues
+// hand out the expensive pairs first.
+// Estimated narrow-phase cost of pair p, in microseconds of one wave: its
+// vertex and cull passes (per vertex, per triangle) and its float-box tests,
+// with each body's kept triangles estimated as its triangles times the
+// fraction of its box inside the other's (the product over the axes: the kept
+// triangles are those near the overlap).  Weights from per-pair clocks of the
+// one-wave kernel on MI355X; the count of tests is what varies most between
+// pairs of the same meshes.  Bucket: 8 per octave, descending cost.
+__device__ inline int lpt_bucket(const DPairs& P, const DMeshes& M, const DBodies& B, int p) {
+    const int i = P.pi[p], j = P.pj[p];
+    const int mA = B.mesh[i], mB = B.mesh[j];
+    const double* a = B.aabb + 6 * (int64_t)i;
+    const double* b = B.aabb + 6 * (int64_t)j;
+    double fa = 1.0, fb = 1.0;
+    for (int k = 0; k < 3; ++k) {
+        const double ov = fmax(fmin(a[3 + k], b[3 + k]) - fmax(a[k], b[k]), 0.0);
+        fa *= fmin(1.0, ov / fmax(a[3 + k] - a[k], 1e-30));
+        fb *= fmin(1.0, ov / fmax(b[3 + k] - b[k], 1e-30));
+    }
+    const double TA = M.ntris[mA], TB = M.ntris[mB];
+    const double c = 2.0 + 0.01 * (M.nverts[mA] + M.nverts[mB]) + 0.005 * (TA + TB) + 0.008 * (TA * fa) * (TB * fb);
+    const int o = (int)(8.0f * __log2f((float)c));   // c >= 2: o >= 8
+    return 63 - (o < 0 ? 0 : (o > 63 ? 63 : o));
 }
 // per-workgroup histograms first, one global atomic per (workgroup, bucket)
-__global__ void __launch_bounds__(256) k_lpt_count(DPairs P, DMeshes M, const int32_t* body_mesh,
-                                                   int32_t* bucket_cnt, unsigned long long* stamp) {
+__global__ void __launch_bounds__(256) k_lpt_count(DPairs P, DMeshes M, DBodies B, int32_t* bucket_cnt,
+                                                   unsigned long long* stamp) {
     stamp_here(stamp);
     __shared__ int h[64];
     if (threadIdx.x < 64) h[threadIdx.x] = 0;
     __syncthreads();
     const int n = *P.n;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
-        atomicAdd(&h[lpt_bucket(P, M, body_mesh, p)], 1);
+        atomicAdd(&h[lpt_bucket(P, M, B, p)], 1);
     __syncthreads();
     if (threadIdx.x < 64 && h[threadIdx.x]) atomicAdd(&bucket_cnt[threadIdx.x], h[threadIdx.x]);
 }
@@ -821,7 +840,6 @@ __global__ void k_lpt_scan(int32_t* bucket_cnt, int32_t* bucket_cur) {
 __device__ inline void box_f(const double* lo, const double* hi, float* f);
 __global__ void __launch_bounds__(256) k_lpt_scatter(DPairs P, DMeshes M, DBodies B, int32_t* bucket_cur,
                                                      int32_t* order) {
-    const int32_t* body_mesh = B.mesh;
     __shared__ int h[64], base[64];
     const int n = *P.n;
     for (int p0 = blockIdx.x * blockDim.x; p0 < n; p0 += gridDim.x * blockDim.x) {
@@ -830,7 +848,7 @@ __global__ void __launch_bounds__(256) k_lpt_scatter(DPairs P, DMeshes M, DBodie
         const int p = p0 + threadIdx.x;
         int b = 0, r = 0;
         if (p < n) {
-            b = lpt_bucket(P, M, body_mesh, p);
+            b = lpt_bucket(P, M, B, p);
             r = atomicAdd(&h[b], 1);
         }
         __syncthreads();
@@ -2136,10 +2154,9 @@ void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const Na
 
 void launch_lpt_order(const DPairs& P, const DMeshes& M, const DBodies& B, int32_t* order, int32_t* bcnt,
                       int32_t* bcur, int32_t pair_cap, hipStream_t s, bool zeroed, unsigned long long* stamp) {
-    const int32_t* body_mesh = B.mesh;
     if (!zeroed) hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, bcnt, 0, nullptr, (int64_t)64);
     const int g = std::max(1, std::min(nblk(pair_cap, 256), 512));
-    hipLaunchKernelGGL(k_lpt_count, dim3(g), dim3(256), 0, s, P, M, body_mesh, bcnt, stamp);
+    hipLaunchKernelGGL(k_lpt_count, dim3(g), dim3(256), 0, s, P, M, B, bcnt, stamp);
     hipLaunchKernelGGL(k_lpt_scan, dim3(1), dim3(64), 0, s, bcnt, bcur);
     hipLaunchKernelGGL(k_lpt_scatter, dim3(g), dim3(256), 0, s, P, M, B, bcur, order);
 }
