@@ -1652,7 +1652,12 @@ __device__ inline bool wall_tri_candidate(const double* wt9, const V3& pmn, cons
 
 // One wave per listed particle (a particle near at least one wall triangle).
 // Entries of particles not in the list were zeroed before this kernel.
-__global__ void __launch_bounds__(64) k_walls(DBodies B, DMeshes M, DMat mat, WallBufs W, HistBufs H, double* wcontrib,
+// Registers capped at 96 (five waves per SIMD): the kernel runs beside the
+// narrow phase on the second stream, and at its natural 206 VGPRs a walls wave
+// left room for only two of the one-wave narrow kernel's waves on its SIMD
+// instead of four (the narrow phase is 3 % faster this way; the walls, which
+// end well before it, spill a little).
+__global__ void __launch_bounds__(64, 5) k_walls(DBodies B, DMeshes M, DMat mat, WallBufs W, HistBufs H, double* wcontrib,
                                               int32_t* wnc, const int32_t* list, const int32_t* n_list, double* scr,
                                               int32_t pts_cap, SimParams sp, int32_t* status) {
     LaneDev wp;
