@@ -256,4 +256,62 @@ int ht_pair_jac(const double* trisA, int TA, const double* trisB, int TB, const 
     return t.loops;
 }
 
+
+// Walk B of one pair twice: with its snap clusters united in the walk (per
+// component, main.cpp:680-746) and with clusters found beforehand over all of
+// the pair's snap hits, each point pointed at its cluster's smallest point
+// (what the GPU's cluster kernel gives walk B for pairs of 40 or more
+// segments).  out: [0] 1 if the components, vertices, cells, edges and
+// T-junction word counts are equal, [1] components, [2] 1 if some cluster
+// spans two components (walk B then unites in the walk instead), [3] snap
+// hits.
+int ht_snap_clusters(const double* seg, int S, double h, int split, int* out) {
+    using namespace zdx;
+    out[0] = out[1] = out[2] = out[3] = 0;
+    if (S <= 0) { out[0] = 1; return 0; }
+    LaneHost wp;
+    std::vector<int> cell9(9 * (size_t)S);
+    for (int s = 0; s < S; ++s) tp_cells(seg, s, h, &cell9[9 * (size_t)s]);
+    std::vector<int> adjn((size_t)S), adjl((size_t)S * S);
+    for (int i = 0; i < S; ++i) adjn[i] = tp_adj_row(seg, cell9.data(), h, S, i, &adjl[(size_t)i * S], S);
+    const int n = 2 * S;
+    const int64_t npairs = (int64_t)n * (n - 1) / 2;
+    std::vector<uint64_t> snapw((size_t)((npairs + 63) >> 6), 0ull);
+    std::vector<int> lab(n);
+    for (int k = 0; k < n; ++k) lab[k] = k;
+    auto root = [&](int x) { while (lab[x] != x) x = lab[x]; return x; };
+    for (int64_t t = 0; t < npairs; ++t) {
+        int a, b;
+        tp_pair_of(t, a, b);
+        if (!tp_snap_hit(seg, cell9.data(), h, a, b)) continue;
+        snapw[(size_t)(t >> 6)] |= 1ull << (t & 63);
+        ++out[3];
+        int ra = root(a), rb = root(b);
+        if (ra != rb) lab[ra < rb ? rb : ra] = ra < rb ? ra : rb;
+    }
+    std::vector<int> glab(n);
+    for (int k = 0; k < n; ++k) glab[k] = root(k);
+    auto run = [&](const int* gl, std::vector<int>& comp, std::vector<double>& V, std::vector<int>& vc,
+                   std::vector<int>& E, int& tjn, std::vector<int>& compof) {
+        std::vector<int> IB((size_t)tp_b_ints(S));
+        comp.assign(TPC_N * (size_t)S, 0); V.assign(3 * (size_t)n, 0.0); vc.assign(3 * (size_t)n, 0);
+        E.assign(2 * (size_t)S, 0);
+        const int nc = tp_components_and_vertices(wp, seg, S, h, split != 0, adjn.data(), adjl.data(), S,
+                                                  snapw.data(), IB.data(), comp.data(), V.data(), vc.data(),
+                                                  E.data(), &tjn, nullptr, nullptr, gl);
+        // the component of every segment (its scratch slot in walk B)
+        compof.assign(IB.begin() + 7 * (size_t)S + 2, IB.begin() + 8 * (size_t)S + 2);
+        return nc;
+    };
+    std::vector<int> c1, c2, v1, v2, e1, e2, f1, f2;
+    std::vector<double> V1, V2;
+    int t1 = 0, t2 = 0;
+    const int n1 = run(nullptr, c1, V1, v1, e1, t1, f1);
+    const int n2 = run(glab.data(), c2, V2, v2, e2, t2, f2);
+    out[1] = n1;
+    for (int k = 0; k < n; ++k)
+        if (f1[k >> 1] != f1[glab[k] >> 1]) out[2] = 1;
+    out[0] = n1 == n2 && t1 == t2 && c1 == c2 && V1 == V2 && v1 == v2 && e1 == e2 ? 1 : 0;
+    return 0;
+}
 }  // extern "C"

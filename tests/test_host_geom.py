@@ -32,6 +32,7 @@ def H(tmp_path_factory):
     L.ht_tri_tri.argtypes = [dp, dp, dp]
     L.ht_pair.argtypes = [dp, C.c_int, dp, C.c_int, dp, dp, dp, dp, C.c_double, C.c_double, C.c_int,
                           dp, C.c_int, dp, C.c_int, dp, C.c_int, np.ctypeslib.ndpointer(dtype=np.int64)]
+    L.ht_snap_clusters.argtypes = [dp, C.c_int, C.c_double, C.c_int, ip]
     return L
 
 
@@ -229,3 +230,39 @@ def test_loop_sensitivities_match_finite_differences(H):
                 assert np.max(np.abs(ana[l, part] - num[l, part])) <= 2e-5 * scale, (na, nb, l, part)
         done += 1
     assert done == len(cases)
+
+
+def test_snap_clusters_found_beforehand_give_the_same_walk(H):
+    """Walk B with snap clusters found over all of a pair's snap hits (each
+    point's cluster named by its smallest point, as the GPU's cluster kernel
+    finds them for pairs of 40 or more segments) gives bitwise the
+    components, vertices, cells, edges and T-junction word counts of the
+    per-component union-find in the walk (main.cpp:680-746): when no cluster
+    spans two components no snap hit joins two components, and otherwise the
+    walk unites per component itself.  Seeded concave and convex pairs, split
+    into components, at the pipeline's tolerance and at four times it (more
+    snaps, larger clusters)."""
+    m = GI.synth_meshes()
+    built = {k: O.build_mesh(v) for k, v in m.items()}
+    rng = synth.Rng(91)
+    names = list(m)
+    n_multi = n_cases = n_big = 0
+    for it in range(80):
+        a, b = ("star2a", "star2b") if it % 2 == 0 else (names[int(rng.uniform(1)[0] * len(names))],
+                                                           names[int(rng.uniform(1)[0] * len(names))])
+        ta, pa, _ = built[a]
+        tb, pb, _ = built[b]
+        pose = GI._pose(rng, pa[4], pb[4], 0.05 + 0.35 * float(rng.uniform(1)[0]), True)
+        cnt, seg, rec, vtx = host_pair(H, ta, tb, pose, pa[5], pa[6], True)
+        S = len(seg)
+        if S == 0:
+            continue
+        h0 = pa[5] * 0.1 if pa[5] > 0 else pa[6] * 1e-2
+        for h in (h0, 4.0 * h0):
+            out = np.zeros(4, dtype=np.int32)
+            H.ht_snap_clusters(np.ascontiguousarray(seg.reshape(-1)), S, h, 1, out)
+            assert out[0] == 1, (a, b, it, h, list(out))
+            n_cases += 1
+            n_multi += out[1] > 1
+            n_big += S >= 40
+    assert n_cases >= 40 and n_multi >= 5 and n_big >= 5, (n_cases, n_multi, n_big)
