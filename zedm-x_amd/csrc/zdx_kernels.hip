@@ -396,29 +396,45 @@ __device__ inline void for_neighbours(const BroadPhaseBufs& bp, const DGrid& G, 
     cell_coords(G, qi, cx, cy, cz);
     // the (up to) three cells along x of a row are consecutive cell indices,
     // so their particles are one contiguous range of the cell order: one range
-    // per (dy, dz), visited in the same order as cell by cell
+    // per (dy, dz).  The nine ranges' bounds are loaded together, then each
+    // range is read four particles at a time, so that a thread waits on a
+    // few load rounds rather than one per candidate
     const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < G.nx ? cx + 1 : G.nx - 1;
-    for (int dz = -1; dz <= 1; ++dz) {
-        int z = cz + dz;
-        if (z < 0 || z >= G.nz) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            int y = cy + dy;
-            if (y < 0 || y >= G.ny) continue;
-            {
-                const int crow = (z * G.ny + y) * G.nx;
-                int s0 = bp.cell_start[crow + x0], s1 = bp.cell_start[crow + x1 + 1];
-                for (int k = s0; k < s1; ++k) {
-                    const int j = bp.sorted[k];
-                    if (j == i) continue;
-                    const double* qj = bp.spos + 4 * (int64_t)k;
-                    const bool iflow = i < j;
-                    const V3 pa = iflow ? v3(xi, yi, zi) : v3(qj[0], qj[1], qj[2]);
-                    const V3 pb = iflow ? v3(qj[0], qj[1], qj[2]) : v3(xi, yi, zi);
-                    const V3 dp = sub(pb, pa);
-                    const double d2 = dot(dp, dp);
-                    const double rs = iflow ? ri + qj[3] : qj[3] + ri;
-                    if (!(d2 > rs * rs)) f(j);
+    int s0[9], s1[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const int z = cz + r / 3 - 1, y = cy + r % 3 - 1;
+        const bool ok = z >= 0 && z < G.nz && y >= 0 && y < G.ny;
+        const int crow = ok ? (z * G.ny + y) * G.nx : 0;
+        s0[r] = ok ? bp.cell_start[crow + x0] : 0;
+        s1[r] = ok ? bp.cell_start[crow + x1 + 1] : 0;
+    }
+    for (int r = 0; r < 9; ++r) {
+        const int e = s1[r];
+        for (int k0 = s0[r]; k0 < e; k0 += 4) {
+            int jj[4];
+            double q[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (k0 + u < e) {
+                    jj[u] = bp.sorted[k0 + u];
+                    const double2* qj = reinterpret_cast<const double2*>(bp.spos + 4 * (int64_t)(k0 + u));
+                    const double2 lo = qj[0], hi = qj[1];
+                    q[u][0] = lo.x; q[u][1] = lo.y; q[u][2] = hi.x; q[u][3] = hi.y;
                 }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (k0 + u >= e) break;
+                const int j = jj[u];
+                if (j == i) continue;
+                const bool iflow = i < j;
+                const V3 pa = iflow ? v3(xi, yi, zi) : v3(q[u][0], q[u][1], q[u][2]);
+                const V3 pb = iflow ? v3(q[u][0], q[u][1], q[u][2]) : v3(xi, yi, zi);
+                const V3 dp = sub(pb, pa);
+                const double d2 = dot(dp, dp);
+                const double rs = iflow ? ri + q[u][3] : q[u][3] + ri;
+                if (!(d2 > rs * rs)) f(j);
             }
         }
     }
@@ -1270,7 +1286,9 @@ struct WalkB {
     int ncomp;       // -1: not evaluated (the step is retried with larger buffers)
     long long to;    // first T-junction word of the item, -1: did not fit
 };
-// pre: the pair's snap clusters were found by k_tp_snap_cc
+// pre: the pair's snap clusters were found by k_tp_snap_cc; SENS = false
+// compiles out the endpoint sensitivities (the time step never asks for them)
+template <bool SENS = true>
 __device__ inline WalkB tp_walk_b_item(const DPairs& P, const DMeshes& M, const DBodies& B, const TopoBufs& T,
                                        int32_t split, int32_t lds_bytes, char* s_walk, const LaneDev& wp, int it,
                                        bool pre = false) {
@@ -1296,8 +1314,8 @@ __device__ inline WalkB tp_walk_b_item(const DPairs& P, const DMeshes& M, const 
             if (!__all(ok)) return -1;
             return tp_components_and_vertices(wp, seg, S, h, split != 0, adj, adj + S, R, T.snapw + so, I,
                                               T.comp + TPC_N * sb, T.V + 6 * sb, T.vcell + 6 * sb,
-                                              T.E + 2 * sb, &tjn, T.ej ? T.ej + 72 * sb : nullptr,
-                                              T.vj ? T.vj + 72 * sb : nullptr, pre ? T.lab + 2 * sb : nullptr);
+                                              T.E + 2 * sb, &tjn, SENS && T.ej ? T.ej + 72 * sb : nullptr,
+                                              SENS && T.vj ? T.vj + 72 * sb : nullptr, pre ? T.lab + 2 * sb : nullptr);
         });
     }
 #ifdef ZDX_WALK_PROFILE
@@ -1423,7 +1441,7 @@ __global__ void __launch_bounds__(256) k_tp_pass_c(DPairs P, DMeshes M, DBodies 
 }
 
 // Walk D with its inputs copied to the scratch: edges, vertex cells.
-template <class Sink>
+template <class Sink, bool SENS = true>
 __device__ inline TopoOut tp_walk_d(const LaneDev& wp, const TopoBufs& T, int S, int ncomp, int64_t sb, long long to,
                                     double h, double area_eps, const V3& dir, char* lds, int lds_bytes, Sink& sink) {
     return with_walk_scratch(T, tp_walk_d_ints(S), tp_walk_d_dbls(S), lds, lds_bytes, [&](int* I, double* D) {
@@ -1436,7 +1454,7 @@ __device__ inline TopoOut tp_walk_d(const LaneDev& wp, const TopoBufs& T, int S,
         wave_copy(vcl, T.vcell + 6 * sb, 3 * nv_all);
         __syncthreads();
         return tp_loops(wp, S, ncomp, h, area_eps, dir, comp, T.V + 6 * sb, vcl, El, T.tjw + to, I, D, sink,
-                        T.vj ? T.vj + 72 * sb : nullptr, D + tp_d_dbls(S));
+                        SENS && T.vj ? T.vj + 72 * sb : nullptr, D + tp_d_dbls(S));
     });
 }
 
@@ -1484,7 +1502,8 @@ __device__ inline void tp_walk_d_sim_item(const DPairs& P, const DMeshes& M, con
     o.comps = 0; o.loops_raw = 0; o.loops = 0; o.status = 0;
     if (ncomp < 0 || to < 0) o.status = TOPO_E2_OVERFLOW;
     else if (ncomp > 0)
-        o = tp_walk_d(wp, T, S, ncomp, P.seg_off[p], to, topo_tol(T, M, mA), M.area_eps[mA], sub(b.pos, a.pos), s_walk,
+        o = tp_walk_d<SimSink, false>(wp, T, S, ncomp, P.seg_off[p], to, topo_tol(T, M, mA), M.area_eps[mA],
+                                      sub(b.pos, a.pos), s_walk,
                       lds_bytes, sink);
 #ifdef ZDX_WALK_PROFILE
     if (lane_id() == 0 && it < 8192) {
@@ -1540,7 +1559,7 @@ __global__ void __launch_bounds__(64, WPS) k_tp_walk_bcd_sim(DPairs P, DMeshes M
 #ifdef ZDX_WALK_PROFILE
         const unsigned long long chain_t0 = wall_clock64();
 #endif
-        const WalkB r = tp_walk_b_item(P, M, B, T, sp.split, lds_bytes, s_walk, wp, it, big && P.topo_big);
+        const WalkB r = tp_walk_b_item<false>(P, M, B, T, sp.split, lds_bytes, s_walk, wp, it, big && P.topo_big);
         __syncthreads();
         tp_pass_c_item(P, M, B, T, it, r.ncomp, r.to, s_walk, lds_bytes);
         __syncthreads();
