@@ -95,73 +95,107 @@ __global__ void __launch_bounds__(64) k_report(const int32_t* status, unsigned l
 // One wave per body of the list (4 bodies per 256-thread workgroup): world
 // vertices, AABB, and the count of non-degenerate triangles (only the mesh's
 // suspect triangles can degenerate, unless the body is far from the origin).
+// The xf_lane_* helpers read lane l's value (l uniform across the wave).
+__device__ inline int xf_lane_i32(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+__device__ inline int64_t xf_lane_i64(int64_t x, int l) {
+    const uint64_t u = (uint64_t)x;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ inline double xf_lane_f64(double x, int l) {
+    return __longlong_as_double((long long)xf_lane_i64((int64_t)__double_as_longlong(x), l));
+}
+
 __global__ void __launch_bounds__(256) k_transform(DBodies B, DMeshes M, const int32_t* list, const int32_t* n_list) {
-    const int nb = list ? *n_list : B.n;
-    const int wpb = blockDim.x / WAVE;
+    const int64_t nb = list ? *n_list : B.n;
+    const int64_t w = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / WAVE);
     const int ln = lane_id();
-    for (int it = blockIdx.x * wpb + threadIdx.x / WAVE; it < nb; it += gridDim.x * wpb) {
-        const int b = list ? list[it] : it;
-        const int m = B.mesh[b];
-        const double* st = B.state + 16 * (int64_t)b;
-        const V3 pos = v3(st[0], st[1], st[2]);
-        Qt q; q.w = st[3]; q.x = st[4]; q.y = st[5]; q.z = st[6];
-        const Qt qn = qnormalize(q);
-        Qt qc; qc.w = qn.w; qc.x = -qn.x; qc.y = -qn.y; qc.z = -qn.z;
-        const int nv = M.nverts[m];
-        const double* bv = M.verts + 3 * (int64_t)M.vert_off[m];
-        double* wv = B.wv + 3 * B.wv_off[b];
-        double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-        // TV rounds of the wave's vertices per pass: all their loads first
-        constexpr int TV = 4;
-        for (int v0 = ln; v0 < nv; v0 += TV * WAVE) {
-            double bx[TV][3];
+    // the wave's bodies are it = w, w + nw, w + 2 nw, ...: lane k loads the
+    // body index, mesh tables and normalised pose of the k-th of the next 64,
+    // so the wave's per-body loads are the vertices alone
+    for (int64_t it0 = w; it0 < nb; it0 += WAVE * nw) {
+        const int64_t mine = it0 + ln * nw;
+        int hb = 0, hnv = 0, hT = 0, hns = 0, hso = 0, hvo = 0, hto = 0;
+        int64_t hwo = 0;
+        double hp[3] = {0.0, 0.0, 0.0};
+        Qt hq;
+        hq.w = 1.0; hq.x = 0.0; hq.y = 0.0; hq.z = 0.0;
+        if (mine < nb) {
+            hb = list ? list[mine] : (int)mine;
+            const int m = B.mesh[hb];
+            const double* st = B.state + 16 * (int64_t)hb;
+            hp[0] = st[0]; hp[1] = st[1]; hp[2] = st[2];
+            Qt q; q.w = st[3]; q.x = st[4]; q.y = st[5]; q.z = st[6];
+            hq = qnormalize(q);
+            hnv = M.nverts[m]; hvo = M.vert_off[m]; hwo = B.wv_off[hb];
+            hT = M.ntris[m]; hto = M.tri_off[m]; hns = M.nsusp[m]; hso = M.susp_off[m];
+        }
+        const int cnt = (int)min((int64_t)WAVE, (nb - it0 + nw - 1) / nw);
+        for (int kb = 0; kb < cnt; ++kb) {
+            const int b = xf_lane_i32(hb, kb);
+            const V3 pos = v3(xf_lane_f64(hp[0], kb), xf_lane_f64(hp[1], kb), xf_lane_f64(hp[2], kb));
+            Qt qn;
+            qn.w = xf_lane_f64(hq.w, kb); qn.x = xf_lane_f64(hq.x, kb);
+            qn.y = xf_lane_f64(hq.y, kb); qn.z = xf_lane_f64(hq.z, kb);
+            Qt qc; qc.w = qn.w; qc.x = -qn.x; qc.y = -qn.y; qc.z = -qn.z;
+            const int nv = xf_lane_i32(hnv, kb);
+            const double* bv = M.verts + 3 * (int64_t)xf_lane_i32(hvo, kb);
+            double* wv = B.wv + 3 * xf_lane_i64(hwo, kb);
+            double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+            // TV rounds of the wave's vertices per pass: all their loads first
+            constexpr int TV = 4;
+            for (int v0 = ln; v0 < nv; v0 += TV * WAVE) {
+                double bx[TV][3];
 #pragma unroll
-            for (int u = 0; u < TV; ++u) {
-                const int v = v0 + u * WAVE;
-                if (v < nv) { bx[u][0] = bv[3 * v]; bx[u][1] = bv[3 * v + 1]; bx[u][2] = bv[3 * v + 2]; }
-            }
+                for (int u = 0; u < TV; ++u) {
+                    const int v = v0 + u * WAVE;
+                    if (v < nv) { bx[u][0] = bv[3 * v]; bx[u][1] = bv[3 * v + 1]; bx[u][2] = bv[3 * v + 2]; }
+                }
 #pragma unroll
-            for (int u = 0; u < TV; ++u) {
-                const int v = v0 + u * WAVE;
-                if (v < nv) {
-                    Qt pv; pv.w = 0.0; pv.x = bx[u][0]; pv.y = bx[u][1]; pv.z = bx[u][2];
-                    Qt r = qmul(qmul(qn, pv), qc);
-                    V3 w = add(v3(r.x, r.y, r.z), pos);
-                    wv[3 * v] = w.x; wv[3 * v + 1] = w.y; wv[3 * v + 2] = w.z;
-                    mn[0] = fmin(mn[0], w.x); mn[1] = fmin(mn[1], w.y); mn[2] = fmin(mn[2], w.z);
-                    mx[0] = fmax(mx[0], w.x); mx[1] = fmax(mx[1], w.y); mx[2] = fmax(mx[2], w.z);
+                for (int u = 0; u < TV; ++u) {
+                    const int v = v0 + u * WAVE;
+                    if (v < nv) {
+                        Qt pv; pv.w = 0.0; pv.x = bx[u][0]; pv.y = bx[u][1]; pv.z = bx[u][2];
+                        Qt r = qmul(qmul(qn, pv), qc);
+                        V3 w = add(v3(r.x, r.y, r.z), pos);
+                        wv[3 * v] = w.x; wv[3 * v + 1] = w.y; wv[3 * v + 2] = w.z;
+                        mn[0] = fmin(mn[0], w.x); mn[1] = fmin(mn[1], w.y); mn[2] = fmin(mn[2], w.z);
+                        mx[0] = fmax(mx[0], w.x); mx[1] = fmax(mx[1], w.y); mx[2] = fmax(mx[2], w.z);
+                    }
                 }
             }
-        }
-        for (int k = 0; k < 3; ++k) {
-            for (int d = 32; d >= 1; d >>= 1) {
-                mn[k] = fmin(mn[k], __shfl_xor(mn[k], d, WAVE));
-                mx[k] = fmax(mx[k], __shfl_xor(mx[k], d, WAVE));
+            for (int k = 0; k < 3; ++k) {
+                for (int d = 32; d >= 1; d >>= 1) {
+                    mn[k] = fmin(mn[k], __shfl_xor(mn[k], d, WAVE));
+                    mx[k] = fmax(mx[k], __shfl_xor(mx[k], d, WAVE));
+                }
             }
-        }
-        if (ln == 0) {
-            double* box = B.aabb + 6 * (int64_t)b;
-            for (int k = 0; k < 3; ++k) { box[k] = mn[k]; box[3 + k] = mx[k]; }
-        }
-        const int T = M.ntris[m];
-        const int32_t* ti = M.tri_idx + 3 * (int64_t)M.tri_off[m];
-        const bool far = !(fabs(pos.x) <= 1e3 && fabs(pos.y) <= 1e3 && fabs(pos.z) <= 1e3);
-        const int nchk = far ? T : M.nsusp[m];
-        const int32_t* sl = M.susp + M.susp_off[m];
-        int bad = 0;
-        if (nchk > 0) {
-            // the world vertices just written by this wave's lanes: make them
-            // visible before other lanes read them back
-            __threadfence_block();
-            for (int k = ln; k < nchk; k += WAVE) {
-                const int t = far ? k : sl[k];
-                V3 a0 = ld3(wv + 3 * ti[3 * t]), a1 = ld3(wv + 3 * ti[3 * t + 1]), a2 = ld3(wv + 3 * ti[3 * t + 2]);
-                V3 n; double d;
-                bad += plane_of(a0, a1, a2, n, d) ? 0 : 1;
+            if (ln == 0) {
+                double* box = B.aabb + 6 * (int64_t)b;
+                for (int k = 0; k < 3; ++k) { box[k] = mn[k]; box[3 + k] = mx[k]; }
             }
-            for (int d = 32; d >= 1; d >>= 1) bad += __shfl_xor(bad, d, WAVE);
+            const int T = xf_lane_i32(hT, kb);
+            const int32_t* ti = M.tri_idx + 3 * (int64_t)xf_lane_i32(hto, kb);
+            const bool far = !(fabs(pos.x) <= 1e3 && fabs(pos.y) <= 1e3 && fabs(pos.z) <= 1e3);
+            const int nchk = far ? T : xf_lane_i32(hns, kb);
+            const int32_t* sl = M.susp + xf_lane_i32(hso, kb);
+            int bad = 0;
+            if (nchk > 0) {
+                // the world vertices just written by this wave's lanes: make them
+                // visible before other lanes read them back
+                __threadfence_block();
+                for (int k = ln; k < nchk; k += WAVE) {
+                    const int t = far ? k : sl[k];
+                    V3 a0 = ld3(wv + 3 * ti[3 * t]), a1 = ld3(wv + 3 * ti[3 * t + 1]), a2 = ld3(wv + 3 * ti[3 * t + 2]);
+                    V3 n; double d;
+                    bad += plane_of(a0, a1, a2, n, d) ? 0 : 1;
+                }
+                for (int d = 32; d >= 1; d >>= 1) bad += __shfl_xor(bad, d, WAVE);
+            }
+            if (ln == 0) B.nondeg[b] = T - bad;
         }
-        if (ln == 0) B.nondeg[b] = T - bad;
     }
 }
 
