@@ -879,11 +879,10 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
                     s->ctx->ev_cc3);
     if (walls) HIPC(hipStreamWaitEvent(st, s->ctx->ev_join, 0));
     // stamps 4 (topology done, with any wait for the wall stream) and 5 (fold
-    // start) come from the fold's first workgroup, 6 from the counters'
+    // start) come from the fold's first workgroup, 6 from the report's
     launch_fold_integrate(s->N, s->d_state[par].p, s->d_state[nxt].p, mat, s->d_adj_off.p, s->d_adj_j.p,
                           s->d_adj_pair.p, s->adj_cap, T, s->W, s->d_wcontrib.p, s->d_wnc.p, s->max_wall_tris,
-                          s->d_F.p, s->d_T.p, s->d_counts.p, s->sp, st, kt + 4, kt + 5);
-    launch_counters(P, T, s->N * s->W, s->d_wnc.p, s->d_ctr.p, st, kt + 6);
+                          s->d_F.p, s->d_T.p, s->d_counts.p, s->sp, P, s->d_ctr.p, st, kt + 4, kt + 5);
     launch_report(s->d_status.p, s->d_ctr.p, kt, s->rep, st);
 }
 

@@ -215,12 +215,11 @@ void launch_fold_integrate(int32_t n, const double* state_in, double* state_out,
                            const int32_t* adj_off, const int32_t* adj_j, const int32_t* adj_pair,
                            int32_t adj_cap, const TopoBufs& T, int32_t n_walls, const double* wcontrib, const int32_t* wncontrib,
                            int32_t max_wall_tris, double* F, double* Tq, int32_t* counts, const SimParams& sp,
-                           hipStream_t s,
+                           const DPairs& P, unsigned long long* counters, hipStream_t s,
                            unsigned long long* stamp_a = nullptr, unsigned long long* stamp_b = nullptr);
-// counters[ZDX_N_COUNTERS] (int64) summed from per-pair / per-wall arrays into
-// `counters`, which must hold zeros (launch_report leaves them so)
-void launch_counters(const DPairs& P, const TopoBufs& T, int32_t n_pw, const int32_t* wncontrib,
-                     unsigned long long* counters, hipStream_t s, unsigned long long* stamp = nullptr);
+// (counters: ZDX_N_COUNTERS int64 summed by the fold's first workgroups from
+// the per-pair and per-wall arrays; they must hold zeros, which launch_report
+// leaves; nullptr skips them)
 void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, int64_t n, hipStream_t s);
 void launch_iota_pairs(int32_t* pi, int32_t* pj, int32_t n, hipStream_t s);

@@ -30,7 +30,8 @@
 //                     every candidate wall triangle, clip, area, force
 //   k_fold_integrate  one thread per particle: ordered, atomic-free fold of
 //                     the contact records in the reference's summation order,
-//                     then the rigid-body integrator
+//                     then the rigid-body integrator; its first workgroups
+//                     also sum the step's counters
 #include <hip/hip_runtime.h>
 
 #include "../../include/zdx.h"
@@ -71,7 +72,8 @@ __global__ void __launch_bounds__(64) k_report(const int32_t* status, unsigned l
         rep->ctr[t] = ctr[t];
         ctr[t] = 0ull;
     } else if (t < 28) {
-        rep->kt[t - 16] = kt[t - 16];
+        // clock 6, the end of the fold, is this kernel's start
+        rep->kt[t - 16] = t == 22 ? (unsigned long long)wall_clock64() : kt[t - 16];
     } else if (t == 28) {
         rep->status = *status;
     }
@@ -1780,14 +1782,77 @@ __global__ void k_walls_clear(int64_t n, int32_t* wnc, uint8_t* wact, unsigned l
 }
 
 // ------------------------------------------------------------ fold + integrate
+// The step's counters summed from the per-pair and per-(particle, wall)
+// arrays by nb workgroups of 256 threads (this one is bid): one atomic per
+// value and workgroup into ctr, which k_report hands over and clears.
+__device__ inline void sum_counters(const DPairs& P, const TopoBufs& T, int64_t n_pw, const int32_t* wnc,
+                                    unsigned long long* ctr, int bid, int nb) {
+    const int n = *P.n;
+    unsigned long long c_loops = 0, c_checks = 0, c_seg = 0, c_comps = 0, c_raw = 0, c_tests = 0, c_hits = 0,
+                       c_wall = 0, c_scan = 0, c_tp = 0, c_smax = 0;
+    for (int p = bid * blockDim.x + threadIdx.x; p < n; p += nb * blockDim.x) {
+        const unsigned long long ns = (unsigned long long)P.nseg[p];
+        c_loops += T.ncontrib[p];
+        c_checks += P.tri_checks[p];
+        c_seg += ns;
+        c_tp += ns > 0 ? 1ull : 0ull;
+        c_smax = ns > c_smax ? ns : c_smax;
+        c_comps += T.comps[p];
+        c_raw += T.loops_raw[p];
+        c_tests += P.tri_tests[p];
+        c_hits += P.tri_hits[p];
+        c_scan += P.tri_scanned[p];
+    }
+    for (int64_t k = bid * blockDim.x + threadIdx.x; k < n_pw; k += nb * blockDim.x) c_wall += wnc[k];
+    unsigned long long v[11] = {c_loops, c_checks, c_seg, c_comps, c_raw, c_tests, c_hits, c_wall, c_scan, c_tp, c_smax};
+    for (int k = 0; k < 10; ++k)
+        for (int d = 32; d >= 1; d >>= 1) v[k] += __shfl_xor(v[k], d, WAVE);
+    for (int d = 32; d >= 1; d >>= 1) {
+        const unsigned long long o = __shfl_xor(v[10], d, WAVE);
+        v[10] = o > v[10] ? o : v[10];
+    }
+    // combine the waves of the workgroup, then one atomic per value
+    __shared__ unsigned long long s[11][16];
+    __shared__ unsigned long long tot[11];
+    const int w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    if (lane_id() == 0)
+        for (int k = 0; k < 11; ++k) s[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 11) {
+        const int k = threadIdx.x;
+        unsigned long long r = s[k][0];
+        for (int q = 1; q < nw; ++q) r = k == 10 ? (s[k][q] > r ? s[k][q] : r) : r + s[k][q];
+        tot[k] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 11) {
+        // v order: loops, checks, segments, comps, raw loops, tests, hits, wall contacts, scanned, pairs with
+        // segments, largest segment count.  ctr: 0 contacts (accepted loops + wall contacts), 1 tri_checks,
+        // 2 segments, 3 comps, 4 loops, 5 pairs, 6 wall contacts, 7 tri_tests, 8 tri_hits, 9 tri_scanned,
+        // 10 pairs with segments, 11 largest segment count
+        const int k = threadIdx.x;
+        const int dst[11] = {0, 1, 2, 3, 4, 7, 8, 6, 9, 10, 11};
+        if (k == 0) atomicAdd(&ctr[0], tot[0] + tot[7]);
+        else if (k == 10) atomicMax(&ctr[11], tot[10]);
+        else atomicAdd(&ctr[dst[k]], tot[k]);
+    }
+    if (bid == 0 && threadIdx.x == 0) atomicAdd(&ctr[5], (unsigned long long)n);
+}
+
+constexpr int FOLD_CTR_GROUPS = 32;
 __global__ void __launch_bounds__(256) k_fold_integrate(int32_t n, const double* sin, double* sout, DMat mat,
                                                         const int32_t* adj_off, const int32_t* adj_j,
                                                         const int32_t* adj_pair, int32_t adj_cap, TopoBufs T,
                                                         int32_t n_walls,
                                                         const double* wcontrib, const int32_t* wnc, int32_t mwt,
                                                         double* Fo, double* To, int32_t* counts, SimParams sp,
+                                                        DPairs P, unsigned long long* ctr,
                                                         unsigned long long* stamp_a, unsigned long long* stamp_b) {
     stamp_here(stamp_a, stamp_b);
+    // the first FOLD_CTR_GROUPS workgroups also sum the step's counters (the
+    // topology and the walls, which they read, are complete)
+    if (ctr && blockIdx.x < FOLD_CTR_GROUPS)
+        sum_counters(P, T, (int64_t)n * n_walls, wnc, ctr, blockIdx.x, min((int)gridDim.x, FOLD_CTR_GROUPS));
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     // forces[i] and torques[i] start at zero and receive, in the reference's
@@ -1835,62 +1900,6 @@ __global__ void __launch_bounds__(256) k_fold_integrate(int32_t n, const double*
     for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) Ib.m[r][c] = mat.Ibinv[9 * (int64_t)i + 3 * r + c];
     integrate(b, Ib, mat.inv_mass[i], F, Tq, v3(sp.g[0], sp.g[1], sp.g[2]), sp.dt);
     store_body(sout + 16 * (int64_t)i, b);
-}
-
-// ------------------------------------------------------------ counters
-__global__ void __launch_bounds__(256) k_counters(DPairs P, TopoBufs T, int32_t n_pw, const int32_t* wnc,
-                                                  unsigned long long* ctr, unsigned long long* stamp) {
-    stamp_here(stamp);
-    const int n = *P.n;
-    unsigned long long c_loops = 0, c_checks = 0, c_seg = 0, c_comps = 0, c_raw = 0, c_tests = 0, c_hits = 0,
-                       c_wall = 0, c_scan = 0, c_tp = 0, c_smax = 0;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        const unsigned long long ns = (unsigned long long)P.nseg[p];
-        c_loops += T.ncontrib[p];
-        c_checks += P.tri_checks[p];
-        c_seg += ns;
-        c_tp += ns > 0 ? 1ull : 0ull;
-        c_smax = ns > c_smax ? ns : c_smax;
-        c_comps += T.comps[p];
-        c_raw += T.loops_raw[p];
-        c_tests += P.tri_tests[p];
-        c_hits += P.tri_hits[p];
-        c_scan += P.tri_scanned[p];
-    }
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n_pw; k += gridDim.x * blockDim.x) c_wall += wnc[k];
-    unsigned long long v[11] = {c_loops, c_checks, c_seg, c_comps, c_raw, c_tests, c_hits, c_wall, c_scan, c_tp, c_smax};
-    for (int k = 0; k < 10; ++k)
-        for (int d = 32; d >= 1; d >>= 1) v[k] += __shfl_xor(v[k], d, WAVE);
-    for (int d = 32; d >= 1; d >>= 1) {
-        const unsigned long long o = __shfl_xor(v[10], d, WAVE);
-        v[10] = o > v[10] ? o : v[10];
-    }
-    // combine the waves of the workgroup, then one atomic per value
-    __shared__ unsigned long long s[11][16];
-    __shared__ unsigned long long tot[11];
-    const int w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
-    if (lane_id() == 0)
-        for (int k = 0; k < 11; ++k) s[k][w] = v[k];
-    __syncthreads();
-    if (threadIdx.x < 11) {
-        const int k = threadIdx.x;
-        unsigned long long r = s[k][0];
-        for (int q = 1; q < nw; ++q) r = k == 10 ? (s[k][q] > r ? s[k][q] : r) : r + s[k][q];
-        tot[k] = r;
-    }
-    __syncthreads();
-    if (threadIdx.x < 11) {
-        // v order: loops, checks, segments, comps, raw loops, tests, hits, wall contacts, scanned, pairs with
-        // segments, largest segment count.  ctr: 0 contacts (accepted loops + wall contacts), 1 tri_checks,
-        // 2 segments, 3 comps, 4 loops, 5 pairs, 6 wall contacts, 7 tri_tests, 8 tri_hits, 9 tri_scanned,
-        // 10 pairs with segments, 11 largest segment count
-        const int k = threadIdx.x;
-        const int dst[11] = {0, 1, 2, 3, 4, 7, 8, 6, 9, 10, 11};
-        if (k == 0) atomicAdd(&ctr[0], tot[0] + tot[7]);
-        else if (k == 10) atomicMax(&ctr[11], tot[10]);
-        else atomicAdd(&ctr[dst[k]], tot[k]);
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctr[5], (unsigned long long)n);
 }
 
 __global__ void k_iota_pairs(int32_t* pi, int32_t* pj, int32_t n) {
@@ -2334,14 +2343,11 @@ void launch_fold_integrate(int32_t n, const double* sin, double* sout, const DMa
                            const int32_t* adj_j, const int32_t* adj_pair, int32_t adj_cap, const TopoBufs& T,
                            int32_t n_walls,
                            const double* wcontrib, const int32_t* wnc, int32_t mwt, double* F, double* Tq,
-                           int32_t* counts, const SimParams& sp, hipStream_t s, unsigned long long* stamp_a,
-                           unsigned long long* stamp_b) {
+                           int32_t* counts, const SimParams& sp, const DPairs& P, unsigned long long* ctr,
+                           hipStream_t s, unsigned long long* stamp_a, unsigned long long* stamp_b) {
     hipLaunchKernelGGL(k_fold_integrate, dim3(nblk(n, 256)), dim3(256), 0, s, n, sin, sout, mat, adj_off, adj_j,
-                       adj_pair, adj_cap, T, n_walls, wcontrib, wnc, mwt, F, Tq, counts, sp, stamp_a, stamp_b);
-}
-void launch_counters(const DPairs& P, const TopoBufs& T, int32_t n_pw, const int32_t* wnc, unsigned long long* ctr,
-                     hipStream_t s, unsigned long long* stamp) {
-    hipLaunchKernelGGL(k_counters, dim3(32), dim3(256), 0, s, P, T, n_pw, wnc, ctr, stamp);
+                       adj_pair, adj_cap, T, n_walls, wcontrib, wnc, mwt, F, Tq, counts, sp, P, ctr, stamp_a,
+                       stamp_b);
 }
 void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s) {
     if (n <= 0) return;
