@@ -825,9 +825,11 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
         z.u64[1] = s->tw.top.p; z.n64[1] = 5;
         launch_step_begin(kt, z, st);
     }
-    launch_broadphase(s->bpbufs(par), st);
-    launch_need_list(s->N, s->d_state[par].p, s->d_radius.p, s->d_cnt_all.p, s->wallbufs(), s->d_need_list.p,
-                     s->d_n_need.p, s->d_wall_list.p, s->d_n_wall.p, true, st, kt + 1);
+    {
+        // the need lists come from the broad phase's count pass
+        const NeedLists need{s->wallbufs(), s->d_need_list.p, s->d_n_need.p, s->d_wall_list.p, s->d_n_wall.p};
+        launch_broadphase(s->bpbufs(par), st, &need);
+    }
     int g_transform = s->transform_groups(), g_narrow = gmax, g_topo = (int)s->topo_threads,
         g_walls = (int)s->wall_threads;
     if (!batched) {
@@ -840,7 +842,7 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
         g_transform = std::max(1, std::min(c[1], g_transform));
         g_walls = std::max(1, std::min(c[2], g_walls));
     }
-    launch_transform(B, M, st, s->d_need_list.p, s->d_n_need.p, g_transform);
+    launch_transform(B, M, st, s->d_need_list.p, s->d_n_need.p, g_transform, kt + 1);
     if (!batched) launch_stamp(kt + 2, st);   // batched: stamped by the work ordering's first kernel
     const bool walls = s->W > 0;
     if (walls) {

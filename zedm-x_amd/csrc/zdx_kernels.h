@@ -168,15 +168,26 @@ struct StepReport {
 // publishes rep->seq.
 void launch_report(const int32_t* status, unsigned long long* ctr, const unsigned long long* kt, StepReport* rep,
                    hipStream_t s);
-// list/n_list: bodies to transform (device list and count; NULL = all bodies)
-void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s, const int32_t* list = nullptr, const int32_t* n_list = nullptr, int32_t groups = 0);
+// list/n_list: bodies to transform (device list and count; NULL = all bodies);
+// stamp: clock written by the first workgroup at its start
+void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s, const int32_t* list = nullptr,
+                      const int32_t* n_list = nullptr, int32_t groups = 0, unsigned long long* stamp = nullptr);
 // bodies that are in a candidate pair or near a wall triangle -> list
 // (and the particles near a wall triangle -> wlist), in no fixed order;
 // zeroed: *n_list and *n_wlist already hold 0
 void launch_need_list(int32_t n, const double* state, const double* radius, const int32_t* cnt_all, const WallBufs& W,
                       int32_t* list, int32_t* n_list, int32_t* wlist, int32_t* n_wlist, bool zeroed, hipStream_t s,
                       unsigned long long* stamp = nullptr);
-void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s);
+// The same lists built by the broad phase's count pass (its counts and
+// n_list, n_wlist zero beforehand): launch_broadphase with need set.
+struct NeedLists {
+    WallBufs W;
+    int32_t* list;
+    int32_t* n_list;
+    int32_t* wlist;
+    int32_t* n_wlist;
+};
+void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s, const NeedLists* need = nullptr);
 // Narrow phase: cull + sweep into the hit arena, the exact tests per hit, and
 // the per-pair compaction into the segment arena.  seg_cap: most segments one
 // pair may store (ST_STAGE beyond); queue: the cull's work counter; resets

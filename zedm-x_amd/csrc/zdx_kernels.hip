@@ -109,7 +109,9 @@ __device__ inline double xf_lane_f64(double x, int l) {
     return __longlong_as_double((long long)xf_lane_i64((int64_t)__double_as_longlong(x), l));
 }
 
-__global__ void __launch_bounds__(256) k_transform(DBodies B, DMeshes M, const int32_t* list, const int32_t* n_list) {
+__global__ void __launch_bounds__(256) k_transform(DBodies B, DMeshes M, const int32_t* list, const int32_t* n_list,
+                                                   unsigned long long* stamp) {
+    stamp_here(stamp);
     const int64_t nb = list ? *n_list : B.n;
     const int64_t w = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x / WAVE);
@@ -480,20 +482,33 @@ constexpr int NEIGH_REG = 16;   // partners kept from the count pass and sorted 
 
 // threads take particles in cell order (neighbouring threads scan the same
 // cells); the first NEIGH_REG partners found are kept for k_bp_fill
-__global__ void __launch_bounds__(256) k_bp_count(BroadPhaseBufs bp) {
+// When need.list is set the same pass also builds the need lists
+// (k_need_lists' test on the particle's counts and walls).
+__device__ inline bool near_any_wall(const WallBufs& W, const double* st, double r);
+__device__ inline void need_append(bool need, bool wall, int i, int32_t* list, int32_t* n_list, int32_t* wlist,
+                                   int32_t* n_wlist);
+__global__ void __launch_bounds__(256) k_bp_count(BroadPhaseBufs bp, NeedLists need) {
     const int si = blockIdx.x * blockDim.x + threadIdx.x;
-    if (si >= bp.n) return;
-    const int i = bp.sorted[si];
-    const DGrid G = *bp.grid;
-    int up = 0, all = 0;
-    int32_t* keep = bp.nbr + (int64_t)NEIGH_REG * si;
-    for_neighbours(bp, G, i, si, [&](int j) {
-        if (all < NEIGH_REG) keep[all] = j;
-        all++;
-        if (j > i) up++;
-    });
-    bp.cnt_up[i] = up;
-    bp.cnt_all[i] = all;
+    bool nd = false, wall = false;
+    int i = -1;
+    if (si < bp.n) {
+        i = bp.sorted[si];
+        const DGrid G = *bp.grid;
+        int up = 0, all = 0;
+        int32_t* keep = bp.nbr + (int64_t)NEIGH_REG * si;
+        for_neighbours(bp, G, i, si, [&](int j) {
+            if (all < NEIGH_REG) keep[all] = j;
+            all++;
+            if (j > i) up++;
+        });
+        bp.cnt_up[i] = up;
+        bp.cnt_all[i] = all;
+        if (need.list) {
+            wall = near_any_wall(need.W, bp.state + 16 * (int64_t)i, bp.radius[i]);
+            nd = all > 0 || wall;
+        }
+    }
+    if (need.list) need_append(nd, wall, i, need.list, need.n_list, need.wlist, need.n_wlist);
 }
 
 __global__ void __launch_bounds__(128) k_bp_fill(BroadPhaseBufs bp) {
@@ -610,28 +625,25 @@ __device__ inline bool near_wall(const WallBufs& W, int w, const V3& pos, double
         if (wall_tri_candidate(W.tris + 9 * (int64_t)(t0 + t), pmn, pmx)) return true;
     return false;
 }
+__device__ inline bool near_any_wall(const WallBufs& W, const double* st, double r) {
+    if (W.n_walls <= 0) return false;
+    const V3 pos = v3(st[0], st[1], st[2]);
+    bool wall = false;
+    for (int w = 0; w < W.n_walls && !wall; ++w) wall = near_wall(W, w, pos, r);
+    return wall;
+}
 
 // Particles in a candidate pair or near a wall triangle (list) and those near
 // a wall triangle (wlist), appended with one atomic per workgroup and list.
 // The order of the lists is free: the transform and the wall kernel write
 // per-body results.
-__global__ void __launch_bounds__(256) k_need_lists(int32_t n, const double* state, const double* radius,
-                                                    const int32_t* cnt_all, WallBufs W, int32_t* list,
-                                                    int32_t* n_list, int32_t* wlist, int32_t* n_wlist,
-                                                    unsigned long long* stamp) {
-    stamp_here(stamp);
+// Appends i (taken by this thread when need / wall) to list / wlist: one
+// atomic per list and 256-thread workgroup; every thread of the workgroup
+// calls it.
+__device__ inline void need_append(bool need, bool wall, int i, int32_t* list, int32_t* n_list, int32_t* wlist,
+                                   int32_t* n_wlist) {
     __shared__ int s_cnt[2][4];
     __shared__ int s_base[2];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool wall = false, need = false;
-    if (i < n) {
-        if (W.n_walls > 0) {
-            const double* p = state + 16 * (int64_t)i;
-            V3 pos = v3(p[0], p[1], p[2]);
-            for (int w = 0; w < W.n_walls && !wall; ++w) wall = near_wall(W, w, pos, radius[i]);
-        }
-        need = cnt_all[i] > 0 || wall;
-    }
     const uint64_t bn = __ballot(need), bw = __ballot(wall);
     const int wid = threadIdx.x / WAVE, lane = lane_id();
     if (lane == 0) {
@@ -653,6 +665,20 @@ __global__ void __launch_bounds__(256) k_need_lists(int32_t n, const double* sta
     }
     if (need) list[on + __popcll(bn & below)] = i;
     if (wall) wlist[ow + __popcll(bw & below)] = i;
+}
+
+__global__ void __launch_bounds__(256) k_need_lists(int32_t n, const double* state, const double* radius,
+                                                    const int32_t* cnt_all, WallBufs W, int32_t* list,
+                                                    int32_t* n_list, int32_t* wlist, int32_t* n_wlist,
+                                                    unsigned long long* stamp) {
+    stamp_here(stamp);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool wall = false, need = false;
+    if (i < n) {
+        wall = near_any_wall(W, state + 16 * (int64_t)i, radius[i]);
+        need = cnt_all[i] > 0 || wall;
+    }
+    need_append(need, wall, i, list, n_list, wlist, n_wlist);
 }
 
 #ifdef ZDX_WALK_PROFILE
@@ -2137,11 +2163,11 @@ void launch_report(const int32_t* status, unsigned long long* ctr, const unsigne
 }
 
 void launch_transform(const DBodies& B, const DMeshes& M, hipStream_t s, const int32_t* list,
-                      const int32_t* n_list, int32_t groups) {
+                      const int32_t* n_list, int32_t groups, unsigned long long* stamp) {
     if (B.n <= 0) return;
     int g = list ? groups : (B.n + 3) / 4;
     if (g < 1) g = 1;
-    hipLaunchKernelGGL(k_transform, dim3(g), dim3(256), 0, s, B, M, list, n_list);
+    hipLaunchKernelGGL(k_transform, dim3(g), dim3(256), 0, s, B, M, list, n_list, stamp);
 }
 
 void launch_need_list(int32_t n, const double* state, const double* radius, const int32_t* cnt_all, const WallBufs& W,
@@ -2155,7 +2181,9 @@ void launch_need_list(int32_t n, const double* state, const double* radius, cons
                        wlist, n_wlist, stamp);
 }
 
-void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s) {
+void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s, const NeedLists* need) {
+    NeedLists nl{};
+    if (need) nl = *need;
     const int n = bp.n;
     hipLaunchKernelGGL(k_bp_bounds_part, dim3(BP_PARTS), dim3(256), 0, s, bp.state, bp.radius, n, bp.part);
     // the cell count of the grid in use goes to a scalar (n_adj, written
@@ -2166,7 +2194,7 @@ void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s) {
     hipLaunchKernelGGL(k_bp_cells, dim3(nblk(n, 256)), dim3(256), 0, s, bp);
     scan_i32(bp.cell_count, ncells, bp.max_cells, bp.scan_tmp, bp.cell_start, nullptr, s);
     hipLaunchKernelGGL(k_bp_scatter, dim3(nblk(n, 256)), dim3(256), 0, s, bp);
-    hipLaunchKernelGGL(k_bp_count, dim3(nblk(n, 256)), dim3(256), 0, s, bp);
+    hipLaunchKernelGGL(k_bp_count, dim3(nblk(n, 256)), dim3(256), 0, s, bp, nl);
     scan_i32x2(bp.cnt_up, bp.cnt_all, n, bp.scan_tmp, bp.pair_off, bp.adj_off, s);
     hipLaunchKernelGGL(k_bp_fill, dim3(nblk(n, 128)), dim3(128), 0, s, bp);
     hipLaunchKernelGGL(k_bp_adjfix, dim3(nblk(n, 256) > 0 ? nblk(n, 256) : 1), dim3(256), 0, s, bp);
