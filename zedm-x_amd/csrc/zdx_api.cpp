@@ -859,15 +859,19 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
         HIPC(hipEventRecord(s->ctx->ev_join, st2));
     }
     DPairs P = s->dpairs(par, batched);
+    TopoBufs T = s->topobufs();
+    HistBufs H = s->hist(par);
+    // batched: the work ordering's scatter pass zeroes the topology's per-pair
+    // outputs, and k_tp_alloc stamps the narrow phase's end
     if (batched)
-        launch_lpt_order(P, M, B, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st, true, kt + 2);
+        launch_lpt_order(P, M, B, s->d_order.p, s->d_bcnt.p, s->d_bcur.p, s->pair_cap, st, true, kt + 2, &T,
+                         H.cur_active);
     NarrowCfg nc = s->ncfg;
     nc.use_queue = batched ? 1 : 0;
     nc.groups = g_narrow;
     nc.wave_groups = persistent_groups(s->ctx, 4);
     launch_narrow(B, M, P, nc, s->d_list_i.p, s->d_list_f.p, s->stage_cap, s->tmax, s->d_queue.p, st, true, kt + 8);
-    if (!batched) launch_stamp(kt + 3, st);   // batched: stamped by k_topo_zero (no host read in between)
-    TopoBufs T = s->topobufs();
+    if (!batched) launch_stamp(kt + 3, st);   // batched: stamped by k_tp_alloc (no host read in between)
     if (!batched) {
         int32_t nt = 0;
         HIPC(hipMemcpyAsync(&nt, s->d_topo_n.p, 4, hipMemcpyDeviceToHost, st));
@@ -875,8 +879,8 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
         T.nthreads = std::max(1, std::min<int>(nt, g_topo));
         T.pass_groups = std::max<int64_t>(1, std::min<int64_t>(nt, T.pass_groups));
     }
-    HistBufs H = s->hist(par);
-    launch_topo_zero(P, T, H.cur_active, s->pair_cap, st, batched ? kt + 3 : nullptr);
+    if (batched) T.stamp_alloc = kt + 3;
+    else launch_topo_zero(P, T, H.cur_active, s->pair_cap, st, nullptr);
     launch_topo_sim(P, M, B, mat, H, T, s->sp, st, true, s->ctx->stream3, s->ctx->ev_fork3, s->ctx->ev_join3,
                     s->ctx->ev_cc3);
     if (walls) HIPC(hipStreamWaitEvent(st, s->ctx->ev_join, 0));

@@ -120,6 +120,7 @@ struct TopoBufs {
     int32_t vmax;
     double k_stiff;
     double tol_override;       // > 0: snap / adjacency tolerance for every pair (per-pair API)
+    unsigned long long* stamp_alloc;   // clock written when k_tp_alloc starts, or nullptr
 };
 
 struct SimParams {
@@ -201,9 +202,12 @@ void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const Na
                    unsigned long long* stamp = nullptr);
 // LPT work order of the pairs (largest triangle product first) into order,
 // and the narrow-phase items (P.items, when set) in that order
+// tz: when set, the scatter pass also zeroes the topology's per-pair outputs
+// (what launch_topo_zero does, with cur_active)
 void launch_lpt_order(const DPairs& P, const DMeshes& M, const DBodies& B, int32_t* order, int32_t* bucket_cnt,
                       int32_t* bucket_cur, int32_t pair_cap, hipStream_t s, bool zeroed = false,
-                      unsigned long long* stamp = nullptr);
+                      unsigned long long* stamp = nullptr, const TopoBufs* tz = nullptr,
+                      uint8_t* cur_active = nullptr);
 // zero outputs of every pair (the topology kernels then visit only the
 // pairs of P.topo_list); cur_active may be NULL
 void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, int32_t pair_cap, hipStream_t s,

@@ -917,7 +917,7 @@ __global__ void k_lpt_scan(int32_t* bucket_cnt, int32_t* bucket_cur) {
 // order within a bucket is free (any order gives the same per-pair results)
 __device__ inline void box_f(const double* lo, const double* hi, float* f);
 __global__ void __launch_bounds__(256) k_lpt_scatter(DPairs P, DMeshes M, DBodies B, int32_t* bucket_cur,
-                                                     int32_t* order) {
+                                                     int32_t* order, TopoBufs T, uint8_t* cur_active, int32_t tz) {
     __shared__ int h[64], base[64];
     const int n = *P.n;
     for (int p0 = blockIdx.x * blockDim.x; p0 < n; p0 += gridDim.x * blockDim.x) {
@@ -933,6 +933,14 @@ __global__ void __launch_bounds__(256) k_lpt_scatter(DPairs P, DMeshes M, DBodie
         if (threadIdx.x < 64 && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&bucket_cur[threadIdx.x], h[threadIdx.x]);
         __syncthreads();
         if (p < n) {
+            if (tz) {
+                // the topology's per-pair outputs start at zero (k_topo_zero)
+                T.ncontrib[p] = 0;
+                T.comps[p] = 0;
+                T.loops_raw[p] = 0;
+                T.tstatus[p] = 0;
+                if (cur_active) cur_active[p] = 0;
+            }
             const int slot = base[b] + r;
             order[slot] = p;
             if (P.items) {
@@ -1091,6 +1099,7 @@ __device__ inline int tp_pa_units(int S) {
 // does not fit gets offsets -1 (the step is retried with larger buffers: the
 // counters keep the totals asked for).
 __global__ void __launch_bounds__(256) k_tp_alloc(DPairs P, TopoBufs T) {
+    stamp_here(T.stamp_alloc);
     const int n = *P.topo_n;
     const int R = T.adj_row;
     for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < n; it += gridDim.x * blockDim.x) {
@@ -2248,12 +2257,16 @@ void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const Na
 }
 
 void launch_lpt_order(const DPairs& P, const DMeshes& M, const DBodies& B, int32_t* order, int32_t* bcnt,
-                      int32_t* bcur, int32_t pair_cap, hipStream_t s, bool zeroed, unsigned long long* stamp) {
+                      int32_t* bcur, int32_t pair_cap, hipStream_t s, bool zeroed, unsigned long long* stamp,
+                      const TopoBufs* tz, uint8_t* cur_active) {
+    TopoBufs T{};
+    if (tz) T = *tz;
     if (!zeroed) hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, bcnt, 0, nullptr, (int64_t)64);
     const int g = std::max(1, std::min(nblk(pair_cap, 256), 512));
     hipLaunchKernelGGL(k_lpt_count, dim3(g), dim3(256), 0, s, P, M, B, bcnt, stamp);
     hipLaunchKernelGGL(k_lpt_scan, dim3(1), dim3(64), 0, s, bcnt, bcur);
-    hipLaunchKernelGGL(k_lpt_scatter, dim3(g), dim3(256), 0, s, P, M, B, bcur, order);
+    hipLaunchKernelGGL(k_lpt_scatter, dim3(g), dim3(256), 0, s, P, M, B, bcur, order, T, tz ? cur_active : nullptr,
+                       tz ? 1 : 0);
 }
 
 void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, int32_t pair_cap, hipStream_t s,
