@@ -292,11 +292,56 @@ __global__ void __launch_bounds__(256) k_scan_apply(const int32_t* in0, const in
     }
 }
 
+// k_scan_apply without k_scan_blocks before it: each workgroup sums the block
+// sums before its own (at most SCAN_ONE_PASS_BLOCKS of them) itself.
+constexpr int SCAN_ONE_PASS_BLOCKS = 1024;
+__global__ void __launch_bounds__(256) k_scan_apply_sum(const int32_t* in0, const int32_t* in1, const int32_t* n_ptr,
+                                                       int32_t n_fixed, const int32_t* bsum0, int32_t bstride,
+                                                       int32_t* out0, int32_t* out1, int32_t* total_out) {
+    __shared__ int sw[4], sp[4];
+    const int n = n_ptr ? *n_ptr : n_fixed;
+    const int64_t base = (int64_t)blockIdx.x * 1024;
+    if (base > n) return;
+    const int nb = (n + 1023) / 1024;
+    const int na = in1 ? 2 : 1;
+    for (int a = 0; a < na; ++a) {
+        const int32_t* in = a ? in1 : in0;
+        int32_t* out = a ? out1 : out0;
+        const int32_t* bsum = bsum0 + a * (int64_t)bstride;
+        int pre = 0;
+        for (int k = threadIdx.x; k < (int)blockIdx.x; k += 256) pre += bsum[k];
+        int v[4], s = 0;
+        for (int k = 0; k < 4; ++k) {
+            int64_t i = base + threadIdx.x * 4 + k;
+            v[k] = (i < n) ? in[i] : 0;
+            s += v[k];
+        }
+        int before, total;
+        (void)block_excl<256>(pre, sp, before);
+        int run = before + block_excl<256>(s, sw, total);
+        for (int k = 0; k < 4; ++k) {
+            int64_t i = base + threadIdx.x * 4 + k;
+            if (i < n) out[i] = run;
+            run += v[k];
+        }
+        if (blockIdx.x == (unsigned)(nb > 0 ? nb - 1 : 0) && threadIdx.x == 0) {
+            out[n] = before + total;
+            if (total_out && a == 0) *total_out = before + total;
+        }
+        __syncthreads();   // sw and sp are reused by the second array
+    }
+}
+
 void scan_i32(const int32_t* in, const int32_t* n_ptr, int32_t n_max, int32_t* tmp, int32_t* out, int32_t* total,
               hipStream_t s) {
     int nb = (n_max + 1023) / 1024;
     if (nb < 1) nb = 1;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, in, (const int32_t*)nullptr, n_ptr, n_max, tmp, nb + 1);
+    if (nb <= SCAN_ONE_PASS_BLOCKS) {
+        hipLaunchKernelGGL(k_scan_apply_sum, dim3(nb), dim3(256), 0, s, in, (const int32_t*)nullptr, n_ptr, n_max, tmp,
+                           nb + 1, out, (int32_t*)nullptr, total);
+        return;
+    }
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, tmp, nb, n_ptr, 1, nb + 1);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, in, (const int32_t*)nullptr, n_ptr, n_max, tmp, nb + 1,
                        out, (int32_t*)nullptr, total);
@@ -307,6 +352,11 @@ void scan_i32x2(const int32_t* in0, const int32_t* in1, int32_t n, int32_t* tmp,
     int nb = (n + 1023) / 1024;
     if (nb < 1) nb = 1;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, in0, in1, (const int32_t*)nullptr, n, tmp, nb + 1);
+    if (nb <= SCAN_ONE_PASS_BLOCKS) {
+        hipLaunchKernelGGL(k_scan_apply_sum, dim3(nb), dim3(256), 0, s, in0, in1, (const int32_t*)nullptr, n, tmp,
+                           nb + 1, out0, out1, (int32_t*)nullptr);
+        return;
+    }
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, tmp, nb, (const int32_t*)nullptr, 2, nb + 1);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, in0, in1, (const int32_t*)nullptr, n, tmp, nb + 1,
                        out0, out1, (int32_t*)nullptr);
