@@ -820,6 +820,7 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
         z.i32[3] = s->d_topo_n.p; z.n32[3] = 1;
         z.i32[4] = (int32_t*)s->d_queue.p; z.n32[4] = NARROW_COUNTERS;
         z.i32[5] = batched ? s->d_bcnt.p : nullptr; z.n32[5] = 64;
+        z.i32[7] = batched ? s->d_bcur.p : nullptr; z.n32[7] = 64;
         z.i32[6] = s->d_topo_nbs.p; z.n32[6] = 2;
         z.u64[0] = s->d_hit_top.p; z.n64[0] = HIT_SHARDS * HIT_TOP_STRIDE;
         z.u64[1] = s->tw.top.p; z.n64[1] = 5;

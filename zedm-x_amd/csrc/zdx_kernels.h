@@ -200,8 +200,9 @@ constexpr int NARROW_COUNTERS = 67 * 32;
 void launch_narrow(const DBodies& B, const DMeshes& M, const DPairs& P, const NarrowCfg& cfg, int32_t* list_i,
                    float* list_f, int32_t seg_cap, int32_t tmax, unsigned int* queue, hipStream_t s, bool zeroed = false,
                    unsigned long long* stamp = nullptr);
-// LPT work order of the pairs (largest triangle product first) into order,
-// and the narrow-phase items (P.items, when set) in that order
+// LPT work order of the pairs (largest estimated cost first) into order,
+// and the narrow-phase items (P.items, when set) in that order; zeroed:
+// bucket_cnt and bucket_cur (64 each) already hold 0
 // tz: when set, the scatter pass also zeroes the topology's per-pair outputs
 // (what launch_topo_zero does, with cur_active)
 void launch_lpt_order(const DPairs& P, const DMeshes& M, const DBodies& B, int32_t* order, int32_t* bucket_cnt,

@@ -959,17 +959,25 @@ __global__ void __launch_bounds__(256) k_lpt_count(DPairs P, DMeshes M, DBodies 
     __syncthreads();
     if (threadIdx.x < 64 && h[threadIdx.x]) atomicAdd(&bucket_cnt[threadIdx.x], h[threadIdx.x]);
 }
-__global__ void k_lpt_scan(int32_t* bucket_cnt, int32_t* bucket_cur) {
-    if (threadIdx.x != 0) return;
-    int acc = 0;
-    for (int b = 0; b < 64; ++b) { int c = bucket_cnt[b]; bucket_cur[b] = acc; acc += c; }
-}
 // order within a bucket is free (any order gives the same per-pair results)
 __device__ inline void box_f(const double* lo, const double* hi, float* f);
-__global__ void __launch_bounds__(256) k_lpt_scatter(DPairs P, DMeshes M, DBodies B, int32_t* bucket_cur,
-                                                     int32_t* order, TopoBufs T, uint8_t* cur_active, int32_t tz) {
-    __shared__ int h[64], base[64];
+__global__ void __launch_bounds__(256) k_lpt_scatter(DPairs P, DMeshes M, DBodies B, const int32_t* bucket_cnt,
+                                                     int32_t* bucket_cur, int32_t* order, TopoBufs T,
+                                                     uint8_t* cur_active, int32_t tz) {
+    __shared__ int h[64], base[64], pre[64];
     const int n = *P.n;
+    // the buckets' first slots: the exclusive prefix of the bucket counts (one
+    // wave); bucket_cur counts each bucket's slots taken so far (from zero)
+    if (threadIdx.x < 64) {
+        const int c = bucket_cnt[threadIdx.x];
+        int x = c;
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const int y = __shfl_up(x, d, WAVE);
+            if ((int)threadIdx.x >= d) x += y;
+        }
+        pre[threadIdx.x] = x - c;
+    }
+    __syncthreads();
     for (int p0 = blockIdx.x * blockDim.x; p0 < n; p0 += gridDim.x * blockDim.x) {
         if (threadIdx.x < 64) h[threadIdx.x] = 0;
         __syncthreads();
@@ -980,7 +988,8 @@ __global__ void __launch_bounds__(256) k_lpt_scatter(DPairs P, DMeshes M, DBodie
             r = atomicAdd(&h[b], 1);
         }
         __syncthreads();
-        if (threadIdx.x < 64 && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&bucket_cur[threadIdx.x], h[threadIdx.x]);
+        if (threadIdx.x < 64 && h[threadIdx.x])
+            base[threadIdx.x] = pre[threadIdx.x] + atomicAdd(&bucket_cur[threadIdx.x], h[threadIdx.x]);
         __syncthreads();
         if (p < n) {
             if (tz) {
@@ -2311,12 +2320,14 @@ void launch_lpt_order(const DPairs& P, const DMeshes& M, const DBodies& B, int32
                       const TopoBufs* tz, uint8_t* cur_active) {
     TopoBufs T{};
     if (tz) T = *tz;
-    if (!zeroed) hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, bcnt, 0, nullptr, (int64_t)64);
+    if (!zeroed) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, bcnt, 0, nullptr, (int64_t)64);
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, bcur, 0, nullptr, (int64_t)64);
+    }
     const int g = std::max(1, std::min(nblk(pair_cap, 256), 512));
     hipLaunchKernelGGL(k_lpt_count, dim3(g), dim3(256), 0, s, P, M, B, bcnt, stamp);
-    hipLaunchKernelGGL(k_lpt_scan, dim3(1), dim3(64), 0, s, bcnt, bcur);
-    hipLaunchKernelGGL(k_lpt_scatter, dim3(g), dim3(256), 0, s, P, M, B, bcur, order, T, tz ? cur_active : nullptr,
-                       tz ? 1 : 0);
+    hipLaunchKernelGGL(k_lpt_scatter, dim3(g), dim3(256), 0, s, P, M, B, (const int32_t*)bcnt, bcur, order, T,
+                       tz ? cur_active : nullptr, tz ? 1 : 0);
 }
 
 void launch_topo_zero(const DPairs& P, const TopoBufs& T, uint8_t* cur_active, int32_t pair_cap, hipStream_t s,
