@@ -53,7 +53,7 @@ constexpr int WAVE = 64;
 // A per-kernel scheme with an atomic from every wave would serialise tens of
 // thousands of same-address atomics per launch.
 __global__ void k_stamp(unsigned long long* slot) { *slot = (unsigned long long)wall_clock64(); }
-__global__ void __launch_bounds__(64) k_step_begin(unsigned long long* kt, StepZero z) {
+__global__ void __launch_bounds__(256) k_step_begin(unsigned long long* kt, StepZero z) {
     if (threadIdx.x == 0) kt[0] = (unsigned long long)wall_clock64();
     for (int a = 0; a < 8; ++a)
         if (z.i32[a])
@@ -365,7 +365,7 @@ void scan_i32x2(const int32_t* in0, const int32_t* in1, int32_t n, int32_t* tmp,
 // ------------------------------------------------------------ broad phase
 
 // Bounds of the particle positions and the largest radius: BP_PARTS
-// workgroups reduce a strided share each, then one workgroup combines them.
+// workgroups reduce a strided share each; k_bp_cells combines them.
 __global__ void __launch_bounds__(256) k_bp_bounds_part(const double* state, const double* radius, int n,
                                                         double* part) {
     double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
@@ -391,30 +391,26 @@ __global__ void __launch_bounds__(256) k_bp_bounds_part(const double* state, con
     }
 }
 
-__global__ void __launch_bounds__(64) k_bp_bounds(const double* part, int nparts, DGrid* g, int max_cells,
-                                                  int32_t* status, int32_t* ncells) {
-    __shared__ double s[7][1];
+// The grid from the BP_PARTS partial bounds, by one wave (every lane ends
+// with the same grid); bad: the bounds are not finite (ST_GRID).
+__device__ inline DGrid bp_grid(const double* part, int nparts, int max_cells, bool& bad) {
     double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
-    for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+    for (int b = lane_id(); b < nparts; b += WAVE)
         for (int k = 0; k < 7; ++k) v[k] = k < 3 ? fmin(v[k], part[7 * b + k]) : fmax(v[k], part[7 * b + k]);
     for (int d = 32; d >= 1; d >>= 1)
         for (int k = 0; k < 7; ++k) {
             const double o = __shfl_xor(v[k], d, WAVE);
             v[k] = k < 3 ? fmin(v[k], o) : fmax(v[k], o);
         }
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 7; ++k) s[k][0] = v[k];
-    __syncthreads();
-    if (threadIdx.x != 0) return;
     DGrid G;
     G.nx = G.ny = G.nz = 1;
-    G.ox = s[0][0]; G.oy = s[1][0]; G.oz = s[2][0];
-    double ex = s[3][0] - s[0][0], ey = s[4][0] - s[1][0], ez = s[5][0] - s[2][0];
-    bool finite = isfinite(ex) && isfinite(ey) && isfinite(ez) && isfinite(s[6][0]);
+    G.ox = v[0]; G.oy = v[1]; G.oz = v[2];
+    double ex = v[3] - v[0], ey = v[4] - v[1], ez = v[5] - v[2];
+    bad = !(isfinite(ex) && isfinite(ey) && isfinite(ez) && isfinite(v[6]));
     // cell edge >= 2 r_max (plus margin for the rounding of (p - o) / h), so
     // every pair with |dp| <= r_i + r_j lies in adjacent cells
-    double h = 2.0 * s[6][0] * (1.0 + 1e-6) + 1e-300;
-    if (!finite) { h = 1.0; ex = ey = ez = 0.0; atomicOr(status, (int)ST_GRID); }
+    double h = 2.0 * v[6] * (1.0 + 1e-6) + 1e-300;
+    if (bad) { h = 1.0; ex = ey = ez = 0.0; }
     for (int it = 0; it < 64; ++it) {
         double fx = floor(ex / h) + 1.0, fy = floor(ey / h) + 1.0, fz = floor(ez / h) + 1.0;
         if (fx * fy * fz <= (double)max_cells) {
@@ -425,8 +421,7 @@ __global__ void __launch_bounds__(64) k_bp_bounds(const double* part, int nparts
     }
     G.h = h;
     G.ncells = G.nx * G.ny * G.nz;
-    *g = G;
-    *ncells = G.ncells;
+    return G;
 }
 
 __device__ inline void cell_coords(const DGrid& G, const double* p, int& cx, int& cy, int& cz) {
@@ -446,10 +441,27 @@ __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, int64_t 
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// The first wave of every workgroup combines the partial bounds into the grid
+// (the same doubles in every workgroup); workgroup 0 publishes it, and its
+// cell count for the cell scan, for the later kernels.
 __global__ void __launch_bounds__(256) k_bp_cells(BroadPhaseBufs bp) {
+    __shared__ DGrid s_g;
+    if (threadIdx.x < WAVE) {
+        bool bad;
+        const DGrid g = bp_grid(bp.part, BP_PARTS, bp.max_cells, bad);
+        if (threadIdx.x == 0) {
+            s_g = g;
+            if (blockIdx.x == 0) {
+                *bp.grid = g;
+                *bp.n_adj = g.ncells;
+                if (bad) atomicOr(bp.status, (int)ST_GRID);
+            }
+        }
+    }
+    __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= bp.n) return;
-    const DGrid G = *bp.grid;
+    const DGrid G = s_g;
     int cx, cy, cz;
     cell_coords(G, bp.state + 16 * (int64_t)i, cx, cy, cz);
     int c = (cz * G.ny + cy) * G.nx + cx;
@@ -2223,7 +2235,7 @@ void launch_stamp(unsigned long long* slot, hipStream_t s) {
     if (slot) hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, s, slot);
 }
 void launch_step_begin(unsigned long long* kt, const StepZero& z, hipStream_t s) {
-    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, kt, z);
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(256), 0, s, kt, z);
 }
 void launch_report(const int32_t* status, unsigned long long* ctr, const unsigned long long* kt, StepReport* rep,
                    hipStream_t s) {
@@ -2254,12 +2266,10 @@ void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s, const NeedLists*
     if (need) nl = *need;
     const int n = bp.n;
     hipLaunchKernelGGL(k_bp_bounds_part, dim3(BP_PARTS), dim3(256), 0, s, bp.state, bp.radius, n, bp.part);
-    // the cell count of the grid in use goes to a scalar (n_adj, written
-    // again at the end) for the cell scan
-    int32_t* ncells = bp.n_adj;
-    hipLaunchKernelGGL(k_bp_bounds, dim3(1), dim3(64), 0, s, bp.part, (int)BP_PARTS, bp.grid, bp.max_cells,
-                       bp.status, ncells);
-    hipLaunchKernelGGL(k_bp_cells, dim3(nblk(n, 256)), dim3(256), 0, s, bp);
+    // k_bp_cells writes the grid and its cell count (to a scalar, n_adj,
+    // written again at the end) for the cell scan
+    const int32_t* ncells = bp.n_adj;
+    hipLaunchKernelGGL(k_bp_cells, dim3(std::max(1, nblk(n, 256))), dim3(256), 0, s, bp);
     scan_i32(bp.cell_count, ncells, bp.max_cells, bp.scan_tmp, bp.cell_start, nullptr, s);
     hipLaunchKernelGGL(k_bp_scatter, dim3(nblk(n, 256)), dim3(256), 0, s, bp);
     hipLaunchKernelGGL(k_bp_count, dim3(nblk(n, 256)), dim3(256), 0, s, bp, nl);
