@@ -824,12 +824,10 @@ void enqueue_step(zdx_sim* s, int par, bool batched) {
         z.i32[6] = s->d_topo_nbs.p; z.n32[6] = 2;
         z.u64[0] = s->d_hit_top.p; z.n64[0] = HIT_SHARDS * HIT_TOP_STRIDE;
         z.u64[1] = s->tw.top.p; z.n64[1] = 5;
-        launch_step_begin(kt, z, st);
-    }
-    {
-        // the need lists come from the broad phase's count pass
+        // the step's start (clock 0 and z) is done by the broad phase's first
+        // kernel; the need lists come from its count pass
         const NeedLists need{s->wallbufs(), s->d_need_list.p, s->d_n_need.p, s->d_wall_list.p, s->d_n_wall.p};
-        launch_broadphase(s->bpbufs(par), st, &need);
+        launch_broadphase(s->bpbufs(par), st, &need, kt, &z);
     }
     int g_transform = s->transform_groups(), g_narrow = gmax, g_topo = (int)s->topo_threads,
         g_walls = (int)s->wall_threads;

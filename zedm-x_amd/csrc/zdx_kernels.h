@@ -143,15 +143,14 @@ struct NarrowCfg {
 
 // Writes the device clock into *slot when the stream reaches it (NULL: no-op).
 void launch_stamp(unsigned long long* slot, hipStream_t s);
-// Small arrays a step clears before its stages run (NULL entries skipped).
+// Small arrays a step clears before its stages run (NULL entries skipped), by
+// launch_broadphase's first kernel.
 struct StepZero {
     int32_t* i32[8];
     int32_t n32[8];
     unsigned long long* u64[2];
     int32_t n64[2];
 };
-// First kernel of a step: the clock into kt[0], then the StepZero arrays.
-void launch_step_begin(unsigned long long* kt, const StepZero& z, hipStream_t s);
 
 // What the host reads after every step, in one block of pinned, coherent host
 // memory that the last kernel of the step writes: status, counters, stage
@@ -188,7 +187,11 @@ struct NeedLists {
     int32_t* wlist;
     int32_t* n_wlist;
 };
-void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s, const NeedLists* need = nullptr);
+// kt0 and z: the broad phase's first kernel also does launch_step_begin's work
+// (kt0 the step's clock 0, z the arrays to zero), so the step needs no kernel
+// of its own for it
+void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s, const NeedLists* need = nullptr,
+                       unsigned long long* kt0 = nullptr, const StepZero* z = nullptr);
 // Narrow phase: cull + sweep into the hit arena, the exact tests per hit, and
 // the per-pair compaction into the segment arena.  seg_cap: most segments one
 // pair may store (ST_STAGE beyond); queue: the cull's work counter; resets

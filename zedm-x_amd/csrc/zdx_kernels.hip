@@ -53,15 +53,6 @@ constexpr int WAVE = 64;
 // A per-kernel scheme with an atomic from every wave would serialise tens of
 // thousands of same-address atomics per launch.
 __global__ void k_stamp(unsigned long long* slot) { *slot = (unsigned long long)wall_clock64(); }
-__global__ void __launch_bounds__(256) k_step_begin(unsigned long long* kt, StepZero z) {
-    if (threadIdx.x == 0) kt[0] = (unsigned long long)wall_clock64();
-    for (int a = 0; a < 8; ++a)
-        if (z.i32[a])
-            for (int k = threadIdx.x; k < z.n32[a]; k += blockDim.x) z.i32[a][k] = 0;
-    for (int a = 0; a < 2; ++a)
-        if (z.u64[a])
-            for (int k = threadIdx.x; k < z.n64[a]; k += blockDim.x) z.u64[a][k] = 0ull;
-}
 
 // The end-of-step report into host memory (fine-grained, so the stores reach
 // it in order once fenced); seq goes last.
@@ -366,8 +357,20 @@ void scan_i32x2(const int32_t* in0, const int32_t* in1, int32_t n, int32_t* tmp,
 
 // Bounds of the particle positions and the largest radius: BP_PARTS
 // workgroups reduce a strided share each; k_bp_cells combines them.
+// With kt0 set it is also the step's first kernel (k_step_begin's work: the
+// clock into *kt0 and the StepZero arrays zeroed, grid-stride).
 __global__ void __launch_bounds__(256) k_bp_bounds_part(const double* state, const double* radius, int n,
-                                                        double* part) {
+                                                        double* part, unsigned long long* kt0, StepZero z) {
+    if (kt0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *kt0 = (unsigned long long)wall_clock64();
+        const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, nt = (int64_t)gridDim.x * blockDim.x;
+        for (int a = 0; a < 8; ++a)
+            if (z.i32[a])
+                for (int64_t k = t0; k < z.n32[a]; k += nt) z.i32[a][k] = 0;
+        for (int a = 0; a < 2; ++a)
+            if (z.u64[a])
+                for (int64_t k = t0; k < z.n64[a]; k += nt) z.u64[a][k] = 0ull;
+    }
     double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double* p = state + 16 * (int64_t)i;
@@ -2234,9 +2237,6 @@ extern "C" int zdx_walk_profile(unsigned long long* out) {
 void launch_stamp(unsigned long long* slot, hipStream_t s) {
     if (slot) hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, s, slot);
 }
-void launch_step_begin(unsigned long long* kt, const StepZero& z, hipStream_t s) {
-    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(256), 0, s, kt, z);
-}
 void launch_report(const int32_t* status, unsigned long long* ctr, const unsigned long long* kt, StepReport* rep,
                    hipStream_t s) {
     hipLaunchKernelGGL(k_report, dim3(1), dim3(64), 0, s, status, ctr, kt, rep);
@@ -2261,11 +2261,15 @@ void launch_need_list(int32_t n, const double* state, const double* radius, cons
                        wlist, n_wlist, stamp);
 }
 
-void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s, const NeedLists* need) {
+void launch_broadphase(const BroadPhaseBufs& bp, hipStream_t s, const NeedLists* need, unsigned long long* kt0,
+                       const StepZero* z) {
     NeedLists nl{};
     if (need) nl = *need;
+    StepZero sz{};
+    if (kt0 && z) sz = *z;
     const int n = bp.n;
-    hipLaunchKernelGGL(k_bp_bounds_part, dim3(BP_PARTS), dim3(256), 0, s, bp.state, bp.radius, n, bp.part);
+    hipLaunchKernelGGL(k_bp_bounds_part, dim3(BP_PARTS), dim3(256), 0, s, bp.state, bp.radius, n, bp.part,
+                       z ? kt0 : nullptr, sz);
     // k_bp_cells writes the grid and its cell count (to a scalar, n_adj,
     // written again at the end) for the cell scan
     const int32_t* ncells = bp.n_adj;
