@@ -1964,6 +1964,11 @@ __global__ void __launch_bounds__(256) k_fold_integrate(int32_t n, const double*
         sum_counters(P, T, (int64_t)n * n_walls, wnc, ctr, blockIdx.x, min((int)gridDim.x, FOLD_CTR_GROUPS));
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    // the body and its inertia first: their loads overlap the fold's chains
+    Body b = load_body(sin + 16 * (int64_t)i);
+    M3 Ib;
+    for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) Ib.m[r][c] = mat.Ibinv[9 * (int64_t)i + 3 * r + c];
+    const double im = mat.inv_mass[i];
     // forces[i] and torques[i] start at zero and receive, in the reference's
     // order (main.cpp:2049-2050 pair loop, then walls 2310-2447): pairs with
     // partners in ascending index, loops in order, then walls and their tris
@@ -2004,10 +2009,7 @@ __global__ void __launch_bounds__(256) k_fold_integrate(int32_t n, const double*
     Fo[3 * (int64_t)i] = F.x; Fo[3 * (int64_t)i + 1] = F.y; Fo[3 * (int64_t)i + 2] = F.z;
     To[3 * (int64_t)i] = Tq.x; To[3 * (int64_t)i + 1] = Tq.y; To[3 * (int64_t)i + 2] = Tq.z;
     counts[i] = cnt;
-    Body b = load_body(sin + 16 * (int64_t)i);
-    M3 Ib;
-    for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) Ib.m[r][c] = mat.Ibinv[9 * (int64_t)i + 3 * r + c];
-    integrate(b, Ib, mat.inv_mass[i], F, Tq, v3(sp.g[0], sp.g[1], sp.g[2]), sp.dt);
+    integrate(b, Ib, im, F, Tq, v3(sp.g[0], sp.g[1], sp.g[2]), sp.dt);
     store_body(sout + 16 * (int64_t)i, b);
 }
 
