@@ -1948,7 +1948,7 @@ __device__ inline void sum_counters(const DPairs& P, const TopoBufs& T, int64_t 
     if (bid == 0 && threadIdx.x == 0) atomicAdd(&ctr[5], (unsigned long long)n);
 }
 
-constexpr int FOLD_CTR_GROUPS = 32;
+constexpr int FOLD_CTR_GROUPS = 128;
 __global__ void __launch_bounds__(256) k_fold_integrate(int32_t n, const double* sin, double* sout, DMat mat,
                                                         const int32_t* adj_off, const int32_t* adj_j,
                                                         const int32_t* adj_pair, int32_t adj_cap, TopoBufs T,
